@@ -1,0 +1,141 @@
+/*
+ * plens_gpu.h -- C ABI of the MI355X (gfx950) DGSEM right-hand side and RK4(3)5[3R+]N update.
+ *
+ * Drop-in for the hot path of PLENS (vachan-potluri/plens):
+ *   PLENS::calc_rhs()            dgsem/plens.cc:3688-3779   -> plens_gpu_calc_rhs()
+ *   PLENS::calc_time_step()      dgsem/plens.cc:3810-3912   -> plens_gpu_calc_dt()   (global stepping)
+ *   PLENS::update()+update_rk4() dgsem/plens.cc:4484-4518, 4334-4403 -> plens_gpu_rk4_step()
+ *   PLENS::run() time loop       dgsem/plens.cc:1547        -> plens_gpu_run()
+ *   NavierStokes::assert_positivity's std::exit (dgsem/plens.cc:2540-2544)
+ *                                                            -> PLENS_E_POSITIVITY + plens_gpu_last_failure()
+ * The reference builds with -DCVAR_GRAD=ON (conservative-variable BR1 gradients,
+ * calc_cell_cons_grad dgsem/plens.cc:2174-2285), Sutherland viscosity and the Hennemann
+ * troubled-cell indicator (dgsem/blender_calculator.cc:123-187, 277-303).
+ *
+ * Arrays are plain host pointers; they are copied at create/set time. No C++ types or exceptions
+ * cross this boundary. Every function returns 0 (PLENS_OK) or a negative PLENS_E_* code.
+ * A handle owns its device memory and one HIP stream; calls on one handle must be serialised by
+ * the caller, distinct handles may be used concurrently.
+ */
+#ifndef PLENS_GPU_H
+#define PLENS_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PLENS_OK = 0,
+    PLENS_E_INVALID = -1,      /* bad argument / unsupported combination */
+    PLENS_E_HIP = -2,          /* HIP runtime error */
+    PLENS_E_POSITIVITY = -3,   /* rho<=0 or e<=0 at some DoF (replaces the reference's std::exit) */
+    PLENS_E_NOMEM = -4,        /* device allocation failed */
+    PLENS_E_NODEVICE = -5      /* no GPU visible */
+};
+
+/* Inviscid surface flux schemes ("inviscid surface flux scheme", dgsem/plens.cc:237-246). */
+enum {
+    PLENS_FLUX_HLLC = 0,           /* NavierStokes::hllc_xflux, modelling/navier_stokes.cc:936-1033 */
+    PLENS_FLUX_RUSANOV = 1,        /* rusanov_xflux, modelling/navier_stokes.cc:1044-1059 */
+    PLENS_FLUX_AUSM_PLUS_UP = 2,   /* ausm_plus_up_xflux, modelling/navier_stokes.cc:1070-1121 */
+    PLENS_FLUX_CHANDRASHEKHAR = 3  /* chandrashekhar_xflux, modelling/navier_stokes.cc:1250-1318 */
+};
+
+/* Boundary condition types (IBCs/). */
+enum {
+    PLENS_BC_PERIODIC = 0,          /* IBCs/periodic.cc:121-165 */
+    PLENS_BC_UNIFORM_INFLOW = 1,    /* IBCs/uniform_inflow.cc:23-77, state from (p, T, vel) */
+    PLENS_BC_OUTFLOW = 2,           /* IBCs/outflow.cc:18-75, prescribed p */
+    PLENS_BC_UNIFORM_TEMP_WALL = 3  /* IBCs/uniform_temp_wall.cc:17-104, (T, vel) */
+};
+
+/* Arithmetic of the flux evaluations. FP64 registers/residual accumulation in both. */
+enum {
+    PLENS_PREC_FP64 = 0,
+    PLENS_PREC_FP32_BULK = 1
+};
+
+typedef struct plens_gpu plens_gpu;
+
+/* Mesh and metric terms of one partition (all owned cells), DoF order = cell-major,
+ * lexicographic nodes inside a cell; n = (N+1)^3, n_f = (N+1)^2. */
+typedef struct {
+    int32_t n_cells;
+    int32_t degree;                 /* N */
+    const double* Ja;               /* [n_cells][9][n]: MetricTerms::JxContra_vecs, comp 3*dir+xyz */
+    const double* detJ;             /* [n_cells][n]:    MetricTerms::detJ */
+    const double* face_normals;     /* [n_cells][6][3][n_f]: FEFaceValues::normal_vector(face_dof) */
+    const double* min_vertex_dist;  /* [n_cells]: cell->minimum_vertex_distance() */
+    const int32_t* face_nbr;        /* [n_cells][6]: neighbour (interior) or periodic partner cell, else -1 */
+    const int32_t* face_bid;        /* [n_cells][6]: -1 for interior faces, else index into the bc table */
+} plens_mesh_desc;
+
+/* Gas model ("Navier-Stokes" subsection, custom gas, dgsem/plens.cc:185-223) and blender
+ * parameters ("blender parameters", dgsem/plens.cc:420-515). */
+typedef struct {
+    double gamma, molecular_weight, prandtl, mu0, T0, S;  /* R = 8.314462618/molecular_weight */
+    int32_t inv_surf_flux;         /* PLENS_FLUX_* */
+    int32_t precision;             /* PLENS_PREC_* */
+    double threshold_factor;       /* default 0.5 */
+    double threshold_exp_factor;   /* default 1.8 */
+    double sharpness_factor;       /* default 9.21024 */
+    double alpha_min;              /* default 1e-3 */
+    double alpha_max;              /* default 0.5 */
+    double wall_blender_limit;     /* default 0.5 */
+} plens_physics;
+
+typedef struct {
+    int32_t type;      /* PLENS_BC_* */
+    double p, T;       /* prescribed pressure / temperature where the type uses them */
+    double vel[3];     /* prescribed (inflow) or wall velocity */
+} plens_bc;
+
+/* Create a solver handle on the current HIP device. */
+int plens_gpu_create(const plens_mesh_desc* mesh, const plens_physics* phys, const plens_bc* bcs,
+                     int32_t n_bcs, plens_gpu** out);
+void plens_gpu_destroy(plens_gpu* h);
+
+/* State Q as [5][n_dof] (rho, rho u, rho v, rho w, rho E), the reference's g_cvars. */
+int plens_gpu_set_state(plens_gpu* h, const double* Q);
+int plens_gpu_get_state(plens_gpu* h, double* Q);
+
+/* == PLENS::calc_rhs(): rhs_out [5][n_dof] (nullable), alpha_out [n_cells] (nullable, the
+ * post-diffusion blender gcrk_alpha), alpha_d_out [n_cells] (nullable). */
+int plens_gpu_calc_rhs(plens_gpu* h, double* rhs_out, double* alpha_out, double* alpha_d_out);
+
+/* == PLENS::calc_time_step() with global stepping; needs mu, so it evaluates mu from the state. */
+int plens_gpu_calc_dt(plens_gpu* h, double courant, double* dt_global);
+
+/* One time step == PLENS::update() with RK order 4: dt from the stage-1 state (courant capped at
+ * 0.9 when >= 1), 5 stages of calc_rhs + 3-register update. *t_inout += dt. */
+int plens_gpu_rk4_step(plens_gpu* h, double courant, double* t_inout);
+
+/* n_steps time steps replayed from a captured HIP graph; *t_inout += sum of dt. The state stays on
+ * the device; positivity failures are reported at the end. */
+int plens_gpu_run(plens_gpu* h, int32_t n_steps, double courant, double* t_inout);
+
+/* Device time (HIP events on the handle's stream) of the graph replays of the last plens_gpu_run. */
+int plens_gpu_last_run_ms(const plens_gpu* h, double* ms);
+
+/* Like plens_gpu_run but with the kernels launched one by one, each bracketed by HIP events;
+ * afterwards plens_gpu_kernel_times() returns the average duration per kernel class. */
+int plens_gpu_run_profiled(plens_gpu* h, int32_t n_steps, double courant, double* t_inout);
+
+/* Last positivity failure: lowest failing DoF index and kind (1 = density, 2 = energy). */
+int plens_gpu_last_failure(const plens_gpu* h, int64_t* dof, int32_t* kind);
+
+/* Cells per workgroup of the cell kernels (autotuner hook); 0 = built-in default. */
+int plens_gpu_set_tuning(plens_gpu* h, int32_t cells_per_block);
+
+/* Average device time of each kernel class over the last plens_gpu_run (HIP events on the
+ * handle's stream): times_ms[0..3] = gradient/blender, alpha, face flux, volume+RK. */
+int plens_gpu_kernel_times(const plens_gpu* h, double* times_ms, int32_t n);
+
+const char* plens_gpu_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,92 @@
+/*
+ * plens_host.h -- deal.II-free host setup for the PLENS DGSEM hot path.
+ *
+ * This is the host layer that plays the role of PLENS's setup members: mesh generation
+ * (read_mesh, dgsem/plens.cc:699-803), DoF distribution and neighbour face matching
+ * (set_dof_handler / form_neighbor_face_matchings, dgsem/plens.cc:918-1009, 1574-1627), the
+ * 1D GLL basis (calc_metric_terms, dgsem/plens.cc:1646-1682), metric terms
+ * (MetricTerms::reinit, dgsem/metric_terms.cc:67-142) and FEFaceValues normals
+ * (dgsem/plens.cc:1751-1756). It fills the plens_mesh_desc consumed by plens_gpu_create().
+ *
+ * Only structured, standard-orientation hexahedral meshes with straight-sided (trilinear,
+ * MappingQ1) cells are generated: that is what the configurations need.
+ *
+ * Conventions (identical to the reference):
+ *   - reference cell [0,1]^3, GLL nodes, lexicographic node id i + j(N+1) + k(N+1)^2;
+ *   - faces 0..5 = -x,+x,-y,+y,-z,+z; face-node maps as FaceDoFInfo::form_maps
+ *     (dgsem/face_dof_info.h:92-150);
+ *   - cell id = i + nx*(j + ny*k); deal.II vertex order inside a cell (x fastest).
+ * All functions return 0 on success and a negative PLENS_E_* code on error.
+ */
+#ifndef PLENS_HOST_H
+#define PLENS_HOST_H
+
+#include <stdint.h>
+#include "plens_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct plens_mesh plens_mesh; /* opaque host mesh with all precomputed arrays */
+
+typedef struct {
+    int32_t degree;            /* polynomial degree N (1..5) */
+    int32_t nx, ny, nz;        /* cells per direction */
+    double lo[3], hi[3];       /* box corners */
+    double y_stretch_ratio;    /* geometric growth of dy away from y=lo[1]; 1 = uniform */
+    double perturb_frac;      /* smooth vertex displacement amplitude / local spacing; 0 = none */
+    uint64_t seed;             /* seed of the displacement wave numbers and phases */
+    int32_t periodic[3];       /* 1 = periodic in that direction */
+    int32_t side_bid[6];       /* bc-table index of the 6 domain sides (used when not periodic) */
+    int32_t periodic_bid[3];   /* bc-table index of the PERIODIC entry of each periodic direction */
+    int32_t transpose_y_face_normals; /* 1 = deal.II face quadrature order on faces 2/3 */
+} plens_box_params;
+
+/* Build a box mesh and all its arrays. */
+int plens_mesh_build_box(const plens_box_params* prm, plens_mesh** out);
+void plens_mesh_destroy(plens_mesh* m);
+
+/* Fill a descriptor that points into the mesh's arrays (valid while the mesh lives). */
+int plens_mesh_fill_desc(const plens_mesh* m, plens_mesh_desc* desc);
+
+/* Sizes and raw arrays (owned by the mesh). */
+int32_t plens_mesh_n_cells(const plens_mesh* m);
+int32_t plens_mesh_degree(const plens_mesh* m);
+const double* plens_mesh_vertices(const plens_mesh* m);     /* [n_cells][8][3], deal.II order */
+const double* plens_mesh_node_coords(const plens_mesh* m);  /* [n_cells][3][n] */
+const double* plens_mesh_Ja(const plens_mesh* m);           /* [n_cells][9][n] */
+const double* plens_mesh_detJ(const plens_mesh* m);         /* [n_cells][n] */
+const double* plens_mesh_face_normals(const plens_mesh* m); /* [n_cells][6][3][n_f] */
+const int32_t* plens_mesh_face_nbr(const plens_mesh* m);    /* [n_cells][6] */
+const int32_t* plens_mesh_face_bid(const plens_mesh* m);    /* [n_cells][6] */
+const double* plens_mesh_min_vertex_dist(const plens_mesh* m); /* [n_cells] */
+
+/* 1D GLL basis on [0,1] of degree N: nodes x[N+1], weights w[N+1], D[(N+1)^2] (row-major,
+ * D[i*(N+1)+j] = l_j'(x_i)) and the 1D nodal->Legendre matrix C1[(N+1)^2]
+ * (C1[i*(N+1)+j] = int_0^1 psi_i phi_j, psi_i orthonormal Legendre). */
+int plens_basis_1d(int32_t degree, double* x, double* w, double* D, double* C1);
+
+/* Cell-local node of face-local dof q on face f (FaceDoFInfo::form_maps, dgsem/face_dof_info.h:92-150). */
+int32_t plens_face_node(int32_t degree, int32_t face, int32_t q);
+
+/* Primitive -> conservative with the reference's formula (modelling/navier_stokes.h:365-378). */
+void plens_prim_to_cons(double gamma, double rho, const double vel[3], double p, double cons[5]);
+
+/* Initial conditions written as Q[5][n_dof] (SoA, cell-major DoF order). */
+/* rho = rho0*(1 + amp*sin(2 pi k.x)) with seeded integer k, 0<|k|<=kmax; uniform vel, p. */
+int plens_ic_density_wave(const plens_mesh* m, double gamma, double rho0, const double vel[3],
+                          double p, double amp, double kmax, uint64_t seed, int32_t k_out[3],
+                          double* Q);
+/* Stationary normal shock at x = x_shock from Rankine-Hugoniot with pre-shock (rho1, p1, Mach)
+ * flowing in +x, plus seeded uniform nodal density noise of relative size `noise` downstream. */
+int plens_ic_normal_shock(const plens_mesh* m, double gamma, double rho1, double p1, double mach,
+                          double x_shock, double noise, uint64_t seed, double* Q);
+/* Uniform state. */
+int plens_ic_uniform(const plens_mesh* m, double gamma, double rho, const double vel[3], double p,
+                     double* Q);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

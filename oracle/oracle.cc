@@ -1,0 +1,1385 @@
+// TEST INFRASTRUCTURE ONLY: CPU restatement of the PLENS explicit DGSEM RHS + RK4 update.
+// Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, never by the product.
+// Each function cites the reference file:line it restates (paths relative to the reference repo).
+#include "oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+using State = std::array<double, 5>;
+using Avars = std::array<double, 9>;
+struct T1 {  // rank-1 tensor in 3D (deal.II Tensor<1,3>)
+    double v[3] = {0, 0, 0};
+    double& operator[](int i) { return v[i]; }
+    double operator[](int i) const { return v[i]; }
+    double norm() const { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+};
+inline double dot(const T1& a, const T1& b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline T1 cross(const T1& a, const T1& b) {
+    T1 c;
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+    return c;
+}
+
+constexpr double R0 = 8.314462618;  // modelling/navier_stokes.h:149
+
+// ---------------------------------------------------------------------------------------------
+// NavierStokes restatement (modelling/navier_stokes.{h,cc})
+// ---------------------------------------------------------------------------------------------
+struct NS {
+    double gma, M, Pr, mu0, T0, S;
+    int scheme;
+    double blender = 1;  // flux_blender_value (ctor default 1, navier_stokes.cc:33)
+
+    double R() const { return R0 / M; }                       // navier_stokes.h:337
+    double cv() const { return R() / (gma - 1); }             // navier_stokes.h:342
+    double mu(double T) const {                               // navier_stokes.h:348-355
+        return mu0 * std::pow(T / T0, 1.5) * (T0 + S) / (T + S);
+    }
+    double kappa(double m) const { return m * R() * gma / ((gma - 1) * Pr); }  // .h:360
+    bool inviscid() const { return mu0 < 1e-16; }             // .h:438
+    void set_blender(double a) {                              // .h:443-448
+        if (a <= 0) blender = 0;
+        else if (a <= 1) blender = a;
+        else blender = 1;
+    }
+    static double get_e(const State& c) {                     // navier_stokes.cc:349-359
+        double ske = 0;
+        for (int d = 0; d < 3; ++d) ske += c[1 + d] * c[1 + d];
+        const double ri = 1 / c[0];
+        ske *= 0.5 * ri;
+        return (c[4] - ske) * ri;
+    }
+    double get_p(const State& c) const {                      // navier_stokes.cc:375-384
+        double ske = 0;
+        for (int d = 0; d < 3; ++d) ske += c[1 + d] * c[1 + d];
+        ske *= 0.5 / c[0];
+        return (gma - 1) * (c[4] - ske);
+    }
+    double get_a(const State& c) const { return std::sqrt(gma * get_p(c) / c[0]); }  // .cc:396-399
+    double get_M(const State& c) const {                      // .cc:408-414
+        double s = 0;
+        const double ri = 1 / c[0];
+        for (int d = 0; d < 3; ++d) s += (c[1 + d] * c[1 + d]) * (ri * ri);
+        return std::sqrt(s) / get_a(c);
+    }
+    void prim_to_cons(double rho, const T1& vel, double p, State& c) const {  // .h:365-378
+        c[0] = rho;
+        c[4] = p / (gma - 1);
+        for (int d = 0; d < 3; ++d) {
+            c[1 + d] = rho * vel[d];
+            c[4] += 0.5 * rho * vel[d] * vel[d];
+        }
+    }
+    void get_inv_flux(const State& c, const T1& dir, State& f) const {  // .cc:424-442
+        T1 vel;
+        const double ri = 1.0 / c[0];
+        for (int d = 0; d < 3; ++d) vel[d] = c[1 + d] * ri;
+        const double p = get_p(c);
+        const double vn = dot(vel, dir);
+        f[0] = c[0] * vn;
+        for (int d = 0; d < 3; ++d) f[1 + d] = vn * c[1 + d] + p * dir[d];
+        f[4] = (c[4] + p) * vn;
+    }
+    static double log_avg(double xl, double xr) {             // navier_stokes.h:494-501
+        const double xi = xl / xr, f = (xi - 1) / (xi + 1), u = f * f,
+                     F = (u < 1e-2 ? 1 + u / 3 + u * u / 5 + u * u * u / 7 : 0.5 * std::log(xi) / f);
+        return 0.5 * (xl + xr) / F;
+    }
+
+    // HLLC, navier_stokes.cc:936-1033 (pressure-based wave speeds, Toro 10.6)
+    void hllc_xflux(const State& l, const State& r, State& f) const {
+        T1 x; x[0] = 1;
+        const double pl = get_p(l), pr = get_p(r);
+        const double pmin = std::min(pl, pr), pmax = std::max(pl, pr);
+        const double ul = l[1] / l[0], ur = r[1] / r[0];
+        const double al = std::sqrt(gma * pl / l[0]), ar = std::sqrt(gma * pr / r[0]);
+        const double p_pvrs = 0.5 * (pl + pr) - 0.5 * (ur - ul) * 0.5 * (l[0] + r[0]) * 0.5 * (al + ar);
+        double ps;
+        if (pmax / pmin < 2 && pmin <= p_pvrs && p_pvrs <= pmax) {
+            ps = p_pvrs;
+        } else if (p_pvrs < pmin) {
+            const double z = (gma - 1) / (2 * gma);
+            ps = std::pow(((al + ar) - (gma - 1) / 2 * (ur - ul)) /
+                              (al / std::pow(pl, z) + ar / std::pow(pr, z)),
+                          1 / z);
+        } else {
+            const double p0 = std::max(0.0, p_pvrs);
+            const double gl = std::sqrt(2 / ((gma + 1) * l[0]) / (p0 + (gma - 1) / (gma + 1) * pl)),
+                         gr = std::sqrt(2 / ((gma + 1) * r[0]) / (p0 + (gma - 1) / (gma + 1) * pr));
+            ps = (gl * pl + gr * pr - (ur - ul)) / (gl + gr);
+        }
+        const double ql = (ps <= pl ? 1 : std::sqrt(1 + (gma + 1) / (2 * gma) * (ps / pl - 1))),
+                     qr = (ps <= pr ? 1 : std::sqrt(1 + (gma + 1) / (2 * gma) * (ps / pr - 1)));
+        const double sl = ul - al * ql, sr = ur + ar * qr;
+        const double s = (pr - pl + l[1] * (sl - ul) - r[1] * (sr - ur)) / (l[0] * (sl - ul) - r[0] * (sr - ur));
+        if (sl > 0) {
+            get_inv_flux(l, x, f);
+        } else if (s > 0) {
+            const double t = (sl - ul) / (sl - s);
+            const State ss = {t * l[0], t * l[0] * s, t * l[2], t * l[3],
+                              t * (l[4] + (s - ul) * (l[0] * s + pl / (sl - ul)))};
+            State lf;
+            get_inv_flux(l, x, lf);
+            for (int v = 0; v < 5; ++v) f[v] = lf[v] + sl * (ss[v] - l[v]);
+        } else if (sr > 0) {
+            const double t = (sr - ur) / (sr - s);
+            const State ss = {t * r[0], t * r[0] * s, t * r[2], t * r[3],
+                              t * (r[4] + (s - ur) * (r[0] * s + pr / (sr - ur)))};
+            State rf;
+            get_inv_flux(r, x, rf);
+            for (int v = 0; v < 5; ++v) f[v] = rf[v] + sr * (ss[v] - r[v]);
+        } else {
+            get_inv_flux(r, x, f);
+        }
+    }
+    // Rusanov, navier_stokes.cc:1044-1059
+    void rusanov_xflux(const State& l, const State& r, State& f) const {
+        T1 x; x[0] = 1;
+        State lf, rf;
+        const double al = get_a(l), ar = get_a(r);
+        const double ul = l[1] / l[0], ur = r[1] / r[0];
+        get_inv_flux(l, x, lf);
+        get_inv_flux(r, x, rf);
+        const double S = std::max(std::fabs(ul) + al, std::fabs(ur) + ar);
+        for (int v = 0; v < 5; ++v) f[v] = 0.5 * (lf[v] + rf[v]) - 0.5 * S * (r[v] - l[v]);
+    }
+    // AUSM+-up, navier_stokes.cc:1070-1121 with modelling/ausm_functions.h; note the
+    // integer division 3/16 == 0 of the reference (navier_stokes.cc:1090) is kept
+    static double m1p(double M) { return 0.5 * (M + std::fabs(M)); }
+    static double m1n(double M) { return 0.5 * (M - std::fabs(M)); }
+    static double m2p(double M) { return 0.25 * (M + 1) * (M + 1); }
+    static double m2n(double M) { return -0.25 * (M - 1) * (M - 1); }
+    static double m4p(double M) { return std::fabs(M) >= 1 ? m1p(M) : m2p(M) * (1 - 16 * 0.125 * m2n(M)); }
+    static double m4n(double M) { return std::fabs(M) >= 1 ? m1n(M) : m2n(M) * (1 + 16 * 0.125 * m2p(M)); }
+    static double p5p(double M, double a) {
+        return std::fabs(M) >= 1 ? m1p(M) / M : m2p(M) * ((2 - M) - 16 * a * M * m2n(M));
+    }
+    static double p5n(double M, double a) {
+        return std::fabs(M) >= 1 ? m1n(M) / M : m2n(M) * ((-2 - M) + 16 * a * M * m2p(M));
+    }
+    void ausm_xflux(const State& l, const State& r, State& f) const {
+        const double Minf = 0.3, Kp = 0.25, Ku = 0.75, sigma = 1.0;
+        const double pl = get_p(l), pr = get_p(r);
+        T1 vl, vr;
+        for (int d = 0; d < 3; ++d) { vl[d] = l[1 + d] / l[0]; vr[d] = r[1 + d] / r[0]; }
+        const double Hl = (l[4] + pl) / l[0], Hr = (r[4] + pr) / r[0];
+        const double a12 = 0.5 * (get_a(l) + get_a(r));
+        const double Ml = vl[0] / a12, Mr = vr[0] / a12;
+        const double Mavgsq = 0.5 * (Ml * Ml + Mr * Mr);
+        const double Mo = std::sqrt(std::min(1.0, std::max(Mavgsq, Minf * Minf)));
+        const double fa = Mo * (2 - Mo), alpha = 3 / 16 * (-4 + 5 * fa * fa);
+        const double M12 = m4p(Ml) + m4n(Mr) -
+                           Kp / fa * std::max(1 - sigma * Mavgsq, 0.0) * 2 * (pr - pl) / (a12 * a12 * (l[0] + r[0]));
+        const double psp = p5p(Ml, alpha), psn = p5n(Mr, alpha);
+        const double p12 = psp * pl + psn * pr - Ku * fa * psp * psn * (l[0] + r[0]) * a12 * (vr[0] - vl[0]);
+        if (M12 > 0) {
+            const double m12 = a12 * M12 * l[0];
+            f[0] = m12; f[1] = m12 * vl[0] + p12; f[2] = m12 * vl[1]; f[3] = m12 * vl[2]; f[4] = m12 * Hl;
+        } else {
+            const double m12 = a12 * M12 * r[0];
+            f[0] = m12; f[1] = m12 * vr[0] + p12; f[2] = m12 * vr[1]; f[3] = m12 * vr[2]; f[4] = m12 * Hr;
+        }
+    }
+    // x eigenvector matrices, navier_stokes.cc:641-678 and 780-825
+    void get_xK(const T1& vel, double a, double H, double K[5][5]) const {
+        K[0][0] = 1;
+        for (int d = 0; d < 3; ++d) K[1 + d][0] = vel[d];
+        K[1][0] -= a;
+        K[4][0] = H - vel[0] * a;
+        K[0][1] = 1;
+        for (int d = 0; d < 3; ++d) K[1 + d][1] = vel[d];
+        K[4][1] = 0.5 * dot(vel, vel);
+        K[0][2] = 0; K[1][2] = 0; K[2][2] = 1; K[3][2] = 0; K[4][2] = vel[1];
+        K[0][3] = 0; K[1][3] = 0; K[2][3] = 0; K[3][3] = 1; K[4][3] = vel[2];
+        K[0][4] = 1;
+        for (int d = 0; d < 3; ++d) K[1 + d][4] = vel[d];
+        K[1][4] += a;
+        K[4][4] = H + vel[0] * a;
+    }
+    void get_xKinv(const T1& vel, double a, double H, double Ki[5][5]) const {
+        const double t = 1 / (gma - 1);
+        Ki[0][0] = H + a * (vel[0] - a) * t;
+        for (int d = 0; d < 3; ++d) Ki[0][1 + d] = -vel[d];
+        Ki[0][1] -= a * t;
+        Ki[0][4] = 1;
+        Ki[1][0] = 2 * (2 * a * a * t - H);
+        for (int d = 0; d < 3; ++d) Ki[1][1 + d] = 2 * vel[d];
+        Ki[1][4] = -2;
+        Ki[2][0] = -2 * vel[1] * a * a * t; Ki[2][1] = 0; Ki[2][2] = 2 * a * a * t; Ki[2][3] = 0; Ki[2][4] = 0;
+        Ki[3][0] = -2 * vel[2] * a * a * t; Ki[3][1] = 0; Ki[3][2] = 0; Ki[3][3] = 2 * a * a * t; Ki[3][4] = 0;
+        Ki[4][0] = H - a * (vel[0] + a) * t;
+        for (int d = 0; d < 3; ++d) Ki[4][1 + d] = -vel[d];
+        Ki[4][1] += a * t;
+        Ki[4][4] = 1;
+        const double fac = 0.5 * (gma - 1) / (a * a);
+        for (int i = 0; i < 5; ++i)
+            for (int j = 0; j < 5; ++j) Ki[i][j] *= fac;
+    }
+    // Chandrashekhar EC flux with hybrid matrix dissipation, navier_stokes.cc:1250-1318
+    void chandrashekhar_xflux(const State& l, const State& r, State& f) const {
+        const double pl = get_p(l), pr = get_p(r);
+        const double bl = 0.5 * l[0] / pl, br = 0.5 * r[0] / pr, bln = log_avg(bl, br);
+        const double p_hat = 0.5 * (l[0] + r[0]) / (bl + br);
+        const double rho_ln = log_avg(l[0], r[0]);
+        T1 va, vsq, vl, vr;
+        const double ril = 1 / l[0], rir = 1 / r[0];
+        for (int d = 0; d < 3; ++d) {
+            vl[d] = l[1 + d] * ril;
+            vr[d] = r[1 + d] * rir;
+            va[d] = 0.5 * (vl[d] + vr[d]);
+            vsq[d] = 0.5 * (vl[d] * vl[d] + vr[d] * vr[d]);
+        }
+        double H_hat = 0.5 / ((gma - 1) * bln) + p_hat / rho_ln;
+        for (int d = 0; d < 3; ++d) H_hat += va[d] * va[d] - 0.5 * vsq[d];
+        f[0] = rho_ln * va[0];
+        f[1] = rho_ln * va[0] * va[0] + p_hat;
+        f[2] = rho_ln * va[0] * va[1];
+        f[3] = rho_ln * va[0] * va[2];
+        f[4] = rho_ln * va[0] * H_hat;
+        const double sql = std::sqrt(l[0]), sqr = std::sqrt(r[0]);
+        T1 vi;
+        const double inv = 1 / (sql + sqr);
+        for (int d = 0; d < 3; ++d) vi[d] = inv * (sql * vl[d] + sqr * vr[d]);
+        const double Hi = ((l[4] + pl) / sql + (r[4] + pr) / sqr) / (sql + sqr);
+        const double ai = std::sqrt((gma - 1) * (Hi - 0.5 * dot(vi, vi)));
+        double K[5][5], Ki[5][5], A[5][5];
+        get_xK(vi, ai, Hi, K);
+        get_xKinv(vi, ai, Hi, Ki);
+        const double uabs = std::fabs(vi[0]), lmax = uabs + ai,
+                     l2 = blender * lmax + (1 - blender) * uabs;
+        const double eig[5] = {lmax, l2, l2, l2, lmax};
+        for (int i = 0; i < 5; ++i)
+            for (int j = 0; j < 5; ++j) {
+                A[i][j] = 0;
+                for (int k = 0; k < 5; ++k) A[i][j] += K[i][k] * eig[k] * Ki[k][j];
+            }
+        for (int i = 0; i < 5; ++i)
+            for (int j = 0; j < 5; ++j) f[i] -= 0.5 * A[i][j] * (r[j] - l[j]);
+    }
+    void xflux(const State& l, const State& r, State& f) const {
+        switch (scheme) {
+            case 0: hllc_xflux(l, r, f); break;
+            case 1: rusanov_xflux(l, r, f); break;
+            case 2: ausm_xflux(l, r, f); break;
+            default: chandrashekhar_xflux(l, r, f); break;
+        }
+    }
+    // rotation to the normal-aligned frame, navier_stokes.cc:482-569
+    void get_inv_surf_flux(const State& o, const State& nb, const T1& n, State& f) const {
+        T1 x; x[0] = 1;
+        T1 m = cross(x, n);
+        const double Mm = m.norm();
+        double R[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        if (Mm > 1e-3) {
+            for (int d = 0; d < 3; ++d) m[d] /= Mm;
+            const double c = dot(x, n), t = 1 - c, s = std::sqrt(1 - c * c);
+            R[0][0] = t * m[0] * m[0] + c;     R[0][1] = t * m[0] * m[1] - s * m[2]; R[0][2] = t * m[0] * m[2] + s * m[1];
+            R[1][0] = t * m[0] * m[1] + s * m[2]; R[1][1] = t * m[1] * m[1] + c;     R[1][2] = t * m[1] * m[2] - s * m[0];
+            R[2][0] = t * m[0] * m[2] - s * m[1]; R[2][1] = t * m[1] * m[2] + s * m[0]; R[2][2] = t * m[2] * m[2] + c;
+        } else {
+            const double dg = n[0] < 0 ? -1 : 1;
+            for (int d = 0; d < 3; ++d) R[d][d] = dg;
+        }
+        T1 om, nm, omr, nmr;
+        for (int d = 0; d < 3; ++d) { om[d] = o[1 + d]; nm[d] = nb[1 + d]; }
+        for (int row = 0; row < 3; ++row)
+            for (int col = 0; col < 3; ++col) {
+                const double RT = R[col][row];
+                omr[row] += RT * om[col];
+                nmr[row] += RT * nm[col];
+            }
+        State orr, nrr;
+        orr[0] = o[0]; nrr[0] = nb[0];
+        for (int d = 0; d < 3; ++d) { orr[1 + d] = omr[d]; nrr[1 + d] = nmr[d]; }
+        orr[4] = o[4]; nrr[4] = nb[4];
+        State fr;
+        xflux(orr, nrr, fr);
+        T1 mfr, mf;
+        for (int d = 0; d < 3; ++d) mfr[d] = fr[1 + d];
+        for (int row = 0; row < 3; ++row)
+            for (int col = 0; col < 3; ++col) mf[row] += R[row][col] * mfr[col];
+        f[0] = fr[0];
+        for (int d = 0; d < 3; ++d) f[1 + d] = mf[d];
+        f[4] = fr[4];
+    }
+    // Chandrashekhar volume flux, navier_stokes.cc:1510-1542
+    void vol_flux(const State& c1, const State& c2, const T1& dir, State& f) const {
+        const double p1 = get_p(c1), p2 = get_p(c2);
+        const double b1 = 0.5 * c1[0] / p1, b2 = 0.5 * c2[0] / p2, bln = log_avg(b1, b2);
+        const double p_hat = 0.5 * (c1[0] + c2[0]) / (b1 + b2);
+        const double rho_ln = log_avg(c1[0], c2[0]);
+        T1 va, vsq;
+        const double r1 = 1 / c1[0], r2 = 1 / c2[0];
+        for (int d = 0; d < 3; ++d) {
+            const double v1 = c1[1 + d] * r1, v2 = c2[1 + d] * r2;
+            va[d] = 0.5 * (v1 + v2);
+            vsq[d] = 0.5 * (v1 * v1 + v2 * v2);
+        }
+        double H_hat = 0.5 / ((gma - 1) * bln) + p_hat / rho_ln;
+        for (int d = 0; d < 3; ++d) H_hat += va[d] * va[d] - 0.5 * vsq[d];
+        const double vn = dot(va, dir);
+        f[0] = vn * rho_ln;
+        for (int d = 0; d < 3; ++d) f[1 + d] = rho_ln * vn * va[d] + p_hat * dir[d];
+        f[4] = rho_ln * vn * H_hat;
+    }
+    // diffusive flux, navier_stokes.cc:599-624 (stress tensor from avars, .cc:578-589)
+    static void dif_flux(const State& c, const Avars& av, const T1& dir, State& f) {
+        double st[3][3];
+        int i = 0;
+        for (int row = 0; row < 3; ++row)
+            for (int col = row; col < 3; ++col) {
+                st[row][col] = av[i];
+                if (col != row) st[col][row] = av[i];
+                ++i;
+            }
+        T1 mf;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) mf[a] += st[a][b] * dir[b];
+        f[0] = 0;
+        f[4] = 0;
+        const double ri = 1.0 / c[0];
+        for (int d = 0; d < 3; ++d) {
+            const double v = c[1 + d] * ri;
+            f[1 + d] = mf[d];
+            f[4] += -av[6 + d] * dir[d] + v * mf[d];
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------------------------
+// 1D bases (QGaussLobatto / FE_DGQ<1> shape_grad / Legendre / QGauss), computed here in long
+// double by Newton on P_N' using the Legendre ODE for P_N''.
+// ---------------------------------------------------------------------------------------------
+void leg(int n, long double x, long double& P, long double& dP) {
+    long double p0 = 1, p1 = x;
+    if (n == 0) { P = 1; dP = 0; return; }
+    for (int k = 2; k <= n; ++k) {
+        const long double p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
+        p0 = p1; p1 = p2;
+    }
+    P = p1;
+    dP = n * (x * p1 - p0) / (x * x - 1);
+}
+long double leg_val(int n, long double x) {
+    long double p0 = 1, p1 = x;
+    if (n == 0) return 1;
+    for (int k = 2; k <= n; ++k) {
+        const long double p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
+        p0 = p1; p1 = p2;
+    }
+    return p1;
+}
+void gll(int N, std::vector<double>& x01, std::vector<double>& w01) {
+    std::vector<long double> x(N + 1);
+    x[0] = -1; x[N] = 1;
+    const long double pi = 3.14159265358979323846264338327950288L;
+    for (int j = 1; j < N; ++j) {
+        long double t = -cosl(pi * j / N);
+        for (int it = 0; it < 200; ++it) {  // Newton on P_N'(t) with P_N'' from the ODE
+            long double P, dP;
+            leg(N, t, P, dP);
+            const long double d2P = (2 * t * dP - N * (N + 1) * P) / (1 - t * t);
+            const long double dt = dP / d2P;
+            t -= dt;
+            if (fabsl(dt) < 1e-32L) break;
+        }
+        x[j] = t;
+    }
+    x01.resize(N + 1); w01.resize(N + 1);
+    for (int j = 0; j <= N; ++j) {
+        const long double P = leg_val(N, x[j]);
+        x01[j] = (double)((x[j] + 1) / 2);
+        w01[j] = (double)(1.0L / (N * (N + 1) * P * P));
+    }
+}
+// FE_DGQ<1>::shape_grad(j, x_i): derivative of the Lagrange polynomial l_j at x_i
+double lagrange_deriv(const std::vector<double>& xn, int j, long double x) {
+    const int np = (int)xn.size();
+    long double s = 0;
+    for (int k = 0; k < np; ++k) {
+        if (k == j) continue;
+        long double p = 1.0L / ((long double)xn[j] - xn[k]);
+        for (int m = 0; m < np; ++m)
+            if (m != j && m != k) p *= (x - xn[m]) / ((long double)xn[j] - xn[m]);
+        s += p;
+    }
+    return (double)s;
+}
+long double lagrange_val(const std::vector<double>& xn, int j, long double x) {
+    long double p = 1;
+    for (int m = 0; m < (int)xn.size(); ++m)
+        if (m != j) p *= (x - xn[m]) / ((long double)xn[j] - xn[m]);
+    return p;
+}
+void gauss(int m, std::vector<double>& xg, std::vector<double>& wg) {
+    xg.resize(m); wg.resize(m);
+    const long double pi = 3.14159265358979323846264338327950288L;
+    for (int j = 0; j < m; ++j) {
+        long double t = -cosl(pi * (j + 0.75L) / (m + 0.5L)), P, dP;
+        for (int it = 0; it < 200; ++it) {
+            leg(m, t, P, dP);
+            const long double dt = P / dP;
+            t -= dt;
+            if (fabsl(dt) < 1e-32L) break;
+        }
+        leg(m, t, P, dP);
+        xg[j] = (double)((t + 1) / 2);
+        wg[j] = (double)(1.0L / ((1 - t * t) * dP * dP));
+    }
+}
+// ChangeOfBasisMatrix ctor (dgsem/change_of_basis_matrix.h:74-98): C_ij = sum_q w_q psi_i(p_q) phi_j(p_q)
+// over QGauss<3>(N+1) with FE_DGQLegendre (orthonormal Legendre on [0,1]) and FE_DGQ (GLL Lagrange).
+void cob_matrix(int N, std::vector<double>& C) {
+    std::vector<double> xn, wn, xg, wg;
+    gll(N, xn, wn);
+    gauss(N + 1, xg, wg);
+    const int np = N + 1, n = np * np * np;
+    std::vector<double> psi(np * np), phi(np * np);  // [poly][quad pt]
+    for (int i = 0; i < np; ++i)
+        for (int q = 0; q < np; ++q) {
+            psi[i * np + q] = (double)(sqrtl(2.0L * i + 1) * leg_val(i, 2.0L * xg[q] - 1));
+            phi[i * np + q] = (double)lagrange_val(xn, i, xg[q]);
+        }
+    C.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        const int i0 = i % np, i1 = (i / np) % np, i2 = i / (np * np);
+        for (int j = 0; j < n; ++j) {
+            const int j0 = j % np, j1 = (j / np) % np, j2 = j / (np * np);
+            double s = 0;
+            for (int q = 0; q < n; ++q) {
+                const int q0 = q % np, q1 = (q / np) % np, q2 = q / (np * np);
+                const double w = wg[q0] * wg[q1] * wg[q2];
+                const double ps = psi[i0 * np + q0] * psi[i1 * np + q1] * psi[i2 * np + q2];
+                const double ph = phi[j0 * np + q0] * phi[j1 * np + q1] * phi[j2 * np + q2];
+                s += w * ps * ph;
+            }
+            C[(size_t)i * n + j] = s;
+        }
+    }
+}
+
+// FaceDoFInfo::form_maps (dgsem/face_dof_info.h:92-150)
+int face_map(int N, int f, int q) {
+    const int np = N + 1, nf = np * np;
+    switch (f) {
+        case 0: return np * q;
+        case 1: return N + np * q;
+        case 2: return (q / np) * nf + (q % np);
+        case 3: return np * N + (q / np) * nf + (q % np);
+        case 4: return q;
+        default: return N * nf + q;
+    }
+}
+
+// RK4Stage5Register3 ctor, dgsem/rk4_stage5_register3.cc:12-36
+struct RKC {
+    double b[5], ao[5], ai[4];
+    RKC() {
+        b[0] = 314199625218.0 / 7198350928319; b[1] = 6410344372641.0 / 17000082738695;
+        b[2] = 292278564125.0 / 5593752632744; b[3] = 5010207514426.0 / 21876007855139;
+        b[4] = 5597675544274.0 / 18784428342765;
+        ao[0] = 4745337637855.0 / 22386579876409; ao[1] = 6808157035527.0 / 13197844641179;
+        ao[2] = 4367509502613.0 / 10454198590847; ao[3] = 1236962429870.0 / 3429868089329;
+        ao[4] = b[4];
+        ai[0] = 546509042554.0 / 9152262712923; ai[1] = 625707605167.0 / 5316659119056;
+        ai[2] = 582400652113.0 / 7078426004906; ai[3] = b[3];
+    }
+};
+
+double blender_from_values(const NS& ns, const oracle_physics& ph, int N, const std::vector<double>& C,
+                           const double* vals) {
+    // BlenderCalculator::get_blender / get_trouble, dgsem/blender_calculator.cc:123-187, 277-303
+    const int np = N + 1, n = np * np * np;
+    std::vector<double> modes(n);
+    for (int row = 0; row < n; ++row) {
+        modes[row] = 0;
+        for (int col = 0; col < n; ++col) modes[row] += C[(size_t)row * n + col] * vals[col];
+    }
+    double etot = 0, e1 = 0, e2 = 0;
+    for (double m : modes) etot += m * m;
+    for (int k = 0; k < N; ++k)
+        for (int j = 0; j < N; ++j)
+            for (int i = 0; i < N; ++i) {
+                const int id = i + j * np + k * np * np;
+                e1 += modes[id] * modes[id];
+            }
+    if (N > 1)
+        for (int k = 0; k < N - 1; ++k)
+            for (int j = 0; j < N - 1; ++j)
+                for (int i = 0; i < N - 1; ++i) {
+                    const int id = i + j * np + k * np * np;
+                    e2 += modes[id] * modes[id];
+                }
+    const double trouble = (N > 2) ? std::max(1 - e1 / etot, 1 - e2 / e1) : 1 - e1 / etot;
+    const double thr = ph.threshold_factor * std::pow(10, -ph.threshold_exp_factor * (std::pow(N + 1, 0.25)));
+    const double at = 1 / (1 + std::exp(-ph.sharpness_factor * (trouble / thr - 1)));
+    (void)ns;
+    if (at < ph.alpha_min) return 0;
+    else if (at < ph.alpha_max) return at;
+    else return ph.alpha_max;
+}
+
+}  // namespace
+
+// =============================================================================================
+// PLENS restatement
+// =============================================================================================
+struct oracle {
+    int N, np, n, nf, nc;
+    size_t ndof;
+    NS ns;
+    oracle_physics ph;
+    std::vector<oracle_bc> bcs;
+    std::vector<int32_t> nbr, bid;
+    std::vector<double> w, x1, D, Qm;                   // w_1d, ref_D_1d, ref_Q_1d
+    std::vector<double> C;                              // change-of-basis matrix
+    // metric terms: JxContra[c][node][dir] (T1), detJ[c][node], Jinv[c][node][3][3]
+    std::vector<T1> Ja;
+    std::vector<double> detJ, Jinv, hmin, normals;     // normals [c][f][q][3]
+    std::vector<T1> sub;                                // subcell normals [c][dir][(N+2)(N+1)^2]
+    // solution vectors
+    std::vector<double> Q, avars, mu, kap, alpha, alpha_d, rhs, rhs1, rhs2, Qn, loc_dt;
+    std::vector<double> s1, s2, s3;                     // surface terms [var][c][f][q]
+    int64_t fail_dof = -1;
+    int fail_kind = 0;
+    double timers[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    RKC rk;
+
+    int tl(int i, int j, int k) const { return i + j * np + k * np * np; }  // CellDoFInfo
+    size_t gid(int c, int i) const { return (size_t)c * n + i; }
+    size_t sidx(int var, int c, int f, int q) const { return (((size_t)var * nc + c) * 6 + f) * nf + q; }
+    const T1& JaV(int c, int node, int dir) const { return Ja[((size_t)c * n + node) * 3 + dir]; }
+    State cons(int c, int i) const {
+        State s;
+        for (int v = 0; v < 5; ++v) s[v] = Q[v * ndof + gid(c, i)];
+        return s;
+    }
+    Avars av(int c, int i) const {
+        Avars a;
+        for (int v = 0; v < 9; ++v) a[v] = avars[v * ndof + gid(c, i)];
+        return a;
+    }
+    T1 normal(int c, int f, int q) const {
+        T1 t;
+        for (int d = 0; d < 3; ++d) t[d] = normals[(((size_t)c * 6 + f) * nf + q) * 3 + d];
+        return t;
+    }
+
+    // ---- MetricTerms::reinit, dgsem/metric_terms.cc:67-142 ----------------------------------
+    void metrics(const double* V, int transpose_y) {
+        Ja.assign((size_t)nc * n * 3, T1());
+        detJ.assign((size_t)nc * n, 0);
+        Jinv.assign((size_t)nc * n * 9, 0);
+        hmin.assign(nc, 0);
+        normals.assign((size_t)nc * 6 * nf * 3, 0);
+        sub.assign((size_t)nc * 3 * (N + 2) * np * np, T1());
+        for (int c = 0; c < nc; ++c) {
+            const double* cv = V + (size_t)c * 24;
+            double dmin = 1e300;
+            for (int a = 0; a < 8; ++a)
+                for (int b = a + 1; b < 8; ++b) {
+                    double d2 = 0;
+                    for (int q = 0; q < 3; ++q) d2 += (cv[3 * a + q] - cv[3 * b + q]) * (cv[3 * a + q] - cv[3 * b + q]);
+                    dmin = std::min(dmin, d2);
+                }
+            hmin[c] = std::sqrt(dmin);
+            for (int k = 0; k < np; ++k)
+                for (int j = 0; j < np; ++j)
+                    for (int i = 0; i < np; ++i) {
+                        const int node = tl(i, j, k);
+                        const double xi[3] = {x1[i], x1[j], x1[k]};
+                        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // J[a][b] = dx_a/dxi_b
+                        for (int v = 0; v < 8; ++v) {
+                            const int bb[3] = {v & 1, (v >> 1) & 1, (v >> 2) & 1};
+                            for (int b = 0; b < 3; ++b) {
+                                double g = bb[b] ? 1.0 : -1.0;
+                                for (int o = 0; o < 3; ++o)
+                                    if (o != b) g *= bb[o] ? xi[o] : 1.0 - xi[o];
+                                for (int a = 0; a < 3; ++a) J[a][b] += g * cv[3 * v + a];
+                            }
+                        }
+                        const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                                           J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                                           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+                        detJ[gid(c, node)] = det;
+                        // inverse by cofactors
+                        double* Ji = &Jinv[gid(c, node) * 9];
+                        Ji[0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+                        Ji[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+                        Ji[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+                        Ji[3] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+                        Ji[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+                        Ji[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+                        Ji[6] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+                        Ji[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+                        Ji[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+                        T1 co[3];  // covariant vectors = rows of J^T
+                        for (int d = 0; d < 3; ++d)
+                            for (int a = 0; a < 3; ++a) co[d][a] = J[a][d];
+                        for (int d = 0; d < 3; ++d)
+                            Ja[((size_t)c * n + node) * 3 + d] = cross(co[(d + 1) % 3], co[(d + 2) % 3]);
+                    }
+            // subcell normals, metric_terms.cc:104-140
+            for (int dir = 0; dir < 3; ++dir) {
+                const int d1 = (dir + 1) % 3, d2 = (dir + 2) % 3;
+                for (int id = 0; id <= N + 1; ++id)
+                    for (int id1 = 0; id1 <= N; ++id1)
+                        for (int id2 = 0; id2 <= N; ++id2) {
+                            int t0[3];
+                            t0[dir] = 0; t0[d1] = id1; t0[d2] = id2;
+                            T1 s = JaV(c, tl(t0[0], t0[1], t0[2]), dir);
+                            for (int l = 0; l <= id - 1; ++l)
+                                for (int m = 0; m <= N; ++m) {
+                                    int tm[3];
+                                    tm[dir] = m; tm[d1] = id1; tm[d2] = id2;
+                                    const T1& J = JaV(c, tl(tm[0], tm[1], tm[2]), dir);
+                                    for (int a = 0; a < 3; ++a) s[a] += Qm[l * np + m] * J[a];
+                                }
+                            sub[sub_index(c, dir, id, id1, id2)] = s;
+                        }
+            }
+            // FEFaceValues normals at face dofs (H1: faces 2/3 read the transposed point)
+            for (int f = 0; f < 6; ++f) {
+                const int dir = f / 2;
+                const double sg = (f % 2) ? 1.0 : -1.0;
+                for (int q = 0; q < nf; ++q) {
+                    int qq = q;
+                    if ((f == 2 || f == 3) && transpose_y) qq = (q % np) * np + q / np;
+                    const T1& J = JaV(c, face_map(N, f, qq), dir);
+                    const double nr = J.norm();
+                    for (int d = 0; d < 3; ++d) normals[(((size_t)c * 6 + f) * nf + q) * 3 + d] = sg * J[d] / nr;
+                }
+            }
+        }
+    }
+    size_t sub_index(int c, int dir, int id, int id1, int id2) const {
+        return ((((size_t)c * 3 + dir) * (N + 2) + id) * np + id1) * np + id2;
+    }
+    // subcell_normals[dir](ti) with ti[dir] = id
+    const T1& subn(int c, int dir, const int ti[3]) const {
+        const int d1 = (dir + 1) % 3, d2 = (dir + 2) % 3;
+        return sub[sub_index(c, dir, ti[dir], ti[d1], ti[d2])];
+    }
+
+    // ---- BC ghost getters (IBCs/*.cc) --------------------------------------------------------
+    void ghost(int stage, int c, int f, int q, const State& in, const Avars& ain, const T1& nrm,
+               State& g, Avars& ag) const {
+        const oracle_bc& b = bcs[bid[(size_t)c * 6 + f]];
+        ag = ain;
+        switch (b.type) {
+            case 0: {  // periodic: partner state/avars at the same face dof (periodic.cc:121-165)
+                const int pc = nbr[(size_t)c * 6 + f];
+                const int pf = (f % 2) ? f - 1 : f + 1;
+                const int node = face_map(N, pf, q);
+                g = cons(pc, node);
+                if (stage == 3) ag = av(pc, node);
+                break;
+            }
+            case 1: {  // uniform inflow (uniform_inflow.cc:23-77)
+                State pr;
+                T1 v; v[0] = b.vel[0]; v[1] = b.vel[1]; v[2] = b.vel[2];
+                const double rho = b.p / (ns.R() * b.T);  // set_BC, plens.cc:1268-1270
+                ns.prim_to_cons(rho, v, b.p, pr);
+                if (stage == 2) g = pr;
+                else for (int k = 0; k < 5; ++k) g[k] = 2 * pr[k] - in[k];
+                break;
+            }
+            case 2: {  // outflow (outflow.cc:18-75)
+                if (stage == 2) {
+                    const double M = ns.get_M(in);
+                    if (ns.inviscid() || M >= 1) g = in;
+                    else {
+                        g = in;
+                        const double pin = ns.get_p(in);
+                        g[4] += 2 * (b.p - pin) / (ns.gma - 1);
+                    }
+                } else g = in;
+                break;
+            }
+            default: {  // uniform temperature wall (uniform_temp_wall.cc:17-104)
+                T1 vw; vw[0] = b.vel[0]; vw[1] = b.vel[1]; vw[2] = b.vel[2];
+                if (stage == 2) {
+                    g[0] = in[0];
+                    g[4] = in[4];
+                    if (ns.inviscid()) {
+                        T1 sm, smr;
+                        for (int d = 0; d < 3; ++d) { sm[d] = in[1 + d]; smr[d] = in[1 + d] - in[0] * vw[d]; }
+                        const double nsm = dot(smr, nrm);
+                        for (int d = 0; d < 3; ++d) g[1 + d] = sm[d] - 2 * nsm * nrm[d];
+                    } else {
+                        for (int d = 0; d < 3; ++d) g[1 + d] = 2 * in[0] * vw[d] - in[1 + d];
+                    }
+                } else {
+                    State pr;
+                    const double pin = ns.get_p(in);
+                    const double rpr = pin / (ns.R() * b.T);
+                    ns.prim_to_cons(rpr, vw, pin, pr);
+                    for (int k = 0; k < 5; ++k) g[k] = 2 * pr[k] - in[k];
+                }
+                break;
+            }
+        }
+    }
+
+    // surf_flux_wrappers[stage-1], navier_stokes.cc:273-319
+    void surf_flux(int stage, const State& u, const Avars& a, const State& ug, const Avars& ag,
+                   const T1& nrm, State& f, const NS& nsl) const {
+        if (stage == 1) {
+            for (int v = 0; v < 5; ++v) f[v] = 0.5 * (u[v] + ug[v]);
+        } else if (stage == 2) {
+            nsl.get_inv_surf_flux(u, ug, nrm, f);
+        } else {
+            State f1, f2;
+            NS::dif_flux(u, a, nrm, f1);
+            NS::dif_flux(ug, ag, nrm, f2);
+            for (int v = 0; v < 5; ++v) f[v] = 0.5 * (f1[v] + f2[v]);
+        }
+    }
+
+    // ---- PLENS::calc_surf_flux, dgsem/plens.cc:1717-1913 --------------------------------------
+    void calc_surf_flux(int stage, std::vector<double>& sf) {
+        sf.assign((size_t)5 * nc * 6 * nf, 0.0);
+        const double rsign = (stage == 1) ? 1.0f : -1.0f;
+#pragma omp parallel for schedule(static)
+        for (int c = 0; c < nc; ++c) {
+            NS nsl = ns;  // per-thread copy: the reference mutates flux_blender_value
+            for (int f = 0; f < 6; ++f) {
+                const int b = bid[(size_t)c * 6 + f];
+                if (b >= 0) {
+                    nsl.set_blender(alpha[c]);
+                    for (int q = 0; q < nf; ++q) {
+                        const int node = face_map(N, f, q);
+                        State u = cons(c, node), ug;
+                        Avars a = av(c, node), ag;
+                        const T1 nrm = normal(c, f, q);
+                        ghost(stage, c, f, q, u, a, nrm, ug, ag);
+                        State fl;
+                        surf_flux(stage, u, a, ug, ag, nrm, fl, nsl);
+                        for (int v = 0; v < 5; ++v) sf[sidx(v, c, f, q)] = fl[v];
+                    }
+                } else {
+                    const int nb = nbr[(size_t)c * 6 + f];
+                    if (!(c < nb)) continue;
+                    const int fn = (f % 2) ? f - 1 : f + 1;  // neighbor_of_neighbor
+                    nsl.set_blender(0.5 * (alpha[c] + alpha[nb]));
+                    for (int q = 0; q < nf; ++q) {
+                        const int qn = q;  // nei_face_matching_dofs: identity on these meshes
+                        const State u = cons(c, face_map(N, f, q)), un = cons(nb, face_map(N, fn, qn));
+                        const Avars a = av(c, face_map(N, f, q)), an = av(nb, face_map(N, fn, qn));
+                        State fl;
+                        surf_flux(stage, u, a, un, an, normal(c, f, q), fl, nsl);
+                        for (int v = 0; v < 5; ++v) {
+                            sf[sidx(v, c, f, q)] = fl[v];
+                            sf[sidx(v, nb, fn, qn)] = rsign * fl[v];
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+    // ---- PLENS::calc_cell_cons_grad, dgsem/plens.cc:2174-2285 ---------------------------------
+    void calc_cell_cons_grad(int c, const std::vector<double>& s1f, std::vector<std::array<State, 3>>& g) const {
+        std::vector<State> st(n);
+        for (int i = 0; i < n; ++i) st[i] = cons(c, i);
+        const double ssign[2] = {1, -1};
+        for (int d = 0; d < 3; ++d)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < 5; ++v) g[i][d][v] = 0;
+        for (int gd = 0; gd < 3; ++gd) {
+            for (int it = 0; it < n; ++it) {
+                const State ct = st[it];
+                const int ti[3] = {it % np, (it / np) % np, it / (np * np)};
+                State fl;
+                for (int m = 0; m <= N; ++m)
+                    for (int md = 0; md < 3; ++md) {
+                        int to[3] = {ti[0], ti[1], ti[2]};
+                        to[md] = m;
+                        const int io = tl(to[0], to[1], to[2]);
+                        const State co = st[io];
+                        for (int v = 0; v < 5; ++v) fl[v] = 0.5 * (ct[v] + co[v]);
+                        const double Jav = 0.5 * (JaV(c, it, md)[gd] + JaV(c, io, md)[gd]);
+                        const double Dm2 = 2 * D[ti[md] * np + m];
+                        for (int v = 0; v < 5; ++v) g[it][gd][v] += Dm2 * fl[v] * Jav;
+                    }
+            }
+            for (int sd = 0; sd < 3; ++sd)
+                for (int lr = 0; lr <= 1; ++lr) {
+                    const int f = 2 * sd + lr;
+                    for (int q = 0; q < nf; ++q) {
+                        const int node = face_map(N, f, q);
+                        const State cs = st[node];
+                        for (int v = 0; v < 5; ++v) {
+                            const double fin = cs[v] * JaV(c, node, sd)[gd];
+                            const double fsu = s1f[sidx(v, c, f, q)] * JaV(c, node, sd)[gd];
+                            g[node][gd][v] -= ssign[lr] * (fsu - fin) / w[lr * N];
+                        }
+                    }
+                }
+            for (int i = 0; i < n; ++i) {
+                const double Ji = 1.0 / detJ[gid(c, i)];
+                for (int v = 0; v < 5; ++v) g[i][gd][v] *= Ji;
+            }
+        }
+    }
+
+    // ---- PLENS::assert_positivity, dgsem/plens.cc:2531-2546 -----------------------------------
+    bool assert_positivity() {
+        for (int c = 0; c < nc; ++c)
+            for (int i = 0; i < n; ++i) {
+                const State s = cons(c, i);
+                if (!(s[0] > 0)) { fail_dof = (int64_t)gid(c, i); fail_kind = 1; return false; }
+                if (!(NS::get_e(s) > 0)) { fail_dof = (int64_t)gid(c, i); fail_kind = 2; return false; }
+            }
+        return true;
+    }
+
+    // ---- PLENS::calc_aux_vars, dgsem/plens.cc:2593-2748 (CVAR_GRAD build) ---------------------
+    void calc_aux_vars() {
+        const double cvv = ns.cv();
+        for (size_t i = 0; i < ndof; ++i) {
+            State s;
+            for (int v = 0; v < 5; ++v) s[v] = Q[v * ndof + i];
+            const double e = NS::get_e(s);
+            mu[i] = ns.mu(e / cvv);
+            kap[i] = ns.kappa(mu[i]);
+        }
+        calc_surf_flux(1, s1);
+#pragma omp parallel
+        {
+            std::vector<std::array<State, 3>> g(n);
+#pragma omp for schedule(static)
+            for (int c = 0; c < nc; ++c) {
+                calc_cell_cons_grad(c, s1, g);
+                for (int i = 0; i < n; ++i) {
+                    const State s = cons(c, i);
+                    const double vel[3] = {s[1] / s[0], s[2] / s[0], s[3] / s[0]};
+                    double vg[3][3];
+                    for (int vd = 0; vd < 3; ++vd)
+                        for (int gd = 0; gd < 3; ++gd)
+                            vg[vd][gd] = (g[i][gd][1 + vd] - g[i][gd][0] * vel[vd]) / s[0];
+                    int sid = 0;
+                    double tr = 0;
+                    for (int d = 0; d < 3; ++d) tr += vg[d][d];
+                    const size_t gi = gid(c, i);
+                    for (int row = 0; row < 3; ++row)
+                        for (int col = row; col < 3; ++col) {
+                            if (col == row) avars[sid * ndof + gi] = vg[row][col] + vg[col][row] - 2.0 / 3 * tr;
+                            else avars[sid * ndof + gi] = vg[row][col] + vg[col][row];
+                            ++sid;
+                        }
+                    const double e = NS::get_e(s);
+                    double eg[3];
+                    for (int gd = 0; gd < 3; ++gd) {
+                        eg[gd] = (g[i][gd][4] - g[i][gd][0] * e) / s[0];
+                        for (int vd = 0; vd < 3; ++vd)
+                            eg[gd] -= (0.5 * g[i][gd][0] * vel[vd] * vel[vd] + s[1 + vd] * vg[vd][gd]) / s[0];
+                    }
+                    for (int d = 0; d < 3; ++d) avars[(6 + d) * ndof + gi] = -eg[d] / cvv;
+                }
+            }
+        }
+        for (size_t i = 0; i < ndof; ++i) {
+            for (int v = 0; v < 6; ++v) avars[v * ndof + i] *= mu[i];
+            for (int v = 6; v < 9; ++v) avars[v * ndof + i] *= kap[i];
+        }
+    }
+
+    // ---- PLENS::calc_blender, dgsem/plens.cc:2858-2962 ------------------------------------------
+    void calc_blender() {
+        std::vector<double> raw(nc);
+#pragma omp parallel
+        {
+            std::vector<double> vals(n);
+#pragma omp for schedule(static)
+            for (int c = 0; c < nc; ++c) {
+                for (int i = 0; i < n; ++i) {
+                    const State s = cons(c, i);
+                    vals[i] = ns.get_p(s) * s[0];  // 'pxrho'
+                }
+                raw[c] = blender_from_values(ns, ph, N, C, vals.data());
+            }
+        }
+        for (int c = 0; c < nc; ++c) {
+            alpha[c] = raw[c];
+            for (int f = 0; f < 6; ++f) {
+                const int b = bid[(size_t)c * 6 + f];
+                if (b >= 0 && bcs[b].type == 3) alpha[c] = std::max(ph.wall_blender_limit, raw[c]);
+            }
+        }
+        const std::vector<double> gh = alpha;  // gh_gcrk_alpha = gcrk_alpha
+        for (int c = 0; c < nc; ++c)
+            for (int f = 0; f < 6; ++f) {
+                if (bid[(size_t)c * 6 + f] >= 0) continue;  // face->at_boundary() (periodic included)
+                const double cur = alpha[c], nei = gh[nbr[(size_t)c * 6 + f]];
+                alpha[c] = std::max(cur, 0.5 * nei);
+            }
+        for (int c = 0; c < nc; ++c) alpha_d[c] = std::min(1.0, alpha[c] / ph.alpha_max);  // region "1"
+    }
+
+    // ---- PLENS::calc_cell_ho_inv_residual, dgsem/plens.cc:3153-3275 ----------------------------
+    void ho_inv(int c, std::vector<State>& r) const {
+        const double ss = -1;
+        std::vector<State> st(n);
+        for (int i = 0; i < n; ++i) st[i] = cons(c, i);
+        const double sgn[2] = {1, -1};
+        for (int i = 0; i < n; ++i) r[i] = State{0, 0, 0, 0, 0};
+        for (int it = 0; it < n; ++it) {
+            const State ct = st[it];
+            const int ti[3] = {it % np, (it / np) % np, it / (np * np)};
+            State fl;
+            for (int m = 0; m <= N; ++m)
+                for (int md = 0; md < 3; ++md) {
+                    int to[3] = {ti[0], ti[1], ti[2]};
+                    to[md] = m;
+                    const int io = tl(to[0], to[1], to[2]);
+                    T1 dir;
+                    for (int a = 0; a < 3; ++a) dir[a] = 0.5 * (JaV(c, it, md)[a] + JaV(c, io, md)[a]);
+                    const double nr = dir.norm();
+                    for (int a = 0; a < 3; ++a) dir[a] /= nr;
+                    ns.vol_flux(ct, st[io], dir, fl);
+                    const double Dn = 2 * D[ti[md] * np + m] * nr;
+                    for (int v = 0; v < 5; ++v) r[it][v] += Dn * fl[v];
+                }
+        }
+        for (int sd = 0; sd < 3; ++sd)
+            for (int lr = 0; lr <= 1; ++lr) {
+                const int f = 2 * sd + lr;
+                for (int q = 0; q < nf; ++q) {
+                    const int node = face_map(N, f, q);
+                    T1 dir = JaV(c, node, sd);
+                    const double nr = dir.norm();
+                    for (int a = 0; a < 3; ++a) dir[a] /= nr;
+                    State fs, fi;
+                    for (int v = 0; v < 5; ++v) fs[v] = -sgn[lr] * s2[sidx(v, c, f, q)];
+                    ns.get_inv_flux(st[node], dir, fi);
+                    for (int v = 0; v < 5; ++v) r[node][v] += -sgn[lr] * (fs[v] - fi[v]) * nr / w[lr * N];
+                }
+            }
+        for (int i = 0; i < n; ++i) {
+            const double jm = 1 / (ss * detJ[gid(c, i)]);
+            for (int v = 0; v < 5; ++v) r[i][v] *= jm;
+        }
+    }
+
+    // ---- PLENS::calc_cell_ho_dif_residual, dgsem/plens.cc:3283-3413 ----------------------------
+    void ho_dif(int c, std::vector<State>& r) const {
+        const double ss = 1;
+        std::vector<State> st(n);
+        std::vector<Avars> sa(n);
+        for (int i = 0; i < n; ++i) { st[i] = cons(c, i); sa[i] = av(c, i); }
+        const double sgn[2] = {1, -1};
+        for (int i = 0; i < n; ++i) r[i] = State{0, 0, 0, 0, 0};
+        for (int it = 0; it < n; ++it) {
+            const int ti[3] = {it % np, (it / np) % np, it / (np * np)};
+            State fl;
+            for (int m = 0; m <= N; ++m)
+                for (int md = 0; md < 3; ++md) {
+                    int to[3] = {ti[0], ti[1], ti[2]};
+                    to[md] = m;
+                    const int io = tl(to[0], to[1], to[2]);
+                    T1 dir = JaV(c, io, md);
+                    const double nr = dir.norm();
+                    for (int a = 0; a < 3; ++a) dir[a] /= nr;
+                    NS::dif_flux(st[io], sa[io], dir, fl);
+                    const double Dn = D[ti[md] * np + m] * nr;
+                    for (int v = 0; v < 5; ++v) r[it][v] += Dn * fl[v];
+                }
+        }
+        for (int sd = 0; sd < 3; ++sd)
+            for (int lr = 0; lr <= 1; ++lr) {
+                const int f = 2 * sd + lr;
+                for (int q = 0; q < nf; ++q) {
+                    const int node = face_map(N, f, q);
+                    T1 dir = JaV(c, node, sd);
+                    const double nr = dir.norm();
+                    for (int a = 0; a < 3; ++a) dir[a] /= nr;
+                    State fs, fi;
+                    for (int v = 0; v < 5; ++v) fs[v] = -sgn[lr] * s3[sidx(v, c, f, q)];
+                    NS::dif_flux(st[node], sa[node], dir, fi);
+                    for (int v = 0; v < 5; ++v) r[node][v] += -sgn[lr] * (fs[v] - fi[v]) * nr / w[lr * N];
+                }
+            }
+        for (int i = 0; i < n; ++i) {
+            const double jm = 1 / (ss * detJ[gid(c, i)]);
+            for (int v = 0; v < 5; ++v) r[i][v] *= jm;
+        }
+    }
+
+    // ---- PLENS::calc_cell_lo_inv_residual, dgsem/plens.cc:3469-3583 ----------------------------
+    void lo_inv(int c, std::vector<State>& r, NS& nsl) const {
+        for (int i = 0; i < n; ++i) r[i] = State{0, 0, 0, 0, 0};
+        nsl.set_blender(alpha[c]);
+        for (int dir = 0; dir < 3; ++dir) {
+            const int d1 = (dir + 1) % 3, d2 = (dir + 2) % 3;
+            for (int id1 = 0; id1 <= N; ++id1)
+                for (int id2 = 0; id2 <= N; ++id2)
+                    for (int id = 1; id <= N; ++id) {
+                        int tlf[3], trt[3];
+                        tlf[dir] = id - 1; tlf[d1] = id1; tlf[d2] = id2;
+                        trt[dir] = id; trt[d1] = id1; trt[d2] = id2;
+                        const int il = tl(tlf[0], tlf[1], tlf[2]), ir = tl(trt[0], trt[1], trt[2]);
+                        const State cl = cons(c, il), cr = cons(c, ir);
+                        T1 nd = subn(c, dir, trt);
+                        const double nn = nd.norm();
+                        for (int a = 0; a < 3; ++a) nd[a] /= nn;
+                        State fl;
+                        nsl.get_inv_surf_flux(cl, cr, nd, fl);
+                        for (int v = 0; v < 5; ++v) {
+                            r[il][v] += fl[v] * nn / w[id - 1];
+                            r[ir][v] -= fl[v] * nn / w[id];
+                        }
+                    }
+        }
+        for (int dir = 0; dir < 3; ++dir)
+            for (int lr = 0; lr <= 1; ++lr) {
+                const int f = 2 * dir + lr;
+                const float cs = lr ? 1.0f : -1.0f, fs = lr ? 1.0f : -1.0f;
+                for (int q = 0; q < nf; ++q) {
+                    const int node = face_map(N, f, q);
+                    const double nn = JaV(c, node, dir).norm();
+                    for (int v = 0; v < 5; ++v) {
+                        const double fl = fs * s2[sidx(v, c, f, q)];
+                        r[node][v] += cs * fl * nn / w[lr * N];
+                    }
+                }
+            }
+        for (int i = 0; i < n; ++i)
+            for (int v = 0; v < 5; ++v) r[i][v] /= -detJ[gid(c, i)];
+    }
+
+    // ---- PLENS::calc_rhs, dgsem/plens.cc:3688-3779 ------------------------------------------------
+    int calc_rhs() {
+        auto t0 = std::chrono::steady_clock::now();
+        auto lap = [&](int k) {
+            auto t1 = std::chrono::steady_clock::now();
+            timers[k] += std::chrono::duration<double>(t1 - t0).count();
+            t0 = t1;
+        };
+        if (!assert_positivity()) return -3;
+        lap(0);
+        if (!ns.inviscid()) calc_aux_vars();
+        lap(1);
+        calc_blender();
+        lap(2);
+        calc_surf_flux(2, s2);
+        if (!ns.inviscid()) calc_surf_flux(3, s3);
+        lap(3);
+        double tho = 0, tlo = 0;
+#pragma omp parallel reduction(+ : tho, tlo)
+        {
+            std::vector<State> hi(n), hd(n), lo(n);
+            NS nsl = ns;
+#pragma omp for schedule(static)
+            for (int c = 0; c < nc; ++c) {
+                auto a0 = std::chrono::steady_clock::now();
+                ho_inv(c, hi);
+                if (!ns.inviscid()) ho_dif(c, hd);
+                auto a1 = std::chrono::steady_clock::now();
+                lo_inv(c, lo, nsl);
+                auto a2 = std::chrono::steady_clock::now();
+                tho += std::chrono::duration<double>(a1 - a0).count();
+                tlo += std::chrono::duration<double>(a2 - a1).count();
+                const double al = alpha[c], ad = alpha_d[c];
+                for (int i = 0; i < n; ++i)
+                    for (int v = 0; v < 5; ++v) {
+                        double& o = rhs[v * ndof + gid(c, i)];
+                        if (!ns.inviscid()) o = (1 - ad) * hd[i][v] + al * lo[i][v] + (1 - al) * hi[i][v];
+                        else o = al * lo[i][v] + (1 - al) * hi[i][v];
+                    }
+            }
+        }
+        (void)tho; (void)tlo;
+        lap(4);
+        return 0;
+    }
+
+    // ---- PLENS::calc_time_step (global stepping), dgsem/plens.cc:3810-3912 -----------------------
+    double calc_time_step(double Co) {
+        if (Co >= 1) Co = 0.9;
+        double proc = 1e6;
+        for (int c = 0; c < nc; ++c) {
+            double cs = 1e6;
+            const double h = hmin[c];
+            for (int i = 0; i < n; ++i) {
+                const State s = cons(c, i);
+                const double a = ns.get_a(s);
+                const double vel[3] = {s[1] / s[0], s[2] / s[0], s[3] / s[0]};
+                const double* Ji = &Jinv[gid(c, i) * 9];
+                double adv[3] = {0, 0, 0};
+                for (int row = 0; row < 3; ++row)
+                    for (int col = 0; col < 3; ++col) adv[row] += Ji[col * 3 + row] * vel[col];  // JinvT
+                const double madv = std::max({adv[0], adv[1], adv[2]});
+                const double cur = Co / (std::pow(N, 1.5) * (madv + a / h + N * N * mu[gid(c, i)] / (h * h * s[0])));
+                if (cur < cs) cs = cur;
+            }
+            if (cs < proc) proc = cs;
+        }
+        return proc;
+    }
+
+    // ---- PLENS::update + update_rk4, dgsem/plens.cc:4334-4403, 4484-4518 --------------------------
+    int rk4_step(double Co, double* t, double fixed_dt = 0) {
+        // gcrk_cvars = g_cvars (the state lives in Q throughout)
+        int e = calc_rhs();
+        if (e) return e;
+        auto t0 = std::chrono::steady_clock::now();
+        const double dt = fixed_dt > 0 ? fixed_dt : calc_time_step(Co);
+        auto t1 = std::chrono::steady_clock::now();
+        timers[6] += std::chrono::duration<double>(t1 - t0).count();
+        for (int stage = 1; stage <= 5; ++stage) {
+            if (stage > 1) {
+                e = calc_rhs();
+                if (e) return e;
+            }
+            auto u0 = std::chrono::steady_clock::now();
+            for (size_t i = 0; i < 5 * ndof; ++i) rhs[i] = rhs[i] * dt;  // multiply_time_step_to_rhs
+            if (stage == 1) {
+                for (size_t i = 0; i < 5 * ndof; ++i) {
+                    Q[i] = Q[i] + rk.ao[0] * rhs[i];
+                    rhs1[i] = rhs[i];
+                }
+            } else if (stage == 2) {
+                for (size_t i = 0; i < 5 * ndof; ++i) {
+                    Q[i] = Q[i] + ((rk.ai[0] - rk.ao[0]) * rhs1[i] + rk.ao[1] * rhs[i]);
+                    rhs2[i] = rhs1[i];
+                    rhs1[i] = rhs[i];
+                }
+            } else {
+                const int s = stage;
+                for (size_t i = 0; i < 5 * ndof; ++i) {
+                    Q[i] = Q[i] + ((rk.b[s - 3] - rk.ai[s - 3]) * rhs2[i] +
+                                   (rk.ai[s - 2] - rk.ao[s - 2]) * rhs1[i] + rk.ao[s - 1] * rhs[i]);
+                    rhs2[i] = rhs1[i];
+                    rhs1[i] = rhs[i];
+                }
+            }
+            auto u1 = std::chrono::steady_clock::now();
+            timers[7] += std::chrono::duration<double>(u1 - u0).count();
+        }
+        *t += dt;
+        return 0;
+    }
+};
+
+// =============================================================================================
+// C API
+// =============================================================================================
+extern "C" {
+
+int oracle_create(int32_t n_cells, int32_t degree, const double* vertices, const int32_t* face_nbr,
+                  const int32_t* face_bid, const oracle_physics* phys, const oracle_bc* bcs, int32_t n_bcs,
+                  int32_t transpose_y, oracle** out) {
+    if (n_cells < 1 || degree < 1 || degree > 8 || !vertices || !face_nbr || !face_bid || !phys || !out)
+        return -1;
+    oracle* o = new (std::nothrow) oracle;
+    if (!o) return -4;
+    o->N = degree; o->np = degree + 1; o->n = o->np * o->np * o->np; o->nf = o->np * o->np;
+    o->nc = n_cells; o->ndof = (size_t)n_cells * o->n;
+    o->ph = *phys;
+    o->ns = NS{phys->gamma, phys->molecular_weight, phys->prandtl, phys->mu0, phys->T0, phys->S,
+               phys->inv_surf_flux};
+    o->bcs.assign(bcs, bcs + n_bcs);
+    o->nbr.assign(face_nbr, face_nbr + (size_t)n_cells * 6);
+    o->bid.assign(face_bid, face_bid + (size_t)n_cells * 6);
+    gll(degree, o->x1, o->w);
+    const int np = o->np;
+    o->D.resize(np * np); o->Qm.resize(np * np);
+    for (int r = 0; r < np; ++r)   // calc_metric_terms, plens.cc:1650-1663
+        for (int c = 0; c < np; ++c) {
+            o->D[r * np + c] = lagrange_deriv(o->x1, c, o->x1[r]);
+            o->Qm[r * np + c] = o->w[r] * o->D[r * np + c];
+        }
+    cob_matrix(degree, o->C);
+    o->metrics(vertices, transpose_y);
+    const size_t nd = o->ndof;
+    o->Q.assign(5 * nd, 0); o->avars.assign(9 * nd, 0); o->mu.assign(nd, 0); o->kap.assign(nd, 0);
+    o->alpha.assign(n_cells, 0); o->alpha_d.assign(n_cells, 0);
+    o->rhs.assign(5 * nd, 0); o->rhs1.assign(5 * nd, 0); o->rhs2.assign(5 * nd, 0);
+    *out = o;
+    return 0;
+}
+
+void oracle_destroy(oracle* o) { delete o; }
+
+int oracle_set_threads(int32_t n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+int oracle_set_state(oracle* o, const double* Q) {
+    std::memcpy(o->Q.data(), Q, sizeof(double) * 5 * o->ndof);
+    return 0;
+}
+int oracle_get_state(const oracle* o, double* Q) {
+    std::memcpy(Q, o->Q.data(), sizeof(double) * 5 * o->ndof);
+    return 0;
+}
+
+int oracle_calc_rhs(oracle* o, double* rhs, double* alpha, double* alpha_d, double* avars) {
+    const int e = o->calc_rhs();
+    if (e) return e;
+    if (rhs) std::memcpy(rhs, o->rhs.data(), sizeof(double) * 5 * o->ndof);
+    if (alpha) std::memcpy(alpha, o->alpha.data(), sizeof(double) * o->nc);
+    if (alpha_d) std::memcpy(alpha_d, o->alpha_d.data(), sizeof(double) * o->nc);
+    if (avars) std::memcpy(avars, o->avars.data(), sizeof(double) * 9 * o->ndof);
+    return 0;
+}
+
+int oracle_calc_dt(oracle* o, double courant, double* dt) {
+    // mu from the state, as calc_aux_vars does before calc_time_step
+    const double cvv = o->ns.cv();
+    for (size_t i = 0; i < o->ndof; ++i) {
+        State s;
+        for (int v = 0; v < 5; ++v) s[v] = o->Q[v * o->ndof + i];
+        o->mu[i] = o->ns.inviscid() ? 0.0 : o->ns.mu(NS::get_e(s) / cvv);
+    }
+    *dt = o->calc_time_step(courant);
+    return 0;
+}
+
+int oracle_rk4_step(oracle* o, double courant, double* t) { return o->rk4_step(courant, t); }
+
+int oracle_rk4_step_fixed_dt(oracle* o, double dt, double* t) { return o->rk4_step(0.0, t, dt); }
+
+int oracle_run(oracle* o, int32_t n_steps, double courant, double* t) {
+    for (int s = 0; s < n_steps; ++s) {
+        const int e = o->rk4_step(courant, t);
+        if (e) return e;
+    }
+    return 0;
+}
+
+int oracle_last_failure(const oracle* o, int64_t* dof, int32_t* kind) {
+    if (dof) *dof = o->fail_dof;
+    if (kind) *kind = o->fail_kind;
+    return 0;
+}
+
+int oracle_section_times(const oracle* o, double* t, int32_t n) {
+    for (int i = 0; i < n && i < 8; ++i) t[i] = o->timers[i];
+    return 0;
+}
+
+int oracle_cell_metrics(const oracle* o, int32_t c, double* Ja, double* detJ, double* normals) {
+    const int n = o->n, nf = o->nf;
+    for (int node = 0; node < n; ++node) {
+        for (int d = 0; d < 3; ++d)
+            for (int a = 0; a < 3; ++a) Ja[(3 * d + a) * n + node] = o->JaV(c, node, d)[a];
+        detJ[node] = o->detJ[o->gid(c, node)];
+    }
+    for (int f = 0; f < 6; ++f)
+        for (int d = 0; d < 3; ++d)
+            for (int q = 0; q < nf; ++q)
+                normals[(f * 3 + d) * nf + q] = o->normals[(((size_t)c * 6 + f) * nf + q) * 3 + d];
+    return 0;
+}
+
+int oracle_cell_subcell_normals(const oracle* o, int32_t c, double* sn) {
+    const int N = o->N, np = o->np;
+    size_t k = 0;
+    for (int dir = 0; dir < 3; ++dir)
+        for (int id = 0; id <= N + 1; ++id)
+            for (int id1 = 0; id1 < np; ++id1)
+                for (int id2 = 0; id2 < np; ++id2) {
+                    const T1& s = o->sub[o->sub_index(c, dir, id, id1, id2)];
+                    for (int a = 0; a < 3; ++a) sn[k++] = s[a];
+                }
+    return 0;
+}
+
+int oracle_basis(int32_t N, double* x, double* w, double* D) {
+    std::vector<double> xx, ww;
+    gll(N, xx, ww);
+    for (int i = 0; i <= N; ++i) { x[i] = xx[i]; w[i] = ww[i]; }
+    if (D)
+        for (int r = 0; r <= N; ++r)
+            for (int c = 0; c <= N; ++c) D[r * (N + 1) + c] = lagrange_deriv(xx, c, xx[r]);
+    return 0;
+}
+
+int oracle_cob_matrix(int32_t N, double* C) {
+    std::vector<double> c;
+    cob_matrix(N, c);
+    std::memcpy(C, c.data(), sizeof(double) * c.size());
+    return 0;
+}
+
+double oracle_log_avg(double xl, double xr) { return NS::log_avg(xl, xr); }
+
+static NS make_ns(const oracle_physics* p, int scheme) {
+    return NS{p->gamma, p->molecular_weight, p->prandtl, p->mu0, p->T0, p->S, scheme};
+}
+
+void oracle_inv_surf_flux(const oracle_physics* p, int32_t scheme, double blender, const double* ul,
+                          const double* ur, const double* n, double* f) {
+    NS ns = make_ns(p, scheme);
+    ns.set_blender(blender);
+    State l, r, fl;
+    for (int v = 0; v < 5; ++v) { l[v] = ul[v]; r[v] = ur[v]; }
+    T1 nn; nn[0] = n[0]; nn[1] = n[1]; nn[2] = n[2];
+    ns.get_inv_surf_flux(l, r, nn, fl);
+    for (int v = 0; v < 5; ++v) f[v] = fl[v];
+}
+
+void oracle_inv_flux(const oracle_physics* p, const double* u, const double* dir, double* f) {
+    NS ns = make_ns(p, 0);
+    State s, fl;
+    for (int v = 0; v < 5; ++v) s[v] = u[v];
+    T1 d; d[0] = dir[0]; d[1] = dir[1]; d[2] = dir[2];
+    ns.get_inv_flux(s, d, fl);
+    for (int v = 0; v < 5; ++v) f[v] = fl[v];
+}
+
+void oracle_vol_flux(const oracle_physics* p, const double* u1, const double* u2, const double* dir, double* f) {
+    NS ns = make_ns(p, 0);
+    State a, b, fl;
+    for (int v = 0; v < 5; ++v) { a[v] = u1[v]; b[v] = u2[v]; }
+    T1 d; d[0] = dir[0]; d[1] = dir[1]; d[2] = dir[2];
+    ns.vol_flux(a, b, d, fl);
+    for (int v = 0; v < 5; ++v) f[v] = fl[v];
+}
+
+void oracle_dif_flux(const double* u, const double* av, const double* dir, double* f) {
+    State s, fl;
+    Avars a;
+    for (int v = 0; v < 5; ++v) s[v] = u[v];
+    for (int v = 0; v < 9; ++v) a[v] = av[v];
+    T1 d; d[0] = dir[0]; d[1] = dir[1]; d[2] = dir[2];
+    NS::dif_flux(s, a, d, fl);
+    for (int v = 0; v < 5; ++v) f[v] = fl[v];
+}
+
+double oracle_blender_value(const oracle_physics* p, int32_t N, const double* var) {
+    std::vector<double> C;
+    cob_matrix(N, C);
+    NS ns = make_ns(p, 0);
+    return blender_from_values(ns, *p, N, C, var);
+}
+
+void oracle_rk4_coeffs(double* ao, double* ai, double* b) {
+    RKC r;
+    for (int i = 0; i < 5; ++i) { ao[i] = r.ao[i]; b[i] = r.b[i]; }
+    for (int i = 0; i < 4; ++i) ai[i] = r.ai[i];
+}
+
+}  // extern "C"

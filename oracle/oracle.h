@@ -1,0 +1,85 @@
+/*
+ * oracle.h -- TEST INFRASTRUCTURE ONLY. CPU restatement of the PLENS explicit DGSEM right-hand
+ * side and RK4(3)5[3R+]N update, used as the parity checker of the GPU path and as the CPU
+ * baseline timed beside it. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * may load it; the product path never does.
+ *
+ * It follows the reference's loop nests and operation order in FP64:
+ *   dgsem/plens.cc 1717-1913 (calc_surf_flux), 2174-2285 (calc_cell_cons_grad, CVAR_GRAD build),
+ *   2531-2546 (assert_positivity), 2593-2748 (calc_aux_vars), 2858-2962 (calc_blender),
+ *   3153-3583 (ho inviscid, ho diffusive, lo inviscid residuals), 3688-3779 (calc_rhs),
+ *   3810-3940 (calc_time_step, multiply_time_step_to_rhs), 4334-4403 (update_rk4),
+ *   4484-4518 (update);
+ *   modelling/navier_stokes.{h,cc} (thermo, fluxes, log mean), modelling/ausm_functions.h;
+ *   dgsem/blender_calculator.cc 24-87, 123-187, 277-303; dgsem/change_of_basis_matrix.h 74-98;
+ *   dgsem/metric_terms.cc 67-142; dgsem/face_dof_info.h 92-150; dgsem/cell_dof_info.h 60-88;
+ *   dgsem/rk4_stage5_register3.cc 12-36; IBCs/{periodic,uniform_inflow,outflow,uniform_temp_wall}.cc.
+ * Its metric terms, basis and face normals are computed here from the cell vertices, independent
+ * of the product's host layer.
+ */
+#ifndef PLENS_ORACLE_H
+#define PLENS_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle oracle;
+
+typedef struct {
+    double gamma, molecular_weight, prandtl, mu0, T0, S;
+    int32_t inv_surf_flux;  /* 0 HLLC, 1 Rusanov, 2 AUSM+-up, 3 Chandrashekhar */
+    double threshold_factor, threshold_exp_factor, sharpness_factor, alpha_min, alpha_max,
+        wall_blender_limit;
+} oracle_physics;
+
+typedef struct { int32_t type; double p, T, vel[3]; } oracle_bc; /* type as PLENS_BC_* */
+
+/* vertices [n_cells][8][3] (deal.II order), face_nbr/face_bid [n_cells][6] as plens_mesh_desc */
+int oracle_create(int32_t n_cells, int32_t degree, const double* vertices, const int32_t* face_nbr,
+                  const int32_t* face_bid, const oracle_physics* phys, const oracle_bc* bcs,
+                  int32_t n_bcs, int32_t transpose_y_face_normals, oracle** out);
+void oracle_destroy(oracle* o);
+int oracle_set_threads(int32_t n);
+int oracle_set_state(oracle* o, const double* Q);
+int oracle_get_state(const oracle* o, double* Q);
+/* calc_rhs on the current state; outputs nullable */
+int oracle_calc_rhs(oracle* o, double* rhs, double* alpha, double* alpha_d, double* avars);
+int oracle_calc_dt(oracle* o, double courant, double* dt);
+/* one update() with RK4; *t += dt */
+int oracle_rk4_step(oracle* o, double courant, double* t);
+/* one update() with a prescribed dt (time-order checks of the RK registers) */
+int oracle_rk4_step_fixed_dt(oracle* o, double dt, double* t);
+int oracle_run(oracle* o, int32_t n_steps, double courant, double* t);
+int oracle_last_failure(const oracle* o, int64_t* dof, int32_t* kind);
+/* section timers (seconds, accumulated) named as plens_test::time(): [0] assert positivity,
+ * [1] aux vars, [2] blender, [3] surface fluxes 2&3, [4] ho residual, [5] lo residual,
+ * [6] time step, [7] updating cvars */
+int oracle_section_times(const oracle* o, double* t, int32_t n);
+
+/* metric terms of the oracle for one cell: Ja [9][n], detJ [n], face normals [6][3][n_f] */
+int oracle_cell_metrics(const oracle* o, int32_t cell, double* Ja, double* detJ, double* normals);
+/* subcell normals of one cell: [3 dir][N+2 id][N+1 id1][N+1 id2][3] */
+int oracle_cell_subcell_normals(const oracle* o, int32_t cell, double* sn);
+
+/* ---- unit-level restatements (for known-answer and invariant tests) ---- */
+int oracle_basis(int32_t degree, double* x, double* w, double* D);
+/* dense (N+1)^3 x (N+1)^3 change-of-basis matrix, ChangeOfBasisMatrix ctor */
+int oracle_cob_matrix(int32_t degree, double* C);
+double oracle_log_avg(double xl, double xr);
+/* inviscid numerical flux through unit normal n (get_inv_surf_flux with rotation) */
+void oracle_inv_surf_flux(const oracle_physics* p, int32_t scheme, double blender, const double* ul,
+                          const double* ur, const double* n, double* f);
+void oracle_inv_flux(const oracle_physics* p, const double* u, const double* dir, double* f);
+void oracle_vol_flux(const oracle_physics* p, const double* u1, const double* u2, const double* dir,
+                     double* f);
+void oracle_dif_flux(const double* u, const double* av, const double* dir, double* f);
+/* blender of one cell from nodal values of the blender variable (get_blender) */
+double oracle_blender_value(const oracle_physics* p, int32_t degree, const double* var);
+void oracle_rk4_coeffs(double* a_outer5, double* a_inner4, double* b5);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

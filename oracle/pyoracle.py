@@ -1,0 +1,187 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker; the
+product path never imports it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_dp = C.POINTER(C.c_double)
+
+
+class OPhys(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("molecular_weight", C.c_double), ("prandtl", C.c_double),
+                ("mu0", C.c_double), ("T0", C.c_double), ("S", C.c_double), ("inv_surf_flux", C.c_int32),
+                ("threshold_factor", C.c_double), ("threshold_exp_factor", C.c_double),
+                ("sharpness_factor", C.c_double), ("alpha_min", C.c_double), ("alpha_max", C.c_double),
+                ("wall_blender_limit", C.c_double)]
+
+
+class OBC(C.Structure):
+    _fields_ = [("type", C.c_int32), ("p", C.c_double), ("T", C.c_double), ("vel", C.c_double * 3)]
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        from . import build
+        build.build()
+    lib = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    ip = C.POINTER(C.c_int32)
+    sig = {
+        "oracle_create": (C.c_int, [C.c_int32, C.c_int32, _dp, ip, ip, C.POINTER(OPhys), C.POINTER(OBC), C.c_int32,
+                                    C.c_int32, C.POINTER(vp)]),
+        "oracle_destroy": (None, [vp]),
+        "oracle_set_threads": (C.c_int, [C.c_int32]),
+        "oracle_set_state": (C.c_int, [vp, _dp]),
+        "oracle_get_state": (C.c_int, [vp, _dp]),
+        "oracle_calc_rhs": (C.c_int, [vp, _dp, _dp, _dp, _dp]),
+        "oracle_calc_dt": (C.c_int, [vp, C.c_double, _dp]),
+        "oracle_rk4_step": (C.c_int, [vp, C.c_double, _dp]),
+        "oracle_run": (C.c_int, [vp, C.c_int32, C.c_double, _dp]),
+        "oracle_rk4_step_fixed_dt": (C.c_int, [vp, C.c_double, _dp]),
+        "oracle_last_failure": (C.c_int, [vp, C.POINTER(C.c_int64), ip]),
+        "oracle_section_times": (C.c_int, [vp, _dp, C.c_int32]),
+        "oracle_cell_metrics": (C.c_int, [vp, C.c_int32, _dp, _dp, _dp]),
+        "oracle_cell_subcell_normals": (C.c_int, [vp, C.c_int32, _dp]),
+        "oracle_basis": (C.c_int, [C.c_int32, _dp, _dp, _dp]),
+        "oracle_cob_matrix": (C.c_int, [C.c_int32, _dp]),
+        "oracle_log_avg": (C.c_double, [C.c_double, C.c_double]),
+        "oracle_inv_surf_flux": (None, [C.POINTER(OPhys), C.c_int32, C.c_double, _dp, _dp, _dp, _dp]),
+        "oracle_inv_flux": (None, [C.POINTER(OPhys), _dp, _dp, _dp]),
+        "oracle_vol_flux": (None, [C.POINTER(OPhys), _dp, _dp, _dp, _dp]),
+        "oracle_dif_flux": (None, [_dp, _dp, _dp, _dp]),
+        "oracle_blender_value": (C.c_double, [C.POINTER(OPhys), C.c_int32, _dp]),
+        "oracle_rk4_coeffs": (None, [_dp, _dp, _dp]),
+    }
+    for k, (r, a) in sig.items():
+        f = getattr(lib, k)
+        f.restype, f.argtypes = r, a
+    _lib = lib
+    return lib
+
+
+def p(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(_dp)
+
+
+def ophys_from(ph):
+    """OPhys from a plens_amd Physics struct (same fields minus precision)."""
+    o = OPhys()
+    for name, _ in OPhys._fields_:
+        setattr(o, name, getattr(ph, name))
+    return o
+
+
+class OraclePlens:
+    """CPU restatement of PLENS::calc_rhs / calc_time_step / update (RK4)."""
+
+    def __init__(self, case=None, mesh=None, physics=None, bcs=None, threads=0):
+        if case is not None:
+            mesh, physics, bcs = case.mesh, case.physics, case.bcs
+        self.lib = load()
+        if threads:
+            self.lib.oracle_set_threads(threads)
+        self.n_cells, self.n_dof, self.n = mesh.n_cells, mesh.n_dof, mesh.n
+        self._ph = ophys_from(physics)
+        arr = (OBC * max(len(bcs), 1))()
+        for i, b in enumerate(bcs):
+            arr[i].type, arr[i].p, arr[i].T = b.type, b.p, b.T
+            for d in range(3):
+                arr[i].vel[d] = b.vel[d]
+        self._bcs = arr
+        self._V = np.ascontiguousarray(mesh.vertices)
+        self._nbr = np.ascontiguousarray(mesh.face_nbr, dtype=np.int32)
+        self._bid = np.ascontiguousarray(mesh.face_bid, dtype=np.int32)
+        self._h = C.c_void_p()
+        ip = C.POINTER(C.c_int32)
+        rc = self.lib.oracle_create(mesh.n_cells, mesh.degree, self._V.ctypes.data_as(_dp),
+                                    self._nbr.ctypes.data_as(ip), self._bid.ctypes.data_as(ip), C.byref(self._ph),
+                                    arr, len(bcs), int(mesh.transpose_y_face_normals), C.byref(self._h))
+        if rc:
+            raise RuntimeError(f"oracle_create failed {rc}")
+        if case is not None:
+            self.set_state(case.Q0)
+
+    def set_state(self, Q):
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        self.lib.oracle_set_state(self._h, Q.ctypes.data_as(_dp))
+
+    def get_state(self):
+        Q = np.empty((5, self.n_dof))
+        self.lib.oracle_get_state(self._h, Q.ctypes.data_as(_dp))
+        return Q
+
+    def _fail(self, rc):
+        dof, kind = C.c_int64(), C.c_int32()
+        self.lib.oracle_last_failure(self._h, C.byref(dof), C.byref(kind))
+        raise RuntimeError(f"oracle positivity failure rc={rc} dof={dof.value} kind={kind.value}")
+
+    def calc_rhs(self, want_avars=False):
+        rhs = np.empty((5, self.n_dof))
+        a = np.empty(self.n_cells)
+        ad = np.empty(self.n_cells)
+        av = np.empty((9, self.n_dof)) if want_avars else None
+        rc = self.lib.oracle_calc_rhs(self._h, rhs.ctypes.data_as(_dp), a.ctypes.data_as(_dp),
+                                      ad.ctypes.data_as(_dp), av.ctypes.data_as(_dp) if av is not None else None)
+        if rc:
+            self._fail(rc)
+        return (rhs, a, ad, av) if want_avars else (rhs, a, ad)
+
+    def calc_time_step(self, courant):
+        dt = C.c_double()
+        self.lib.oracle_calc_dt(self._h, courant, C.byref(dt))
+        return dt.value
+
+    def update(self, courant):
+        t = C.c_double(0.0)
+        rc = self.lib.oracle_rk4_step(self._h, courant, C.byref(t))
+        if rc:
+            self._fail(rc)
+        return t.value
+
+    def update_fixed_dt(self, dt):
+        t = C.c_double(0.0)
+        rc = self.lib.oracle_rk4_step_fixed_dt(self._h, dt, C.byref(t))
+        if rc:
+            self._fail(rc)
+        return t.value
+
+    def run(self, n_steps, courant):
+        t = C.c_double(0.0)
+        rc = self.lib.oracle_run(self._h, n_steps, courant, C.byref(t))
+        if rc:
+            self._fail(rc)
+        return t.value
+
+    def section_times(self):
+        t = np.zeros(8)
+        self.lib.oracle_section_times(self._h, t.ctypes.data_as(_dp), 8)
+        return t
+
+    def cell_metrics(self, c, n, nf):
+        Ja, dJ, nr = np.empty((9, n)), np.empty(n), np.empty((6, 3, nf))
+        self.lib.oracle_cell_metrics(self._h, c, Ja.ctypes.data_as(_dp), dJ.ctypes.data_as(_dp),
+                                     nr.ctypes.data_as(_dp))
+        return Ja, dJ, nr
+
+    def close(self):
+        if self._h:
+            self.lib.oracle_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

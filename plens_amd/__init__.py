@@ -1,0 +1,8 @@
+"""MI355X-native DGSEM right-hand side + RK4(3)5[3R+]N update of PLENS (gfx950 HIP kernels).
+
+The native library libplens_gpu.so (built in-tree by plens_amd.build) holds the kernels, the C ABI
+of include/plens_gpu.h and the host setup of include/plens_host.h.
+"""
+from . import _lib  # noqa: F401
+from .cases import CONFIGS, Mesh, periodic_cube, shock_tube, wall_flow  # noqa: F401
+from .solver import GpuPlens, PositivityError  # noqa: F401

@@ -1,0 +1,1184 @@
+// MI355X (gfx950) DGSEM right-hand side + RK4(3)5[3R+]N update behind the C ABI of plens_gpu.h.
+//
+// Per RK stage four kernels run on the handle's stream:
+//   k_grad   one workgroup per batch of cells, one thread per GLL node: positivity check,
+//            Sutherland mu/k, Hennemann blender (sum-factorised modal transform), the stage-1
+//            time-step candidates, BR1 conservative gradients with stage-1 face values gathered
+//            from the neighbours, and the auxiliary variables (stress, heat flux)
+//            (PLENS::calc_aux_vars / calc_cell_cons_grad / calc_blender, dgsem/plens.cc:2593-2748,
+//            2174-2285, 2858-2962; calc_time_step 3810-3912)
+//   k_alpha  one thread per cell: wall override, face-neighbour diffusion and alpha_d of the
+//            blender; block 0 also reduces the time step (dgsem/plens.cc:2905-2960, 3872-3875)
+//   k_face   one thread per face node: stage-2 inviscid numerical flux and stage-3 BR1 viscous
+//            flux, computed once by the lower-index cell of an interior face and stored for both
+//            sides, each side on its own for boundary (incl. periodic) faces
+//            (PLENS::calc_surf_flux, dgsem/plens.cc:1717-1913)
+//   k_vol    one workgroup per batch of cells, one thread per node: split-form inviscid volume
+//            term, viscous volume term, surface terms, first-order subcell FV for blended cells,
+//            the alpha blend and the fused low-storage RK epilogue (dgsem/plens.cc:3153-3583,
+//            3688-3779, 4334-4403)
+// A time step (5 stages) is captured once into a HIP graph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "flux.h"
+#include "plens_gpu.h"
+#include "plens_host.h"
+
+using namespace plens_dev;
+
+namespace {
+
+constexpr int kMaxNP = 6;
+constexpr int kThreadsFace = 256;
+constexpr int kThreadsAlpha = 256;
+
+struct BCDev {
+    int type;
+    double p, T, vel[3], cons_pr[5];
+};
+
+struct Basis {
+    double w[kMaxNP];
+    double D[kMaxNP * kMaxNP];
+    double Q[kMaxNP * kMaxNP];
+    double C1[kMaxNP * kMaxNP];
+};
+
+struct KP {
+    int nc, cpb, ntasks, n_dt_part;
+    long long ndof;
+    const double* Q;
+    double* Qout;
+    const double* Ja;
+    const double* detJ;
+    const double* nrm;
+    const double* hmin;
+    const int* nbr;
+    const int* bid;
+    const int* wall;
+    const int* tasks;
+    const BCDev* bcs;
+    const Basis* basis;
+    double* avars;
+    double* araw;
+    double* alpha;
+    double* alpha_d;
+    double* sinv;
+    double* svis;
+    double* Rnew;
+    const double* R1;
+    const double* R2;
+    double* rhs_out;
+    double* dt_part;
+    double* dt;
+    double* t_acc;
+    unsigned long long* fail;
+    Gas gas;
+    int scheme;
+    double thr, sharp, amin, amax, wlimit, courant;
+    int stage;    // 0: write rhs only; 1..5: RK stage
+    int want_dt;  // compute the stage-1 time step
+    double c1, c2, c3;
+};
+
+// FaceDoFInfo::form_maps (dgsem/face_dof_info.h:92-150)
+template <int N>
+__device__ __forceinline__ int face_node(int f, int q) {
+    constexpr int NP = N + 1, NF = NP * NP;
+    switch (f) {
+        case 0: return NP * q;
+        case 1: return N + NP * q;
+        case 2: return (q / NP) * NF + (q % NP);
+        case 3: return NP * N + (q / NP) * NF + (q % NP);
+        case 4: return q;
+        default: return N * NF + q;
+    }
+}
+// face-local dof of cell node (i,j,k) on face f
+template <int N>
+__device__ __forceinline__ int face_dof(int f, int i, int j, int k) {
+    constexpr int NP = N + 1;
+    return (f < 2) ? j + NP * k : (f < 4) ? i + NP * k : i + NP * j;
+}
+__device__ __forceinline__ bool on_face(int f, int i, int j, int k, int N) {
+    const int t = (f < 2) ? i : (f < 4) ? j : k;
+    return (f & 1) ? (t == N) : (t == 0);
+}
+
+__device__ __forceinline__ void load_state(const double* Q, long long ndof, long long g, double u[5]) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) u[v] = Q[v * ndof + g];
+}
+
+__host__ __device__ __forceinline__ void prim_to_cons(double gma, double rho, const double vel[3], double p, double c[5]) {
+    c[0] = rho;
+    c[4] = p / (gma - 1.0);
+    for (int d = 0; d < 3; ++d) {
+        c[1 + d] = rho * vel[d];
+        c[4] += 0.5 * rho * vel[d] * vel[d];
+    }
+}
+
+// BC ghost state (IBCs/*.cc): stage 1 and 3 share the state rule, stage 2 has its own
+__device__ void bc_ghost_state(const BCDev& b, int stage, const Gas& g, const double u[5], const double n[3],
+                               double ug[5]) {
+    if (b.type == PLENS_BC_UNIFORM_INFLOW) {
+        if (stage == 2) {
+            for (int v = 0; v < 5; ++v) ug[v] = b.cons_pr[v];
+        } else {
+            for (int v = 0; v < 5; ++v) ug[v] = 2.0 * b.cons_pr[v] - u[v];
+        }
+    } else if (b.type == PLENS_BC_OUTFLOW) {
+        for (int v = 0; v < 5; ++v) ug[v] = u[v];
+        if (stage == 2 && !g.inviscid) {
+            // NavierStokes::get_M (navier_stokes.cc:408-414)
+            const double ri = 1.0 / u[0];
+            const double sp = (u[1] * u[1]) * (ri * ri) + (u[2] * u[2]) * (ri * ri) + (u[3] * u[3]) * (ri * ri);
+            const double M = sqrt(sp) / get_a(u, g.gma);
+            if (!(M >= 1.0)) ug[4] += 2.0 * (b.p - get_p(u, g.gma)) / (g.gma - 1.0);
+        }
+    } else {  // uniform temperature wall
+        if (stage == 2) {
+            ug[0] = u[0];
+            ug[4] = u[4];
+            if (g.inviscid) {
+                const double rel0 = u[1] - u[0] * b.vel[0], rel1 = u[2] - u[0] * b.vel[1], rel2 = u[3] - u[0] * b.vel[2];
+                const double sn = rel0 * n[0] + rel1 * n[1] + rel2 * n[2];
+                for (int d = 0; d < 3; ++d) ug[1 + d] = u[1 + d] - 2.0 * sn * n[d];
+            } else {
+                for (int d = 0; d < 3; ++d) ug[1 + d] = 2.0 * u[0] * b.vel[d] - u[1 + d];
+            }
+        } else {
+            const double pin = get_p(u, g.gma);
+            double pr[5];
+            prim_to_cons(g.gma, pin / (g.R * b.T), b.vel, pin, pr);
+            for (int v = 0; v < 5; ++v) ug[v] = 2.0 * pr[v] - u[v];
+        }
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void load_basis(const Basis* b, double* sD, double* sW, double* sQm, double* sC) {
+    constexpr int NP = N + 1;
+    for (int t = threadIdx.x; t < NP * NP; t += blockDim.x) {
+        const int r = t / NP, c = t % NP;
+        sD[t] = b->D[r * kMaxNP + c];
+        sQm[t] = b->Q[r * kMaxNP + c];
+        sC[t] = b->C1[r * kMaxNP + c];
+        if (t < NP) sW[t] = b->w[t];
+    }
+}
+
+// =============================================================================================
+// k_grad
+// =============================================================================================
+template <int N>
+__global__ void __launch_bounds__(256) k_grad(KP p) {
+    constexpr int NP = N + 1, NN = NP * NP * NP;
+    extern __shared__ double smem[];
+    __shared__ double sD[NP * NP], sW[NP], sQm[NP * NP], sC[NP * NP];
+    const int B = p.cpb * NN;
+    double* sQ = smem;
+    double* sJa = sQ + 5 * B;
+    double* sV = sJa + 9 * B;
+    double* sT = sV + B;
+    load_basis<N>(p.basis, sD, sW, sQm, sC);
+
+    const int tid = threadIdx.x, lc = tid / NN, node = tid - lc * NN;
+    const int c = blockIdx.x * p.cpb + lc;
+    const bool inblk = lc < p.cpb;
+    const bool act = inblk && c < p.nc;
+    const int li = lc * NN + node;
+    const int ti = node % NP, tj = (node / NP) % NP, tk = node / (NP * NP);
+    const long long g = (long long)c * NN + node;
+    double u[5] = {1, 0, 0, 0, 1}, detJ = 1;
+    if (act) {
+        load_state(p.Q, p.ndof, g, u);
+        for (int v = 0; v < 5; ++v) sQ[v * B + li] = u[v];
+        for (int a = 0; a < 9; ++a) sJa[a * B + li] = p.Ja[((long long)c * 9 + a) * NN + node];
+        detJ = p.detJ[g];
+        // PLENS::assert_positivity (dgsem/plens.cc:2531-2546)
+        if (!(u[0] > 0.0)) atomicMin(p.fail, (unsigned long long)g * 4ull + 1ull);
+        else if (!(get_e(u) > 0.0)) atomicMin(p.fail, (unsigned long long)g * 4ull + 2ull);
+    } else if (inblk) {
+        for (int v = 0; v < 5; ++v) sQ[v * B + li] = u[v];
+        for (int a = 0; a < 9; ++a) sJa[a * B + li] = (a % 4 == 0) ? 1.0 : 0.0;
+    }
+    const Gas& gs = p.gas;
+    const double e = get_e(u);
+    const double mu = gs.inviscid ? 0.0 : sutherland(gs, e / gs.cv);
+    const double kap = conductivity(gs, mu);
+
+    // ---- blender: modal energies of rho*p (BlenderCalculator::get_blender, blender_calculator.cc:123-187)
+    if (inblk) sV[li] = get_p(u, gs.gma) * u[0];
+    __syncthreads();
+    if (inblk) {  // x pass
+        double s = 0;
+        for (int a = 0; a < NP; ++a) s += sC[ti * NP + a] * sV[lc * NN + a + NP * (tj + NP * tk)];
+        sT[li] = s;
+    }
+    __syncthreads();
+    if (inblk) {  // y pass
+        double s = 0;
+        for (int b = 0; b < NP; ++b) s += sC[tj * NP + b] * sT[lc * NN + ti + NP * (b + NP * tk)];
+        sV[li] = s;
+    }
+    __syncthreads();
+    if (inblk) {  // z pass, squared mode
+        double s = 0;
+        for (int cc = 0; cc < NP; ++cc) s += sC[tk * NP + cc] * sV[lc * NN + ti + NP * (tj + NP * cc)];
+        sT[li] = s * s;
+    }
+    __syncthreads();
+    if (act && node == 0) {  // get_trouble (blender_calculator.cc:277-303), Hennemann modification for N>2
+        double et = 0, e1 = 0, e2 = 0;
+        for (int m = 0; m < NN; ++m) {
+            const int mi = m % NP, mj = (m / NP) % NP, mk = m / (NP * NP);
+            const double v = sT[lc * NN + m];
+            et += v;
+            if (mi < N && mj < N && mk < N) e1 += v;
+            if (mi < N - 1 && mj < N - 1 && mk < N - 1) e2 += v;
+        }
+        const double trouble = (N > 2) ? fmax(1.0 - e1 / et, 1.0 - e2 / e1) : 1.0 - e1 / et;
+        const double at = 1.0 / (1.0 + exp(-p.sharp * (trouble / p.thr - 1.0)));
+        p.araw[c] = (at < p.amin) ? 0.0 : (at < p.amax ? at : p.amax);
+    }
+
+    // ---- stage-1 time step candidates (PLENS::calc_time_step, dgsem/plens.cc:3836-3866)
+    if (p.want_dt) {
+        __syncthreads();
+        if (inblk) {
+            double cur = 1e300;
+            if (act) {
+                const double h = p.hmin[c];
+                const double a = get_a(u, gs.gma);
+                const double vel[3] = {u[1] / u[0], u[2] / u[0], u[3] / u[0]};
+                double madv = -1e300;
+                for (int r = 0; r < 3; ++r) {  // (J^-T v)_r with J^-1 rows = Ja^i / detJ
+                    double s = 0;
+                    for (int cc = 0; cc < 3; ++cc) s += (sJa[(3 * cc + r) * B + li] / detJ) * vel[cc];
+                    madv = fmax(madv, s);
+                }
+                double co = p.courant >= 1.0 ? 0.9 : p.courant;
+                cur = co / (pow((double)N, 1.5) * (madv + a / h + (double)(N * N) * mu / (h * h * u[0])));
+            }
+            sT[li] = cur;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double m = 1e300;
+            for (int t = 0; t < B; ++t) m = fmin(m, sT[t]);
+            p.dt_part[blockIdx.x] = m;
+        }
+    }
+    if (gs.inviscid) return;
+
+    // ---- BR1 conservative gradients (calc_cell_cons_grad, dgsem/plens.cc:2174-2285)
+    if (!act) return;
+    double gr[3][5];
+    for (int d = 0; d < 3; ++d)
+        for (int v = 0; v < 5; ++v) gr[d][v] = 0.0;
+    const int tt[3] = {ti, tj, tk};
+    for (int m = 0; m <= N; ++m) {
+        for (int md = 0; md < 3; ++md) {
+            int to[3] = {ti, tj, tk};
+            to[md] = m;
+            const int jo = lc * NN + to[0] + NP * (to[1] + NP * to[2]);
+            double fl[5];
+            for (int v = 0; v < 5; ++v) fl[v] = 0.5 * (u[v] + sQ[v * B + jo]);
+            const double Dm2 = 2.0 * sD[tt[md] * NP + m];
+            for (int gd = 0; gd < 3; ++gd) {
+                const double jav = 0.5 * (sJa[(3 * md + gd) * B + li] + sJa[(3 * md + gd) * B + jo]);
+                for (int v = 0; v < 5; ++v) gr[gd][v] += Dm2 * fl[v] * jav;
+            }
+        }
+    }
+    for (int f = 0; f < 6; ++f) {
+        if (!on_face(f, ti, tj, tk, N)) continue;
+        const int q = face_dof<N>(f, ti, tj, tk);
+        const int sd = f >> 1, lr = f & 1;
+        const int b = p.bid[c * 6 + f];
+        double us[5];
+        if (b < 0 || p.bcs[b].type == PLENS_BC_PERIODIC) {
+            const int nb = p.nbr[c * 6 + f];
+            double un[5];
+            load_state(p.Q, p.ndof, (long long)nb * NN + face_node<N>(f ^ 1, q), un);
+            for (int v = 0; v < 5; ++v) us[v] = 0.5 * (u[v] + un[v]);
+        } else {
+            double ug[5];
+            bc_ghost_state(p.bcs[b], 1, gs, u, nullptr, ug);
+            for (int v = 0; v < 5; ++v) us[v] = 0.5 * (u[v] + ug[v]);
+        }
+        const double sg = lr ? -1.0 : 1.0;
+        const double wend = sW[lr * N];
+        for (int gd = 0; gd < 3; ++gd) {
+            const double ja = sJa[(3 * sd + gd) * B + li];
+            for (int v = 0; v < 5; ++v) gr[gd][v] -= sg * (us[v] * ja - u[v] * ja) / wend;
+        }
+    }
+    const double iJ = 1.0 / detJ;
+    for (int gd = 0; gd < 3; ++gd)
+        for (int v = 0; v < 5; ++v) gr[gd][v] *= iJ;
+
+    // ---- auxiliary variables (calc_aux_vars, dgsem/plens.cc:2656-2746)
+    const double vel[3] = {u[1] / u[0], u[2] / u[0], u[3] / u[0]};
+    double vg[3][3];
+    for (int vd = 0; vd < 3; ++vd)
+        for (int gd = 0; gd < 3; ++gd) vg[vd][gd] = (gr[gd][1 + vd] - gr[gd][0] * vel[vd]) / u[0];
+    const double tr = vg[0][0] + vg[1][1] + vg[2][2];
+    double av[9];
+    av[0] = vg[0][0] + vg[0][0] - 2.0 / 3 * tr;
+    av[1] = vg[0][1] + vg[1][0];
+    av[2] = vg[0][2] + vg[2][0];
+    av[3] = vg[1][1] + vg[1][1] - 2.0 / 3 * tr;
+    av[4] = vg[1][2] + vg[2][1];
+    av[5] = vg[2][2] + vg[2][2] - 2.0 / 3 * tr;
+    for (int gd = 0; gd < 3; ++gd) {
+        double eg = (gr[gd][4] - gr[gd][0] * e) / u[0];
+        for (int vd = 0; vd < 3; ++vd) eg -= (0.5 * gr[gd][0] * vel[vd] * vel[vd] + u[1 + vd] * vg[vd][gd]) / u[0];
+        av[6 + gd] = -eg / gs.cv;
+    }
+    for (int a = 0; a < 6; ++a) p.avars[a * p.ndof + g] = av[a] * mu;
+    for (int a = 6; a < 9; ++a) p.avars[a * p.ndof + g] = av[a] * kap;
+}
+
+// =============================================================================================
+// k_alpha
+// =============================================================================================
+__global__ void __launch_bounds__(kThreadsAlpha) k_alpha(KP p) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < p.nc) {
+        // wall override (dgsem/plens.cc:2909-2920) of this cell and of the neighbours' pre-diffusion values
+        double a = p.araw[c];
+        if (p.wall[c]) a = fmax(p.wlimit, a);
+        for (int f = 0; f < 6; ++f) {
+            if (p.bid[c * 6 + f] >= 0) continue;  // boundary faces (periodic included) are skipped
+            const int nb = p.nbr[c * 6 + f];
+            double an = p.araw[nb];
+            if (p.wall[nb]) an = fmax(p.wlimit, an);
+            a = fmax(a, 0.5 * an);
+        }
+        p.alpha[c] = a;
+        p.alpha_d[c] = fmin(1.0, a / p.amax);
+    }
+    if (p.want_dt && blockIdx.x == 0) {
+        __shared__ double red[kThreadsAlpha];
+        double m = 1e300;
+        for (int t = threadIdx.x; t < p.n_dt_part; t += blockDim.x) m = fmin(m, p.dt_part[t]);
+        red[threadIdx.x] = m;
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            p.dt[0] = red[0];
+            if (p.t_acc) p.t_acc[0] += red[0];
+        }
+    }
+}
+
+// =============================================================================================
+// k_face
+// =============================================================================================
+template <int N>
+__global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
+    constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (long long)p.ntasks * NF) return;
+    const int t = (int)(gid / NF), q = (int)(gid - (long long)t * NF);
+    const int cf = p.tasks[t];
+    const int c = cf / 6, f = cf - 6 * c;
+    const int b = p.bid[cf], nb = p.nbr[cf];
+    const Gas& gs = p.gas;
+    const long long g = (long long)c * NN + face_node<N>(f, q);
+    double u[5], ug[5], n[3];
+    load_state(p.Q, p.ndof, g, u);
+    for (int d = 0; d < 3; ++d) n[d] = p.nrm[((long long)cf * 3 + d) * NF + q];
+    const bool interior = b < 0;
+    const bool periodic = !interior && p.bcs[b].type == PLENS_BC_PERIODIC;
+    long long gn = 0;
+    double bl;
+    if (interior || periodic) {
+        gn = (long long)nb * NN + face_node<N>(f ^ 1, q);
+        load_state(p.Q, p.ndof, gn, ug);
+        bl = interior ? 0.5 * (p.alpha[c] + p.alpha[nb]) : p.alpha[c];
+    } else {
+        bc_ghost_state(p.bcs[b], 2, gs, u, n, ug);
+        bl = p.alpha[c];
+    }
+    bl = bl <= 0.0 ? 0.0 : (bl <= 1.0 ? bl : 1.0);  // set_flux_blender_value
+    double F[5];
+    inv_surf_flux(p.scheme, bl, u, ug, n, gs.gma, F);
+    const long long o = (long long)cf * 5 * NF + q;
+    for (int v = 0; v < 5; ++v) p.sinv[o + v * NF] = F[v];
+    const long long on = ((long long)nb * 6 + (f ^ 1)) * 5 * NF + q;
+    if (interior)
+        for (int v = 0; v < 5; ++v) p.sinv[on + v * NF] = -F[v];
+    if (gs.inviscid) return;
+    // stage 3: BR1 average of the viscous fluxes (navier_stokes.cc:1644-1652)
+    double av[9], avg[9];
+    for (int a = 0; a < 9; ++a) av[a] = p.avars[a * p.ndof + g];
+    if (interior || periodic) {
+        for (int a = 0; a < 9; ++a) avg[a] = p.avars[a * p.ndof + gn];
+    } else {
+        for (int a = 0; a < 9; ++a) avg[a] = av[a];
+        if (p.bcs[b].type != PLENS_BC_OUTFLOW) bc_ghost_state(p.bcs[b], 3, gs, u, n, ug);
+        else for (int v = 0; v < 5; ++v) ug[v] = u[v];
+    }
+    double f1[5], f2[5];
+    dif_flux(u, av, n, f1);
+    dif_flux(ug, avg, n, f2);
+    for (int v = 0; v < 5; ++v) F[v] = 0.5 * (f1[v] + f2[v]);
+    for (int v = 0; v < 5; ++v) p.svis[o + v * NF] = F[v];
+    if (interior)
+        for (int v = 0; v < 5; ++v) p.svis[on + v * NF] = -F[v];
+}
+
+// =============================================================================================
+// k_vol
+// =============================================================================================
+template <int N>
+__global__ void __launch_bounds__(256) k_vol(KP p) {
+    constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
+    extern __shared__ double smem[];
+    __shared__ double sD[NP * NP], sW[NP], sQm[NP * NP], sC[NP * NP];
+    const int B = p.cpb * NN;
+    double* sQ = smem;          // 5B
+    double* sJa = sQ + 5 * B;   // 9B
+    double* sX = sJa + 9 * B;   // 6B scratch (viscous fluxes / subcell fluxes)
+    load_basis<N>(p.basis, sD, sW, sQm, sC);
+
+    const int tid = threadIdx.x, lc = tid / NN, node = tid - lc * NN;
+    const int c = blockIdx.x * p.cpb + lc;
+    const bool inblk = lc < p.cpb;
+    const bool act = inblk && c < p.nc;
+    const int li = lc * NN + node;
+    const int ti = node % NP, tj = (node / NP) % NP, tk = node / (NP * NP);
+    const int tt[3] = {ti, tj, tk};
+    const long long g = (long long)c * NN + node;
+    const Gas& gs = p.gas;
+    double u[5] = {1, 0, 0, 0, 1}, detJ = 1;
+    if (act) {
+        load_state(p.Q, p.ndof, g, u);
+        for (int v = 0; v < 5; ++v) sQ[v * B + li] = u[v];
+        for (int a = 0; a < 9; ++a) sJa[a * B + li] = p.Ja[((long long)c * 9 + a) * NN + node];
+        detJ = p.detJ[g];
+    } else if (inblk) {
+        for (int v = 0; v < 5; ++v) sQ[v * B + li] = u[v];
+        for (int a = 0; a < 9; ++a) sJa[a * B + li] = (a % 4 == 0) ? 1.0 : 0.0;
+    }
+    const double alpha = act ? p.alpha[c] : 0.0;
+    const double alpha_d = act ? p.alpha_d[c] : 0.0;
+    __syncthreads();
+
+    // ---- high-order inviscid residual (calc_cell_ho_inv_residual, dgsem/plens.cc:3153-3275)
+    double ri[5] = {0, 0, 0, 0, 0};
+    if (inblk) {
+        for (int m = 0; m <= N; ++m) {
+            for (int md = 0; md < 3; ++md) {
+                int to[3] = {ti, tj, tk};
+                to[md] = m;
+                const int jo = lc * NN + to[0] + NP * (to[1] + NP * to[2]);
+                double uj[5], dir[3];
+                for (int v = 0; v < 5; ++v) uj[v] = sQ[v * B + jo];
+                for (int a = 0; a < 3; ++a) dir[a] = 0.5 * (sJa[(3 * md + a) * B + li] + sJa[(3 * md + a) * B + jo]);
+                const double nr = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+                for (int a = 0; a < 3; ++a) dir[a] /= nr;
+                double fl[5];
+                vol_flux(u, uj, dir, gs.gma, fl);
+                const double Dn = 2.0 * sD[tt[md] * NP + m] * nr;
+                for (int v = 0; v < 5; ++v) ri[v] += Dn * fl[v];
+            }
+        }
+    }
+    double rd[5] = {0, 0, 0, 0, 0};
+    double av[9];
+    if (!gs.inviscid) {
+        for (int a = 0; a < 9; ++a) av[a] = act ? p.avars[a * p.ndof + g] : 0.0;
+        // ---- high-order viscous volume term (calc_cell_ho_dif_residual, dgsem/plens.cc:3283-3413),
+        //      one reference direction at a time through LDS
+        for (int md = 0; md < 3; ++md) {
+            __syncthreads();
+            if (inblk) {
+                double dir[3];
+                for (int a = 0; a < 3; ++a) dir[a] = sJa[(3 * md + a) * B + li];
+                const double nr = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+                for (int a = 0; a < 3; ++a) dir[a] /= nr;
+                double fl[5];
+                dif_flux(u, av, dir, fl);
+                sX[0 * B + li] = nr;
+                for (int v = 1; v < 5; ++v) sX[v * B + li] = fl[v];
+            }
+            __syncthreads();
+            if (inblk) {
+                for (int m = 0; m <= N; ++m) {
+                    int to[3] = {ti, tj, tk};
+                    to[md] = m;
+                    const int jo = lc * NN + to[0] + NP * (to[1] + NP * to[2]);
+                    const double Dn = sD[tt[md] * NP + m] * sX[jo];
+                    for (int v = 1; v < 5; ++v) rd[v] += Dn * sX[v * B + jo];
+                }
+            }
+        }
+    }
+
+    // ---- surface terms of both high-order residuals
+    if (act) {
+        for (int f = 0; f < 6; ++f) {
+            if (!on_face(f, ti, tj, tk, N)) continue;
+            const int q = face_dof<N>(f, ti, tj, tk);
+            const int sd = f >> 1, lr = f & 1;
+            const double sg = lr ? -1.0 : 1.0;
+            const double wend = sW[lr * N];
+            double dir[3];
+            for (int a = 0; a < 3; ++a) dir[a] = sJa[(3 * sd + a) * B + li];
+            const double nr = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+            for (int a = 0; a < 3; ++a) dir[a] /= nr;
+            const long long o = ((long long)c * 6 + f) * 5 * NF + q;
+            double fi[5];
+            inv_flux(u, dir, gs.gma, fi);
+            for (int v = 0; v < 5; ++v) {
+                const double fs = -sg * p.sinv[o + v * NF];
+                ri[v] += -sg * (fs - fi[v]) * nr / wend;
+            }
+            if (!gs.inviscid) {
+                dif_flux(u, av, dir, fi);
+                for (int v = 0; v < 5; ++v) {
+                    const double fs = -sg * p.svis[o + v * NF];
+                    rd[v] += -sg * (fs - fi[v]) * nr / wend;
+                }
+            }
+        }
+    }
+    {
+        const double jm = 1.0 / (-1.0 * detJ);
+        for (int v = 0; v < 5; ++v) ri[v] *= jm;
+        const double jp = 1.0 / (1.0 * detJ);
+        for (int v = 0; v < 5; ++v) rd[v] *= jp;
+    }
+
+    // ---- low-order subcell FV residual for blended cells (calc_cell_lo_inv_residual, 3469-3583)
+    double rl[5] = {0, 0, 0, 0, 0};
+    bool any_blend = false;
+    {
+        __shared__ int s_any;
+        if (tid == 0) s_any = 0;
+        __syncthreads();
+        if (act && alpha > 0.0 && node == 0) atomicOr(&s_any, 1);
+        __syncthreads();
+        any_blend = s_any != 0;
+    }
+    if (any_blend) {
+        const double bl = alpha <= 0.0 ? 0.0 : (alpha <= 1.0 ? alpha : 1.0);
+        for (int dir = 0; dir < 3; ++dir) {
+            __syncthreads();
+            if (act && alpha > 0.0 && tt[dir] < N) {
+                // subcell normal n_(id-1,id), id = tt[dir]+1 (MetricTerms::reinit, metric_terms.cc:104-140)
+                const int id = tt[dir] + 1;
+                double s[3];
+                int t0[3] = {ti, tj, tk};
+                t0[dir] = 0;
+                const int j0 = lc * NN + t0[0] + NP * (t0[1] + NP * t0[2]);
+                for (int a = 0; a < 3; ++a) s[a] = sJa[(3 * dir + a) * B + j0];
+                for (int l = 0; l < id; ++l)
+                    for (int m = 0; m <= N; ++m) {
+                        int tm[3] = {ti, tj, tk};
+                        tm[dir] = m;
+                        const int jm = lc * NN + tm[0] + NP * (tm[1] + NP * tm[2]);
+                        for (int a = 0; a < 3; ++a) s[a] += sQm[l * NP + m] * sJa[(3 * dir + a) * B + jm];
+                    }
+                const double nn = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+                for (int a = 0; a < 3; ++a) s[a] /= nn;
+                int tr[3] = {ti, tj, tk};
+                tr[dir] += 1;
+                const int jr = lc * NN + tr[0] + NP * (tr[1] + NP * tr[2]);
+                double ur[5], fl[5];
+                for (int v = 0; v < 5; ++v) ur[v] = sQ[v * B + jr];
+                inv_surf_flux(p.scheme, bl, u, ur, s, gs.gma, fl);
+                for (int v = 0; v < 5; ++v) sX[v * B + li] = fl[v] * nn;
+            }
+            __syncthreads();
+            if (act && alpha > 0.0) {
+                if (tt[dir] > 0) {
+                    int tl_[3] = {ti, tj, tk};
+                    tl_[dir] -= 1;
+                    const int jl = lc * NN + tl_[0] + NP * (tl_[1] + NP * tl_[2]);
+                    for (int v = 0; v < 5; ++v) rl[v] -= sX[v * B + jl] / sW[tt[dir]];
+                }
+                if (tt[dir] < N)
+                    for (int v = 0; v < 5; ++v) rl[v] += sX[v * B + li] / sW[tt[dir]];
+            }
+        }
+        if (act && alpha > 0.0) {
+            for (int f = 0; f < 6; ++f) {
+                if (!on_face(f, ti, tj, tk, N)) continue;
+                const int q = face_dof<N>(f, ti, tj, tk);
+                const int sd = f >> 1, lr = f & 1;
+                const double cs = lr ? 1.0 : -1.0;
+                const double d0 = sJa[(3 * sd + 0) * B + li], d1 = sJa[(3 * sd + 1) * B + li], d2 = sJa[(3 * sd + 2) * B + li];
+                const double nn = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+                const long long o = ((long long)c * 6 + f) * 5 * NF + q;
+                for (int v = 0; v < 5; ++v) {
+                    const double fl = cs * p.sinv[o + v * NF];
+                    rl[v] += cs * fl * nn / sW[lr * N];
+                }
+            }
+            for (int v = 0; v < 5; ++v) rl[v] /= -detJ;
+        }
+    }
+    if (!act) return;
+
+    // ---- blend (calc_rhs, dgsem/plens.cc:3753-3771) and RK epilogue (update_rk4, 4353-4397)
+    double rhs[5];
+    for (int v = 0; v < 5; ++v) {
+        if (!gs.inviscid) rhs[v] = (1.0 - alpha_d) * rd[v] + alpha * rl[v] + (1.0 - alpha) * ri[v];
+        else rhs[v] = alpha * rl[v] + (1.0 - alpha) * ri[v];
+    }
+    if (p.stage == 0) {
+        for (int v = 0; v < 5; ++v) p.rhs_out[v * p.ndof + g] = rhs[v];
+        return;
+    }
+    const double dt = p.dt[0];
+    for (int v = 0; v < 5; ++v) {
+        const long long k = v * p.ndof + g;
+        const double R = rhs[v] * dt;
+        double qn;
+        if (p.stage == 1) {
+            qn = u[v] + p.c3 * R;
+        } else if (p.stage == 2) {
+            qn = u[v] + (p.c2 * p.R1[k] + p.c3 * R);
+        } else {
+            qn = u[v] + (p.c1 * p.R2[k] + p.c2 * p.R1[k] + p.c3 * R);
+        }
+        p.Qout[k] = qn;
+        p.Rnew[k] = R;
+    }
+}
+
+}  // namespace
+
+// =============================================================================================
+// host side
+// =============================================================================================
+struct plens_gpu {
+    int N = 0, NN = 0, NF = 0, nc = 0, cpb = 1, n_bcs = 0, ntasks = 0;
+    long long ndof = 0;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    // device buffers
+    double *Qa = nullptr, *Qb = nullptr, *R[3] = {nullptr, nullptr, nullptr};
+    double *Ja = nullptr, *detJ = nullptr, *nrm = nullptr, *hmin = nullptr;
+    int *nbr = nullptr, *bid = nullptr, *wall = nullptr, *tasks = nullptr;
+    BCDev* bcs = nullptr;
+    Basis* basis = nullptr;
+    double *avars = nullptr, *araw = nullptr, *alpha = nullptr, *alpha_d = nullptr;
+    double *sinv = nullptr, *svis = nullptr, *rhs = nullptr;
+    double *dt_part = nullptr, *scal = nullptr;  // scal[0] = dt, scal[1] = t accumulator
+    unsigned long long* fail = nullptr;
+    bool state_in_b = false;
+    KP base{};
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    double graph_courant = -1;
+    int graph_cpb = -1;
+    double ktimes[4] = {0, 0, 0, 0};
+    double last_run_ms = 0;
+    hipEvent_t ev_run[2] = {nullptr, nullptr};
+    int64_t fail_dof = -1;
+    int32_t fail_kind = 0;
+};
+
+namespace {
+
+#define HIPCHK(x)                                   \
+    do {                                            \
+        hipError_t e_ = (x);                        \
+        if (e_ != hipSuccess) return PLENS_E_HIP;   \
+    } while (0)
+
+int default_cpb(int N) {
+    switch (N) {
+        case 1: return 16;
+        case 2: return 4;
+        case 3: return 2;
+        default: return 1;
+    }
+}
+int threads_for(int cpb, int NN) { return ((cpb * NN + 63) / 64) * 64; }
+
+template <typename T>
+int dmalloc(T** p, size_t count) {
+    if (count == 0) count = 1;
+    if (hipMalloc((void**)p, sizeof(T) * count) != hipSuccess) return PLENS_E_NOMEM;
+    return PLENS_OK;
+}
+
+const double kRkB[5] = {314199625218.0 / 7198350928319, 6410344372641.0 / 17000082738695,
+                        292278564125.0 / 5593752632744, 5010207514426.0 / 21876007855139,
+                        5597675544274.0 / 18784428342765};
+const double kRkAo[5] = {4745337637855.0 / 22386579876409, 6808157035527.0 / 13197844641179,
+                         4367509502613.0 / 10454198590847, 1236962429870.0 / 3429868089329,
+                         5597675544274.0 / 18784428342765};
+const double kRkAi[4] = {546509042554.0 / 9152262712923, 625707605167.0 / 5316659119056,
+                         582400652113.0 / 7078426004906, 5010207514426.0 / 21876007855139};
+
+template <int N>
+int launch_stage_t(plens_gpu* h, const KP& kp, hipEvent_t* ev) {
+    constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
+    const int thr = threads_for(kp.cpb, NN);
+    const int blocks = (kp.nc + kp.cpb - 1) / kp.cpb;
+    const size_t lds_grad = sizeof(double) * 16 * kp.cpb * NN;
+    const size_t lds_vol = sizeof(double) * 20 * kp.cpb * NN;
+    if (ev) HIPCHK(hipEventRecord(ev[0], h->stream));
+    hipLaunchKernelGGL(k_grad<N>, dim3(blocks), dim3(thr), lds_grad, h->stream, kp);
+    HIPCHK(hipGetLastError());
+    if (ev) HIPCHK(hipEventRecord(ev[1], h->stream));
+    hipLaunchKernelGGL(k_alpha, dim3((kp.nc + kThreadsAlpha - 1) / kThreadsAlpha), dim3(kThreadsAlpha), 0,
+                       h->stream, kp);
+    HIPCHK(hipGetLastError());
+    if (ev) HIPCHK(hipEventRecord(ev[2], h->stream));
+    const long long fthreads = (long long)kp.ntasks * NF;
+    hipLaunchKernelGGL(k_face<N>, dim3((unsigned)((fthreads + kThreadsFace - 1) / kThreadsFace)),
+                       dim3(kThreadsFace), 0, h->stream, kp);
+    HIPCHK(hipGetLastError());
+    if (ev) HIPCHK(hipEventRecord(ev[3], h->stream));
+    hipLaunchKernelGGL(k_vol<N>, dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
+    HIPCHK(hipGetLastError());
+    if (ev) HIPCHK(hipEventRecord(ev[4], h->stream));
+    return PLENS_OK;
+}
+
+int launch_stage(plens_gpu* h, const KP& kp, hipEvent_t* ev = nullptr) {
+    switch (h->N) {
+        case 1: return launch_stage_t<1>(h, kp, ev);
+        case 2: return launch_stage_t<2>(h, kp, ev);
+        case 3: return launch_stage_t<3>(h, kp, ev);
+        case 4: return launch_stage_t<4>(h, kp, ev);
+        case 5: return launch_stage_t<5>(h, kp, ev);
+        default: return PLENS_E_INVALID;
+    }
+}
+
+// Kernel parameters of RK stage s (1..5) of a step that starts from buffer `from_b`.
+KP stage_params(plens_gpu* h, int s, bool from_b, double courant) {
+    KP kp = h->base;
+    // Q ping-pong: stage s reads buffer (from ^ (s-1)&1)
+    const bool rb = ((s - 1) & 1) ? !from_b : from_b;
+    kp.Q = rb ? h->Qb : h->Qa;
+    kp.Qout = rb ? h->Qa : h->Qb;
+    // residual registers rotate through R[0..2]: write R_new into the slot of R''
+    static const int wr[6] = {0, 0, 1, 0, 1, 0};   // slot written at stage s
+    static const int r1[6] = {0, 0, 0, 1, 0, 1};   // slot of R' read at stage s
+    static const int r2[6] = {0, 0, 0, 0, 1, 0};   // slot of R'' read at stage s
+    kp.Rnew = h->R[wr[s]];
+    kp.R1 = h->R[r1[s]];
+    kp.R2 = h->R[r2[s]];
+    kp.stage = s;
+    kp.want_dt = (s == 1);
+    kp.courant = courant;
+    if (s == 1) {
+        kp.c1 = 0; kp.c2 = 0; kp.c3 = kRkAo[0];
+    } else if (s == 2) {
+        kp.c1 = 0; kp.c2 = kRkAi[0] - kRkAo[0]; kp.c3 = kRkAo[1];
+    } else {
+        kp.c1 = kRkB[s - 3] - kRkAi[s - 3];
+        kp.c2 = kRkAi[s - 2] - kRkAo[s - 2];
+        kp.c3 = kRkAo[s - 1];
+    }
+    return kp;
+}
+
+int launch_step(plens_gpu* h, bool from_b, double courant, hipEvent_t* ev = nullptr) {
+    for (int s = 1; s <= 5; ++s) {
+        KP kp = stage_params(h, s, from_b, courant);
+        const int e = launch_stage(h, kp, ev ? ev + 5 * (s - 1) : nullptr);
+        if (e) return e;
+    }
+    return PLENS_OK;
+}
+
+int read_failure(plens_gpu* h) {
+    unsigned long long code = 0;
+    HIPCHK(hipMemcpyAsync(&code, h->fail, sizeof(code), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (code != ~0ull) {
+        h->fail_dof = (int64_t)(code >> 2);
+        h->fail_kind = (int32_t)(code & 3ull);
+        return PLENS_E_POSITIVITY;
+    }
+    return PLENS_OK;
+}
+
+int reset_failure(plens_gpu* h) {
+    const unsigned long long init = ~0ull;
+    HIPCHK(hipMemcpyAsync(h->fail, &init, sizeof(init), hipMemcpyHostToDevice, h->stream));
+    return PLENS_OK;
+}
+
+void destroy_graphs(plens_gpu* h) {
+    for (auto& gx : h->graph)
+        if (gx) {
+            (void)hipGraphExecDestroy(gx);
+            gx = nullptr;
+        }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* plens_gpu_strerror(int code) {
+    switch (code) {
+        case PLENS_OK: return "ok";
+        case PLENS_E_INVALID: return "invalid argument";
+        case PLENS_E_HIP: return "HIP runtime error";
+        case PLENS_E_POSITIVITY: return "positivity violated (rho<=0 or e<=0)";
+        case PLENS_E_NOMEM: return "device allocation failed";
+        case PLENS_E_NODEVICE: return "no GPU device";
+        default: return "unknown error";
+    }
+}
+
+int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const plens_bc* bcs, int32_t n_bcs,
+                     plens_gpu** out) {
+    if (!m || !ph || !out || (n_bcs > 0 && !bcs)) return PLENS_E_INVALID;
+    if (m->degree < 1 || m->degree > 5 || m->n_cells < 1) return PLENS_E_INVALID;
+    if (!m->Ja || !m->detJ || !m->face_normals || !m->min_vertex_dist || !m->face_nbr || !m->face_bid)
+        return PLENS_E_INVALID;
+    if (ph->inv_surf_flux < 0 || ph->inv_surf_flux > 3) return PLENS_E_INVALID;
+    if (ph->precision != PLENS_PREC_FP64) return PLENS_E_INVALID;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PLENS_E_NODEVICE;
+    const int N = m->degree, NP = N + 1, NN = NP * NP * NP, NF = NP * NP, nc = m->n_cells;
+    // validate the connectivity
+    for (int c = 0; c < nc; ++c)
+        for (int f = 0; f < 6; ++f) {
+            const int b = m->face_bid[c * 6 + f], nb = m->face_nbr[c * 6 + f];
+            if (b >= n_bcs) return PLENS_E_INVALID;
+            const bool needs_nbr = b < 0 || bcs[b].type == PLENS_BC_PERIODIC;
+            if (needs_nbr && (nb < 0 || nb >= nc)) return PLENS_E_INVALID;
+            if (b >= 0 && (bcs[b].type < 0 || bcs[b].type > 3)) return PLENS_E_INVALID;
+        }
+    plens_gpu* h = new (std::nothrow) plens_gpu;
+    if (!h) return PLENS_E_NOMEM;
+    h->N = N; h->NN = NN; h->NF = NF; h->nc = nc; h->n_bcs = n_bcs;
+    h->ndof = (long long)nc * NN;
+    h->cpb = default_cpb(N);
+    (void)hipGetDevice(&h->device);
+    int rc = PLENS_OK;
+    auto fail = [&](int code) {
+        plens_gpu_destroy(h);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(PLENS_E_HIP);
+    const size_t nd = (size_t)h->ndof;
+    // face tasks: boundary faces of every cell, interior faces from the lower-index cell
+    std::vector<int> tasks;
+    std::vector<int> wall(nc, 0);
+    for (int c = 0; c < nc; ++c)
+        for (int f = 0; f < 6; ++f) {
+            const int b = m->face_bid[c * 6 + f];
+            if (b >= 0) {
+                tasks.push_back(c * 6 + f);
+                if (bcs[b].type == PLENS_BC_UNIFORM_TEMP_WALL) wall[c] = 1;
+            } else if (c < m->face_nbr[c * 6 + f]) {
+                tasks.push_back(c * 6 + f);
+            }
+        }
+    h->ntasks = (int)tasks.size();
+    const int nblk_max = (nc + 0) / 1 + 1;
+    if ((rc = dmalloc(&h->Qa, 5 * nd)) || (rc = dmalloc(&h->Qb, 5 * nd)) || (rc = dmalloc(&h->R[0], 5 * nd)) ||
+        (rc = dmalloc(&h->R[1], 5 * nd)) || (rc = dmalloc(&h->R[2], 5 * nd)) || (rc = dmalloc(&h->rhs, 5 * nd)) ||
+        (rc = dmalloc(&h->Ja, 9 * nd)) || (rc = dmalloc(&h->detJ, nd)) ||
+        (rc = dmalloc(&h->nrm, (size_t)nc * 18 * NF)) || (rc = dmalloc(&h->hmin, nc)) ||
+        (rc = dmalloc(&h->nbr, (size_t)nc * 6)) || (rc = dmalloc(&h->bid, (size_t)nc * 6)) ||
+        (rc = dmalloc(&h->wall, nc)) || (rc = dmalloc(&h->tasks, tasks.size())) ||
+        (rc = dmalloc(&h->bcs, std::max(n_bcs, 1))) || (rc = dmalloc(&h->basis, 1)) ||
+        (rc = dmalloc(&h->avars, 9 * nd)) || (rc = dmalloc(&h->araw, nc)) || (rc = dmalloc(&h->alpha, nc)) ||
+        (rc = dmalloc(&h->alpha_d, nc)) || (rc = dmalloc(&h->sinv, (size_t)nc * 30 * NF)) ||
+        (rc = dmalloc(&h->svis, (size_t)nc * 30 * NF)) || (rc = dmalloc(&h->dt_part, nblk_max)) ||
+        (rc = dmalloc(&h->scal, 2)) || (rc = dmalloc(&h->fail, 1)))
+        return fail(rc);
+    // host-side tables
+    std::vector<BCDev> hb(std::max(n_bcs, 1));
+    const double R = 8.314462618 / ph->molecular_weight;  // NavierStokes::get_R
+    for (int i = 0; i < n_bcs; ++i) {
+        BCDev& d = hb[i];
+        d.type = bcs[i].type; d.p = bcs[i].p; d.T = bcs[i].T;
+        for (int a = 0; a < 3; ++a) d.vel[a] = bcs[i].vel[a];
+        if (d.type == PLENS_BC_UNIFORM_INFLOW) {  // set_BC (dgsem/plens.cc:1268-1270)
+            const double rho = d.p / (R * d.T);
+            prim_to_cons(ph->gamma, rho, d.vel, d.p, d.cons_pr);
+        } else {
+            for (int v = 0; v < 5; ++v) d.cons_pr[v] = 0;
+        }
+    }
+    Basis hbas{};
+    {
+        double x[kMaxNP], w[kMaxNP], D[kMaxNP * kMaxNP], C1[kMaxNP * kMaxNP];
+        if (plens_basis_1d(N, x, w, D, C1)) return fail(PLENS_E_INVALID);
+        for (int i = 0; i < NP; ++i) {
+            hbas.w[i] = w[i];
+            for (int j = 0; j < NP; ++j) {
+                hbas.D[i * kMaxNP + j] = D[i * NP + j];
+                hbas.Q[i * kMaxNP + j] = w[i] * D[i * NP + j];  // ref_Q_1d = w_i D_ij
+                hbas.C1[i * kMaxNP + j] = C1[i * NP + j];
+            }
+        }
+    }
+    if (hipMemcpy(h->Ja, m->Ja, sizeof(double) * 9 * nd, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->detJ, m->detJ, sizeof(double) * nd, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->nrm, m->face_normals, sizeof(double) * nc * 18 * NF, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->hmin, m->min_vertex_dist, sizeof(double) * nc, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->nbr, m->face_nbr, sizeof(int) * nc * 6, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->bid, m->face_bid, sizeof(int) * nc * 6, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->wall, wall.data(), sizeof(int) * nc, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->tasks, tasks.data(), sizeof(int) * tasks.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->bcs, hb.data(), sizeof(BCDev) * hb.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->basis, &hbas, sizeof(Basis), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(PLENS_E_HIP);
+    // zero-initialised work arrays (avars is read on faces of inviscid runs only through unused paths)
+    if (hipMemset(h->avars, 0, sizeof(double) * 9 * nd) != hipSuccess ||
+        hipMemset(h->R[0], 0, sizeof(double) * 5 * nd) != hipSuccess ||
+        hipMemset(h->R[1], 0, sizeof(double) * 5 * nd) != hipSuccess ||
+        hipMemset(h->R[2], 0, sizeof(double) * 5 * nd) != hipSuccess ||
+        hipMemset(h->alpha, 0, sizeof(double) * nc) != hipSuccess ||
+        hipMemset(h->scal, 0, sizeof(double) * 2) != hipSuccess)
+        return fail(PLENS_E_HIP);
+    const unsigned long long init = ~0ull;
+    if (hipMemcpy(h->fail, &init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess) return fail(PLENS_E_HIP);
+
+    KP& k = h->base;
+    k.nc = nc; k.cpb = h->cpb; k.ntasks = h->ntasks; k.ndof = h->ndof;
+    k.Ja = h->Ja; k.detJ = h->detJ; k.nrm = h->nrm; k.hmin = h->hmin;
+    k.nbr = h->nbr; k.bid = h->bid; k.wall = h->wall; k.tasks = h->tasks; k.bcs = h->bcs; k.basis = h->basis;
+    k.avars = h->avars; k.araw = h->araw; k.alpha = h->alpha; k.alpha_d = h->alpha_d;
+    k.sinv = h->sinv; k.svis = h->svis; k.rhs_out = h->rhs;
+    k.dt_part = h->dt_part; k.dt = h->scal; k.t_acc = h->scal + 1; k.fail = h->fail;
+    k.gas.gma = ph->gamma; k.gas.R = R; k.gas.cv = R / (ph->gamma - 1); k.gas.mu0 = ph->mu0; k.gas.T0 = ph->T0;
+    k.gas.S = ph->S; k.gas.Pr = ph->prandtl; k.gas.inviscid = ph->mu0 < 1e-16;
+    k.scheme = ph->inv_surf_flux;
+    k.thr = ph->threshold_factor * pow(10.0, -ph->threshold_exp_factor * (pow(N + 1, 0.25)));
+    k.sharp = ph->sharpness_factor; k.amin = ph->alpha_min; k.amax = ph->alpha_max;
+    k.wlimit = ph->wall_blender_limit;
+    k.n_dt_part = (nc + h->cpb - 1) / h->cpb;
+    *out = h;
+    return PLENS_OK;
+}
+
+void plens_gpu_destroy(plens_gpu* h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    destroy_graphs(h);
+    for (auto& x : h->ev_run)
+        if (x) (void)hipEventDestroy(x);
+    void* ptrs[] = {h->Qa, h->Qb, h->R[0], h->R[1], h->R[2], h->Ja, h->detJ, h->nrm, h->hmin, h->nbr,
+                    h->bid, h->wall, h->tasks, h->bcs, h->basis, h->avars, h->araw, h->alpha, h->alpha_d,
+                    h->sinv, h->svis, h->rhs, h->dt_part, h->scal, h->fail};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+int plens_gpu_set_tuning(plens_gpu* h, int32_t cpb) {
+    if (!h || cpb < 0) return PLENS_E_INVALID;
+    if (cpb == 0) cpb = default_cpb(h->N);
+    if (threads_for(cpb, h->NN) > 256) return PLENS_E_INVALID;
+    h->cpb = cpb;
+    h->base.cpb = cpb;
+    h->base.n_dt_part = (h->nc + cpb - 1) / cpb;
+    destroy_graphs(h);
+    return PLENS_OK;
+}
+
+int plens_gpu_set_state(plens_gpu* h, const double* Q) {
+    if (!h || !Q) return PLENS_E_INVALID;
+    HIPCHK(hipMemcpyAsync(h->Qa, Q, sizeof(double) * 5 * h->ndof, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->state_in_b = false;
+    return PLENS_OK;
+}
+
+int plens_gpu_get_state(plens_gpu* h, double* Q) {
+    if (!h || !Q) return PLENS_E_INVALID;
+    HIPCHK(hipMemcpyAsync(Q, h->state_in_b ? h->Qb : h->Qa, sizeof(double) * 5 * h->ndof, hipMemcpyDeviceToHost,
+                          h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return PLENS_OK;
+}
+
+int plens_gpu_calc_rhs(plens_gpu* h, double* rhs_out, double* alpha_out, double* alpha_d_out) {
+    if (!h) return PLENS_E_INVALID;
+    int e = reset_failure(h);
+    if (e) return e;
+    KP kp = h->base;
+    kp.Q = h->state_in_b ? h->Qb : h->Qa;
+    kp.Qout = nullptr;
+    kp.stage = 0;
+    kp.want_dt = 0;
+    kp.t_acc = nullptr;
+    if ((e = launch_stage(h, kp))) return e;
+    if ((e = read_failure(h))) return e;
+    if (rhs_out)
+        HIPCHK(hipMemcpyAsync(rhs_out, h->rhs, sizeof(double) * 5 * h->ndof, hipMemcpyDeviceToHost, h->stream));
+    if (alpha_out)
+        HIPCHK(hipMemcpyAsync(alpha_out, h->alpha, sizeof(double) * h->nc, hipMemcpyDeviceToHost, h->stream));
+    if (alpha_d_out)
+        HIPCHK(hipMemcpyAsync(alpha_d_out, h->alpha_d, sizeof(double) * h->nc, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return PLENS_OK;
+}
+
+int plens_gpu_calc_dt(plens_gpu* h, double courant, double* dt) {
+    if (!h || !dt) return PLENS_E_INVALID;
+    int e = reset_failure(h);
+    if (e) return e;
+    KP kp = h->base;
+    kp.Q = h->state_in_b ? h->Qb : h->Qa;
+    kp.want_dt = 1;
+    kp.courant = courant;
+    kp.t_acc = nullptr;
+    const int NN = h->NN;
+    const int thr = threads_for(kp.cpb, NN);
+    const int blocks = (kp.nc + kp.cpb - 1) / kp.cpb;
+    const size_t lds = sizeof(double) * 16 * kp.cpb * NN;
+    // gradient kernel with the time-step candidates (it also refreshes avars), then the reduction
+    switch (h->N) {
+        case 1: hipLaunchKernelGGL(k_grad<1>, dim3(blocks), dim3(thr), lds, h->stream, kp); break;
+        case 2: hipLaunchKernelGGL(k_grad<2>, dim3(blocks), dim3(thr), lds, h->stream, kp); break;
+        case 3: hipLaunchKernelGGL(k_grad<3>, dim3(blocks), dim3(thr), lds, h->stream, kp); break;
+        case 4: hipLaunchKernelGGL(k_grad<4>, dim3(blocks), dim3(thr), lds, h->stream, kp); break;
+        default: hipLaunchKernelGGL(k_grad<5>, dim3(blocks), dim3(thr), lds, h->stream, kp); break;
+    }
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_alpha, dim3((kp.nc + kThreadsAlpha - 1) / kThreadsAlpha), dim3(kThreadsAlpha), 0,
+                       h->stream, kp);
+    HIPCHK(hipGetLastError());
+    if ((e = read_failure(h))) return e;
+    HIPCHK(hipMemcpyAsync(dt, h->scal, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return PLENS_OK;
+}
+
+int plens_gpu_rk4_step(plens_gpu* h, double courant, double* t_inout) {
+    if (!h) return PLENS_E_INVALID;
+    int e = reset_failure(h);
+    if (e) return e;
+    const double zero = 0.0;
+    HIPCHK(hipMemcpyAsync(h->scal + 1, &zero, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if ((e = launch_step(h, h->state_in_b, courant))) return e;
+    h->state_in_b = !h->state_in_b;
+    if ((e = read_failure(h))) return e;
+    double dt = 0;
+    HIPCHK(hipMemcpyAsync(&dt, h->scal + 1, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (t_inout) *t_inout += dt;
+    return PLENS_OK;
+}
+
+int plens_gpu_run(plens_gpu* h, int32_t n_steps, double courant, double* t_inout) {
+    if (!h || n_steps < 0) return PLENS_E_INVALID;
+    int e = reset_failure(h);
+    if (e) return e;
+    const double zero = 0.0;
+    HIPCHK(hipMemcpyAsync(h->scal + 1, &zero, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if (h->graph_courant != courant || h->graph_cpb != h->cpb) {
+        destroy_graphs(h);
+        for (int par = 0; par < 2; ++par) {
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+            const int le = launch_step(h, par == 1, courant);
+            hipError_t ce = hipStreamEndCapture(h->stream, &g);
+            if (le) return le;
+            if (ce != hipSuccess) return PLENS_E_HIP;
+            hipError_t ie = hipGraphInstantiate(&h->graph[par], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) return PLENS_E_HIP;
+        }
+        h->graph_courant = courant;
+        h->graph_cpb = h->cpb;
+    }
+    if (!h->ev_run[0]) {
+        HIPCHK(hipEventCreate(&h->ev_run[0]));
+        HIPCHK(hipEventCreate(&h->ev_run[1]));
+    }
+    HIPCHK(hipEventRecord(h->ev_run[0], h->stream));
+    for (int s = 0; s < n_steps; ++s) {
+        HIPCHK(hipGraphLaunch(h->graph[h->state_in_b ? 1 : 0], h->stream));
+        h->state_in_b = !h->state_in_b;
+    }
+    HIPCHK(hipEventRecord(h->ev_run[1], h->stream));
+    HIPCHK(hipEventSynchronize(h->ev_run[1]));
+    {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, h->ev_run[0], h->ev_run[1]));
+        h->last_run_ms = ms;
+    }
+    if ((e = read_failure(h))) return e;
+    double tt = 0;
+    HIPCHK(hipMemcpyAsync(&tt, h->scal + 1, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (t_inout) *t_inout += tt;
+    return PLENS_OK;
+}
+
+// Profiled run: kernels launched directly with HIP events around each kernel on the handle's
+// stream; accumulates per-kernel-class average durations.
+int plens_gpu_run_profiled(plens_gpu* h, int32_t n_steps, double courant, double* t_inout) {
+    if (!h || n_steps < 1) return PLENS_E_INVALID;
+    int e = reset_failure(h);
+    if (e) return e;
+    const double zero = 0.0;
+    HIPCHK(hipMemcpyAsync(h->scal + 1, &zero, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    std::vector<hipEvent_t> ev(25);
+    for (auto& x : ev) HIPCHK(hipEventCreate(&x));
+    double acc[4] = {0, 0, 0, 0};
+    for (int s = 0; s < n_steps; ++s) {
+        if ((e = launch_step(h, h->state_in_b, courant, ev.data()))) return e;
+        h->state_in_b = !h->state_in_b;
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (int st = 0; st < 5; ++st)
+            for (int k = 0; k < 4; ++k) {
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, ev[5 * st + k], ev[5 * st + k + 1]));
+                acc[k] += ms;
+            }
+    }
+    for (auto& x : ev) (void)hipEventDestroy(x);
+    for (int k = 0; k < 4; ++k) h->ktimes[k] = acc[k] / (5.0 * n_steps);
+    if ((e = read_failure(h))) return e;
+    double tt = 0;
+    HIPCHK(hipMemcpyAsync(&tt, h->scal + 1, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (t_inout) *t_inout += tt;
+    return PLENS_OK;
+}
+
+int plens_gpu_kernel_times(const plens_gpu* h, double* t, int32_t n) {
+    if (!h || !t) return PLENS_E_INVALID;
+    for (int i = 0; i < n && i < 4; ++i) t[i] = h->ktimes[i];
+    return PLENS_OK;
+}
+
+int plens_gpu_last_run_ms(const plens_gpu* h, double* ms) {
+    if (!h || !ms) return PLENS_E_INVALID;
+    *ms = h->last_run_ms;
+    return PLENS_OK;
+}
+
+int plens_gpu_last_failure(const plens_gpu* h, int64_t* dof, int32_t* kind) {
+    if (!h) return PLENS_E_INVALID;
+    if (dof) *dof = h->fail_dof;
+    if (kind) *kind = h->fail_kind;
+    return PLENS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+"""Host-side mirror of the PLENS hot-path members (dgsem/plens.h:827-896) over the GPU C ABI.
+
+    PLENS::calc_rhs()        -> GpuPlens.calc_rhs()        (dgsem/plens.cc:3688-3779)
+    PLENS::calc_time_step()  -> GpuPlens.calc_time_step()  (dgsem/plens.cc:3810-3912, global stepping)
+    PLENS::update()          -> GpuPlens.update()          (dgsem/plens.cc:4484-4518, RK order 4)
+    PLENS::run() loop        -> GpuPlens.run(n_steps)      (dgsem/plens.cc:1547)
+
+A positivity failure raises PositivityError carrying the lowest failing DoF and its kind, where
+the reference prints the location and calls std::exit (dgsem/plens.cc:2540-2544).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+class PositivityError(L.PlensError):
+    def __init__(self, code, dof, kind):
+        self.dof = dof
+        self.kind = {1: "density", 2: "energy"}.get(kind, "unknown")
+        super().__init__(code, f"positivity failed at dof {dof} ({self.kind})")
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+class GpuPlens:
+    def __init__(self, case=None, mesh=None, physics=None, bcs=None, cells_per_block=0):
+        if case is not None:
+            mesh, physics, bcs = case.mesh, case.physics, case.bcs
+        self.lib = L.load()
+        self.mesh = mesh
+        self.n_cells, self.n_dof = mesh.n_cells, mesh.n_dof
+        self._desc = mesh.desc()
+        arr = (L.BC * max(len(bcs), 1))()
+        for i, b in enumerate(bcs):
+            arr[i] = b
+        self._bcs = arr
+        self._phys = physics
+        self._h = C.c_void_p()
+        L.check(self.lib.plens_gpu_create(C.byref(self._desc), C.byref(self._phys), arr, len(bcs),
+                                          C.byref(self._h)), "plens_gpu_create")
+        if cells_per_block:
+            L.check(self.lib.plens_gpu_set_tuning(self._h, cells_per_block), "plens_gpu_set_tuning")
+        if case is not None:
+            self.set_state(case.Q0)
+
+    def _check(self, code, what):
+        if code == L.PLENS_E_POSITIVITY:
+            dof, kind = C.c_int64(), C.c_int32()
+            self.lib.plens_gpu_last_failure(self._h, C.byref(dof), C.byref(kind))
+            raise PositivityError(code, dof.value, kind.value)
+        L.check(code, what)
+
+    def set_state(self, Q):
+        Q = np.ascontiguousarray(Q, dtype=np.float64).reshape(5, self.n_dof)
+        self._check(self.lib.plens_gpu_set_state(self._h, _dptr(Q)), "set_state")
+
+    def get_state(self):
+        Q = np.empty((5, self.n_dof))
+        self._check(self.lib.plens_gpu_get_state(self._h, _dptr(Q)), "get_state")
+        return Q
+
+    def calc_rhs(self):
+        rhs = np.empty((5, self.n_dof))
+        a = np.empty(self.n_cells)
+        ad = np.empty(self.n_cells)
+        self._check(self.lib.plens_gpu_calc_rhs(self._h, _dptr(rhs), _dptr(a), _dptr(ad)), "calc_rhs")
+        return rhs, a, ad
+
+    def calc_time_step(self, courant):
+        dt = C.c_double()
+        self._check(self.lib.plens_gpu_calc_dt(self._h, courant, C.byref(dt)), "calc_dt")
+        return dt.value
+
+    def update(self, courant):
+        t = C.c_double(0.0)
+        self._check(self.lib.plens_gpu_rk4_step(self._h, courant, C.byref(t)), "rk4_step")
+        return t.value
+
+    def run(self, n_steps, courant):
+        t = C.c_double(0.0)
+        self._check(self.lib.plens_gpu_run(self._h, n_steps, courant, C.byref(t)), "run")
+        return t.value
+
+    def last_run_ms(self):
+        ms = C.c_double()
+        self._check(self.lib.plens_gpu_last_run_ms(self._h, C.byref(ms)), "last_run_ms")
+        return ms.value
+
+    def run_profiled(self, n_steps, courant):
+        t = C.c_double(0.0)
+        self._check(self.lib.plens_gpu_run_profiled(self._h, n_steps, courant, C.byref(t)), "run_profiled")
+        times = np.zeros(4)
+        self.lib.plens_gpu_kernel_times(self._h, _dptr(times), 4)
+        return t.value, times
+
+    def set_tuning(self, cells_per_block):
+        self._check(self.lib.plens_gpu_set_tuning(self._h, cells_per_block), "set_tuning")
+
+    def close(self):
+        if self._h:
+            self.lib.plens_gpu_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
