@@ -30,11 +30,14 @@ def kernel_bytes_per_dof(N, viscous=True):
     # k_grad: read Q (5), Ja (9), detJ (1); write avars (9, viscous)
     b["grad"] = 8 * (5 + 9 + 1 + (9 if viscous else 0))
     b["alpha"] = 0.0
+    # k_lo: blended cells only (none in the default state); early-exit blocks read alpha
+    b["lo"] = 0.0
     # k_face: per face node read both states (2x5) and avars (2x9), normal (3); write both sides' fluxes
     # (2 x 2 x 5); amortised per DoF (each face node shared by two cells)
     b["face"] = 8 * face_per_dof / 2 * (10 + (18 if viscous else 0) + 3 + 10 * (2 if viscous else 1))
     # k_vol: read Q (5), Ja (9), detJ (1), avars (9), stored face fluxes (2 x 5 per face node),
-    # R' and R'' (stage average 4/5 and 3/5 of 5); write Q (5), R (5)
+    # R' and R'' (stage average 4/5 and 3/5 of 5); write Q (5), R (5); the subcell residual of
+    # blended cells is not counted (none at the measured state)
     b["vol"] = 8 * (5 + 9 + 1 + (9 if viscous else 0) + face_per_dof * 5 * (2 if viscous else 1)
                     + 5 * 4 / 5 + 5 * 3 / 5 + 10)
     return b
@@ -91,7 +94,7 @@ def main():
     print(f"[bench] {args.steps} steps: device {dev_ms:.2f} ms, wall {wall * 1e3:.2f} ms", file=sys.stderr, flush=True)
 
     _, kt = g.run_profiled(args.profile_steps, case.courant)
-    names = ["grad", "alpha", "face", "vol"]
+    names = ["grad", "alpha", "face", "lo", "vol"]
     kb = kernel_bytes_per_dof(N, viscous=True)
     kernels = {}
     for i, k in enumerate(names):
@@ -108,6 +111,14 @@ def main():
                           "achieved_GBs": 400.0 * n_dof / (stage_ms * 1e-3) / 1e9,
                           "frac": 400.0 * n_dof / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
                 "kernels": kernels}
+
+    # achievable HBM bandwidth on this device: streaming copy of 1 GiB (2 GiB of traffic per copy)
+    import ctypes as C
+    from plens_amd import _lib as L
+    cms = C.c_double()
+    copy_bytes = 1 << 30
+    L.check(L.load().plens_gpu_stream_copy(copy_bytes, 10, C.byref(cms)), "stream_copy")
+    roofline["stream_copy_GBs"] = 2 * copy_bytes / (cms.value * 1e-3) / 1e9
 
     cpu, parity = (None, None)
     if not args.no_cpu:

@@ -130,8 +130,13 @@ int plens_gpu_last_failure(const plens_gpu* h, int64_t* dof, int32_t* kind);
 int plens_gpu_set_tuning(plens_gpu* h, int32_t cells_per_block);
 
 /* Average device time of each kernel class over the last plens_gpu_run (HIP events on the
- * handle's stream): times_ms[0..3] = gradient/blender, alpha, face flux, volume+RK. */
+ * handle's stream): times_ms[0..4] = gradient/blender, alpha, face flux, subcell FV, volume+RK. */
 int plens_gpu_kernel_times(const plens_gpu* h, double* times_ms, int32_t n);
+
+/* Bandwidth probe with no counterpart in the reference: copies `bytes` between two device buffers
+ * `iters` times with a streaming kernel; returns the average time per copy (2*bytes of HBM
+ * traffic). Used to calibrate hardware byte counters and to state achievable bandwidth. */
+int plens_gpu_stream_copy(int64_t bytes, int32_t iters, double* ms_per_copy);
 
 const char* plens_gpu_strerror(int code);
 

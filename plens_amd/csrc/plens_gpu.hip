@@ -76,6 +76,7 @@ struct KP {
     const double* R1;
     const double* R2;
     double* rhs_out;
+    double* rlo;
     double* dt_part;
     double* dt;
     double* t_acc;
@@ -443,18 +444,134 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
 }
 
 // =============================================================================================
-// k_vol
+// k_lo: first-order subcell FV residual of blended cells (calc_cell_lo_inv_residual, 3469-3583)
+// =============================================================================================
+template <int N>
+__global__ void __launch_bounds__(256) k_lo(KP p) {
+    constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
+    extern __shared__ double smem[];
+    __shared__ double sD[NP * NP], sW[NP], sQm[NP * NP], sC[NP * NP];
+    __shared__ int s_any;
+    const int B = p.cpb * NN;
+    const int tid = threadIdx.x, lc = tid / NN, node = tid - lc * NN;
+    const int c = blockIdx.x * p.cpb + lc;
+    const bool inblk = lc < p.cpb;
+    const bool act = inblk && c < p.nc;
+    const double alpha = act ? p.alpha[c] : 0.0;
+    if (tid == 0) s_any = 0;
+    __syncthreads();
+    if (act && node == 0 && alpha > 0.0) atomicOr(&s_any, 1);
+    __syncthreads();
+    if (!s_any) return;  // block-uniform: no blended cell in this block
+    double* sQ = smem;          // 5B
+    double* sJa = sQ + 5 * B;   // 9B
+    double* sX = sJa + 9 * B;   // 5B
+    load_basis<N>(p.basis, sD, sW, sQm, sC);
+    const int li = lc * NN + node;
+    const int ti = node % NP, tj = (node / NP) % NP, tk = node / (NP * NP);
+    const int tt[3] = {ti, tj, tk};
+    const long long g = (long long)c * NN + node;
+    const Gas& gs = p.gas;
+    double u[5] = {1, 0, 0, 0, 1}, detJ = 1;
+    if (act) {
+        load_state(p.Q, p.ndof, g, u);
+        detJ = p.detJ[g];
+    }
+    if (inblk) {
+        for (int v = 0; v < 5; ++v) sQ[v * B + li] = u[v];
+        for (int a = 0; a < 9; ++a) sJa[a * B + li] = act ? p.Ja[((long long)c * 9 + a) * NN + node] : ((a % 4 == 0) ? 1.0 : 0.0);
+    }
+    const bool mine = act && alpha > 0.0;
+    const double bl = alpha <= 0.0 ? 0.0 : (alpha <= 1.0 ? alpha : 1.0);
+    double rl[5] = {0, 0, 0, 0, 0};
+    for (int dir = 0; dir < 3; ++dir) {
+        __syncthreads();
+        if (mine && tt[dir] < N) {
+            // subcell normal n_(id-1,id), id = tt[dir]+1 (MetricTerms::reinit, metric_terms.cc:104-140)
+            const int id = tt[dir] + 1;
+            int t0[3] = {ti, tj, tk};
+            t0[dir] = 0;
+            const int j0 = lc * NN + t0[0] + NP * (t0[1] + NP * t0[2]);
+            double s[3];
+            for (int a = 0; a < 3; ++a) s[a] = sJa[(3 * dir + a) * B + j0];
+            for (int l = 0; l < id; ++l)
+                for (int m = 0; m <= N; ++m) {
+                    int tm[3] = {ti, tj, tk};
+                    tm[dir] = m;
+                    const int jm = lc * NN + tm[0] + NP * (tm[1] + NP * tm[2]);
+                    for (int a = 0; a < 3; ++a) s[a] += sQm[l * NP + m] * sJa[(3 * dir + a) * B + jm];
+                }
+            const double nn = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+            for (int a = 0; a < 3; ++a) s[a] /= nn;
+            int tr[3] = {ti, tj, tk};
+            tr[dir] += 1;
+            const int jr = lc * NN + tr[0] + NP * (tr[1] + NP * tr[2]);
+            double ur[5], fl[5];
+            for (int v = 0; v < 5; ++v) ur[v] = sQ[v * B + jr];
+            inv_surf_flux(p.scheme, bl, u, ur, s, gs.gma, fl);
+            for (int v = 0; v < 5; ++v) sX[v * B + li] = fl[v] * nn;
+        }
+        __syncthreads();
+        if (mine) {
+            if (tt[dir] > 0) {
+                int tl_[3] = {ti, tj, tk};
+                tl_[dir] -= 1;
+                const int jl = lc * NN + tl_[0] + NP * (tl_[1] + NP * tl_[2]);
+                for (int v = 0; v < 5; ++v) rl[v] -= sX[v * B + jl] / sW[tt[dir]];
+            }
+            if (tt[dir] < N)
+                for (int v = 0; v < 5; ++v) rl[v] += sX[v * B + li] / sW[tt[dir]];
+        }
+    }
+    if (!mine) return;
+    for (int f = 0; f < 6; ++f) {
+        if (!on_face(f, ti, tj, tk, N)) continue;
+        const int q = face_dof<N>(f, ti, tj, tk);
+        const int sd = f >> 1, lr = f & 1;
+        const double cs = lr ? 1.0 : -1.0;
+        const double d0 = sJa[(3 * sd + 0) * B + li], d1 = sJa[(3 * sd + 1) * B + li], d2 = sJa[(3 * sd + 2) * B + li];
+        const double nn = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        const long long o = ((long long)c * 6 + f) * 5 * NF + q;
+        for (int v = 0; v < 5; ++v) {
+            const double fl = cs * p.sinv[o + v * NF];
+            rl[v] += cs * fl * nn / sW[lr * N];
+        }
+    }
+    for (int v = 0; v < 5; ++v) p.rlo[v * p.ndof + g] = rl[v] / -detJ;
+}
+
+// Chandrashekhar two-point flux along an unnormalised direction a from per-node precomputed
+// (rho, beta = rho/2p, ln rho, ln beta, v, |v|^2): equal to |a| * vol_flux(u_i, u_j, a/|a|) with the
+// logarithms of the log means taken per node (navier_stokes.cc:1510-1542, navier_stokes.h:494-501)
+__device__ __forceinline__ double ln_mean(double x, double y, double lx, double ly) {
+    const double s = x + y, dl = x - y;
+    const double f = dl / s, u = f * f;
+    if (u < 1e-2) return 0.5 * s / (1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0))));
+    return dl / (lx - ly);
+}
+
+// =============================================================================================
+// k_vol: high-order inviscid + viscous residuals, blend, RK epilogue
 // =============================================================================================
 template <int N>
 __global__ void __launch_bounds__(256) k_vol(KP p) {
     constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
+    constexpr int NPAIR = NP * (NP - 1) / 2;       // unordered pairs on one line
+    constexpr int NITEM = NP * NP * NPAIR;          // pairs of one direction in one cell
     extern __shared__ double smem[];
     __shared__ double sD[NP * NP], sW[NP], sQm[NP * NP], sC[NP * NP];
+    __shared__ int sPair[NP * NP];                  // pair index of (a,b), a<b, on a line
     const int B = p.cpb * NN;
-    double* sQ = smem;          // 5B
-    double* sJa = sQ + 5 * B;   // 9B
-    double* sX = sJa + 9 * B;   // 6B scratch (viscous fluxes / subcell fluxes)
+    const int BP = p.cpb * NITEM;
+    double* sA = smem;            // 8B: rho, beta, ln rho, ln beta, v0, v1, v2, |v|^2
+    double* sJa = sA + 8 * B;     // 9B
+    double* sF = sJa + 9 * B;     // 5*BP pair fluxes (also viscous scratch, 5B <= 5BP for N>=2)
     load_basis<N>(p.basis, sD, sW, sQm, sC);
+    for (int t = threadIdx.x; t < NP * NP; t += blockDim.x) {
+        const int a = t / NP, b = t % NP;
+        const int lo = a < b ? a : b, hi = a < b ? b : a;
+        sPair[t] = (lo == hi) ? -1 : lo * NP - lo * (lo + 1) / 2 + (hi - lo - 1);
+    }
 
     const int tid = threadIdx.x, lc = tid / NN, node = tid - lc * NN;
     const int c = blockIdx.x * p.cpb + lc;
@@ -465,57 +582,100 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
     const int tt[3] = {ti, tj, tk};
     const long long g = (long long)c * NN + node;
     const Gas& gs = p.gas;
-    double u[5] = {1, 0, 0, 0, 1}, detJ = 1;
+    double u[5] = {1, 0, 0, 0, 2.5}, detJ = 1;
     if (act) {
         load_state(p.Q, p.ndof, g, u);
-        for (int v = 0; v < 5; ++v) sQ[v * B + li] = u[v];
-        for (int a = 0; a < 9; ++a) sJa[a * B + li] = p.Ja[((long long)c * 9 + a) * NN + node];
         detJ = p.detJ[g];
-    } else if (inblk) {
-        for (int v = 0; v < 5; ++v) sQ[v * B + li] = u[v];
-        for (int a = 0; a < 9; ++a) sJa[a * B + li] = (a % 4 == 0) ? 1.0 : 0.0;
+    }
+    if (inblk) {
+        const double pr = get_p(u, gs.gma);
+        const double beta = 0.5 * u[0] / pr;
+        const double v0 = u[1] / u[0], v1 = u[2] / u[0], v2 = u[3] / u[0];
+        sA[0 * B + li] = u[0];
+        sA[1 * B + li] = beta;
+        sA[2 * B + li] = log(u[0]);
+        sA[3 * B + li] = log(beta);
+        sA[4 * B + li] = v0;
+        sA[5 * B + li] = v1;
+        sA[6 * B + li] = v2;
+        sA[7 * B + li] = v0 * v0 + v1 * v1 + v2 * v2;
+        for (int a = 0; a < 9; ++a) sJa[a * B + li] = act ? p.Ja[((long long)c * 9 + a) * NN + node] : ((a % 4 == 0) ? 1.0 : 0.0);
     }
     const double alpha = act ? p.alpha[c] : 0.0;
     const double alpha_d = act ? p.alpha_d[c] : 0.0;
-    __syncthreads();
+    const double gm1 = gs.gma - 1.0;
 
-    // ---- high-order inviscid residual (calc_cell_ho_inv_residual, dgsem/plens.cc:3153-3275)
+    // ---- high-order inviscid volume term, each unordered pair once (calc_cell_ho_inv_residual,
+    //      dgsem/plens.cc:3202-3234): r_i += 2 D_ij F#(u_i, u_j; (Ja_i+Ja_j)/2), F#_ji = F#_ij
     double ri[5] = {0, 0, 0, 0, 0};
-    if (inblk) {
-        for (int m = 0; m <= N; ++m) {
-            for (int md = 0; md < 3; ++md) {
-                int to[3] = {ti, tj, tk};
-                to[md] = m;
-                const int jo = lc * NN + to[0] + NP * (to[1] + NP * to[2]);
-                double uj[5], dir[3];
-                for (int v = 0; v < 5; ++v) uj[v] = sQ[v * B + jo];
-                for (int a = 0; a < 3; ++a) dir[a] = 0.5 * (sJa[(3 * md + a) * B + li] + sJa[(3 * md + a) * B + jo]);
-                const double nr = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
-                for (int a = 0; a < 3; ++a) dir[a] /= nr;
-                double fl[5];
-                vol_flux(u, uj, dir, gs.gma, fl);
-                const double Dn = 2.0 * sD[tt[md] * NP + m] * nr;
-                for (int v = 0; v < 5; ++v) ri[v] += Dn * fl[v];
+    for (int md = 0; md < 3; ++md) {
+        __syncthreads();
+        for (int it = tid; it < BP; it += blockDim.x) {
+            const int cl = it / NITEM, rem = it - cl * NITEM;
+            const int line = rem / NPAIR, pr = rem - line * NPAIR;
+            // pair index -> (a,b)
+            int a = 0, base = 0;
+            while (pr >= base + (NP - 1 - a)) { base += NP - 1 - a; ++a; }
+            const int b = a + 1 + (pr - base);
+            const int l1 = line % NP, l2 = line / NP;   // indices of the other two directions
+            int ia[3], ib[3];
+            ia[md] = a; ib[md] = b;
+            ia[(md + 1) % 3] = l1; ib[(md + 1) % 3] = l1;
+            ia[(md + 2) % 3] = l2; ib[(md + 2) % 3] = l2;
+            const int ja = cl * NN + ia[0] + NP * (ia[1] + NP * ia[2]);
+            const int jb = cl * NN + ib[0] + NP * (ib[1] + NP * ib[2]);
+            double dir[3];
+            for (int q = 0; q < 3; ++q) dir[q] = 0.5 * (sJa[(3 * md + q) * B + ja] + sJa[(3 * md + q) * B + jb]);
+            const double ra = sA[ja], rb = sA[jb];
+            const double ba = sA[B + ja], bb = sA[B + jb];
+            const double rho_ln = ln_mean(ra, rb, sA[2 * B + ja], sA[2 * B + jb]);
+            const double b_ln = ln_mean(ba, bb, sA[3 * B + ja], sA[3 * B + jb]);
+            const double p_hat = 0.5 * (ra + rb) / (ba + bb);
+            const double va0 = 0.5 * (sA[4 * B + ja] + sA[4 * B + jb]);
+            const double va1 = 0.5 * (sA[5 * B + ja] + sA[5 * B + jb]);
+            const double va2 = 0.5 * (sA[6 * B + ja] + sA[6 * B + jb]);
+            const double H = 0.5 / (gm1 * b_ln) + p_hat / rho_ln + (va0 * va0 + va1 * va1 + va2 * va2) -
+                             0.25 * (sA[7 * B + ja] + sA[7 * B + jb]);
+            const double rv = rho_ln * (va0 * dir[0] + va1 * dir[1] + va2 * dir[2]);
+            sF[0 * BP + it] = rv;
+            sF[1 * BP + it] = rv * va0 + p_hat * dir[0];
+            sF[2 * BP + it] = rv * va1 + p_hat * dir[1];
+            sF[3 * BP + it] = rv * va2 + p_hat * dir[2];
+            sF[4 * BP + it] = rv * H;
+        }
+        __syncthreads();
+        if (inblk) {
+            const int me = tt[md];
+            const int l1 = tt[(md + 1) % 3], l2 = tt[(md + 2) % 3];
+            const int pbase = lc * NITEM + (l1 + NP * l2) * NPAIR;
+            for (int m = 0; m <= N; ++m) {
+                const double d2 = 2.0 * sD[me * NP + m];
+                if (m == me) {  // consistent diagonal term: F#(u,u; Ja) = f(u) . Ja
+                    double dj[3], fi[5];
+                    for (int q = 0; q < 3; ++q) dj[q] = sJa[(3 * md + q) * B + li];
+                    inv_flux(u, dj, gs.gma, fi);
+                    for (int v = 0; v < 5; ++v) ri[v] += d2 * fi[v];
+                } else {
+                    const int k = pbase + sPair[me * NP + m];
+                    for (int v = 0; v < 5; ++v) ri[v] += d2 * sF[v * BP + k];
+                }
             }
         }
     }
+
+    // ---- high-order viscous volume term (calc_cell_ho_dif_residual, dgsem/plens.cc:3283-3413)
     double rd[5] = {0, 0, 0, 0, 0};
     double av[9];
     if (!gs.inviscid) {
         for (int a = 0; a < 9; ++a) av[a] = act ? p.avars[a * p.ndof + g] : 0.0;
-        // ---- high-order viscous volume term (calc_cell_ho_dif_residual, dgsem/plens.cc:3283-3413),
-        //      one reference direction at a time through LDS
         for (int md = 0; md < 3; ++md) {
             __syncthreads();
             if (inblk) {
                 double dir[3];
                 for (int a = 0; a < 3; ++a) dir[a] = sJa[(3 * md + a) * B + li];
-                const double nr = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
-                for (int a = 0; a < 3; ++a) dir[a] /= nr;
                 double fl[5];
-                dif_flux(u, av, dir, fl);
-                sX[0 * B + li] = nr;
-                for (int v = 1; v < 5; ++v) sX[v * B + li] = fl[v];
+                dif_flux(u, av, dir, fl);  // linear in dir: |Ja| F_v(Ja/|Ja|)
+                for (int v = 1; v < 5; ++v) sF[v * B + li] = fl[v];
             }
             __syncthreads();
             if (inblk) {
@@ -523,38 +683,37 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
                     int to[3] = {ti, tj, tk};
                     to[md] = m;
                     const int jo = lc * NN + to[0] + NP * (to[1] + NP * to[2]);
-                    const double Dn = sD[tt[md] * NP + m] * sX[jo];
-                    for (int v = 1; v < 5; ++v) rd[v] += Dn * sX[v * B + jo];
+                    const double Dv = sD[tt[md] * NP + m];
+                    for (int v = 1; v < 5; ++v) rd[v] += Dv * sF[v * B + jo];
                 }
             }
         }
     }
+    if (!act) return;
 
-    // ---- surface terms of both high-order residuals
-    if (act) {
-        for (int f = 0; f < 6; ++f) {
-            if (!on_face(f, ti, tj, tk, N)) continue;
-            const int q = face_dof<N>(f, ti, tj, tk);
-            const int sd = f >> 1, lr = f & 1;
-            const double sg = lr ? -1.0 : 1.0;
-            const double wend = sW[lr * N];
-            double dir[3];
-            for (int a = 0; a < 3; ++a) dir[a] = sJa[(3 * sd + a) * B + li];
-            const double nr = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
-            for (int a = 0; a < 3; ++a) dir[a] /= nr;
-            const long long o = ((long long)c * 6 + f) * 5 * NF + q;
-            double fi[5];
-            inv_flux(u, dir, gs.gma, fi);
+    // ---- surface terms of both high-order residuals (stored stage-2/3 fluxes, outward normals)
+    for (int f = 0; f < 6; ++f) {
+        if (!on_face(f, ti, tj, tk, N)) continue;
+        const int q = face_dof<N>(f, ti, tj, tk);
+        const int sd = f >> 1, lr = f & 1;
+        const double sg = lr ? -1.0 : 1.0;
+        const double wend = sW[lr * N];
+        double dir[3];
+        for (int a = 0; a < 3; ++a) dir[a] = sJa[(3 * sd + a) * B + li];
+        const double nr = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+        for (int a = 0; a < 3; ++a) dir[a] /= nr;
+        const long long o = ((long long)c * 6 + f) * 5 * NF + q;
+        double fi[5];
+        inv_flux(u, dir, gs.gma, fi);
+        for (int v = 0; v < 5; ++v) {
+            const double fs = -sg * p.sinv[o + v * NF];
+            ri[v] += -sg * (fs - fi[v]) * nr / wend;
+        }
+        if (!gs.inviscid) {
+            dif_flux(u, av, dir, fi);
             for (int v = 0; v < 5; ++v) {
-                const double fs = -sg * p.sinv[o + v * NF];
-                ri[v] += -sg * (fs - fi[v]) * nr / wend;
-            }
-            if (!gs.inviscid) {
-                dif_flux(u, av, dir, fi);
-                for (int v = 0; v < 5; ++v) {
-                    const double fs = -sg * p.svis[o + v * NF];
-                    rd[v] += -sg * (fs - fi[v]) * nr / wend;
-                }
+                const double fs = -sg * p.svis[o + v * NF];
+                rd[v] += -sg * (fs - fi[v]) * nr / wend;
             }
         }
     }
@@ -565,78 +724,10 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
         for (int v = 0; v < 5; ++v) rd[v] *= jp;
     }
 
-    // ---- low-order subcell FV residual for blended cells (calc_cell_lo_inv_residual, 3469-3583)
-    double rl[5] = {0, 0, 0, 0, 0};
-    bool any_blend = false;
-    {
-        __shared__ int s_any;
-        if (tid == 0) s_any = 0;
-        __syncthreads();
-        if (act && alpha > 0.0 && node == 0) atomicOr(&s_any, 1);
-        __syncthreads();
-        any_blend = s_any != 0;
-    }
-    if (any_blend) {
-        const double bl = alpha <= 0.0 ? 0.0 : (alpha <= 1.0 ? alpha : 1.0);
-        for (int dir = 0; dir < 3; ++dir) {
-            __syncthreads();
-            if (act && alpha > 0.0 && tt[dir] < N) {
-                // subcell normal n_(id-1,id), id = tt[dir]+1 (MetricTerms::reinit, metric_terms.cc:104-140)
-                const int id = tt[dir] + 1;
-                double s[3];
-                int t0[3] = {ti, tj, tk};
-                t0[dir] = 0;
-                const int j0 = lc * NN + t0[0] + NP * (t0[1] + NP * t0[2]);
-                for (int a = 0; a < 3; ++a) s[a] = sJa[(3 * dir + a) * B + j0];
-                for (int l = 0; l < id; ++l)
-                    for (int m = 0; m <= N; ++m) {
-                        int tm[3] = {ti, tj, tk};
-                        tm[dir] = m;
-                        const int jm = lc * NN + tm[0] + NP * (tm[1] + NP * tm[2]);
-                        for (int a = 0; a < 3; ++a) s[a] += sQm[l * NP + m] * sJa[(3 * dir + a) * B + jm];
-                    }
-                const double nn = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
-                for (int a = 0; a < 3; ++a) s[a] /= nn;
-                int tr[3] = {ti, tj, tk};
-                tr[dir] += 1;
-                const int jr = lc * NN + tr[0] + NP * (tr[1] + NP * tr[2]);
-                double ur[5], fl[5];
-                for (int v = 0; v < 5; ++v) ur[v] = sQ[v * B + jr];
-                inv_surf_flux(p.scheme, bl, u, ur, s, gs.gma, fl);
-                for (int v = 0; v < 5; ++v) sX[v * B + li] = fl[v] * nn;
-            }
-            __syncthreads();
-            if (act && alpha > 0.0) {
-                if (tt[dir] > 0) {
-                    int tl_[3] = {ti, tj, tk};
-                    tl_[dir] -= 1;
-                    const int jl = lc * NN + tl_[0] + NP * (tl_[1] + NP * tl_[2]);
-                    for (int v = 0; v < 5; ++v) rl[v] -= sX[v * B + jl] / sW[tt[dir]];
-                }
-                if (tt[dir] < N)
-                    for (int v = 0; v < 5; ++v) rl[v] += sX[v * B + li] / sW[tt[dir]];
-            }
-        }
-        if (act && alpha > 0.0) {
-            for (int f = 0; f < 6; ++f) {
-                if (!on_face(f, ti, tj, tk, N)) continue;
-                const int q = face_dof<N>(f, ti, tj, tk);
-                const int sd = f >> 1, lr = f & 1;
-                const double cs = lr ? 1.0 : -1.0;
-                const double d0 = sJa[(3 * sd + 0) * B + li], d1 = sJa[(3 * sd + 1) * B + li], d2 = sJa[(3 * sd + 2) * B + li];
-                const double nn = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-                const long long o = ((long long)c * 6 + f) * 5 * NF + q;
-                for (int v = 0; v < 5; ++v) {
-                    const double fl = cs * p.sinv[o + v * NF];
-                    rl[v] += cs * fl * nn / sW[lr * N];
-                }
-            }
-            for (int v = 0; v < 5; ++v) rl[v] /= -detJ;
-        }
-    }
-    if (!act) return;
-
     // ---- blend (calc_rhs, dgsem/plens.cc:3753-3771) and RK epilogue (update_rk4, 4353-4397)
+    double rl[5] = {0, 0, 0, 0, 0};
+    if (alpha > 0.0)
+        for (int v = 0; v < 5; ++v) rl[v] = p.rlo[v * p.ndof + g];
     double rhs[5];
     for (int v = 0; v < 5; ++v) {
         if (!gs.inviscid) rhs[v] = (1.0 - alpha_d) * rd[v] + alpha * rl[v] + (1.0 - alpha) * ri[v];
@@ -663,6 +754,14 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
     }
 }
 
+// Streaming copy (16-byte vector loads/stores, grid-stride) used to calibrate the HBM counters and
+// to measure the achievable bandwidth of the device.
+__global__ void __launch_bounds__(256) k_stream_copy(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                     long long n2) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) dst[i] = src[i];
+}
+
 }  // namespace
 
 // =============================================================================================
@@ -674,13 +773,13 @@ struct plens_gpu {
     hipStream_t stream = nullptr;
     int device = 0;
     // device buffers
-    double *Qa = nullptr, *Qb = nullptr, *R[3] = {nullptr, nullptr, nullptr};
+    double *Qa = nullptr, *Qb = nullptr, *R[2] = {nullptr, nullptr};
     double *Ja = nullptr, *detJ = nullptr, *nrm = nullptr, *hmin = nullptr;
     int *nbr = nullptr, *bid = nullptr, *wall = nullptr, *tasks = nullptr;
     BCDev* bcs = nullptr;
     Basis* basis = nullptr;
     double *avars = nullptr, *araw = nullptr, *alpha = nullptr, *alpha_d = nullptr;
-    double *sinv = nullptr, *svis = nullptr, *rhs = nullptr;
+    double *sinv = nullptr, *svis = nullptr, *rhs = nullptr, *rlo = nullptr;
     double *dt_part = nullptr, *scal = nullptr;  // scal[0] = dt, scal[1] = t accumulator
     unsigned long long* fail = nullptr;
     bool state_in_b = false;
@@ -688,7 +787,7 @@ struct plens_gpu {
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     double graph_courant = -1;
     int graph_cpb = -1;
-    double ktimes[4] = {0, 0, 0, 0};
+    double ktimes[5] = {0, 0, 0, 0, 0};
     double last_run_ms = 0;
     hipEvent_t ev_run[2] = {nullptr, nullptr};
     int64_t fail_dof = -1;
@@ -735,7 +834,9 @@ int launch_stage_t(plens_gpu* h, const KP& kp, hipEvent_t* ev) {
     const int thr = threads_for(kp.cpb, NN);
     const int blocks = (kp.nc + kp.cpb - 1) / kp.cpb;
     const size_t lds_grad = sizeof(double) * 16 * kp.cpb * NN;
-    const size_t lds_vol = sizeof(double) * 20 * kp.cpb * NN;
+    const size_t lds_lo = sizeof(double) * 19 * kp.cpb * NN;
+    const size_t nitem = (size_t)NP * NP * (NP * (NP - 1) / 2);
+    const size_t lds_vol = sizeof(double) * (17 * kp.cpb * NN + 5 * std::max((size_t)kp.cpb * nitem, (size_t)kp.cpb * NN));
     if (ev) HIPCHK(hipEventRecord(ev[0], h->stream));
     hipLaunchKernelGGL(k_grad<N>, dim3(blocks), dim3(thr), lds_grad, h->stream, kp);
     HIPCHK(hipGetLastError());
@@ -749,9 +850,12 @@ int launch_stage_t(plens_gpu* h, const KP& kp, hipEvent_t* ev) {
                        dim3(kThreadsFace), 0, h->stream, kp);
     HIPCHK(hipGetLastError());
     if (ev) HIPCHK(hipEventRecord(ev[3], h->stream));
-    hipLaunchKernelGGL(k_vol<N>, dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
+    hipLaunchKernelGGL(k_lo<N>, dim3(blocks), dim3(thr), lds_lo, h->stream, kp);
     HIPCHK(hipGetLastError());
     if (ev) HIPCHK(hipEventRecord(ev[4], h->stream));
+    hipLaunchKernelGGL(k_vol<N>, dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
+    HIPCHK(hipGetLastError());
+    if (ev) HIPCHK(hipEventRecord(ev[5], h->stream));
     return PLENS_OK;
 }
 
@@ -798,7 +902,7 @@ KP stage_params(plens_gpu* h, int s, bool from_b, double courant) {
 int launch_step(plens_gpu* h, bool from_b, double courant, hipEvent_t* ev = nullptr) {
     for (int s = 1; s <= 5; ++s) {
         KP kp = stage_params(h, s, from_b, courant);
-        const int e = launch_stage(h, kp, ev ? ev + 5 * (s - 1) : nullptr);
+        const int e = launch_stage(h, kp, ev ? ev + 6 * (s - 1) : nullptr);
         if (e) return e;
     }
     return PLENS_OK;
@@ -895,7 +999,7 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
     h->ntasks = (int)tasks.size();
     const int nblk_max = (nc + 0) / 1 + 1;
     if ((rc = dmalloc(&h->Qa, 5 * nd)) || (rc = dmalloc(&h->Qb, 5 * nd)) || (rc = dmalloc(&h->R[0], 5 * nd)) ||
-        (rc = dmalloc(&h->R[1], 5 * nd)) || (rc = dmalloc(&h->R[2], 5 * nd)) || (rc = dmalloc(&h->rhs, 5 * nd)) ||
+        (rc = dmalloc(&h->R[1], 5 * nd)) || (rc = dmalloc(&h->rhs, 5 * nd)) || (rc = dmalloc(&h->rlo, 5 * nd)) ||
         (rc = dmalloc(&h->Ja, 9 * nd)) || (rc = dmalloc(&h->detJ, nd)) ||
         (rc = dmalloc(&h->nrm, (size_t)nc * 18 * NF)) || (rc = dmalloc(&h->hmin, nc)) ||
         (rc = dmalloc(&h->nbr, (size_t)nc * 6)) || (rc = dmalloc(&h->bid, (size_t)nc * 6)) ||
@@ -948,7 +1052,7 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
     if (hipMemset(h->avars, 0, sizeof(double) * 9 * nd) != hipSuccess ||
         hipMemset(h->R[0], 0, sizeof(double) * 5 * nd) != hipSuccess ||
         hipMemset(h->R[1], 0, sizeof(double) * 5 * nd) != hipSuccess ||
-        hipMemset(h->R[2], 0, sizeof(double) * 5 * nd) != hipSuccess ||
+        hipMemset(h->rlo, 0, sizeof(double) * 5 * nd) != hipSuccess ||
         hipMemset(h->alpha, 0, sizeof(double) * nc) != hipSuccess ||
         hipMemset(h->scal, 0, sizeof(double) * 2) != hipSuccess)
         return fail(PLENS_E_HIP);
@@ -960,7 +1064,7 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
     k.Ja = h->Ja; k.detJ = h->detJ; k.nrm = h->nrm; k.hmin = h->hmin;
     k.nbr = h->nbr; k.bid = h->bid; k.wall = h->wall; k.tasks = h->tasks; k.bcs = h->bcs; k.basis = h->basis;
     k.avars = h->avars; k.araw = h->araw; k.alpha = h->alpha; k.alpha_d = h->alpha_d;
-    k.sinv = h->sinv; k.svis = h->svis; k.rhs_out = h->rhs;
+    k.sinv = h->sinv; k.svis = h->svis; k.rhs_out = h->rhs; k.rlo = h->rlo;
     k.dt_part = h->dt_part; k.dt = h->scal; k.t_acc = h->scal + 1; k.fail = h->fail;
     k.gas.gma = ph->gamma; k.gas.R = R; k.gas.cv = R / (ph->gamma - 1); k.gas.mu0 = ph->mu0; k.gas.T0 = ph->T0;
     k.gas.S = ph->S; k.gas.Pr = ph->prandtl; k.gas.inviscid = ph->mu0 < 1e-16;
@@ -979,9 +1083,9 @@ void plens_gpu_destroy(plens_gpu* h) {
     destroy_graphs(h);
     for (auto& x : h->ev_run)
         if (x) (void)hipEventDestroy(x);
-    void* ptrs[] = {h->Qa, h->Qb, h->R[0], h->R[1], h->R[2], h->Ja, h->detJ, h->nrm, h->hmin, h->nbr,
+    void* ptrs[] = {h->Qa, h->Qb, h->R[0], h->R[1], h->Ja, h->detJ, h->nrm, h->hmin, h->nbr,
                     h->bid, h->wall, h->tasks, h->bcs, h->basis, h->avars, h->araw, h->alpha, h->alpha_d,
-                    h->sinv, h->svis, h->rhs, h->dt_part, h->scal, h->fail};
+                    h->sinv, h->svis, h->rhs, h->rlo, h->dt_part, h->scal, h->fail};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1138,22 +1242,22 @@ int plens_gpu_run_profiled(plens_gpu* h, int32_t n_steps, double courant, double
     if (e) return e;
     const double zero = 0.0;
     HIPCHK(hipMemcpyAsync(h->scal + 1, &zero, sizeof(double), hipMemcpyHostToDevice, h->stream));
-    std::vector<hipEvent_t> ev(25);
+    std::vector<hipEvent_t> ev(30);
     for (auto& x : ev) HIPCHK(hipEventCreate(&x));
-    double acc[4] = {0, 0, 0, 0};
+    double acc[5] = {0, 0, 0, 0, 0};
     for (int s = 0; s < n_steps; ++s) {
         if ((e = launch_step(h, h->state_in_b, courant, ev.data()))) return e;
         h->state_in_b = !h->state_in_b;
         HIPCHK(hipStreamSynchronize(h->stream));
         for (int st = 0; st < 5; ++st)
-            for (int k = 0; k < 4; ++k) {
+            for (int k = 0; k < 5; ++k) {
                 float ms = 0;
-                HIPCHK(hipEventElapsedTime(&ms, ev[5 * st + k], ev[5 * st + k + 1]));
+                HIPCHK(hipEventElapsedTime(&ms, ev[6 * st + k], ev[6 * st + k + 1]));
                 acc[k] += ms;
             }
     }
     for (auto& x : ev) (void)hipEventDestroy(x);
-    for (int k = 0; k < 4; ++k) h->ktimes[k] = acc[k] / (5.0 * n_steps);
+    for (int k = 0; k < 5; ++k) h->ktimes[k] = acc[k] / (5.0 * n_steps);
     if ((e = read_failure(h))) return e;
     double tt = 0;
     HIPCHK(hipMemcpyAsync(&tt, h->scal + 1, sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -1164,8 +1268,40 @@ int plens_gpu_run_profiled(plens_gpu* h, int32_t n_steps, double courant, double
 
 int plens_gpu_kernel_times(const plens_gpu* h, double* t, int32_t n) {
     if (!h || !t) return PLENS_E_INVALID;
-    for (int i = 0; i < n && i < 4; ++i) t[i] = h->ktimes[i];
+    for (int i = 0; i < n && i < 5; ++i) t[i] = h->ktimes[i];
     return PLENS_OK;
+}
+
+int plens_gpu_stream_copy(int64_t bytes, int32_t iters, double* ms_per_copy) {
+    if (bytes < 16 || iters < 1 || !ms_per_copy) return PLENS_E_INVALID;
+    const long long n2 = bytes / 16;
+    double2 *a = nullptr, *b = nullptr;
+    if (hipMalloc((void**)&a, n2 * 16) != hipSuccess) return PLENS_E_NOMEM;
+    if (hipMalloc((void**)&b, n2 * 16) != hipSuccess) {
+        (void)hipFree(a);
+        return PLENS_E_NOMEM;
+    }
+    int rc = PLENS_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const int blocks = 256 * 8 * 4;
+    float ms = 0;
+    if (hipMemset(a, 0, n2 * 16) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        rc = PLENS_E_HIP;
+    } else {
+        hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, nullptr, a, b, n2);  // warm-up
+        (void)hipEventRecord(e0, nullptr);
+        for (int i = 0; i < iters; ++i)
+            hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, nullptr, a, b, n2);
+        (void)hipEventRecord(e1, nullptr);
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = PLENS_E_HIP;
+        *ms_per_copy = ms / iters;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return rc;
 }
 
 int plens_gpu_last_run_ms(const plens_gpu* h, double* ms) {

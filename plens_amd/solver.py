@@ -93,8 +93,8 @@ class GpuPlens:
     def run_profiled(self, n_steps, courant):
         t = C.c_double(0.0)
         self._check(self.lib.plens_gpu_run_profiled(self._h, n_steps, courant, C.byref(t)), "run_profiled")
-        times = np.zeros(4)
-        self.lib.plens_gpu_kernel_times(self._h, _dptr(times), 4)
+        times = np.zeros(5)
+        self.lib.plens_gpu_kernel_times(self._h, _dptr(times), 5)
         return t.value, times
 
     def set_tuning(self, cells_per_block):
