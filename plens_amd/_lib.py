@@ -21,6 +21,7 @@ BC_PERIODIC, BC_UNIFORM_INFLOW, BC_OUTFLOW, BC_UNIFORM_TEMP_WALL = 0, 1, 2, 3
 PREC_FP64, PREC_FP32_BULK = 0, 1
 
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int32)
 
 
@@ -63,6 +64,11 @@ HOST_SYMBOLS = [
     "plens_mesh_detJ", "plens_mesh_face_normals", "plens_mesh_face_nbr", "plens_mesh_face_bid",
     "plens_mesh_min_vertex_dist", "plens_basis_1d", "plens_prim_to_cons", "plens_ic_density_wave",
     "plens_ic_normal_shock", "plens_ic_uniform", "plens_face_node",
+]
+MLP_SYMBOLS = [
+    "plens_mlp_n_params", "plens_mlp_create", "plens_mlp_destroy", "plens_mlp_get_params",
+    "plens_mlp_device_bf16_params", "plens_mlp_predict", "plens_mlp_grad", "plens_mlp_step", "plens_mlp_train",
+    "plens_gpu_set_learned_indicator", "plens_gpu_indicator_stats", "plens_gpu_indicator_samples",
 ]
 
 _lib = None
@@ -116,6 +122,18 @@ def load():
                                             C.c_double, C.c_uint64, _dp]),
         "plens_ic_uniform": (C.c_int, [vp, C.c_double, C.c_double, _dp, C.c_double, _dp]),
         "plens_face_node": (C.c_int32, [C.c_int32, C.c_int32, C.c_int32]),
+        "plens_mlp_n_params": (C.c_int32, []),
+        "plens_mlp_create": (C.c_int, [_fp, C.POINTER(vp)]),
+        "plens_mlp_destroy": (C.c_int, [vp]),
+        "plens_mlp_get_params": (C.c_int, [vp, _fp]),
+        "plens_mlp_device_bf16_params": (C.c_void_p, [vp]),
+        "plens_mlp_predict": (C.c_int, [vp, _fp, C.c_int64, _fp]),
+        "plens_mlp_grad": (C.c_int, [vp, _fp, _fp, C.c_int32, _fp, _fp]),
+        "plens_mlp_step": (C.c_int, [vp, _fp, _fp, C.c_int32, C.c_float]),
+        "plens_mlp_train": (C.c_int, [vp, _fp, _fp, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _fp]),
+        "plens_gpu_set_learned_indicator": (C.c_int, [vp, C.c_void_p, C.c_double]),
+        "plens_gpu_indicator_stats": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_int32]),
+        "plens_gpu_indicator_samples": (C.c_int, [vp, _fp, _dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

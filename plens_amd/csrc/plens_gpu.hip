@@ -30,12 +30,17 @@
 #include "flux.h"
 #include "plens_gpu.h"
 #include "plens_host.h"
+#include "plens_mlp.h"
 
 using namespace plens_dev;
 
 namespace {
 
 constexpr int kMaxNP = 6;
+// learned-indicator network layout (matches plens_amd/csrc/mlp.hip): W1[64][125], b1, W2[64][64], b2, w3, b3
+constexpr int kMlpIn = 125, kMlpH = 64;
+constexpr int kMlpB1 = kMlpH * kMlpIn, kMlpW2 = kMlpB1 + kMlpH, kMlpB2 = kMlpW2 + kMlpH * kMlpH,
+              kMlpW3 = kMlpB2 + kMlpH, kMlpB3 = kMlpW3 + kMlpH;
 constexpr int kThreadsFace = 256;
 constexpr int kThreadsAlpha = 256;
 
@@ -77,6 +82,12 @@ struct KP {
     const double* R2;
     double* rhs_out;
     double* rlo;
+    int* blist;
+    int* bcount;
+    const float* mlp;        // BF16-rounded MLP parameters (nullptr: plain indicator)
+    double mlp_tol;          // fallback threshold |alpha_mlp - alpha_plain|
+    unsigned long long* mlp_stats;  // [0] cells evaluated, [1] fallbacks
+    float* feat;             // optional features [n_cells][125] of the indicator
     double* dt_part;
     double* dt;
     double* t_acc;
@@ -84,10 +95,28 @@ struct KP {
     Gas gas;
     int scheme;
     double thr, sharp, amin, amax, wlimit, courant;
+    double iw0, iwN;  // 1 / w_0, 1 / w_N of the GLL rule
     int stage;    // 0: write rhs only; 1..5: RK stage
     int want_dt;  // compute the stage-1 time step
     double c1, c2, c3;
 };
+
+// Workgroups are dispatched round-robin over the 8 XCDs of MI355X (workgroup b on XCD b % 8), each
+// XCD with its own L2. This bijection of [0, nblk) gives XCD x one contiguous range of work, so face
+// gathers of neighbouring cells hit the same L2.
+// largest power of two strictly below n (first stride of a tree sum over n elements)
+template <int n>
+constexpr int kPow2Half() {
+    int p = 1;
+    while (2 * p < n) p *= 2;
+    return p;
+}
+
+__device__ __forceinline__ int xcd_block(int b, int nblk) {
+    constexpr int X = 8;
+    const int q = nblk / X, r = nblk % X, x = b % X;
+    return x * q + min(x, r) + b / X;
+}
 
 // FaceDoFInfo::form_maps (dgsem/face_dof_info.h:92-150)
 template <int N>
@@ -116,6 +145,11 @@ __device__ __forceinline__ bool on_face(int f, int i, int j, int k, int N) {
 __device__ __forceinline__ void load_state(const double* Q, long long ndof, long long g, double u[5]) {
 #pragma unroll
     for (int v = 0; v < 5; ++v) u[v] = Q[v * ndof + g];
+}
+
+__device__ __forceinline__ float mlp_gelu(float x) {
+    const float t = tanhf(0.7978845608028654f * (x + 0.044715f * x * x * x));
+    return 0.5f * x * (1.0f + t);
 }
 
 __host__ __device__ __forceinline__ void prim_to_cons(double gma, double rho, const double vel[3], double p, double c[5]) {
@@ -165,6 +199,7 @@ __device__ void bc_ghost_state(const BCDev& b, int stage, const Gas& g, const do
     }
 }
 
+// basis tables into LDS; sW holds the reciprocal GLL weights 1/w_i (every use divides by w_i)
 template <int N>
 __device__ __forceinline__ void load_basis(const Basis* b, double* sD, double* sW, double* sQm, double* sC) {
     constexpr int NP = N + 1;
@@ -173,7 +208,7 @@ __device__ __forceinline__ void load_basis(const Basis* b, double* sD, double* s
         sD[t] = b->D[r * kMaxNP + c];
         sQm[t] = b->Q[r * kMaxNP + c];
         sC[t] = b->C1[r * kMaxNP + c];
-        if (t < NP) sW[t] = b->w[t];
+        if (t < NP) sW[t] = 1.0 / b->w[t];
     }
 }
 
@@ -189,17 +224,19 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
     double* sQ = smem;
     double* sJa = sQ + 5 * B;
     double* sV = sJa + 9 * B;
-    double* sT = sV + B;
+    double* sT = sV + B;   // Legendre modes of the blender variable
+    double* sE = sT + B;   // 3B modal-energy partial sums
     load_basis<N>(p.basis, sD, sW, sQm, sC);
 
     const int tid = threadIdx.x, lc = tid / NN, node = tid - lc * NN;
-    const int c = blockIdx.x * p.cpb + lc;
+    const int c = xcd_block(blockIdx.x, gridDim.x) * p.cpb + lc;
     const bool inblk = lc < p.cpb;
     const bool act = inblk && c < p.nc;
     const int li = lc * NN + node;
     const int ti = node % NP, tj = (node / NP) % NP, tk = node / (NP * NP);
     const long long g = (long long)c * NN + node;
     double u[5] = {1, 0, 0, 0, 1}, detJ = 1;
+    if (blockIdx.x == 0 && tid == 0) p.bcount[0] = 0;  // blended-cell list of this stage
     if (act) {
         load_state(p.Q, p.ndof, g, u);
         for (int v = 0; v < 5; ++v) sQ[v * B + li] = u[v];
@@ -232,24 +269,71 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
         sV[li] = s;
     }
     __syncthreads();
-    if (inblk) {  // z pass, squared mode
-        double s = 0;
-        for (int cc = 0; cc < NP; ++cc) s += sC[tk * NP + cc] * sV[lc * NN + ti + NP * (tj + NP * cc)];
-        sT[li] = s * s;
+    if (inblk) {  // z pass: Legendre mode of this node's index; energies E, E_{N-1}, E_{N-2} summands
+        double m = 0;
+        for (int cc = 0; cc < NP; ++cc) m += sC[tk * NP + cc] * sV[lc * NN + ti + NP * (tj + NP * cc)];
+        sT[li] = m;
+        const double m2 = m * m;
+        sE[li] = m2;
+        sE[B + li] = (ti < N && tj < N && tk < N) ? m2 : 0.0;
+        sE[2 * B + li] = (ti < N - 1 && tj < N - 1 && tk < N - 1) ? m2 : 0.0;
+    }
+    // per-cell tree sums over the NN modes
+    for (int st = kPow2Half<NN>(); st > 0; st >>= 1) {
+        __syncthreads();
+        if (inblk && node < st && node + st < NN) {
+            sE[li] += sE[li + st];
+            sE[B + li] += sE[B + li + st];
+            sE[2 * B + li] += sE[2 * B + li + st];
+        }
     }
     __syncthreads();
+    double a_plain = 0.0;
     if (act && node == 0) {  // get_trouble (blender_calculator.cc:277-303), Hennemann modification for N>2
-        double et = 0, e1 = 0, e2 = 0;
-        for (int m = 0; m < NN; ++m) {
-            const int mi = m % NP, mj = (m / NP) % NP, mk = m / (NP * NP);
-            const double v = sT[lc * NN + m];
-            et += v;
-            if (mi < N && mj < N && mk < N) e1 += v;
-            if (mi < N - 1 && mj < N - 1 && mk < N - 1) e2 += v;
-        }
+        const double et = sE[li], e1 = sE[B + li], e2 = sE[2 * B + li];
         const double trouble = (N > 2) ? fmax(1.0 - e1 / et, 1.0 - e2 / e1) : 1.0 - e1 / et;
         const double at = 1.0 / (1.0 + exp(-p.sharp * (trouble / p.thr - 1.0)));
-        p.araw[c] = (at < p.amin) ? 0.0 : (at < p.amax ? at : p.amax);
+        a_plain = (at < p.amin) ? 0.0 : (at < p.amax ? at : p.amax);
+        p.araw[c] = a_plain;
+    }
+    // ---- learned indicator: features x_k = m_k / m_0 zero-padded to 125; MLP 125-64-64-1
+    //      (GELU, 0.5 sigmoid) on the BF16-rounded weights; plain value kept where they disagree
+    if (p.feat && act) {
+        const double m0 = sT[lc * NN];
+        for (int k = node; k < kMlpIn; k += NN)
+            p.feat[(long long)c * kMlpIn + k] = (k < NN) ? (float)(sT[lc * NN + k] / m0) : 0.0f;
+    }
+    if (p.mlp && N >= 3 && N <= 4) {
+        float* sH = reinterpret_cast<float*>(sV) + lc * 2 * kMlpH;  // 128 floats per cell (NN >= 64 doubles)
+        const float* W = p.mlp;
+        if (inblk && node < kMlpH) {
+            const double m0 = sT[lc * NN];
+            float acc = W[kMlpB1 + node];
+#pragma unroll 2
+            for (int k = 0; k < NN; ++k) acc = fmaf((float)(sT[lc * NN + k] / m0), W[node * kMlpIn + k], acc);
+            sH[node] = mlp_gelu(acc);
+        }
+        __syncthreads();
+        if (inblk && node < kMlpH) {
+            float acc = W[kMlpB2 + node];
+#pragma unroll 2
+            for (int k = 0; k < kMlpH; ++k) acc = fmaf(sH[k], W[kMlpW2 + node * kMlpH + k], acc);
+            sH[kMlpH + node] = mlp_gelu(acc);
+        }
+        __syncthreads();
+        if (act && node == 0) {
+            float acc = W[kMlpB3];
+#pragma unroll 2
+            for (int k = 0; k < kMlpH; ++k) acc = fmaf(sH[kMlpH + k], W[kMlpW3 + k], acc);
+            double a = (double)(0.5f / (1.0f + expf(-acc)));
+            a = (a < p.amin) ? 0.0 : (a < p.amax ? a : p.amax);
+            atomicAdd(&p.mlp_stats[0], 1ull);
+            if (fabs(a - a_plain) > p.mlp_tol) {
+                atomicAdd(&p.mlp_stats[1], 1ull);
+                a = a_plain;
+            }
+            p.araw[c] = a;
+        }
     }
 
     // ---- stage-1 time step candidates (PLENS::calc_time_step, dgsem/plens.cc:3836-3866)
@@ -270,12 +354,16 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
                 double co = p.courant >= 1.0 ? 0.9 : p.courant;
                 cur = co / (pow((double)N, 1.5) * (madv + a / h + (double)(N * N) * mu / (h * h * u[0])));
             }
-            sT[li] = cur;
+            sE[li] = cur;
+        }
+        for (int st = kPow2Half<NN>(); st > 0; st >>= 1) {  // per-cell tree minimum
+            __syncthreads();
+            if (inblk && node < st && node + st < NN) sE[li] = fmin(sE[li], sE[li + st]);
         }
         __syncthreads();
         if (tid == 0) {
             double m = 1e300;
-            for (int t = 0; t < B; ++t) m = fmin(m, sT[t]);
+            for (int cl = 0; cl < p.cpb; ++cl) m = fmin(m, sE[cl * NN]);
             p.dt_part[blockIdx.x] = m;
         }
     }
@@ -287,6 +375,7 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
     for (int d = 0; d < 3; ++d)
         for (int v = 0; v < 5; ++v) gr[d][v] = 0.0;
     const int tt[3] = {ti, tj, tk};
+#pragma unroll 1
     for (int m = 0; m <= N; ++m) {
         for (int md = 0; md < 3; ++md) {
             int to[3] = {ti, tj, tk};
@@ -318,10 +407,10 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
             for (int v = 0; v < 5; ++v) us[v] = 0.5 * (u[v] + ug[v]);
         }
         const double sg = lr ? -1.0 : 1.0;
-        const double wend = sW[lr * N];
+        const double iw = sW[lr * N];
         for (int gd = 0; gd < 3; ++gd) {
             const double ja = sJa[(3 * sd + gd) * B + li];
-            for (int v = 0; v < 5; ++v) gr[gd][v] -= sg * (us[v] * ja - u[v] * ja) / wend;
+            for (int v = 0; v < 5; ++v) gr[gd][v] -= sg * (us[v] * ja - u[v] * ja) * iw;
         }
     }
     const double iJ = 1.0 / detJ;
@@ -368,6 +457,7 @@ __global__ void __launch_bounds__(kThreadsAlpha) k_alpha(KP p) {
         }
         p.alpha[c] = a;
         p.alpha_d[c] = fmin(1.0, a / p.amax);
+        if (a > 0.0) p.blist[atomicAdd(p.bcount, 1)] = c;  // order irrelevant: cells are independent
     }
     if (p.want_dt && blockIdx.x == 0) {
         __shared__ double red[kThreadsAlpha];
@@ -392,13 +482,14 @@ __global__ void __launch_bounds__(kThreadsAlpha) k_alpha(KP p) {
 template <int N>
 __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
-    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long gid = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (gid >= (long long)p.ntasks * NF) return;
     const int t = (int)(gid / NF), q = (int)(gid - (long long)t * NF);
     const int cf = p.tasks[t];
     const int c = cf / 6, f = cf - 6 * c;
     const int b = p.bid[cf], nb = p.nbr[cf];
     const Gas& gs = p.gas;
+    const int sd = f >> 1;
     const long long g = (long long)c * NN + face_node<N>(f, q);
     double u[5], ug[5], n[3];
     load_state(p.Q, p.ndof, g, u);
@@ -418,11 +509,39 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     bl = bl <= 0.0 ? 0.0 : (bl <= 1.0 ? bl : 1.0);  // set_flux_blender_value
     double F[5];
     inv_surf_flux(p.scheme, bl, u, ug, n, gs.gma, F);
+    // Each side stores its finished surface term of calc_cell_ho_inv/dif_residual
+    // (dgsem/plens.cc:3237-3268, 3373-3406): -s (-s F*_side - f(u; Ja/|Ja|)) |Ja| / w_end, with
+    // F*_side the outward flux of that side (-F for the neighbour of an interior face) and
+    // s = +1 / -1 for left / right faces.
+    double dirc[3], nwc, dirn[3] = {0, 0, 0}, nwn = 0;
+    {
+        double ja[3];
+        for (int a = 0; a < 3; ++a) ja[a] = p.Ja[((long long)c * 9 + 3 * sd + a) * NN + (g - (long long)c * NN)];
+        const double nr = sqrt(ja[0] * ja[0] + ja[1] * ja[1] + ja[2] * ja[2]);
+        const double inr = 1.0 / nr;
+        for (int a = 0; a < 3; ++a) dirc[a] = ja[a] * inr;
+        nwc = nr * ((f & 1) ? p.iwN : p.iw0);
+    }
+    if (interior) {
+        double ja[3];
+        for (int a = 0; a < 3; ++a) ja[a] = p.Ja[((long long)nb * 9 + 3 * sd + a) * NN + (gn - (long long)nb * NN)];
+        const double nr = sqrt(ja[0] * ja[0] + ja[1] * ja[1] + ja[2] * ja[2]);
+        const double inr = 1.0 / nr;
+        for (int a = 0; a < 3; ++a) dirn[a] = ja[a] * inr;
+        nwn = nr * ((f & 1) ? p.iw0 : p.iwN);
+    }
+    const double sgc = (f & 1) ? -1.0 : 1.0, sgn = -sgc;
     const long long o = (long long)cf * 5 * NF + q;
-    for (int v = 0; v < 5; ++v) p.sinv[o + v * NF] = F[v];
     const long long on = ((long long)nb * 6 + (f ^ 1)) * 5 * NF + q;
-    if (interior)
-        for (int v = 0; v < 5; ++v) p.sinv[on + v * NF] = -F[v];
+    {
+        double fi[5];
+        inv_flux(u, dirc, gs.gma, fi);
+        for (int v = 0; v < 5; ++v) p.sinv[o + v * NF] = -sgc * (-sgc * F[v] - fi[v]) * nwc;
+        if (interior) {
+            inv_flux(ug, dirn, gs.gma, fi);
+            for (int v = 0; v < 5; ++v) p.sinv[on + v * NF] = -sgn * (-sgn * (-F[v]) - fi[v]) * nwn;
+        }
+    }
     if (gs.inviscid) return;
     // stage 3: BR1 average of the viscous fluxes (navier_stokes.cc:1644-1652)
     double av[9], avg[9];
@@ -438,9 +557,13 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     dif_flux(u, av, n, f1);
     dif_flux(ug, avg, n, f2);
     for (int v = 0; v < 5; ++v) F[v] = 0.5 * (f1[v] + f2[v]);
-    for (int v = 0; v < 5; ++v) p.svis[o + v * NF] = F[v];
-    if (interior)
-        for (int v = 0; v < 5; ++v) p.svis[on + v * NF] = -F[v];
+    double fi[5];
+    dif_flux(u, av, dirc, fi);
+    for (int v = 0; v < 5; ++v) p.svis[o + v * NF] = -sgc * (-sgc * F[v] - fi[v]) * nwc;
+    if (interior) {
+        dif_flux(ug, avg, dirn, fi);
+        for (int v = 0; v < 5; ++v) p.svis[on + v * NF] = -sgn * (-sgn * (-F[v]) - fi[v]) * nwn;
+    }
 }
 
 // =============================================================================================
@@ -451,93 +574,110 @@ __global__ void __launch_bounds__(256) k_lo(KP p) {
     constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
     extern __shared__ double smem[];
     __shared__ double sD[NP * NP], sW[NP], sQm[NP * NP], sC[NP * NP];
-    __shared__ int s_any;
+    const int count = p.bcount[0];  // blended cells of this stage, listed by k_alpha
+    if ((int)blockIdx.x * p.cpb >= count) return;
     const int B = p.cpb * NN;
-    const int tid = threadIdx.x, lc = tid / NN, node = tid - lc * NN;
-    const int c = blockIdx.x * p.cpb + lc;
-    const bool inblk = lc < p.cpb;
-    const bool act = inblk && c < p.nc;
-    const double alpha = act ? p.alpha[c] : 0.0;
-    if (tid == 0) s_any = 0;
-    __syncthreads();
-    if (act && node == 0 && alpha > 0.0) atomicOr(&s_any, 1);
-    __syncthreads();
-    if (!s_any) return;  // block-uniform: no blended cell in this block
     double* sQ = smem;          // 5B
     double* sJa = sQ + 5 * B;   // 9B
     double* sX = sJa + 9 * B;   // 5B
     load_basis<N>(p.basis, sD, sW, sQm, sC);
+    const int tid = threadIdx.x, lc = tid / NN, node = tid - lc * NN;
+    const bool inblk = lc < p.cpb;
     const int li = lc * NN + node;
     const int ti = node % NP, tj = (node / NP) % NP, tk = node / (NP * NP);
     const int tt[3] = {ti, tj, tk};
-    const long long g = (long long)c * NN + node;
     const Gas& gs = p.gas;
-    double u[5] = {1, 0, 0, 0, 1}, detJ = 1;
-    if (act) {
-        load_state(p.Q, p.ndof, g, u);
-        detJ = p.detJ[g];
-    }
-    if (inblk) {
-        for (int v = 0; v < 5; ++v) sQ[v * B + li] = u[v];
-        for (int a = 0; a < 9; ++a) sJa[a * B + li] = act ? p.Ja[((long long)c * 9 + a) * NN + node] : ((a % 4 == 0) ? 1.0 : 0.0);
-    }
-    const bool mine = act && alpha > 0.0;
-    const double bl = alpha <= 0.0 ? 0.0 : (alpha <= 1.0 ? alpha : 1.0);
-    double rl[5] = {0, 0, 0, 0, 0};
-    for (int dir = 0; dir < 3; ++dir) {
-        __syncthreads();
-        if (mine && tt[dir] < N) {
-            // subcell normal n_(id-1,id), id = tt[dir]+1 (MetricTerms::reinit, metric_terms.cc:104-140)
-            const int id = tt[dir] + 1;
-            int t0[3] = {ti, tj, tk};
-            t0[dir] = 0;
-            const int j0 = lc * NN + t0[0] + NP * (t0[1] + NP * t0[2]);
-            double s[3];
-            for (int a = 0; a < 3; ++a) s[a] = sJa[(3 * dir + a) * B + j0];
-            for (int l = 0; l < id; ++l)
-                for (int m = 0; m <= N; ++m) {
-                    int tm[3] = {ti, tj, tk};
-                    tm[dir] = m;
-                    const int jm = lc * NN + tm[0] + NP * (tm[1] + NP * tm[2]);
-                    for (int a = 0; a < 3; ++a) s[a] += sQm[l * NP + m] * sJa[(3 * dir + a) * B + jm];
-                }
-            const double nn = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
-            for (int a = 0; a < 3; ++a) s[a] /= nn;
-            int tr[3] = {ti, tj, tk};
-            tr[dir] += 1;
-            const int jr = lc * NN + tr[0] + NP * (tr[1] + NP * tr[2]);
-            double ur[5], fl[5];
-            for (int v = 0; v < 5; ++v) ur[v] = sQ[v * B + jr];
-            inv_surf_flux(p.scheme, bl, u, ur, s, gs.gma, fl);
-            for (int v = 0; v < 5; ++v) sX[v * B + li] = fl[v] * nn;
+    for (int base = blockIdx.x * p.cpb; base < count; base += gridDim.x * p.cpb) {
+        const int idx = base + lc;
+        const bool act = inblk && idx < count;
+        const int c = act ? p.blist[idx] : 0;
+        const long long g = (long long)c * NN + node;
+        const double alpha = act ? p.alpha[c] : 0.0;
+        const double bl = alpha <= 0.0 ? 0.0 : (alpha <= 1.0 ? alpha : 1.0);
+        double u[5] = {1, 0, 0, 0, 1}, detJ = 1;
+        if (act) {
+            load_state(p.Q, p.ndof, g, u);
+            detJ = p.detJ[g];
         }
-        __syncthreads();
-        if (mine) {
-            if (tt[dir] > 0) {
-                int tl_[3] = {ti, tj, tk};
-                tl_[dir] -= 1;
-                const int jl = lc * NN + tl_[0] + NP * (tl_[1] + NP * tl_[2]);
-                for (int v = 0; v < 5; ++v) rl[v] -= sX[v * B + jl] / sW[tt[dir]];
+        __syncthreads();  // previous iteration done with LDS
+        if (inblk) {
+            for (int v = 0; v < 5; ++v) sQ[v * B + li] = u[v];
+            for (int a = 0; a < 9; ++a)
+                sJa[a * B + li] = act ? p.Ja[((long long)c * 9 + a) * NN + node] : ((a % 4 == 0) ? 1.0 : 0.0);
+        }
+        double rl[5] = {0, 0, 0, 0, 0};
+        for (int dir = 0; dir < 3; ++dir) {
+            __syncthreads();
+            if (act && tt[dir] < N) {
+                // subcell normal n_(id-1,id), id = tt[dir]+1 (MetricTerms::reinit, metric_terms.cc:104-140)
+                const int id = tt[dir] + 1;
+                int t0[3] = {ti, tj, tk};
+                t0[dir] = 0;
+                const int j0 = lc * NN + t0[0] + NP * (t0[1] + NP * t0[2]);
+                double sn[3];
+                for (int a = 0; a < 3; ++a) sn[a] = sJa[(3 * dir + a) * B + j0];
+                for (int l = 0; l < id; ++l)
+                    for (int m = 0; m <= N; ++m) {
+                        int tm[3] = {ti, tj, tk};
+                        tm[dir] = m;
+                        const int jm = lc * NN + tm[0] + NP * (tm[1] + NP * tm[2]);
+                        for (int a = 0; a < 3; ++a) sn[a] += sQm[l * NP + m] * sJa[(3 * dir + a) * B + jm];
+                    }
+                const double nn = sqrt(sn[0] * sn[0] + sn[1] * sn[1] + sn[2] * sn[2]);
+                for (int a = 0; a < 3; ++a) sn[a] /= nn;
+                int tr[3] = {ti, tj, tk};
+                tr[dir] += 1;
+                const int jr = lc * NN + tr[0] + NP * (tr[1] + NP * tr[2]);
+                double ur[5], fl[5];
+                for (int v = 0; v < 5; ++v) ur[v] = sQ[v * B + jr];
+                inv_surf_flux(p.scheme, bl, u, ur, sn, gs.gma, fl);
+                for (int v = 0; v < 5; ++v) sX[v * B + li] = fl[v] * nn;
             }
-            if (tt[dir] < N)
-                for (int v = 0; v < 5; ++v) rl[v] += sX[v * B + li] / sW[tt[dir]];
+            __syncthreads();
+            if (act) {
+                if (tt[dir] > 0) {
+                    int tl_[3] = {ti, tj, tk};
+                    tl_[dir] -= 1;
+                    const int jl = lc * NN + tl_[0] + NP * (tl_[1] + NP * tl_[2]);
+                    for (int v = 0; v < 5; ++v) rl[v] -= sX[v * B + jl] * sW[tt[dir]];
+                }
+                if (tt[dir] < N)
+                    for (int v = 0; v < 5; ++v) rl[v] += sX[v * B + li] * sW[tt[dir]];
+            }
+        }
+        if (act) {
+            // boundary subcells use the stage-2 face flux (dgsem/plens.cc:3544-3575): F*|Ja|/w_end,
+            // recovered from the stored surface term S = (F* + s f_in) |Ja| / w_end
+            for (int f = 0; f < 6; ++f) {
+                if (!on_face(f, ti, tj, tk, N)) continue;
+                const int q = face_dof<N>(f, ti, tj, tk);
+                const int sd = f >> 1, lr = f & 1;
+                const double sg = lr ? -1.0 : 1.0;
+                double dir[3];
+                for (int a = 0; a < 3; ++a) dir[a] = sJa[(3 * sd + a) * B + li];
+                const double nn = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+                const double inn = 1.0 / nn;
+                for (int a = 0; a < 3; ++a) dir[a] *= inn;
+                const double nw = nn * sW[lr * N];
+                double fi[5];
+                inv_flux(u, dir, gs.gma, fi);
+                const long long o = ((long long)c * 6 + f) * 5 * NF + q;
+                for (int v = 0; v < 5; ++v) rl[v] += p.sinv[o + v * NF] - sg * fi[v] * nw;
+            }
+            const double imJ = 1.0 / -detJ;
+            for (int v = 0; v < 5; ++v) p.rlo[v * p.ndof + g] = rl[v] * imJ;
         }
     }
-    if (!mine) return;
-    for (int f = 0; f < 6; ++f) {
-        if (!on_face(f, ti, tj, tk, N)) continue;
-        const int q = face_dof<N>(f, ti, tj, tk);
-        const int sd = f >> 1, lr = f & 1;
-        const double cs = lr ? 1.0 : -1.0;
-        const double d0 = sJa[(3 * sd + 0) * B + li], d1 = sJa[(3 * sd + 1) * B + li], d2 = sJa[(3 * sd + 2) * B + li];
-        const double nn = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-        const long long o = ((long long)c * 6 + f) * 5 * NF + q;
-        for (int v = 0; v < 5; ++v) {
-            const double fl = cs * p.sinv[o + v * NF];
-            rl[v] += cs * fl * nn / sW[lr * N];
-        }
-    }
-    for (int v = 0; v < 5; ++v) p.rlo[v * p.ndof + g] = rl[v] / -detJ;
+}
+
+// 1/x from the hardware reciprocal refined by two Newton steps (full double precision; the quotient
+// a*rcp(b) differs from a correctly rounded a/b by at most one extra rounding)
+__device__ __forceinline__ double rcp64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
 }
 
 // Chandrashekhar two-point flux along an unnormalised direction a from per-node precomputed
@@ -561,16 +701,19 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
     extern __shared__ double smem[];
     __shared__ double sD[NP * NP], sW[NP], sQm[NP * NP], sC[NP * NP];
     __shared__ int sPair[NP * NP];                  // pair index of (a,b), a<b, on a line
+    __shared__ int sPA[NPAIR > 0 ? NPAIR : 1], sPB[NPAIR > 0 ? NPAIR : 1];
     const int B = p.cpb * NN;
     const int BP = p.cpb * NITEM;
     double* sA = smem;            // 8B: rho, beta, ln rho, ln beta, v0, v1, v2, |v|^2
-    double* sJa = sA + 8 * B;     // 9B
-    double* sF = sJa + 9 * B;     // 5*BP pair fluxes (also viscous scratch, 5B <= 5BP for N>=2)
+    double* sJa = sA + 8 * B;     // 3B: Ja^md of the direction being processed
+    double* sF = sJa + 3 * B;     // 5*max(BP,B) pair fluxes / viscous scratch
     load_basis<N>(p.basis, sD, sW, sQm, sC);
     for (int t = threadIdx.x; t < NP * NP; t += blockDim.x) {
         const int a = t / NP, b = t % NP;
         const int lo = a < b ? a : b, hi = a < b ? b : a;
-        sPair[t] = (lo == hi) ? -1 : lo * NP - lo * (lo + 1) / 2 + (hi - lo - 1);
+        const int k = (lo == hi) ? -1 : lo * NP - lo * (lo + 1) / 2 + (hi - lo - 1);
+        sPair[t] = k;
+        if (a < b) { sPA[k] = a; sPB[k] = b; }
     }
 
     const int tid = threadIdx.x, lc = tid / NN, node = tid - lc * NN;
@@ -599,24 +742,26 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
         sA[5 * B + li] = v1;
         sA[6 * B + li] = v2;
         sA[7 * B + li] = v0 * v0 + v1 * v1 + v2 * v2;
-        for (int a = 0; a < 9; ++a) sJa[a * B + li] = act ? p.Ja[((long long)c * 9 + a) * NN + node] : ((a % 4 == 0) ? 1.0 : 0.0);
     }
+    double ja_own[9];
+    for (int a = 0; a < 9; ++a) ja_own[a] = act ? p.Ja[((long long)c * 9 + a) * NN + node] : ((a % 4 == 0) ? 1.0 : 0.0);
     const double alpha = act ? p.alpha[c] : 0.0;
     const double alpha_d = act ? p.alpha_d[c] : 0.0;
-    const double gm1 = gs.gma - 1.0;
+    const double half_gm1_inv = 0.5 / (gs.gma - 1.0);
 
     // ---- high-order inviscid volume term, each unordered pair once (calc_cell_ho_inv_residual,
     //      dgsem/plens.cc:3202-3234): r_i += 2 D_ij F#(u_i, u_j; (Ja_i+Ja_j)/2), F#_ji = F#_ij
     double ri[5] = {0, 0, 0, 0, 0};
+#pragma unroll
     for (int md = 0; md < 3; ++md) {
+        __syncthreads();  // previous direction done with sJa / sF
+        if (inblk)
+            for (int q = 0; q < 3; ++q) sJa[q * B + li] = ja_own[3 * md + q];
         __syncthreads();
         for (int it = tid; it < BP; it += blockDim.x) {
             const int cl = it / NITEM, rem = it - cl * NITEM;
             const int line = rem / NPAIR, pr = rem - line * NPAIR;
-            // pair index -> (a,b)
-            int a = 0, base = 0;
-            while (pr >= base + (NP - 1 - a)) { base += NP - 1 - a; ++a; }
-            const int b = a + 1 + (pr - base);
+            const int a = sPA[pr], b = sPB[pr];
             const int l1 = line % NP, l2 = line / NP;   // indices of the other two directions
             int ia[3], ib[3];
             ia[md] = a; ib[md] = b;
@@ -625,16 +770,37 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
             const int ja = cl * NN + ia[0] + NP * (ia[1] + NP * ia[2]);
             const int jb = cl * NN + ib[0] + NP * (ib[1] + NP * ib[2]);
             double dir[3];
-            for (int q = 0; q < 3; ++q) dir[q] = 0.5 * (sJa[(3 * md + q) * B + ja] + sJa[(3 * md + q) * B + jb]);
+            for (int q = 0; q < 3; ++q) dir[q] = 0.5 * (sJa[q * B + ja] + sJa[q * B + jb]);
+            // log means with shared reciprocals: f = (x-y)/(x+y); series below f^2 = 1e-2, else
+            // (x-y)/(ln x - ln y) with the per-node logarithms
             const double ra = sA[ja], rb = sA[jb];
             const double ba = sA[B + ja], bb = sA[B + jb];
-            const double rho_ln = ln_mean(ra, rb, sA[2 * B + ja], sA[2 * B + jb]);
-            const double b_ln = ln_mean(ba, bb, sA[3 * B + ja], sA[3 * B + jb]);
-            const double p_hat = 0.5 * (ra + rb) / (ba + bb);
+            const double sr = ra + rb, dr = ra - rb, sb = ba + bb, db = ba - bb;
+            const double irs = rcp64(sr), ibs = rcp64(sb);
+            const double fr = dr * irs, fb = db * ibs;
+            const double ur = fr * fr, ub = fb * fb;
+            double rho_ln, inv_rho_ln, inv_b_ln;
+            if (ur < 1e-2) {
+                const double F = 1.0 + ur * (1.0 / 3.0 + ur * (1.0 / 5.0 + ur * (1.0 / 7.0)));
+                inv_rho_ln = 2.0 * F * irs;
+                rho_ln = 0.5 * sr * rcp64(F);
+            } else {
+                const double dl = sA[2 * B + ja] - sA[2 * B + jb];
+                const double idr = rcp64(dr);
+                inv_rho_ln = dl * idr;
+                rho_ln = dr * rcp64(dl);
+            }
+            if (ub < 1e-2) {
+                const double F = 1.0 + ub * (1.0 / 3.0 + ub * (1.0 / 5.0 + ub * (1.0 / 7.0)));
+                inv_b_ln = 2.0 * F * ibs;
+            } else {
+                inv_b_ln = (sA[3 * B + ja] - sA[3 * B + jb]) * rcp64(db);
+            }
+            const double p_hat = 0.5 * sr * ibs;
             const double va0 = 0.5 * (sA[4 * B + ja] + sA[4 * B + jb]);
             const double va1 = 0.5 * (sA[5 * B + ja] + sA[5 * B + jb]);
             const double va2 = 0.5 * (sA[6 * B + ja] + sA[6 * B + jb]);
-            const double H = 0.5 / (gm1 * b_ln) + p_hat / rho_ln + (va0 * va0 + va1 * va1 + va2 * va2) -
+            const double H = half_gm1_inv * inv_b_ln + p_hat * inv_rho_ln + (va0 * va0 + va1 * va1 + va2 * va2) -
                              0.25 * (sA[7 * B + ja] + sA[7 * B + jb]);
             const double rv = rho_ln * (va0 * dir[0] + va1 * dir[1] + va2 * dir[2]);
             sF[0 * BP + it] = rv;
@@ -652,7 +818,7 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
                 const double d2 = 2.0 * sD[me * NP + m];
                 if (m == me) {  // consistent diagonal term: F#(u,u; Ja) = f(u) . Ja
                     double dj[3], fi[5];
-                    for (int q = 0; q < 3; ++q) dj[q] = sJa[(3 * md + q) * B + li];
+                    for (int q = 0; q < 3; ++q) dj[q] = ja_own[3 * md + q];
                     inv_flux(u, dj, gs.gma, fi);
                     for (int v = 0; v < 5; ++v) ri[v] += d2 * fi[v];
                 } else {
@@ -668,11 +834,12 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
     double av[9];
     if (!gs.inviscid) {
         for (int a = 0; a < 9; ++a) av[a] = act ? p.avars[a * p.ndof + g] : 0.0;
+#pragma unroll
         for (int md = 0; md < 3; ++md) {
             __syncthreads();
             if (inblk) {
                 double dir[3];
-                for (int a = 0; a < 3; ++a) dir[a] = sJa[(3 * md + a) * B + li];
+                for (int a = 0; a < 3; ++a) dir[a] = ja_own[3 * md + a];
                 double fl[5];
                 dif_flux(u, av, dir, fl);  // linear in dir: |Ja| F_v(Ja/|Ja|)
                 for (int v = 1; v < 5; ++v) sF[v * B + li] = fl[v];
@@ -691,31 +858,13 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
     }
     if (!act) return;
 
-    // ---- surface terms of both high-order residuals (stored stage-2/3 fluxes, outward normals)
+    // ---- surface terms of both high-order residuals, finished per face node by k_face
     for (int f = 0; f < 6; ++f) {
         if (!on_face(f, ti, tj, tk, N)) continue;
-        const int q = face_dof<N>(f, ti, tj, tk);
-        const int sd = f >> 1, lr = f & 1;
-        const double sg = lr ? -1.0 : 1.0;
-        const double wend = sW[lr * N];
-        double dir[3];
-        for (int a = 0; a < 3; ++a) dir[a] = sJa[(3 * sd + a) * B + li];
-        const double nr = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
-        for (int a = 0; a < 3; ++a) dir[a] /= nr;
-        const long long o = ((long long)c * 6 + f) * 5 * NF + q;
-        double fi[5];
-        inv_flux(u, dir, gs.gma, fi);
-        for (int v = 0; v < 5; ++v) {
-            const double fs = -sg * p.sinv[o + v * NF];
-            ri[v] += -sg * (fs - fi[v]) * nr / wend;
-        }
-        if (!gs.inviscid) {
-            dif_flux(u, av, dir, fi);
-            for (int v = 0; v < 5; ++v) {
-                const double fs = -sg * p.svis[o + v * NF];
-                rd[v] += -sg * (fs - fi[v]) * nr / wend;
-            }
-        }
+        const long long o = ((long long)c * 6 + f) * 5 * NF + face_dof<N>(f, ti, tj, tk);
+        for (int v = 0; v < 5; ++v) ri[v] += p.sinv[o + v * NF];
+        if (!gs.inviscid)
+            for (int v = 0; v < 5; ++v) rd[v] += p.svis[o + v * NF];
     }
     {
         const double jm = 1.0 / (-1.0 * detJ);
@@ -780,6 +929,9 @@ struct plens_gpu {
     Basis* basis = nullptr;
     double *avars = nullptr, *araw = nullptr, *alpha = nullptr, *alpha_d = nullptr;
     double *sinv = nullptr, *svis = nullptr, *rhs = nullptr, *rlo = nullptr;
+    int *blist = nullptr, *bcount = nullptr;
+    unsigned long long* mlp_stats = nullptr;
+    float* feat = nullptr;
     double *dt_part = nullptr, *scal = nullptr;  // scal[0] = dt, scal[1] = t accumulator
     unsigned long long* fail = nullptr;
     bool state_in_b = false;
@@ -833,10 +985,10 @@ int launch_stage_t(plens_gpu* h, const KP& kp, hipEvent_t* ev) {
     constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
     const int thr = threads_for(kp.cpb, NN);
     const int blocks = (kp.nc + kp.cpb - 1) / kp.cpb;
-    const size_t lds_grad = sizeof(double) * 16 * kp.cpb * NN;
+    const size_t lds_grad = sizeof(double) * 19 * kp.cpb * NN;
     const size_t lds_lo = sizeof(double) * 19 * kp.cpb * NN;
     const size_t nitem = (size_t)NP * NP * (NP * (NP - 1) / 2);
-    const size_t lds_vol = sizeof(double) * (17 * kp.cpb * NN + 5 * std::max((size_t)kp.cpb * nitem, (size_t)kp.cpb * NN));
+    const size_t lds_vol = sizeof(double) * (11 * kp.cpb * NN + 5 * std::max((size_t)kp.cpb * nitem, (size_t)kp.cpb * NN));
     if (ev) HIPCHK(hipEventRecord(ev[0], h->stream));
     hipLaunchKernelGGL(k_grad<N>, dim3(blocks), dim3(thr), lds_grad, h->stream, kp);
     HIPCHK(hipGetLastError());
@@ -850,7 +1002,7 @@ int launch_stage_t(plens_gpu* h, const KP& kp, hipEvent_t* ev) {
                        dim3(kThreadsFace), 0, h->stream, kp);
     HIPCHK(hipGetLastError());
     if (ev) HIPCHK(hipEventRecord(ev[3], h->stream));
-    hipLaunchKernelGGL(k_lo<N>, dim3(blocks), dim3(thr), lds_lo, h->stream, kp);
+    hipLaunchKernelGGL(k_lo<N>, dim3(std::min(blocks, 256)), dim3(thr), lds_lo, h->stream, kp);
     HIPCHK(hipGetLastError());
     if (ev) HIPCHK(hipEventRecord(ev[4], h->stream));
     hipLaunchKernelGGL(k_vol<N>, dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
@@ -1008,7 +1160,8 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
         (rc = dmalloc(&h->avars, 9 * nd)) || (rc = dmalloc(&h->araw, nc)) || (rc = dmalloc(&h->alpha, nc)) ||
         (rc = dmalloc(&h->alpha_d, nc)) || (rc = dmalloc(&h->sinv, (size_t)nc * 30 * NF)) ||
         (rc = dmalloc(&h->svis, (size_t)nc * 30 * NF)) || (rc = dmalloc(&h->dt_part, nblk_max)) ||
-        (rc = dmalloc(&h->scal, 2)) || (rc = dmalloc(&h->fail, 1)))
+        (rc = dmalloc(&h->scal, 2)) || (rc = dmalloc(&h->fail, 1)) || (rc = dmalloc(&h->blist, nc)) ||
+        (rc = dmalloc(&h->bcount, 1)) || (rc = dmalloc(&h->mlp_stats, 2)))
         return fail(rc);
     // host-side tables
     std::vector<BCDev> hb(std::max(n_bcs, 1));
@@ -1054,7 +1207,9 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
         hipMemset(h->R[1], 0, sizeof(double) * 5 * nd) != hipSuccess ||
         hipMemset(h->rlo, 0, sizeof(double) * 5 * nd) != hipSuccess ||
         hipMemset(h->alpha, 0, sizeof(double) * nc) != hipSuccess ||
-        hipMemset(h->scal, 0, sizeof(double) * 2) != hipSuccess)
+        hipMemset(h->scal, 0, sizeof(double) * 2) != hipSuccess ||
+        hipMemset(h->bcount, 0, sizeof(int)) != hipSuccess ||
+        hipMemset(h->mlp_stats, 0, 2 * sizeof(unsigned long long)) != hipSuccess)
         return fail(PLENS_E_HIP);
     const unsigned long long init = ~0ull;
     if (hipMemcpy(h->fail, &init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess) return fail(PLENS_E_HIP);
@@ -1065,6 +1220,8 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
     k.nbr = h->nbr; k.bid = h->bid; k.wall = h->wall; k.tasks = h->tasks; k.bcs = h->bcs; k.basis = h->basis;
     k.avars = h->avars; k.araw = h->araw; k.alpha = h->alpha; k.alpha_d = h->alpha_d;
     k.sinv = h->sinv; k.svis = h->svis; k.rhs_out = h->rhs; k.rlo = h->rlo;
+    k.blist = h->blist; k.bcount = h->bcount;
+    k.mlp = nullptr; k.mlp_tol = 0.05; k.mlp_stats = h->mlp_stats; k.feat = nullptr;
     k.dt_part = h->dt_part; k.dt = h->scal; k.t_acc = h->scal + 1; k.fail = h->fail;
     k.gas.gma = ph->gamma; k.gas.R = R; k.gas.cv = R / (ph->gamma - 1); k.gas.mu0 = ph->mu0; k.gas.T0 = ph->T0;
     k.gas.S = ph->S; k.gas.Pr = ph->prandtl; k.gas.inviscid = ph->mu0 < 1e-16;
@@ -1073,6 +1230,8 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
     k.sharp = ph->sharpness_factor; k.amin = ph->alpha_min; k.amax = ph->alpha_max;
     k.wlimit = ph->wall_blender_limit;
     k.n_dt_part = (nc + h->cpb - 1) / h->cpb;
+    k.iw0 = 1.0 / hbas.w[0];
+    k.iwN = 1.0 / hbas.w[N];
     *out = h;
     return PLENS_OK;
 }
@@ -1085,7 +1244,7 @@ void plens_gpu_destroy(plens_gpu* h) {
         if (x) (void)hipEventDestroy(x);
     void* ptrs[] = {h->Qa, h->Qb, h->R[0], h->R[1], h->Ja, h->detJ, h->nrm, h->hmin, h->nbr,
                     h->bid, h->wall, h->tasks, h->bcs, h->basis, h->avars, h->araw, h->alpha, h->alpha_d,
-                    h->sinv, h->svis, h->rhs, h->rlo, h->dt_part, h->scal, h->fail};
+                    h->sinv, h->svis, h->rhs, h->rlo, h->dt_part, h->scal, h->fail, h->blist, h->bcount, h->mlp_stats, h->feat};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1153,7 +1312,7 @@ int plens_gpu_calc_dt(plens_gpu* h, double courant, double* dt) {
     const int NN = h->NN;
     const int thr = threads_for(kp.cpb, NN);
     const int blocks = (kp.nc + kp.cpb - 1) / kp.cpb;
-    const size_t lds = sizeof(double) * 16 * kp.cpb * NN;
+    const size_t lds = sizeof(double) * 19 * kp.cpb * NN;
     // gradient kernel with the time-step candidates (it also refreshes avars), then the reduction
     switch (h->N) {
         case 1: hipLaunchKernelGGL(k_grad<1>, dim3(blocks), dim3(thr), lds, h->stream, kp); break;
@@ -1302,6 +1461,58 @@ int plens_gpu_stream_copy(int64_t bytes, int32_t iters, double* ms_per_copy) {
     (void)hipFree(a);
     (void)hipFree(b);
     return rc;
+}
+
+int plens_gpu_set_learned_indicator(plens_gpu* h, const float* mlp_params_device, double tolerance) {
+    if (!h || tolerance < 0) return PLENS_E_INVALID;
+    h->base.mlp = mlp_params_device;
+    h->base.mlp_tol = tolerance;
+    destroy_graphs(h);
+    return PLENS_OK;
+}
+
+int plens_gpu_indicator_stats(plens_gpu* h, int64_t* evaluated, int64_t* fallbacks, int32_t reset) {
+    if (!h) return PLENS_E_INVALID;
+    unsigned long long st[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(st, h->mlp_stats, sizeof(st), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (evaluated) *evaluated = (int64_t)st[0];
+    if (fallbacks) *fallbacks = (int64_t)st[1];
+    if (reset) HIPCHK(hipMemsetAsync(h->mlp_stats, 0, sizeof(st), h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return PLENS_OK;
+}
+
+int plens_gpu_indicator_samples(plens_gpu* h, float* feats, double* alpha_raw) {
+    if (!h || (!feats && !alpha_raw)) return PLENS_E_INVALID;
+    if (!h->feat && hipMalloc((void**)&h->feat, sizeof(float) * (size_t)h->nc * kMlpIn) != hipSuccess)
+        return PLENS_E_NOMEM;
+    int e = reset_failure(h);
+    if (e) return e;
+    KP kp = h->base;
+    kp.Q = h->state_in_b ? h->Qb : h->Qa;
+    kp.want_dt = 0;
+    kp.t_acc = nullptr;
+    kp.feat = h->feat;  // blender values of the current indicator (plain unless a network is set)
+    const int NN = h->NN;
+    const int thr = threads_for(kp.cpb, NN);
+    const int blocks = (kp.nc + kp.cpb - 1) / kp.cpb;
+    const size_t lds = sizeof(double) * 19 * kp.cpb * NN;
+    switch (h->N) {
+        case 1: hipLaunchKernelGGL(k_grad<1>, dim3(blocks), dim3(thr), lds, h->stream, kp); break;
+        case 2: hipLaunchKernelGGL(k_grad<2>, dim3(blocks), dim3(thr), lds, h->stream, kp); break;
+        case 3: hipLaunchKernelGGL(k_grad<3>, dim3(blocks), dim3(thr), lds, h->stream, kp); break;
+        case 4: hipLaunchKernelGGL(k_grad<4>, dim3(blocks), dim3(thr), lds, h->stream, kp); break;
+        default: hipLaunchKernelGGL(k_grad<5>, dim3(blocks), dim3(thr), lds, h->stream, kp); break;
+    }
+    HIPCHK(hipGetLastError());
+    if ((e = read_failure(h))) return e;
+    if (feats)
+        HIPCHK(hipMemcpyAsync(feats, h->feat, sizeof(float) * (size_t)h->nc * kMlpIn, hipMemcpyDeviceToHost, h->stream));
+    if (alpha_raw)
+        HIPCHK(hipMemcpyAsync(alpha_raw, h->araw, sizeof(double) * h->nc, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return PLENS_OK;
 }
 
 int plens_gpu_last_run_ms(const plens_gpu* h, double* ms) {

@@ -123,3 +123,17 @@ def test_positivity_failure_reported():
     with pytest.raises(PositivityError) as ei:
         g.calc_rhs()
     assert ei.value.dof == bad and ei.value.kind == "density"
+
+
+def test_rhs_many_blended_cells():
+    """noisy state on a 16^3 periodic cube: thousands of blended cells, more than one pass of the
+    subcell kernel's grid over the blended-cell list"""
+    from plens_amd import periodic_cube
+    case = periodic_cube(n=16, degree=3, flux="HLLC")
+    rng = np.random.default_rng(11)
+    Q = case.Q0.copy()
+    Q[0] *= 1 + 0.2 * rng.uniform(-1, 1, Q.shape[1])
+    case.Q0 = Q
+    g, o = _rhs_parity(case, "many-blended")
+    _, a, _ = o.calc_rhs()
+    assert (a > 0).sum() > 2048, (a > 0).sum()

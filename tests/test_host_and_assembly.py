@@ -18,13 +18,15 @@ dp = C.POINTER(C.c_double)
 
 def test_c_abi_exports_every_declared_symbol():
     lib = L.load()
-    for hdr in ("plens_gpu.h", "plens_host.h"):
+    hdrs = sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h"))
+    assert {"plens_gpu.h", "plens_host.h", "plens_mlp.h"} <= set(hdrs)
+    for hdr in hdrs:
         text = open(os.path.join(ROOT, "include", hdr)).read()
         names = set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]*?\b(plens_\w+)\s*\(", text, re.M))
         assert names, hdr
         for nm in names:
             assert hasattr(lib, nm), f"{nm} declared in {hdr} but not exported"
-    for nm in L.GPU_SYMBOLS + L.HOST_SYMBOLS:
+    for nm in L.GPU_SYMBOLS + L.HOST_SYMBOLS + L.MLP_SYMBOLS:
         assert hasattr(lib, nm)
 
 
