@@ -406,11 +406,13 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
             bc_ghost_state(p.bcs[b], 1, gs, u, nullptr, ug);
             for (int v = 0; v < 5; ++v) us[v] = 0.5 * (u[v] + ug[v]);
         }
-        const double sg = lr ? -1.0 : 1.0;
-        const double iw = sW[lr * N];
+        // g -= s (u* Ja - u Ja)[gd] / w_end, evaluated as (u* - u) (s Ja[gd] / w_end)
+        const double sw = (lr ? -1.0 : 1.0) * sW[lr * N];
+        double du[5];
+        for (int v = 0; v < 5; ++v) du[v] = us[v] - u[v];
         for (int gd = 0; gd < 3; ++gd) {
-            const double ja = sJa[(3 * sd + gd) * B + li];
-            for (int v = 0; v < 5; ++v) gr[gd][v] -= sg * (us[v] * ja - u[v] * ja) * iw;
+            const double jw = sJa[(3 * sd + gd) * B + li] * sw;
+            for (int v = 0; v < 5; ++v) gr[gd][v] -= du[v] * jw;
         }
     }
     const double iJ = 1.0 / detJ;
@@ -814,17 +816,18 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
             const int me = tt[md];
             const int l1 = tt[(md + 1) % 3], l2 = tt[(md + 2) % 3];
             const int pbase = lc * NITEM + (l1 + NP * l2) * NPAIR;
-            for (int m = 0; m <= N; ++m) {
+            {  // consistent diagonal term: F#(u,u; Ja) = f(u) . Ja
+                double dj[3], fi[5];
+                for (int q = 0; q < 3; ++q) dj[q] = ja_own[3 * md + q];
+                inv_flux(u, dj, gs.gma, fi);
+                const double d2 = 2.0 * sD[me * NP + me];
+                for (int v = 0; v < 5; ++v) ri[v] += d2 * fi[v];
+            }
+            for (int kk = 0; kk < N; ++kk) {  // the N other nodes of the line
+                const int m = kk < me ? kk : kk + 1;
                 const double d2 = 2.0 * sD[me * NP + m];
-                if (m == me) {  // consistent diagonal term: F#(u,u; Ja) = f(u) . Ja
-                    double dj[3], fi[5];
-                    for (int q = 0; q < 3; ++q) dj[q] = ja_own[3 * md + q];
-                    inv_flux(u, dj, gs.gma, fi);
-                    for (int v = 0; v < 5; ++v) ri[v] += d2 * fi[v];
-                } else {
-                    const int k = pbase + sPair[me * NP + m];
-                    for (int v = 0; v < 5; ++v) ri[v] += d2 * sF[v * BP + k];
-                }
+                const int k = pbase + sPair[me * NP + m];
+                for (int v = 0; v < 5; ++v) ri[v] += d2 * sF[v * BP + k];
             }
         }
     }
