@@ -7,6 +7,7 @@ the reference defaults (dgsem/plens.cc:420-515) unless a workload says otherwise
 """
 import ctypes as C
 import math
+import dataclasses
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -194,6 +195,16 @@ def wall_flow(degree=4, nx=128, ny=64, nz=2, seed=2, flux="HLLC", mach=6.0, T_in
             "ic")
     return Case("wall_flow", mesh, ph, bcs, Q, 0.4, steps,
                 {"seed": seed, "cells": (nx, ny, nz), "degree": degree, "u_inf": u_inf})
+
+
+def with_precision(case, precision):
+    """The same case run in another arithmetic variant: "fp64" (plain) or "fp32_bulk" (FP32 fluxes,
+    avars and face terms; FP64 states, residual registers, residual accumulation, metrics, blender,
+    time step and the near-equal log-mean branch)."""
+    codes = {"fp64": L.PREC_FP64, "fp32_bulk": L.PREC_FP32_BULK}
+    ph = L.Physics.from_buffer_copy(case.physics)
+    ph.precision = codes[precision] if isinstance(precision, str) else int(precision)
+    return dataclasses.replace(case, physics=ph, meta={**case.meta, "precision": precision})
 
 
 CONFIGS = {"periodic_cube": periodic_cube, "shock_tube": shock_tube, "wall_flow": wall_flow}

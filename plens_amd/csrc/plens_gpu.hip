@@ -77,6 +77,9 @@ struct KP {
     double* alpha_d;
     double* sinv;
     double* svis;
+    float* avars_f;  // FP32-bulk storage of avars and the face terms
+    float* sinv_f;
+    float* svis_f;
     double* Rnew;
     const double* R1;
     const double* R2;
@@ -100,6 +103,17 @@ struct KP {
     int want_dt;  // compute the stage-1 time step
     double c1, c2, c3;
 };
+
+// storage of the flux-arithmetic type: FP64 arrays for the plain path, FP32 for FP32-bulk
+template <typename T>
+__device__ __forceinline__ T* tsel(double* d, float* f);
+template <>
+__device__ __forceinline__ double* tsel<double>(double* d, float*) { return d; }
+template <>
+__device__ __forceinline__ float* tsel<float>(double*, float* f) { return f; }
+__device__ __forceinline__ double rcp64(double x);
+__device__ __forceinline__ double rcpt(double x) { return rcp64(x); }
+__device__ __forceinline__ float rcpt(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // Workgroups are dispatched round-robin over the 8 XCDs of MI355X (workgroup b on XCD b % 8), each
 // XCD with its own L2. This bijection of [0, nblk) gives XCD x one contiguous range of work, so face
@@ -215,17 +229,17 @@ __device__ __forceinline__ void load_basis(const Basis* b, double* sD, double* s
 // =============================================================================================
 // k_grad
 // =============================================================================================
-template <int N>
+template <int N, typename TF>
 __global__ void __launch_bounds__(256) k_grad(KP p) {
     constexpr int NP = N + 1, NN = NP * NP * NP;
     extern __shared__ double smem[];
     __shared__ double sD[NP * NP], sW[NP], sQm[NP * NP], sC[NP * NP];
     const int B = p.cpb * NN;
-    double* sQ = smem;
-    double* sJa = sQ + 5 * B;
-    double* sV = sJa + 9 * B;
+    double* sV = smem;
     double* sT = sV + B;   // Legendre modes of the blender variable
     double* sE = sT + B;   // 3B modal-energy partial sums
+    TF* sQ = reinterpret_cast<TF*>(sE + 3 * B);  // 5B states in the flux arithmetic
+    TF* sJa = sQ + 5 * B;                        // 9B contravariant vectors
     load_basis<N>(p.basis, sD, sW, sQm, sC);
 
     const int tid = threadIdx.x, lc = tid / NN, node = tid - lc * NN;
@@ -239,15 +253,15 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
     if (blockIdx.x == 0 && tid == 0) p.bcount[0] = 0;  // blended-cell list of this stage
     if (act) {
         load_state(p.Q, p.ndof, g, u);
-        for (int v = 0; v < 5; ++v) sQ[v * B + li] = u[v];
-        for (int a = 0; a < 9; ++a) sJa[a * B + li] = p.Ja[((long long)c * 9 + a) * NN + node];
+        for (int v = 0; v < 5; ++v) sQ[v * B + li] = (TF)u[v];
+        for (int a = 0; a < 9; ++a) sJa[a * B + li] = (TF)p.Ja[((long long)c * 9 + a) * NN + node];
         detJ = p.detJ[g];
         // PLENS::assert_positivity (dgsem/plens.cc:2531-2546)
         if (!(u[0] > 0.0)) atomicMin(p.fail, (unsigned long long)g * 4ull + 1ull);
         else if (!(get_e(u) > 0.0)) atomicMin(p.fail, (unsigned long long)g * 4ull + 2ull);
     } else if (inblk) {
-        for (int v = 0; v < 5; ++v) sQ[v * B + li] = u[v];
-        for (int a = 0; a < 9; ++a) sJa[a * B + li] = (a % 4 == 0) ? 1.0 : 0.0;
+        for (int v = 0; v < 5; ++v) sQ[v * B + li] = (TF)u[v];
+        for (int a = 0; a < 9; ++a) sJa[a * B + li] = (a % 4 == 0) ? TF(1) : TF(0);
     }
     const Gas& gs = p.gas;
     const double e = get_e(u);
@@ -348,7 +362,7 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
                 double madv = -1e300;
                 for (int r = 0; r < 3; ++r) {  // (J^-T v)_r with J^-1 rows = Ja^i / detJ
                     double s = 0;
-                    for (int cc = 0; cc < 3; ++cc) s += (sJa[(3 * cc + r) * B + li] / detJ) * vel[cc];
+                    for (int cc = 0; cc < 3; ++cc) s += (p.Ja[((long long)c * 9 + 3 * cc + r) * NN + node] / detJ) * vel[cc];
                     madv = fmax(madv, s);
                 }
                 double co = p.courant >= 1.0 ? 0.9 : p.courant;
@@ -371,9 +385,11 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
 
     // ---- BR1 conservative gradients (calc_cell_cons_grad, dgsem/plens.cc:2174-2285)
     if (!act) return;
-    double gr[3][5];
+    TF gr[3][5];
     for (int d = 0; d < 3; ++d)
-        for (int v = 0; v < 5; ++v) gr[d][v] = 0.0;
+        for (int v = 0; v < 5; ++v) gr[d][v] = TF(0);
+    TF ut[5];
+    for (int v = 0; v < 5; ++v) ut[v] = (TF)u[v];
     const int tt[3] = {ti, tj, tk};
 #pragma unroll 1
     for (int m = 0; m <= N; ++m) {
@@ -381,11 +397,11 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
             int to[3] = {ti, tj, tk};
             to[md] = m;
             const int jo = lc * NN + to[0] + NP * (to[1] + NP * to[2]);
-            double fl[5];
-            for (int v = 0; v < 5; ++v) fl[v] = 0.5 * (u[v] + sQ[v * B + jo]);
-            const double Dm2 = 2.0 * sD[tt[md] * NP + m];
+            TF fl[5];
+            for (int v = 0; v < 5; ++v) fl[v] = TF(0.5) * (ut[v] + sQ[v * B + jo]);
+            const TF Dm2 = (TF)(2.0 * sD[tt[md] * NP + m]);
             for (int gd = 0; gd < 3; ++gd) {
-                const double jav = 0.5 * (sJa[(3 * md + gd) * B + li] + sJa[(3 * md + gd) * B + jo]);
+                const TF jav = TF(0.5) * (sJa[(3 * md + gd) * B + li] + sJa[(3 * md + gd) * B + jo]);
                 for (int v = 0; v < 5; ++v) gr[gd][v] += Dm2 * fl[v] * jav;
             }
         }
@@ -407,38 +423,41 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
             for (int v = 0; v < 5; ++v) us[v] = 0.5 * (u[v] + ug[v]);
         }
         // g -= s (u* Ja - u Ja)[gd] / w_end, evaluated as (u* - u) (s Ja[gd] / w_end)
-        const double sw = (lr ? -1.0 : 1.0) * sW[lr * N];
-        double du[5];
-        for (int v = 0; v < 5; ++v) du[v] = us[v] - u[v];
+        const TF sw = (TF)((lr ? -1.0 : 1.0) * sW[lr * N]);
+        TF du[5];
+        for (int v = 0; v < 5; ++v) du[v] = (TF)(us[v] - u[v]);
         for (int gd = 0; gd < 3; ++gd) {
-            const double jw = sJa[(3 * sd + gd) * B + li] * sw;
+            const TF jw = sJa[(3 * sd + gd) * B + li] * sw;
             for (int v = 0; v < 5; ++v) gr[gd][v] -= du[v] * jw;
         }
     }
-    const double iJ = 1.0 / detJ;
+    const TF iJ = (TF)(1.0 / detJ);
     for (int gd = 0; gd < 3; ++gd)
         for (int v = 0; v < 5; ++v) gr[gd][v] *= iJ;
 
     // ---- auxiliary variables (calc_aux_vars, dgsem/plens.cc:2656-2746)
-    const double vel[3] = {u[1] / u[0], u[2] / u[0], u[3] / u[0]};
-    double vg[3][3];
+    const TF vel[3] = {(TF)(u[1] / u[0]), (TF)(u[2] / u[0]), (TF)(u[3] / u[0])};
+    TF vg[3][3];
     for (int vd = 0; vd < 3; ++vd)
-        for (int gd = 0; gd < 3; ++gd) vg[vd][gd] = (gr[gd][1 + vd] - gr[gd][0] * vel[vd]) / u[0];
-    const double tr = vg[0][0] + vg[1][1] + vg[2][2];
-    double av[9];
-    av[0] = vg[0][0] + vg[0][0] - 2.0 / 3 * tr;
+        for (int gd = 0; gd < 3; ++gd) vg[vd][gd] = (gr[gd][1 + vd] - gr[gd][0] * vel[vd]) / ut[0];
+    const TF tr = vg[0][0] + vg[1][1] + vg[2][2];
+    const TF c23 = (TF)(2.0 / 3);
+    TF av[9];
+    av[0] = vg[0][0] + vg[0][0] - c23 * tr;
     av[1] = vg[0][1] + vg[1][0];
     av[2] = vg[0][2] + vg[2][0];
-    av[3] = vg[1][1] + vg[1][1] - 2.0 / 3 * tr;
+    av[3] = vg[1][1] + vg[1][1] - c23 * tr;
     av[4] = vg[1][2] + vg[2][1];
-    av[5] = vg[2][2] + vg[2][2] - 2.0 / 3 * tr;
+    av[5] = vg[2][2] + vg[2][2] - c23 * tr;
+    const TF et = (TF)e;
     for (int gd = 0; gd < 3; ++gd) {
-        double eg = (gr[gd][4] - gr[gd][0] * e) / u[0];
-        for (int vd = 0; vd < 3; ++vd) eg -= (0.5 * gr[gd][0] * vel[vd] * vel[vd] + u[1 + vd] * vg[vd][gd]) / u[0];
-        av[6 + gd] = -eg / gs.cv;
+        TF eg = (gr[gd][4] - gr[gd][0] * et) / ut[0];
+        for (int vd = 0; vd < 3; ++vd) eg -= (TF(0.5) * gr[gd][0] * vel[vd] * vel[vd] + ut[1 + vd] * vg[vd][gd]) / ut[0];
+        av[6 + gd] = -eg / (TF)gs.cv;
     }
-    for (int a = 0; a < 6; ++a) p.avars[a * p.ndof + g] = av[a] * mu;
-    for (int a = 6; a < 9; ++a) p.avars[a * p.ndof + g] = av[a] * kap;
+    TF* avo = tsel<TF>(p.avars, p.avars_f);
+    for (int a = 0; a < 6; ++a) avo[a * p.ndof + g] = av[a] * (TF)mu;
+    for (int a = 6; a < 9; ++a) avo[a * p.ndof + g] = av[a] * (TF)kap;
 }
 
 // =============================================================================================
@@ -481,7 +500,7 @@ __global__ void __launch_bounds__(kThreadsAlpha) k_alpha(KP p) {
 // =============================================================================================
 // k_face
 // =============================================================================================
-template <int N>
+template <int N, typename TF>
 __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
     const long long gid = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -509,69 +528,80 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
         bl = p.alpha[c];
     }
     bl = bl <= 0.0 ? 0.0 : (bl <= 1.0 ? bl : 1.0);  // set_flux_blender_value
-    double F[5];
-    inv_surf_flux(p.scheme, bl, u, ug, n, gs.gma, F);
+    TF ut[5], ugt[5], nt[3];
+    for (int v = 0; v < 5; ++v) {
+        ut[v] = (TF)u[v];
+        ugt[v] = (TF)ug[v];
+    }
+    for (int d = 0; d < 3; ++d) nt[d] = (TF)n[d];
+    const TF gma = (TF)gs.gma;
+    TF F[5];
+    inv_surf_flux(p.scheme, (TF)bl, ut, ugt, nt, gma, F);
     // Each side stores its finished surface term of calc_cell_ho_inv/dif_residual
     // (dgsem/plens.cc:3237-3268, 3373-3406): -s (-s F*_side - f(u; Ja/|Ja|)) |Ja| / w_end, with
     // F*_side the outward flux of that side (-F for the neighbour of an interior face) and
     // s = +1 / -1 for left / right faces.
-    double dirc[3], nwc, dirn[3] = {0, 0, 0}, nwn = 0;
+    TF dirc[3], nwc, dirn[3] = {TF(0), TF(0), TF(0)}, nwn = TF(0);
     {
-        double ja[3];
-        for (int a = 0; a < 3; ++a) ja[a] = p.Ja[((long long)c * 9 + 3 * sd + a) * NN + (g - (long long)c * NN)];
-        const double nr = sqrt(ja[0] * ja[0] + ja[1] * ja[1] + ja[2] * ja[2]);
-        const double inr = 1.0 / nr;
+        TF ja[3];
+        for (int a = 0; a < 3; ++a) ja[a] = (TF)p.Ja[((long long)c * 9 + 3 * sd + a) * NN + (g - (long long)c * NN)];
+        const TF nr = sq(ja[0] * ja[0] + ja[1] * ja[1] + ja[2] * ja[2]);
+        const TF inr = dv(TF(1), nr);
         for (int a = 0; a < 3; ++a) dirc[a] = ja[a] * inr;
-        nwc = nr * ((f & 1) ? p.iwN : p.iw0);
+        nwc = nr * (TF)((f & 1) ? p.iwN : p.iw0);
     }
     if (interior) {
-        double ja[3];
-        for (int a = 0; a < 3; ++a) ja[a] = p.Ja[((long long)nb * 9 + 3 * sd + a) * NN + (gn - (long long)nb * NN)];
-        const double nr = sqrt(ja[0] * ja[0] + ja[1] * ja[1] + ja[2] * ja[2]);
-        const double inr = 1.0 / nr;
+        TF ja[3];
+        for (int a = 0; a < 3; ++a) ja[a] = (TF)p.Ja[((long long)nb * 9 + 3 * sd + a) * NN + (gn - (long long)nb * NN)];
+        const TF nr = sq(ja[0] * ja[0] + ja[1] * ja[1] + ja[2] * ja[2]);
+        const TF inr = dv(TF(1), nr);
         for (int a = 0; a < 3; ++a) dirn[a] = ja[a] * inr;
-        nwn = nr * ((f & 1) ? p.iw0 : p.iwN);
+        nwn = nr * (TF)((f & 1) ? p.iw0 : p.iwN);
     }
-    const double sgc = (f & 1) ? -1.0 : 1.0, sgn = -sgc;
+    const TF sgc = (f & 1) ? TF(-1) : TF(1), sgn = -sgc;
     const long long o = (long long)cf * 5 * NF + q;
     const long long on = ((long long)nb * 6 + (f ^ 1)) * 5 * NF + q;
+    TF* sinv = tsel<TF>(p.sinv, p.sinv_f);
     {
-        double fi[5];
-        inv_flux(u, dirc, gs.gma, fi);
-        for (int v = 0; v < 5; ++v) p.sinv[o + v * NF] = -sgc * (-sgc * F[v] - fi[v]) * nwc;
+        TF fi[5];
+        inv_flux(ut, dirc, gma, fi);
+        for (int v = 0; v < 5; ++v) sinv[o + v * NF] = -sgc * (-sgc * F[v] - fi[v]) * nwc;
         if (interior) {
-            inv_flux(ug, dirn, gs.gma, fi);
-            for (int v = 0; v < 5; ++v) p.sinv[on + v * NF] = -sgn * (-sgn * (-F[v]) - fi[v]) * nwn;
+            inv_flux(ugt, dirn, gma, fi);
+            for (int v = 0; v < 5; ++v) sinv[on + v * NF] = -sgn * (-sgn * (-F[v]) - fi[v]) * nwn;
         }
     }
     if (gs.inviscid) return;
     // stage 3: BR1 average of the viscous fluxes (navier_stokes.cc:1644-1652)
-    double av[9], avg[9];
-    for (int a = 0; a < 9; ++a) av[a] = p.avars[a * p.ndof + g];
+    const TF* avars = tsel<TF>(p.avars, p.avars_f);
+    TF* svis = tsel<TF>(p.svis, p.svis_f);
+    TF av[9], avg[9];
+    for (int a = 0; a < 9; ++a) av[a] = avars[a * p.ndof + g];
     if (interior || periodic) {
-        for (int a = 0; a < 9; ++a) avg[a] = p.avars[a * p.ndof + gn];
+        for (int a = 0; a < 9; ++a) avg[a] = avars[a * p.ndof + gn];
     } else {
         for (int a = 0; a < 9; ++a) avg[a] = av[a];
         if (p.bcs[b].type != PLENS_BC_OUTFLOW) bc_ghost_state(p.bcs[b], 3, gs, u, n, ug);
         else for (int v = 0; v < 5; ++v) ug[v] = u[v];
+        for (int v = 0; v < 5; ++v) ugt[v] = (TF)ug[v];
     }
-    double f1[5], f2[5];
-    dif_flux(u, av, n, f1);
-    dif_flux(ug, avg, n, f2);
-    for (int v = 0; v < 5; ++v) F[v] = 0.5 * (f1[v] + f2[v]);
-    double fi[5];
-    dif_flux(u, av, dirc, fi);
-    for (int v = 0; v < 5; ++v) p.svis[o + v * NF] = -sgc * (-sgc * F[v] - fi[v]) * nwc;
+    TF f1[5], f2[5];
+    dif_flux(ut, av, nt, f1);
+    dif_flux(ugt, avg, nt, f2);
+    for (int v = 0; v < 5; ++v) F[v] = TF(0.5) * (f1[v] + f2[v]);
+    TF fi[5];
+    dif_flux(ut, av, dirc, fi);
+    for (int v = 0; v < 5; ++v) svis[o + v * NF] = -sgc * (-sgc * F[v] - fi[v]) * nwc;
     if (interior) {
-        dif_flux(ug, avg, dirn, fi);
-        for (int v = 0; v < 5; ++v) p.svis[on + v * NF] = -sgn * (-sgn * (-F[v]) - fi[v]) * nwn;
+        dif_flux(ugt, avg, dirn, fi);
+        for (int v = 0; v < 5; ++v) svis[on + v * NF] = -sgn * (-sgn * (-F[v]) - fi[v]) * nwn;
     }
 }
 
 // =============================================================================================
 // k_lo: first-order subcell FV residual of blended cells (calc_cell_lo_inv_residual, 3469-3583)
 // =============================================================================================
-template <int N>
+template <int N, typename TF>
 __global__ void __launch_bounds__(256) k_lo(KP p) {
     constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
     extern __shared__ double smem[];
@@ -579,9 +609,11 @@ __global__ void __launch_bounds__(256) k_lo(KP p) {
     const int count = p.bcount[0];  // blended cells of this stage, listed by k_alpha
     if ((int)blockIdx.x * p.cpb >= count) return;
     const int B = p.cpb * NN;
-    double* sQ = smem;          // 5B
-    double* sJa = sQ + 5 * B;   // 9B
-    double* sX = sJa + 9 * B;   // 5B
+    TF* sQ = reinterpret_cast<TF*>(smem);  // 5B
+    TF* sJa = sQ + 5 * B;                  // 9B
+    TF* sX = sJa + 9 * B;                  // 5B
+    const TF gma = (TF)p.gas.gma;
+    const TF* sinv = tsel<TF>(p.sinv, p.sinv_f);
     load_basis<N>(p.basis, sD, sW, sQm, sC);
     const int tid = threadIdx.x, lc = tid / NN, node = tid - lc * NN;
     const bool inblk = lc < p.cpb;
@@ -595,17 +627,19 @@ __global__ void __launch_bounds__(256) k_lo(KP p) {
         const int c = act ? p.blist[idx] : 0;
         const long long g = (long long)c * NN + node;
         const double alpha = act ? p.alpha[c] : 0.0;
-        const double bl = alpha <= 0.0 ? 0.0 : (alpha <= 1.0 ? alpha : 1.0);
-        double u[5] = {1, 0, 0, 0, 1}, detJ = 1;
+        const TF bl = (TF)(alpha <= 0.0 ? 0.0 : (alpha <= 1.0 ? alpha : 1.0));
+        double u0[5] = {1, 0, 0, 0, 1}, detJ = 1;
         if (act) {
-            load_state(p.Q, p.ndof, g, u);
+            load_state(p.Q, p.ndof, g, u0);
             detJ = p.detJ[g];
         }
+        TF u[5];
+        for (int v = 0; v < 5; ++v) u[v] = (TF)u0[v];
         __syncthreads();  // previous iteration done with LDS
         if (inblk) {
             for (int v = 0; v < 5; ++v) sQ[v * B + li] = u[v];
             for (int a = 0; a < 9; ++a)
-                sJa[a * B + li] = act ? p.Ja[((long long)c * 9 + a) * NN + node] : ((a % 4 == 0) ? 1.0 : 0.0);
+                sJa[a * B + li] = act ? (TF)p.Ja[((long long)c * 9 + a) * NN + node] : ((a % 4 == 0) ? TF(1) : TF(0));
         }
         double rl[5] = {0, 0, 0, 0, 0};
         for (int dir = 0; dir < 3; ++dir) {
@@ -616,23 +650,23 @@ __global__ void __launch_bounds__(256) k_lo(KP p) {
                 int t0[3] = {ti, tj, tk};
                 t0[dir] = 0;
                 const int j0 = lc * NN + t0[0] + NP * (t0[1] + NP * t0[2]);
-                double sn[3];
+                TF sn[3];
                 for (int a = 0; a < 3; ++a) sn[a] = sJa[(3 * dir + a) * B + j0];
                 for (int l = 0; l < id; ++l)
                     for (int m = 0; m <= N; ++m) {
                         int tm[3] = {ti, tj, tk};
                         tm[dir] = m;
                         const int jm = lc * NN + tm[0] + NP * (tm[1] + NP * tm[2]);
-                        for (int a = 0; a < 3; ++a) sn[a] += sQm[l * NP + m] * sJa[(3 * dir + a) * B + jm];
+                        for (int a = 0; a < 3; ++a) sn[a] += (TF)sQm[l * NP + m] * sJa[(3 * dir + a) * B + jm];
                     }
-                const double nn = sqrt(sn[0] * sn[0] + sn[1] * sn[1] + sn[2] * sn[2]);
-                for (int a = 0; a < 3; ++a) sn[a] /= nn;
+                const TF nn = sq(sn[0] * sn[0] + sn[1] * sn[1] + sn[2] * sn[2]);
+                for (int a = 0; a < 3; ++a) sn[a] = dv(sn[a], nn);
                 int tr[3] = {ti, tj, tk};
                 tr[dir] += 1;
                 const int jr = lc * NN + tr[0] + NP * (tr[1] + NP * tr[2]);
-                double ur[5], fl[5];
+                TF ur[5], fl[5];
                 for (int v = 0; v < 5; ++v) ur[v] = sQ[v * B + jr];
-                inv_surf_flux(p.scheme, bl, u, ur, sn, gs.gma, fl);
+                inv_surf_flux(p.scheme, bl, u, ur, sn, gma, fl);
                 for (int v = 0; v < 5; ++v) sX[v * B + li] = fl[v] * nn;
             }
             __syncthreads();
@@ -641,10 +675,10 @@ __global__ void __launch_bounds__(256) k_lo(KP p) {
                     int tl_[3] = {ti, tj, tk};
                     tl_[dir] -= 1;
                     const int jl = lc * NN + tl_[0] + NP * (tl_[1] + NP * tl_[2]);
-                    for (int v = 0; v < 5; ++v) rl[v] -= sX[v * B + jl] * sW[tt[dir]];
+                    for (int v = 0; v < 5; ++v) rl[v] -= (double)sX[v * B + jl] * sW[tt[dir]];
                 }
                 if (tt[dir] < N)
-                    for (int v = 0; v < 5; ++v) rl[v] += sX[v * B + li] * sW[tt[dir]];
+                    for (int v = 0; v < 5; ++v) rl[v] += (double)sX[v * B + li] * sW[tt[dir]];
             }
         }
         if (act) {
@@ -654,17 +688,17 @@ __global__ void __launch_bounds__(256) k_lo(KP p) {
                 if (!on_face(f, ti, tj, tk, N)) continue;
                 const int q = face_dof<N>(f, ti, tj, tk);
                 const int sd = f >> 1, lr = f & 1;
-                const double sg = lr ? -1.0 : 1.0;
-                double dir[3];
+                const TF sg = lr ? TF(-1) : TF(1);
+                TF dir[3];
                 for (int a = 0; a < 3; ++a) dir[a] = sJa[(3 * sd + a) * B + li];
-                const double nn = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
-                const double inn = 1.0 / nn;
+                const TF nn = sq(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+                const TF inn = dv(TF(1), nn);
                 for (int a = 0; a < 3; ++a) dir[a] *= inn;
-                const double nw = nn * sW[lr * N];
-                double fi[5];
-                inv_flux(u, dir, gs.gma, fi);
+                const TF nw = nn * (TF)sW[lr * N];
+                TF fi[5];
+                inv_flux(u, dir, gma, fi);
                 const long long o = ((long long)c * 6 + f) * 5 * NF + q;
-                for (int v = 0; v < 5; ++v) rl[v] += p.sinv[o + v * NF] - sg * fi[v] * nw;
+                for (int v = 0; v < 5; ++v) rl[v] += (double)(sinv[o + v * NF] - sg * fi[v] * nw);
             }
             const double imJ = 1.0 / -detJ;
             for (int v = 0; v < 5; ++v) p.rlo[v * p.ndof + g] = rl[v] * imJ;
@@ -695,7 +729,7 @@ __device__ __forceinline__ double ln_mean(double x, double y, double lx, double 
 // =============================================================================================
 // k_vol: high-order inviscid + viscous residuals, blend, RK epilogue
 // =============================================================================================
-template <int N>
+template <int N, typename TF>
 __global__ void __launch_bounds__(256) k_vol(KP p) {
     constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
     constexpr int NPAIR = NP * (NP - 1) / 2;       // unordered pairs on one line
@@ -706,9 +740,10 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
     __shared__ int sPA[NPAIR > 0 ? NPAIR : 1], sPB[NPAIR > 0 ? NPAIR : 1];
     const int B = p.cpb * NN;
     const int BP = p.cpb * NITEM;
-    double* sA = smem;            // 8B: rho, beta, ln rho, ln beta, v0, v1, v2, |v|^2
-    double* sJa = sA + 8 * B;     // 3B: Ja^md of the direction being processed
-    double* sF = sJa + 3 * B;     // 5*max(BP,B) pair fluxes / viscous scratch
+    double* sR = smem;                               // 2B: rho, beta (FP64 in every variant)
+    TF* sA = reinterpret_cast<TF*>(sR + 2 * B);      // 6B: ln rho, ln beta, v0, v1, v2, |v|^2
+    TF* sJa = sA + 6 * B;                            // 3B: Ja^md of the direction being processed
+    TF* sF = sJa + 3 * B;                            // 5*max(BP,B) pair fluxes / viscous scratch
     load_basis<N>(p.basis, sD, sW, sQm, sC);
     for (int t = threadIdx.x; t < NP * NP; t += blockDim.x) {
         const int a = t / NP, b = t % NP;
@@ -736,20 +771,22 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
         const double pr = get_p(u, gs.gma);
         const double beta = 0.5 * u[0] / pr;
         const double v0 = u[1] / u[0], v1 = u[2] / u[0], v2 = u[3] / u[0];
-        sA[0 * B + li] = u[0];
-        sA[1 * B + li] = beta;
-        sA[2 * B + li] = log(u[0]);
-        sA[3 * B + li] = log(beta);
-        sA[4 * B + li] = v0;
-        sA[5 * B + li] = v1;
-        sA[6 * B + li] = v2;
-        sA[7 * B + li] = v0 * v0 + v1 * v1 + v2 * v2;
+        sR[0 * B + li] = u[0];
+        sR[1 * B + li] = beta;
+        sA[0 * B + li] = lg((TF)u[0]);
+        sA[1 * B + li] = lg((TF)beta);
+        sA[2 * B + li] = (TF)v0;
+        sA[3 * B + li] = (TF)v1;
+        sA[4 * B + li] = (TF)v2;
+        sA[5 * B + li] = (TF)(v0 * v0 + v1 * v1 + v2 * v2);
     }
-    double ja_own[9];
-    for (int a = 0; a < 9; ++a) ja_own[a] = act ? p.Ja[((long long)c * 9 + a) * NN + node] : ((a % 4 == 0) ? 1.0 : 0.0);
+    TF ja_own[9], ut[5];
+    for (int a = 0; a < 9; ++a) ja_own[a] = act ? (TF)p.Ja[((long long)c * 9 + a) * NN + node] : ((a % 4 == 0) ? TF(1) : TF(0));
+    for (int v = 0; v < 5; ++v) ut[v] = (TF)u[v];
     const double alpha = act ? p.alpha[c] : 0.0;
     const double alpha_d = act ? p.alpha_d[c] : 0.0;
-    const double half_gm1_inv = 0.5 / (gs.gma - 1.0);
+    const TF half_gm1_inv = (TF)(0.5 / (gs.gma - 1.0));
+    const TF gma = (TF)gs.gma;
 
     // ---- high-order inviscid volume term, each unordered pair once (calc_cell_ho_inv_residual,
     //      dgsem/plens.cc:3202-3234): r_i += 2 D_ij F#(u_i, u_j; (Ja_i+Ja_j)/2), F#_ji = F#_ij
@@ -771,40 +808,40 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
             ia[(md + 2) % 3] = l2; ib[(md + 2) % 3] = l2;
             const int ja = cl * NN + ia[0] + NP * (ia[1] + NP * ia[2]);
             const int jb = cl * NN + ib[0] + NP * (ib[1] + NP * ib[2]);
-            double dir[3];
-            for (int q = 0; q < 3; ++q) dir[q] = 0.5 * (sJa[q * B + ja] + sJa[q * B + jb]);
-            // log means with shared reciprocals: f = (x-y)/(x+y); series below f^2 = 1e-2, else
-            // (x-y)/(ln x - ln y) with the per-node logarithms
-            const double ra = sA[ja], rb = sA[jb];
-            const double ba = sA[B + ja], bb = sA[B + jb];
+            TF dir[3];
+            for (int q = 0; q < 3; ++q) dir[q] = TF(0.5) * (sJa[q * B + ja] + sJa[q * B + jb]);
+            // log means with shared reciprocals: f = (x-y)/(x+y); series below f^2 = 1e-2 (sums,
+            // differences, branch and series in FP64 in every variant), else (x-y)/(ln x - ln y)
+            // with the per-node logarithms
+            const double ra = sR[ja], rb = sR[jb];
+            const double ba = sR[B + ja], bb = sR[B + jb];
             const double sr = ra + rb, dr = ra - rb, sb = ba + bb, db = ba - bb;
             const double irs = rcp64(sr), ibs = rcp64(sb);
             const double fr = dr * irs, fb = db * ibs;
             const double ur = fr * fr, ub = fb * fb;
-            double rho_ln, inv_rho_ln, inv_b_ln;
+            TF rho_ln, inv_rho_ln, inv_b_ln;
             if (ur < 1e-2) {
                 const double F = 1.0 + ur * (1.0 / 3.0 + ur * (1.0 / 5.0 + ur * (1.0 / 7.0)));
-                inv_rho_ln = 2.0 * F * irs;
-                rho_ln = 0.5 * sr * rcp64(F);
+                inv_rho_ln = (TF)(2.0 * F * irs);
+                rho_ln = (TF)(0.5 * sr * rcp64(F));
             } else {
-                const double dl = sA[2 * B + ja] - sA[2 * B + jb];
-                const double idr = rcp64(dr);
-                inv_rho_ln = dl * idr;
-                rho_ln = dr * rcp64(dl);
+                const TF dl = sA[ja] - sA[jb];
+                inv_rho_ln = dl * rcpt((TF)dr);
+                rho_ln = (TF)dr * rcpt(dl);
             }
             if (ub < 1e-2) {
                 const double F = 1.0 + ub * (1.0 / 3.0 + ub * (1.0 / 5.0 + ub * (1.0 / 7.0)));
-                inv_b_ln = 2.0 * F * ibs;
+                inv_b_ln = (TF)(2.0 * F * ibs);
             } else {
-                inv_b_ln = (sA[3 * B + ja] - sA[3 * B + jb]) * rcp64(db);
+                inv_b_ln = (sA[B + ja] - sA[B + jb]) * rcpt((TF)db);
             }
-            const double p_hat = 0.5 * sr * ibs;
-            const double va0 = 0.5 * (sA[4 * B + ja] + sA[4 * B + jb]);
-            const double va1 = 0.5 * (sA[5 * B + ja] + sA[5 * B + jb]);
-            const double va2 = 0.5 * (sA[6 * B + ja] + sA[6 * B + jb]);
-            const double H = half_gm1_inv * inv_b_ln + p_hat * inv_rho_ln + (va0 * va0 + va1 * va1 + va2 * va2) -
-                             0.25 * (sA[7 * B + ja] + sA[7 * B + jb]);
-            const double rv = rho_ln * (va0 * dir[0] + va1 * dir[1] + va2 * dir[2]);
+            const TF p_hat = (TF)(0.5 * sr * ibs);
+            const TF va0 = TF(0.5) * (sA[2 * B + ja] + sA[2 * B + jb]);
+            const TF va1 = TF(0.5) * (sA[3 * B + ja] + sA[3 * B + jb]);
+            const TF va2 = TF(0.5) * (sA[4 * B + ja] + sA[4 * B + jb]);
+            const TF H = half_gm1_inv * inv_b_ln + p_hat * inv_rho_ln + (va0 * va0 + va1 * va1 + va2 * va2) -
+                         TF(0.25) * (sA[5 * B + ja] + sA[5 * B + jb]);
+            const TF rv = rho_ln * (va0 * dir[0] + va1 * dir[1] + va2 * dir[2]);
             sF[0 * BP + it] = rv;
             sF[1 * BP + it] = rv * va0 + p_hat * dir[0];
             sF[2 * BP + it] = rv * va1 + p_hat * dir[1];
@@ -817,34 +854,35 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
             const int l1 = tt[(md + 1) % 3], l2 = tt[(md + 2) % 3];
             const int pbase = lc * NITEM + (l1 + NP * l2) * NPAIR;
             {  // consistent diagonal term: F#(u,u; Ja) = f(u) . Ja
-                double dj[3], fi[5];
+                TF dj[3], fi[5];
                 for (int q = 0; q < 3; ++q) dj[q] = ja_own[3 * md + q];
-                inv_flux(u, dj, gs.gma, fi);
+                inv_flux(ut, dj, gma, fi);
                 const double d2 = 2.0 * sD[me * NP + me];
-                for (int v = 0; v < 5; ++v) ri[v] += d2 * fi[v];
+                for (int v = 0; v < 5; ++v) ri[v] += d2 * (double)fi[v];
             }
             for (int kk = 0; kk < N; ++kk) {  // the N other nodes of the line
                 const int m = kk < me ? kk : kk + 1;
                 const double d2 = 2.0 * sD[me * NP + m];
                 const int k = pbase + sPair[me * NP + m];
-                for (int v = 0; v < 5; ++v) ri[v] += d2 * sF[v * BP + k];
+                for (int v = 0; v < 5; ++v) ri[v] += d2 * (double)sF[v * BP + k];
             }
         }
     }
 
     // ---- high-order viscous volume term (calc_cell_ho_dif_residual, dgsem/plens.cc:3283-3413)
     double rd[5] = {0, 0, 0, 0, 0};
-    double av[9];
+    TF av[9];
     if (!gs.inviscid) {
-        for (int a = 0; a < 9; ++a) av[a] = act ? p.avars[a * p.ndof + g] : 0.0;
+        const TF* avars = tsel<TF>(p.avars, p.avars_f);
+        for (int a = 0; a < 9; ++a) av[a] = act ? avars[a * p.ndof + g] : TF(0);
 #pragma unroll
         for (int md = 0; md < 3; ++md) {
             __syncthreads();
             if (inblk) {
-                double dir[3];
+                TF dir[3];
                 for (int a = 0; a < 3; ++a) dir[a] = ja_own[3 * md + a];
-                double fl[5];
-                dif_flux(u, av, dir, fl);  // linear in dir: |Ja| F_v(Ja/|Ja|)
+                TF fl[5];
+                dif_flux(ut, av, dir, fl);  // linear in dir: |Ja| F_v(Ja/|Ja|)
                 for (int v = 1; v < 5; ++v) sF[v * B + li] = fl[v];
             }
             __syncthreads();
@@ -854,7 +892,7 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
                     to[md] = m;
                     const int jo = lc * NN + to[0] + NP * (to[1] + NP * to[2]);
                     const double Dv = sD[tt[md] * NP + m];
-                    for (int v = 1; v < 5; ++v) rd[v] += Dv * sF[v * B + jo];
+                    for (int v = 1; v < 5; ++v) rd[v] += Dv * (double)sF[v * B + jo];
                 }
             }
         }
@@ -862,12 +900,14 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
     if (!act) return;
 
     // ---- surface terms of both high-order residuals, finished per face node by k_face
+    const TF* sinv = tsel<TF>(p.sinv, p.sinv_f);
+    const TF* svis = tsel<TF>(p.svis, p.svis_f);
     for (int f = 0; f < 6; ++f) {
         if (!on_face(f, ti, tj, tk, N)) continue;
         const long long o = ((long long)c * 6 + f) * 5 * NF + face_dof<N>(f, ti, tj, tk);
-        for (int v = 0; v < 5; ++v) ri[v] += p.sinv[o + v * NF];
+        for (int v = 0; v < 5; ++v) ri[v] += (double)sinv[o + v * NF];
         if (!gs.inviscid)
-            for (int v = 0; v < 5; ++v) rd[v] += p.svis[o + v * NF];
+            for (int v = 0; v < 5; ++v) rd[v] += (double)svis[o + v * NF];
     }
     {
         const double jm = 1.0 / (-1.0 * detJ);
@@ -932,6 +972,8 @@ struct plens_gpu {
     Basis* basis = nullptr;
     double *avars = nullptr, *araw = nullptr, *alpha = nullptr, *alpha_d = nullptr;
     double *sinv = nullptr, *svis = nullptr, *rhs = nullptr, *rlo = nullptr;
+    float *avars_f = nullptr, *sinv_f = nullptr, *svis_f = nullptr;  // FP32-bulk variant
+    int precision = PLENS_PREC_FP64;
     int *blist = nullptr, *bcount = nullptr;
     unsigned long long* mlp_stats = nullptr;
     float* feat = nullptr;
@@ -983,17 +1025,48 @@ const double kRkAo[5] = {4745337637855.0 / 22386579876409, 6808157035527.0 / 131
 const double kRkAi[4] = {546509042554.0 / 9152262712923, 625707605167.0 / 5316659119056,
                          582400652113.0 / 7078426004906, 5010207514426.0 / 21876007855139};
 
-template <int N>
+// dynamic LDS of k_grad: FP64 blender/modal arrays (5B) + flux-type states and metrics (14B)
+template <typename TF>
+size_t lds_grad_bytes(int cpb, int NN) {
+    return sizeof(double) * 5 * (size_t)cpb * NN + sizeof(TF) * 14 * (size_t)cpb * NN;
+}
+
+template <typename TF>
+int launch_grad_p(plens_gpu* h, const KP& kp) {
+    const int thr = threads_for(kp.cpb, h->NN);
+    const int blocks = (kp.nc + kp.cpb - 1) / kp.cpb;
+    const size_t lds = lds_grad_bytes<TF>(kp.cpb, h->NN);
+    switch (h->N) {
+        case 1: hipLaunchKernelGGL((k_grad<1, TF>), dim3(blocks), dim3(thr), lds, h->stream, kp); break;
+        case 2: hipLaunchKernelGGL((k_grad<2, TF>), dim3(blocks), dim3(thr), lds, h->stream, kp); break;
+        case 3: hipLaunchKernelGGL((k_grad<3, TF>), dim3(blocks), dim3(thr), lds, h->stream, kp); break;
+        case 4: hipLaunchKernelGGL((k_grad<4, TF>), dim3(blocks), dim3(thr), lds, h->stream, kp); break;
+        default: hipLaunchKernelGGL((k_grad<5, TF>), dim3(blocks), dim3(thr), lds, h->stream, kp); break;
+    }
+    HIPCHK(hipGetLastError());
+    return PLENS_OK;
+}
+
+// k_grad alone (time-step candidates, blender indicator, avars) for the precision of the handle
+int launch_grad(plens_gpu* h, const KP& kp) {
+    return h->precision == PLENS_PREC_FP32_BULK ? launch_grad_p<float>(h, kp) : launch_grad_p<double>(h, kp);
+}
+
+template <int N, typename TF>
 int launch_stage_t(plens_gpu* h, const KP& kp, hipEvent_t* ev) {
     constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
     const int thr = threads_for(kp.cpb, NN);
     const int blocks = (kp.nc + kp.cpb - 1) / kp.cpb;
-    const size_t lds_grad = sizeof(double) * 19 * kp.cpb * NN;
-    const size_t lds_lo = sizeof(double) * 19 * kp.cpb * NN;
+    const size_t B = (size_t)kp.cpb * NN;
+    // dynamic LDS: FP64 blender/modal arrays + flux-type states and metrics (k_grad); flux-type
+    // states, metrics and subcell fluxes (k_lo); FP64 rho/beta + flux-type node data, metrics and
+    // pair fluxes (k_vol)
+    const size_t lds_grad = lds_grad_bytes<TF>(kp.cpb, NN);
+    const size_t lds_lo = sizeof(TF) * 19 * B;
     const size_t nitem = (size_t)NP * NP * (NP * (NP - 1) / 2);
-    const size_t lds_vol = sizeof(double) * (11 * kp.cpb * NN + 5 * std::max((size_t)kp.cpb * nitem, (size_t)kp.cpb * NN));
+    const size_t lds_vol = sizeof(double) * 2 * B + sizeof(TF) * (9 * B + 5 * std::max((size_t)kp.cpb * nitem, B));
     if (ev) HIPCHK(hipEventRecord(ev[0], h->stream));
-    hipLaunchKernelGGL(k_grad<N>, dim3(blocks), dim3(thr), lds_grad, h->stream, kp);
+    hipLaunchKernelGGL((k_grad<N, TF>), dim3(blocks), dim3(thr), lds_grad, h->stream, kp);
     HIPCHK(hipGetLastError());
     if (ev) HIPCHK(hipEventRecord(ev[1], h->stream));
     hipLaunchKernelGGL(k_alpha, dim3((kp.nc + kThreadsAlpha - 1) / kThreadsAlpha), dim3(kThreadsAlpha), 0,
@@ -1001,28 +1074,33 @@ int launch_stage_t(plens_gpu* h, const KP& kp, hipEvent_t* ev) {
     HIPCHK(hipGetLastError());
     if (ev) HIPCHK(hipEventRecord(ev[2], h->stream));
     const long long fthreads = (long long)kp.ntasks * NF;
-    hipLaunchKernelGGL(k_face<N>, dim3((unsigned)((fthreads + kThreadsFace - 1) / kThreadsFace)),
+    hipLaunchKernelGGL((k_face<N, TF>), dim3((unsigned)((fthreads + kThreadsFace - 1) / kThreadsFace)),
                        dim3(kThreadsFace), 0, h->stream, kp);
     HIPCHK(hipGetLastError());
     if (ev) HIPCHK(hipEventRecord(ev[3], h->stream));
-    hipLaunchKernelGGL(k_lo<N>, dim3(std::min(blocks, 256)), dim3(thr), lds_lo, h->stream, kp);
+    hipLaunchKernelGGL((k_lo<N, TF>), dim3(std::min(blocks, 256)), dim3(thr), lds_lo, h->stream, kp);
     HIPCHK(hipGetLastError());
     if (ev) HIPCHK(hipEventRecord(ev[4], h->stream));
-    hipLaunchKernelGGL(k_vol<N>, dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
+    hipLaunchKernelGGL((k_vol<N, TF>), dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
     HIPCHK(hipGetLastError());
     if (ev) HIPCHK(hipEventRecord(ev[5], h->stream));
     return PLENS_OK;
 }
 
-int launch_stage(plens_gpu* h, const KP& kp, hipEvent_t* ev = nullptr) {
+template <typename TF>
+int launch_stage_p(plens_gpu* h, const KP& kp, hipEvent_t* ev) {
     switch (h->N) {
-        case 1: return launch_stage_t<1>(h, kp, ev);
-        case 2: return launch_stage_t<2>(h, kp, ev);
-        case 3: return launch_stage_t<3>(h, kp, ev);
-        case 4: return launch_stage_t<4>(h, kp, ev);
-        case 5: return launch_stage_t<5>(h, kp, ev);
+        case 1: return launch_stage_t<1, TF>(h, kp, ev);
+        case 2: return launch_stage_t<2, TF>(h, kp, ev);
+        case 3: return launch_stage_t<3, TF>(h, kp, ev);
+        case 4: return launch_stage_t<4, TF>(h, kp, ev);
+        case 5: return launch_stage_t<5, TF>(h, kp, ev);
         default: return PLENS_E_INVALID;
     }
+}
+
+int launch_stage(plens_gpu* h, const KP& kp, hipEvent_t* ev = nullptr) {
+    return h->precision == PLENS_PREC_FP32_BULK ? launch_stage_p<float>(h, kp, ev) : launch_stage_p<double>(h, kp, ev);
 }
 
 // Kernel parameters of RK stage s (1..5) of a step that starts from buffer `from_b`.
@@ -1112,7 +1190,7 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
     if (!m->Ja || !m->detJ || !m->face_normals || !m->min_vertex_dist || !m->face_nbr || !m->face_bid)
         return PLENS_E_INVALID;
     if (ph->inv_surf_flux < 0 || ph->inv_surf_flux > 3) return PLENS_E_INVALID;
-    if (ph->precision != PLENS_PREC_FP64) return PLENS_E_INVALID;
+    if (ph->precision != PLENS_PREC_FP64 && ph->precision != PLENS_PREC_FP32_BULK) return PLENS_E_INVALID;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PLENS_E_NODEVICE;
     const int N = m->degree, NP = N + 1, NN = NP * NP * NP, NF = NP * NP, nc = m->n_cells;
@@ -1130,6 +1208,7 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
     h->N = N; h->NN = NN; h->NF = NF; h->nc = nc; h->n_bcs = n_bcs;
     h->ndof = (long long)nc * NN;
     h->cpb = default_cpb(N);
+    h->precision = ph->precision;
     (void)hipGetDevice(&h->device);
     int rc = PLENS_OK;
     auto fail = [&](int code) {
@@ -1160,12 +1239,22 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
         (rc = dmalloc(&h->nbr, (size_t)nc * 6)) || (rc = dmalloc(&h->bid, (size_t)nc * 6)) ||
         (rc = dmalloc(&h->wall, nc)) || (rc = dmalloc(&h->tasks, tasks.size())) ||
         (rc = dmalloc(&h->bcs, std::max(n_bcs, 1))) || (rc = dmalloc(&h->basis, 1)) ||
-        (rc = dmalloc(&h->avars, 9 * nd)) || (rc = dmalloc(&h->araw, nc)) || (rc = dmalloc(&h->alpha, nc)) ||
-        (rc = dmalloc(&h->alpha_d, nc)) || (rc = dmalloc(&h->sinv, (size_t)nc * 30 * NF)) ||
-        (rc = dmalloc(&h->svis, (size_t)nc * 30 * NF)) || (rc = dmalloc(&h->dt_part, nblk_max)) ||
+        (rc = dmalloc(&h->araw, nc)) || (rc = dmalloc(&h->alpha, nc)) ||
+        (rc = dmalloc(&h->alpha_d, nc)) || (rc = dmalloc(&h->dt_part, nblk_max)) ||
         (rc = dmalloc(&h->scal, 2)) || (rc = dmalloc(&h->fail, 1)) || (rc = dmalloc(&h->blist, nc)) ||
         (rc = dmalloc(&h->bcount, 1)) || (rc = dmalloc(&h->mlp_stats, 2)))
         return fail(rc);
+    // viscous auxiliary variables and finished face terms in the flux arithmetic of the variant
+    const size_t nface = (size_t)nc * 30 * NF;
+    if (h->precision == PLENS_PREC_FP64) {
+        if ((rc = dmalloc(&h->avars, 9 * nd)) || (rc = dmalloc(&h->sinv, nface)) || (rc = dmalloc(&h->svis, nface)) ||
+            hipMemset(h->avars, 0, sizeof(double) * 9 * nd) != hipSuccess)
+            return fail(rc ? rc : PLENS_E_HIP);
+    } else {
+        if ((rc = dmalloc(&h->avars_f, 9 * nd)) || (rc = dmalloc(&h->sinv_f, nface)) ||
+            (rc = dmalloc(&h->svis_f, nface)) || hipMemset(h->avars_f, 0, sizeof(float) * 9 * nd) != hipSuccess)
+            return fail(rc ? rc : PLENS_E_HIP);
+    }
     // host-side tables
     std::vector<BCDev> hb(std::max(n_bcs, 1));
     const double R = 8.314462618 / ph->molecular_weight;  // NavierStokes::get_R
@@ -1204,9 +1293,8 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
         hipMemcpy(h->bcs, hb.data(), sizeof(BCDev) * hb.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->basis, &hbas, sizeof(Basis), hipMemcpyHostToDevice) != hipSuccess)
         return fail(PLENS_E_HIP);
-    // zero-initialised work arrays (avars is read on faces of inviscid runs only through unused paths)
-    if (hipMemset(h->avars, 0, sizeof(double) * 9 * nd) != hipSuccess ||
-        hipMemset(h->R[0], 0, sizeof(double) * 5 * nd) != hipSuccess ||
+    // zero-initialised work arrays
+    if (hipMemset(h->R[0], 0, sizeof(double) * 5 * nd) != hipSuccess ||
         hipMemset(h->R[1], 0, sizeof(double) * 5 * nd) != hipSuccess ||
         hipMemset(h->rlo, 0, sizeof(double) * 5 * nd) != hipSuccess ||
         hipMemset(h->alpha, 0, sizeof(double) * nc) != hipSuccess ||
@@ -1222,7 +1310,8 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
     k.Ja = h->Ja; k.detJ = h->detJ; k.nrm = h->nrm; k.hmin = h->hmin;
     k.nbr = h->nbr; k.bid = h->bid; k.wall = h->wall; k.tasks = h->tasks; k.bcs = h->bcs; k.basis = h->basis;
     k.avars = h->avars; k.araw = h->araw; k.alpha = h->alpha; k.alpha_d = h->alpha_d;
-    k.sinv = h->sinv; k.svis = h->svis; k.rhs_out = h->rhs; k.rlo = h->rlo;
+    k.sinv = h->sinv; k.svis = h->svis; k.avars_f = h->avars_f; k.sinv_f = h->sinv_f; k.svis_f = h->svis_f;
+    k.rhs_out = h->rhs; k.rlo = h->rlo;
     k.blist = h->blist; k.bcount = h->bcount;
     k.mlp = nullptr; k.mlp_tol = 0.05; k.mlp_stats = h->mlp_stats; k.feat = nullptr;
     k.dt_part = h->dt_part; k.dt = h->scal; k.t_acc = h->scal + 1; k.fail = h->fail;
@@ -1247,7 +1336,8 @@ void plens_gpu_destroy(plens_gpu* h) {
         if (x) (void)hipEventDestroy(x);
     void* ptrs[] = {h->Qa, h->Qb, h->R[0], h->R[1], h->Ja, h->detJ, h->nrm, h->hmin, h->nbr,
                     h->bid, h->wall, h->tasks, h->bcs, h->basis, h->avars, h->araw, h->alpha, h->alpha_d,
-                    h->sinv, h->svis, h->rhs, h->rlo, h->dt_part, h->scal, h->fail, h->blist, h->bcount, h->mlp_stats, h->feat};
+                    h->sinv, h->svis, h->rhs, h->rlo, h->dt_part, h->scal, h->fail, h->blist, h->bcount, h->mlp_stats, h->feat,
+                    h->avars_f, h->sinv_f, h->svis_f};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1312,19 +1402,8 @@ int plens_gpu_calc_dt(plens_gpu* h, double courant, double* dt) {
     kp.want_dt = 1;
     kp.courant = courant;
     kp.t_acc = nullptr;
-    const int NN = h->NN;
-    const int thr = threads_for(kp.cpb, NN);
-    const int blocks = (kp.nc + kp.cpb - 1) / kp.cpb;
-    const size_t lds = sizeof(double) * 19 * kp.cpb * NN;
     // gradient kernel with the time-step candidates (it also refreshes avars), then the reduction
-    switch (h->N) {
-        case 1: hipLaunchKernelGGL(k_grad<1>, dim3(blocks), dim3(thr), lds, h->stream, kp); break;
-        case 2: hipLaunchKernelGGL(k_grad<2>, dim3(blocks), dim3(thr), lds, h->stream, kp); break;
-        case 3: hipLaunchKernelGGL(k_grad<3>, dim3(blocks), dim3(thr), lds, h->stream, kp); break;
-        case 4: hipLaunchKernelGGL(k_grad<4>, dim3(blocks), dim3(thr), lds, h->stream, kp); break;
-        default: hipLaunchKernelGGL(k_grad<5>, dim3(blocks), dim3(thr), lds, h->stream, kp); break;
-    }
-    HIPCHK(hipGetLastError());
+    if ((e = launch_grad(h, kp))) return e;
     hipLaunchKernelGGL(k_alpha, dim3((kp.nc + kThreadsAlpha - 1) / kThreadsAlpha), dim3(kThreadsAlpha), 0,
                        h->stream, kp);
     HIPCHK(hipGetLastError());
@@ -1497,18 +1576,7 @@ int plens_gpu_indicator_samples(plens_gpu* h, float* feats, double* alpha_raw) {
     kp.want_dt = 0;
     kp.t_acc = nullptr;
     kp.feat = h->feat;  // blender values of the current indicator (plain unless a network is set)
-    const int NN = h->NN;
-    const int thr = threads_for(kp.cpb, NN);
-    const int blocks = (kp.nc + kp.cpb - 1) / kp.cpb;
-    const size_t lds = sizeof(double) * 19 * kp.cpb * NN;
-    switch (h->N) {
-        case 1: hipLaunchKernelGGL(k_grad<1>, dim3(blocks), dim3(thr), lds, h->stream, kp); break;
-        case 2: hipLaunchKernelGGL(k_grad<2>, dim3(blocks), dim3(thr), lds, h->stream, kp); break;
-        case 3: hipLaunchKernelGGL(k_grad<3>, dim3(blocks), dim3(thr), lds, h->stream, kp); break;
-        case 4: hipLaunchKernelGGL(k_grad<4>, dim3(blocks), dim3(thr), lds, h->stream, kp); break;
-        default: hipLaunchKernelGGL(k_grad<5>, dim3(blocks), dim3(thr), lds, h->stream, kp); break;
-    }
-    HIPCHK(hipGetLastError());
+    if ((e = launch_grad(h, kp))) return e;
     if ((e = read_failure(h))) return e;
     if (feats)
         HIPCHK(hipMemcpyAsync(feats, h->feat, sizeof(float) * (size_t)h->nc * kMlpIn, hipMemcpyDeviceToHost, h->stream));
