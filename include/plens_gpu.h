@@ -140,6 +140,36 @@ int plens_gpu_stream_copy(int64_t bytes, int32_t iters, double* ms_per_copy);
 
 const char* plens_gpu_strerror(int code);
 
+/* ---- batched uneven solves ------------------------------------------------------------------
+ * No counterpart in the reference, where each solve is its own PLENS::run (dgsem/plens.cc:1547)
+ * driving update/update_rk4/calc_rhs (4484-4518, 4334-4403, 3688-3779). A batch takes the current
+ * states of existing handles (any degrees and mesh sizes; per-handle boundary conditions, Sutherland
+ * viscosity and time step) and advances all of them together: handles of one degree and precision
+ * are merged into one group whose kernels run over the cells of all its instances, and one time
+ * step of every group is captured into one HIP graph replayed per step. Every instance keeps its
+ * own global time step (minimum over its own cells, as in calc_time_step 3872-3875). The
+ * instances must agree on gamma, R, Pr, surface flux and blender parameters within a group.
+ * The source handles are not modified. */
+typedef struct plens_batch plens_batch;
+
+/* tuning_db: path of a text file of autotuned launch shapes (cells per workgroup of the cell
+ * kernels, workgroup size of the face kernel) keyed by degree, precision and log2 of the group's
+ * cell count. Shapes found there are used; missing ones are timed on the group (RHS evaluations,
+ * the state is untouched) and written back. NULL keeps the built-in defaults. */
+int plens_gpu_batch_create(plens_gpu* const* instances, int32_t n, const char* tuning_db, plens_batch** out);
+void plens_gpu_batch_destroy(plens_batch* b);
+
+/* n_steps RK4 steps of every instance, instance i at Courant number courant[i]; t_inout[n]
+ * (nullable) is advanced by each instance's own accumulated time. On PLENS_E_POSITIVITY the
+ * failing instance and DoF are in plens_gpu_batch_last_failure. */
+int plens_gpu_batch_run(plens_batch* b, int32_t n_steps, const double* courant, double* t_inout);
+int plens_gpu_batch_get_state(const plens_batch* b, int32_t instance, double* Q);
+int plens_gpu_batch_last_run_ms(const plens_batch* b, double* ms);
+int plens_gpu_batch_last_failure(const plens_batch* b, int32_t* instance, int64_t* dof, int32_t* kind);
+int plens_gpu_batch_n_groups(const plens_batch* b);
+int plens_gpu_batch_group_info(const plens_batch* b, int32_t group, int32_t* degree, int32_t* n_instances,
+                               int64_t* n_dof, int32_t* cells_per_workgroup, int32_t* face_workgroup);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,5 +4,6 @@ The native library libplens_gpu.so (built in-tree by plens_amd.build) holds the 
 of include/plens_gpu.h and the host setup of include/plens_host.h.
 """
 from . import _lib  # noqa: F401
-from .cases import CONFIGS, Mesh, periodic_cube, shock_tube, wall_flow, with_precision  # noqa: F401
+from .batch import GpuBatch  # noqa: F401
+from .cases import CONFIGS, Mesh, batch_instance, periodic_cube, shock_tube, wall_flow, with_precision  # noqa: F401
 from .solver import GpuPlens, PositivityError  # noqa: F401

@@ -137,13 +137,13 @@ def _ic(fn, mesh, *args):
     return Q
 
 
-def periodic_cube(degree=3, n=8, seed=0, flux="Chandrashekhar", mu0=1e-3, steps=100):
+def periodic_cube(degree=3, n=8, seed=0, flux="Chandrashekhar", mu0=1e-3, steps=100, velocity=(0.5, 0.3, 0.2)):
     """Periodic unit cube, density wave rho = 1 + 0.1 sin(2 pi k.x), p = 1, u = (0.5, 0.3, 0.2),
     seeded integer k with 0 < |k| <= 2; CFL 0.4. The blender stays at its floor."""
     lib = L.load()
     mesh = Mesh(degree, n, n, n, (0, 0, 0), (1, 1, 1), (1, 1, 1), [-1] * 6, (0, 0, 0))
     ph = make_physics(mu0=mu0, flux=flux)
-    vel = (C.c_double * 3)(0.5, 0.3, 0.2)
+    vel = (C.c_double * 3)(*velocity)
     kk = (C.c_int32 * 3)()
     Q = np.zeros((5, mesh.n_dof))
     L.check(lib.plens_ic_density_wave(mesh.handle, 1.4, 1.0, vel, 1.0, 0.1, 2.0, seed, kk,
@@ -195,6 +195,37 @@ def wall_flow(degree=4, nx=128, ny=64, nz=2, seed=2, flux="HLLC", mach=6.0, T_in
             "ic")
     return Case("wall_flow", mesh, ph, bcs, Q, 0.4, steps,
                 {"seed": seed, "cells": (nx, ny, nz), "degree": degree, "u_inf": u_inf})
+
+
+def batch_instance(i, seed0=100, min_n=4, max_n=32, flux="HLLC"):
+    """Instance i (seed seed0 + i) of the batched uneven-solve workload: a periodic density-wave box
+    (as periodic_cube) or a normal-shock tube (as shock_tube), drawn with even probability; about
+    n^3 cells with n log-uniform in [min_n, max_n] and even (box n x n x n; tube n^3/m^2 x m x m with
+    m = max(1, n / 4)); degree uniform in {2, 3, 4, 5}; Reynolds number
+    log-uniform in [1e3, 1e5]; Mach uniform in [0.3, 8] (box flow speed) or [1.5, 8] (tube shock).
+    The step count of every instance is 50; the Courant number is the instance's own (below)."""
+    rng = np.random.default_rng(seed0 + i)
+    kind = "box" if rng.random() < 0.5 else "tube"
+    n = int(np.exp(rng.uniform(np.log(min_n), np.log(max_n))) + 0.5)
+    n = max(min_n, min(max_n, n + (n & 1)))
+    degree = int(rng.integers(2, 6))
+    reynolds = float(10.0 ** rng.uniform(3.0, 5.0))
+    if kind == "box":
+        mach = float(rng.uniform(0.3, 8.0))
+        d = np.array([0.5, 0.3, 0.2]) / np.linalg.norm([0.5, 0.3, 0.2])
+        u = d * mach * math.sqrt(1.4)  # p = rho = 1
+        case = periodic_cube(degree=degree, n=n, seed=seed0 + i, flux=flux, mu0=float(np.linalg.norm(u)) / reynolds,
+                             steps=50, velocity=tuple(u))
+    else:
+        mach = float(rng.uniform(1.5, 8.0))
+        m = max(1, n // 4)
+        case = shock_tube(degree=degree, nx=max(1, n ** 3 // (m * m)), ny=m, nz=m, seed=seed0 + i, flux=flux,
+                          mach=mach, reynolds=reynolds, steps=50)
+    # Courant number of the instance: 0.4, halved for shocks above Mach 6, where the blended scheme
+    # loses positivity at 0.4 within the first steps (the oracle fails the same way)
+    case.courant = 0.2 if (kind == "tube" and mach > 6.0) else 0.4
+    case.meta.update({"instance": i, "kind": kind, "mach": mach, "reynolds": reynolds})
+    return case
 
 
 def with_precision(case, precision):

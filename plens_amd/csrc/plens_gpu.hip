@@ -24,7 +24,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "flux.h"
@@ -57,7 +59,7 @@ struct Basis {
 };
 
 struct KP {
-    int nc, cpb, ntasks, n_dt_part;
+    int nc, cpb, ntasks, ninst;
     long long ndof;
     const double* Q;
     double* Qout;
@@ -91,13 +93,18 @@ struct KP {
     double mlp_tol;          // fallback threshold |alpha_mlp - alpha_plain|
     unsigned long long* mlp_stats;  // [0] cells evaluated, [1] fallbacks
     float* feat;             // optional features [n_cells][125] of the indicator
-    double* dt_part;
-    double* dt;
-    double* t_acc;
-    unsigned long long* fail;
+    double* dt_cell;         // stage-1 time-step candidate of every cell
+    const int* cinst;        // instance of every cell (batched solves; all 0 for one instance)
+    const int* ioff;         // cell range [ioff[i], ioff[i+1]) of instance i
+    const double* visc;      // Sutherland mu0, T0, S of every instance
+    double* dt;              // time step of every instance
+    double* t_acc;           // accumulated time of every instance
+    int face_threads;        // workgroup size of the face kernel
+    unsigned long long* fail;  // [0] lowest failing code of the run, [1] [0] latched after the first failing stage
     Gas gas;
     int scheme;
     double thr, sharp, amin, amax, wlimit, courant;
+    const double* cfl;       // Courant number of every instance (batches; nullptr: courant)
     double iw0, iwN;  // 1 / w_0, 1 / w_N of the GLL rule
     int stage;    // 0: write rhs only; 1..5: RK stage
     int want_dt;  // compute the stage-1 time step
@@ -265,7 +272,15 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
     }
     const Gas& gs = p.gas;
     const double e = get_e(u);
-    const double mu = gs.inviscid ? 0.0 : sutherland(gs, e / gs.cv);
+    double mu = 0.0;
+    if (!gs.inviscid) {
+        const int inst = act ? p.cinst[c] : 0;
+        Gas gi = gs;
+        gi.mu0 = p.visc[3 * inst];
+        gi.T0 = p.visc[3 * inst + 1];
+        gi.S = p.visc[3 * inst + 2];
+        mu = sutherland(gi, e / gs.cv);
+    }
     const double kap = conductivity(gs, mu);
 
     // ---- blender: modal energies of rho*p (BlenderCalculator::get_blender, blender_calculator.cc:123-187)
@@ -365,7 +380,8 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
                     for (int cc = 0; cc < 3; ++cc) s += (p.Ja[((long long)c * 9 + 3 * cc + r) * NN + node] / detJ) * vel[cc];
                     madv = fmax(madv, s);
                 }
-                double co = p.courant >= 1.0 ? 0.9 : p.courant;
+                const double cr = p.cfl ? p.cfl[p.cinst[c]] : p.courant;
+                const double co = cr >= 1.0 ? 0.9 : cr;
                 cur = co / (pow((double)N, 1.5) * (madv + a / h + (double)(N * N) * mu / (h * h * u[0])));
             }
             sE[li] = cur;
@@ -375,11 +391,7 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
             if (inblk && node < st && node + st < NN) sE[li] = fmin(sE[li], sE[li + st]);
         }
         __syncthreads();
-        if (tid == 0) {
-            double m = 1e300;
-            for (int cl = 0; cl < p.cpb; ++cl) m = fmin(m, sE[cl * NN]);
-            p.dt_part[blockIdx.x] = m;
-        }
+        if (act && node == 0) p.dt_cell[c] = sE[li];
     }
     if (gs.inviscid) return;
 
@@ -480,10 +492,16 @@ __global__ void __launch_bounds__(kThreadsAlpha) k_alpha(KP p) {
         p.alpha_d[c] = fmin(1.0, a / p.amax);
         if (a > 0.0) p.blist[atomicAdd(p.bcount, 1)] = c;  // order irrelevant: cells are independent
     }
-    if (p.want_dt && blockIdx.x == 0) {
-        __shared__ double red[kThreadsAlpha];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // latch the failure of the first failing stage
+        const unsigned long long f = p.fail[0];
+        if (f != ~0ull && p.fail[1] == ~0ull) p.fail[1] = f;
+    }
+    if (!p.want_dt) return;
+    // global time step of every instance: minimum over its cells (dgsem/plens.cc:3872-3875, 3905-3910)
+    __shared__ double red[kThreadsAlpha];
+    for (int i = blockIdx.x; i < p.ninst; i += gridDim.x) {
         double m = 1e300;
-        for (int t = threadIdx.x; t < p.n_dt_part; t += blockDim.x) m = fmin(m, p.dt_part[t]);
+        for (int t = p.ioff[i] + threadIdx.x; t < p.ioff[i + 1]; t += blockDim.x) m = fmin(m, p.dt_cell[t]);
         red[threadIdx.x] = m;
         __syncthreads();
         for (int s = blockDim.x / 2; s > 0; s >>= 1) {
@@ -491,9 +509,10 @@ __global__ void __launch_bounds__(kThreadsAlpha) k_alpha(KP p) {
             __syncthreads();
         }
         if (threadIdx.x == 0) {
-            p.dt[0] = red[0];
-            if (p.t_acc) p.t_acc[0] += red[0];
+            p.dt[i] = red[0];
+            if (p.t_acc) p.t_acc[i] += red[0];
         }
+        __syncthreads();
     }
 }
 
@@ -929,7 +948,7 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
         for (int v = 0; v < 5; ++v) p.rhs_out[v * p.ndof + g] = rhs[v];
         return;
     }
-    const double dt = p.dt[0];
+    const double dt = p.dt[p.cinst[c]];
     for (int v = 0; v < 5; ++v) {
         const long long k = v * p.ndof + g;
         const double R = rhs[v] * dt;
@@ -977,8 +996,15 @@ struct plens_gpu {
     int *blist = nullptr, *bcount = nullptr;
     unsigned long long* mlp_stats = nullptr;
     float* feat = nullptr;
-    double *dt_part = nullptr, *scal = nullptr;  // scal[0] = dt, scal[1] = t accumulator
+    int ninst = 1;                                // instances merged into this handle
+    int* cinst = nullptr;                         // instance of every cell
+    int* ioff = nullptr;                          // cell offsets of the instances
+    double* visc = nullptr;                       // Sutherland mu0, T0, S of every instance
+    double *dt_cell = nullptr, *scal = nullptr;   // scal[0..ninst) = dt, scal[ninst..2 ninst) = time accumulators
+    double* cfl = nullptr;                        // Courant numbers of the instances (batch groups)
     unsigned long long* fail = nullptr;
+    int face_threads = 256;
+    bool own_stream = true;                       // false for the groups of a batch (batch stream)
     bool state_in_b = false;
     KP base{};
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -1074,8 +1100,8 @@ int launch_stage_t(plens_gpu* h, const KP& kp, hipEvent_t* ev) {
     HIPCHK(hipGetLastError());
     if (ev) HIPCHK(hipEventRecord(ev[2], h->stream));
     const long long fthreads = (long long)kp.ntasks * NF;
-    hipLaunchKernelGGL((k_face<N, TF>), dim3((unsigned)((fthreads + kThreadsFace - 1) / kThreadsFace)),
-                       dim3(kThreadsFace), 0, h->stream, kp);
+    const int ft = kp.face_threads;
+    hipLaunchKernelGGL((k_face<N, TF>), dim3((unsigned)((fthreads + ft - 1) / ft)), dim3(ft), 0, h->stream, kp);
     HIPCHK(hipGetLastError());
     if (ev) HIPCHK(hipEventRecord(ev[3], h->stream));
     hipLaunchKernelGGL((k_lo<N, TF>), dim3(std::min(blocks, 256)), dim3(thr), lds_lo, h->stream, kp);
@@ -1141,22 +1167,83 @@ int launch_step(plens_gpu* h, bool from_b, double courant, hipEvent_t* ev = null
     return PLENS_OK;
 }
 
+// Positivity failure of the last call: the lowest failing DoF of the first stage that failed (the
+// reference exits at its first failing check, dgsem/plens.cc:2540-2544), or of the lone gradient
+// pass of calls without a latch.
 int read_failure(plens_gpu* h) {
-    unsigned long long code = 0;
-    HIPCHK(hipMemcpyAsync(&code, h->fail, sizeof(code), hipMemcpyDeviceToHost, h->stream));
+    unsigned long long code[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(code, h->fail, sizeof(code), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    if (code != ~0ull) {
-        h->fail_dof = (int64_t)(code >> 2);
-        h->fail_kind = (int32_t)(code & 3ull);
+    const unsigned long long c = code[1] != ~0ull ? code[1] : code[0];
+    if (c != ~0ull) {
+        h->fail_dof = (int64_t)(c >> 2);
+        h->fail_kind = (int32_t)(c & 3ull);
         return PLENS_E_POSITIVITY;
     }
     return PLENS_OK;
 }
 
 int reset_failure(plens_gpu* h) {
-    const unsigned long long init = ~0ull;
-    HIPCHK(hipMemcpyAsync(h->fail, &init, sizeof(init), hipMemcpyHostToDevice, h->stream));
+    static const unsigned long long init[2] = {~0ull, ~0ull};
+    HIPCHK(hipMemcpyAsync(h->fail, init, sizeof(init), hipMemcpyHostToDevice, h->stream));
     return PLENS_OK;
+}
+
+// Device buffers of a handle whose sizes (N, nc, ntasks, n_bcs, ninst, precision) are set; work
+// arrays start at zero.
+int alloc_buffers(plens_gpu* h) {
+    const size_t nd = (size_t)h->ndof, nc = (size_t)h->nc, NF = (size_t)h->NF, nface = nc * 30 * NF;
+    int rc;
+    if ((rc = dmalloc(&h->Qa, 5 * nd)) || (rc = dmalloc(&h->Qb, 5 * nd)) || (rc = dmalloc(&h->R[0], 5 * nd)) ||
+        (rc = dmalloc(&h->R[1], 5 * nd)) || (rc = dmalloc(&h->rhs, 5 * nd)) || (rc = dmalloc(&h->rlo, 5 * nd)) ||
+        (rc = dmalloc(&h->Ja, 9 * nd)) || (rc = dmalloc(&h->detJ, nd)) || (rc = dmalloc(&h->nrm, nc * 18 * NF)) ||
+        (rc = dmalloc(&h->hmin, nc)) || (rc = dmalloc(&h->nbr, nc * 6)) || (rc = dmalloc(&h->bid, nc * 6)) ||
+        (rc = dmalloc(&h->wall, nc)) || (rc = dmalloc(&h->tasks, (size_t)h->ntasks)) ||
+        (rc = dmalloc(&h->bcs, (size_t)std::max(h->n_bcs, 1))) || (rc = dmalloc(&h->basis, 1)) ||
+        (rc = dmalloc(&h->araw, nc)) || (rc = dmalloc(&h->alpha, nc)) || (rc = dmalloc(&h->alpha_d, nc)) ||
+        (rc = dmalloc(&h->dt_cell, nc)) || (rc = dmalloc(&h->cinst, nc)) ||
+        (rc = dmalloc(&h->ioff, (size_t)h->ninst + 1)) || (rc = dmalloc(&h->visc, 3 * (size_t)h->ninst)) ||
+        (rc = dmalloc(&h->scal, 2 * (size_t)h->ninst)) || (rc = dmalloc(&h->fail, 2)) ||
+        (rc = dmalloc(&h->blist, nc)) || (rc = dmalloc(&h->bcount, 1)) || (rc = dmalloc(&h->mlp_stats, 2)))
+        return rc;
+    // viscous auxiliary variables and finished face terms in the flux arithmetic of the variant
+    if (h->precision == PLENS_PREC_FP64) {
+        if ((rc = dmalloc(&h->avars, 9 * nd)) || (rc = dmalloc(&h->sinv, nface)) || (rc = dmalloc(&h->svis, nface)))
+            return rc;
+        if (hipMemset(h->avars, 0, sizeof(double) * 9 * nd) != hipSuccess) return PLENS_E_HIP;
+    } else {
+        if ((rc = dmalloc(&h->avars_f, 9 * nd)) || (rc = dmalloc(&h->sinv_f, nface)) ||
+            (rc = dmalloc(&h->svis_f, nface)))
+            return rc;
+        if (hipMemset(h->avars_f, 0, sizeof(float) * 9 * nd) != hipSuccess) return PLENS_E_HIP;
+    }
+    if (hipMemset(h->R[0], 0, sizeof(double) * 5 * nd) != hipSuccess ||
+        hipMemset(h->R[1], 0, sizeof(double) * 5 * nd) != hipSuccess ||
+        hipMemset(h->rlo, 0, sizeof(double) * 5 * nd) != hipSuccess ||
+        hipMemset(h->alpha, 0, sizeof(double) * nc) != hipSuccess ||
+        hipMemset(h->scal, 0, sizeof(double) * 2 * h->ninst) != hipSuccess ||
+        hipMemset(h->bcount, 0, sizeof(int)) != hipSuccess ||
+        hipMemset(h->mlp_stats, 0, 2 * sizeof(unsigned long long)) != hipSuccess)
+        return PLENS_E_HIP;
+    const unsigned long long init[2] = {~0ull, ~0ull};
+    if (hipMemcpy(h->fail, init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess) return PLENS_E_HIP;
+    return PLENS_OK;
+}
+
+// Buffer pointers and sizes of the handle into its base kernel parameters.
+void bind_buffers(plens_gpu* h) {
+    KP& k = h->base;
+    k.nc = h->nc; k.cpb = h->cpb; k.ntasks = h->ntasks; k.ndof = h->ndof; k.ninst = h->ninst;
+    k.Ja = h->Ja; k.detJ = h->detJ; k.nrm = h->nrm; k.hmin = h->hmin;
+    k.nbr = h->nbr; k.bid = h->bid; k.wall = h->wall; k.tasks = h->tasks; k.bcs = h->bcs; k.basis = h->basis;
+    k.avars = h->avars; k.araw = h->araw; k.alpha = h->alpha; k.alpha_d = h->alpha_d;
+    k.sinv = h->sinv; k.svis = h->svis; k.avars_f = h->avars_f; k.sinv_f = h->sinv_f; k.svis_f = h->svis_f;
+    k.rhs_out = h->rhs; k.rlo = h->rlo;
+    k.blist = h->blist; k.bcount = h->bcount;
+    k.mlp_stats = h->mlp_stats; k.feat = nullptr;
+    k.dt_cell = h->dt_cell; k.cinst = h->cinst; k.ioff = h->ioff; k.visc = h->visc;
+    k.dt = h->scal; k.t_acc = h->scal + h->ninst; k.fail = h->fail;
+    k.face_threads = h->face_threads;
 }
 
 void destroy_graphs(plens_gpu* h) {
@@ -1231,30 +1318,7 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
             }
         }
     h->ntasks = (int)tasks.size();
-    const int nblk_max = (nc + 0) / 1 + 1;
-    if ((rc = dmalloc(&h->Qa, 5 * nd)) || (rc = dmalloc(&h->Qb, 5 * nd)) || (rc = dmalloc(&h->R[0], 5 * nd)) ||
-        (rc = dmalloc(&h->R[1], 5 * nd)) || (rc = dmalloc(&h->rhs, 5 * nd)) || (rc = dmalloc(&h->rlo, 5 * nd)) ||
-        (rc = dmalloc(&h->Ja, 9 * nd)) || (rc = dmalloc(&h->detJ, nd)) ||
-        (rc = dmalloc(&h->nrm, (size_t)nc * 18 * NF)) || (rc = dmalloc(&h->hmin, nc)) ||
-        (rc = dmalloc(&h->nbr, (size_t)nc * 6)) || (rc = dmalloc(&h->bid, (size_t)nc * 6)) ||
-        (rc = dmalloc(&h->wall, nc)) || (rc = dmalloc(&h->tasks, tasks.size())) ||
-        (rc = dmalloc(&h->bcs, std::max(n_bcs, 1))) || (rc = dmalloc(&h->basis, 1)) ||
-        (rc = dmalloc(&h->araw, nc)) || (rc = dmalloc(&h->alpha, nc)) ||
-        (rc = dmalloc(&h->alpha_d, nc)) || (rc = dmalloc(&h->dt_part, nblk_max)) ||
-        (rc = dmalloc(&h->scal, 2)) || (rc = dmalloc(&h->fail, 1)) || (rc = dmalloc(&h->blist, nc)) ||
-        (rc = dmalloc(&h->bcount, 1)) || (rc = dmalloc(&h->mlp_stats, 2)))
-        return fail(rc);
-    // viscous auxiliary variables and finished face terms in the flux arithmetic of the variant
-    const size_t nface = (size_t)nc * 30 * NF;
-    if (h->precision == PLENS_PREC_FP64) {
-        if ((rc = dmalloc(&h->avars, 9 * nd)) || (rc = dmalloc(&h->sinv, nface)) || (rc = dmalloc(&h->svis, nface)) ||
-            hipMemset(h->avars, 0, sizeof(double) * 9 * nd) != hipSuccess)
-            return fail(rc ? rc : PLENS_E_HIP);
-    } else {
-        if ((rc = dmalloc(&h->avars_f, 9 * nd)) || (rc = dmalloc(&h->sinv_f, nface)) ||
-            (rc = dmalloc(&h->svis_f, nface)) || hipMemset(h->avars_f, 0, sizeof(float) * 9 * nd) != hipSuccess)
-            return fail(rc ? rc : PLENS_E_HIP);
-    }
+    if ((rc = alloc_buffers(h))) return fail(rc);
     // host-side tables
     std::vector<BCDev> hb(std::max(n_bcs, 1));
     const double R = 8.314462618 / ph->molecular_weight;  // NavierStokes::get_R
@@ -1282,6 +1346,8 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
             }
         }
     }
+    const int ioff_h[2] = {0, nc};
+    const double visc_h[3] = {ph->mu0, ph->T0, ph->S};
     if (hipMemcpy(h->Ja, m->Ja, sizeof(double) * 9 * nd, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->detJ, m->detJ, sizeof(double) * nd, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->nrm, m->face_normals, sizeof(double) * nc * 18 * NF, hipMemcpyHostToDevice) != hipSuccess ||
@@ -1291,37 +1357,21 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
         hipMemcpy(h->wall, wall.data(), sizeof(int) * nc, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->tasks, tasks.data(), sizeof(int) * tasks.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->bcs, hb.data(), sizeof(BCDev) * hb.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(h->basis, &hbas, sizeof(Basis), hipMemcpyHostToDevice) != hipSuccess)
+        hipMemcpy(h->basis, &hbas, sizeof(Basis), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(h->cinst, 0, sizeof(int) * nc) != hipSuccess ||
+        hipMemcpy(h->ioff, ioff_h, sizeof(ioff_h), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->visc, visc_h, sizeof(visc_h), hipMemcpyHostToDevice) != hipSuccess)
         return fail(PLENS_E_HIP);
-    // zero-initialised work arrays
-    if (hipMemset(h->R[0], 0, sizeof(double) * 5 * nd) != hipSuccess ||
-        hipMemset(h->R[1], 0, sizeof(double) * 5 * nd) != hipSuccess ||
-        hipMemset(h->rlo, 0, sizeof(double) * 5 * nd) != hipSuccess ||
-        hipMemset(h->alpha, 0, sizeof(double) * nc) != hipSuccess ||
-        hipMemset(h->scal, 0, sizeof(double) * 2) != hipSuccess ||
-        hipMemset(h->bcount, 0, sizeof(int)) != hipSuccess ||
-        hipMemset(h->mlp_stats, 0, 2 * sizeof(unsigned long long)) != hipSuccess)
-        return fail(PLENS_E_HIP);
-    const unsigned long long init = ~0ull;
-    if (hipMemcpy(h->fail, &init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess) return fail(PLENS_E_HIP);
 
     KP& k = h->base;
-    k.nc = nc; k.cpb = h->cpb; k.ntasks = h->ntasks; k.ndof = h->ndof;
-    k.Ja = h->Ja; k.detJ = h->detJ; k.nrm = h->nrm; k.hmin = h->hmin;
-    k.nbr = h->nbr; k.bid = h->bid; k.wall = h->wall; k.tasks = h->tasks; k.bcs = h->bcs; k.basis = h->basis;
-    k.avars = h->avars; k.araw = h->araw; k.alpha = h->alpha; k.alpha_d = h->alpha_d;
-    k.sinv = h->sinv; k.svis = h->svis; k.avars_f = h->avars_f; k.sinv_f = h->sinv_f; k.svis_f = h->svis_f;
-    k.rhs_out = h->rhs; k.rlo = h->rlo;
-    k.blist = h->blist; k.bcount = h->bcount;
-    k.mlp = nullptr; k.mlp_tol = 0.05; k.mlp_stats = h->mlp_stats; k.feat = nullptr;
-    k.dt_part = h->dt_part; k.dt = h->scal; k.t_acc = h->scal + 1; k.fail = h->fail;
+    bind_buffers(h);
+    k.mlp = nullptr; k.mlp_tol = 0.05;
     k.gas.gma = ph->gamma; k.gas.R = R; k.gas.cv = R / (ph->gamma - 1); k.gas.mu0 = ph->mu0; k.gas.T0 = ph->T0;
     k.gas.S = ph->S; k.gas.Pr = ph->prandtl; k.gas.inviscid = ph->mu0 < 1e-16;
     k.scheme = ph->inv_surf_flux;
     k.thr = ph->threshold_factor * pow(10.0, -ph->threshold_exp_factor * (pow(N + 1, 0.25)));
     k.sharp = ph->sharpness_factor; k.amin = ph->alpha_min; k.amax = ph->alpha_max;
     k.wlimit = ph->wall_blender_limit;
-    k.n_dt_part = (nc + h->cpb - 1) / h->cpb;
     k.iw0 = 1.0 / hbas.w[0];
     k.iwN = 1.0 / hbas.w[N];
     *out = h;
@@ -1336,11 +1386,11 @@ void plens_gpu_destroy(plens_gpu* h) {
         if (x) (void)hipEventDestroy(x);
     void* ptrs[] = {h->Qa, h->Qb, h->R[0], h->R[1], h->Ja, h->detJ, h->nrm, h->hmin, h->nbr,
                     h->bid, h->wall, h->tasks, h->bcs, h->basis, h->avars, h->araw, h->alpha, h->alpha_d,
-                    h->sinv, h->svis, h->rhs, h->rlo, h->dt_part, h->scal, h->fail, h->blist, h->bcount, h->mlp_stats, h->feat,
-                    h->avars_f, h->sinv_f, h->svis_f};
+                    h->sinv, h->svis, h->rhs, h->rlo, h->dt_cell, h->scal, h->fail, h->blist, h->bcount, h->mlp_stats, h->feat,
+                    h->avars_f, h->sinv_f, h->svis_f, h->cinst, h->ioff, h->visc, h->cfl};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
 
@@ -1350,7 +1400,6 @@ int plens_gpu_set_tuning(plens_gpu* h, int32_t cpb) {
     if (threads_for(cpb, h->NN) > 256) return PLENS_E_INVALID;
     h->cpb = cpb;
     h->base.cpb = cpb;
-    h->base.n_dt_part = (h->nc + cpb - 1) / cpb;
     destroy_graphs(h);
     return PLENS_OK;
 }
@@ -1418,12 +1467,12 @@ int plens_gpu_rk4_step(plens_gpu* h, double courant, double* t_inout) {
     int e = reset_failure(h);
     if (e) return e;
     const double zero = 0.0;
-    HIPCHK(hipMemcpyAsync(h->scal + 1, &zero, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->scal + h->ninst, &zero, sizeof(double), hipMemcpyHostToDevice, h->stream));
     if ((e = launch_step(h, h->state_in_b, courant))) return e;
     h->state_in_b = !h->state_in_b;
     if ((e = read_failure(h))) return e;
     double dt = 0;
-    HIPCHK(hipMemcpyAsync(&dt, h->scal + 1, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(&dt, h->scal + h->ninst, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     if (t_inout) *t_inout += dt;
     return PLENS_OK;
@@ -1434,7 +1483,7 @@ int plens_gpu_run(plens_gpu* h, int32_t n_steps, double courant, double* t_inout
     int e = reset_failure(h);
     if (e) return e;
     const double zero = 0.0;
-    HIPCHK(hipMemcpyAsync(h->scal + 1, &zero, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->scal + h->ninst, &zero, sizeof(double), hipMemcpyHostToDevice, h->stream));
     if (h->graph_courant != courant || h->graph_cpb != h->cpb) {
         destroy_graphs(h);
         for (int par = 0; par < 2; ++par) {
@@ -1469,7 +1518,7 @@ int plens_gpu_run(plens_gpu* h, int32_t n_steps, double courant, double* t_inout
     }
     if ((e = read_failure(h))) return e;
     double tt = 0;
-    HIPCHK(hipMemcpyAsync(&tt, h->scal + 1, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(&tt, h->scal + h->ninst, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     if (t_inout) *t_inout += tt;
     return PLENS_OK;
@@ -1482,7 +1531,7 @@ int plens_gpu_run_profiled(plens_gpu* h, int32_t n_steps, double courant, double
     int e = reset_failure(h);
     if (e) return e;
     const double zero = 0.0;
-    HIPCHK(hipMemcpyAsync(h->scal + 1, &zero, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->scal + h->ninst, &zero, sizeof(double), hipMemcpyHostToDevice, h->stream));
     std::vector<hipEvent_t> ev(30);
     for (auto& x : ev) HIPCHK(hipEventCreate(&x));
     double acc[5] = {0, 0, 0, 0, 0};
@@ -1501,7 +1550,7 @@ int plens_gpu_run_profiled(plens_gpu* h, int32_t n_steps, double courant, double
     for (int k = 0; k < 5; ++k) h->ktimes[k] = acc[k] / (5.0 * n_steps);
     if ((e = read_failure(h))) return e;
     double tt = 0;
-    HIPCHK(hipMemcpyAsync(&tt, h->scal + 1, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(&tt, h->scal + h->ninst, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     if (t_inout) *t_inout += tt;
     return PLENS_OK;
@@ -1596,6 +1645,424 @@ int plens_gpu_last_failure(const plens_gpu* h, int64_t* dof, int32_t* kind) {
     if (!h) return PLENS_E_INVALID;
     if (dof) *dof = h->fail_dof;
     if (kind) *kind = h->fail_kind;
+    return PLENS_OK;
+}
+
+}  // extern "C"
+
+// =============================================================================================
+// batched uneven solves: handles of one degree are merged into one group (cells, faces, boundary
+// tables and instances concatenated); a step of every group is captured into one HIP graph
+// =============================================================================================
+struct plens_batch {
+    int n = 0;
+    hipStream_t stream = nullptr;
+    std::vector<plens_gpu*> groups;
+    std::vector<int> inst_group, inst_local;      // instance -> group, index inside the group
+    std::vector<long long> inst_dof0, inst_ndof;  // DoF range of the instance inside its group
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    bool state_in_b = false;
+    double last_run_ms = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int fail_inst = -1;
+    int64_t fail_dof = -1;
+    int32_t fail_kind = 0;
+};
+
+namespace {
+
+bool same_scheme(const KP& a, const KP& b) {
+    return a.gas.gma == b.gas.gma && a.gas.R == b.gas.R && a.gas.cv == b.gas.cv && a.gas.Pr == b.gas.Pr &&
+           a.gas.inviscid == b.gas.inviscid && a.scheme == b.scheme && a.thr == b.thr && a.sharp == b.sharp &&
+           a.amin == b.amin && a.amax == b.amax && a.wlimit == b.wlimit;
+}
+
+// Merge single-instance handles of one degree and precision into one handle on `stream`.
+int merge_group(const std::vector<plens_gpu*>& mem, hipStream_t stream, plens_gpu** out) {
+    const plens_gpu* m0 = mem[0];
+    for (const plens_gpu* m : mem)
+        if (m->N != m0->N || m->precision != m0->precision || m->ninst != 1 || !same_scheme(m->base, m0->base))
+            return PLENS_E_INVALID;
+    plens_gpu* h = new (std::nothrow) plens_gpu;
+    if (!h) return PLENS_E_NOMEM;
+    auto fail = [&](int code) {
+        plens_gpu_destroy(h);
+        return code;
+    };
+    h->N = m0->N; h->NN = m0->NN; h->NF = m0->NF; h->precision = m0->precision;
+    h->device = m0->device;
+    h->stream = stream;
+    h->own_stream = false;
+    h->ninst = (int)mem.size();
+    for (const plens_gpu* m : mem) {
+        h->nc += m->nc;
+        h->ntasks += m->ntasks;
+        h->n_bcs += m->n_bcs;
+    }
+    h->ndof = (long long)h->nc * h->NN;
+    h->cpb = default_cpb(h->N);
+    int rc = alloc_buffers(h);
+    if (rc) return fail(rc);
+    const int NN = h->NN, NF = h->NF;
+    std::vector<int> nbr((size_t)h->nc * 6), bid((size_t)h->nc * 6), tasks((size_t)std::max(h->ntasks, 1)),
+        cinst(h->nc), ioff(h->ninst + 1);
+    std::vector<double> visc(3 * (size_t)h->ninst);
+    int co = 0, to = 0, bo = 0;
+    for (int i = 0; i < h->ninst; ++i) {
+        const plens_gpu* m = mem[i];
+        const long long d0 = (long long)co * NN;
+        const double* Qm = m->state_in_b ? m->Qb : m->Qa;
+        for (int v = 0; v < 5; ++v)
+            HIPCHK(hipMemcpy(h->Qa + v * h->ndof + d0, Qm + v * m->ndof, sizeof(double) * m->ndof,
+                             hipMemcpyDeviceToDevice));
+        HIPCHK(hipMemcpy(h->Ja + (size_t)co * 9 * NN, m->Ja, sizeof(double) * 9 * m->ndof, hipMemcpyDeviceToDevice));
+        HIPCHK(hipMemcpy(h->detJ + d0, m->detJ, sizeof(double) * m->ndof, hipMemcpyDeviceToDevice));
+        HIPCHK(hipMemcpy(h->nrm + (size_t)co * 18 * NF, m->nrm, sizeof(double) * m->nc * 18 * NF,
+                         hipMemcpyDeviceToDevice));
+        HIPCHK(hipMemcpy(h->hmin + co, m->hmin, sizeof(double) * m->nc, hipMemcpyDeviceToDevice));
+        HIPCHK(hipMemcpy(h->wall + co, m->wall, sizeof(int) * m->nc, hipMemcpyDeviceToDevice));
+        if (m->n_bcs > 0)
+            HIPCHK(hipMemcpy(h->bcs + bo, m->bcs, sizeof(BCDev) * m->n_bcs, hipMemcpyDeviceToDevice));
+        HIPCHK(hipMemcpy(nbr.data() + (size_t)co * 6, m->nbr, sizeof(int) * m->nc * 6, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(bid.data() + (size_t)co * 6, m->bid, sizeof(int) * m->nc * 6, hipMemcpyDeviceToHost));
+        if (m->ntasks > 0)
+            HIPCHK(hipMemcpy(tasks.data() + to, m->tasks, sizeof(int) * m->ntasks, hipMemcpyDeviceToHost));
+        for (int k = co * 6; k < (co + m->nc) * 6; ++k) {
+            if (nbr[k] >= 0) nbr[k] += co;
+            if (bid[k] >= 0) bid[k] += bo;
+        }
+        for (int k = to; k < to + m->ntasks; ++k) tasks[k] += co * 6;
+        for (int c = co; c < co + m->nc; ++c) cinst[c] = i;
+        ioff[i] = co;
+        visc[3 * i] = m->base.gas.mu0;
+        visc[3 * i + 1] = m->base.gas.T0;
+        visc[3 * i + 2] = m->base.gas.S;
+        co += m->nc;
+        to += m->ntasks;
+        bo += m->n_bcs;
+    }
+    ioff[h->ninst] = co;
+    if (hipMemcpy(h->nbr, nbr.data(), sizeof(int) * nbr.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->bid, bid.data(), sizeof(int) * bid.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->tasks, tasks.data(), sizeof(int) * h->ntasks, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->cinst, cinst.data(), sizeof(int) * cinst.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->ioff, ioff.data(), sizeof(int) * ioff.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->visc, visc.data(), sizeof(double) * visc.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->basis, m0->basis, sizeof(Basis), hipMemcpyDeviceToDevice) != hipSuccess)
+        return fail(PLENS_E_HIP);
+    if ((rc = dmalloc(&h->cfl, (size_t)h->ninst))) return fail(rc);
+    h->base = m0->base;
+    bind_buffers(h);
+    h->base.cfl = h->cfl;
+    h->base.mlp = m0->base.mlp;
+    h->base.mlp_tol = m0->base.mlp_tol;
+    *out = h;
+    return PLENS_OK;
+}
+
+// Launch shapes of one group: cells per workgroup (k_grad, k_lo, k_vol) and the face-kernel
+// workgroup size, timed on RHS evaluations (stage 0 leaves the state untouched).
+double time_rhs(plens_gpu* h, int reps) {
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return 1e30;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return 1e30;
+    }
+    KP kp = h->base;
+    kp.Q = h->state_in_b ? h->Qb : h->Qa;
+    kp.Qout = nullptr;
+    kp.stage = 0;
+    kp.want_dt = 0;
+    kp.t_acc = nullptr;
+    double best = 1e30;
+    bool ok = launch_stage(h, kp) == PLENS_OK;  // warm-up
+    for (int r = 0; ok && r < reps; ++r) {
+        ok = hipEventRecord(e0, h->stream) == hipSuccess && launch_stage(h, kp) == PLENS_OK &&
+             hipEventRecord(e1, h->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+        float ms = 0;
+        if (ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) best = std::min(best, (double)ms);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ok ? best : 1e30;
+}
+
+std::string tuning_key(const plens_gpu* h) {
+    int cls = 0;
+    while ((1 << (cls + 1)) <= h->nc) ++cls;  // floor(log2(cells))
+    char buf[64];
+    snprintf(buf, sizeof(buf), "%d %d %d", h->N, h->precision, cls);
+    return buf;
+}
+
+void apply_shape(plens_gpu* h, int cpb, int ft) {
+    h->cpb = cpb;
+    h->face_threads = ft;
+    h->base.cpb = cpb;
+    h->base.face_threads = ft;
+}
+
+int tune_group(plens_gpu* h, std::map<std::string, std::pair<int, int>>& db, bool& db_changed) {
+    const std::string key = tuning_key(h);
+    auto it = db.find(key);
+    if (it != db.end() && threads_for(it->second.first, h->NN) <= 256 &&
+        (it->second.second == 64 || it->second.second == 128 || it->second.second == 256)) {
+        apply_shape(h, it->second.first, it->second.second);
+        return PLENS_OK;
+    }
+    int best_cpb = h->cpb;
+    double best = 1e30;
+    for (int cpb = 1; threads_for(cpb, h->NN) <= 256; cpb *= 2) {
+        apply_shape(h, cpb, 256);
+        const double t = time_rhs(h, 3);
+        if (t < best) {
+            best = t;
+            best_cpb = cpb;
+        }
+    }
+    int best_ft = 256;
+    for (int ft : {64, 128, 256}) {
+        apply_shape(h, best_cpb, ft);
+        const double t = time_rhs(h, 3);
+        if (t < best) {
+            best = t;
+            best_ft = ft;
+        }
+    }
+    if (best >= 1e30) return PLENS_E_HIP;
+    apply_shape(h, best_cpb, best_ft);
+    db[key] = {best_cpb, best_ft};
+    db_changed = true;
+    return PLENS_OK;
+}
+
+void read_db(const char* path, std::map<std::string, std::pair<int, int>>& db) {
+    FILE* f = fopen(path, "r");
+    if (!f) return;
+    char line[256];
+    while (fgets(line, sizeof(line), f)) {
+        int N, prec, cls, cpb, ft;
+        if (line[0] == '#') continue;
+        if (sscanf(line, "%d %d %d %d %d", &N, &prec, &cls, &cpb, &ft) == 5) {
+            char buf[64];
+            snprintf(buf, sizeof(buf), "%d %d %d", N, prec, cls);
+            db[buf] = {cpb, ft};
+        }
+    }
+    fclose(f);
+}
+
+int write_db(const char* path, const std::map<std::string, std::pair<int, int>>& db) {
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE* f = fopen(tmp.c_str(), "w");
+    if (!f) return PLENS_E_INVALID;
+    fprintf(f, "# degree precision floor(log2(cells)) cells_per_workgroup face_workgroup\n");
+    for (const auto& kv : db) fprintf(f, "%s %d %d\n", kv.first.c_str(), kv.second.first, kv.second.second);
+    if (fclose(f) != 0) return PLENS_E_INVALID;
+    return rename(tmp.c_str(), path) == 0 ? PLENS_OK : PLENS_E_INVALID;
+}
+
+void batch_destroy_graphs(plens_batch* b) {
+    for (auto& g : b->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+}  // namespace
+
+extern "C" {
+
+int plens_gpu_batch_create(plens_gpu* const* instances, int32_t n, const char* tuning_db, plens_batch** out) {
+    if (!instances || n < 1 || !out) return PLENS_E_INVALID;
+    for (int i = 0; i < n; ++i)
+        if (!instances[i] || instances[i]->ninst != 1) return PLENS_E_INVALID;
+    plens_batch* b = new (std::nothrow) plens_batch;
+    if (!b) return PLENS_E_NOMEM;
+    auto fail = [&](int code) {
+        plens_gpu_batch_destroy(b);
+        return code;
+    };
+    b->n = n;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(PLENS_E_HIP);
+    for (int i = 0; i < n; ++i)  // the instances' pending work is done before their buffers are read
+        if (hipStreamSynchronize(instances[i]->stream) != hipSuccess) return fail(PLENS_E_HIP);
+    // groups by (degree, precision), in order of first appearance
+    std::vector<std::pair<int, int>> keys;
+    std::vector<std::vector<plens_gpu*>> members;
+    b->inst_group.resize(n);
+    b->inst_local.resize(n);
+    for (int i = 0; i < n; ++i) {
+        const std::pair<int, int> k{instances[i]->N, instances[i]->precision};
+        size_t g = std::find(keys.begin(), keys.end(), k) - keys.begin();
+        if (g == keys.size()) {
+            keys.push_back(k);
+            members.emplace_back();
+        }
+        b->inst_group[i] = (int)g;
+        b->inst_local[i] = (int)members[g].size();
+        members[g].push_back(instances[i]);
+    }
+    for (auto& mem : members) {
+        plens_gpu* g = nullptr;
+        const int rc = merge_group(mem, b->stream, &g);
+        if (rc) return fail(rc);
+        b->groups.push_back(g);
+    }
+    b->inst_dof0.resize(n);
+    b->inst_ndof.resize(n);
+    {
+        std::vector<long long> acc(b->groups.size(), 0);
+        for (int i = 0; i < n; ++i) {
+            const int g = b->inst_group[i];
+            b->inst_dof0[i] = acc[g];
+            b->inst_ndof[i] = instances[i]->ndof;
+            acc[g] += instances[i]->ndof;
+        }
+    }
+    if (tuning_db) {
+        std::map<std::string, std::pair<int, int>> db;
+        read_db(tuning_db, db);
+        bool changed = false;
+        for (plens_gpu* g : b->groups) {
+            const int rc = tune_group(g, db, changed);
+            if (rc) return fail(rc);
+        }
+        if (changed) {
+            const int rc = write_db(tuning_db, db);
+            if (rc) return fail(rc);
+        }
+    }
+    *out = b;
+    return PLENS_OK;
+}
+
+void plens_gpu_batch_destroy(plens_batch* b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_destroy_graphs(b);
+    for (plens_gpu* g : b->groups) plens_gpu_destroy(g);
+    for (auto& e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int plens_gpu_batch_run(plens_batch* b, int32_t n_steps, const double* courant, double* t_inout) {
+    if (!b || n_steps < 0 || !courant) return PLENS_E_INVALID;
+    for (int i = 0; i < b->n; ++i)
+        if (!(courant[i] > 0.0)) return PLENS_E_INVALID;
+    b->fail_inst = -1;
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+        plens_gpu* g = b->groups[gi];
+        int e = reset_failure(g);
+        if (e) return e;
+        HIPCHK(hipMemsetAsync(g->scal + g->ninst, 0, sizeof(double) * g->ninst, b->stream));
+        std::vector<double> c(g->ninst);
+        for (int i = 0; i < b->n; ++i)
+            if (b->inst_group[i] == (int)gi) c[b->inst_local[i]] = courant[i];
+        HIPCHK(hipMemcpy(g->cfl, c.data(), sizeof(double) * g->ninst, hipMemcpyHostToDevice));
+    }
+    if (!b->graph[0]) {
+        batch_destroy_graphs(b);
+        for (int par = 0; par < 2; ++par) {
+            hipGraph_t gr = nullptr;
+            HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            int le = PLENS_OK;
+            for (plens_gpu* g : b->groups)
+                if (!le) le = launch_step(g, par == 1, 0.0);  // Courant numbers from the groups' tables
+            hipError_t ce = hipStreamEndCapture(b->stream, &gr);
+            if (le) {
+                if (gr) (void)hipGraphDestroy(gr);
+                return le;
+            }
+            if (ce != hipSuccess) return PLENS_E_HIP;
+            hipError_t ie = hipGraphInstantiate(&b->graph[par], gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (ie != hipSuccess) return PLENS_E_HIP;
+        }
+    }
+    if (!b->ev[0]) {
+        HIPCHK(hipEventCreate(&b->ev[0]));
+        HIPCHK(hipEventCreate(&b->ev[1]));
+    }
+    HIPCHK(hipEventRecord(b->ev[0], b->stream));
+    for (int s = 0; s < n_steps; ++s) {
+        HIPCHK(hipGraphLaunch(b->graph[b->state_in_b ? 1 : 0], b->stream));
+        b->state_in_b = !b->state_in_b;
+    }
+    HIPCHK(hipEventRecord(b->ev[1], b->stream));
+    HIPCHK(hipEventSynchronize(b->ev[1]));
+    {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+        b->last_run_ms = ms;
+    }
+    for (plens_gpu* g : b->groups) g->state_in_b = b->state_in_b;
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+        plens_gpu* g = b->groups[gi];
+        const int e = read_failure(g);
+        if (e == PLENS_E_POSITIVITY) {
+            for (int i = 0; i < b->n; ++i)  // map the merged DoF back to its instance
+                if (b->inst_group[i] == (int)gi && g->fail_dof >= b->inst_dof0[i] &&
+                    g->fail_dof < b->inst_dof0[i] + b->inst_ndof[i]) {
+                    b->fail_inst = i;
+                    b->fail_dof = g->fail_dof - b->inst_dof0[i];
+                    b->fail_kind = g->fail_kind;
+                }
+            return e;
+        }
+        if (e) return e;
+    }
+    if (t_inout) {
+        for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+            plens_gpu* g = b->groups[gi];
+            std::vector<double> t(g->ninst);
+            HIPCHK(hipMemcpy(t.data(), g->scal + g->ninst, sizeof(double) * g->ninst, hipMemcpyDeviceToHost));
+            for (int i = 0; i < b->n; ++i)
+                if (b->inst_group[i] == (int)gi) t_inout[i] += t[b->inst_local[i]];
+        }
+    }
+    return PLENS_OK;
+}
+
+int plens_gpu_batch_get_state(const plens_batch* b, int32_t i, double* Q) {
+    if (!b || !Q || i < 0 || i >= b->n) return PLENS_E_INVALID;
+    const plens_gpu* g = b->groups[b->inst_group[i]];
+    const double* src = g->state_in_b ? g->Qb : g->Qa;
+    for (int v = 0; v < 5; ++v)
+        HIPCHK(hipMemcpy(Q + v * b->inst_ndof[i], src + v * g->ndof + b->inst_dof0[i],
+                         sizeof(double) * b->inst_ndof[i], hipMemcpyDeviceToHost));
+    return PLENS_OK;
+}
+
+int plens_gpu_batch_last_run_ms(const plens_batch* b, double* ms) {
+    if (!b || !ms) return PLENS_E_INVALID;
+    *ms = b->last_run_ms;
+    return PLENS_OK;
+}
+
+int plens_gpu_batch_last_failure(const plens_batch* b, int32_t* instance, int64_t* dof, int32_t* kind) {
+    if (!b) return PLENS_E_INVALID;
+    if (instance) *instance = b->fail_inst;
+    if (dof) *dof = b->fail_dof;
+    if (kind) *kind = b->fail_kind;
+    return PLENS_OK;
+}
+
+int plens_gpu_batch_n_groups(const plens_batch* b) { return b ? (int)b->groups.size() : PLENS_E_INVALID; }
+
+int plens_gpu_batch_group_info(const plens_batch* b, int32_t group, int32_t* degree, int32_t* n_instances,
+                               int64_t* n_dof, int32_t* cells_per_workgroup, int32_t* face_workgroup) {
+    if (!b || group < 0 || group >= (int)b->groups.size()) return PLENS_E_INVALID;
+    const plens_gpu* g = b->groups[group];
+    if (degree) *degree = g->N;
+    if (n_instances) *n_instances = g->ninst;
+    if (n_dof) *n_dof = g->ndof;
+    if (cells_per_workgroup) *cells_per_workgroup = g->cpb;
+    if (face_workgroup) *face_workgroup = g->face_threads;
     return PLENS_OK;
 }
 
