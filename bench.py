@@ -1,7 +1,14 @@
 """Benchmark of the MI355X DGSEM hot path: DOF-stage-updates/s at N=4 on the curvilinear viscous
 wall-flow workload (128x64x2 hex cells, N=4, 2,048,000 DoF x 5 vars, Mach 6, Re 1e5, isothermal
-wall), FP64. One step = dt reduction + 5 x (calc_rhs + low-storage RK update), replayed from a HIP
-graph with the state resident in HBM.
+wall). One step = dt reduction + 5 x (calc_rhs + low-storage RK update), replayed from a HIP graph
+with the state resident in HBM.
+
+`value` is the plain FP64 path. `variants` runs the same workload and step count from the same
+initial state in the FP32-bulk arithmetic and in FP32-bulk with the learned troubled-cell indicator
+(the network of plens_amd/indicator_mlp.npy) and reports each one's rate, its ratio to the plain
+rate and its solution relative L2 difference from the plain FP64 state after the same steps.
+`batched` advances the 96 seeded uneven instances (degrees 2-5, 4^3-32^3 cells) 50 steps, once
+through one graph launch per step for all of them and once instance by instance.
 
 Prints ONE JSON line. `roofline` is for the dominant kernel (volume + RK epilogue) from HIP-event
 durations on the handle's stream; `cpu_baseline` times the CPU oracle on a bounded sample of the
@@ -59,6 +66,84 @@ def cpu_baseline(case, gpu_state_after_1, seconds_target=20.0):
                       f"OpenMP over cells"}, err
 
 
+def fp64_sensitivity(case, args, plain_state):
+    """relative L2 change of the plain FP64 state after warmup + steps when the initial state is
+    perturbed by a relative 2^-24 (FP32 round-off) and 1e-15 (FP64 round-off) uniform noise: the
+    spread that the workload itself puts on any change of arithmetic"""
+    from plens_amd import GpuPlens
+    rng = np.random.default_rng(3)
+    out = {}
+    for label, eps in (("input_perturbation_2^-24", 2.0 ** -24), ("input_perturbation_1e-15", 1e-15)):
+        g = GpuPlens(case)
+        g.set_state(case.Q0 * (1.0 + eps * rng.uniform(-1.0, 1.0, case.Q0.shape)))
+        g.run(args.warmup + args.steps, case.courant)
+        out[label] = float(np.linalg.norm(g.get_state() - plain_state) / np.linalg.norm(plain_state))
+        g.close()
+    return out
+
+
+def run_variants(case, args, plain_value, plain_state):
+    """the same workload, steps and initial state in the FP32-bulk arithmetic, without and with the
+    learned indicator; each variant's state difference from the plain FP64 state is set beside the
+    FP64 path's own spread under an FP32-sized perturbation of its input"""
+    from plens_amd import GpuPlens, with_precision
+    from plens_amd.indicator_study import load_trained
+    from plens_amd.learned import stats, use_network
+    net = load_trained()
+    sens = fp64_sensitivity(case, args, plain_state)
+    print(f"[bench] FP64 spread under input perturbations: {sens}", file=sys.stderr, flush=True)
+    out = {"fp64_sensitivity": sens}
+    for label, prec, learned in (("fp32_bulk", "fp32_bulk", False), ("fp32_bulk+learned", "fp32_bulk", True),
+                                 ("fp64+learned", "fp64", True)):
+        g = GpuPlens(with_precision(case, prec))
+        if learned:
+            use_network(g, net)
+        g.run(args.warmup, case.courant)
+        if learned:
+            stats(g, reset=True)
+        g.run(args.steps, case.courant)
+        rate = case.mesh.n_dof * 5 * args.steps / (g.last_run_ms() * 1e-3)
+        err = float(np.linalg.norm(g.get_state() - plain_state) / np.linalg.norm(plain_state))
+        rec = {"value": rate, "vs_plain_fp64": rate / plain_value, "ms_per_step": g.last_run_ms() / args.steps,
+               "rel_l2_vs_plain_fp64": err,
+               "rel_l2_over_fp64_spread": err / max(sens["input_perturbation_2^-24"], 1e-300)}
+        if learned:
+            ev, fb = stats(g)
+            rec["indicator_fallback_fraction"] = fb / max(ev, 1)
+        out[label] = rec
+        print(f"[bench] {label}: {rate:.4e} DOF-stage/s ({rate / plain_value:.3f}x), rel L2 vs FP64 {err:.2e}",
+              file=sys.stderr, flush=True)
+        g.close()
+    return out
+
+
+def run_batched(n_instances=96, steps=50):
+    """96 seeded uneven instances, 50 steps: one graph launch per step for all vs one by one"""
+    from plens_amd import GpuBatch, GpuPlens, batch_instance
+    t0 = time.perf_counter()
+    cases = [batch_instance(i) for i in range(n_instances)]
+    ndof = sum(c.mesh.n_dof for c in cases)
+    cfl = [c.courant for c in cases]
+    print(f"[bench] batched: {n_instances} instances, {ndof} DoF, built in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    seq_ms = 0.0
+    seq = [GpuPlens(c) for c in cases]
+    for g, c in zip(seq, cases):
+        g.run(steps, c.courant)
+        seq_ms += g.last_run_ms()
+    tuning = os.path.join(ROOT, "plens_amd", "tuning.txt")
+    b = GpuBatch([GpuPlens(c) for c in cases], tuning_db=tuning)
+    b.run(steps, cfl)
+    bms = b.last_run_ms()
+    same = all(np.array_equal(b.get_state(i), seq[i].get_state()) for i in range(n_instances))
+    out = {"instances": n_instances, "steps": steps, "n_dof": ndof,
+           "value": ndof * 5 * steps / (bms * 1e-3), "sequential_value": ndof * 5 * steps / (seq_ms * 1e-3),
+           "speedup_vs_sequential": seq_ms / bms, "bitwise_equal_to_sequential": same, "groups": b.groups()}
+    print(f"[bench] batched {out['value']:.4e} vs sequential {out['sequential_value']:.4e} DOF-stage/s",
+          file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +152,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cells-per-block", type=int, default=0)
+    ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the multi-GPU sharding of this path is a design note, "
@@ -92,6 +179,7 @@ def main():
     ms_per_step = dev_ms / args.steps
     value = n_dof * 5 * args.steps / (dev_ms * 1e-3)
     print(f"[bench] {args.steps} steps: device {dev_ms:.2f} ms, wall {wall * 1e3:.2f} ms", file=sys.stderr, flush=True)
+    final_plain = g.get_state()  # state after warmup + steps, the reference of the variants
 
     _, kt = g.run_profiled(args.profile_steps, case.courant)
     names = ["grad", "alpha", "face", "lo", "vol"]
@@ -120,10 +208,12 @@ def main():
     L.check(L.load().plens_gpu_stream_copy(copy_bytes, 10, C.byref(cms)), "stream_copy")
     roofline["stream_copy_GBs"] = 2 * copy_bytes / (cms.value * 1e-3) / 1e9
 
+    g.close()
+    variants = None if args.no_variants else run_variants(case, args, value, final_plain)
+    batched = None if args.no_batch else run_batched()
     cpu, parity = (None, None)
     if not args.no_cpu:
         cpu, parity = cpu_baseline(case, state1)
-    g.close()
     out = {
         "metric": METRIC, "value": value, "unit": "DOF-stage-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
@@ -131,6 +221,7 @@ def main():
         "config": {"workload": "wall_flow 128x64x2 hex, N=4, 2,048,000 DoF x 5, viscous, HLLC, isothermal wall",
                    "n_dof": n_dof, "degree": N, "variant": "plain FP64"},
         "roofline": roofline, "cpu_baseline": cpu, "parity_1step_rel_l2_vs_oracle": parity,
+        "variants": variants, "batched": batched,
     }
     print(json.dumps(out))
 

@@ -35,9 +35,12 @@ int plens_mlp_train(plens_mlp* m, const float* X, const float* target, int64_t n
                     float lr, uint64_t seed, float* epoch_loss);
 
 /* Solver side: use the network (device parameters from plens_mlp_device_bf16_params, which must
- * outlive its use; NULL = plain indicator) for the cell blender value; wherever
- * |alpha_mlp - alpha_plain| > tolerance the plain value is used. N = 3 and 4 only (plain otherwise). */
-int plens_gpu_set_learned_indicator(plens_gpu* h, const float* mlp_params_device, double tolerance);
+ * outlive its use; NULL = plain indicator) for the cell blender value. Network values below `floor`
+ * count as 0 (the network's own alpha_min) and are capped at alpha_max; wherever
+ * |alpha_mlp - alpha_plain| > tolerance the plain value is used. Degrees 3 and 4 only
+ * (PLENS_E_INVALID otherwise). No counterpart in the reference, whose blender value is always
+ * BlenderCalculator::get_blender (dgsem/blender_calculator.cc:123-187). */
+int plens_gpu_set_learned_indicator(plens_gpu* h, const float* mlp_params_device, double tolerance, double floor);
 /* cells evaluated by the network and fallbacks to the plain value since the last reset */
 int plens_gpu_indicator_stats(plens_gpu* h, int64_t* evaluated, int64_t* fallbacks, int32_t reset);
 /* features [n_cells][125] and cell blender values [n_cells] (before wall override and diffusion) of

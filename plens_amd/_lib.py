@@ -142,7 +142,7 @@ def load():
         "plens_mlp_grad": (C.c_int, [vp, _fp, _fp, C.c_int32, _fp, _fp]),
         "plens_mlp_step": (C.c_int, [vp, _fp, _fp, C.c_int32, C.c_float]),
         "plens_mlp_train": (C.c_int, [vp, _fp, _fp, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _fp]),
-        "plens_gpu_set_learned_indicator": (C.c_int, [vp, C.c_void_p, C.c_double]),
+        "plens_gpu_set_learned_indicator": (C.c_int, [vp, C.c_void_p, C.c_double, C.c_double]),
         "plens_gpu_indicator_stats": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_int32]),
         "plens_gpu_indicator_samples": (C.c_int, [vp, _fp, _dp]),
     }

@@ -91,6 +91,7 @@ struct KP {
     int* bcount;
     const float* mlp;        // BF16-rounded MLP parameters (nullptr: plain indicator)
     double mlp_tol;          // fallback threshold |alpha_mlp - alpha_plain|
+    double mlp_floor;        // network values below this are taken as 0
     unsigned long long* mlp_stats;  // [0] cells evaluated, [1] fallbacks
     float* feat;             // optional features [n_cells][125] of the indicator
     double* dt_cell;         // stage-1 time-step candidate of every cell
@@ -325,44 +326,11 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
         a_plain = (at < p.amin) ? 0.0 : (at < p.amax ? at : p.amax);
         p.araw[c] = a_plain;
     }
-    // ---- learned indicator: features x_k = m_k / m_0 zero-padded to 125; MLP 125-64-64-1
-    //      (GELU, 0.5 sigmoid) on the BF16-rounded weights; plain value kept where they disagree
+    // ---- learned indicator features x_k = m_k / m_0 zero-padded to 125 (the network runs in k_mlp)
     if (p.feat && act) {
         const double m0 = sT[lc * NN];
         for (int k = node; k < kMlpIn; k += NN)
             p.feat[(long long)c * kMlpIn + k] = (k < NN) ? (float)(sT[lc * NN + k] / m0) : 0.0f;
-    }
-    if (p.mlp && N >= 3 && N <= 4) {
-        float* sH = reinterpret_cast<float*>(sV) + lc * 2 * kMlpH;  // 128 floats per cell (NN >= 64 doubles)
-        const float* W = p.mlp;
-        if (inblk && node < kMlpH) {
-            const double m0 = sT[lc * NN];
-            float acc = W[kMlpB1 + node];
-#pragma unroll 2
-            for (int k = 0; k < NN; ++k) acc = fmaf((float)(sT[lc * NN + k] / m0), W[node * kMlpIn + k], acc);
-            sH[node] = mlp_gelu(acc);
-        }
-        __syncthreads();
-        if (inblk && node < kMlpH) {
-            float acc = W[kMlpB2 + node];
-#pragma unroll 2
-            for (int k = 0; k < kMlpH; ++k) acc = fmaf(sH[k], W[kMlpW2 + node * kMlpH + k], acc);
-            sH[kMlpH + node] = mlp_gelu(acc);
-        }
-        __syncthreads();
-        if (act && node == 0) {
-            float acc = W[kMlpB3];
-#pragma unroll 2
-            for (int k = 0; k < kMlpH; ++k) acc = fmaf(sH[kMlpH + k], W[kMlpW3 + k], acc);
-            double a = (double)(0.5f / (1.0f + expf(-acc)));
-            a = (a < p.amin) ? 0.0 : (a < p.amax ? a : p.amax);
-            atomicAdd(&p.mlp_stats[0], 1ull);
-            if (fabs(a - a_plain) > p.mlp_tol) {
-                atomicAdd(&p.mlp_stats[1], 1ull);
-                a = a_plain;
-            }
-            p.araw[c] = a;
-        }
     }
 
     // ---- stage-1 time step candidates (PLENS::calc_time_step, dgsem/plens.cc:3836-3866)
@@ -475,6 +443,83 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
 // =============================================================================================
 // k_alpha
 // =============================================================================================
+// =============================================================================================
+// k_mlp: learned troubled-cell indicator on kMlpCells cells per workgroup. MLP 125-64-64-1, GELU,
+// 0.5 sigmoid, BF16-rounded weights, FP32 accumulation in input order; the network value (zero
+// below mlp_floor, capped at alpha_max) replaces the plain blender value unless they differ by more
+// than mlp_tol. Transposed tiles in LDS: X^T[k][cell], W1^T[k][h], H1^T[k][cell], W2^T[k][h]; each
+// thread accumulates 4 cells x 4 hidden units.
+// =============================================================================================
+constexpr int kMlpCells = 64;
+__global__ void __launch_bounds__(256) k_mlp(KP p) {
+    __shared__ float sX[kMlpIn * kMlpCells];   // X^T, later H2^T (64 x 64)
+    __shared__ float sW[kMlpIn * kMlpH];       // W1^T, later W2^T
+    __shared__ float sH[kMlpH * kMlpCells];    // H1^T
+    __shared__ unsigned long long sCnt[2];
+    const int tid = threadIdx.x;
+    const int c0 = blockIdx.x * kMlpCells;
+    const int ncell = min(kMlpCells, p.nc - c0);
+    const float* W = p.mlp;
+    if (tid < 2) sCnt[tid] = 0;
+    for (int e = tid; e < kMlpCells * kMlpIn; e += 256) {
+        const int cl = e / kMlpIn, k = e - cl * kMlpIn;
+        sX[k * kMlpCells + cl] = cl < ncell ? p.feat[(long long)(c0 + cl) * kMlpIn + k] : 0.0f;
+    }
+    for (int e = tid; e < kMlpH * kMlpIn; e += 256) {
+        const int h = e / kMlpIn, k = e - h * kMlpIn;
+        sW[k * kMlpH + h] = W[e];
+    }
+    __syncthreads();
+    const int hg = (tid & 15) * 4, cg = (tid >> 4) * 4;
+    float acc[4][4];
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) acc[a][b] = W[kMlpB1 + hg + b];
+    for (int k = 0; k < kMlpIn; ++k) {
+        const float4 x = *reinterpret_cast<const float4*>(&sX[k * kMlpCells + cg]);
+        const float4 w = *reinterpret_cast<const float4*>(&sW[k * kMlpH + hg]);
+        const float xv[4] = {x.x, x.y, x.z, x.w}, wv[4] = {w.x, w.y, w.z, w.w};
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) acc[a][b] = fmaf(xv[a], wv[b], acc[a][b]);
+    }
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) sH[(hg + b) * kMlpCells + cg + a] = mlp_gelu(acc[a][b]);
+    __syncthreads();
+    for (int e = tid; e < kMlpH * kMlpH; e += 256) {
+        const int h = e / kMlpH, k = e - h * kMlpH;
+        sW[k * kMlpH + h] = W[kMlpW2 + e];
+    }
+    __syncthreads();
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) acc[a][b] = W[kMlpB2 + hg + b];
+    for (int k = 0; k < kMlpH; ++k) {
+        const float4 x = *reinterpret_cast<const float4*>(&sH[k * kMlpCells + cg]);
+        const float4 w = *reinterpret_cast<const float4*>(&sW[k * kMlpH + hg]);
+        const float xv[4] = {x.x, x.y, x.z, x.w}, wv[4] = {w.x, w.y, w.z, w.w};
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) acc[a][b] = fmaf(xv[a], wv[b], acc[a][b]);
+    }
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) sX[(hg + b) * kMlpCells + cg + a] = mlp_gelu(acc[a][b]);
+    __syncthreads();
+    if (tid < ncell) {
+        const int c = c0 + tid;
+        float z = W[kMlpB3];
+        for (int k = 0; k < kMlpH; ++k) z = fmaf(sX[k * kMlpCells + tid], W[kMlpW3 + k], z);
+        double a = (double)(0.5f / (1.0f + expf(-z)));
+        a = (a < p.mlp_floor) ? 0.0 : (a < p.amax ? a : p.amax);
+        const double a_plain = p.araw[c];
+        const bool fb = fabs(a - a_plain) > p.mlp_tol;
+        p.araw[c] = fb ? a_plain : a;
+        atomicAdd(&sCnt[0], 1ull);
+        if (fb) atomicAdd(&sCnt[1], 1ull);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        atomicAdd(&p.mlp_stats[0], sCnt[0]);
+        if (sCnt[1]) atomicAdd(&p.mlp_stats[1], sCnt[1]);
+    }
+}
+
 __global__ void __launch_bounds__(kThreadsAlpha) k_alpha(KP p) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < p.nc) {
@@ -1073,9 +1118,14 @@ int launch_grad_p(plens_gpu* h, const KP& kp) {
     return PLENS_OK;
 }
 
-// k_grad alone (time-step candidates, blender indicator, avars) for the precision of the handle
+// k_grad (time-step candidates, blender indicator, avars) for the precision of the handle, followed
+// by the network of the learned indicator when one is set
 int launch_grad(plens_gpu* h, const KP& kp) {
-    return h->precision == PLENS_PREC_FP32_BULK ? launch_grad_p<float>(h, kp) : launch_grad_p<double>(h, kp);
+    const int e = h->precision == PLENS_PREC_FP32_BULK ? launch_grad_p<float>(h, kp) : launch_grad_p<double>(h, kp);
+    if (e || !kp.mlp || !kp.feat) return e;
+    hipLaunchKernelGGL(k_mlp, dim3((kp.nc + kMlpCells - 1) / kMlpCells), dim3(256), 0, h->stream, kp);
+    HIPCHK(hipGetLastError());
+    return PLENS_OK;
 }
 
 template <int N, typename TF>
@@ -1094,6 +1144,10 @@ int launch_stage_t(plens_gpu* h, const KP& kp, hipEvent_t* ev) {
     if (ev) HIPCHK(hipEventRecord(ev[0], h->stream));
     hipLaunchKernelGGL((k_grad<N, TF>), dim3(blocks), dim3(thr), lds_grad, h->stream, kp);
     HIPCHK(hipGetLastError());
+    if (kp.mlp && kp.feat) {
+        hipLaunchKernelGGL(k_mlp, dim3((kp.nc + kMlpCells - 1) / kMlpCells), dim3(256), 0, h->stream, kp);
+        HIPCHK(hipGetLastError());
+    }
     if (ev) HIPCHK(hipEventRecord(ev[1], h->stream));
     hipLaunchKernelGGL(k_alpha, dim3((kp.nc + kThreadsAlpha - 1) / kThreadsAlpha), dim3(kThreadsAlpha), 0,
                        h->stream, kp);
@@ -1365,7 +1419,7 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
 
     KP& k = h->base;
     bind_buffers(h);
-    k.mlp = nullptr; k.mlp_tol = 0.05;
+    k.mlp = nullptr; k.mlp_tol = 0.05; k.mlp_floor = ph->alpha_min;
     k.gas.gma = ph->gamma; k.gas.R = R; k.gas.cv = R / (ph->gamma - 1); k.gas.mu0 = ph->mu0; k.gas.T0 = ph->T0;
     k.gas.S = ph->S; k.gas.Pr = ph->prandtl; k.gas.inviscid = ph->mu0 < 1e-16;
     k.scheme = ph->inv_surf_flux;
@@ -1594,10 +1648,17 @@ int plens_gpu_stream_copy(int64_t bytes, int32_t iters, double* ms_per_copy) {
     return rc;
 }
 
-int plens_gpu_set_learned_indicator(plens_gpu* h, const float* mlp_params_device, double tolerance) {
-    if (!h || tolerance < 0) return PLENS_E_INVALID;
+int plens_gpu_set_learned_indicator(plens_gpu* h, const float* mlp_params_device, double tolerance,
+                                    double floor) {
+    if (!h || !(tolerance >= 0) || !(floor >= 0)) return PLENS_E_INVALID;
+    if (mlp_params_device && (h->N < 3 || h->N > 4)) return PLENS_E_INVALID;  // features are 125 modes
+    if (mlp_params_device && !h->feat &&
+        hipMalloc((void**)&h->feat, sizeof(float) * (size_t)h->nc * kMlpIn) != hipSuccess)
+        return PLENS_E_NOMEM;
     h->base.mlp = mlp_params_device;
     h->base.mlp_tol = tolerance;
+    h->base.mlp_floor = floor;
+    h->base.feat = mlp_params_device ? h->feat : nullptr;
     destroy_graphs(h);
     return PLENS_OK;
 }
@@ -1754,8 +1815,18 @@ int merge_group(const std::vector<plens_gpu*>& mem, hipStream_t stream, plens_gp
     h->base = m0->base;
     bind_buffers(h);
     h->base.cfl = h->cfl;
+    for (const plens_gpu* m : mem)
+        if (m->base.mlp != m0->base.mlp || m->base.mlp_tol != m0->base.mlp_tol ||
+            m->base.mlp_floor != m0->base.mlp_floor)
+            return fail(PLENS_E_INVALID);
     h->base.mlp = m0->base.mlp;
     h->base.mlp_tol = m0->base.mlp_tol;
+    h->base.mlp_floor = m0->base.mlp_floor;
+    h->base.feat = nullptr;
+    if (h->base.mlp) {
+        if ((rc = dmalloc(&h->feat, (size_t)h->nc * kMlpIn))) return fail(rc);
+        h->base.feat = h->feat;
+    }
     *out = h;
     return PLENS_OK;
 }

@@ -126,10 +126,13 @@ def collect(case, steps, every=10):
     return np.concatenate(Xs), np.concatenate(ys).astype(np.float32)
 
 
-def use_network(solver, mlp, tolerance=0.05):
-    """Switch a solver handle to the network (mlp=None: back to the plain indicator)."""
+def use_network(solver, mlp, tolerance=0.05, floor=0.02):
+    """Switch a solver handle to the network (mlp=None: back to the plain indicator). Network values
+    below `floor` count as 0, so that small positive outputs in smooth cells do not turn on the
+    subcell blending; |alpha_mlp - alpha_plain| > tolerance falls back to the plain value."""
     ptr = mlp.device_params() if mlp is not None else None
-    solver._check(L.load().plens_gpu_set_learned_indicator(solver._h, ptr, tolerance), "set_learned_indicator")
+    solver._check(L.load().plens_gpu_set_learned_indicator(solver._h, ptr, tolerance, floor),
+                  "set_learned_indicator")
 
 
 def stats(solver, reset=True):
