@@ -29,25 +29,41 @@ METRIC = "DOF-stage-updates/s at N=4: plain FP64 vs mixed+learned, at equal L2 e
 HBM_PEAK_GBS = 8000.0
 
 
-def kernel_bytes_per_dof(N, viscous=True):
-    """Algorithmic HBM bytes per DoF of one launch of each kernel class (compulsory traffic)."""
+def kernel_bytes_per_dof(N, viscous=True, fb=8):
+    """Algorithmic HBM bytes per DoF of one launch of each kernel class (compulsory traffic); fb is
+    the byte size of the flux-arithmetic arrays (avars and the finished face terms: 8 for FP64, 4 for
+    FP32-bulk); states, metrics and residual registers are FP64 in every variant."""
     n, nf = (N + 1) ** 3, (N + 1) ** 2
     face_per_dof = 6.0 * nf / n  # face nodes per cell node
+    v = 1 if viscous else 0
     b = {}
     # k_grad: read Q (5), Ja (9), detJ (1); write avars (9, viscous)
-    b["grad"] = 8 * (5 + 9 + 1 + (9 if viscous else 0))
+    b["grad"] = 8 * (5 + 9 + 1) + fb * 9 * v
     b["alpha"] = 0.0
-    # k_lo: blended cells only (none in the default state); early-exit blocks read alpha
+    # k_lo: blended cells only; not counted
     b["lo"] = 0.0
-    # k_face: per face node read both states (2x5) and avars (2x9), normal (3); write both sides' fluxes
-    # (2 x 2 x 5); amortised per DoF (each face node shared by two cells)
-    b["face"] = 8 * face_per_dof / 2 * (10 + (18 if viscous else 0) + 3 + 10 * (2 if viscous else 1))
-    # k_vol: read Q (5), Ja (9), detJ (1), avars (9), stored face fluxes (2 x 5 per face node),
-    # R' and R'' (stage average 4/5 and 3/5 of 5); write Q (5), R (5); the subcell residual of
-    # blended cells is not counted (none at the measured state)
-    b["vol"] = 8 * (5 + 9 + 1 + (9 if viscous else 0) + face_per_dof * 5 * (2 if viscous else 1)
-                    + 5 * 4 / 5 + 5 * 3 / 5 + 10)
+    # k_face, per face node (each shared by two cells): read both states (2x5), the normal (3), both
+    # sides' Ja rows (2x3) and avars (2x9, viscous); write both sides' finished terms (2x5 inviscid,
+    # 2x5 viscous)
+    b["face"] = face_per_dof / 2 * (8 * (10 + 3 + 6) + fb * (18 * v + 10 * (1 + v)))
+    # k_vol: read Q (5), Ja (9), detJ (1), avars (9, viscous), the finished face terms of its face
+    # nodes (5 inviscid + 5 viscous), R' and R'' (stage averages 4/5 and 3/5 of 5); write Q (5), R (5)
+    b["vol"] = 8 * (5 + 9 + 1 + 5 * 4 / 5 + 5 * 3 / 5 + 10) + fb * (9 * v + face_per_dof * 5 * (1 + v))
     return b
+
+
+def measured_traffic(precision, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary of this workload
+    (calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE passes), or None"""
+    path = os.path.join(ROOT, "profiles", f"wall_flow_N4_{precision}_kernels.md")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                return float(parts[2]), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def cpu_baseline(case, gpu_state_after_1, seconds_target=20.0):
@@ -154,14 +170,16 @@ def main():
     ap.add_argument("--cells-per-block", type=int, default=0)
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_bulk"],
+                    help="arithmetic of the main leg (fp64 is the metric's plain path; fp32_bulk for profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the multi-GPU sharding of this path is a design note, "
               "not a built path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    from plens_amd import GpuPlens, wall_flow
-    case = wall_flow()
+    from plens_amd import GpuPlens, wall_flow, with_precision
+    case = with_precision(wall_flow(), args.precision)
     N, n_dof = case.mesh.degree, case.mesh.n_dof
     print(f"[bench] workload wall_flow {case.meta['cells']} N={N} n_dof={n_dof}", file=sys.stderr, flush=True)
 
@@ -181,23 +199,32 @@ def main():
     print(f"[bench] {args.steps} steps: device {dev_ms:.2f} ms, wall {wall * 1e3:.2f} ms", file=sys.stderr, flush=True)
     final_plain = g.get_state()  # state after warmup + steps, the reference of the variants
 
+    # per-kernel durations: every class bracketed by HIP events in a stepwise run (includes the
+    # event and launch gaps), and the gradient, face and volume kernels launched back to back with
+    # the parameters of the 5 stages in turn (one launch's device time, as a kernel trace reports it)
     _, kt = g.run_profiled(args.profile_steps, case.courant)
     names = ["grad", "alpha", "face", "lo", "vol"]
-    kb = kernel_bytes_per_dof(N, viscous=True)
+    b2b = {k: g.time_kernel(k, reps=10) for k in ("grad", "face", "vol")}
+    kb = kernel_bytes_per_dof(N, viscous=True, fb=8 if args.precision == "fp64" else 4)
     kernels = {}
     for i, k in enumerate(names):
-        ms = kt[i]
+        ms = b2b.get(k, kt[i])
         bytes_launch = kb[k] * n_dof
-        kernels[k] = {"avg_ms": ms, "alg_bytes": bytes_launch,
+        kernels[k] = {"avg_ms": ms, "avg_ms_event_bracketed": kt[i], "alg_bytes": bytes_launch,
                       "achieved_GBs": bytes_launch / (ms * 1e-3) / 1e9 if ms > 0 else None}
     dom = max(names, key=lambda k: kernels[k]["avg_ms"])
     ach = kernels[dom]["achieved_GBs"]
-    stage_ms = float(sum(kt))
-    roofline = {"bound": "hbm", "kernel": f"k_{dom}", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": ach / HBM_PEAK_GBS, "traffic": None,
-                "stage": {"alg_bytes_per_dof": 400.0, "stage_ms": stage_ms,
-                          "achieved_GBs": 400.0 * n_dof / (stage_ms * 1e-3) / 1e9,
-                          "frac": 400.0 * n_dof / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+    stage_ms = float(sum(kernels[k]["avg_ms"] for k in names))
+    fb = 8 if args.precision == "fp64" else 4
+    stage_bytes = 8 * 32 + fb * 18  # the two-pass stage's compulsory bytes per DoF (viscous)
+    tname = f"k_{dom}<{N},{'double' if args.precision == 'fp64' else 'float'}>"
+    traffic, tsrc = measured_traffic(args.precision, tname)
+    roofline = {"bound": "hbm", "kernel": tname, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": tsrc,
+                "alg_bytes_per_launch": kernels[dom]["alg_bytes"],
+                "stage": {"alg_bytes_per_dof": stage_bytes, "stage_ms": stage_ms,
+                          "achieved_GBs": stage_bytes * n_dof / (stage_ms * 1e-3) / 1e9,
+                          "frac": stage_bytes * n_dof / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
                 "kernels": kernels}
 
     # achievable HBM bandwidth on this device: streaming copy of 1 GiB (2 GiB of traffic per copy)
@@ -217,9 +244,10 @@ def main():
     out = {
         "metric": METRIC, "value": value, "unit": "DOF-stage-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded mesh perturbation, uniform Mach-6 inflow IC)",
+        "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic (seeded mesh perturbation, uniform Mach-6 inflow IC)",
         "config": {"workload": "wall_flow 128x64x2 hex, N=4, 2,048,000 DoF x 5, viscous, HLLC, isothermal wall",
-                   "n_dof": n_dof, "degree": N, "variant": "plain FP64"},
+                   "n_dof": n_dof, "degree": N,
+                   "variant": "plain FP64" if args.precision == "fp64" else "FP32-bulk"},
         "roofline": roofline, "cpu_baseline": cpu, "parity_1step_rel_l2_vs_oracle": parity,
         "variants": variants, "batched": batched,
     }

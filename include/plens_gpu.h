@@ -133,6 +133,14 @@ int plens_gpu_set_tuning(plens_gpu* h, int32_t cells_per_block);
  * handle's stream): times_ms[0..4] = gradient/blender, alpha, face flux, subcell FV, volume+RK. */
 int plens_gpu_kernel_times(const plens_gpu* h, double* times_ms, int32_t n);
 
+/* Kernel timing with no counterpart in the reference: after one RHS evaluation of the current
+ * state, launches kernel class `kernel` (0 gradient/blender, 2 face flux, 4 volume + RK update)
+ * back to back on the handle's stream, `reps` times with the parameters of each of the 5 RK stages
+ * in turn (new states go to a scratch buffer), between two HIP events, and returns the average
+ * duration of one launch. The state is not modified; the residual registers are (they are dead
+ * between steps). */
+int plens_gpu_time_kernel(plens_gpu* h, int32_t kernel, int32_t reps, double* avg_ms);
+
 /* Bandwidth probe with no counterpart in the reference: copies `bytes` between two device buffers
  * `iters` times with a streaming kernel; returns the average time per copy (2*bytes of HBM
  * traffic). Used to calibrate hardware byte counters and to state achievable bandwidth. */

@@ -1128,8 +1128,10 @@ int launch_grad(plens_gpu* h, const KP& kp) {
     return PLENS_OK;
 }
 
+// Kernel classes of a stage, in launch order: 0 k_grad (+ k_mlp with a learned indicator),
+// 1 k_alpha, 2 k_face, 3 k_lo, 4 k_vol.
 template <int N, typename TF>
-int launch_stage_t(plens_gpu* h, const KP& kp, hipEvent_t* ev) {
+int launch_one_t(plens_gpu* h, const KP& kp, int kernel) {
     constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
     const int thr = threads_for(kp.cpb, NN);
     const int blocks = (kp.nc + kp.cpb - 1) / kp.cpb;
@@ -1137,50 +1139,70 @@ int launch_stage_t(plens_gpu* h, const KP& kp, hipEvent_t* ev) {
     // dynamic LDS: FP64 blender/modal arrays + flux-type states and metrics (k_grad); flux-type
     // states, metrics and subcell fluxes (k_lo); FP64 rho/beta + flux-type node data, metrics and
     // pair fluxes (k_vol)
-    const size_t lds_grad = lds_grad_bytes<TF>(kp.cpb, NN);
-    const size_t lds_lo = sizeof(TF) * 19 * B;
-    const size_t nitem = (size_t)NP * NP * (NP * (NP - 1) / 2);
-    const size_t lds_vol = sizeof(double) * 2 * B + sizeof(TF) * (9 * B + 5 * std::max((size_t)kp.cpb * nitem, B));
-    if (ev) HIPCHK(hipEventRecord(ev[0], h->stream));
-    hipLaunchKernelGGL((k_grad<N, TF>), dim3(blocks), dim3(thr), lds_grad, h->stream, kp);
-    HIPCHK(hipGetLastError());
-    if (kp.mlp && kp.feat) {
-        hipLaunchKernelGGL(k_mlp, dim3((kp.nc + kMlpCells - 1) / kMlpCells), dim3(256), 0, h->stream, kp);
-        HIPCHK(hipGetLastError());
+    switch (kernel) {
+        case 0:
+            hipLaunchKernelGGL((k_grad<N, TF>), dim3(blocks), dim3(thr), lds_grad_bytes<TF>(kp.cpb, NN), h->stream,
+                               kp);
+            HIPCHK(hipGetLastError());
+            if (kp.mlp && kp.feat)
+                hipLaunchKernelGGL(k_mlp, dim3((kp.nc + kMlpCells - 1) / kMlpCells), dim3(256), 0, h->stream, kp);
+            break;
+        case 1:
+            hipLaunchKernelGGL(k_alpha, dim3((kp.nc + kThreadsAlpha - 1) / kThreadsAlpha), dim3(kThreadsAlpha), 0,
+                               h->stream, kp);
+            break;
+        case 2: {
+            const long long fthreads = (long long)kp.ntasks * NF;
+            const int ft = kp.face_threads;
+            hipLaunchKernelGGL((k_face<N, TF>), dim3((unsigned)((fthreads + ft - 1) / ft)), dim3(ft), 0, h->stream,
+                               kp);
+            break;
+        }
+        case 3:
+            hipLaunchKernelGGL((k_lo<N, TF>), dim3(std::min(blocks, 256)), dim3(thr), sizeof(TF) * 19 * B, h->stream,
+                               kp);
+            break;
+        case 4: {
+            const size_t nitem = (size_t)NP * NP * (NP * (NP - 1) / 2);
+            const size_t lds_vol =
+                sizeof(double) * 2 * B + sizeof(TF) * (9 * B + 5 * std::max((size_t)kp.cpb * nitem, B));
+            hipLaunchKernelGGL((k_vol<N, TF>), dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
+            break;
+        }
+        default:
+            return PLENS_E_INVALID;
     }
-    if (ev) HIPCHK(hipEventRecord(ev[1], h->stream));
-    hipLaunchKernelGGL(k_alpha, dim3((kp.nc + kThreadsAlpha - 1) / kThreadsAlpha), dim3(kThreadsAlpha), 0,
-                       h->stream, kp);
     HIPCHK(hipGetLastError());
-    if (ev) HIPCHK(hipEventRecord(ev[2], h->stream));
-    const long long fthreads = (long long)kp.ntasks * NF;
-    const int ft = kp.face_threads;
-    hipLaunchKernelGGL((k_face<N, TF>), dim3((unsigned)((fthreads + ft - 1) / ft)), dim3(ft), 0, h->stream, kp);
-    HIPCHK(hipGetLastError());
-    if (ev) HIPCHK(hipEventRecord(ev[3], h->stream));
-    hipLaunchKernelGGL((k_lo<N, TF>), dim3(std::min(blocks, 256)), dim3(thr), lds_lo, h->stream, kp);
-    HIPCHK(hipGetLastError());
-    if (ev) HIPCHK(hipEventRecord(ev[4], h->stream));
-    hipLaunchKernelGGL((k_vol<N, TF>), dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
-    HIPCHK(hipGetLastError());
-    if (ev) HIPCHK(hipEventRecord(ev[5], h->stream));
     return PLENS_OK;
 }
 
 template <typename TF>
-int launch_stage_p(plens_gpu* h, const KP& kp, hipEvent_t* ev) {
+int launch_one_p(plens_gpu* h, const KP& kp, int kernel) {
     switch (h->N) {
-        case 1: return launch_stage_t<1, TF>(h, kp, ev);
-        case 2: return launch_stage_t<2, TF>(h, kp, ev);
-        case 3: return launch_stage_t<3, TF>(h, kp, ev);
-        case 4: return launch_stage_t<4, TF>(h, kp, ev);
-        case 5: return launch_stage_t<5, TF>(h, kp, ev);
+        case 1: return launch_one_t<1, TF>(h, kp, kernel);
+        case 2: return launch_one_t<2, TF>(h, kp, kernel);
+        case 3: return launch_one_t<3, TF>(h, kp, kernel);
+        case 4: return launch_one_t<4, TF>(h, kp, kernel);
+        case 5: return launch_one_t<5, TF>(h, kp, kernel);
         default: return PLENS_E_INVALID;
     }
 }
 
+int launch_one(plens_gpu* h, const KP& kp, int kernel) {
+    return h->precision == PLENS_PREC_FP32_BULK ? launch_one_p<float>(h, kp, kernel)
+                                                : launch_one_p<double>(h, kp, kernel);
+}
+
+// One RK stage (or, with kp.stage == 0, one RHS evaluation); ev[0..5] (nullable) bracket the
+// kernel classes.
 int launch_stage(plens_gpu* h, const KP& kp, hipEvent_t* ev = nullptr) {
-    return h->precision == PLENS_PREC_FP32_BULK ? launch_stage_p<float>(h, kp, ev) : launch_stage_p<double>(h, kp, ev);
+    for (int k = 0; k < 5; ++k) {
+        if (ev) HIPCHK(hipEventRecord(ev[k], h->stream));
+        const int e = launch_one(h, kp, k);
+        if (e) return e;
+    }
+    if (ev) HIPCHK(hipEventRecord(ev[5], h->stream));
+    return PLENS_OK;
 }
 
 // Kernel parameters of RK stage s (1..5) of a step that starts from buffer `from_b`.
@@ -1614,6 +1636,40 @@ int plens_gpu_kernel_times(const plens_gpu* h, double* t, int32_t n) {
     if (!h || !t) return PLENS_E_INVALID;
     for (int i = 0; i < n && i < 5; ++i) t[i] = h->ktimes[i];
     return PLENS_OK;
+}
+
+int plens_gpu_time_kernel(plens_gpu* h, int32_t kernel, int32_t reps, double* avg_ms) {
+    if (!h || !avg_ms || reps < 1 || (kernel != 0 && kernel != 2 && kernel != 4)) return PLENS_E_INVALID;
+    int e = plens_gpu_calc_rhs(h, nullptr, nullptr, nullptr);  // every input of the kernel current
+    if (e) return e;
+    // the kernel with the parameters of RK stages 1..5 in turn (the mix of a time step), reading the
+    // current state and writing the stage's new state to a scratch buffer, so the state is kept
+    double* tmp = nullptr;
+    if (hipMalloc((void**)&tmp, sizeof(double) * 5 * h->ndof) != hipSuccess) return PLENS_E_NOMEM;
+    const double* cur = h->state_in_b ? h->Qb : h->Qa;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = PLENS_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = PLENS_E_HIP;
+    for (int r = -1; !rc && r < reps; ++r) {  // r = -1: warm-up
+        if (r == 0 && hipEventRecord(e0, h->stream) != hipSuccess) rc = PLENS_E_HIP;
+        for (int st = 1; !rc && st <= 5; ++st) {
+            KP kp = stage_params(h, st, h->state_in_b, 0.4);
+            kp.Q = cur;
+            kp.Qout = tmp;
+            kp.t_acc = nullptr;
+            rc = launch_one(h, kp, kernel);
+        }
+    }
+    if (!rc && (hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+        rc = PLENS_E_HIP;
+    float ms = 0;
+    if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = PLENS_E_HIP;
+    if (!rc) *avg_ms = ms / (5.0 * reps);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(tmp);
+    return rc;
 }
 
 int plens_gpu_stream_copy(int64_t bytes, int32_t iters, double* ms_per_copy) {

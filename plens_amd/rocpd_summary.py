@@ -5,7 +5,11 @@
 --trace: kernel-trace database -> per-kernel calls / average duration (us).
 --fetch/--write: databases of `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs -> average KiB
 per dispatch per kernel. The streaming copy kernel (k_stream_copy, --copy-bytes per copy, 1 GiB by
-default) calibrates both counters: factor = expected KiB / measured KiB.
+default) calibrates both counters: factor = expected KiB / measured KiB. The calibrated HBM bytes
+per dispatch of every kernel are also printed as `traffic_bytes_per_launch <kernel> <bytes>` lines.
+--pmc: database of any other `rocprofv3 --pmc` run -> average value per dispatch of every counter.
+--flops: with --pmc holding SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 and --trace, the FP64 VALU
+flop rate per kernel (64 lanes per wave instruction, FMA = 2 flops) against the 78.6 TFLOP/s peak.
 """
 import argparse
 import sqlite3
@@ -39,6 +43,8 @@ def main():
     ap.add_argument("--write")
     ap.add_argument("--copy-bytes", type=float, default=float(1 << 30))
     ap.add_argument("--n-dof", type=float, default=2048000.0)
+    ap.add_argument("--pmc")
+    ap.add_argument("--flops", action="store_true")
     a = ap.parse_args()
     if a.trace:
         print("kernel                               calls   avg_us")
@@ -55,6 +61,28 @@ def main():
         for k in sorted(fe):
             f, w = fe[k] * cf, wr.get(k, 0.0) * cw
             print(f"{k:36s} {fe[k]:11.1f} {wr.get(k, 0.0):11.1f}   {(f + w) * 1024 / a.n_dof:8.1f}")
+        print()
+        for k in sorted(fe):
+            f, w = fe[k] * cf, wr.get(k, 0.0) * cw
+            print(f"traffic_bytes_per_launch {k.replace(' ', '')} {(f + w) * 1024:.0f}")
+    if a.pmc:
+        con = sqlite3.connect(a.pmc)
+        names = [r[0] for r in con.execute("select distinct counter_name from counters_collection")]
+        per = {n: counter_stats(a.pmc, n) for n in names}
+        kernels = sorted({k for d in per.values() for k in d})
+        print("\nkernel                               " + " ".join(f"{n:>22s}" for n in names))
+        for k in kernels:
+            print(f"{k:36s} " + " ".join(f"{per[n].get(k, 0.0):22.1f}" for n in names))
+        if a.flops and a.trace:
+            tr = trace_stats(a.trace)
+            print("\nkernel                               FP64 GFLOP/launch   avg_us   FP64 TFLOP/s   frac of 78.6")
+            for k in kernels:
+                fl = 64.0 * (per.get("SQ_INSTS_VALU_ADD_F64", {}).get(k, 0.0) + per.get("SQ_INSTS_VALU_MUL_F64", {}).get(k, 0.0)
+                             + per.get("SQ_INSTS_VALU_TRANS_F64", {}).get(k, 0.0)
+                             + 2.0 * per.get("SQ_INSTS_VALU_FMA_F64", {}).get(k, 0.0))
+                if k in tr and fl > 0:
+                    us = tr[k][1]
+                    print(f"{k:36s} {fl / 1e9:17.3f} {us:8.2f} {fl / (us * 1e-6) / 1e12:14.2f} {fl / (us * 1e-6) / 78.6e12:13.3f}")
 
 
 if __name__ == "__main__":

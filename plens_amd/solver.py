@@ -97,6 +97,14 @@ class GpuPlens:
         self.lib.plens_gpu_kernel_times(self._h, _dptr(times), 5)
         return t.value, times
 
+    def time_kernel(self, kernel, reps=20):
+        """average duration (ms) of one launch of kernel class `kernel` ("grad", "face" or "vol"),
+        launched `reps` times back to back after an RHS evaluation; the state is unchanged"""
+        k = {"grad": 0, "face": 2, "vol": 4}[kernel]
+        ms = C.c_double()
+        self._check(self.lib.plens_gpu_time_kernel(self._h, k, reps, C.byref(ms)), "time_kernel")
+        return ms.value
+
     def set_tuning(self, cells_per_block):
         self._check(self.lib.plens_gpu_set_tuning(self._h, cells_per_block), "set_tuning")
 
