@@ -77,11 +77,11 @@ struct KP {
     double* araw;
     double* alpha;
     double* alpha_d;
-    double* sinv;
-    double* svis;
+    double* sinv;            // raw inviscid surface term of the face nodes of blended cells (k_lo)
+    double* sface;           // surface term of both residuals, weighted by the blend and 1/J
     float* avars_f;  // FP32-bulk storage of avars and the face terms
     float* sinv_f;
-    float* svis_f;
+    float* sface_f;
     double* Rnew;
     const double* R1;
     const double* R2;
@@ -625,20 +625,39 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     const TF sgc = (f & 1) ? TF(-1) : TF(1), sgn = -sgc;
     const long long o = (long long)cf * 5 * NF + q;
     const long long on = ((long long)nb * 6 + (f ^ 1)) * 5 * NF + q;
-    TF* sinv = tsel<TF>(p.sinv, p.sinv_f);
+    TF tc[5], tn[5];  // inviscid surface terms of this side and of the neighbour side
     {
         TF fi[5];
         inv_flux(ut, dirc, gma, fi);
-        for (int v = 0; v < 5; ++v) sinv[o + v * NF] = -sgc * (-sgc * F[v] - fi[v]) * nwc;
+        for (int v = 0; v < 5; ++v) tc[v] = -sgc * (-sgc * F[v] - fi[v]) * nwc;
         if (interior) {
             inv_flux(ugt, dirn, gma, fi);
-            for (int v = 0; v < 5; ++v) sinv[on + v * NF] = -sgn * (-sgn * (-F[v]) - fi[v]) * nwn;
+            for (int v = 0; v < 5; ++v) tn[v] = -sgn * (-sgn * (-F[v]) - fi[v]) * nwn;
         }
     }
-    if (gs.inviscid) return;
+    // blended cells keep the raw inviscid term: k_lo takes the boundary subcell flux from it
+    TF* sinv = tsel<TF>(p.sinv, p.sinv_f);
+    const double ac = p.alpha[c], an = interior ? p.alpha[nb] : 0.0;
+    if (ac > 0.0)
+        for (int v = 0; v < 5; ++v) sinv[o + v * NF] = tc[v];
+    if (interior && an > 0.0)
+        for (int v = 0; v < 5; ++v) sinv[on + v * NF] = tn[v];
+    // each side's contribution to its blended RHS (calc_rhs, dgsem/plens.cc:3753-3771): the
+    // inviscid term scaled by (1 - alpha)/(-J), the viscous one by (1 - alpha_d)/J (the residual
+    // scalings of calc_cell_ho_inv/dif_residual, 3271-3274, 3409-3412)
+    const double Jc = p.detJ[g], Jn = interior ? p.detJ[gn] : 1.0;
+    const TF wic = (TF)((1.0 - ac) / (-1.0 * Jc)), win = (TF)((1.0 - an) / (-1.0 * Jn));
+    TF* sface = tsel<TF>(p.sface, p.sface_f);
+    if (gs.inviscid) {
+        for (int v = 0; v < 5; ++v) sface[o + v * NF] = wic * tc[v];
+        if (interior)
+            for (int v = 0; v < 5; ++v) sface[on + v * NF] = win * tn[v];
+        return;
+    }
+    const TF wvc = (TF)((1.0 - p.alpha_d[c]) / (1.0 * Jc));
+    const TF wvn = interior ? (TF)((1.0 - p.alpha_d[nb]) / (1.0 * Jn)) : TF(0);
     // stage 3: BR1 average of the viscous fluxes (navier_stokes.cc:1644-1652)
     const TF* avars = tsel<TF>(p.avars, p.avars_f);
-    TF* svis = tsel<TF>(p.svis, p.svis_f);
     TF av[9], avg[9];
     for (int a = 0; a < 9; ++a) av[a] = avars[a * p.ndof + g];
     if (interior || periodic) {
@@ -655,10 +674,10 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     for (int v = 0; v < 5; ++v) F[v] = TF(0.5) * (f1[v] + f2[v]);
     TF fi[5];
     dif_flux(ut, av, dirc, fi);
-    for (int v = 0; v < 5; ++v) svis[o + v * NF] = -sgc * (-sgc * F[v] - fi[v]) * nwc;
+    for (int v = 0; v < 5; ++v) sface[o + v * NF] = wic * tc[v] + wvc * (-sgc * (-sgc * F[v] - fi[v]) * nwc);
     if (interior) {
         dif_flux(ugt, avg, dirn, fi);
-        for (int v = 0; v < 5; ++v) svis[on + v * NF] = -sgn * (-sgn * (-F[v]) - fi[v]) * nwn;
+        for (int v = 0; v < 5; ++v) sface[on + v * NF] = win * tn[v] + wvn * (-sgn * (-sgn * (-F[v]) - fi[v]) * nwn);
     }
 }
 
@@ -963,15 +982,13 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
     }
     if (!act) return;
 
-    // ---- surface terms of both high-order residuals, finished per face node by k_face
-    const TF* sinv = tsel<TF>(p.sinv, p.sinv_f);
-    const TF* svis = tsel<TF>(p.svis, p.svis_f);
+    // ---- surface terms of both high-order residuals, finished and weighted per face node by k_face
+    const TF* sface = tsel<TF>(p.sface, p.sface_f);
+    double sf[5] = {0, 0, 0, 0, 0};
     for (int f = 0; f < 6; ++f) {
         if (!on_face(f, ti, tj, tk, N)) continue;
         const long long o = ((long long)c * 6 + f) * 5 * NF + face_dof<N>(f, ti, tj, tk);
-        for (int v = 0; v < 5; ++v) ri[v] += (double)sinv[o + v * NF];
-        if (!gs.inviscid)
-            for (int v = 0; v < 5; ++v) rd[v] += (double)svis[o + v * NF];
+        for (int v = 0; v < 5; ++v) sf[v] += (double)sface[o + v * NF];
     }
     {
         const double jm = 1.0 / (-1.0 * detJ);
@@ -986,8 +1003,8 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
         for (int v = 0; v < 5; ++v) rl[v] = p.rlo[v * p.ndof + g];
     double rhs[5];
     for (int v = 0; v < 5; ++v) {
-        if (!gs.inviscid) rhs[v] = (1.0 - alpha_d) * rd[v] + alpha * rl[v] + (1.0 - alpha) * ri[v];
-        else rhs[v] = alpha * rl[v] + (1.0 - alpha) * ri[v];
+        if (!gs.inviscid) rhs[v] = (1.0 - alpha_d) * rd[v] + alpha * rl[v] + (1.0 - alpha) * ri[v] + sf[v];
+        else rhs[v] = alpha * rl[v] + (1.0 - alpha) * ri[v] + sf[v];
     }
     if (p.stage == 0) {
         for (int v = 0; v < 5; ++v) p.rhs_out[v * p.ndof + g] = rhs[v];
@@ -1035,8 +1052,8 @@ struct plens_gpu {
     BCDev* bcs = nullptr;
     Basis* basis = nullptr;
     double *avars = nullptr, *araw = nullptr, *alpha = nullptr, *alpha_d = nullptr;
-    double *sinv = nullptr, *svis = nullptr, *rhs = nullptr, *rlo = nullptr;
-    float *avars_f = nullptr, *sinv_f = nullptr, *svis_f = nullptr;  // FP32-bulk variant
+    double *sinv = nullptr, *sface = nullptr, *rhs = nullptr, *rlo = nullptr;
+    float *avars_f = nullptr, *sinv_f = nullptr, *sface_f = nullptr;  // FP32-bulk variant
     int precision = PLENS_PREC_FP64;
     int *blist = nullptr, *bcount = nullptr;
     unsigned long long* mlp_stats = nullptr;
@@ -1284,12 +1301,12 @@ int alloc_buffers(plens_gpu* h) {
         return rc;
     // viscous auxiliary variables and finished face terms in the flux arithmetic of the variant
     if (h->precision == PLENS_PREC_FP64) {
-        if ((rc = dmalloc(&h->avars, 9 * nd)) || (rc = dmalloc(&h->sinv, nface)) || (rc = dmalloc(&h->svis, nface)))
+        if ((rc = dmalloc(&h->avars, 9 * nd)) || (rc = dmalloc(&h->sinv, nface)) || (rc = dmalloc(&h->sface, nface)))
             return rc;
         if (hipMemset(h->avars, 0, sizeof(double) * 9 * nd) != hipSuccess) return PLENS_E_HIP;
     } else {
         if ((rc = dmalloc(&h->avars_f, 9 * nd)) || (rc = dmalloc(&h->sinv_f, nface)) ||
-            (rc = dmalloc(&h->svis_f, nface)))
+            (rc = dmalloc(&h->sface_f, nface)))
             return rc;
         if (hipMemset(h->avars_f, 0, sizeof(float) * 9 * nd) != hipSuccess) return PLENS_E_HIP;
     }
@@ -1313,7 +1330,7 @@ void bind_buffers(plens_gpu* h) {
     k.Ja = h->Ja; k.detJ = h->detJ; k.nrm = h->nrm; k.hmin = h->hmin;
     k.nbr = h->nbr; k.bid = h->bid; k.wall = h->wall; k.tasks = h->tasks; k.bcs = h->bcs; k.basis = h->basis;
     k.avars = h->avars; k.araw = h->araw; k.alpha = h->alpha; k.alpha_d = h->alpha_d;
-    k.sinv = h->sinv; k.svis = h->svis; k.avars_f = h->avars_f; k.sinv_f = h->sinv_f; k.svis_f = h->svis_f;
+    k.sinv = h->sinv; k.sface = h->sface; k.avars_f = h->avars_f; k.sinv_f = h->sinv_f; k.sface_f = h->sface_f;
     k.rhs_out = h->rhs; k.rlo = h->rlo;
     k.blist = h->blist; k.bcount = h->bcount;
     k.mlp_stats = h->mlp_stats; k.feat = nullptr;
@@ -1462,8 +1479,8 @@ void plens_gpu_destroy(plens_gpu* h) {
         if (x) (void)hipEventDestroy(x);
     void* ptrs[] = {h->Qa, h->Qb, h->R[0], h->R[1], h->Ja, h->detJ, h->nrm, h->hmin, h->nbr,
                     h->bid, h->wall, h->tasks, h->bcs, h->basis, h->avars, h->araw, h->alpha, h->alpha_d,
-                    h->sinv, h->svis, h->rhs, h->rlo, h->dt_cell, h->scal, h->fail, h->blist, h->bcount, h->mlp_stats, h->feat,
-                    h->avars_f, h->sinv_f, h->svis_f, h->cinst, h->ioff, h->visc, h->cfl};
+                    h->sinv, h->sface, h->rhs, h->rlo, h->dt_cell, h->scal, h->fail, h->blist, h->bcount, h->mlp_stats, h->feat,
+                    h->avars_f, h->sinv_f, h->sface_f, h->cinst, h->ioff, h->visc, h->cfl};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
