@@ -64,12 +64,17 @@ typedef struct plens_gpu plens_gpu;
 typedef struct {
     int32_t n_cells;
     int32_t degree;                 /* N */
-    const double* Ja;               /* [n_cells][9][n]: MetricTerms::JxContra_vecs, comp 3*dir+xyz */
-    const double* detJ;             /* [n_cells][n]:    MetricTerms::detJ */
+    const double* Ja;               /* [n_cells][9][n]: MetricTerms::JxContra_vecs, comp 3*dir+xyz (nullable, see vertices) */
+    const double* detJ;             /* [n_cells][n]:    MetricTerms::detJ (nullable, see vertices) */
     const double* face_normals;     /* [n_cells][6][3][n_f]: FEFaceValues::normal_vector(face_dof) */
     const double* min_vertex_dist;  /* [n_cells]: cell->minimum_vertex_distance() */
     const int32_t* face_nbr;        /* [n_cells][6]: neighbour (interior) or periodic partner cell, else -1 */
     const int32_t* face_bid;        /* [n_cells][6]: -1 for interior faces, else index into the bc table */
+    const double* vertices;         /* [n_cells][8][3] in deal.II vertex order, nullable. Given, the cells
+                                     * are trilinear (MappingQ1): the kernels derive Ja and detJ at
+                                     * every node from each cell's map instead of reading stored arrays
+                                     * (Ja and detJ may then be NULL; if given, create() checks that the
+                                     * map reproduces them to 1e-10 relative, else PLENS_E_INVALID). */
 } plens_mesh_desc;
 
 /* Gas model ("Navier-Stokes" subsection, custom gas, dgsem/plens.cc:185-223) and blender

@@ -27,7 +27,8 @@ _ip = C.POINTER(C.c_int32)
 
 class MeshDesc(C.Structure):
     _fields_ = [("n_cells", C.c_int32), ("degree", C.c_int32), ("Ja", _dp), ("detJ", _dp),
-                ("face_normals", _dp), ("min_vertex_dist", _dp), ("face_nbr", _ip), ("face_bid", _ip)]
+                ("face_normals", _dp), ("min_vertex_dist", _dp), ("face_nbr", _ip), ("face_bid", _ip),
+                ("vertices", _dp)]
 
 
 class Physics(C.Structure):

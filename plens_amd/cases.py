@@ -65,8 +65,10 @@ class Mesh:
     def handle(self):
         return self._h
 
-    def desc(self):
-        """plens_mesh_desc pointing into this object's (contiguous) numpy arrays."""
+    def desc(self, with_vertices=True):
+        """plens_mesh_desc pointing into this object's (contiguous) numpy arrays; with_vertices
+        hands over the cell vertices too, so the kernels derive the metric terms from the trilinear
+        maps instead of reading the stored Ja and detJ."""
         d = L.MeshDesc()
         d.n_cells, d.degree = self.n_cells, self.degree
         dp = C.POINTER(C.c_double)
@@ -77,6 +79,7 @@ class Mesh:
         d.min_vertex_dist = self.min_vertex_dist.ctypes.data_as(dp)
         d.face_nbr = self.face_nbr.ctypes.data_as(ip)
         d.face_bid = self.face_bid.ctypes.data_as(ip)
+        d.vertices = self.vertices.ctypes.data_as(dp) if with_vertices else None
         return d
 
     def close(self):

@@ -379,6 +379,7 @@ extern "C" int plens_mesh_fill_desc(const plens_mesh* m, plens_mesh_desc* d) {
     d->min_vertex_dist = m->hmin.data();
     d->face_nbr = m->face_nbr.data();
     d->face_bid = m->face_bid.data();
+    d->vertices = m->vertices.data();
     return PLENS_OK;
 }
 

@@ -52,6 +52,7 @@ struct BCDev {
 };
 
 struct Basis {
+    double x[kMaxNP];  // GLL nodes on [0,1]
     double w[kMaxNP];
     double D[kMaxNP * kMaxNP];
     double Q[kMaxNP * kMaxNP];
@@ -63,8 +64,11 @@ struct KP {
     long long ndof;
     const double* Q;
     double* Qout;
-    const double* Ja;
+    const double* Ja;        // stored metric terms (nullptr when derived from cmap)
     const double* detJ;
+    const double* cmap;      // trilinear map of every cell: [3][8] monomial coefficients (nullable)
+    const double* fmet;      // per face task and face node: Ja/|Ja|, |Ja|/w_end, detJ of the cell side,
+                             // then the same of the neighbour side ([task][10][n_f])
     const double* nrm;
     const double* hmin;
     const int* nbr;
@@ -169,6 +173,37 @@ __device__ __forceinline__ void load_state(const double* Q, long long ndof, long
     for (int v = 0; v < 5; ++v) u[v] = Q[v * ndof + g];
 }
 
+// Metric terms of a node (MetricTerms::reinit, dgsem/metric_terms.cc:82-102): Ja^d = a_{d+1} x a_{d+2}
+// with a_d = dx/dxi_d, detJ = a_0 . Ja^0; from the cell's trilinear map
+// x(xi) = k0 + k1 xi0 + k2 xi1 + k3 xi2 + k4 xi0 xi1 + k5 xi1 xi2 + k6 xi0 xi2 + k7 xi0 xi1 xi2
+// when the handle has one, else loaded from the stored arrays.
+template <int N>
+__device__ __forceinline__ void node_metrics(const KP& p, int c, int node, double ja[9], double& J) {
+    constexpr int NP = N + 1, NN = NP * NP * NP;
+    if (p.cmap) {
+        const double x0 = p.basis->x[node % NP], x1 = p.basis->x[(node / NP) % NP], x2 = p.basis->x[node / (NP * NP)];
+        const double* k = p.cmap + (long long)c * 24;
+        double a[3][3];
+        for (int i = 0; i < 3; ++i) {
+            const double* m = k + 8 * i;
+            a[0][i] = m[1] + m[4] * x1 + m[6] * x2 + m[7] * (x1 * x2);
+            a[1][i] = m[2] + m[4] * x0 + m[5] * x2 + m[7] * (x0 * x2);
+            a[2][i] = m[3] + m[5] * x1 + m[6] * x0 + m[7] * (x0 * x1);
+        }
+        for (int d = 0; d < 3; ++d) {
+            const double* b = a[(d + 1) % 3];
+            const double* e = a[(d + 2) % 3];
+            ja[3 * d + 0] = b[1] * e[2] - b[2] * e[1];
+            ja[3 * d + 1] = b[2] * e[0] - b[0] * e[2];
+            ja[3 * d + 2] = b[0] * e[1] - b[1] * e[0];
+        }
+        J = a[0][0] * ja[0] + a[0][1] * ja[1] + a[0][2] * ja[2];
+    } else {
+        for (int q = 0; q < 9; ++q) ja[q] = p.Ja[((long long)c * 9 + q) * NN + node];
+        J = p.detJ[(long long)c * NN + node];
+    }
+}
+
 __device__ __forceinline__ float mlp_gelu(float x) {
     const float t = tanhf(0.7978845608028654f * (x + 0.044715f * x * x * x));
     return 0.5f * x * (1.0f + t);
@@ -262,8 +297,9 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
     if (act) {
         load_state(p.Q, p.ndof, g, u);
         for (int v = 0; v < 5; ++v) sQ[v * B + li] = (TF)u[v];
-        for (int a = 0; a < 9; ++a) sJa[a * B + li] = (TF)p.Ja[((long long)c * 9 + a) * NN + node];
-        detJ = p.detJ[g];
+        double ja[9];
+        node_metrics<N>(p, c, node, ja, detJ);
+        for (int a = 0; a < 9; ++a) sJa[a * B + li] = (TF)ja[a];
         // PLENS::assert_positivity (dgsem/plens.cc:2531-2546)
         if (!(u[0] > 0.0)) atomicMin(p.fail, (unsigned long long)g * 4ull + 1ull);
         else if (!(get_e(u) > 0.0)) atomicMin(p.fail, (unsigned long long)g * 4ull + 2ull);
@@ -343,9 +379,11 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
                 const double a = get_a(u, gs.gma);
                 const double vel[3] = {u[1] / u[0], u[2] / u[0], u[3] / u[0]};
                 double madv = -1e300;
+                double ja[9], dJ;
+                node_metrics<N>(p, c, node, ja, dJ);
                 for (int r = 0; r < 3; ++r) {  // (J^-T v)_r with J^-1 rows = Ja^i / detJ
                     double s = 0;
-                    for (int cc = 0; cc < 3; ++cc) s += (p.Ja[((long long)c * 9 + 3 * cc + r) * NN + node] / detJ) * vel[cc];
+                    for (int cc = 0; cc < 3; ++cc) s += (ja[3 * cc + r] / detJ) * vel[cc];
                     madv = fmax(madv, s);
                 }
                 const double cr = p.cfl ? p.cfl[p.cinst[c]] : p.courant;
@@ -605,22 +643,17 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     // (dgsem/plens.cc:3237-3268, 3373-3406): -s (-s F*_side - f(u; Ja/|Ja|)) |Ja| / w_end, with
     // F*_side the outward flux of that side (-F for the neighbour of an interior face) and
     // s = +1 / -1 for left / right faces.
-    TF dirc[3], nwc, dirn[3] = {TF(0), TF(0), TF(0)}, nwn = TF(0);
-    {
-        TF ja[3];
-        for (int a = 0; a < 3; ++a) ja[a] = (TF)p.Ja[((long long)c * 9 + 3 * sd + a) * NN + (g - (long long)c * NN)];
-        const TF nr = sq(ja[0] * ja[0] + ja[1] * ja[1] + ja[2] * ja[2]);
-        const TF inr = dv(TF(1), nr);
-        for (int a = 0; a < 3; ++a) dirc[a] = ja[a] * inr;
-        nwc = nr * (TF)((f & 1) ? p.iwN : p.iw0);
-    }
+    // face-node metric terms of both sides (direction Ja/|Ja|, weight |Ja|/w_end, detJ)
+    const double* fm = p.fmet + (long long)t * 10 * NF + q;
+    const TF dirc[3] = {(TF)fm[0], (TF)fm[NF], (TF)fm[2 * NF]};
+    const TF nwc = (TF)fm[3 * NF];
+    const double Jc = fm[4 * NF];
+    TF dirn[3] = {TF(0), TF(0), TF(0)}, nwn = TF(0);
+    double Jn = 1.0;
     if (interior) {
-        TF ja[3];
-        for (int a = 0; a < 3; ++a) ja[a] = (TF)p.Ja[((long long)nb * 9 + 3 * sd + a) * NN + (gn - (long long)nb * NN)];
-        const TF nr = sq(ja[0] * ja[0] + ja[1] * ja[1] + ja[2] * ja[2]);
-        const TF inr = dv(TF(1), nr);
-        for (int a = 0; a < 3; ++a) dirn[a] = ja[a] * inr;
-        nwn = nr * (TF)((f & 1) ? p.iw0 : p.iwN);
+        for (int a = 0; a < 3; ++a) dirn[a] = (TF)fm[(5 + a) * NF];
+        nwn = (TF)fm[8 * NF];
+        Jn = fm[9 * NF];
     }
     const TF sgc = (f & 1) ? TF(-1) : TF(1), sgn = -sgc;
     const long long o = (long long)cf * 5 * NF + q;
@@ -645,7 +678,6 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     // each side's contribution to its blended RHS (calc_rhs, dgsem/plens.cc:3753-3771): the
     // inviscid term scaled by (1 - alpha)/(-J), the viscous one by (1 - alpha_d)/J (the residual
     // scalings of calc_cell_ho_inv/dif_residual, 3271-3274, 3409-3412)
-    const double Jc = p.detJ[g], Jn = interior ? p.detJ[gn] : 1.0;
     const TF wic = (TF)((1.0 - ac) / (-1.0 * Jc)), win = (TF)((1.0 - an) / (-1.0 * Jn));
     TF* sface = tsel<TF>(p.sface, p.sface_f);
     if (gs.inviscid) {
@@ -712,17 +744,17 @@ __global__ void __launch_bounds__(256) k_lo(KP p) {
         const double alpha = act ? p.alpha[c] : 0.0;
         const TF bl = (TF)(alpha <= 0.0 ? 0.0 : (alpha <= 1.0 ? alpha : 1.0));
         double u0[5] = {1, 0, 0, 0, 1}, detJ = 1;
+        double jm[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
         if (act) {
             load_state(p.Q, p.ndof, g, u0);
-            detJ = p.detJ[g];
+            node_metrics<N>(p, c, node, jm, detJ);
         }
         TF u[5];
         for (int v = 0; v < 5; ++v) u[v] = (TF)u0[v];
         __syncthreads();  // previous iteration done with LDS
         if (inblk) {
             for (int v = 0; v < 5; ++v) sQ[v * B + li] = u[v];
-            for (int a = 0; a < 9; ++a)
-                sJa[a * B + li] = act ? (TF)p.Ja[((long long)c * 9 + a) * NN + node] : ((a % 4 == 0) ? TF(1) : TF(0));
+            for (int a = 0; a < 9; ++a) sJa[a * B + li] = (TF)jm[a];
         }
         double rl[5] = {0, 0, 0, 0, 0};
         for (int dir = 0; dir < 3; ++dir) {
@@ -846,9 +878,10 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
     const long long g = (long long)c * NN + node;
     const Gas& gs = p.gas;
     double u[5] = {1, 0, 0, 0, 2.5}, detJ = 1;
+    double jm[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     if (act) {
         load_state(p.Q, p.ndof, g, u);
-        detJ = p.detJ[g];
+        node_metrics<N>(p, c, node, jm, detJ);
     }
     if (inblk) {
         const double pr = get_p(u, gs.gma);
@@ -864,7 +897,7 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
         sA[5 * B + li] = (TF)(v0 * v0 + v1 * v1 + v2 * v2);
     }
     TF ja_own[9], ut[5];
-    for (int a = 0; a < 9; ++a) ja_own[a] = act ? (TF)p.Ja[((long long)c * 9 + a) * NN + node] : ((a % 4 == 0) ? TF(1) : TF(0));
+    for (int a = 0; a < 9; ++a) ja_own[a] = (TF)jm[a];
     for (int v = 0; v < 5; ++v) ut[v] = (TF)u[v];
     const double alpha = act ? p.alpha[c] : 0.0;
     const double alpha_d = act ? p.alpha_d[c] : 0.0;
@@ -1048,6 +1081,9 @@ struct plens_gpu {
     // device buffers
     double *Qa = nullptr, *Qb = nullptr, *R[2] = {nullptr, nullptr};
     double *Ja = nullptr, *detJ = nullptr, *nrm = nullptr, *hmin = nullptr;
+    double* cmap = nullptr;  // trilinear maps of the cells; the stored metric arrays are not kept then
+    double* fmet = nullptr;  // face-node metric terms of the face tasks
+    bool use_cmap = false;
     int *nbr = nullptr, *bid = nullptr, *wall = nullptr, *tasks = nullptr;
     BCDev* bcs = nullptr;
     Basis* basis = nullptr;
@@ -1282,6 +1318,43 @@ int reset_failure(plens_gpu* h) {
     return PLENS_OK;
 }
 
+// Monomial coefficients k[i][0..7] of the trilinear map x_i(xi) of a cell from its 8 vertices in
+// deal.II order (vertex v at xi = (v & 1, (v >> 1) & 1, (v >> 2) & 1)), for node_metrics.
+void trilinear_coeffs(const double* V, double* k) {
+    for (int i = 0; i < 3; ++i) {
+        const double v0 = V[i], v1 = V[3 + i], v2 = V[6 + i], v3 = V[9 + i];
+        const double v4 = V[12 + i], v5 = V[15 + i], v6 = V[18 + i], v7 = V[21 + i];
+        double* m = k + 8 * i;
+        m[0] = v0;
+        m[1] = v1 - v0;
+        m[2] = v2 - v0;
+        m[3] = v4 - v0;
+        m[4] = v3 - v2 - v1 + v0;
+        m[5] = v6 - v4 - v2 + v0;
+        m[6] = v5 - v4 - v1 + v0;
+        m[7] = v7 - v6 - v5 + v4 - v3 + v2 + v1 - v0;
+    }
+}
+
+// host copy of node_metrics' trilinear branch (validation of the given metric terms)
+void host_node_metrics(const double* k, double x0, double x1, double x2, double ja[9], double& J) {
+    double a[3][3];
+    for (int i = 0; i < 3; ++i) {
+        const double* m = k + 8 * i;
+        a[0][i] = m[1] + m[4] * x1 + m[6] * x2 + m[7] * (x1 * x2);
+        a[1][i] = m[2] + m[4] * x0 + m[5] * x2 + m[7] * (x0 * x2);
+        a[2][i] = m[3] + m[5] * x1 + m[6] * x0 + m[7] * (x0 * x1);
+    }
+    for (int d = 0; d < 3; ++d) {
+        const double* b = a[(d + 1) % 3];
+        const double* e = a[(d + 2) % 3];
+        ja[3 * d + 0] = b[1] * e[2] - b[2] * e[1];
+        ja[3 * d + 1] = b[2] * e[0] - b[0] * e[2];
+        ja[3 * d + 2] = b[0] * e[1] - b[1] * e[0];
+    }
+    J = a[0][0] * ja[0] + a[0][1] * ja[1] + a[0][2] * ja[2];
+}
+
 // Device buffers of a handle whose sizes (N, nc, ntasks, n_bcs, ninst, precision) are set; work
 // arrays start at zero.
 int alloc_buffers(plens_gpu* h) {
@@ -1289,7 +1362,7 @@ int alloc_buffers(plens_gpu* h) {
     int rc;
     if ((rc = dmalloc(&h->Qa, 5 * nd)) || (rc = dmalloc(&h->Qb, 5 * nd)) || (rc = dmalloc(&h->R[0], 5 * nd)) ||
         (rc = dmalloc(&h->R[1], 5 * nd)) || (rc = dmalloc(&h->rhs, 5 * nd)) || (rc = dmalloc(&h->rlo, 5 * nd)) ||
-        (rc = dmalloc(&h->Ja, 9 * nd)) || (rc = dmalloc(&h->detJ, nd)) || (rc = dmalloc(&h->nrm, nc * 18 * NF)) ||
+        (rc = dmalloc(&h->nrm, nc * 18 * NF)) || (rc = dmalloc(&h->fmet, (size_t)h->ntasks * 10 * NF)) ||
         (rc = dmalloc(&h->hmin, nc)) || (rc = dmalloc(&h->nbr, nc * 6)) || (rc = dmalloc(&h->bid, nc * 6)) ||
         (rc = dmalloc(&h->wall, nc)) || (rc = dmalloc(&h->tasks, (size_t)h->ntasks)) ||
         (rc = dmalloc(&h->bcs, (size_t)std::max(h->n_bcs, 1))) || (rc = dmalloc(&h->basis, 1)) ||
@@ -1299,6 +1372,11 @@ int alloc_buffers(plens_gpu* h) {
         (rc = dmalloc(&h->scal, 2 * (size_t)h->ninst)) || (rc = dmalloc(&h->fail, 2)) ||
         (rc = dmalloc(&h->blist, nc)) || (rc = dmalloc(&h->bcount, 1)) || (rc = dmalloc(&h->mlp_stats, 2)))
         return rc;
+    if (h->use_cmap) {
+        if ((rc = dmalloc(&h->cmap, nc * 24))) return rc;
+    } else if ((rc = dmalloc(&h->Ja, 9 * nd)) || (rc = dmalloc(&h->detJ, nd))) {
+        return rc;
+    }
     // viscous auxiliary variables and finished face terms in the flux arithmetic of the variant
     if (h->precision == PLENS_PREC_FP64) {
         if ((rc = dmalloc(&h->avars, 9 * nd)) || (rc = dmalloc(&h->sinv, nface)) || (rc = dmalloc(&h->sface, nface)))
@@ -1327,7 +1405,7 @@ int alloc_buffers(plens_gpu* h) {
 void bind_buffers(plens_gpu* h) {
     KP& k = h->base;
     k.nc = h->nc; k.cpb = h->cpb; k.ntasks = h->ntasks; k.ndof = h->ndof; k.ninst = h->ninst;
-    k.Ja = h->Ja; k.detJ = h->detJ; k.nrm = h->nrm; k.hmin = h->hmin;
+    k.Ja = h->Ja; k.detJ = h->detJ; k.cmap = h->cmap; k.fmet = h->fmet; k.nrm = h->nrm; k.hmin = h->hmin;
     k.nbr = h->nbr; k.bid = h->bid; k.wall = h->wall; k.tasks = h->tasks; k.bcs = h->bcs; k.basis = h->basis;
     k.avars = h->avars; k.araw = h->araw; k.alpha = h->alpha; k.alpha_d = h->alpha_d;
     k.sinv = h->sinv; k.sface = h->sface; k.avars_f = h->avars_f; k.sinv_f = h->sinv_f; k.sface_f = h->sface_f;
@@ -1367,8 +1445,8 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
                      plens_gpu** out) {
     if (!m || !ph || !out || (n_bcs > 0 && !bcs)) return PLENS_E_INVALID;
     if (m->degree < 1 || m->degree > 5 || m->n_cells < 1) return PLENS_E_INVALID;
-    if (!m->Ja || !m->detJ || !m->face_normals || !m->min_vertex_dist || !m->face_nbr || !m->face_bid)
-        return PLENS_E_INVALID;
+    if (!m->face_normals || !m->min_vertex_dist || !m->face_nbr || !m->face_bid) return PLENS_E_INVALID;
+    if (!m->vertices && (!m->Ja || !m->detJ)) return PLENS_E_INVALID;
     if (ph->inv_surf_flux < 0 || ph->inv_surf_flux > 3) return PLENS_E_INVALID;
     if (ph->precision != PLENS_PREC_FP64 && ph->precision != PLENS_PREC_FP32_BULK) return PLENS_E_INVALID;
     int ndev = 0;
@@ -1389,6 +1467,7 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
     h->ndof = (long long)nc * NN;
     h->cpb = default_cpb(N);
     h->precision = ph->precision;
+    h->use_cmap = m->vertices != nullptr;
     (void)hipGetDevice(&h->device);
     int rc = PLENS_OK;
     auto fail = [&](int code) {
@@ -1431,6 +1510,7 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
         double x[kMaxNP], w[kMaxNP], D[kMaxNP * kMaxNP], C1[kMaxNP * kMaxNP];
         if (plens_basis_1d(N, x, w, D, C1)) return fail(PLENS_E_INVALID);
         for (int i = 0; i < NP; ++i) {
+            hbas.x[i] = x[i];
             hbas.w[i] = w[i];
             for (int j = 0; j < NP; ++j) {
                 hbas.D[i * kMaxNP + j] = D[i * NP + j];
@@ -1441,9 +1521,70 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
     }
     const int ioff_h[2] = {0, nc};
     const double visc_h[3] = {ph->mu0, ph->T0, ph->S};
-    if (hipMemcpy(h->Ja, m->Ja, sizeof(double) * 9 * nd, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(h->detJ, m->detJ, sizeof(double) * nd, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(h->nrm, m->face_normals, sizeof(double) * nc * 18 * NF, hipMemcpyHostToDevice) != hipSuccess ||
+    std::vector<double> cm;
+    if (h->use_cmap) {
+        // the kernels derive the metric terms from the trilinear map of every cell; given stored
+        // terms, they must agree with it (the cells are trilinear, MappingQ1)
+        cm.resize((size_t)nc * 24);
+        for (int c = 0; c < nc; ++c) {
+            trilinear_coeffs(m->vertices + (size_t)c * 24, cm.data() + (size_t)c * 24);
+            if (!m->Ja || !m->detJ) continue;
+            double ja[9], J, scale = 0, jscale = 0, err = 0, jerr = 0;
+            for (int node = 0; node < NN; ++node) {
+                host_node_metrics(cm.data() + (size_t)c * 24, hbas.x[node % NP], hbas.x[(node / NP) % NP],
+                                  hbas.x[node / (NP * NP)], ja, J);
+                for (int q = 0; q < 9; ++q) {
+                    const double given = m->Ja[((size_t)c * 9 + q) * NN + node];
+                    scale = std::max(scale, std::fabs(given));
+                    err = std::max(err, std::fabs(ja[q] - given));
+                }
+                const double gJ = m->detJ[(size_t)c * NN + node];
+                jscale = std::max(jscale, std::fabs(gJ));
+                jerr = std::max(jerr, std::fabs(J - gJ));
+            }
+            if (!(err <= 1e-10 * scale) || !(jerr <= 1e-10 * jscale)) return fail(PLENS_E_INVALID);
+        }
+        if (hipMemcpy(h->cmap, cm.data(), sizeof(double) * cm.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(PLENS_E_HIP);
+    } else if (hipMemcpy(h->Ja, m->Ja, sizeof(double) * 9 * nd, hipMemcpyHostToDevice) != hipSuccess ||
+               hipMemcpy(h->detJ, m->detJ, sizeof(double) * nd, hipMemcpyHostToDevice) != hipSuccess) {
+        return fail(PLENS_E_HIP);
+    }
+    {  // face-node metric terms of every face task, both sides (calc_cell_ho_*_residual surface terms)
+        auto metrics_at = [&](int c, int node, double ja[9], double& J) {
+            if (h->use_cmap) {
+                host_node_metrics(cm.data() + (size_t)c * 24, hbas.x[node % NP], hbas.x[(node / NP) % NP],
+                                  hbas.x[node / (NP * NP)], ja, J);
+            } else {
+                for (int q = 0; q < 9; ++q) ja[q] = m->Ja[((size_t)c * 9 + q) * NN + node];
+                J = m->detJ[(size_t)c * NN + node];
+            }
+        };
+        const double iw0 = 1.0 / hbas.w[0], iwN = 1.0 / hbas.w[N];
+        std::vector<double> fm((size_t)h->ntasks * 10 * NF, 0.0);
+        for (int t = 0; t < h->ntasks; ++t) {
+            const int cf = tasks[t], c = cf / 6, f = cf - 6 * c, sd = f >> 1;
+            const bool interior = m->face_bid[cf] < 0;
+            const int nb = m->face_nbr[cf];
+            for (int q = 0; q < NF; ++q) {
+                double* o = fm.data() + (size_t)t * 10 * NF + q;
+                for (int side = 0; side < (interior ? 2 : 1); ++side) {
+                    double ja[9], J;
+                    metrics_at(side ? nb : c, plens_face_node(N, side ? (f ^ 1) : f, q), ja, J);
+                    const double nr = std::sqrt(ja[3 * sd] * ja[3 * sd] + ja[3 * sd + 1] * ja[3 * sd + 1] +
+                                                ja[3 * sd + 2] * ja[3 * sd + 2]);
+                    const double inr = 1.0 / nr;
+                    for (int a = 0; a < 3; ++a) o[(5 * side + a) * NF] = ja[3 * sd + a] * inr;
+                    o[(5 * side + 3) * NF] = nr * (((f & 1) != side) ? iwN : iw0);
+                    o[(5 * side + 4) * NF] = J;
+                }
+                if (!interior) o[9 * NF] = 1.0;
+            }
+        }
+        if (hipMemcpy(h->fmet, fm.data(), sizeof(double) * fm.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(PLENS_E_HIP);
+    }
+    if (hipMemcpy(h->nrm, m->face_normals, sizeof(double) * nc * 18 * NF, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->hmin, m->min_vertex_dist, sizeof(double) * nc, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->nbr, m->face_nbr, sizeof(int) * nc * 6, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->bid, m->face_bid, sizeof(int) * nc * 6, hipMemcpyHostToDevice) != hipSuccess ||
@@ -1480,7 +1621,7 @@ void plens_gpu_destroy(plens_gpu* h) {
     void* ptrs[] = {h->Qa, h->Qb, h->R[0], h->R[1], h->Ja, h->detJ, h->nrm, h->hmin, h->nbr,
                     h->bid, h->wall, h->tasks, h->bcs, h->basis, h->avars, h->araw, h->alpha, h->alpha_d,
                     h->sinv, h->sface, h->rhs, h->rlo, h->dt_cell, h->scal, h->fail, h->blist, h->bcount, h->mlp_stats, h->feat,
-                    h->avars_f, h->sinv_f, h->sface_f, h->cinst, h->ioff, h->visc, h->cfl};
+                    h->avars_f, h->sinv_f, h->sface_f, h->cinst, h->ioff, h->visc, h->cfl, h->cmap, h->fmet};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
@@ -1815,7 +1956,8 @@ bool same_scheme(const KP& a, const KP& b) {
 int merge_group(const std::vector<plens_gpu*>& mem, hipStream_t stream, plens_gpu** out) {
     const plens_gpu* m0 = mem[0];
     for (const plens_gpu* m : mem)
-        if (m->N != m0->N || m->precision != m0->precision || m->ninst != 1 || !same_scheme(m->base, m0->base))
+        if (m->N != m0->N || m->precision != m0->precision || m->ninst != 1 || m->use_cmap != m0->use_cmap ||
+            !same_scheme(m->base, m0->base))
             return PLENS_E_INVALID;
     plens_gpu* h = new (std::nothrow) plens_gpu;
     if (!h) return PLENS_E_NOMEM;
@@ -1823,7 +1965,7 @@ int merge_group(const std::vector<plens_gpu*>& mem, hipStream_t stream, plens_gp
         plens_gpu_destroy(h);
         return code;
     };
-    h->N = m0->N; h->NN = m0->NN; h->NF = m0->NF; h->precision = m0->precision;
+    h->N = m0->N; h->NN = m0->NN; h->NF = m0->NF; h->precision = m0->precision; h->use_cmap = m0->use_cmap;
     h->device = m0->device;
     h->stream = stream;
     h->own_stream = false;
@@ -1849,8 +1991,12 @@ int merge_group(const std::vector<plens_gpu*>& mem, hipStream_t stream, plens_gp
         for (int v = 0; v < 5; ++v)
             HIPCHK(hipMemcpy(h->Qa + v * h->ndof + d0, Qm + v * m->ndof, sizeof(double) * m->ndof,
                              hipMemcpyDeviceToDevice));
-        HIPCHK(hipMemcpy(h->Ja + (size_t)co * 9 * NN, m->Ja, sizeof(double) * 9 * m->ndof, hipMemcpyDeviceToDevice));
-        HIPCHK(hipMemcpy(h->detJ + d0, m->detJ, sizeof(double) * m->ndof, hipMemcpyDeviceToDevice));
+        if (h->use_cmap) {
+            HIPCHK(hipMemcpy(h->cmap + (size_t)co * 24, m->cmap, sizeof(double) * 24 * m->nc, hipMemcpyDeviceToDevice));
+        } else {
+            HIPCHK(hipMemcpy(h->Ja + (size_t)co * 9 * NN, m->Ja, sizeof(double) * 9 * m->ndof, hipMemcpyDeviceToDevice));
+            HIPCHK(hipMemcpy(h->detJ + d0, m->detJ, sizeof(double) * m->ndof, hipMemcpyDeviceToDevice));
+        }
         HIPCHK(hipMemcpy(h->nrm + (size_t)co * 18 * NF, m->nrm, sizeof(double) * m->nc * 18 * NF,
                          hipMemcpyDeviceToDevice));
         HIPCHK(hipMemcpy(h->hmin + co, m->hmin, sizeof(double) * m->nc, hipMemcpyDeviceToDevice));
@@ -1859,8 +2005,11 @@ int merge_group(const std::vector<plens_gpu*>& mem, hipStream_t stream, plens_gp
             HIPCHK(hipMemcpy(h->bcs + bo, m->bcs, sizeof(BCDev) * m->n_bcs, hipMemcpyDeviceToDevice));
         HIPCHK(hipMemcpy(nbr.data() + (size_t)co * 6, m->nbr, sizeof(int) * m->nc * 6, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(bid.data() + (size_t)co * 6, m->bid, sizeof(int) * m->nc * 6, hipMemcpyDeviceToHost));
-        if (m->ntasks > 0)
+        if (m->ntasks > 0) {
             HIPCHK(hipMemcpy(tasks.data() + to, m->tasks, sizeof(int) * m->ntasks, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(h->fmet + (size_t)to * 10 * NF, m->fmet, sizeof(double) * m->ntasks * 10 * NF,
+                             hipMemcpyDeviceToDevice));
+        }
         for (int k = co * 6; k < (co + m->nc) * 6; ++k) {
             if (nbr[k] >= 0) nbr[k] += co;
             if (bid[k] >= 0) bid[k] += bo;

@@ -27,13 +27,15 @@ def _dptr(a):
 
 
 class GpuPlens:
-    def __init__(self, case=None, mesh=None, physics=None, bcs=None, cells_per_block=0):
+    def __init__(self, case=None, mesh=None, physics=None, bcs=None, cells_per_block=0, metrics="map"):
+        """metrics: "map" derives the metric terms in the kernels from each cell's trilinear map
+        (cell vertices handed over); "stored" reads the host-computed Ja and detJ arrays."""
         if case is not None:
             mesh, physics, bcs = case.mesh, case.physics, case.bcs
         self.lib = L.load()
         self.mesh = mesh
         self.n_cells, self.n_dof = mesh.n_cells, mesh.n_dof
-        self._desc = mesh.desc()
+        self._desc = mesh.desc(with_vertices={"map": True, "stored": False}[metrics])
         arr = (L.BC * max(len(bcs), 1))()
         for i, b in enumerate(bcs):
             arr[i] = b

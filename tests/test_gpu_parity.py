@@ -137,3 +137,32 @@ def test_rhs_many_blended_cells():
     g, o = _rhs_parity(case, "many-blended")
     _, a, _ = o.calc_rhs()
     assert (a > 0).sum() > 2048, (a > 0).sum()
+
+
+@pytest.mark.parametrize("flux", ["HLLC", "Chandrashekhar"])
+def test_rhs_wall_flow_stored_metrics(flux):
+    """the stored-metric path (Ja and detJ read from the host arrays) against the oracle"""
+    from oracle.pyoracle import OraclePlens
+    from plens_amd import GpuPlens, wall_flow
+    case = wall_flow(nx=6, ny=4, nz=2, flux=flux)
+    g, o = GpuPlens(case, metrics="stored"), OraclePlens(case)
+    rg, _, _ = g.calc_rhs()
+    ro, _, _ = o.calc_rhs()
+    err = rel_l2(rg, ro)
+    print(f"[wall-{flux} stored metrics] rhs rel L2 = {err:.3e}", flush=True)
+    assert err <= RHS_TOL
+    g.run(10, case.courant)
+    o.run(10, case.courant)
+    assert rel_l2(g.get_state(), o.get_state()) <= RUN_TOL
+
+
+def test_map_must_reproduce_given_metrics():
+    """vertices handed over together with metric terms that no trilinear map of them produces"""
+    from plens_amd import GpuPlens, wall_flow
+    from plens_amd._lib import PLENS_E_INVALID, PlensError
+    case = wall_flow(nx=4, ny=2, nz=2)
+    case.mesh.Ja[1, 4, 7] *= 1.001
+    with pytest.raises(PlensError) as ei:
+        GpuPlens(case)
+    assert ei.value.code == PLENS_E_INVALID
+    GpuPlens(case, metrics="stored")  # the arrays alone are accepted as given
