@@ -30,26 +30,35 @@ HBM_PEAK_GBS = 8000.0
 
 
 def kernel_bytes_per_dof(N, viscous=True, fb=8):
-    """Algorithmic HBM bytes per DoF of one launch of each kernel class (compulsory traffic); fb is
-    the byte size of the flux-arithmetic arrays (avars and the finished face terms: 8 for FP64, 4 for
-    FP32-bulk); states, metrics and residual registers are FP64 in every variant."""
+    """Algorithmic HBM bytes per DoF of one launch of each kernel class (compulsory traffic of this
+    design); fb is the byte size of the flux-arithmetic arrays (avars and the face terms: 8 for
+    FP64, 4 for FP32-bulk); states, residual registers and metric data are FP64 in every variant.
+    Metric terms are derived from each cell's trilinear map: 24 doubles per cell (map_b per DoF)."""
     n, nf = (N + 1) ** 3, (N + 1) ** 2
     face_per_dof = 6.0 * nf / n  # face nodes per cell node
+    map_b = 24 * 8 / n
     v = 1 if viscous else 0
     b = {}
-    # k_grad: read Q (5), Ja (9), detJ (1); write avars (9, viscous)
-    b["grad"] = 8 * (5 + 9 + 1) + fb * 9 * v
+    # k_grad: read Q (5) and the map; write avars (9, viscous)
+    b["grad"] = 8 * 5 + map_b + fb * 9 * v
     b["alpha"] = 0.0
     # k_lo: blended cells only; not counted
     b["lo"] = 0.0
-    # k_face, per face node (each shared by two cells): read both states (2x5), the normal (3), both
-    # sides' Ja rows (2x3) and avars (2x9, viscous); write both sides' finished terms (2x5 inviscid,
-    # 2x5 viscous)
-    b["face"] = face_per_dof / 2 * (8 * (10 + 3 + 6) + fb * (18 * v + 10 * (1 + v)))
-    # k_vol: read Q (5), Ja (9), detJ (1), avars (9, viscous), the finished face terms of its face
-    # nodes (5 inviscid + 5 viscous), R' and R'' (stage averages 4/5 and 3/5 of 5); write Q (5), R (5)
-    b["vol"] = 8 * (5 + 9 + 1 + 5 * 4 / 5 + 5 * 3 / 5 + 10) + fb * (9 * v + face_per_dof * 5 * (1 + v))
+    # k_face, per face node (each shared by two cells): read both states (2x5), the normal (3), the
+    # face-node metric terms of both sides (10) and both avars (2x9, viscous); write both sides'
+    # weighted surface terms (2x5)
+    b["face"] = face_per_dof / 2 * (8 * (10 + 3 + 10) + fb * (18 * v + 10))
+    # k_vol: read Q (5), the map, avars (9, viscous), the weighted surface terms of its face nodes
+    # (5 per face node), R' and R'' (stage averages 4/5 and 3/5 of 5); write Q (5), R (5)
+    b["vol"] = 8 * (5 + 5 * 4 / 5 + 5 * 3 / 5 + 10) + map_b + fb * (9 * v + face_per_dof * 5)
     return b
+
+
+def stage_bytes_per_dof(N, fb=8):
+    """compulsory bytes per DoF-stage of the two-pass algorithm (lift, then residual) with the fused
+    RK epilogue: Q read + write, R write, R' and R'' reads (stage averages), avars write + read, and
+    the cells' trilinear maps (the metric terms are not read)"""
+    return 8 * (5 + 5 + 5 + 4 + 3) + fb * 18 + 24 * 8 / (N + 1) ** 3
 
 
 def measured_traffic(precision, kernel):
@@ -216,7 +225,7 @@ def main():
     ach = kernels[dom]["achieved_GBs"]
     stage_ms = float(sum(kernels[k]["avg_ms"] for k in names))
     fb = 8 if args.precision == "fp64" else 4
-    stage_bytes = 8 * 32 + fb * 18  # the two-pass stage's compulsory bytes per DoF (viscous)
+    stage_bytes = stage_bytes_per_dof(N, fb)
     tname = f"k_{dom}<{N},{'double' if args.precision == 'fp64' else 'float'}>"
     traffic, tsrc = measured_traffic(args.precision, tname)
     roofline = {"bound": "hbm", "kernel": tname, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
