@@ -20,8 +20,18 @@ struct Gas {
     int inviscid;
 };
 
-// ---- arithmetic helpers (double: IEEE; float: native single-precision instructions) ----------
-__device__ __forceinline__ double dv(double a, double b) { return a / b; }
+// ---- arithmetic helpers (double: reciprocal-based division within one rounding of IEEE, IEEE
+//      square root and logarithm; float: native single-precision instructions) -------------------
+// 1/x from the hardware reciprocal refined by two Newton steps (full double precision; the quotient
+// a*rcp(b) differs from a correctly rounded a/b by at most one extra rounding)
+__device__ __forceinline__ double rcp64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ __forceinline__ double dv(double a, double b) { return a * rcp64(b); }
 __device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double sq(double a) { return sqrt(a); }
 __device__ __forceinline__ float sq(float a) { return __builtin_amdgcn_sqrtf(a); }

@@ -123,7 +123,6 @@ template <>
 __device__ __forceinline__ double* tsel<double>(double* d, float*) { return d; }
 template <>
 __device__ __forceinline__ float* tsel<float>(double*, float* f) { return f; }
-__device__ __forceinline__ double rcp64(double x);
 __device__ __forceinline__ double rcpt(double x) { return rcp64(x); }
 __device__ __forceinline__ float rcpt(float x) { return __builtin_amdgcn_rcpf(x); }
 
@@ -821,15 +820,6 @@ __global__ void __launch_bounds__(256) k_lo(KP p) {
     }
 }
 
-// 1/x from the hardware reciprocal refined by two Newton steps (full double precision; the quotient
-// a*rcp(b) differs from a correctly rounded a/b by at most one extra rounding)
-__device__ __forceinline__ double rcp64(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    double e = fma(-x, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-x, r, 1.0);
-    return fma(r, e, r);
-}
 
 // Chandrashekhar two-point flux along an unnormalised direction a from per-node precomputed
 // (rho, beta = rho/2p, ln rho, ln beta, v, |v|^2): equal to |a| * vol_flux(u_i, u_j, a/|a|) with the
