@@ -57,6 +57,14 @@ enum {
     PLENS_PREC_FP32_BULK = 1
 };
 
+/* Variables of the BR1 gradient (lifting) step of calc_aux_vars (dgsem/plens.cc:2593-2748):
+ * conservative variables in the CVAR_GRAD build (calc_cell_cons_grad, 2174-2285), entropy
+ * variables otherwise (calc_cell_evar_grad, 2294-2405; the reference's default, main.cc:76-81). */
+enum {
+    PLENS_GRAD_CONSERVATIVE = 0,
+    PLENS_GRAD_ENTROPY = 1
+};
+
 typedef struct plens_gpu plens_gpu;
 
 /* Mesh and metric terms of one partition (all owned cells), DoF order = cell-major,
@@ -89,6 +97,7 @@ typedef struct {
     double alpha_min;              /* default 1e-3 */
     double alpha_max;              /* default 0.5 */
     double wall_blender_limit;     /* default 0.5 */
+    int32_t gradient_variables;    /* PLENS_GRAD_*: variables of the BR1 lifting */
 } plens_physics;
 
 typedef struct {

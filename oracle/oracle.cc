@@ -70,6 +70,20 @@ struct NS {
         return (gma - 1) * (c[4] - ske);
     }
     double get_a(const State& c) const { return std::sqrt(gma * get_p(c) / c[0]); }  // .cc:396-399
+    void cons_to_entropy(const State& c, State& w) const {    // navier_stokes.h:404-430
+        const double p = get_p(c);
+        const double s = std::log(p) - gma * std::log(c[0]);
+        const double beta = 0.5 * c[0] / p;
+        const double rhoinv = 1 / c[0];
+        double v[3], vv = 0;
+        for (int d = 0; d < 3; ++d) {
+            v[d] = c[1 + d] * rhoinv;
+            vv += v[d] * v[d];
+        }
+        w[0] = (gma - s) / (gma - 1) - beta * vv;
+        for (int d = 0; d < 3; ++d) w[1 + d] = 2 * beta * v[d];
+        w[4] = -2 * beta;
+    }
     double get_M(const State& c) const {                      // .cc:408-414
         double s = 0;
         const double ri = 1 / c[0];
@@ -554,6 +568,7 @@ struct oracle {
     // solution vectors
     std::vector<double> Q, avars, mu, kap, alpha, alpha_d, rhs, rhs1, rhs2, Qn, loc_dt;
     std::vector<double> s1, s2, s3;                     // surface terms [var][c][f][q]
+    std::vector<double> ws1;                            // entropy variables of s1 (default build)
     int64_t fail_dof = -1;
     int fail_kind = 0;
     double timers[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -837,6 +852,49 @@ struct oracle {
         }
     }
 
+    // ---- PLENS::calc_cell_evar_grad, dgsem/plens.cc:2294-2405 (default build) ------------------
+    // ws1: entropy variables of the stage-1 face values (calc_aux_vars, 2610-2637)
+    void calc_cell_evar_grad(int c, const std::vector<double>& ws1, std::vector<std::array<State, 3>>& g) const {
+        std::vector<State> ev(n);
+        for (int i = 0; i < n; ++i) ns.cons_to_entropy(cons(c, i), ev[i]);
+        const double ssign[2] = {1, -1};
+        for (int d = 0; d < 3; ++d)
+            for (int i = 0; i < n; ++i)
+                for (int v = 0; v < 5; ++v) g[i][d][v] = 0;
+        for (int gd = 0; gd < 3; ++gd) {
+            for (int it = 0; it < n; ++it) {
+                const int ti[3] = {it % np, (it / np) % np, it / (np * np)};
+                for (int m = 0; m <= N; ++m)
+                    for (int md = 0; md < 3; ++md) {
+                        int to[3] = {ti[0], ti[1], ti[2]};
+                        to[md] = m;
+                        const int io = tl(to[0], to[1], to[2]);
+                        const State eo = ev[io];
+                        const double Jc = JaV(c, io, md)[gd];
+                        const double Dv = D[ti[md] * np + m];
+                        for (int v = 0; v < 5; ++v) g[it][gd][v] += Dv * eo[v] * Jc;
+                    }
+            }
+            for (int sd = 0; sd < 3; ++sd)
+                for (int lr = 0; lr <= 1; ++lr) {
+                    const int f = 2 * sd + lr;
+                    for (int q = 0; q < nf; ++q) {
+                        const int node = face_map(N, f, q);
+                        const State e = ev[node];
+                        for (int v = 0; v < 5; ++v) {
+                            const double fin = e[v] * JaV(c, node, sd)[gd];
+                            const double fsu = ws1[sidx(v, c, f, q)] * JaV(c, node, sd)[gd];
+                            g[node][gd][v] -= ssign[lr] * (fsu - fin) / w[lr * N];
+                        }
+                    }
+                }
+            for (int i = 0; i < n; ++i) {
+                const double Ji = 1.0 / detJ[gid(c, i)];
+                for (int v = 0; v < 5; ++v) g[i][gd][v] *= Ji;
+            }
+        }
+    }
+
     // ---- PLENS::assert_positivity, dgsem/plens.cc:2531-2546 -----------------------------------
     bool assert_positivity() {
         for (int c = 0; c < nc; ++c)
@@ -848,7 +906,7 @@ struct oracle {
         return true;
     }
 
-    // ---- PLENS::calc_aux_vars, dgsem/plens.cc:2593-2748 (CVAR_GRAD build) ---------------------
+    // ---- PLENS::calc_aux_vars, dgsem/plens.cc:2593-2748 (both builds) --------------------------
     void calc_aux_vars() {
         const double cvv = ns.cv();
         for (size_t i = 0; i < ndof; ++i) {
@@ -859,6 +917,19 @@ struct oracle {
             kap[i] = ns.kappa(mu[i]);
         }
         calc_surf_flux(1, s1);
+        if (ph.gradient_variables == 1) {
+            // entropy variables of the stage-1 face values (dgsem/plens.cc:2610-2637)
+            ws1.assign(s1.size(), 0.0);
+            const size_t per_var = s1.size() / 5;
+            for (size_t k = 0; k < per_var; ++k) {
+                State cs, ws;
+                for (int v = 0; v < 5; ++v) cs[v] = s1[v * per_var + k];
+                ns.cons_to_entropy(cs, ws);
+                for (int v = 0; v < 5; ++v) ws1[v * per_var + k] = ws[v];
+            }
+            calc_aux_vars_entropy();
+            return;
+        }
 #pragma omp parallel
         {
             std::vector<std::array<State, 3>> g(n);
@@ -890,6 +961,42 @@ struct oracle {
                             eg[gd] -= (0.5 * g[i][gd][0] * vel[vd] * vel[vd] + s[1 + vd] * vg[vd][gd]) / s[0];
                     }
                     for (int d = 0; d < 3; ++d) avars[(6 + d) * ndof + gi] = -eg[d] / cvv;
+                }
+            }
+        }
+        for (size_t i = 0; i < ndof; ++i) {
+            for (int v = 0; v < 6; ++v) avars[v * ndof + i] *= mu[i];
+            for (int v = 6; v < 9; ++v) avars[v * ndof + i] *= kap[i];
+        }
+    }
+
+    // entropy-variable branch of calc_aux_vars (dgsem/plens.cc:2639-2746 without CVAR_GRAD)
+    void calc_aux_vars_entropy() {
+#pragma omp parallel
+        {
+            std::vector<std::array<State, 3>> g(n);
+#pragma omp for schedule(static)
+            for (int c = 0; c < nc; ++c) {
+                calc_cell_evar_grad(c, ws1, g);
+                for (int i = 0; i < n; ++i) {
+                    State et;
+                    ns.cons_to_entropy(cons(c, i), et);
+                    double vg[3][3];
+                    for (int vd = 0; vd < 3; ++vd)
+                        for (int gd = 0; gd < 3; ++gd)
+                            vg[vd][gd] = (et[1 + vd] * g[i][gd][4] / et[4] - g[i][gd][1 + vd]) / et[4];
+                    int sid = 0;
+                    double tr = 0;
+                    for (int d = 0; d < 3; ++d) tr += vg[d][d];
+                    const size_t gi = gid(c, i);
+                    for (int row = 0; row < 3; ++row)
+                        for (int col = row; col < 3; ++col) {
+                            if (col == row) avars[sid * ndof + gi] = vg[row][col] + vg[col][row] - 2.0 / 3 * tr;
+                            else avars[sid * ndof + gi] = vg[row][col] + vg[col][row];
+                            ++sid;
+                        }
+                    for (int d = 0; d < 3; ++d)
+                        avars[(6 + d) * ndof + gi] = -(g[i][d][4] / (et[4] * et[4] * ns.R()));
                 }
             }
         }

@@ -32,6 +32,7 @@ typedef struct {
     int32_t inv_surf_flux;  /* 0 HLLC, 1 Rusanov, 2 AUSM+-up, 3 Chandrashekhar */
     double threshold_factor, threshold_exp_factor, sharpness_factor, alpha_min, alpha_max,
         wall_blender_limit;
+    int32_t gradient_variables;  /* 0 conservative (CVAR_GRAD build), 1 entropy (default build) */
 } oracle_physics;
 
 typedef struct { int32_t type; double p, T, vel[3]; } oracle_bc; /* type as PLENS_BC_* */

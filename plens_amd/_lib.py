@@ -19,6 +19,7 @@ PLENS_E_NODEVICE = -5
 FLUX = {"HLLC": 0, "Rusanov": 1, "AUSM+-up": 2, "Chandrashekhar": 3}
 BC_PERIODIC, BC_UNIFORM_INFLOW, BC_OUTFLOW, BC_UNIFORM_TEMP_WALL = 0, 1, 2, 3
 PREC_FP64, PREC_FP32_BULK = 0, 1
+GRAD_CONSERVATIVE, GRAD_ENTROPY = 0, 1
 
 _dp = C.POINTER(C.c_double)
 _fp = C.POINTER(C.c_float)
@@ -37,7 +38,7 @@ class Physics(C.Structure):
                 ("inv_surf_flux", C.c_int32), ("precision", C.c_int32),
                 ("threshold_factor", C.c_double), ("threshold_exp_factor", C.c_double),
                 ("sharpness_factor", C.c_double), ("alpha_min", C.c_double), ("alpha_max", C.c_double),
-                ("wall_blender_limit", C.c_double)]
+                ("wall_blender_limit", C.c_double), ("gradient_variables", C.c_int32)]
 
 
 class BC(C.Structure):

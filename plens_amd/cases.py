@@ -96,7 +96,8 @@ class Mesh:
 
 def make_physics(gamma=1.4, prandtl=0.72, mu0=1e-3, T0=1.0, S=SUTH_RATIO, flux="HLLC",
                  precision=L.PREC_FP64, wall_blender_limit=0.5, alpha_max=0.5, alpha_min=1e-3,
-                 threshold_factor=0.5, threshold_exp_factor=1.8, sharpness_factor=9.21024):
+                 threshold_factor=0.5, threshold_exp_factor=1.8, sharpness_factor=9.21024,
+                 gradient_variables=L.GRAD_CONSERVATIVE):
     ph = L.Physics()
     ph.gamma, ph.molecular_weight, ph.prandtl = gamma, R0, prandtl
     ph.mu0, ph.T0, ph.S = mu0, T0, S
@@ -105,6 +106,7 @@ def make_physics(gamma=1.4, prandtl=0.72, mu0=1e-3, T0=1.0, S=SUTH_RATIO, flux="
     ph.threshold_factor, ph.threshold_exp_factor = threshold_factor, threshold_exp_factor
     ph.sharpness_factor, ph.alpha_min, ph.alpha_max = sharpness_factor, alpha_min, alpha_max
     ph.wall_blender_limit = wall_blender_limit
+    ph.gradient_variables = gradient_variables
     return ph
 
 
@@ -229,6 +231,15 @@ def batch_instance(i, seed0=100, min_n=4, max_n=32, flux="HLLC"):
     case.courant = 0.2 if (kind == "tube" and mach > 6.0) else 0.4
     case.meta.update({"instance": i, "kind": kind, "mach": mach, "reynolds": reynolds})
     return case
+
+
+def with_gradient_variables(case, variables):
+    """The same case with the BR1 lifting in "conservative" (CVAR_GRAD build) or "entropy"
+    variables (the reference's default build)."""
+    codes = {"conservative": L.GRAD_CONSERVATIVE, "entropy": L.GRAD_ENTROPY}
+    ph = L.Physics.from_buffer_copy(case.physics)
+    ph.gradient_variables = codes[variables]
+    return dataclasses.replace(case, physics=ph, meta={**case.meta, "gradient_variables": variables})
 
 
 def with_precision(case, precision):

@@ -105,6 +105,7 @@ struct KP {
     double* dt;              // time step of every instance
     double* t_acc;           // accumulated time of every instance
     int face_threads;        // workgroup size of the face kernel
+    int evar;                // BR1 lifting in entropy variables (default build) instead of conservative
     unsigned long long* fail;  // [0] lowest failing code of the run, [1] [0] latched after the first failing stage
     Gas gas;
     int scheme;
@@ -217,6 +218,22 @@ __host__ __device__ __forceinline__ void prim_to_cons(double gma, double rho, co
     }
 }
 
+// entropy variables of a conservative state (NavierStokes::cons_to_entropy, navier_stokes.h:404-430)
+__device__ __forceinline__ void cons_to_entropy(const double u[5], double gma, double w[5]) {
+    const double p = get_p(u, gma);
+    const double s = log(p) - gma * log(u[0]);
+    const double beta = 0.5 * u[0] / p;
+    const double rhoinv = 1.0 / u[0];
+    double vv = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        const double v = u[1 + d] * rhoinv;
+        vv += v * v;
+        w[1 + d] = 2.0 * beta * v;
+    }
+    w[0] = (gma - s) / (gma - 1.0) - beta * vv;
+    w[4] = -2.0 * beta;
+}
+
 // BC ghost state (IBCs/*.cc): stage 1 and 3 share the state rule, stage 2 has its own
 __device__ void bc_ghost_state(const BCDev& b, int stage, const Gas& g, const double u[5], const double n[3],
                                double ug[5]) {
@@ -295,7 +312,13 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
     if (blockIdx.x == 0 && tid == 0) p.bcount[0] = 0;  // blended-cell list of this stage
     if (act) {
         load_state(p.Q, p.ndof, g, u);
-        for (int v = 0; v < 5; ++v) sQ[v * B + li] = (TF)u[v];
+        if (p.evar) {  // the lifted variables: entropy variables (calc_cell_evar_grad, 2307-2326)
+            double w[5];
+            cons_to_entropy(u, p.gas.gma, w);
+            for (int v = 0; v < 5; ++v) sQ[v * B + li] = (TF)w[v];
+        } else {
+            for (int v = 0; v < 5; ++v) sQ[v * B + li] = (TF)u[v];
+        }
         double ja[9];
         node_metrics<N>(p, c, node, ja, detJ);
         for (int a = 0; a < 9; ++a) sJa[a * B + li] = (TF)ja[a];
@@ -400,7 +423,9 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
     }
     if (gs.inviscid) return;
 
-    // ---- BR1 conservative gradients (calc_cell_cons_grad, dgsem/plens.cc:2174-2285)
+    // ---- BR1 gradients: conservative variables, split-form volume term (calc_cell_cons_grad,
+    //      dgsem/plens.cc:2174-2285), or entropy variables, strong-form volume term with the other
+    //      node's metric terms (calc_cell_evar_grad, 2294-2405)
     if (!act) return;
     TF gr[3][5];
     for (int d = 0; d < 3; ++d)
@@ -408,18 +433,34 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
     TF ut[5];
     for (int v = 0; v < 5; ++v) ut[v] = (TF)u[v];
     const int tt[3] = {ti, tj, tk};
+    if (p.evar) {
 #pragma unroll 1
-    for (int m = 0; m <= N; ++m) {
-        for (int md = 0; md < 3; ++md) {
-            int to[3] = {ti, tj, tk};
-            to[md] = m;
-            const int jo = lc * NN + to[0] + NP * (to[1] + NP * to[2]);
-            TF fl[5];
-            for (int v = 0; v < 5; ++v) fl[v] = TF(0.5) * (ut[v] + sQ[v * B + jo]);
-            const TF Dm2 = (TF)(2.0 * sD[tt[md] * NP + m]);
-            for (int gd = 0; gd < 3; ++gd) {
-                const TF jav = TF(0.5) * (sJa[(3 * md + gd) * B + li] + sJa[(3 * md + gd) * B + jo]);
-                for (int v = 0; v < 5; ++v) gr[gd][v] += Dm2 * fl[v] * jav;
+        for (int m = 0; m <= N; ++m) {
+            for (int md = 0; md < 3; ++md) {
+                int to[3] = {ti, tj, tk};
+                to[md] = m;
+                const int jo = lc * NN + to[0] + NP * (to[1] + NP * to[2]);
+                const TF Dv = (TF)sD[tt[md] * NP + m];
+                for (int gd = 0; gd < 3; ++gd) {
+                    const TF jw = Dv * sJa[(3 * md + gd) * B + jo];
+                    for (int v = 0; v < 5; ++v) gr[gd][v] += sQ[v * B + jo] * jw;
+                }
+            }
+        }
+    } else {
+#pragma unroll 1
+        for (int m = 0; m <= N; ++m) {
+            for (int md = 0; md < 3; ++md) {
+                int to[3] = {ti, tj, tk};
+                to[md] = m;
+                const int jo = lc * NN + to[0] + NP * (to[1] + NP * to[2]);
+                TF fl[5];
+                for (int v = 0; v < 5; ++v) fl[v] = TF(0.5) * (ut[v] + sQ[v * B + jo]);
+                const TF Dm2 = (TF)(2.0 * sD[tt[md] * NP + m]);
+                for (int gd = 0; gd < 3; ++gd) {
+                    const TF jav = TF(0.5) * (sJa[(3 * md + gd) * B + li] + sJa[(3 * md + gd) * B + jo]);
+                    for (int v = 0; v < 5; ++v) gr[gd][v] += Dm2 * fl[v] * jav;
+                }
             }
         }
     }
@@ -439,10 +480,18 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
             bc_ghost_state(p.bcs[b], 1, gs, u, nullptr, ug);
             for (int v = 0; v < 5; ++v) us[v] = 0.5 * (u[v] + ug[v]);
         }
-        // g -= s (u* Ja - u Ja)[gd] / w_end, evaluated as (u* - u) (s Ja[gd] / w_end)
+        // g -= s (u* Ja - u Ja)[gd] / w_end, evaluated as (u* - u) (s Ja[gd] / w_end); in entropy
+        // variables u* is replaced by the entropy variables of the averaged conservative state
+        // (calc_aux_vars, dgsem/plens.cc:2621-2636)
         const TF sw = (TF)((lr ? -1.0 : 1.0) * sW[lr * N]);
         TF du[5];
-        for (int v = 0; v < 5; ++v) du[v] = (TF)(us[v] - u[v]);
+        if (p.evar) {
+            double ws[5];
+            cons_to_entropy(us, gs.gma, ws);
+            for (int v = 0; v < 5; ++v) du[v] = (TF)(ws[v] - (double)sQ[v * B + li]);
+        } else {
+            for (int v = 0; v < 5; ++v) du[v] = (TF)(us[v] - u[v]);
+        }
         for (int gd = 0; gd < 3; ++gd) {
             const TF jw = sJa[(3 * sd + gd) * B + li] * sw;
             for (int v = 0; v < 5; ++v) gr[gd][v] -= du[v] * jw;
@@ -453,25 +502,33 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
         for (int v = 0; v < 5; ++v) gr[gd][v] *= iJ;
 
     // ---- auxiliary variables (calc_aux_vars, dgsem/plens.cc:2656-2746)
-    const TF vel[3] = {(TF)(u[1] / u[0]), (TF)(u[2] / u[0]), (TF)(u[3] / u[0])};
-    TF vg[3][3];
-    for (int vd = 0; vd < 3; ++vd)
-        for (int gd = 0; gd < 3; ++gd) vg[vd][gd] = (gr[gd][1 + vd] - gr[gd][0] * vel[vd]) / ut[0];
+    TF vg[3][3], av[9];
+    if (p.evar) {  // velocity and temperature gradients from the entropy variables
+        TF wt[5];
+        for (int v = 0; v < 5; ++v) wt[v] = sQ[v * B + li];
+        for (int vd = 0; vd < 3; ++vd)
+            for (int gd = 0; gd < 3; ++gd) vg[vd][gd] = (wt[1 + vd] * gr[gd][4] / wt[4] - gr[gd][1 + vd]) / wt[4];
+        const TF iR = (TF)(1.0 / gs.R);
+        for (int gd = 0; gd < 3; ++gd) av[6 + gd] = -(gr[gd][4] / (wt[4] * wt[4]) * iR);
+    } else {
+        const TF vel[3] = {(TF)(u[1] / u[0]), (TF)(u[2] / u[0]), (TF)(u[3] / u[0])};
+        for (int vd = 0; vd < 3; ++vd)
+            for (int gd = 0; gd < 3; ++gd) vg[vd][gd] = (gr[gd][1 + vd] - gr[gd][0] * vel[vd]) / ut[0];
+        const TF et = (TF)e;
+        for (int gd = 0; gd < 3; ++gd) {
+            TF eg = (gr[gd][4] - gr[gd][0] * et) / ut[0];
+            for (int vd = 0; vd < 3; ++vd) eg -= (TF(0.5) * gr[gd][0] * vel[vd] * vel[vd] + ut[1 + vd] * vg[vd][gd]) / ut[0];
+            av[6 + gd] = -eg / (TF)gs.cv;
+        }
+    }
     const TF tr = vg[0][0] + vg[1][1] + vg[2][2];
     const TF c23 = (TF)(2.0 / 3);
-    TF av[9];
     av[0] = vg[0][0] + vg[0][0] - c23 * tr;
     av[1] = vg[0][1] + vg[1][0];
     av[2] = vg[0][2] + vg[2][0];
     av[3] = vg[1][1] + vg[1][1] - c23 * tr;
     av[4] = vg[1][2] + vg[2][1];
     av[5] = vg[2][2] + vg[2][2] - c23 * tr;
-    const TF et = (TF)e;
-    for (int gd = 0; gd < 3; ++gd) {
-        TF eg = (gr[gd][4] - gr[gd][0] * et) / ut[0];
-        for (int vd = 0; vd < 3; ++vd) eg -= (TF(0.5) * gr[gd][0] * vel[vd] * vel[vd] + ut[1 + vd] * vg[vd][gd]) / ut[0];
-        av[6 + gd] = -eg / (TF)gs.cv;
-    }
     TF* avo = tsel<TF>(p.avars, p.avars_f);
     for (int a = 0; a < 6; ++a) avo[a * p.ndof + g] = av[a] * (TF)mu;
     for (int a = 6; a < 9; ++a) avo[a * p.ndof + g] = av[a] * (TF)kap;
@@ -1439,6 +1496,8 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
     if (!m->vertices && (!m->Ja || !m->detJ)) return PLENS_E_INVALID;
     if (ph->inv_surf_flux < 0 || ph->inv_surf_flux > 3) return PLENS_E_INVALID;
     if (ph->precision != PLENS_PREC_FP64 && ph->precision != PLENS_PREC_FP32_BULK) return PLENS_E_INVALID;
+    if (ph->gradient_variables != PLENS_GRAD_CONSERVATIVE && ph->gradient_variables != PLENS_GRAD_ENTROPY)
+        return PLENS_E_INVALID;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PLENS_E_NODEVICE;
     const int N = m->degree, NP = N + 1, NN = NP * NP * NP, NF = NP * NP, nc = m->n_cells;
@@ -1593,6 +1652,7 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
     k.gas.gma = ph->gamma; k.gas.R = R; k.gas.cv = R / (ph->gamma - 1); k.gas.mu0 = ph->mu0; k.gas.T0 = ph->T0;
     k.gas.S = ph->S; k.gas.Pr = ph->prandtl; k.gas.inviscid = ph->mu0 < 1e-16;
     k.scheme = ph->inv_surf_flux;
+    k.evar = ph->gradient_variables == PLENS_GRAD_ENTROPY;
     k.thr = ph->threshold_factor * pow(10.0, -ph->threshold_exp_factor * (pow(N + 1, 0.25)));
     k.sharp = ph->sharpness_factor; k.amin = ph->alpha_min; k.amax = ph->alpha_max;
     k.wlimit = ph->wall_blender_limit;
@@ -1939,7 +1999,7 @@ namespace {
 bool same_scheme(const KP& a, const KP& b) {
     return a.gas.gma == b.gas.gma && a.gas.R == b.gas.R && a.gas.cv == b.gas.cv && a.gas.Pr == b.gas.Pr &&
            a.gas.inviscid == b.gas.inviscid && a.scheme == b.scheme && a.thr == b.thr && a.sharp == b.sharp &&
-           a.amin == b.amin && a.amax == b.amax && a.wlimit == b.wlimit;
+           a.amin == b.amin && a.amax == b.amax && a.wlimit == b.wlimit && a.evar == b.evar;
 }
 
 // Merge single-instance handles of one degree and precision into one handle on `stream`.

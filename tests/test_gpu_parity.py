@@ -166,3 +166,37 @@ def test_map_must_reproduce_given_metrics():
         GpuPlens(case)
     assert ei.value.code == PLENS_E_INVALID
     GpuPlens(case, metrics="stored")  # the arrays alone are accepted as given
+
+
+# ---- entropy-variable lifting (the reference's default build, calc_cell_evar_grad) ------------
+def _entropy(case):
+    from plens_amd import with_gradient_variables
+    return with_gradient_variables(case, "entropy")
+
+
+@pytest.mark.parametrize("flux", ["HLLC", "Chandrashekhar"])
+def test_rhs_entropy_lifting_cube(flux):
+    from plens_amd import periodic_cube
+    _rhs_parity(_entropy(periodic_cube(n=4, degree=3, flux=flux)), f"evar-cube-{flux}")
+
+
+def test_rhs_entropy_lifting_shock_tube():
+    from plens_amd import shock_tube
+    _rhs_parity(_entropy(shock_tube(nx=16, ny=2, nz=2)), "evar-shock")
+
+
+@pytest.mark.parametrize("degree", [2, 4])
+def test_rhs_entropy_lifting_wall_flow(degree):
+    from plens_amd import wall_flow
+    _rhs_parity(_entropy(wall_flow(degree=degree, nx=6, ny=4, nz=2)), f"evar-wall-N{degree}")
+
+
+def test_run_entropy_lifting():
+    from plens_amd import periodic_cube, wall_flow
+    for case, steps in ((_entropy(periodic_cube(n=4, degree=3)), 100), (_entropy(wall_flow(nx=6, ny=4, nz=2)), 10)):
+        g, o = _pair(case)
+        g.run(steps, case.courant)
+        o.run(steps, case.courant)
+        err = rel_l2(g.get_state(), o.get_state())
+        print(f"evar {case.name} {steps} steps rel L2 {err:.3e}", flush=True)
+        assert err <= RUN_TOL

@@ -248,6 +248,106 @@ def test_br1_gradient_exact_for_polynomials():
     assert cv > 0
 
 
+def test_br1_entropy_gradient_exact_for_polynomials():
+    """entropy-variable lifting (the reference's default build, calc_cell_evar_grad): a state whose
+    entropy variables are globally linear on perturbed trilinear cells, outflow (copy) ghosts
+    everywhere. The lifting of w is then exact, so tau and q equal the analytic values, obtained here
+    by central differences of the analytic velocity and temperature fields."""
+    N = 3
+    g = 1.4
+    # perturbed trilinear cells: w Ja^d is a polynomial of degree <= 3 per direction, so the
+    # strong-form lifting of a linear w is still exact (and needs each node's own metric terms)
+    m = cases.Mesh(N, 2, 2, 2, (0, 0, 0), (1, 1, 1), (0, 0, 0), [0] * 6, (-1, -1, -1), perturb_frac=0.1, seed=5)
+    ph = cases.make_physics(mu0=1.0, T0=1.0, S=0.4, gradient_variables=L.GRAD_ENTROPY)
+    case = cases.Case("egrad", m, ph, [cases.make_bc(L.BC_OUTFLOW, p=1.0)], np.zeros((5, m.n_dof)), 0.4, 1)
+
+    def w_of(X, Y, Z):  # linear entropy variables
+        return np.array([3.0 + 0.1 * Z, 0.05 * Y, 0.1 * Z, 0.02 * X, -(1.0 + 0.1 * X)])
+
+    def prim(X, Y, Z):  # (rho, v, p) from the entropy variables, R = 1
+        w = w_of(X, Y, Z)
+        beta = -0.5 * w[4]
+        v = w[1:4] / (2 * beta)
+        s = g - (g - 1) * (w[0] + beta * (v ** 2).sum(0))
+        rho = (2 * beta * np.exp(s)) ** (1 / (1 - g))
+        return rho, v, rho / (2 * beta)
+
+    X, Y, Z = (m.node_coords[:, d, :].reshape(-1) for d in range(3))
+    rho, v, p = prim(X, Y, Z)
+    Q = np.vstack([rho, rho * v, p / (g - 1) + 0.5 * rho * (v ** 2).sum(0)])
+    o = PO.OraclePlens(case)
+    o.set_state(Q)
+    _, _, _, av = o.calc_rhs(want_avars=True)
+    h = 1e-5
+    gv = np.zeros((3, 3, X.size))  # d v_i / d x_j
+    gT = np.zeros((3, X.size))
+    for j in range(3):
+        dx = [0.0, 0.0, 0.0]
+        dx[j] = h
+        rp, vp, pp = prim(X + dx[0], Y + dx[1], Z + dx[2])
+        rm, vm, pm = prim(X - dx[0], Y - dx[1], Z - dx[2])
+        gv[:, j] = (vp - vm) / (2 * h)
+        gT[j] = (pp / rp - pm / rm) / (2 * h)
+    T = p / rho
+    mu = 1.0 * T ** 1.5 * 1.4 / (T + 0.4)
+    k = mu * g / (0.4 * 0.72)
+    tr = gv[0, 0] + gv[1, 1] + gv[2, 2]
+    idx = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    for s_, (i, j) in enumerate(idx):
+        tau = gv[i, j] + gv[j, i] - (2.0 / 3 * tr if i == j else 0.0)
+        assert np.abs(av[s_] - mu * tau).max() < 1e-8 * np.abs(mu * tau).max() + 1e-9, s_
+    for d in range(3):
+        assert np.abs(av[6 + d] + k * gT[d]).max() < 1e-8 * np.abs(k * gT).max() + 1e-9, d
+    # the conservative-variable lifting of the same (non-polynomial) conservative state is not exact
+    case_c = cases.Case("cgrad", m, cases.make_physics(mu0=1.0, T0=1.0, S=0.4),
+                        [cases.make_bc(L.BC_OUTFLOW, p=1.0)], Q, 0.4, 1)
+    oc = PO.OraclePlens(case_c)
+    _, _, _, avc = oc.calc_rhs(want_avars=True)
+    assert np.abs(avc - av).max() > 1e-7
+
+
+def test_br1_entropy_jump_lifting_closed_form():
+    """two cells with constant entropy variables that jump in w4 (velocity 0): the volume term
+    vanishes and the BR1 surface term lifts the jump, g_x = +-(w4* - w4) / (w_N h) on the shared face
+    nodes and 0 elsewhere (outflow copy ghosts outside), where w* are the entropy variables of the
+    averaged conservative state (calc_aux_vars, dgsem/plens.cc:2621-2636), here w4* = -rho*/p*;
+    q_x = -k dT/dx with T = -1/(R w4), so dT/dx = g_x / (R w4^2)"""
+    N, g, h = 3, 1.4, 0.5
+    m = cases.Mesh(N, 2, 1, 1, (0, 0, 0), (1.0, 0.5, 0.5), (0, 0, 0), [0] * 6, (-1, -1, -1))
+    ph = cases.make_physics(mu0=1.0, T0=1.0, S=0.4, gradient_variables=L.GRAD_ENTROPY)
+    case = cases.Case("jump", m, ph, [cases.make_bc(L.BC_OUTFLOW, p=1.0)], np.zeros((5, m.n_dof)), 0.4, 1)
+    w4 = np.array([-1.0, -1.2])
+    n = m.n
+    Q = np.zeros((5, m.n_dof))
+    for c in range(2):
+        beta = -0.5 * w4[c]
+        s = g - (g - 1) * 3.0  # w0 = 3, v = 0
+        rho = (2 * beta * np.exp(s)) ** (1 / (1 - g))
+        p = rho / (2 * beta)
+        Q[0, c * n:(c + 1) * n] = rho
+        Q[4, c * n:(c + 1) * n] = p / (g - 1)
+    o = PO.OraclePlens(case)
+    o.set_state(Q)
+    _, _, _, av = o.calc_rhs(want_avars=True)
+    wN = 1.0 / 12.0  # GLL end weight on [0,1] at N=3
+    np_ = N + 1
+    rho_s = 0.5 * (Q[0, 0] + Q[0, n])
+    p_s = 0.5 * (Q[4, 0] + Q[4, n]) * (g - 1)
+    w4s = -rho_s / p_s
+    for c, (own, face_i) in enumerate(((w4[0], N), (w4[1], 0))):
+        rho = Q[0, c * n]
+        T = (Q[4, c * n] * (g - 1)) / rho
+        mu = T ** 1.5 * 1.4 / (T + 0.4)
+        k = mu * g / (0.4 * 0.72)
+        gx = (w4s - own) / (wN * h) * (1 if face_i == N else -1)
+        for node in range(n):
+            on = node % np_ == face_i
+            expect = -k * gx / (own * own) if on else 0.0
+            assert abs(av[6, c * n + node] - expect) < 1e-10 * abs(k * gx), (c, node)
+            assert abs(av[7, c * n + node]) < 1e-12 and abs(av[8, c * n + node]) < 1e-12
+            assert np.abs(av[:6, c * n + node]).max() < 1e-12
+
+
 def test_rk4_temporal_order():
     """fixed-dt runs to the same final time: the low-storage register update converges at 4th order
     (blender switched off with alpha_min = 1 so the right-hand side is smooth in time)"""
