@@ -48,7 +48,14 @@ enum {
     PLENS_BC_PERIODIC = 0,          /* IBCs/periodic.cc:121-165 */
     PLENS_BC_UNIFORM_INFLOW = 1,    /* IBCs/uniform_inflow.cc:23-77, state from (p, T, vel) */
     PLENS_BC_OUTFLOW = 2,           /* IBCs/outflow.cc:18-75, prescribed p */
-    PLENS_BC_UNIFORM_TEMP_WALL = 3  /* IBCs/uniform_temp_wall.cc:17-104, (T, vel) */
+    PLENS_BC_UNIFORM_TEMP_WALL = 3, /* IBCs/uniform_temp_wall.cc:17-104, (T, vel) */
+    PLENS_BC_FREE = 4,              /* IBCs/free.cc, inner state and avars */
+    PLENS_BC_SYMMETRY = 5,          /* IBCs/symmetry.cc */
+    PLENS_BC_EMPTY = 6,             /* IBCs/empty.cc */
+    PLENS_BC_INSULATED_WALL = 7,    /* IBCs/insulated_wall.cc, (vel); the stage-3 heat flux has its
+                                     * normal component reversed as documented there (the
+                                     * reference's copy loop at line 99 indexes past the 3-vector) */
+    PLENS_BC_ZPG_INFLOW = 8         /* IBCs/zpg_inflow.cc, (vel) with inner density and pressure */
 };
 
 /* Arithmetic of the flux evaluations. FP64 registers/residual accumulation in both. */

@@ -725,7 +725,43 @@ struct oracle {
                 } else g = in;
                 break;
             }
-            default: {  // uniform temperature wall (uniform_temp_wall.cc:17-104)
+            case 4:  // free (free.cc:10-60): inner state and avars at every stage
+                g = in;
+                break;
+            case 5:    // symmetry (symmetry.cc:17-96)
+            case 6: {  // empty (empty.cc): as symmetry, except the stage-2 ghost
+                T1 vin, vt;
+                for (int d = 0; d < 3; ++d) vin[d] = in[1 + d] / in[0];
+                const double vn = dot(vin, nrm);
+                for (int d = 0; d < 3; ++d) vt[d] = vin[d] - vn * nrm[d];
+                const double pin = ns.get_p(in);
+                State pr;
+                ns.prim_to_cons(in[0], vt, pin, pr);
+                if (stage != 2) {
+                    for (int k = 0; k < 5; ++k) g[k] = 2 * pr[k] - in[k];
+                } else if (b.type == 6) {
+                    g = pr;
+                } else {
+                    T1 sm;
+                    for (int d = 0; d < 3; ++d) sm[d] = in[1 + d];
+                    const double smn = dot(sm, nrm);
+                    g[0] = in[0];
+                    g[4] = in[4];
+                    for (int d = 0; d < 3; ++d) g[1 + d] = sm[d] - 2 * smn * nrm[d];
+                }
+                break;
+            }
+            case 8: {  // ZPG inflow (zpg_inflow.cc): prescribed velocity, inner density and pressure
+                T1 vw; vw[0] = b.vel[0]; vw[1] = b.vel[1]; vw[2] = b.vel[2];
+                State pr;
+                ns.prim_to_cons(in[0], vw, ns.get_p(in), pr);
+                if (stage == 2) g = pr;
+                else for (int k = 0; k < 5; ++k) g[k] = 2 * pr[k] - in[k];
+                break;
+            }
+            default: {  // uniform temperature wall (uniform_temp_wall.cc:17-104), insulated wall
+                        // (insulated_wall.cc): the same stage-2 rule; stage 1/3 use the inner
+                        // temperature for the insulated wall
                 T1 vw; vw[0] = b.vel[0]; vw[1] = b.vel[1]; vw[2] = b.vel[2];
                 if (stage == 2) {
                     g[0] = in[0];
@@ -741,9 +777,17 @@ struct oracle {
                 } else {
                     State pr;
                     const double pin = ns.get_p(in);
-                    const double rpr = pin / (ns.R() * b.T);
+                    const double rpr = (b.type == 7) ? in[0] : pin / (ns.R() * b.T);
                     ns.prim_to_cons(rpr, vw, pin, pr);
                     for (int k = 0; k < 5; ++k) g[k] = 2 * pr[k] - in[k];
+                    if (b.type == 7 && stage == 3) {
+                        // heat flux with its normal component reversed, shear stresses kept
+                        // (insulated_wall.cc:95-101 as documented there; the reference's loop writes
+                        // heat_flux_in[6..8], out of the bounds of the 3-vector, at line 99)
+                        double qn = 0;
+                        for (int d = 0; d < 3; ++d) qn += ain[6 + d] * nrm[d];
+                        for (int d = 0; d < 3; ++d) ag[6 + d] = ain[6 + d] - 2 * qn * nrm[d];
+                    }
                 }
                 break;
             }
@@ -1025,7 +1069,8 @@ struct oracle {
             alpha[c] = raw[c];
             for (int f = 0; f < 6; ++f) {
                 const int b = bid[(size_t)c * 6 + f];
-                if (b >= 0 && bcs[b].type == 3) alpha[c] = std::max(ph.wall_blender_limit, raw[c]);
+                if (b >= 0 && (bcs[b].type == 3 || bcs[b].type == 7))  // uniform temp / insulated wall
+                    alpha[c] = std::max(ph.wall_blender_limit, raw[c]);
             }
         }
         const std::vector<double> gh = alpha;  // gh_gcrk_alpha = gcrk_alpha

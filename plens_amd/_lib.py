@@ -18,6 +18,7 @@ PLENS_E_NODEVICE = -5
 
 FLUX = {"HLLC": 0, "Rusanov": 1, "AUSM+-up": 2, "Chandrashekhar": 3}
 BC_PERIODIC, BC_UNIFORM_INFLOW, BC_OUTFLOW, BC_UNIFORM_TEMP_WALL = 0, 1, 2, 3
+BC_FREE, BC_SYMMETRY, BC_EMPTY, BC_INSULATED_WALL, BC_ZPG_INFLOW = 4, 5, 6, 7, 8
 PREC_FP64, PREC_FP32_BULK = 0, 1
 GRAD_CONSERVATIVE, GRAD_ENTROPY = 0, 1
 

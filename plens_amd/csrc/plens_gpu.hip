@@ -234,7 +234,8 @@ __device__ __forceinline__ void cons_to_entropy(const double u[5], double gma, d
     w[4] = -2.0 * beta;
 }
 
-// BC ghost state (IBCs/*.cc): stage 1 and 3 share the state rule, stage 2 has its own
+// BC ghost state (IBCs/*.cc): stage 1 and 3 share the state rule, stage 2 has its own; n is the
+// outward unit normal of the face node
 __device__ void bc_ghost_state(const BCDev& b, int stage, const Gas& g, const double u[5], const double n[3],
                                double ug[5]) {
     if (b.type == PLENS_BC_UNIFORM_INFLOW) {
@@ -252,7 +253,38 @@ __device__ void bc_ghost_state(const BCDev& b, int stage, const Gas& g, const do
             const double M = sqrt(sp) / get_a(u, g.gma);
             if (!(M >= 1.0)) ug[4] += 2.0 * (b.p - get_p(u, g.gma)) / (g.gma - 1.0);
         }
-    } else {  // uniform temperature wall
+    } else if (b.type == PLENS_BC_FREE) {  // free.cc: the inner state
+        for (int v = 0; v < 5; ++v) ug[v] = u[v];
+    } else if (b.type == PLENS_BC_SYMMETRY || b.type == PLENS_BC_EMPTY) {
+        // symmetry.cc / empty.cc: stage 1/3 mirror about the state with the normal velocity removed;
+        // stage 2 reflects the normal momentum (symmetry) or is that state itself (empty)
+        if (stage == 2 && b.type == PLENS_BC_SYMMETRY) {
+            const double smn = u[1] * n[0] + u[2] * n[1] + u[3] * n[2];
+            ug[0] = u[0];
+            ug[4] = u[4];
+            for (int d = 0; d < 3; ++d) ug[1 + d] = u[1 + d] - 2.0 * smn * n[d];
+        } else {
+            const double ri = 1.0 / u[0];
+            const double vin[3] = {u[1] * ri, u[2] * ri, u[3] * ri};
+            const double vn = vin[0] * n[0] + vin[1] * n[1] + vin[2] * n[2];
+            const double vt[3] = {vin[0] - vn * n[0], vin[1] - vn * n[1], vin[2] - vn * n[2]};
+            double pr[5];
+            prim_to_cons(g.gma, u[0], vt, get_p(u, g.gma), pr);
+            if (stage == 2) {
+                for (int v = 0; v < 5; ++v) ug[v] = pr[v];
+            } else {
+                for (int v = 0; v < 5; ++v) ug[v] = 2.0 * pr[v] - u[v];
+            }
+        }
+    } else if (b.type == PLENS_BC_ZPG_INFLOW) {  // zpg_inflow.cc: prescribed velocity, inner rho and p
+        double pr[5];
+        prim_to_cons(g.gma, u[0], b.vel, get_p(u, g.gma), pr);
+        if (stage == 2) {
+            for (int v = 0; v < 5; ++v) ug[v] = pr[v];
+        } else {
+            for (int v = 0; v < 5; ++v) ug[v] = 2.0 * pr[v] - u[v];
+        }
+    } else {  // uniform temperature wall (uniform_temp_wall.cc), insulated wall (insulated_wall.cc)
         if (stage == 2) {
             ug[0] = u[0];
             ug[4] = u[4];
@@ -266,7 +298,8 @@ __device__ void bc_ghost_state(const BCDev& b, int stage, const Gas& g, const do
         } else {
             const double pin = get_p(u, g.gma);
             double pr[5];
-            prim_to_cons(g.gma, pin / (g.R * b.T), b.vel, pin, pr);
+            const double rho = (b.type == PLENS_BC_INSULATED_WALL) ? u[0] : pin / (g.R * b.T);
+            prim_to_cons(g.gma, rho, b.vel, pin, pr);
             for (int v = 0; v < 5; ++v) ug[v] = 2.0 * pr[v] - u[v];
         }
     }
@@ -476,8 +509,9 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
             load_state(p.Q, p.ndof, (long long)nb * NN + face_node<N>(f ^ 1, q), un);
             for (int v = 0; v < 5; ++v) us[v] = 0.5 * (u[v] + un[v]);
         } else {
-            double ug[5];
-            bc_ghost_state(p.bcs[b], 1, gs, u, nullptr, ug);
+            double ug[5], nf[3];
+            for (int d = 0; d < 3; ++d) nf[d] = p.nrm[((long long)(c * 6 + f) * 3 + d) * (NP * NP) + q];
+            bc_ghost_state(p.bcs[b], 1, gs, u, nf, ug);
             for (int v = 0; v < 5; ++v) us[v] = 0.5 * (u[v] + ug[v]);
         }
         // g -= s (u* Ja - u Ja)[gd] / w_end, evaluated as (u* - u) (s Ja[gd] / w_end); in entropy
@@ -752,6 +786,12 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
         for (int a = 0; a < 9; ++a) avg[a] = avars[a * p.ndof + gn];
     } else {
         for (int a = 0; a < 9; ++a) avg[a] = av[a];
+        if (p.bcs[b].type == PLENS_BC_INSULATED_WALL) {
+            // heat flux with its normal component reversed, shear stresses kept (insulated_wall.cc,
+            // as documented there; its copy loop at line 99 indexes past the 3-vector)
+            const TF qn = avg[6] * nt[0] + avg[7] * nt[1] + avg[8] * nt[2];
+            for (int d = 0; d < 3; ++d) avg[6 + d] = av[6 + d] - TF(2) * qn * nt[d];
+        }
         if (p.bcs[b].type != PLENS_BC_OUTFLOW) bc_ghost_state(p.bcs[b], 3, gs, u, n, ug);
         else for (int v = 0; v < 5; ++v) ug[v] = u[v];
         for (int v = 0; v < 5; ++v) ugt[v] = (TF)ug[v];
@@ -1508,7 +1548,7 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
             if (b >= n_bcs) return PLENS_E_INVALID;
             const bool needs_nbr = b < 0 || bcs[b].type == PLENS_BC_PERIODIC;
             if (needs_nbr && (nb < 0 || nb >= nc)) return PLENS_E_INVALID;
-            if (b >= 0 && (bcs[b].type < 0 || bcs[b].type > 3)) return PLENS_E_INVALID;
+            if (b >= 0 && (bcs[b].type < 0 || bcs[b].type > PLENS_BC_ZPG_INFLOW)) return PLENS_E_INVALID;
         }
     plens_gpu* h = new (std::nothrow) plens_gpu;
     if (!h) return PLENS_E_NOMEM;
@@ -1533,7 +1573,7 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
             const int b = m->face_bid[c * 6 + f];
             if (b >= 0) {
                 tasks.push_back(c * 6 + f);
-                if (bcs[b].type == PLENS_BC_UNIFORM_TEMP_WALL) wall[c] = 1;
+                if (bcs[b].type == PLENS_BC_UNIFORM_TEMP_WALL || bcs[b].type == PLENS_BC_INSULATED_WALL) wall[c] = 1;
             } else if (c < m->face_nbr[c * 6 + f]) {
                 tasks.push_back(c * 6 + f);
             }

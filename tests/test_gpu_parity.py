@@ -200,3 +200,30 @@ def test_run_entropy_lifting():
         err = rel_l2(g.get_state(), o.get_state())
         print(f"evar {case.name} {steps} steps rel L2 {err:.3e}", flush=True)
         assert err <= RUN_TOL
+
+
+# ---- the remaining boundary conditions on the y faces, with y velocity and y variation ----------
+@pytest.mark.parametrize("bc_type", ["free", "symmetry", "empty", "insulated_wall", "zpg_inflow"])
+@pytest.mark.parametrize("wall_limit", [0.0, 0.5])
+def test_rhs_and_steps_remaining_bcs(bc_type, wall_limit):
+    from plens_amd import _lib as L
+    from plens_amd.cases import make_bc
+    from tests.test_host_and_assembly import y_faces_case
+    bc = {"free": make_bc(L.BC_FREE), "symmetry": make_bc(L.BC_SYMMETRY), "empty": make_bc(L.BC_EMPTY),
+          "insulated_wall": make_bc(L.BC_INSULATED_WALL, vel=(0.3, 0.0, 0.1)),
+          "zpg_inflow": make_bc(L.BC_ZPG_INFLOW, vel=(0.5, 0.05, 0.0))}[bc_type]
+    case = y_faces_case(bc, n=3, N=3, mu0=2e-3)
+    case.physics.wall_blender_limit = wall_limit
+    rng = np.random.default_rng(7)
+    X, Y = (case.mesh.node_coords[:, d, :].reshape(-1) for d in (0, 1))
+    Q = case.Q0.copy()
+    Q[0] *= 1 + 0.05 * np.cos(np.pi * Y) + 0.01 * rng.uniform(-1, 1, X.size)
+    Q[2] = Q[0] * (0.1 * np.sin(2 * np.pi * X) + 0.05)
+    Q[4] = 1.0 / 0.4 + 0.5 * (Q[1] ** 2 + Q[2] ** 2) / Q[0]
+    case.Q0 = Q
+    g, o = _rhs_parity(case, f"bc-{bc_type}-limit{wall_limit}")
+    g.run(10, case.courant)
+    o.run(10, case.courant)
+    err = rel_l2(g.get_state(), o.get_state())
+    print(f"bc-{bc_type} 10 steps rel L2 {err:.3e}", flush=True)
+    assert err <= RUN_TOL

@@ -374,3 +374,72 @@ def test_oracle_positivity_failure_lowest_dof():
     o.set_state(Q)
     with pytest.raises(RuntimeError, match="dof=50 kind=2"):
         o.calc_rhs()
+
+
+# ---- the remaining boundary conditions (IBCs/free.cc, symmetry.cc, empty.cc, insulated_wall.cc,
+#      zpg_inflow.cc) ----------------------------------------------------------------------------
+def y_faces_case(bc, n=3, N=3, mu0=1e-3, flux="HLLC"):
+    """x-periodic and z-periodic box with `bc` on both y faces (or periodic y when bc is None) and a
+    density wave along x carried by u = (0.5, 0, 0), p = 1: the state does not depend on y and has
+    no y velocity"""
+    per = bc is None
+    bcs = [cases.make_bc(L.BC_PERIODIC)] + ([] if per else [bc])
+    m = cases.Mesh(N, n, n, n, (0, 0, 0), (1, 1, 1), (1, 1 if per else 0, 1),
+                   [-1, -1, -1 if per else 1, -1 if per else 1, -1, -1], (0, 0 if per else -1, 0))
+    ph = cases.make_physics(mu0=mu0, flux=flux, wall_blender_limit=0.0)
+    X = m.node_coords[:, 0, :].reshape(-1)
+    rho = 1 + 0.1 * np.sin(2 * np.pi * X)
+    Q = np.vstack([rho, 0.5 * rho, 0 * X, 0 * X, 1.0 / 0.4 + 0.5 * rho * 0.25])
+    return cases.Case("yfaces", m, ph, bcs, Q, 0.4, 1)
+
+
+@pytest.mark.parametrize("bc", [
+    cases.make_bc(L.BC_FREE), cases.make_bc(L.BC_SYMMETRY), cases.make_bc(L.BC_EMPTY),
+    cases.make_bc(L.BC_INSULATED_WALL, vel=(0.5, 0, 0)), cases.make_bc(L.BC_ZPG_INFLOW, vel=(0.5, 0, 0))],
+    ids=["free", "symmetry", "empty", "insulated_wall", "zpg_inflow"])
+def test_y_face_bcs_reproduce_periodic_for_y_independent_flow(bc):
+    """for a y-independent state without y velocity (and, for the wall / ZPG inflow, the wall / inflow
+    velocity equal to the flow velocity) every one of these ghost states equals the inner state, so
+    the RHS equals that of the y-periodic box"""
+    r_bc, a_bc, _ = PO.OraclePlens(y_faces_case(bc)).calc_rhs()
+    r_p, a_p, _ = PO.OraclePlens(y_faces_case(None)).calc_rhs()
+    assert np.abs(r_p).max() > 1e-2
+    assert np.abs(r_bc - r_p).max() < 1e-11 * np.abs(r_p).max()
+    assert np.array_equal(a_bc > 0, a_p > 0)
+
+
+def test_symmetry_bc_equals_mirrored_domain_inviscid():
+    """inviscid flow with y velocity through the symmetry planes: the half box y in [0,1] with
+    symmetry on both y faces gives the RHS of the upper half of the y-periodic box y in [-1,1]
+    holding the mirrored state (rho, p even about y = 0; v_y = +-(0.2 + 0.1 sin 2 pi x) in the upper /
+    lower cells, so the mirrored state jumps at y = 0 and y = +-1): the stage-2 symmetry ghost is the
+    mirror image of the inner state and the flux blender of a boundary face is the cell's own value"""
+    N, nx, m = 3, 4, 3
+    g = 1.4
+
+    def state(mesh, lower_cells):
+        X, Y = (mesh.node_coords[:, d, :].reshape(-1) for d in (0, 1))
+        sgn = np.ones(mesh.n_cells)
+        sgn[lower_cells] = -1.0
+        sgn = np.repeat(sgn, mesh.n)
+        rho = 1 + 0.1 * np.sin(2 * np.pi * X) * np.cos(np.pi * Y)
+        vx, vy = 0.5 + 0 * X, sgn * (0.2 + 0.1 * np.sin(2 * np.pi * X))
+        p = 1 + 0.05 * np.cos(2 * np.pi * X) * np.cos(np.pi * Y)
+        return np.vstack([rho, rho * vx, rho * vy, 0 * X, p / (g - 1) + 0.5 * rho * (vx ** 2 + vy ** 2)])
+
+    ph = cases.make_physics(mu0=0.0, flux="HLLC", wall_blender_limit=0.0)
+    half = cases.Mesh(N, nx, m, 2, (0, 0, 0), (1, 1, 0.5), (1, 0, 1), [-1, -1, 1, 1, -1, -1], (0, -1, 0))
+    full = cases.Mesh(N, nx, 2 * m, 2, (0, -1, 0), (1, 1, 0.5), (1, 1, 1), [-1] * 6, (0, 0, 0))
+    bh = [cases.make_bc(L.BC_PERIODIC), cases.make_bc(L.BC_SYMMETRY)]
+    full_lower = [nx * (j + 2 * m * k) + i for k in range(2) for j in range(m) for i in range(nx)]
+    ch = cases.Case("half", half, ph, bh, state(half, []), 0.4, 1)
+    cf = cases.Case("full", full, ph, [cases.make_bc(L.BC_PERIODIC)], state(full, full_lower), 0.4, 1)
+    rh, _, _ = PO.OraclePlens(ch).calc_rhs()
+    rf, _, _ = PO.OraclePlens(cf).calc_rhs()
+    n = half.n
+    upper = np.concatenate([np.arange(c * n, (c + 1) * n) for k in range(2) for j in range(m, 2 * m)
+                            for c in [nx * (j + 2 * m * k) + i for i in range(nx)]])
+    lower_half = np.concatenate([np.arange(c * n, (c + 1) * n) for k in range(2) for j in range(m)
+                                 for c in [nx * (j + m * k) + i for i in range(nx)]])
+    assert np.abs(rh).max() > 1e-2
+    assert np.abs(rh[:, lower_half] - rf[:, upper]).max() < 1e-10 * np.abs(rf).max()
