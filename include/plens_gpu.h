@@ -154,6 +154,21 @@ int plens_gpu_set_tuning(plens_gpu* h, int32_t cells_per_block);
  * handle's stream): times_ms[0..4] = gradient/blender, alpha, face flux, subcell FV, volume+RK. */
 int plens_gpu_kernel_times(const plens_gpu* h, double* times_ms, int32_t n);
 
+/* Local time stepping ("time integration" parameters request local stepping, local stepping start
+ * time, limit local time step, local step neighbor/global factor; PLENS::calc_time_step,
+ * dgsem/plens.cc:3877-3911): once the solution time at the start of a step exceeds start_time, every
+ * cell advances with its own stable step, limited (limit != 0) to neighbor_factor x each interior
+ * neighbour's step and global_factor x the global step; the returned time advances by the global
+ * step. The solution time is taken from *t_inout of plens_gpu_run/rk4_step (NULL: continues). */
+int plens_gpu_set_local_stepping(plens_gpu* h, int32_t request, double start_time, int32_t limit,
+                                 double neighbor_factor, double global_factor);
+
+/* Steady-state error after the last step (PLENS::calc_ss_error, dgsem/plens.cc:3985-4035):
+ * ||(rhoE - rhoE_start)/dt_cell||_L2 / ||rhoE||_L2 with QGauss(N+1) through the mapping, rhoE_start
+ * the energy at the start of that step and dt_cell its (local or global) step. PLENS_E_INVALID
+ * before the first step. */
+int plens_gpu_ss_error(plens_gpu* h, double* err);
+
 /* Kernel timing with no counterpart in the reference: after one RHS evaluation of the current
  * state, launches kernel class `kernel` (0 gradient/blender, 2 face flux, 4 volume + RK update)
  * back to back on the handle's stream, `reps` times with the parameters of each of the 5 RK stages

@@ -569,6 +569,11 @@ struct oracle {
     std::vector<double> Q, avars, mu, kap, alpha, alpha_d, rhs, rhs1, rhs2, Qn, loc_dt;
     std::vector<double> s1, s2, s3;                     // surface terms [var][c][f][q]
     std::vector<double> ws1;                            // entropy variables of s1 (default build)
+    std::vector<double> verts;                          // cell vertices [c][8][3]
+    std::vector<double> rhoE_old;                       // rho E at the start of the last step
+    // local time stepping ("time integration" parameters, dgsem/plens.cc:540-575, 1481-1482)
+    int lts_request = 0, lts_limit = 0;
+    double lts_start = 100, lts_nf = 1.05, lts_gf = 1.0;
     int64_t fail_dof = -1;
     int fail_kind = 0;
     double timers[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -1260,10 +1265,12 @@ struct oracle {
         return 0;
     }
 
-    // ---- PLENS::calc_time_step (global stepping), dgsem/plens.cc:3810-3912 -----------------------
-    double calc_time_step(double Co) {
+    // ---- PLENS::calc_time_step, dgsem/plens.cc:3810-3912: per-cell steps into loc_dt, then global
+    //      stepping (every cell gets the minimum) or, past the start time, local stepping
+    double calc_time_step(double Co, double cur_time = 0.0) {
         if (Co >= 1) Co = 0.9;
         double proc = 1e6;
+        loc_dt.assign(nc, 0.0);
         for (int c = 0; c < nc; ++c) {
             double cs = 1e6;
             const double h = hmin[c];
@@ -1279,9 +1286,70 @@ struct oracle {
                 const double cur = Co / (std::pow(N, 1.5) * (madv + a / h + N * N * mu[gid(c, i)] / (h * h * s[0])));
                 if (cur < cs) cs = cur;
             }
+            loc_dt[c] = cs;
             if (cs < proc) proc = cs;
         }
+        if (lts_request && cur_time > lts_start) {
+            const std::vector<double> gh = loc_dt;  // gh_loc_time_steps
+            for (int c = 0; c < nc; ++c) {
+                double m = gh[c];
+                if (lts_limit) {
+                    for (int f = 0; f < 6; ++f)
+                        if (bid[(size_t)c * 6 + f] < 0) m = std::min(m, lts_nf * gh[nbr[(size_t)c * 6 + f]]);
+                    m = std::min(m, lts_gf * proc);
+                }
+                loc_dt[c] = m;
+            }
+        } else {
+            for (int c = 0; c < nc; ++c) loc_dt[c] = proc;
+        }
         return proc;
+    }
+
+    // ---- PLENS::calc_ss_error, dgsem/plens.cc:3985-4035: L2 norm of (rhoE - rhoE_old)/dt_cell over
+    //      the L2 norm of rhoE, both by QGauss(N+1) through the trilinear mapping
+    double calc_ss_error() const {
+        std::vector<double> xg, wg;
+        gauss(np, xg, wg);
+        std::vector<double> L((size_t)np * np);  // L[q][i] = l_i(xg_q), GLL Lagrange basis
+        for (int q = 0; q < np; ++q)
+            for (int i = 0; i < np; ++i) L[q * np + i] = (double)lagrange_val(x1, i, xg[q]);
+        double num = 0, den = 0;
+        for (int c = 0; c < nc; ++c) {
+            const double* cv = &verts[(size_t)c * 24];
+            for (int qk = 0; qk < np; ++qk)
+                for (int qj = 0; qj < np; ++qj)
+                    for (int qi = 0; qi < np; ++qi) {
+                        const double xi[3] = {xg[qi], xg[qj], xg[qk]};
+                        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                        for (int v = 0; v < 8; ++v) {
+                            const int bb[3] = {v & 1, (v >> 1) & 1, (v >> 2) & 1};
+                            for (int b = 0; b < 3; ++b) {
+                                double gv = bb[b] ? 1.0 : -1.0;
+                                for (int oo = 0; oo < 3; ++oo)
+                                    if (oo != b) gv *= bb[oo] ? xi[oo] : 1.0 - xi[oo];
+                                for (int a = 0; a < 3; ++a) J[a][b] += gv * cv[3 * v + a];
+                            }
+                        }
+                        const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                                           J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                                           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+                        const double jxw = std::fabs(det) * wg[qi] * wg[qj] * wg[qk];
+                        double e = 0, d = 0;
+                        for (int k = 0; k < np; ++k)
+                            for (int j = 0; j < np; ++j)
+                                for (int i = 0; i < np; ++i) {
+                                    const double lw = L[qi * np + i] * L[qj * np + j] * L[qk * np + k];
+                                    const size_t gi = gid(c, tl(i, j, k));
+                                    const double E = Q[4 * ndof + gi];
+                                    e += lw * E;
+                                    d += lw * (E - rhoE_old[gi]) / loc_dt[c];
+                                }
+                        den += e * e * jxw;
+                        num += d * d * jxw;
+                    }
+        }
+        return std::sqrt(num) / std::sqrt(den);
     }
 
     // ---- PLENS::update + update_rk4, dgsem/plens.cc:4334-4403, 4484-4518 --------------------------
@@ -1290,7 +1358,9 @@ struct oracle {
         int e = calc_rhs();
         if (e) return e;
         auto t0 = std::chrono::steady_clock::now();
-        const double dt = fixed_dt > 0 ? fixed_dt : calc_time_step(Co);
+        const double dt = fixed_dt > 0 ? fixed_dt : calc_time_step(Co, *t);
+        if (fixed_dt > 0) loc_dt.assign(nc, fixed_dt);
+        std::vector<double> E0(Q.begin() + 4 * ndof, Q.begin() + 5 * ndof);  // g_cvars[4] at the start
         auto t1 = std::chrono::steady_clock::now();
         timers[6] += std::chrono::duration<double>(t1 - t0).count();
         for (int stage = 1; stage <= 5; ++stage) {
@@ -1299,7 +1369,8 @@ struct oracle {
                 if (e) return e;
             }
             auto u0 = std::chrono::steady_clock::now();
-            for (size_t i = 0; i < 5 * ndof; ++i) rhs[i] = rhs[i] * dt;  // multiply_time_step_to_rhs
+            for (int v = 0; v < 5; ++v)  // multiply_time_step_to_rhs (per-cell steps)
+                for (size_t i = 0; i < ndof; ++i) rhs[v * ndof + i] *= loc_dt[i / n];
             if (stage == 1) {
                 for (size_t i = 0; i < 5 * ndof; ++i) {
                     Q[i] = Q[i] + rk.ao[0] * rhs[i];
@@ -1323,6 +1394,7 @@ struct oracle {
             auto u1 = std::chrono::steady_clock::now();
             timers[7] += std::chrono::duration<double>(u1 - u0).count();
         }
+        rhoE_old = E0;  // update(), dgsem/plens.cc:4507-4508
         *t += dt;
         return 0;
     }
@@ -1358,6 +1430,7 @@ int oracle_create(int32_t n_cells, int32_t degree, const double* vertices, const
         }
     cob_matrix(degree, o->C);
     o->metrics(vertices, transpose_y);
+    o->verts.assign(vertices, vertices + (size_t)n_cells * 24);
     const size_t nd = o->ndof;
     o->Q.assign(5 * nd, 0); o->avars.assign(9 * nd, 0); o->mu.assign(nd, 0); o->kap.assign(nd, 0);
     o->alpha.assign(n_cells, 0); o->alpha_d.assign(n_cells, 0);
@@ -1424,6 +1497,28 @@ int oracle_run(oracle* o, int32_t n_steps, double courant, double* t) {
 int oracle_last_failure(const oracle* o, int64_t* dof, int32_t* kind) {
     if (dof) *dof = o->fail_dof;
     if (kind) *kind = o->fail_kind;
+    return 0;
+}
+
+int oracle_set_local_stepping(oracle* o, int32_t request, double start_time, int32_t limit,
+                               double neighbor_factor, double global_factor) {
+    o->lts_request = request;
+    o->lts_start = start_time;
+    o->lts_limit = limit;
+    o->lts_nf = neighbor_factor;
+    o->lts_gf = global_factor;
+    return 0;
+}
+
+int oracle_loc_dt(const oracle* o, double* dt) {
+    if ((int)o->loc_dt.size() != o->nc) return -1;
+    for (int c = 0; c < o->nc; ++c) dt[c] = o->loc_dt[c];
+    return 0;
+}
+
+int oracle_ss_error(const oracle* o, double* err) {
+    if (o->rhoE_old.size() != o->ndof || (int)o->loc_dt.size() != o->nc) return -1;  // no step taken yet
+    *err = o->calc_ss_error();
     return 0;
 }
 

@@ -54,6 +54,14 @@ int oracle_rk4_step(oracle* o, double courant, double* t);
 int oracle_rk4_step_fixed_dt(oracle* o, double dt, double* t);
 int oracle_run(oracle* o, int32_t n_steps, double courant, double* t);
 int oracle_last_failure(const oracle* o, int64_t* dof, int32_t* kind);
+/* local time stepping parameters ("time integration": request local stepping, local stepping start
+ * time, limit local time step, local step neighbor/global factor) */
+int oracle_set_local_stepping(oracle* o, int32_t request, double start_time, int32_t limit,
+                              double neighbor_factor, double global_factor);
+/* per-cell time steps of the last step (loc_time_steps) */
+int oracle_loc_dt(const oracle* o, double* dt);
+/* steady-state error after at least one step (calc_ss_error) */
+int oracle_ss_error(const oracle* o, double* err);
 /* section timers (seconds, accumulated) named as plens_test::time(): [0] assert positivity,
  * [1] aux vars, [2] blender, [3] surface fluxes 2&3, [4] ho residual, [5] lo residual,
  * [6] time step, [7] updating cvars */

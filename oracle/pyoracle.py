@@ -63,6 +63,9 @@ def load():
         "oracle_dif_flux": (None, [_dp, _dp, _dp, _dp]),
         "oracle_blender_value": (C.c_double, [C.POINTER(OPhys), C.c_int32, _dp]),
         "oracle_rk4_coeffs": (None, [_dp, _dp, _dp]),
+        "oracle_set_local_stepping": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_double]),
+        "oracle_loc_dt": (C.c_int, [C.c_void_p, _dp]),
+        "oracle_ss_error": (C.c_int, [C.c_void_p, _dp]),
     }
     for k, (r, a) in sig.items():
         f = getattr(lib, k)
@@ -143,12 +146,13 @@ class OraclePlens:
         self.lib.oracle_calc_dt(self._h, courant, C.byref(dt))
         return dt.value
 
-    def update(self, courant):
-        t = C.c_double(0.0)
+    def update(self, courant, t0=0.0):
+        """one RK4 step from solution time t0; returns the step"""
+        t = C.c_double(t0)
         rc = self.lib.oracle_rk4_step(self._h, courant, C.byref(t))
         if rc:
             self._fail(rc)
-        return t.value
+        return t.value - t0
 
     def update_fixed_dt(self, dt):
         t = C.c_double(0.0)
@@ -157,12 +161,30 @@ class OraclePlens:
             self._fail(rc)
         return t.value
 
-    def run(self, n_steps, courant):
-        t = C.c_double(0.0)
+    def run(self, n_steps, courant, t0=0.0):
+        """n_steps RK4 steps from solution time t0; returns the elapsed time"""
+        t = C.c_double(t0)
         rc = self.lib.oracle_run(self._h, n_steps, courant, C.byref(t))
         if rc:
             self._fail(rc)
-        return t.value
+        return t.value - t0
+
+    def set_local_stepping(self, request=True, start_time=0.0, limit=False, neighbor_factor=1.05,
+                           global_factor=1.0):
+        self.lib.oracle_set_local_stepping(self._h, int(request), start_time, int(limit), neighbor_factor,
+                                           global_factor)
+
+    def loc_dt(self):
+        out = np.empty(self.n_cells)
+        if self.lib.oracle_loc_dt(self._h, out.ctypes.data_as(_dp)):
+            raise RuntimeError("no step taken yet")
+        return out
+
+    def ss_error(self):
+        e = C.c_double()
+        if self.lib.oracle_ss_error(self._h, C.byref(e)):
+            raise RuntimeError("no step taken yet")
+        return e.value
 
     def section_times(self):
         t = np.zeros(8)

@@ -60,6 +60,7 @@ GPU_SYMBOLS = [
     "plens_gpu_calc_rhs", "plens_gpu_calc_dt", "plens_gpu_rk4_step", "plens_gpu_run",
     "plens_gpu_last_failure", "plens_gpu_set_tuning", "plens_gpu_kernel_times", "plens_gpu_strerror",
     "plens_gpu_last_run_ms", "plens_gpu_run_profiled", "plens_gpu_stream_copy", "plens_gpu_time_kernel",
+    "plens_gpu_set_local_stepping", "plens_gpu_ss_error",
     "plens_gpu_batch_create", "plens_gpu_batch_destroy", "plens_gpu_batch_run", "plens_gpu_batch_get_state",
     "plens_gpu_batch_last_run_ms", "plens_gpu_batch_last_failure", "plens_gpu_batch_n_groups",
     "plens_gpu_batch_group_info",
@@ -108,6 +109,8 @@ def load():
         "plens_gpu_last_run_ms": (C.c_int, [vp, _dp]),
         "plens_gpu_stream_copy": (C.c_int, [C.c_int64, C.c_int32, _dp]),
         "plens_gpu_time_kernel": (C.c_int, [vp, C.c_int32, C.c_int32, _dp]),
+        "plens_gpu_set_local_stepping": (C.c_int, [vp, C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_double]),
+        "plens_gpu_ss_error": (C.c_int, [vp, _dp]),
         "plens_gpu_batch_create": (C.c_int, [C.POINTER(vp), C.c_int32, C.c_char_p, C.POINTER(vp)]),
         "plens_gpu_batch_destroy": (None, [vp]),
         "plens_gpu_batch_run": (C.c_int, [vp, C.c_int32, _dp, _dp]),

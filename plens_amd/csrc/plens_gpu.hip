@@ -57,6 +57,9 @@ struct Basis {
     double D[kMaxNP * kMaxNP];
     double Q[kMaxNP * kMaxNP];
     double C1[kMaxNP * kMaxNP];
+    double gw[kMaxNP];             // QGauss(N+1) weights on [0,1]
+    double GL[kMaxNP * kMaxNP];    // GL[q][i] = l_i(xg_q): GLL Lagrange basis at the Gauss points
+    double gx[kMaxNP];             // QGauss(N+1) points on [0,1]
 };
 
 struct KP {
@@ -106,6 +109,13 @@ struct KP {
     double* t_acc;           // accumulated time of every instance
     int face_threads;        // workgroup size of the face kernel
     int evar;                // BR1 lifting in entropy variables (default build) instead of conservative
+    double* tabs;            // [2 ninst]: solution time of every instance, and its value at the start
+                             // of the current step
+    double* dt_lts;          // per-cell step of local time stepping (nullptr: the instance's step)
+    int lts_limit;
+    double lts_start, lts_nf, lts_gf;
+    double* rhoE_old;        // rho E at the start of the last step (steady-state error)
+    int save_E;              // this launch records rhoE_old
     unsigned long long* fail;  // [0] lowest failing code of the run, [1] [0] latched after the first failing stage
     Gas gas;
     int scheme;
@@ -453,6 +463,7 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
         }
         __syncthreads();
         if (act && node == 0) p.dt_cell[c] = sE[li];
+        if (act && p.save_E) p.rhoE_old[g] = u[4];  // update(), dgsem/plens.cc:4507-4508
     }
     if (gs.inviscid) return;
 
@@ -683,9 +694,144 @@ __global__ void __launch_bounds__(kThreadsAlpha) k_alpha(KP p) {
         }
         if (threadIdx.x == 0) {
             p.dt[i] = red[0];
-            if (p.t_acc) p.t_acc[i] += red[0];
+            if (p.t_acc) {
+                p.t_acc[i] += red[0];
+                p.tabs[p.ninst + i] = p.tabs[i];  // solution time at the start of this step
+                p.tabs[i] += red[0];
+            }
         }
         __syncthreads();
+    }
+}
+
+// =============================================================================================
+// k_lts: per-cell steps of local time stepping (calc_time_step, dgsem/plens.cc:3880-3911), after the
+// instance's global step is known: past the start time the cell's own stable step, limited to
+// neighbour_factor x every interior neighbour's step and global_factor x the global step when the
+// limiter is on; before it the global step
+// =============================================================================================
+__global__ void __launch_bounds__(kThreadsAlpha) k_lts(KP p) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= p.nc) return;
+    const int inst = p.cinst[c];
+    const double glob = p.dt[inst];
+    double m = glob;
+    if (p.tabs[p.ninst + inst] > p.lts_start) {
+        m = p.dt_cell[c];
+        if (p.lts_limit) {
+            for (int f = 0; f < 6; ++f)
+                if (p.bid[c * 6 + f] < 0) m = fmin(m, p.lts_nf * p.dt_cell[p.nbr[c * 6 + f]]);
+            m = fmin(m, p.lts_gf * glob);
+        }
+    }
+    p.dt_lts[c] = m;
+}
+
+// =============================================================================================
+// k_ss: per-cell L2 integrals of the steady-state error (calc_ss_error, dgsem/plens.cc:3985-4035):
+// rho E and (rho E - rhoE_old)/dt_cell interpolated to QGauss(N+1) points (sum-factorised), squared,
+// weighted by |det J| of the mapping at the point and the Gauss weights; part[2c] = numerator,
+// part[2c+1] = denominator. k_ss_reduce sums them in cell order.
+// =============================================================================================
+template <int N>
+__global__ void __launch_bounds__(256) k_ss(KP p, double* part) {
+    constexpr int NP = N + 1, NN = NP * NP * NP;
+    __shared__ double sA[2][256], sB[2][256], sG[NP * NP];
+    const int tid = threadIdx.x, lc = tid / NN, node = tid - lc * NN;
+    const int c = blockIdx.x * p.cpb + lc;
+    const bool act = lc < p.cpb && c < p.nc;
+    const Basis* bs = p.basis;
+    for (int t = tid; t < NP * NP; t += blockDim.x) sG[t] = bs->GL[(t / NP) * kMaxNP + t % NP];
+    const int ti = node % NP, tj = (node / NP) % NP, tk = node / (NP * NP);
+    const long long g = (long long)c * NN + node;
+    if (act) {
+        const double E = p.Q[4 * p.ndof + g];
+        const double dtc = p.dt_lts ? p.dt_lts[c] : p.dt[p.cinst[c]];
+        sA[0][tid] = E;
+        sA[1][tid] = (E - p.rhoE_old[g]) / dtc;
+    }
+    __syncthreads();
+    // three interpolation passes: node index (i,j,k) -> Gauss index (a,b,c), one direction each
+    for (int pass = 0; pass < 3; ++pass) {
+        double (*src)[256] = (pass & 1) ? sB : sA;
+        double (*dst)[256] = (pass & 1) ? sA : sB;
+        if (act) {
+            const int tq[3] = {ti, tj, tk};
+            for (int v = 0; v < 2; ++v) {
+                double acc = 0;
+                for (int m = 0; m < NP; ++m) {
+                    int to[3] = {ti, tj, tk};
+                    to[pass] = m;
+                    acc += sG[tq[pass] * NP + m] * src[v][lc * NN + to[0] + NP * (to[1] + NP * to[2])];
+                }
+                dst[v][tid] = acc;
+            }
+        }
+        __syncthreads();
+    }
+    // after three passes the values sit in sB at Gauss point (ti, tj, tk)
+    double num = 0, den = 0;
+    if (act) {
+        double J;
+        if (p.cmap) {
+            const double* k = p.cmap + (long long)c * 24;
+            const double x0 = bs->gx[ti], x1 = bs->gx[tj], x2 = bs->gx[tk];
+            double a[3][3];
+            for (int i = 0; i < 3; ++i) {
+                const double* m = k + 8 * i;
+                a[0][i] = m[1] + m[4] * x1 + m[6] * x2 + m[7] * (x1 * x2);
+                a[1][i] = m[2] + m[4] * x0 + m[5] * x2 + m[7] * (x0 * x2);
+                a[2][i] = m[3] + m[5] * x1 + m[6] * x0 + m[7] * (x0 * x1);
+            }
+            J = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+        } else {  // stored metric terms: detJ interpolated (exact for trilinear cells when N >= 2)
+            J = 0;
+            for (int kk = 0; kk < NP; ++kk)
+                for (int jj = 0; jj < NP; ++jj)
+                    for (int ii = 0; ii < NP; ++ii)
+                        J += sG[ti * NP + ii] * sG[tj * NP + jj] * sG[tk * NP + kk] *
+                             p.detJ[(long long)c * NN + ii + NP * (jj + NP * kk)];
+        }
+        const double jxw = fabs(J) * bs->gw[ti] * bs->gw[tj] * bs->gw[tk];
+        den = sB[0][tid] * sB[0][tid] * jxw;
+        num = sB[1][tid] * sB[1][tid] * jxw;
+    }
+    __syncthreads();
+    sA[0][tid] = num;
+    sA[1][tid] = den;
+    __syncthreads();
+    if (act && node == 0) {
+        double sn = 0, sd = 0;
+        for (int q = 0; q < NN; ++q) {
+            sn += sA[0][lc * NN + q];
+            sd += sA[1][lc * NN + q];
+        }
+        part[2 * (long long)c] = sn;
+        part[2 * (long long)c + 1] = sd;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_ss_reduce(const double* part, int nc, double* out) {
+    __shared__ double rn[256], rd[256];
+    double sn = 0, sd = 0;
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+        sn += part[2 * (long long)c];
+        sd += part[2 * (long long)c + 1];
+    }
+    rn[threadIdx.x] = sn;
+    rd[threadIdx.x] = sd;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            rn[threadIdx.x] += rn[threadIdx.x + s];
+            rd[threadIdx.x] += rd[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = rn[0];
+        out[1] = rd[0];
     }
 }
 
@@ -1130,7 +1276,7 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
         for (int v = 0; v < 5; ++v) p.rhs_out[v * p.ndof + g] = rhs[v];
         return;
     }
-    const double dt = p.dt[p.cinst[c]];
+    const double dt = p.dt_lts ? p.dt_lts[c] : p.dt[p.cinst[c]];
     for (int v = 0; v < 5; ++v) {
         const long long k = v * p.ndof + g;
         const double R = rhs[v] * dt;
@@ -1170,6 +1316,11 @@ struct plens_gpu {
     double *Ja = nullptr, *detJ = nullptr, *nrm = nullptr, *hmin = nullptr;
     double* cmap = nullptr;  // trilinear maps of the cells; the stored metric arrays are not kept then
     double* fmet = nullptr;  // face-node metric terms of the face tasks
+    double* tabs = nullptr;      // solution time per instance, and at the start of the current step
+    double* rhoE_old = nullptr;  // rho E at the start of the last step
+    double* dt_lts = nullptr;    // local time stepping: per-cell steps (allocated on request)
+    double* ss_part = nullptr;   // steady-state error: per-cell integrals, then the two sums
+    long long steps_taken = 0;
     bool use_cmap = false;
     int *nbr = nullptr, *bid = nullptr, *wall = nullptr, *tasks = nullptr;
     BCDev* bcs = nullptr;
@@ -1290,6 +1441,11 @@ int launch_one_t(plens_gpu* h, const KP& kp, int kernel) {
         case 1:
             hipLaunchKernelGGL(k_alpha, dim3((kp.nc + kThreadsAlpha - 1) / kThreadsAlpha), dim3(kThreadsAlpha), 0,
                                h->stream, kp);
+            if (kp.dt_lts && kp.want_dt && kp.t_acc) {
+                HIPCHK(hipGetLastError());
+                hipLaunchKernelGGL(k_lts, dim3((kp.nc + kThreadsAlpha - 1) / kThreadsAlpha), dim3(kThreadsAlpha), 0,
+                                   h->stream, kp);
+            }
             break;
         case 2: {
             const long long fthreads = (long long)kp.ntasks * NF;
@@ -1361,6 +1517,7 @@ KP stage_params(plens_gpu* h, int s, bool from_b, double courant) {
     kp.R2 = h->R[r2[s]];
     kp.stage = s;
     kp.want_dt = (s == 1);
+    kp.save_E = (s == 1);
     kp.courant = courant;
     if (s == 1) {
         kp.c1 = 0; kp.c2 = 0; kp.c3 = kRkAo[0];
@@ -1403,6 +1560,35 @@ int reset_failure(plens_gpu* h) {
     static const unsigned long long init[2] = {~0ull, ~0ull};
     HIPCHK(hipMemcpyAsync(h->fail, init, sizeof(init), hipMemcpyHostToDevice, h->stream));
     return PLENS_OK;
+}
+
+// n-point Gauss-Legendre rule on [0,1] (Newton iteration on P_n from the Chebyshev-like guesses)
+void gauss_legendre01(int n, double* x, double* w) {
+    for (int j = 0; j < n; ++j) {
+        long double t = -std::cos(3.14159265358979323846L * (j + 0.75L) / (n + 0.5L)), P = 0, dP = 0;
+        for (int it = 0; it < 100; ++it) {
+            long double p0 = 1, p1 = t;
+            for (int k = 2; k <= n; ++k) {
+                const long double p2 = ((2 * k - 1) * t * p1 - (k - 1) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            P = (n == 0) ? 1 : p1;
+            dP = n * (t * P - p0) / (t * t - 1);
+            const long double dt = P / dP;
+            t -= dt;
+            if (std::fabs((double)dt) < 1e-19) break;
+        }
+        long double p0 = 1, p1 = t;
+        for (int k = 2; k <= n; ++k) {
+            const long double p2 = ((2 * k - 1) * t * p1 - (k - 1) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        dP = n * (t * p1 - p0) / (t * t - 1);
+        x[j] = (double)((t + 1) / 2);
+        w[j] = (double)(1.0L / ((1 - t * t) * dP * dP));
+    }
 }
 
 // Monomial coefficients k[i][0..7] of the trilinear map x_i(xi) of a cell from its 8 vertices in
@@ -1457,8 +1643,11 @@ int alloc_buffers(plens_gpu* h) {
         (rc = dmalloc(&h->dt_cell, nc)) || (rc = dmalloc(&h->cinst, nc)) ||
         (rc = dmalloc(&h->ioff, (size_t)h->ninst + 1)) || (rc = dmalloc(&h->visc, 3 * (size_t)h->ninst)) ||
         (rc = dmalloc(&h->scal, 2 * (size_t)h->ninst)) || (rc = dmalloc(&h->fail, 2)) ||
-        (rc = dmalloc(&h->blist, nc)) || (rc = dmalloc(&h->bcount, 1)) || (rc = dmalloc(&h->mlp_stats, 2)))
+        (rc = dmalloc(&h->blist, nc)) || (rc = dmalloc(&h->bcount, 1)) || (rc = dmalloc(&h->mlp_stats, 2)) ||
+        (rc = dmalloc(&h->tabs, 2 * (size_t)h->ninst)) || (rc = dmalloc(&h->rhoE_old, nd)) ||
+        (rc = dmalloc(&h->ss_part, 2 * nc + 2)))
         return rc;
+    if (hipMemset(h->tabs, 0, sizeof(double) * 2 * h->ninst) != hipSuccess) return PLENS_E_HIP;
     if (h->use_cmap) {
         if ((rc = dmalloc(&h->cmap, nc * 24))) return rc;
     } else if ((rc = dmalloc(&h->Ja, 9 * nd)) || (rc = dmalloc(&h->detJ, nd))) {
@@ -1502,6 +1691,10 @@ void bind_buffers(plens_gpu* h) {
     k.dt_cell = h->dt_cell; k.cinst = h->cinst; k.ioff = h->ioff; k.visc = h->visc;
     k.dt = h->scal; k.t_acc = h->scal + h->ninst; k.fail = h->fail;
     k.face_threads = h->face_threads;
+    k.tabs = h->tabs;
+    k.rhoE_old = h->rhoE_old;
+    k.dt_lts = h->dt_lts;
+    k.save_E = 0;
 }
 
 void destroy_graphs(plens_gpu* h) {
@@ -1596,11 +1789,21 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
     }
     Basis hbas{};
     {
-        double x[kMaxNP], w[kMaxNP], D[kMaxNP * kMaxNP], C1[kMaxNP * kMaxNP];
+        double x[kMaxNP], w[kMaxNP], D[kMaxNP * kMaxNP], C1[kMaxNP * kMaxNP], gx[kMaxNP], gw[kMaxNP];
         if (plens_basis_1d(N, x, w, D, C1)) return fail(PLENS_E_INVALID);
+        gauss_legendre01(NP, gx, gw);
+        for (int q = 0; q < NP; ++q)  // GLL Lagrange basis at the Gauss points
+            for (int i = 0; i < NP; ++i) {
+                double l = 1.0;
+                for (int m2 = 0; m2 < NP; ++m2)
+                    if (m2 != i) l *= (gx[q] - x[m2]) / (x[i] - x[m2]);
+                hbas.GL[q * kMaxNP + i] = l;
+            }
         for (int i = 0; i < NP; ++i) {
             hbas.x[i] = x[i];
             hbas.w[i] = w[i];
+            hbas.gx[i] = gx[i];
+            hbas.gw[i] = gw[i];
             for (int j = 0; j < NP; ++j) {
                 hbas.D[i * kMaxNP + j] = D[i * NP + j];
                 hbas.Q[i * kMaxNP + j] = w[i] * D[i * NP + j];  // ref_Q_1d = w_i D_ij
@@ -1711,7 +1914,8 @@ void plens_gpu_destroy(plens_gpu* h) {
     void* ptrs[] = {h->Qa, h->Qb, h->R[0], h->R[1], h->Ja, h->detJ, h->nrm, h->hmin, h->nbr,
                     h->bid, h->wall, h->tasks, h->bcs, h->basis, h->avars, h->araw, h->alpha, h->alpha_d,
                     h->sinv, h->sface, h->rhs, h->rlo, h->dt_cell, h->scal, h->fail, h->blist, h->bcount, h->mlp_stats, h->feat,
-                    h->avars_f, h->sinv_f, h->sface_f, h->cinst, h->ioff, h->visc, h->cfl, h->cmap, h->fmet};
+                    h->avars_f, h->sinv_f, h->sface_f, h->cinst, h->ioff, h->visc, h->cfl, h->cmap, h->fmet,
+                    h->tabs, h->rhoE_old, h->dt_lts, h->ss_part};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
@@ -1792,8 +1996,11 @@ int plens_gpu_rk4_step(plens_gpu* h, double courant, double* t_inout) {
     if (e) return e;
     const double zero = 0.0;
     HIPCHK(hipMemcpyAsync(h->scal + h->ninst, &zero, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if (t_inout)  // solution time at the start of the call (local time stepping starts by it)
+        HIPCHK(hipMemcpyAsync(h->tabs, t_inout, sizeof(double), hipMemcpyHostToDevice, h->stream));
     if ((e = launch_step(h, h->state_in_b, courant))) return e;
     h->state_in_b = !h->state_in_b;
+    h->steps_taken += 1;
     if ((e = read_failure(h))) return e;
     double dt = 0;
     HIPCHK(hipMemcpyAsync(&dt, h->scal + h->ninst, sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -1808,6 +2015,8 @@ int plens_gpu_run(plens_gpu* h, int32_t n_steps, double courant, double* t_inout
     if (e) return e;
     const double zero = 0.0;
     HIPCHK(hipMemcpyAsync(h->scal + h->ninst, &zero, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if (t_inout)  // solution time at the start of the call (local time stepping starts by it)
+        HIPCHK(hipMemcpyAsync(h->tabs, t_inout, sizeof(double), hipMemcpyHostToDevice, h->stream));
     if (h->graph_courant != courant || h->graph_cpb != h->cpb) {
         destroy_graphs(h);
         for (int par = 0; par < 2; ++par) {
@@ -1833,6 +2042,7 @@ int plens_gpu_run(plens_gpu* h, int32_t n_steps, double courant, double* t_inout
         HIPCHK(hipGraphLaunch(h->graph[h->state_in_b ? 1 : 0], h->stream));
         h->state_in_b = !h->state_in_b;
     }
+    h->steps_taken += n_steps;
     HIPCHK(hipEventRecord(h->ev_run[1], h->stream));
     HIPCHK(hipEventSynchronize(h->ev_run[1]));
     {
@@ -1856,12 +2066,15 @@ int plens_gpu_run_profiled(plens_gpu* h, int32_t n_steps, double courant, double
     if (e) return e;
     const double zero = 0.0;
     HIPCHK(hipMemcpyAsync(h->scal + h->ninst, &zero, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if (t_inout)  // solution time at the start of the call (local time stepping starts by it)
+        HIPCHK(hipMemcpyAsync(h->tabs, t_inout, sizeof(double), hipMemcpyHostToDevice, h->stream));
     std::vector<hipEvent_t> ev(30);
     for (auto& x : ev) HIPCHK(hipEventCreate(&x));
     double acc[5] = {0, 0, 0, 0, 0};
     for (int s = 0; s < n_steps; ++s) {
         if ((e = launch_step(h, h->state_in_b, courant, ev.data()))) return e;
         h->state_in_b = !h->state_in_b;
+        h->steps_taken += 1;
         HIPCHK(hipStreamSynchronize(h->stream));
         for (int st = 0; st < 5; ++st)
             for (int k = 0; k < 5; ++k) {
@@ -1886,6 +2099,45 @@ int plens_gpu_kernel_times(const plens_gpu* h, double* t, int32_t n) {
     return PLENS_OK;
 }
 
+int plens_gpu_set_local_stepping(plens_gpu* h, int32_t request, double start_time, int32_t limit,
+                                 double neighbor_factor, double global_factor) {
+    if (!h || !(neighbor_factor > 0) || !(global_factor > 0)) return PLENS_E_INVALID;
+    if (request && !h->dt_lts && hipMalloc((void**)&h->dt_lts, sizeof(double) * h->nc) != hipSuccess)
+        return PLENS_E_NOMEM;
+    h->base.dt_lts = request ? h->dt_lts : nullptr;
+    h->base.lts_start = start_time;
+    h->base.lts_limit = limit;
+    h->base.lts_nf = neighbor_factor;
+    h->base.lts_gf = global_factor;
+    destroy_graphs(h);
+    return PLENS_OK;
+}
+
+int plens_gpu_ss_error(plens_gpu* h, double* err) {
+    if (!h || !err) return PLENS_E_INVALID;
+    if (h->steps_taken < 1) return PLENS_E_INVALID;
+    KP kp = h->base;
+    kp.Q = h->state_in_b ? h->Qb : h->Qa;
+    const int thr = threads_for(kp.cpb, h->NN);
+    const int blocks = (kp.nc + kp.cpb - 1) / kp.cpb;
+    switch (h->N) {
+        case 1: hipLaunchKernelGGL(k_ss<1>, dim3(blocks), dim3(thr), 0, h->stream, kp, h->ss_part); break;
+        case 2: hipLaunchKernelGGL(k_ss<2>, dim3(blocks), dim3(thr), 0, h->stream, kp, h->ss_part); break;
+        case 3: hipLaunchKernelGGL(k_ss<3>, dim3(blocks), dim3(thr), 0, h->stream, kp, h->ss_part); break;
+        case 4: hipLaunchKernelGGL(k_ss<4>, dim3(blocks), dim3(thr), 0, h->stream, kp, h->ss_part); break;
+        default: hipLaunchKernelGGL(k_ss<5>, dim3(blocks), dim3(thr), 0, h->stream, kp, h->ss_part); break;
+    }
+    HIPCHK(hipGetLastError());
+    double* out = h->ss_part + 2 * (size_t)h->nc;
+    hipLaunchKernelGGL(k_ss_reduce, dim3(1), dim3(256), 0, h->stream, h->ss_part, h->nc, out);
+    HIPCHK(hipGetLastError());
+    double nd[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(nd, out, sizeof(nd), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *err = std::sqrt(nd[0]) / std::sqrt(nd[1]);
+    return PLENS_OK;
+}
+
 int plens_gpu_time_kernel(plens_gpu* h, int32_t kernel, int32_t reps, double* avg_ms) {
     if (!h || !avg_ms || reps < 1 || (kernel != 0 && kernel != 2 && kernel != 4)) return PLENS_E_INVALID;
     int e = plens_gpu_calc_rhs(h, nullptr, nullptr, nullptr);  // every input of the kernel current
@@ -1905,6 +2157,7 @@ int plens_gpu_time_kernel(plens_gpu* h, int32_t kernel, int32_t reps, double* av
             kp.Q = cur;
             kp.Qout = tmp;
             kp.t_acc = nullptr;
+            kp.save_E = 0;  // the energy of the last step stays for the steady-state error
             rc = launch_one(h, kp, kernel);
         }
     }
@@ -2047,7 +2300,7 @@ int merge_group(const std::vector<plens_gpu*>& mem, hipStream_t stream, plens_gp
     const plens_gpu* m0 = mem[0];
     for (const plens_gpu* m : mem)
         if (m->N != m0->N || m->precision != m0->precision || m->ninst != 1 || m->use_cmap != m0->use_cmap ||
-            !same_scheme(m->base, m0->base))
+            m->base.dt_lts || !same_scheme(m->base, m0->base))
             return PLENS_E_INVALID;
     plens_gpu* h = new (std::nothrow) plens_gpu;
     if (!h) return PLENS_E_NOMEM;

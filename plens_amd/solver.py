@@ -77,15 +77,17 @@ class GpuPlens:
         self._check(self.lib.plens_gpu_calc_dt(self._h, courant, C.byref(dt)), "calc_dt")
         return dt.value
 
-    def update(self, courant):
-        t = C.c_double(0.0)
+    def update(self, courant, t0=0.0):
+        """one RK4 step from solution time t0; returns the step"""
+        t = C.c_double(t0)
         self._check(self.lib.plens_gpu_rk4_step(self._h, courant, C.byref(t)), "rk4_step")
-        return t.value
+        return t.value - t0
 
-    def run(self, n_steps, courant):
-        t = C.c_double(0.0)
+    def run(self, n_steps, courant, t0=0.0):
+        """n_steps RK4 steps from solution time t0; returns the elapsed time"""
+        t = C.c_double(t0)
         self._check(self.lib.plens_gpu_run(self._h, n_steps, courant, C.byref(t)), "run")
-        return t.value
+        return t.value - t0
 
     def last_run_ms(self):
         ms = C.c_double()
@@ -98,6 +100,16 @@ class GpuPlens:
         times = np.zeros(5)
         self.lib.plens_gpu_kernel_times(self._h, _dptr(times), 5)
         return t.value, times
+
+    def set_local_stepping(self, request=True, start_time=0.0, limit=False, neighbor_factor=1.05,
+                           global_factor=1.0):
+        self._check(self.lib.plens_gpu_set_local_stepping(self._h, int(request), start_time, int(limit),
+                                                          neighbor_factor, global_factor), "set_local_stepping")
+
+    def ss_error(self):
+        e = C.c_double()
+        self._check(self.lib.plens_gpu_ss_error(self._h, C.byref(e)), "ss_error")
+        return e.value
 
     def time_kernel(self, kernel, reps=20):
         """average duration (ms) of one launch of kernel class `kernel` ("grad", "face" or "vol"),

@@ -227,3 +227,63 @@ def test_rhs_and_steps_remaining_bcs(bc_type, wall_limit):
     err = rel_l2(g.get_state(), o.get_state())
     print(f"bc-{bc_type} 10 steps rel L2 {err:.3e}", flush=True)
     assert err <= RUN_TOL
+
+
+# ---- local time stepping and the steady-state error -------------------------------------------
+@pytest.mark.parametrize("limit,start", [(False, -1.0), (True, -1.0), (False, "mid")])
+def test_local_stepping_parity(limit, start):
+    from oracle.pyoracle import OraclePlens
+    from plens_amd import GpuPlens
+    from tests.test_host_and_assembly import _stretched_tube
+    case = _stretched_tube()
+    if start == "mid":  # switch on after three global steps, inside one graph-replayed run
+        probe = OraclePlens(case)
+        start = probe.run(3, case.courant) * 0.999
+    g, o = GpuPlens(case), OraclePlens(case)
+    kw = dict(start_time=start, limit=limit, neighbor_factor=1.1, global_factor=3.0)
+    g.set_local_stepping(True, **kw)
+    o.set_local_stepping(True, **kw)
+    tg = g.run(6, case.courant)
+    to = o.run(6, case.courant)
+    err = rel_l2(g.get_state(), o.get_state())
+    eg, eo = g.ss_error(), o.ss_error()
+    print(f"LTS limit={limit} start={start}: 6 steps rel L2 {err:.3e}, ss error gpu {eg:.10e} cpu {eo:.10e}",
+          flush=True)
+    assert abs(tg - to) <= 1e-12 * to
+    assert err <= RUN_TOL
+    assert abs(eg - eo) <= 1e-10 * eo
+
+
+def test_ss_error_global_stepping_and_measurement_calls():
+    from oracle.pyoracle import OraclePlens
+    from plens_amd import GpuPlens, shock_tube
+    from plens_amd._lib import PlensError
+    case = shock_tube(nx=16, ny=2, nz=2)
+    g, o = GpuPlens(case), OraclePlens(case)
+    with pytest.raises(PlensError):
+        g.ss_error()  # no step yet
+    g.run(3, case.courant)
+    o.run(3, case.courant)
+    e0 = g.ss_error()
+    g.calc_rhs()
+    g.time_kernel("grad", reps=2)
+    g.time_kernel("vol", reps=2)
+    e1 = g.ss_error()
+    print(f"ss error gpu {e0:.10e} cpu {o.ss_error():.10e}", flush=True)
+    assert e0 == e1  # RHS evaluations and kernel timing leave the last step's record alone
+    assert abs(e0 - o.ss_error()) <= 1e-10 * o.ss_error()
+    # calc_time_step replaces the steps the error divides by, as it replaces loc_time_steps
+    g.calc_time_step(case.courant)
+    o.calc_time_step(case.courant)
+    assert abs(g.ss_error() - o.ss_error()) <= 1e-10 * o.ss_error()
+
+
+def test_batch_rejects_local_stepping():
+    from plens_amd import GpuBatch, GpuPlens, periodic_cube
+    from plens_amd._lib import PLENS_E_INVALID, PlensError
+    case = periodic_cube(n=2, degree=2)
+    a, b = GpuPlens(case), GpuPlens(case)
+    b.set_local_stepping(True)
+    with pytest.raises(PlensError) as ei:
+        GpuBatch([a, b])
+    assert ei.value.code == PLENS_E_INVALID

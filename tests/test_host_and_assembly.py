@@ -443,3 +443,92 @@ def test_symmetry_bc_equals_mirrored_domain_inviscid():
                                  for c in [nx * (j + m * k) + i for i in range(nx)]])
     assert np.abs(rh).max() > 1e-2
     assert np.abs(rh[:, lower_half] - rf[:, upper]).max() < 1e-10 * np.abs(rf).max()
+
+
+# ---- local time stepping and the steady-state error (dgsem/plens.cc:3810-3912, 3985-4035) --------
+def _stretched_tube():
+    """periodic box, y spacing stretched by 1.5 per cell (stable steps differ between cells), a
+    density wave along x and y carried by u = (0.5, 0.3, 0), p = 1, viscous"""
+    m = cases.Mesh(3, 4, 6, 2, (0, 0, 0), (1, 1, 1), (1, 1, 1), [-1] * 6, (0, 0, 0), y_stretch_ratio=1.5)
+    X, Y = (m.node_coords[:, d, :].reshape(-1) for d in (0, 1))
+    rho = 1 + 0.1 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y)
+    Q = np.vstack([rho, 0.5 * rho, 0.3 * rho, 0 * X, 1.0 / 0.4 + 0.5 * rho * 0.34])
+    return cases.Case("stretched", m, cases.make_physics(mu0=1e-3, flux="HLLC"), [cases.make_bc(L.BC_PERIODIC)],
+                      Q, 0.4, 1)
+
+
+def test_local_stepping_limited_by_global_factor_one_is_global_stepping():
+    """with the limiter on and global factor 1 every cell step is min(own, neighbours x 1.05, global)
+    = global: the run equals global stepping"""
+    case = _stretched_tube()
+    a, b = PO.OraclePlens(case), PO.OraclePlens(case)
+    b.set_local_stepping(True, start_time=-1.0, limit=True, neighbor_factor=1.05, global_factor=1.0)
+    ta, tb = a.run(3, case.courant), b.run(3, case.courant)
+    assert ta == tb
+    assert np.array_equal(a.get_state(), b.get_state())
+    assert np.all(b.loc_dt() == b.loc_dt()[0])
+
+
+def test_local_stepping_steps_and_limits():
+    """past the start time each cell advances with its own stable step (not below the global one,
+    equal to it in the limiting cell); the limiter caps a step at neighbour_factor x every interior
+    neighbour's own step and global_factor x the global step; before the start time the steps are
+    global"""
+    case = _stretched_tube()
+    g = PO.OraclePlens(case)
+    dt_glob = g.update(case.courant)
+    assert np.all(g.loc_dt() == dt_glob)
+    o = PO.OraclePlens(case)
+    o.set_local_stepping(True, start_time=0.5, limit=False)
+    o.update(case.courant, t0=0.0)  # before the start time: global
+    assert np.all(o.loc_dt() == o.loc_dt()[0])
+    o2 = PO.OraclePlens(case)
+    o2.set_local_stepping(True, start_time=0.5, limit=False)
+    dt2 = o2.update(case.courant, t0=1.0)  # (1 + dt) - 1: dt to a few ulp of 1
+    own = o2.loc_dt()
+    assert abs(own.min() - dt2) < 1e-11 * dt2 and own.max() > 1.1 * own.min()
+    dt2 = own.min()
+    o3 = PO.OraclePlens(case)
+    o3.set_local_stepping(True, start_time=0.5, limit=True, neighbor_factor=1.1, global_factor=3.0)
+    o3.update(case.courant, t0=1.0)
+    lim = o3.loc_dt()
+    nbr, bid = case.mesh.face_nbr, case.mesh.face_bid
+    for c in range(case.mesh.n_cells):
+        cap = min([own[c], 3.0 * dt2] + [1.1 * own[nbr[c, f]] for f in range(6) if bid[c, f] < 0])
+        assert lim[c] == cap
+
+
+def test_ss_error_against_exact_integrals():
+    """steady-state error after two steps on box cells (constant J, so QGauss(N+1) is exact for the
+    squared degree-N field): ||(rhoE - rhoE_start)/dt_cell|| / ||rhoE|| with the integrals evaluated
+    here by an 8-point Gauss rule on the nodal interpolants; rhoE_start is the state at the start of
+    the last step and dt_cell that step's per-cell step (local stepping, unlimited)"""
+    case = _stretched_tube()
+    o = PO.OraclePlens(case)
+    o.set_local_stepping(True, start_time=-1.0, limit=False)
+    o.update(case.courant)
+    E_start = o.get_state()[4].copy()
+    o.update(case.courant)
+    E = o.get_state()[4]
+    dtc = o.loc_dt()
+    m = case.mesh
+    N, n = m.degree, m.n
+    xg, wg = np.polynomial.legendre.leggauss(8)
+    xg, wg = 0.5 * (xg + 1), 0.5 * wg
+    xn = m.node_coords[0, 0, :N + 1]
+    xn = (xn - xn[0]) / (xn[-1] - xn[0])  # GLL nodes on [0,1]
+    L = np.array([[np.prod([(x - xn[k]) / (xn[j] - xn[k]) for k in range(N + 1) if k != j]) for j in range(N + 1)]
+                  for x in xg])
+    num = den = 0.0
+    for c in range(m.n_cells):
+        v = m.vertices[c]
+        vol = np.prod(v.max(0) - v.min(0))
+        def integ(f):
+            F = f.reshape(N + 1, N + 1, N + 1)  # [k][j][i]
+            Fq = np.einsum("ai,bj,ck,kji->cba", L, L, L, F)
+            return vol * np.einsum("a,b,c,cba->", wg, wg, wg, Fq ** 2)
+        den += integ(E[c * n:(c + 1) * n])
+        num += integ((E[c * n:(c + 1) * n] - E_start[c * n:(c + 1) * n]) / dtc[c])
+    ref = np.sqrt(num) / np.sqrt(den)
+    assert ref > 0
+    assert abs(o.ss_error() - ref) < 1e-10 * ref
