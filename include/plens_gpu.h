@@ -35,12 +35,28 @@ enum {
     PLENS_E_NODEVICE = -5      /* no GPU visible */
 };
 
-/* Inviscid surface flux schemes ("inviscid surface flux scheme", dgsem/plens.cc:237-246). */
+/* Inviscid surface flux schemes ("inviscid surface flux scheme", dgsem/plens.cc:237-246; selection
+ * dgsem/plens.cc:831-845, modelling/navier_stokes.cc:140-197). The prm value "SLAU2" never reaches
+ * slau2_xflux in the reference (plens.cc:845 compares instead of assigning) and has no value here. */
 enum {
-    PLENS_FLUX_HLLC = 0,           /* NavierStokes::hllc_xflux, modelling/navier_stokes.cc:936-1033 */
-    PLENS_FLUX_RUSANOV = 1,        /* rusanov_xflux, modelling/navier_stokes.cc:1044-1059 */
-    PLENS_FLUX_AUSM_PLUS_UP = 2,   /* ausm_plus_up_xflux, modelling/navier_stokes.cc:1070-1121 */
-    PLENS_FLUX_CHANDRASHEKHAR = 3  /* chandrashekhar_xflux, modelling/navier_stokes.cc:1250-1318 */
+    PLENS_FLUX_HLLC = 0,           /* "HLLC": hllc_xflux, modelling/navier_stokes.cc:936-1033 */
+    PLENS_FLUX_RUSANOV = 1,        /* "Rusanov": rusanov_xflux, modelling/navier_stokes.cc:1044-1059 */
+    PLENS_FLUX_AUSM_PLUS_UP = 2,   /* "AUSM+-up": ausm_plus_up_xflux, modelling/navier_stokes.cc:1070-1121 */
+    PLENS_FLUX_CHANDRASHEKHAR = 3, /* "Chandrashekhar": chandrashekhar_xflux, navier_stokes.cc:1250-1318 */
+    PLENS_FLUX_RUSANOV_HLLC = 4,   /* "Rusanov-HLLC": rusanov_hllc_blend_xflux, navier_stokes.cc:1132-1141 */
+    PLENS_FLUX_RUSANOV_AUSM_PLUS_UP = 5, /* "Rusanov-AUSM+-up": navier_stokes.cc:1150-1159 */
+    PLENS_FLUX_MODIFIED_SW = 6,    /* "Modified SW": modified_sw_xflux, navier_stokes.cc:1174-1236 */
+    PLENS_FLUX_KG = 7,             /* "KG": kennedy_gruber_xflux, navier_stokes.cc:1325-1357 */
+    PLENS_FLUX_RUSANOV_KG = 8,     /* "Rusanov-KG": rusanov_kennedy_gruber_blend_xflux, .cc:1364-1374 */
+    PLENS_FLUX_IR = 9              /* "IR": ismail_roe_xflux, navier_stokes.cc:1381-1446 */
+};
+
+/* Inviscid volume (two-point) flux of the split form ("inviscid volume flux scheme",
+ * dgsem/plens.cc:247-252, 847-850; modelling/navier_stokes.cc:203-228). */
+enum {
+    PLENS_VOL_FLUX_CHANDRASHEKHAR = 0, /* chandrashekhar_vol_flux, navier_stokes.cc:1510-1542 */
+    PLENS_VOL_FLUX_KG = 1,             /* kennedy_gruber_vol_flux, navier_stokes.cc:1549-1575 */
+    PLENS_VOL_FLUX_IR = 2              /* ismail_roe_vol_flux, navier_stokes.cc:1590-1622 */
 };
 
 /* Boundary condition types (IBCs/). */
@@ -105,6 +121,7 @@ typedef struct {
     double alpha_max;              /* default 0.5 */
     double wall_blender_limit;     /* default 0.5 */
     int32_t gradient_variables;    /* PLENS_GRAD_*: variables of the BR1 lifting */
+    int32_t inv_vol_flux;          /* PLENS_VOL_FLUX_* */
 } plens_physics;
 
 typedef struct {

@@ -43,6 +43,7 @@ struct NS {
     double gma, M, Pr, mu0, T0, S;
     int scheme;
     double blender = 1;  // flux_blender_value (ctor default 1, navier_stokes.cc:33)
+    int vscheme = 0;     // inv_vol_flux_scheme: 0 Chandrashekhar, 1 Kennedy-Gruber, 2 Ismail-Roe
 
     double R() const { return R0 / M; }                       // navier_stokes.h:337
     double cv() const { return R() / (gma - 1); }             // navier_stokes.h:342
@@ -284,12 +285,103 @@ struct NS {
         for (int i = 0; i < 5; ++i)
             for (int j = 0; j < 5; ++j) f[i] -= 0.5 * A[i][j] * (r[j] - l[j]);
     }
-    void xflux(const State& l, const State& r, State& f) const {
+    // Rusanov blends: Rusanov weighted by flux_blender_value, navier_stokes.cc:1132-1160, 1364-1374
+    void rusanov_blend_xflux(const State& l, const State& r, int other, State& f) const {
+        State fr, fo;
+        rusanov_xflux(l, r, fr);
+        if (other == 0) hllc_xflux(l, r, fo);
+        else if (other == 2) ausm_xflux(l, r, fo);
+        else kennedy_gruber_xflux(l, r, fo);
+        for (int v = 0; v < 5; ++v) f[v] = blender * fr[v] + (1 - blender) * fo[v];
+    }
+    // modified Steger-Warming, navier_stokes.cc:1174-1236 (pos_smooth/neg_smooth, .h:464-476)
+    void modified_sw_xflux(const State& l, const State& r, State& f) const {
+        const double pl = get_p(l), pr = get_p(r);
+        const double q = 0.5 * (pr - pl) / std::min(pl, pr);
+        const double om = 0.5 / (1 + q * q);
+        T1 vp, vn;
+        for (int d = 0; d < 3; ++d) {
+            vp[d] = (1 - om) * l[1 + d] / l[0] + om * r[1 + d] / r[0];
+            vn[d] = (1 - om) * r[1 + d] / r[0] + om * l[1 + d] / l[0];
+        }
+        const double rp = (1 - om) * l[0] + om * r[0], rn = (1 - om) * r[0] + om * l[0];
+        const double pp = (1 - om) * pl + om * pr, pn = (1 - om) * pr + om * pl;
+        const double ap = std::sqrt(gma * pp / rp), an = std::sqrt(gma * pn / rn);
+        const double Hp = gma * pp / ((gma - 1) * rp) + 0.5 * dot(vp, vp),
+                     Hn = gma * pn / ((gma - 1) * rn) + 0.5 * dot(vn, vn);
+        double Kp[5][5], Kip[5][5], Kn[5][5], Kin[5][5];
+        get_xK(vp, ap, Hp, Kp);
+        get_xK(vn, an, Hn, Kn);
+        get_xKinv(vp, ap, Hp, Kip);
+        get_xKinv(vn, an, Hn, Kin);
+        auto ps = [](double x, double e) { return 0.5 * (x + std::sqrt(x * x + e * e)); };
+        auto ng = [](double x, double e) { return 0.5 * (x - std::sqrt(x * x + e * e)); };
+        const double ep = 0.3 * ap, en = 0.3 * an;
+        const double lp[5] = {ps(vp[0] - ap, ep), ps(vp[0], ep), ps(vp[0], ep), ps(vp[0], ep), ps(vp[0] + ap, ep)};
+        const double ln[5] = {ng(vn[0] - an, en), ng(vn[0], en), ng(vn[0], en), ng(vn[0], en), ng(vn[0] + an, en)};
+        for (int i = 0; i < 5; ++i) {
+            f[i] = 0;
+            for (int j = 0; j < 5; ++j) {
+                double Ap = 0, An = 0;
+                for (int k = 0; k < 5; ++k) {
+                    Ap += Kp[i][k] * lp[k] * Kip[k][j];
+                    An += Kn[i][k] * ln[k] * Kin[k][j];
+                }
+                f[i] += Ap * l[j] + An * r[j];
+            }
+        }
+    }
+    // Kennedy-Gruber with a local Lax-Friedrichs term, navier_stokes.cc:1325-1357
+    void kennedy_gruber_xflux(const State& l, const State& r, State& f) const {
+        T1 x; x[0] = 1;
+        kennedy_gruber_vol_flux(l, r, x, f);
+        const double el = get_e(l), er = get_e(r);
+        const double lm = std::max(std::fabs(l[1] / l[0]) + std::sqrt(gma * (gma - 1) * el),
+                                   std::fabs(r[1] / r[0]) + std::sqrt(gma * (gma - 1) * er));
+        for (int v = 0; v < 5; ++v) f[v] -= 0.5 * lm * (r[v] - l[v]);
+    }
+    // Ismail-Roe with entropy-variable matrix dissipation, navier_stokes.cc:1381-1446
+    void ismail_roe_xflux(const State& l, const State& r, State& f) const {
+        T1 x; x[0] = 1;
+        double rh, vh[3], p2, hh;
+        ismail_roe_means(l, r, rh, vh, p2, hh);
+        ismail_roe_vol_flux(l, r, x, f);
+        State ch;
+        ch[0] = rh;
+        for (int d = 0; d < 3; ++d) ch[1 + d] = rh * vh[d];
+        ch[4] = rh * hh - p2;
+        State Wl, Wr;
+        cons_to_entropy(l, Wl);
+        cons_to_entropy(r, Wr);
+        double H[5][5];
+        for (int i = 0; i < 5; ++i) H[i][0] = ch[i];
+        for (int d = 0; d < 3; ++d) {
+            T1 e; e[d] = 1;
+            State fl;
+            get_inv_flux(ch, e, fl);
+            for (int i = 0; i < 5; ++i) H[i][1 + d] = fl[i];
+        }
+        for (int i = 0; i < 4; ++i) H[i][4] = H[4][i];
+        H[4][4] = rh * hh * hh - gma * p2 * p2 / (rh * (gma - 1));
+        const double lm = std::fabs(vh[0]) + std::sqrt(gma * p2 / rh);
+        for (int i = 0; i < 5; ++i) {
+            double s = 0;
+            for (int j = 0; j < 5; ++j) s += H[i][j] * (Wr[j] - Wl[j]);
+            f[i] -= 0.5 * lm * s;
+        }
+    }
+    void xflux(const State& l, const State& r, State& f) const {  // set_inv_surf_flux_scheme, .cc:140-197
         switch (scheme) {
             case 0: hllc_xflux(l, r, f); break;
             case 1: rusanov_xflux(l, r, f); break;
             case 2: ausm_xflux(l, r, f); break;
-            default: chandrashekhar_xflux(l, r, f); break;
+            case 3: chandrashekhar_xflux(l, r, f); break;
+            case 4: rusanov_blend_xflux(l, r, 0, f); break;
+            case 5: rusanov_blend_xflux(l, r, 2, f); break;
+            case 6: modified_sw_xflux(l, r, f); break;
+            case 7: kennedy_gruber_xflux(l, r, f); break;
+            case 8: rusanov_blend_xflux(l, r, 7, f); break;
+            default: ismail_roe_xflux(l, r, f); break;
         }
     }
     // rotation to the normal-aligned frame, navier_stokes.cc:482-569
@@ -330,8 +422,56 @@ struct NS {
         for (int d = 0; d < 3; ++d) f[1 + d] = mf[d];
         f[4] = fr[4];
     }
+    void vol_flux(const State& c1, const State& c2, const T1& dir, State& f) const {  // .cc:203-228
+        if (vscheme == 1) kennedy_gruber_vol_flux(c1, c2, dir, f);
+        else if (vscheme == 2) ismail_roe_vol_flux(c1, c2, dir, f);
+        else chandrashekhar_vol_flux(c1, c2, dir, f);
+    }
+    // Kennedy-Gruber volume flux, navier_stokes.cc:1549-1575
+    void kennedy_gruber_vol_flux(const State& c1, const State& c2, const T1& dir, State& f) const {
+        T1 v1, v2, va;
+        for (int d = 0; d < 3; ++d) {
+            v1[d] = c1[1 + d] / c1[0];
+            v2[d] = c2[1 + d] / c2[0];
+            va[d] = 0.5 * (v1[d] + v2[d]);
+        }
+        const double ra = 0.5 * (c1[0] + c2[0]);
+        const double e1 = get_e(c1), e2 = get_e(c2);
+        const double Ea = 0.5 * (e1 + e2 + 0.5 * (dot(v1, v1) + dot(v2, v2)));
+        const double pa = (gma - 1) * 0.5 * (c1[0] * e1 + c2[0] * e2);
+        const double vn = dot(va, dir);
+        f[0] = vn * ra;
+        for (int d = 0; d < 3; ++d) f[1 + d] = ra * vn * va[d] + pa * dir[d];
+        f[4] = vn * (ra * Ea + pa);
+    }
+    // Ismail-Roe means through z = sqrt(rho/p) (1, v, p), navier_stokes.cc:1590-1610
+    void ismail_roe_means(const State& c1, const State& c2, double& rh, double vh[3], double& p2,
+                          double& hh) const {
+        const double a1 = get_a(c1), a2 = get_a(c2), g = std::sqrt(gma);
+        const double z11 = g / a1, z12 = g / a2, z51 = c1[0] * a1 / g, z52 = c2[0] * a2 / g;
+        const double z5ln = log_avg(z51, z52);
+        rh = 0.5 * (z11 + z12) * z5ln;
+        const double p1 = (z51 + z52) / (z11 + z12);
+        p2 = 0.5 * ((gma + 1) * z5ln / log_avg(z11, z12) + (gma - 1) * p1) / gma;
+        hh = gma * p2 / ((gma - 1) * rh);
+        for (int d = 0; d < 3; ++d) {
+            vh[d] = (z11 * c1[1 + d] / c1[0] + z12 * c2[1 + d] / c2[0]) / (z11 + z12);
+            hh += 0.5 * vh[d] * vh[d];
+        }
+    }
+    // Ismail-Roe volume flux, navier_stokes.cc:1590-1622
+    void ismail_roe_vol_flux(const State& c1, const State& c2, const T1& dir, State& f) const {
+        const double a1 = get_a(c1), a2 = get_a(c2), g = std::sqrt(gma);
+        const double p1 = (c1[0] * a1 / g + c2[0] * a2 / g) / (g / a1 + g / a2);
+        double rh, vh[3], p2, hh;
+        ismail_roe_means(c1, c2, rh, vh, p2, hh);
+        const double vn = vh[0] * dir[0] + vh[1] * dir[1] + vh[2] * dir[2];
+        f[0] = rh * vn;
+        for (int d = 0; d < 3; ++d) f[1 + d] = rh * vh[d] * vn + p1 * dir[d];
+        f[4] = rh * vn * hh;
+    }
     // Chandrashekhar volume flux, navier_stokes.cc:1510-1542
-    void vol_flux(const State& c1, const State& c2, const T1& dir, State& f) const {
+    void chandrashekhar_vol_flux(const State& c1, const State& c2, const T1& dir, State& f) const {
         const double p1 = get_p(c1), p2 = get_p(c2);
         const double b1 = 0.5 * c1[0] / p1, b2 = 0.5 * c2[0] / p2, bln = log_avg(b1, b2);
         const double p_hat = 0.5 * (c1[0] + c2[0]) / (b1 + b2);
@@ -1417,6 +1557,7 @@ int oracle_create(int32_t n_cells, int32_t degree, const double* vertices, const
     o->ph = *phys;
     o->ns = NS{phys->gamma, phys->molecular_weight, phys->prandtl, phys->mu0, phys->T0, phys->S,
                phys->inv_surf_flux};
+    o->ns.vscheme = phys->inv_vol_flux;
     o->bcs.assign(bcs, bcs + n_bcs);
     o->nbr.assign(face_nbr, face_nbr + (size_t)n_cells * 6);
     o->bid.assign(face_bid, face_bid + (size_t)n_cells * 6);
@@ -1574,7 +1715,9 @@ int oracle_cob_matrix(int32_t N, double* C) {
 double oracle_log_avg(double xl, double xr) { return NS::log_avg(xl, xr); }
 
 static NS make_ns(const oracle_physics* p, int scheme) {
-    return NS{p->gamma, p->molecular_weight, p->prandtl, p->mu0, p->T0, p->S, scheme};
+    NS ns{p->gamma, p->molecular_weight, p->prandtl, p->mu0, p->T0, p->S, scheme};
+    ns.vscheme = p->inv_vol_flux;
+    return ns;
 }
 
 void oracle_inv_surf_flux(const oracle_physics* p, int32_t scheme, double blender, const double* ul,

@@ -29,10 +29,13 @@ typedef struct oracle oracle;
 
 typedef struct {
     double gamma, molecular_weight, prandtl, mu0, T0, S;
-    int32_t inv_surf_flux;  /* 0 HLLC, 1 Rusanov, 2 AUSM+-up, 3 Chandrashekhar */
+    int32_t inv_surf_flux;  /* 0 HLLC, 1 Rusanov, 2 AUSM+-up, 3 Chandrashekhar, 4 Rusanov-HLLC,
+                               5 Rusanov-AUSM+-up, 6 modified Steger-Warming, 7 Kennedy-Gruber,
+                               8 Rusanov-Kennedy-Gruber, 9 Ismail-Roe */
     double threshold_factor, threshold_exp_factor, sharpness_factor, alpha_min, alpha_max,
         wall_blender_limit;
     int32_t gradient_variables;  /* 0 conservative (CVAR_GRAD build), 1 entropy (default build) */
+    int32_t inv_vol_flux;        /* 0 Chandrashekhar, 1 Kennedy-Gruber, 2 Ismail-Roe */
 } oracle_physics;
 
 typedef struct { int32_t type; double p, T, vel[3]; } oracle_bc; /* type as PLENS_BC_* */

@@ -18,7 +18,8 @@ class OPhys(C.Structure):
                 ("mu0", C.c_double), ("T0", C.c_double), ("S", C.c_double), ("inv_surf_flux", C.c_int32),
                 ("threshold_factor", C.c_double), ("threshold_exp_factor", C.c_double),
                 ("sharpness_factor", C.c_double), ("alpha_min", C.c_double), ("alpha_max", C.c_double),
-                ("wall_blender_limit", C.c_double), ("gradient_variables", C.c_int32)]
+                ("wall_blender_limit", C.c_double), ("gradient_variables", C.c_int32),
+                ("inv_vol_flux", C.c_int32)]
 
 
 class OBC(C.Structure):

@@ -16,7 +16,9 @@ PLENS_E_POSITIVITY = -3
 PLENS_E_NOMEM = -4
 PLENS_E_NODEVICE = -5
 
-FLUX = {"HLLC": 0, "Rusanov": 1, "AUSM+-up": 2, "Chandrashekhar": 3}
+FLUX = {"HLLC": 0, "Rusanov": 1, "AUSM+-up": 2, "Chandrashekhar": 3, "Rusanov-HLLC": 4,
+        "Rusanov-AUSM+-up": 5, "Modified SW": 6, "KG": 7, "Rusanov-KG": 8, "IR": 9}
+VOL_FLUX = {"Chandrashekhar": 0, "KG": 1, "IR": 2}
 BC_PERIODIC, BC_UNIFORM_INFLOW, BC_OUTFLOW, BC_UNIFORM_TEMP_WALL = 0, 1, 2, 3
 BC_FREE, BC_SYMMETRY, BC_EMPTY, BC_INSULATED_WALL, BC_ZPG_INFLOW = 4, 5, 6, 7, 8
 PREC_FP64, PREC_FP32_BULK = 0, 1
@@ -39,7 +41,8 @@ class Physics(C.Structure):
                 ("inv_surf_flux", C.c_int32), ("precision", C.c_int32),
                 ("threshold_factor", C.c_double), ("threshold_exp_factor", C.c_double),
                 ("sharpness_factor", C.c_double), ("alpha_min", C.c_double), ("alpha_max", C.c_double),
-                ("wall_blender_limit", C.c_double), ("gradient_variables", C.c_int32)]
+                ("wall_blender_limit", C.c_double), ("gradient_variables", C.c_int32),
+                ("inv_vol_flux", C.c_int32)]
 
 
 class BC(C.Structure):

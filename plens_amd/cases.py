@@ -97,7 +97,7 @@ class Mesh:
 def make_physics(gamma=1.4, prandtl=0.72, mu0=1e-3, T0=1.0, S=SUTH_RATIO, flux="HLLC",
                  precision=L.PREC_FP64, wall_blender_limit=0.5, alpha_max=0.5, alpha_min=1e-3,
                  threshold_factor=0.5, threshold_exp_factor=1.8, sharpness_factor=9.21024,
-                 gradient_variables=L.GRAD_CONSERVATIVE):
+                 gradient_variables=L.GRAD_CONSERVATIVE, vol_flux="Chandrashekhar"):
     ph = L.Physics()
     ph.gamma, ph.molecular_weight, ph.prandtl = gamma, R0, prandtl
     ph.mu0, ph.T0, ph.S = mu0, T0, S
@@ -107,6 +107,7 @@ def make_physics(gamma=1.4, prandtl=0.72, mu0=1e-3, T0=1.0, S=SUTH_RATIO, flux="
     ph.sharpness_factor, ph.alpha_min, ph.alpha_max = sharpness_factor, alpha_min, alpha_max
     ph.wall_blender_limit = wall_blender_limit
     ph.gradient_variables = gradient_variables
+    ph.inv_vol_flux = L.VOL_FLUX[vol_flux] if isinstance(vol_flux, str) else int(vol_flux)
     return ph
 
 
@@ -240,6 +241,20 @@ def with_gradient_variables(case, variables):
     ph = L.Physics.from_buffer_copy(case.physics)
     ph.gradient_variables = codes[variables]
     return dataclasses.replace(case, physics=ph, meta={**case.meta, "gradient_variables": variables})
+
+
+def with_fluxes(case, surface=None, volume=None):
+    """The same case with another inviscid surface flux and/or volume (two-point) flux scheme,
+    named as the reference's "inviscid surface/volume flux scheme" parameters."""
+    ph = L.Physics.from_buffer_copy(case.physics)
+    meta = dict(case.meta)
+    if surface is not None:
+        ph.inv_surf_flux = L.FLUX[surface]
+        meta["flux"] = surface
+    if volume is not None:
+        ph.inv_vol_flux = L.VOL_FLUX[volume]
+        meta["vol_flux"] = volume
+    return dataclasses.replace(case, physics=ph, meta=meta)
 
 
 def with_precision(case, precision):

@@ -248,9 +248,179 @@ __device__ void chandrashekhar_xflux(const T l[5], const T r[5], T gma, T blende
     for (int v = 0; v < 5; ++v) f[v] -= T(0.5) * kw[v];
 }
 
+// K diag(lam) K^-1 x for the x-direction eigenvectors of velocity v, sound speed a and enthalpy H
+// (get_xK / get_xKinv, navier_stokes.cc:641-678, 780-825; the rows as in chandrashekhar_xflux)
+template <typename T>
+__device__ __forceinline__ void kxlam_kinv(const T v[3], T a, T H, T gma, const T lam[5], const T x[5], T y[5]) {
+    const T t = dv(T(1), gma - T(1)), a2t = T(2) * a * a * t;
+    const T fac = dv(T(0.5) * (gma - T(1)), a * a);
+    T w[5];
+    w[0] = fac * ((H + a * (v[0] - a) * t) * x[0] + (-v[0] - a * t) * x[1] - v[1] * x[2] - v[2] * x[3] + x[4]);
+    w[1] = fac * (T(2) * (T(2) * a * a * t - H) * x[0] + T(2) * v[0] * x[1] + T(2) * v[1] * x[2] +
+                  T(2) * v[2] * x[3] - T(2) * x[4]);
+    w[2] = fac * (-v[1] * a2t * x[0] + a2t * x[2]);
+    w[3] = fac * (-v[2] * a2t * x[0] + a2t * x[3]);
+    w[4] = fac * ((H - a * (v[0] + a) * t) * x[0] + (-v[0] + a * t) * x[1] - v[1] * x[2] - v[2] * x[3] + x[4]);
+    for (int k = 0; k < 5; ++k) w[k] *= lam[k];
+    const T v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    y[0] = w[0] + w[1] + w[4];
+    y[1] = (v[0] - a) * w[0] + v[0] * w[1] + (v[0] + a) * w[4];
+    y[2] = v[1] * w[0] + v[1] * w[1] + w[2] + v[1] * w[4];
+    y[3] = v[2] * w[0] + v[2] * w[1] + w[3] + v[2] * w[4];
+    y[4] = (H - v[0] * a) * w[0] + T(0.5) * v2 * w[1] + v[1] * w[2] + v[2] * w[3] + (H + v[0] * a) * w[4];
+}
+
+// modified Steger-Warming x-flux (modified_sw_xflux, navier_stokes.cc:1160-1235): pressure-switched
+// states, smoothed split eigenvalues (eps 0.3), f = A+ u_l + A- u_r
+template <typename T>
+__device__ void modified_sw_xflux(const T l[5], const T r[5], T gma, T f[5]) {
+    const T pl = get_p(l, gma), pr = get_p(r, gma);
+    const T sw = dv(T(0.5) * (pr - pl), mn(pl, pr));
+    const T om = dv(T(0.5), T(1) + sw * sw);
+    const T ril = dv(T(1), l[0]), rir = dv(T(1), r[0]);
+    T vp[3], vn[3];
+    for (int d = 0; d < 3; ++d) {
+        vp[d] = (T(1) - om) * (l[1 + d] * ril) + om * (r[1 + d] * rir);
+        vn[d] = (T(1) - om) * (r[1 + d] * rir) + om * (l[1 + d] * ril);
+    }
+    const T rp = (T(1) - om) * l[0] + om * r[0], rn = (T(1) - om) * r[0] + om * l[0];
+    const T pp = (T(1) - om) * pl + om * pr, pn = (T(1) - om) * pr + om * pl;
+    const T ap = sq(dv(gma * pp, rp)), an = sq(dv(gma * pn, rn));
+    const T Hp = dv(gma * pp, (gma - T(1)) * rp) + T(0.5) * (vp[0] * vp[0] + vp[1] * vp[1] + vp[2] * vp[2]);
+    const T Hn = dv(gma * pn, (gma - T(1)) * rn) + T(0.5) * (vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2]);
+    const T ep = T(0.3) * ap, en = T(0.3) * an;
+    auto psm = [](T x, T e) { return T(0.5) * (x + sq(x * x + e * e)); };  // pos_smooth (navier_stokes.h:464)
+    auto nsm = [](T x, T e) { return T(0.5) * (x - sq(x * x + e * e)); };  // neg_smooth (navier_stokes.h:473)
+    const T lp[5] = {psm(vp[0] - ap, ep), psm(vp[0], ep), psm(vp[0], ep), psm(vp[0], ep), psm(vp[0] + ap, ep)};
+    const T ln[5] = {nsm(vn[0] - an, en), nsm(vn[0], en), nsm(vn[0], en), nsm(vn[0], en), nsm(vn[0] + an, en)};
+    T fp[5], fn[5];
+    kxlam_kinv(vp, ap, Hp, gma, lp, l, fp);
+    kxlam_kinv(vn, an, Hn, gma, ln, r, fn);
+    for (int v = 0; v < 5; ++v) f[v] = fp[v] + fn[v];
+}
+
+// Kennedy-Gruber x-flux with the Rusanov-type stabilisation (kennedy_gruber_xflux,
+// navier_stokes.cc:1322-1355)
+template <typename T>
+__device__ void kennedy_gruber_xflux(const T l[5], const T r[5], T gma, T f[5]) {
+    const T ril = dv(T(1), l[0]), rir = dv(T(1), r[0]);
+    T vl[3], vr[3], va[3];
+    for (int d = 0; d < 3; ++d) {
+        vl[d] = l[1 + d] * ril;
+        vr[d] = r[1 + d] * rir;
+        va[d] = T(0.5) * (vl[d] + vr[d]);
+    }
+    const T ra = T(0.5) * (l[0] + r[0]);
+    const T el = get_e(l), er = get_e(r);
+    const T Ea = T(0.5) * (el + er + T(0.5) * ((vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]) +
+                                                (vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2])));
+    const T pa = (gma - T(1)) * T(0.5) * (l[0] * el + r[0] * er);
+    f[0] = va[0] * ra;
+    for (int d = 0; d < 3; ++d) f[1 + d] = ra * va[0] * va[d];
+    f[1] += pa;
+    f[4] = va[0] * (ra * Ea + pa);
+    const T lm = mx(ab(vl[0]) + sq(gma * (gma - T(1)) * el), ab(vr[0]) + sq(gma * (gma - T(1)) * er));
+    for (int v = 0; v < 5; ++v) f[v] -= T(0.5) * lm * (r[v] - l[v]);
+}
+
+// Ismail-Roe x-flux with the entropy-variable stabilisation (ismail_roe_xflux,
+// navier_stokes.cc:1373-1442)
+template <typename T>
+__device__ void ismail_roe_xflux(const T l[5], const T r[5], T gma, T f[5]) {
+    const T gs = sq(gma);
+    const T al = get_a(l, gma), ar = get_a(r, gma);
+    const T z1l = dv(gs, al), z1r = dv(gs, ar);
+    const T z5l = dv(l[0] * al, gs), z5r = dv(r[0] * ar, gs);
+    const T z5ln = log_avg_t(z5l, z5r);
+    const T rh = T(0.5) * (z1l + z1r) * z5ln;
+    const T p1 = dv(z5l + z5r, z1l + z1r);
+    const T p2 = dv(T(0.5) * (dv((gma + T(1)) * z5ln, log_avg_t(z1l, z1r)) + (gma - T(1)) * p1), gma);
+    T hh = dv(gma * p2, (gma - T(1)) * rh);
+    T vl[3], vr[3], vh[3];
+    for (int d = 0; d < 3; ++d) {
+        vl[d] = dv(l[1 + d], l[0]);
+        vr[d] = dv(r[1 + d], r[0]);
+        vh[d] = dv(z1l * vl[d] + z1r * vr[d], z1l + z1r);
+        hh += T(0.5) * (vh[d] * vh[d]);
+    }
+    f[0] = rh * vh[0];
+    for (int d = 0; d < 3; ++d) f[1 + d] = rh * vh[0] * vh[d];
+    f[1] += p1;
+    f[4] = rh * hh * vh[0];
+    const T ch[5] = {rh, rh * vh[0], rh * vh[1], rh * vh[2], rh * hh - p2};
+    const T pl = get_p(l, gma), pr = get_p(r, gma);
+    const T sl = lg(pl) - gma * lg(l[0]), sr = lg(pr) - gma * lg(r[0]);
+    T dV[5];
+    dV[0] = (dv(gma - sr, gma - T(1)) - dv(T(0.5) * r[0] * (vr[0] * vr[0] + vr[1] * vr[1] + vr[2] * vr[2]), pr)) -
+            (dv(gma - sl, gma - T(1)) - dv(T(0.5) * l[0] * (vl[0] * vl[0] + vl[1] * vl[1] + vl[2] * vl[2]), pl));
+    for (int d = 0; d < 3; ++d) dV[1 + d] = dv(r[1 + d], pr) - dv(l[1 + d], pl);
+    dV[4] = -dv(r[0], pr) + dv(l[0], pl);
+    T H[5][5];
+    for (int i = 0; i < 5; ++i) H[i][0] = ch[i];
+    for (int d = 0; d < 3; ++d) {
+        const T e[3] = {T(d == 0), T(d == 1), T(d == 2)};
+        T fl[5];
+        inv_flux(ch, e, gma, fl);
+        for (int i = 0; i < 5; ++i) H[i][1 + d] = fl[i];
+    }
+    for (int i = 0; i < 4; ++i) H[i][4] = H[4][i];
+    H[4][4] = rh * hh * hh - dv(gma * p2 * p2, rh * (gma - T(1)));
+    const T lm = ab(vh[0]) + sq(dv(gma * p2, rh));
+    for (int i = 0; i < 5; ++i) {
+        T s = T(0);
+        for (int j = 0; j < 5; ++j) s += H[i][j] * dV[j];
+        f[i] -= T(0.5) * lm * s;
+    }
+}
+
+// two-point volume fluxes along an unnormalised direction a (linear in a): Kennedy-Gruber
+// (kennedy_gruber_vol_flux, navier_stokes.cc:1551-1574) and Ismail-Roe (ismail_roe_vol_flux,
+// 1589-1617)
+template <typename T>
+__device__ void kennedy_gruber_vol_flux(const T u1[5], const T u2[5], const T a[3], T gma, T f[5]) {
+    const T r1 = dv(T(1), u1[0]), r2 = dv(T(1), u2[0]);
+    T v1[3], v2[3], va[3];
+    for (int d = 0; d < 3; ++d) {
+        v1[d] = u1[1 + d] * r1;
+        v2[d] = u2[1 + d] * r2;
+        va[d] = T(0.5) * (v1[d] + v2[d]);
+    }
+    const T ra = T(0.5) * (u1[0] + u2[0]);
+    const T e1 = get_e(u1), e2 = get_e(u2);
+    const T Ea = T(0.5) * (e1 + e2 + T(0.5) * ((v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]) +
+                                                (v2[0] * v2[0] + v2[1] * v2[1] + v2[2] * v2[2])));
+    const T pa = (gma - T(1)) * T(0.5) * (u1[0] * e1 + u2[0] * e2);
+    const T vn = va[0] * a[0] + va[1] * a[1] + va[2] * a[2];
+    f[0] = vn * ra;
+    for (int d = 0; d < 3; ++d) f[1 + d] = ra * vn * va[d] + pa * a[d];
+    f[4] = vn * (ra * Ea + pa);
+}
+template <typename T>
+__device__ void ismail_roe_vol_flux(const T u1[5], const T u2[5], const T a[3], T gma, T f[5]) {
+    const T gs = sq(gma);
+    const T a1 = get_a(u1, gma), a2 = get_a(u2, gma);
+    const T z11 = dv(gs, a1), z12 = dv(gs, a2);
+    const T z51 = dv(u1[0] * a1, gs), z52 = dv(u2[0] * a2, gs);
+    const T z5ln = log_avg_t(z51, z52);
+    const T rh = T(0.5) * (z11 + z12) * z5ln;
+    const T p1 = dv(z51 + z52, z11 + z12);
+    const T p2 = dv(T(0.5) * (dv((gma + T(1)) * z5ln, log_avg_t(z11, z12)) + (gma - T(1)) * p1), gma);
+    T hh = dv(gma * p2, (gma - T(1)) * rh);
+    T vh[3];
+    for (int d = 0; d < 3; ++d) {
+        vh[d] = dv(z11 * dv(u1[1 + d], u1[0]) + z12 * dv(u2[1 + d], u2[0]), z11 + z12);
+        hh += T(0.5) * (vh[d] * vh[d]);
+    }
+    const T vn = vh[0] * a[0] + vh[1] * a[1] + vh[2] * a[2];
+    f[0] = rh * vn;
+    for (int d = 0; d < 3; ++d) f[1 + d] = rh * vh[d] * vn + p1 * a[d];
+    f[4] = rh * vn * hh;
+}
+
 // numerical flux through unit normal n with the normal-aligned rotation of
 // get_inv_surf_flux (navier_stokes.cc:482-569)
-template <typename T>
+// EXT = false compiles schemes 0-3 only (the kernels' common instantiation); EXT = true all ten
+template <bool EXT, typename T>
 __device__ void inv_surf_flux(int scheme, T blender, const T ul[5], const T ur[5], const T n[3], T gma, T f[5]) {
     // m = x cross n = (0, -n2, n1)
     T m1 = -n[2], m2 = n[1];
@@ -276,11 +446,32 @@ __device__ void inv_surf_flux(int scheme, T blender, const T ul[5], const T ur[5
         lr[1 + a] = R[0][a] * ul[1] + R[1][a] * ul[2] + R[2][a] * ul[3];
         rr[1 + a] = R[0][a] * ur[1] + R[1][a] * ur[2] + R[2][a] * ur[3];
     }
-    switch (scheme) {
-        case 0: hllc_xflux(lr, rr, gma, fr); break;
-        case 1: rusanov_xflux(lr, rr, gma, fr); break;
-        case 2: ausm_xflux(lr, rr, gma, fr); break;
-        default: chandrashekhar_xflux(lr, rr, gma, blender, fr); break;
+    if constexpr (!EXT) {
+        switch (scheme) {
+            case 0: hllc_xflux(lr, rr, gma, fr); break;
+            case 1: rusanov_xflux(lr, rr, gma, fr); break;
+            case 2: ausm_xflux(lr, rr, gma, fr); break;
+            default: chandrashekhar_xflux(lr, rr, gma, blender, fr); break;
+        }
+    } else {
+        switch (scheme) {
+            case 0: hllc_xflux(lr, rr, gma, fr); break;
+            case 1: rusanov_xflux(lr, rr, gma, fr); break;
+            case 2: ausm_xflux(lr, rr, gma, fr); break;
+            case 3: chandrashekhar_xflux(lr, rr, gma, blender, fr); break;
+            case 6: modified_sw_xflux(lr, rr, gma, fr); break;
+            case 7: kennedy_gruber_xflux(lr, rr, gma, fr); break;
+            case 9: ismail_roe_xflux(lr, rr, gma, fr); break;
+            default: {  // Rusanov blends, weight = flux blender (navier_stokes.cc:1132-1160, 1364-1374)
+                T fa[5], fb[5];
+                rusanov_xflux(lr, rr, gma, fa);
+                if (scheme == 4) hllc_xflux(lr, rr, gma, fb);
+                else if (scheme == 5) ausm_xflux(lr, rr, gma, fb);
+                else kennedy_gruber_xflux(lr, rr, gma, fb);
+                for (int v = 0; v < 5; ++v) fr[v] = blender * fa[v] + (T(1) - blender) * fb[v];
+                break;
+            }
+        }
     }
     f[0] = fr[0];
     f[4] = fr[4];

@@ -118,7 +118,8 @@ struct KP {
     int save_E;              // this launch records rhoE_old
     unsigned long long* fail;  // [0] lowest failing code of the run, [1] [0] latched after the first failing stage
     Gas gas;
-    int scheme;
+    int scheme;              // PLENS_FLUX_*
+    int vscheme;             // PLENS_VOL_FLUX_*
     double thr, sharp, amin, amax, wlimit, courant;
     const double* cfl;       // Courant number of every instance (batches; nullptr: courant)
     double iw0, iwN;  // 1 / w_0, 1 / w_N of the GLL rule
@@ -838,7 +839,7 @@ __global__ void __launch_bounds__(256) k_ss_reduce(const double* part, int nc, d
 // =============================================================================================
 // k_face
 // =============================================================================================
-template <int N, typename TF>
+template <int N, typename TF, bool EXT>
 __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
     const long long gid = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -874,7 +875,7 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     for (int d = 0; d < 3; ++d) nt[d] = (TF)n[d];
     const TF gma = (TF)gs.gma;
     TF F[5];
-    inv_surf_flux(p.scheme, (TF)bl, ut, ugt, nt, gma, F);
+    inv_surf_flux<EXT>(p.scheme, (TF)bl, ut, ugt, nt, gma, F);
     // Each side stores its finished surface term of calc_cell_ho_inv/dif_residual
     // (dgsem/plens.cc:3237-3268, 3373-3406): -s (-s F*_side - f(u; Ja/|Ja|)) |Ja| / w_end, with
     // F*_side the outward flux of that side (-F for the neighbour of an interior face) and
@@ -958,7 +959,7 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
 // =============================================================================================
 // k_lo: first-order subcell FV residual of blended cells (calc_cell_lo_inv_residual, 3469-3583)
 // =============================================================================================
-template <int N, typename TF>
+template <int N, typename TF, bool EXT>
 __global__ void __launch_bounds__(256) k_lo(KP p) {
     constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
     extern __shared__ double smem[];
@@ -1023,7 +1024,7 @@ __global__ void __launch_bounds__(256) k_lo(KP p) {
                 const int jr = lc * NN + tr[0] + NP * (tr[1] + NP * tr[2]);
                 TF ur[5], fl[5];
                 for (int v = 0; v < 5; ++v) ur[v] = sQ[v * B + jr];
-                inv_surf_flux(p.scheme, bl, u, ur, sn, gma, fl);
+                inv_surf_flux<EXT>(p.scheme, bl, u, ur, sn, gma, fl);
                 for (int v = 0; v < 5; ++v) sX[v * B + li] = fl[v] * nn;
             }
             __syncthreads();
@@ -1074,10 +1075,99 @@ __device__ __forceinline__ double ln_mean(double x, double y, double lx, double 
     return dl / (lx - ly);
 }
 
+// Pair fluxes of the volume term from per-node data in LDS (sR: 2 FP64 slots, sA: 6 flux-type
+// slots, B entries each). Logarithmic means share reciprocals: f = (x-y)/(x+y); below f^2 = 1e-2
+// the series of log_avg (navier_stokes.h:494-501: sums, differences, branch and series in FP64 in
+// every variant), else (x-y)/(ln x - ln y) with the per-node logarithms.
+template <typename TF>
+struct LogMean {
+    double isum;   // 1/(x+y)
+    TF xln, inv;   // mean and its reciprocal
+};
+template <typename TF>
+__device__ __forceinline__ LogMean<TF> log_mean(double xa, double xb, TF lxa, TF lxb) {
+    LogMean<TF> m;
+    const double s = xa + xb, d = xa - xb;
+    m.isum = rcp64(s);
+    const double f = d * m.isum, u = f * f;
+    if (u < 1e-2) {
+        const double F = 1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0)));
+        m.inv = (TF)(2.0 * F * m.isum);
+        m.xln = (TF)(0.5 * s * rcp64(F));
+    } else {
+        const TF dl = lxa - lxb;
+        m.inv = dl * rcpt((TF)d);
+        m.xln = (TF)d * rcpt(dl);
+    }
+    return m;
+}
+// Chandrashekhar (chandrashekhar_vol_flux, navier_stokes.cc:1510-1542). sR: rho, beta = rho/(2p);
+// sA: ln rho, ln beta, v0, v1, v2, |v|^2
+template <typename TF>
+__device__ __forceinline__ void pair_chandrashekhar(const double* sR, const TF* sA, int B, int ja, int jb,
+                                                    const TF dir[3], TF half_gm1_inv, TF f[5]) {
+    const LogMean<TF> mr = log_mean<TF>(sR[ja], sR[jb], sA[ja], sA[jb]);
+    const LogMean<TF> mb = log_mean<TF>(sR[B + ja], sR[B + jb], sA[B + ja], sA[B + jb]);
+    const TF p_hat = (TF)(0.5 * (sR[ja] + sR[jb]) * mb.isum);
+    const TF va0 = TF(0.5) * (sA[2 * B + ja] + sA[2 * B + jb]);
+    const TF va1 = TF(0.5) * (sA[3 * B + ja] + sA[3 * B + jb]);
+    const TF va2 = TF(0.5) * (sA[4 * B + ja] + sA[4 * B + jb]);
+    const TF H = half_gm1_inv * mb.inv + p_hat * mr.inv + (va0 * va0 + va1 * va1 + va2 * va2) -
+                 TF(0.25) * (sA[5 * B + ja] + sA[5 * B + jb]);
+    const TF rv = mr.xln * (va0 * dir[0] + va1 * dir[1] + va2 * dir[2]);
+    f[0] = rv;
+    f[1] = rv * va0 + p_hat * dir[0];
+    f[2] = rv * va1 + p_hat * dir[1];
+    f[3] = rv * va2 + p_hat * dir[2];
+    f[4] = rv * H;
+}
+// Kennedy-Gruber (kennedy_gruber_vol_flux, navier_stokes.cc:1549-1575). sR: rho, p; sA: e, -, v0,
+// v1, v2, |v|^2
+template <typename TF>
+__device__ __forceinline__ void pair_kennedy_gruber(const double* sR, const TF* sA, int B, int ja, int jb,
+                                                    const TF dir[3], TF f[5]) {
+    const TF ra = (TF)(0.5 * (sR[ja] + sR[jb]));
+    const TF pa = (TF)(0.5 * (sR[B + ja] + sR[B + jb]));
+    const TF va0 = TF(0.5) * (sA[2 * B + ja] + sA[2 * B + jb]);
+    const TF va1 = TF(0.5) * (sA[3 * B + ja] + sA[3 * B + jb]);
+    const TF va2 = TF(0.5) * (sA[4 * B + ja] + sA[4 * B + jb]);
+    const TF Ea = TF(0.5) * (sA[ja] + sA[jb] + TF(0.5) * (sA[5 * B + ja] + sA[5 * B + jb]));
+    const TF rv = ra * (va0 * dir[0] + va1 * dir[1] + va2 * dir[2]);
+    f[0] = rv;
+    f[1] = rv * va0 + pa * dir[0];
+    f[2] = rv * va1 + pa * dir[1];
+    f[3] = rv * va2 + pa * dir[2];
+    f[4] = rv * Ea + pa * (va0 * dir[0] + va1 * dir[1] + va2 * dir[2]);
+}
+// Ismail-Roe (ismail_roe_vol_flux, navier_stokes.cc:1590-1622) through z1 = sqrt(rho/p),
+// z5 = sqrt(rho p). sR: z5, z1; sA: ln z5, ln z1, v0, v1, v2, -
+template <typename TF>
+__device__ __forceinline__ void pair_ismail_roe(const double* sR, const TF* sA, int B, int ja, int jb,
+                                                const TF dir[3], TF gma, TF f[5]) {
+    const LogMean<TF> m5 = log_mean<TF>(sR[ja], sR[jb], sA[ja], sA[jb]);
+    const LogMean<TF> m1 = log_mean<TF>(sR[B + ja], sR[B + jb], sA[B + ja], sA[B + jb]);
+    const TF za = (TF)sR[B + ja], zb = (TF)sR[B + jb];
+    const TF iz = (TF)m1.isum;                                  // 1/(z1a + z1b)
+    const TF rh = (TF)(0.5 * (sR[B + ja] + sR[B + jb])) * m5.xln;
+    const TF p1 = (TF)((sR[ja] + sR[jb]) * m1.isum);
+    const TF p2 = dv(TF(0.5) * ((gma + TF(1)) * m5.xln * m1.inv + (gma - TF(1)) * p1), gma);
+    const TF vh0 = (za * sA[2 * B + ja] + zb * sA[2 * B + jb]) * iz;
+    const TF vh1 = (za * sA[3 * B + ja] + zb * sA[3 * B + jb]) * iz;
+    const TF vh2 = (za * sA[4 * B + ja] + zb * sA[4 * B + jb]) * iz;
+    const TF vn = vh0 * dir[0] + vh1 * dir[1] + vh2 * dir[2];
+    const TF rv = rh * vn;
+    f[0] = rv;
+    f[1] = rv * vh0 + p1 * dir[0];
+    f[2] = rv * vh1 + p1 * dir[1];
+    f[3] = rv * vh2 + p1 * dir[2];
+    // rho_hat vn h_hat with h_hat = gamma p2 / ((gamma-1) rho_hat) + |v_hat|^2 / 2
+    f[4] = vn * (dv(gma, gma - TF(1)) * p2 + TF(0.5) * rh * (vh0 * vh0 + vh1 * vh1 + vh2 * vh2));
+}
+
 // =============================================================================================
 // k_vol: high-order inviscid + viscous residuals, blend, RK epilogue
 // =============================================================================================
-template <int N, typename TF>
+template <int N, typename TF, bool EXT>
 __global__ void __launch_bounds__(256) k_vol(KP p) {
     constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
     constexpr int NPAIR = NP * (NP - 1) / 2;       // unordered pairs on one line
@@ -1088,8 +1178,8 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
     __shared__ int sPA[NPAIR > 0 ? NPAIR : 1], sPB[NPAIR > 0 ? NPAIR : 1];
     const int B = p.cpb * NN;
     const int BP = p.cpb * NITEM;
-    double* sR = smem;                               // 2B: rho, beta (FP64 in every variant)
-    TF* sA = reinterpret_cast<TF*>(sR + 2 * B);      // 6B: ln rho, ln beta, v0, v1, v2, |v|^2
+    double* sR = smem;                               // 2B: per-node FP64 data of the pair flux
+    TF* sA = reinterpret_cast<TF*>(sR + 2 * B);      // 6B: per-node flux-type data of the pair flux
     TF* sJa = sA + 6 * B;                            // 3B: Ja^md of the direction being processed
     TF* sF = sJa + 3 * B;                            // 5*max(BP,B) pair fluxes / viscous scratch
     load_basis<N>(p.basis, sD, sW, sQm, sC);
@@ -1116,14 +1206,27 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
         load_state(p.Q, p.ndof, g, u);
         node_metrics<N>(p, c, node, jm, detJ);
     }
+    const int vs = EXT ? p.vscheme : PLENS_VOL_FLUX_CHANDRASHEKHAR;
     if (inblk) {
         const double pr = get_p(u, gs.gma);
-        const double beta = 0.5 * u[0] / pr;
         const double v0 = u[1] / u[0], v1 = u[2] / u[0], v2 = u[3] / u[0];
-        sR[0 * B + li] = u[0];
-        sR[1 * B + li] = beta;
-        sA[0 * B + li] = lg((TF)u[0]);
-        sA[1 * B + li] = lg((TF)beta);
+        if (vs == PLENS_VOL_FLUX_CHANDRASHEKHAR) {
+            const double beta = 0.5 * u[0] / pr;
+            sR[0 * B + li] = u[0];
+            sR[1 * B + li] = beta;
+            sA[0 * B + li] = lg((TF)u[0]);
+            sA[1 * B + li] = lg((TF)beta);
+        } else if (vs == PLENS_VOL_FLUX_KG) {
+            sR[0 * B + li] = u[0];
+            sR[1 * B + li] = pr;
+            sA[0 * B + li] = (TF)get_e(u);
+        } else {
+            const double z5 = sqrt(u[0] * pr), z1 = sqrt(u[0] / pr);
+            sR[0 * B + li] = z5;
+            sR[1 * B + li] = z1;
+            sA[0 * B + li] = lg((TF)z5);
+            sA[1 * B + li] = lg((TF)z1);
+        }
         sA[2 * B + li] = (TF)v0;
         sA[3 * B + li] = (TF)v1;
         sA[4 * B + li] = (TF)v2;
@@ -1159,43 +1262,11 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
             const int jb = cl * NN + ib[0] + NP * (ib[1] + NP * ib[2]);
             TF dir[3];
             for (int q = 0; q < 3; ++q) dir[q] = TF(0.5) * (sJa[q * B + ja] + sJa[q * B + jb]);
-            // log means with shared reciprocals: f = (x-y)/(x+y); series below f^2 = 1e-2 (sums,
-            // differences, branch and series in FP64 in every variant), else (x-y)/(ln x - ln y)
-            // with the per-node logarithms
-            const double ra = sR[ja], rb = sR[jb];
-            const double ba = sR[B + ja], bb = sR[B + jb];
-            const double sr = ra + rb, dr = ra - rb, sb = ba + bb, db = ba - bb;
-            const double irs = rcp64(sr), ibs = rcp64(sb);
-            const double fr = dr * irs, fb = db * ibs;
-            const double ur = fr * fr, ub = fb * fb;
-            TF rho_ln, inv_rho_ln, inv_b_ln;
-            if (ur < 1e-2) {
-                const double F = 1.0 + ur * (1.0 / 3.0 + ur * (1.0 / 5.0 + ur * (1.0 / 7.0)));
-                inv_rho_ln = (TF)(2.0 * F * irs);
-                rho_ln = (TF)(0.5 * sr * rcp64(F));
-            } else {
-                const TF dl = sA[ja] - sA[jb];
-                inv_rho_ln = dl * rcpt((TF)dr);
-                rho_ln = (TF)dr * rcpt(dl);
-            }
-            if (ub < 1e-2) {
-                const double F = 1.0 + ub * (1.0 / 3.0 + ub * (1.0 / 5.0 + ub * (1.0 / 7.0)));
-                inv_b_ln = (TF)(2.0 * F * ibs);
-            } else {
-                inv_b_ln = (sA[B + ja] - sA[B + jb]) * rcpt((TF)db);
-            }
-            const TF p_hat = (TF)(0.5 * sr * ibs);
-            const TF va0 = TF(0.5) * (sA[2 * B + ja] + sA[2 * B + jb]);
-            const TF va1 = TF(0.5) * (sA[3 * B + ja] + sA[3 * B + jb]);
-            const TF va2 = TF(0.5) * (sA[4 * B + ja] + sA[4 * B + jb]);
-            const TF H = half_gm1_inv * inv_b_ln + p_hat * inv_rho_ln + (va0 * va0 + va1 * va1 + va2 * va2) -
-                         TF(0.25) * (sA[5 * B + ja] + sA[5 * B + jb]);
-            const TF rv = rho_ln * (va0 * dir[0] + va1 * dir[1] + va2 * dir[2]);
-            sF[0 * BP + it] = rv;
-            sF[1 * BP + it] = rv * va0 + p_hat * dir[0];
-            sF[2 * BP + it] = rv * va1 + p_hat * dir[1];
-            sF[3 * BP + it] = rv * va2 + p_hat * dir[2];
-            sF[4 * BP + it] = rv * H;
+            TF fp[5];
+            if (vs == PLENS_VOL_FLUX_KG) pair_kennedy_gruber(sR, sA, B, ja, jb, dir, fp);
+            else if (vs == PLENS_VOL_FLUX_IR) pair_ismail_roe(sR, sA, B, ja, jb, dir, gma, fp);
+            else pair_chandrashekhar(sR, sA, B, ja, jb, dir, half_gm1_inv, fp);
+            for (int v = 0; v < 5; ++v) sF[v * BP + it] = fp[v];
         }
         __syncthreads();
         if (inblk) {
@@ -1427,6 +1498,9 @@ int launch_one_t(plens_gpu* h, const KP& kp, int kernel) {
     const int thr = threads_for(kp.cpb, NN);
     const int blocks = (kp.nc + kp.cpb - 1) / kp.cpb;
     const size_t B = (size_t)kp.cpb * NN;
+    // surface schemes 4-9 and the Kennedy-Gruber / Ismail-Roe volume fluxes run in a second
+    // instantiation so that the common one keeps its register budget
+    const bool ext = kp.scheme > PLENS_FLUX_CHANDRASHEKHAR || kp.vscheme != PLENS_VOL_FLUX_CHANDRASHEKHAR;
     // dynamic LDS: FP64 blender/modal arrays + flux-type states and metrics (k_grad); flux-type
     // states, metrics and subcell fluxes (k_lo); FP64 rho/beta + flux-type node data, metrics and
     // pair fluxes (k_vol)
@@ -1450,19 +1524,30 @@ int launch_one_t(plens_gpu* h, const KP& kp, int kernel) {
         case 2: {
             const long long fthreads = (long long)kp.ntasks * NF;
             const int ft = kp.face_threads;
-            hipLaunchKernelGGL((k_face<N, TF>), dim3((unsigned)((fthreads + ft - 1) / ft)), dim3(ft), 0, h->stream,
-                               kp);
+            if (ext)
+                hipLaunchKernelGGL((k_face<N, TF, true>), dim3((unsigned)((fthreads + ft - 1) / ft)), dim3(ft), 0,
+                                   h->stream, kp);
+            else
+                hipLaunchKernelGGL((k_face<N, TF, false>), dim3((unsigned)((fthreads + ft - 1) / ft)), dim3(ft), 0,
+                                   h->stream, kp);
             break;
         }
         case 3:
-            hipLaunchKernelGGL((k_lo<N, TF>), dim3(std::min(blocks, 256)), dim3(thr), sizeof(TF) * 19 * B, h->stream,
-                               kp);
+            if (ext)
+                hipLaunchKernelGGL((k_lo<N, TF, true>), dim3(std::min(blocks, 256)), dim3(thr), sizeof(TF) * 19 * B,
+                                   h->stream, kp);
+            else
+                hipLaunchKernelGGL((k_lo<N, TF, false>), dim3(std::min(blocks, 256)), dim3(thr), sizeof(TF) * 19 * B,
+                                   h->stream, kp);
             break;
         case 4: {
             const size_t nitem = (size_t)NP * NP * (NP * (NP - 1) / 2);
             const size_t lds_vol =
                 sizeof(double) * 2 * B + sizeof(TF) * (9 * B + 5 * std::max((size_t)kp.cpb * nitem, B));
-            hipLaunchKernelGGL((k_vol<N, TF>), dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
+            if (ext)
+                hipLaunchKernelGGL((k_vol<N, TF, true>), dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
+            else
+                hipLaunchKernelGGL((k_vol<N, TF, false>), dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
             break;
         }
         default:
@@ -1727,7 +1812,8 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
     if (m->degree < 1 || m->degree > 5 || m->n_cells < 1) return PLENS_E_INVALID;
     if (!m->face_normals || !m->min_vertex_dist || !m->face_nbr || !m->face_bid) return PLENS_E_INVALID;
     if (!m->vertices && (!m->Ja || !m->detJ)) return PLENS_E_INVALID;
-    if (ph->inv_surf_flux < 0 || ph->inv_surf_flux > 3) return PLENS_E_INVALID;
+    if (ph->inv_surf_flux < 0 || ph->inv_surf_flux > PLENS_FLUX_IR) return PLENS_E_INVALID;
+    if (ph->inv_vol_flux < 0 || ph->inv_vol_flux > PLENS_VOL_FLUX_IR) return PLENS_E_INVALID;
     if (ph->precision != PLENS_PREC_FP64 && ph->precision != PLENS_PREC_FP32_BULK) return PLENS_E_INVALID;
     if (ph->gradient_variables != PLENS_GRAD_CONSERVATIVE && ph->gradient_variables != PLENS_GRAD_ENTROPY)
         return PLENS_E_INVALID;
@@ -1895,6 +1981,7 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
     k.gas.gma = ph->gamma; k.gas.R = R; k.gas.cv = R / (ph->gamma - 1); k.gas.mu0 = ph->mu0; k.gas.T0 = ph->T0;
     k.gas.S = ph->S; k.gas.Pr = ph->prandtl; k.gas.inviscid = ph->mu0 < 1e-16;
     k.scheme = ph->inv_surf_flux;
+    k.vscheme = ph->inv_vol_flux;
     k.evar = ph->gradient_variables == PLENS_GRAD_ENTROPY;
     k.thr = ph->threshold_factor * pow(10.0, -ph->threshold_exp_factor * (pow(N + 1, 0.25)));
     k.sharp = ph->sharpness_factor; k.amin = ph->alpha_min; k.amax = ph->alpha_max;
@@ -2291,7 +2378,7 @@ namespace {
 
 bool same_scheme(const KP& a, const KP& b) {
     return a.gas.gma == b.gas.gma && a.gas.R == b.gas.R && a.gas.cv == b.gas.cv && a.gas.Pr == b.gas.Pr &&
-           a.gas.inviscid == b.gas.inviscid && a.scheme == b.scheme && a.thr == b.thr && a.sharp == b.sharp &&
+           a.gas.inviscid == b.gas.inviscid && a.scheme == b.scheme && a.vscheme == b.vscheme && a.thr == b.thr && a.sharp == b.sharp &&
            a.amin == b.amin && a.amax == b.amax && a.wlimit == b.wlimit && a.evar == b.evar;
 }
 

@@ -69,6 +69,15 @@ def test_fp32_rhs_shock_tube_blended(flux):
     _rhs(shock_tube(nx=16, ny=2, nz=2, flux=flux), f"shock-{flux}")
 
 
+@pytest.mark.parametrize("surface,volume", [("Rusanov-HLLC", "Chandrashekhar"),
+                                            ("Rusanov-AUSM+-up", "Chandrashekhar"), ("Modified SW", "KG"),
+                                            ("KG", "KG"), ("Rusanov-KG", "KG"), ("IR", "IR")])
+def test_fp32_rhs_remaining_fluxes(surface, volume):
+    from plens_amd import periodic_cube, shock_tube, with_fluxes
+    _rhs(with_fluxes(periodic_cube(n=4, degree=3), surface=surface, volume=volume), f"cube-{surface}-{volume}")
+    _rhs(with_fluxes(shock_tube(nx=16, ny=2, nz=2), surface=surface, volume=volume), f"shock-{surface}-{volume}")
+
+
 def test_fp32_dt_matches_fp64():
     from plens_amd import wall_flow
     case = wall_flow(nx=6, ny=4, nz=2)

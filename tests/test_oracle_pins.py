@@ -192,7 +192,10 @@ def unit(rng):
     return n / np.linalg.norm(n)
 
 
-@pytest.mark.parametrize("scheme", [0, 1, 2, 3])
+ALL_SCHEMES = list(range(10))  # HLLC ... Ismail-Roe, as PLENS_FLUX_*
+
+
+@pytest.mark.parametrize("scheme", ALL_SCHEMES)
 def test_surface_flux_consistency_and_conservation(scheme):
     rng = np.random.default_rng(2 + scheme)
     ph = phys(flux=scheme)
@@ -206,7 +209,7 @@ def test_surface_flux_consistency_and_conservation(scheme):
         assert np.abs(f1 + f2).max() <= 1e-11 * (1 + np.abs(f1).max())
 
 
-@pytest.mark.parametrize("scheme", [0, 1, 2, 3])
+@pytest.mark.parametrize("scheme", ALL_SCHEMES)
 def test_surface_flux_rotational_invariance(scheme):
     rng = np.random.default_rng(10 + scheme)
     ph = phys(flux=scheme)
@@ -301,6 +304,188 @@ def test_chandrashekhar_surface_dissipates_entropy():
         for a in (0.0, 0.5, 1.0):
             fs = surf(ph, 3, a, ul, ur, n)
             assert (entropy_vars(ur) - entropy_vars(ul)) @ (fs - fec) <= 1e-12
+
+
+def vol(ph, u1, u2, n):
+    f = np.zeros(5)
+    PO.load().oracle_vol_flux(C.byref(ph), p_(u1), p_(u2), p_(n), f.ctypes.data_as(dp))
+    return f
+
+
+def ir_phys():
+    ph = phys(flux=9)
+    ph.inv_vol_flux = 2
+    return ph
+
+
+def test_ismail_roe_volume_flux_entropy_conservative():
+    """Tadmor's condition (w2-w1).F#(u1,u2) = psi2 - psi1, psi = rho u.n, for the Ismail-Roe flux
+    (its entropy is the same -rho s/(gamma-1)), plus symmetry and consistency."""
+    ph = ir_phys()
+    rng = np.random.default_rng(16)
+    for _ in range(100):
+        u1, u2, n = rand_state(rng), rand_state(rng), unit(rng)
+        f, fs = vol(ph, u1, u2, n), vol(ph, u2, u1, n)
+        assert np.abs(f - fs).max() < 1e-12 * np.abs(f).max()
+        lhs = (entropy_vars(u2) - entropy_vars(u1)) @ f
+        rhs = u2[1:4] @ n - u1[1:4] @ n
+        assert abs(lhs - rhs) < 1e-8 * (1 + abs(rhs))
+        assert np.abs(vol(ph, u1, u1, n) - euler_flux(u1, n)).max() < 1e-12 * np.abs(euler_flux(u1, n)).max()
+    # a dropped term fails it: Chandrashekhar's flux is entropy conservative too, KG's is not
+    ph.inv_vol_flux = 1
+    u1, u2, n = rand_state(rng), rand_state(rng), unit(rng)
+    lhs = (entropy_vars(u2) - entropy_vars(u1)) @ vol(ph, u1, u2, n)
+    assert abs(lhs - (u2[1:4] @ n - u1[1:4] @ n)) > 1e-6
+
+
+@pytest.mark.parametrize("N", [2, 4])
+def test_kennedy_gruber_split_form(N):
+    """With any D whose rows sum to zero, sum_j 2 D_ij F#(u_i,u_j) of a flux built from products of
+    arithmetic means equals the split form of Kennedy & Gruber (Gassner et al. 2016, eq. 3.10):
+    (a b c): 1/4 [D(abc) + a D(bc) + b D(ac) + c D(ab) + bc Da + ac Db + ab Dc], (a b):
+    1/2 [D(ab) + a Db + b Da] — with a, b, c among rho, v.n, v_d, E/rho and p."""
+    ph = phys()
+    ph.inv_vol_flux = 1
+    _, _, D = basis(N)
+    rng = np.random.default_rng(17)
+    u = [rand_state(rng) for _ in range(N + 1)]
+    n = unit(rng)
+    rho = np.array([q[0] for q in u])
+    v = np.array([q[1:4] / q[0] for q in u])
+    vn = v @ n
+    eps = np.array([q[4] / q[0] for q in u])
+    p = 0.4 * (np.array([q[4] for q in u]) - 0.5 * rho * (v * v).sum(1))
+    d = lambda x: D @ x
+
+    def tri(a, b, c):
+        return 0.25 * (d(a * b * c) + a * d(b * c) + b * d(a * c) + c * d(a * b) + b * c * d(a) + a * c * d(b)
+                       + a * b * d(c))
+
+    def duo(a, b):
+        return 0.5 * (d(a * b) + a * d(b) + b * d(a))
+
+    expect = np.zeros((N + 1, 5))
+    expect[:, 0] = duo(rho, vn)
+    for k in range(3):
+        expect[:, 1 + k] = tri(rho, vn, v[:, k]) + d(p) * n[k]
+    expect[:, 4] = tri(rho, vn, eps) + duo(vn, p)
+    got = np.zeros((N + 1, 5))
+    for i in range(N + 1):
+        for j in range(N + 1):
+            got[i] += 2 * D[i, j] * vol(ph, u[i], u[j], n)
+    assert np.abs(got - expect).max() < 1e-11 * np.abs(expect).max()
+
+
+def test_kennedy_gruber_surface_stabilisation():
+    # KG surface flux = KG two-point flux along x minus 1/2 max(|v_x| + a) (u_R - u_L)
+    ph = phys(flux=7)
+    ph.inv_vol_flux = 1
+    rng = np.random.default_rng(18)
+    x = np.array([1.0, 0, 0])
+    for _ in range(20):
+        ul, ur = rand_state(rng), rand_state(rng)
+        c = lambda u: math.sqrt(1.4 * 0.4 * (u[4] - 0.5 * u[1:4] @ u[1:4] / u[0]) / u[0]) + abs(u[1] / u[0])
+        expect = vol(ph, ul, ur, x) - 0.5 * max(c(ul), c(ur)) * (ur - ul)
+        assert np.abs(surf(ph, 7, 0, ul, ur, x) - expect).max() < 1e-12 * np.abs(expect).max()
+
+
+def cons_from_entropy(w, g=1.4):
+    # inverse of entropy_vars: -rho/p = w4, v = -w_{1..3}/w4, s from w0
+    rp = -w[4]
+    v = w[1:4] / rp
+    s = g - (g - 1) * (w[0] + 0.5 * rp * v @ v)
+    # p rho^-g = e^s and rho/p = rp  =>  rho^(1-g) = rp e^s
+    rho = (rp * math.exp(s)) ** (1 / (1 - g))
+    p = rho / rp
+    return np.array([rho, *(rho * v), p / (g - 1) + 0.5 * rho * v @ v])
+
+
+def test_ismail_roe_surface_dissipation_is_entropy_jacobian():
+    """IR surface flux = F#_x - 1/2 lambda du/dw (w_R - w_L), with du/dw the Jacobian of the
+    conservative state in entropy variables at the IR mean state (rho^, v^, p2^) and
+    lambda = |v^_x| + sqrt(gamma p2^/rho^). The mean state is read back from the two-point flux in
+    three directions; du/dw comes from central differences of the inverse entropy map."""
+    ph = ir_phys()
+    rng = np.random.default_rng(19)
+    E = np.eye(3)
+    for _ in range(20):
+        ul, ur = rand_state(rng), rand_state(rng)
+        fx, fy = vol(ph, ul, ur, E[0]), vol(ph, ul, ur, E[1])
+        fz = vol(ph, ul, ur, E[2])
+        m = np.array([fx[0], fy[0], fz[0]])          # rho^ v^
+        rh = m[0] * m[1] / fx[2]                     # rho^ v0 v1 = f_x[2]
+        vh = m / rh
+        hh = fx[4] / m[0]
+        p2 = (hh - 0.5 * vh @ vh) * rh * 0.4 / 1.4
+        uh = np.array([rh, *(rh * vh), p2 / 0.4 + 0.5 * rh * vh @ vh])
+        wh = entropy_vars(uh)
+        J = np.zeros((5, 5))
+        for k in range(5):
+            h = 1e-6 * max(1.0, abs(wh[k]))
+            dw = np.zeros(5)
+            dw[k] = h
+            J[:, k] = (cons_from_entropy(wh + dw) - cons_from_entropy(wh - dw)) / (2 * h)
+        lam = abs(vh[0]) + math.sqrt(1.4 * p2 / rh)
+        expect = fx - 0.5 * lam * J @ (entropy_vars(ur) - entropy_vars(ul))
+        got = surf(ph, 9, 0, ul, ur, E[0])
+        assert np.abs(got - expect).max() < 1e-6 * np.abs(expect).max()
+        # the dissipation removes entropy
+        assert (entropy_vars(ur) - entropy_vars(ul)) @ (got - fx) <= 1e-12
+
+
+def test_modified_steger_warming_matrix_functions():
+    """Modified Steger-Warming: F = A+(u+) u_L + A-(u-) u_R with A+- = g+-(df/du) the matrix
+    functions of the Euler flux Jacobian (numerical: central differences, numpy eigensolver)
+    with g+-(x) = (x +- sqrt(x^2 + (0.3 a)^2))/2, at the pressure-weighted states of Druguet,
+    Candler & Nompelis (2005): w = 1/2 / (1 + (1/2 dp / min p)^2), u+ = (1-w) L + w R,
+    u- = (1-w) R + w L in rho, v, p."""
+    ph = phys(flux=6)
+    rng = np.random.default_rng(20)
+    x = np.array([1.0, 0, 0])
+
+    def prim(u):
+        v = u[1:4] / u[0]
+        return u[0], v, 0.4 * (u[4] - 0.5 * u[0] * v @ v)
+
+    def matfun(u, sgn):
+        J = np.zeros((5, 5))
+        for k in range(5):
+            h = 1e-7 * max(1.0, abs(u[k]))
+            du = np.zeros(5)
+            du[k] = h
+            J[:, k] = (euler_flux(u + du, x) - euler_flux(u - du, x)) / (2 * h)
+        lam, V = np.linalg.eig(J)
+        lam = lam.real
+        rho, v, p = prim(u)
+        e = 0.3 * math.sqrt(1.4 * p / rho)
+        g = 0.5 * (lam + sgn * np.sqrt(lam * lam + e * e))
+        return (V @ np.diag(g) @ np.linalg.inv(V)).real
+
+    for _ in range(20):
+        ul, ur = rand_state(rng), rand_state(rng)
+        (rl, vl, pl), (rr, vr, pr) = prim(ul), prim(ur)
+        w = 0.5 / (1 + (0.5 * (pr - pl) / min(pl, pr)) ** 2)
+
+        def mix(a, b):
+            rho, v, p = (1 - w) * a[0] + w * b[0], (1 - w) * a[1] + w * b[1], (1 - w) * a[2] + w * b[2]
+            return np.array([rho, *(rho * v), p / 0.4 + 0.5 * rho * v @ v])
+
+        up, un = mix((rl, vl, pl), (rr, vr, pr)), mix((rr, vr, pr), (rl, vl, pl))
+        expect = matfun(up, 1) @ ul + matfun(un, -1) @ ur
+        got = surf(ph, 6, 0, ul, ur, x)
+        assert np.abs(got - expect).max() < 1e-6 * np.abs(expect).max()
+
+
+@pytest.mark.parametrize("scheme,other", [(4, 0), (5, 2), (8, 7)])
+def test_rusanov_blends(scheme, other):
+    # Rusanov gets the flux-blender weight, the other scheme its complement
+    rng = np.random.default_rng(21)
+    ph = phys()
+    for _ in range(10):
+        ul, ur, n = rand_state(rng), rand_state(rng), unit(rng)
+        a = rng.uniform(0, 1)
+        expect = a * surf(ph, 1, a, ul, ur, n) + (1 - a) * surf(ph, other, a, ul, ur, n)
+        assert np.abs(surf(ph, scheme, a, ul, ur, n) - expect).max() < 1e-13 * np.abs(expect).max()
 
 
 def test_dif_flux_closed_form():
