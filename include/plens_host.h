@@ -8,8 +8,9 @@
  * (MetricTerms::reinit, dgsem/metric_terms.cc:67-142) and FEFaceValues normals
  * (dgsem/plens.cc:1751-1756). It fills the plens_mesh_desc consumed by plens_gpu_create().
  *
- * Only structured, standard-orientation hexahedral meshes with straight-sided (trilinear,
- * MappingQ1) cells are generated: that is what the configurations need.
+ * Structured, standard-orientation hexahedral boxes with straight-sided (trilinear, MappingQ1)
+ * cells, and curved meshes of given hexahedra on the reference's manifold descriptions with a
+ * MappingQGeneric of degree q.
  *
  * Conventions (identical to the reference):
  *   - reference cell [0,1]^3, GLL nodes, lexicographic node id i + j(N+1) + k(N+1)^2;
@@ -45,6 +46,42 @@ typedef struct {
 
 /* Build a box mesh and all its arrays. */
 int plens_mesh_build_box(const plens_box_params* prm, plens_mesh** out);
+
+/* Manifold descriptions of curved meshes ("curved subtype", dgsem/plens.cc:104-130, 737-797;
+ * manifolds/cylinder.cc, manifolds/nose_cylinder.cc). */
+enum {
+    PLENS_MANIFOLD_CYLINDER = 1,       /* CylindricalManifold(axis_dir, axis_point) on everything */
+    PLENS_MANIFOLD_NOSE_CYLINDER = 2   /* SphericalManifold(nose_center) on cells whose vertex mean
+                                        * lies on the +axis_dir side of the plane through
+                                        * axis_point (the separation point), CylindricalManifold(
+                                        * axis_dir, axis_point) on the others; the nose center must
+                                        * lie on the -axis_dir side, no cell center on the plane */
+};
+typedef struct {
+    int32_t type;             /* PLENS_MANIFOLD_* */
+    double axis_dir[3];       /* "axis direction" (any length) */
+    double axis_point[3];     /* cylinder: "axis point"; nose cylinder: "separation point" */
+    double nose_center[3];    /* nose cylinder: "nose center" */
+} plens_manifold_desc;
+
+/* Build a curved mesh: hexahedra in standard orientation given by their 8 global vertex ids
+ * (deal.II vertex order), neighbours and boundary indices as for the box, a manifold description
+ * and the degree q >= 1 of the MappingQGeneric (read_mesh, dgsem/plens.cc:699-803). The mapping's
+ * support points (GLL(q) lattice of each cell) are placed as MappingQGeneric does: vertices, line
+ * points from the line's manifold with weights (1-t, t), face and cell interior points by linear
+ * transfinite (Gordon-Hall) interpolation of the perimeter through the face's / cell's manifold;
+ * a line or face shared by cells of different manifolds takes the manifold of the cell with the
+ * highest index (set_all_manifold_ids in cell order). Metric terms, node coordinates and face
+ * normals follow from the degree-q map at the GLL(N) nodes exactly as for the box. */
+int plens_mesh_build_curved(int32_t degree, int32_t mapping_degree, int32_t n_cells, int32_t n_vertices,
+                            const double* vertex_coords, const int32_t* cell_vertices,
+                            const int32_t* face_nbr, const int32_t* face_bid,
+                            const plens_manifold_desc* manifold, int32_t transpose_y_face_normals,
+                            plens_mesh** out);
+/* Mapping degree q (1 for boxes) and the support points [n_cells][(q+1)^3][3], lexicographic over
+ * the GLL(q) lattice of the reference cell (for q = 1: the 8 vertices). */
+int32_t plens_mesh_mapping_degree(const plens_mesh* m);
+const double* plens_mesh_support_points(const plens_mesh* m);
 void plens_mesh_destroy(plens_mesh* m);
 
 /* Fill a descriptor that points into the mesh's arrays (valid while the mesh lives). */

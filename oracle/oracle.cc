@@ -710,6 +710,8 @@ struct oracle {
     std::vector<double> s1, s2, s3;                     // surface terms [var][c][f][q]
     std::vector<double> ws1;                            // entropy variables of s1 (default build)
     std::vector<double> verts;                          // cell vertices [c][8][3]
+    int mq = 1;                                         // degree of the MappingQGeneric
+    std::vector<double> sup, tq;                        // support points [c][(mq+1)^3][3], GLL(mq)
     std::vector<double> rhoE_old;                       // rho E at the start of the last step
     // local time stepping ("time integration" parameters, dgsem/plens.cc:540-575, 1481-1482)
     int lts_request = 0, lts_limit = 0;
@@ -739,6 +741,43 @@ struct oracle {
         return t;
     }
 
+    // Jacobian J[a][b] = dx_a/dxi_b of cell c's map at reference point xi: trilinear from the 8
+    // vertices (MappingQGeneric(1)), else the degree-mq Lagrange interpolant of the support points
+    void jacobian(int c, const double xi[3], double J[3][3]) const {
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) J[a][b] = 0;
+        if (mq == 1) {
+            const double* cv = &verts[(size_t)c * 24];
+            for (int v = 0; v < 8; ++v) {
+                const int bb[3] = {v & 1, (v >> 1) & 1, (v >> 2) & 1};
+                for (int b = 0; b < 3; ++b) {
+                    double g = bb[b] ? 1.0 : -1.0;
+                    for (int o = 0; o < 3; ++o)
+                        if (o != b) g *= bb[o] ? xi[o] : 1.0 - xi[o];
+                    for (int a = 0; a < 3; ++a) J[a][b] += g * cv[3 * v + a];
+                }
+            }
+            return;
+        }
+        const int qp = mq + 1;
+        std::vector<double> L(3 * qp), dL(3 * qp);
+        for (int d = 0; d < 3; ++d)
+            for (int j = 0; j < qp; ++j) {
+                L[d * qp + j] = (double)lagrange_val(tq, j, xi[d]);
+                dL[d * qp + j] = lagrange_deriv(tq, j, xi[d]);
+            }
+        const double* S = &sup[(size_t)c * qp * qp * qp * 3];
+        for (int k = 0; k < qp; ++k)
+            for (int j = 0; j < qp; ++j)
+                for (int i = 0; i < qp; ++i) {
+                    const double* sp = S + 3 * ((size_t)i + qp * ((size_t)j + (size_t)qp * k));
+                    const double g[3] = {dL[i] * L[qp + j] * L[2 * qp + k], L[i] * dL[qp + j] * L[2 * qp + k],
+                                         L[i] * L[qp + j] * dL[2 * qp + k]};
+                    for (int a = 0; a < 3; ++a)
+                        for (int b = 0; b < 3; ++b) J[a][b] += g[b] * sp[a];
+                }
+    }
+
     // ---- MetricTerms::reinit, dgsem/metric_terms.cc:67-142 ----------------------------------
     void metrics(const double* V, int transpose_y) {
         Ja.assign((size_t)nc * n * 3, T1());
@@ -762,16 +801,8 @@ struct oracle {
                     for (int i = 0; i < np; ++i) {
                         const int node = tl(i, j, k);
                         const double xi[3] = {x1[i], x1[j], x1[k]};
-                        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // J[a][b] = dx_a/dxi_b
-                        for (int v = 0; v < 8; ++v) {
-                            const int bb[3] = {v & 1, (v >> 1) & 1, (v >> 2) & 1};
-                            for (int b = 0; b < 3; ++b) {
-                                double g = bb[b] ? 1.0 : -1.0;
-                                for (int o = 0; o < 3; ++o)
-                                    if (o != b) g *= bb[o] ? xi[o] : 1.0 - xi[o];
-                                for (int a = 0; a < 3; ++a) J[a][b] += g * cv[3 * v + a];
-                            }
-                        }
+                        double J[3][3];  // J[a][b] = dx_a/dxi_b
+                        jacobian(c, xi, J);
                         const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
                                            J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
                                            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
@@ -1447,7 +1478,7 @@ struct oracle {
     }
 
     // ---- PLENS::calc_ss_error, dgsem/plens.cc:3985-4035: L2 norm of (rhoE - rhoE_old)/dt_cell over
-    //      the L2 norm of rhoE, both by QGauss(N+1) through the trilinear mapping
+    //      the L2 norm of rhoE, both by QGauss(N+1) through the mapping
     double calc_ss_error() const {
         std::vector<double> xg, wg;
         gauss(np, xg, wg);
@@ -1456,21 +1487,12 @@ struct oracle {
             for (int i = 0; i < np; ++i) L[q * np + i] = (double)lagrange_val(x1, i, xg[q]);
         double num = 0, den = 0;
         for (int c = 0; c < nc; ++c) {
-            const double* cv = &verts[(size_t)c * 24];
             for (int qk = 0; qk < np; ++qk)
                 for (int qj = 0; qj < np; ++qj)
                     for (int qi = 0; qi < np; ++qi) {
                         const double xi[3] = {xg[qi], xg[qj], xg[qk]};
-                        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-                        for (int v = 0; v < 8; ++v) {
-                            const int bb[3] = {v & 1, (v >> 1) & 1, (v >> 2) & 1};
-                            for (int b = 0; b < 3; ++b) {
-                                double gv = bb[b] ? 1.0 : -1.0;
-                                for (int oo = 0; oo < 3; ++oo)
-                                    if (oo != b) gv *= bb[oo] ? xi[oo] : 1.0 - xi[oo];
-                                for (int a = 0; a < 3; ++a) J[a][b] += gv * cv[3 * v + a];
-                            }
-                        }
+                        double J[3][3];
+                        jacobian(c, xi, J);
                         const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
                                            J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
                                            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
@@ -1570,13 +1592,39 @@ int oracle_create(int32_t n_cells, int32_t degree, const double* vertices, const
             o->Qm[r * np + c] = o->w[r] * o->D[r * np + c];
         }
     cob_matrix(degree, o->C);
-    o->metrics(vertices, transpose_y);
     o->verts.assign(vertices, vertices + (size_t)n_cells * 24);
+    o->metrics(vertices, transpose_y);
     const size_t nd = o->ndof;
     o->Q.assign(5 * nd, 0); o->avars.assign(9 * nd, 0); o->mu.assign(nd, 0); o->kap.assign(nd, 0);
     o->alpha.assign(n_cells, 0); o->alpha_d.assign(n_cells, 0);
     o->rhs.assign(5 * nd, 0); o->rhs1.assign(5 * nd, 0); o->rhs2.assign(5 * nd, 0);
     *out = o;
+    return 0;
+}
+
+int oracle_create_mapped(int32_t n_cells, int32_t degree, int32_t map_degree, const double* support,
+                         const int32_t* face_nbr, const int32_t* face_bid, const oracle_physics* phys,
+                         const oracle_bc* bcs, int32_t n_bcs, int32_t transpose_y, oracle** out) {
+    if (map_degree < 1 || map_degree > 8 || !support || n_cells < 1) return -1;
+    const int qp = map_degree + 1, nsp = qp * qp * qp;
+    std::vector<double> corners((size_t)n_cells * 24);
+    for (int c = 0; c < n_cells; ++c)
+        for (int v = 0; v < 8; ++v) {
+            const int s = (v & 1) * map_degree + qp * (((v >> 1) & 1) * map_degree + qp * (((v >> 2) & 1) * map_degree));
+            for (int d = 0; d < 3; ++d)
+                corners[(size_t)c * 24 + 3 * v + d] = support[((size_t)c * nsp + s) * 3 + d];
+        }
+    if (map_degree == 1)
+        return oracle_create(n_cells, degree, corners.data(), face_nbr, face_bid, phys, bcs, n_bcs, transpose_y, out);
+    // build on the corners, then redo the metric terms from the degree-q map
+    int e = oracle_create(n_cells, degree, corners.data(), face_nbr, face_bid, phys, bcs, n_bcs, transpose_y, out);
+    if (e) return e;
+    oracle* o = *out;
+    o->mq = map_degree;
+    o->sup.assign(support, support + (size_t)n_cells * nsp * 3);
+    std::vector<double> wq;
+    gll(map_degree, o->tq, wq);
+    o->metrics(corners.data(), transpose_y);
     return 0;
 }
 

@@ -44,6 +44,11 @@ typedef struct { int32_t type; double p, T, vel[3]; } oracle_bc; /* type as PLEN
 int oracle_create(int32_t n_cells, int32_t degree, const double* vertices, const int32_t* face_nbr,
                   const int32_t* face_bid, const oracle_physics* phys, const oracle_bc* bcs,
                   int32_t n_bcs, int32_t transpose_y_face_normals, oracle** out);
+/* The same for a MappingQGeneric of degree map_degree given by its support points
+ * [n_cells][(q+1)^3][3] (lexicographic over the GLL(q) lattice; the corners are the vertices). */
+int oracle_create_mapped(int32_t n_cells, int32_t degree, int32_t map_degree, const double* support,
+                         const int32_t* face_nbr, const int32_t* face_bid, const oracle_physics* phys,
+                         const oracle_bc* bcs, int32_t n_bcs, int32_t transpose_y, oracle** out);
 void oracle_destroy(oracle* o);
 int oracle_set_threads(int32_t n);
 int oracle_set_state(oracle* o, const double* Q);

@@ -42,6 +42,8 @@ def load():
     sig = {
         "oracle_create": (C.c_int, [C.c_int32, C.c_int32, _dp, ip, ip, C.POINTER(OPhys), C.POINTER(OBC), C.c_int32,
                                     C.c_int32, C.POINTER(vp)]),
+        "oracle_create_mapped": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _dp, ip, ip, C.POINTER(OPhys),
+                                           C.POINTER(OBC), C.c_int32, C.c_int32, C.POINTER(vp)]),
         "oracle_destroy": (None, [vp]),
         "oracle_set_threads": (C.c_int, [C.c_int32]),
         "oracle_set_state": (C.c_int, [vp, _dp]),
@@ -109,9 +111,18 @@ class OraclePlens:
         self._bid = np.ascontiguousarray(mesh.face_bid, dtype=np.int32)
         self._h = C.c_void_p()
         ip = C.POINTER(C.c_int32)
-        rc = self.lib.oracle_create(mesh.n_cells, mesh.degree, self._V.ctypes.data_as(_dp),
-                                    self._nbr.ctypes.data_as(ip), self._bid.ctypes.data_as(ip), C.byref(self._ph),
-                                    arr, len(bcs), int(mesh.transpose_y_face_normals), C.byref(self._h))
+        q = getattr(mesh, "mapping_degree", 1)
+        if q > 1:
+            self._S = np.ascontiguousarray(mesh.support_points, dtype=np.float64)
+            rc = self.lib.oracle_create_mapped(mesh.n_cells, mesh.degree, q, self._S.ctypes.data_as(_dp),
+                                               self._nbr.ctypes.data_as(ip), self._bid.ctypes.data_as(ip),
+                                               C.byref(self._ph), arr, len(bcs), int(mesh.transpose_y_face_normals),
+                                               C.byref(self._h))
+        else:
+            rc = self.lib.oracle_create(mesh.n_cells, mesh.degree, self._V.ctypes.data_as(_dp),
+                                        self._nbr.ctypes.data_as(ip), self._bid.ctypes.data_as(ip),
+                                        C.byref(self._ph), arr, len(bcs), int(mesh.transpose_y_face_normals),
+                                        C.byref(self._h))
         if rc:
             raise RuntimeError(f"oracle_create failed {rc}")
         if case is not None:

@@ -57,6 +57,14 @@ class BoxParams(C.Structure):
                 ("transpose_y_face_normals", C.c_int32)]
 
 
+class ManifoldDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("axis_dir", C.c_double * 3), ("axis_point", C.c_double * 3),
+                ("nose_center", C.c_double * 3)]
+
+
+MANIFOLD_CYLINDER, MANIFOLD_NOSE_CYLINDER = 1, 2
+
+
 # exported symbols (checked by the CPU tests against include/*.h)
 GPU_SYMBOLS = [
     "plens_gpu_create", "plens_gpu_destroy", "plens_gpu_set_state", "plens_gpu_get_state",
@@ -73,7 +81,8 @@ HOST_SYMBOLS = [
     "plens_mesh_degree", "plens_mesh_vertices", "plens_mesh_node_coords", "plens_mesh_Ja",
     "plens_mesh_detJ", "plens_mesh_face_normals", "plens_mesh_face_nbr", "plens_mesh_face_bid",
     "plens_mesh_min_vertex_dist", "plens_basis_1d", "plens_prim_to_cons", "plens_ic_density_wave",
-    "plens_ic_normal_shock", "plens_ic_uniform", "plens_face_node",
+    "plens_ic_normal_shock", "plens_ic_uniform", "plens_face_node", "plens_mesh_build_curved",
+    "plens_mesh_mapping_degree", "plens_mesh_support_points",
 ]
 MLP_SYMBOLS = [
     "plens_mlp_n_params", "plens_mlp_create", "plens_mlp_destroy", "plens_mlp_get_params",
@@ -135,6 +144,10 @@ def load():
         "plens_mesh_face_nbr": (_ip, [vp]),
         "plens_mesh_face_bid": (_ip, [vp]),
         "plens_mesh_min_vertex_dist": (_dp, [vp]),
+        "plens_mesh_build_curved": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, _dp, _ip, _ip, _ip,
+                                              C.POINTER(ManifoldDesc), C.c_int32, C.POINTER(vp)]),
+        "plens_mesh_mapping_degree": (C.c_int32, [vp]),
+        "plens_mesh_support_points": (_dp, [vp]),
         "plens_basis_1d": (C.c_int, [C.c_int32, _dp, _dp, _dp, _dp]),
         "plens_prim_to_cons": (None, [C.c_double, C.c_double, _dp, C.c_double, _dp]),
         "plens_ic_density_wave": (C.c_int, [vp, C.c_double, C.c_double, _dp, C.c_double, C.c_double,

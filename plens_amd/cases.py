@@ -19,11 +19,20 @@ SUTH_RATIO = 110.4 / 273.15  # Sutherland S/T0 of air, used for all nondimension
 
 
 class Mesh:
-    """Owning wrapper of a host plens_mesh (structured box)."""
+    """Owning wrapper of a host plens_mesh: a structured box (constructor) or a curved mesh of given
+    hexahedra on a manifold description (Mesh.curved)."""
 
     def __init__(self, degree, nx, ny, nz, lo, hi, periodic, side_bid, periodic_bid,
-                 y_stretch_ratio=1.0, perturb_frac=0.0, seed=0, transpose_y_face_normals=True):
+                 y_stretch_ratio=1.0, perturb_frac=0.0, seed=0, transpose_y_face_normals=True, _curved=None):
         lib = L.load()
+        if _curved is not None:
+            self._h = _curved
+            self.degree = degree
+            self.n_cells = nx
+            self.shape = (nx, 1, 1)
+            self.transpose_y_face_normals = bool(transpose_y_face_normals)
+            self._load(lib)
+            return
         p = L.BoxParams()
         p.degree, p.nx, p.ny, p.nz = degree, nx, ny, nz
         for d in range(3):
@@ -42,10 +51,34 @@ class Mesh:
         self.degree = degree
         self.n_cells = nx * ny * nz
         self.shape = (nx, ny, nz)
+        self.transpose_y_face_normals = bool(transpose_y_face_normals)
+        self._load(lib)
+
+    @classmethod
+    def curved(cls, degree, mapping_degree, vertices, cells, face_nbr, face_bid, manifold,
+               transpose_y_face_normals=True):
+        """Curved mesh (plens_mesh_build_curved): vertices [n_v][3], cells [n_c][8] global vertex
+        ids in deal.II order, face_nbr / face_bid [n_c][6], manifold an L.ManifoldDesc."""
+        lib = L.load()
+        X = np.ascontiguousarray(vertices, dtype=np.float64)
+        cv = np.ascontiguousarray(cells, dtype=np.int32)
+        nb = np.ascontiguousarray(face_nbr, dtype=np.int32)
+        bd = np.ascontiguousarray(face_bid, dtype=np.int32)
+        h = C.c_void_p()
+        ip = C.POINTER(C.c_int32)
+        L.check(lib.plens_mesh_build_curved(degree, mapping_degree, cv.shape[0], X.shape[0],
+                                            X.ctypes.data_as(C.POINTER(C.c_double)), cv.ctypes.data_as(ip),
+                                            nb.ctypes.data_as(ip), bd.ctypes.data_as(ip), C.byref(manifold),
+                                            int(transpose_y_face_normals), C.byref(h)), "plens_mesh_build_curved")
+        return cls(degree, cv.shape[0], 1, 1, None, None, None, None, None,
+                   transpose_y_face_normals=transpose_y_face_normals, _curved=h)
+
+    def _load(self, lib):
+        degree = self.degree
         self.n = (degree + 1) ** 3
         self.nf = (degree + 1) ** 2
         self.n_dof = self.n_cells * self.n
-        self.transpose_y_face_normals = bool(transpose_y_face_normals)
+        self.mapping_degree = int(lib.plens_mesh_mapping_degree(self._h))
 
         def arr(fn, shape, dtype=np.float64):
             ptr = fn(self._h)
@@ -60,6 +93,8 @@ class Mesh:
         self.face_nbr = arr(lib.plens_mesh_face_nbr, (nc, 6), np.int32)
         self.face_bid = arr(lib.plens_mesh_face_bid, (nc, 6), np.int32)
         self.min_vertex_dist = arr(lib.plens_mesh_min_vertex_dist, (nc,))
+        qp = self.mapping_degree + 1
+        self.support_points = arr(lib.plens_mesh_support_points, (nc, qp ** 3, 3))
 
     @property
     def handle(self):
@@ -79,7 +114,8 @@ class Mesh:
         d.min_vertex_dist = self.min_vertex_dist.ctypes.data_as(dp)
         d.face_nbr = self.face_nbr.ctypes.data_as(ip)
         d.face_bid = self.face_bid.ctypes.data_as(ip)
-        d.vertices = self.vertices.ctypes.data_as(dp) if with_vertices else None
+        # a curved (q > 1) map is not trilinear: its metric terms are always handed over stored
+        d.vertices = self.vertices.ctypes.data_as(dp) if (with_vertices and self.mapping_degree == 1) else None
         return d
 
     def close(self):
@@ -201,6 +237,154 @@ def wall_flow(degree=4, nx=128, ny=64, nz=2, seed=2, flux="HLLC", mach=6.0, T_in
             "ic")
     return Case("wall_flow", mesh, ph, bcs, Q, 0.4, steps,
                 {"seed": seed, "cells": (nx, ny, nz), "degree": degree, "u_inf": u_inf})
+
+
+def _structured_cells(X, side_bid):
+    """Cells, neighbours and boundary indices of a structured lattice of vertex positions
+    X[k][j][i][3] (i fastest): cell (i, j, k) -> index i + n0 (j + n1 k), reference directions
+    (xi_0, xi_1, xi_2) along (i, j, k); side_bid[f] for the six sides."""
+    n2, n1, n0 = X.shape[0] - 1, X.shape[1] - 1, X.shape[2] - 1
+    vid = lambda i, j, k: i + (n0 + 1) * (j + (n1 + 1) * k)
+    nc = n0 * n1 * n2
+    cells = np.zeros((nc, 8), dtype=np.int32)
+    nbr = -np.ones((nc, 6), dtype=np.int32)
+    bid = -np.ones((nc, 6), dtype=np.int32)
+    dims = (n0, n1, n2)
+    for k in range(n2):
+        for j in range(n1):
+            for i in range(n0):
+                c = i + n0 * (j + n1 * k)
+                for v in range(8):
+                    cells[c, v] = vid(i + (v & 1), j + ((v >> 1) & 1), k + ((v >> 2) & 1))
+                for f in range(6):
+                    I = [i, j, k]
+                    d, side = f // 2, f % 2
+                    I[d] += 1 if side else -1
+                    if 0 <= I[d] < dims[d]:
+                        nbr[c, f] = I[0] + n0 * (I[1] + n1 * I[2])
+                    else:
+                        bid[c, f] = side_bid[f]
+    return X.reshape(-1, 3), cells, nbr, bid
+
+
+def _smooth_state(mesh, g, rho0, vel, p0, amp, wave):
+    """rho = rho0 (1 + amp sin(wave . x)), uniform velocity and pressure, at the mesh's nodes."""
+    x = mesh.node_coords  # [nc][3][n]
+    rho = rho0 * (1.0 + amp * np.sin(np.einsum("d,cdn->cn", np.asarray(wave, dtype=np.float64), x))).reshape(-1)
+    v = np.asarray(vel, dtype=np.float64)
+    Q = np.empty((5, mesh.n_dof))
+    Q[0] = rho
+    for d in range(3):
+        Q[1 + d] = rho * v[d]
+    Q[4] = p0 / (g - 1) + 0.5 * rho * (v @ v)
+    return Q
+
+
+def cylinder_flare(degree=3, mapping_degree=2, n_axial=12, n_normal=4, n_azim=2, mach=3.0, reynolds=1e4,
+                   radius=1.0, thickness=0.3, length=2.0, flare_start=1.0, flare_deg=15.0, wedge_deg=10.0,
+                   flux="HLLC", wall=True, steps=20, transpose_y_face_normals=True, perturb=0.0, seed=3):
+    """Curved mesh on the "cylinder" manifold description (axis x through the origin): an
+    azimuthal wedge of the layer over a cylinder of the given radius that turns into a conical
+    flare at x = flare_start; layer thickness `thickness` in r, wedge angle `wedge_deg`, MappingQ of
+    degree `mapping_degree`. Mach-`mach` uniform inflow (rho = 1, p = 1 / gamma) at x = 0, outflow
+    at x = length and at the layer's outer surface, isothermal wall (T = 1.5 T_inf; `wall=False`:
+    symmetry) on the body, symmetry on both wedge planes; the initial state adds a 5% smooth density
+    wave to the free stream. (The wall ghost state 2 u_wall - u has density 2 p / (R T_w) - rho,
+    which vanishes for the free stream at T_w = 2 T_inf, and the stage-3 viscous flux divides by it.)
+    `perturb` > 0 moves the interior vertices by seeded smooth displacements of that fraction of the
+    local spacing in r, phi and x."""
+    g = 1.4
+    p_inf, T_inf = 1.0 / g, 1.0 / g
+    u_inf = mach
+    xs = np.linspace(0.0, length, n_axial + 1)
+    rb = radius + np.maximum(xs - flare_start, 0.0) * math.tan(math.radians(flare_deg))
+    X = np.zeros((n_azim + 1, n_normal + 1, n_axial + 1, 3))
+    rng = np.random.default_rng(seed)
+    kw = rng.integers(1, 4, size=(3, 3))
+    phase = rng.uniform(0, 2 * math.pi, 3)
+    for k in range(n_azim + 1):
+        for j in range(n_normal + 1):
+            for i in range(n_axial + 1):
+                r = rb[i] + thickness * j / n_normal
+                ph = math.radians(wedge_deg) * k / n_azim
+                x = xs[i]
+                inner = 0 < i < n_axial and 0 < j < n_normal and 0 < k < n_azim
+                if perturb > 0 and inner:
+                    arg = 2 * math.pi * (kw @ np.array([i / n_axial, j / n_normal, k / n_azim])) + phase
+                    r += perturb * thickness / n_normal * math.sin(arg[0])
+                    ph += perturb * math.radians(wedge_deg) / n_azim * math.sin(arg[1])
+                    x += perturb * length / n_axial * math.sin(arg[2])
+                X[k, j, i] = (x, r * math.cos(ph), r * math.sin(ph))
+    V, cells, nbr, bid = _structured_cells(X, [0, 1, 2, 1, 3, 3])
+    md = L.ManifoldDesc()
+    md.type = L.MANIFOLD_CYLINDER
+    md.axis_dir[:] = [1.0, 0.0, 0.0]
+    md.axis_point[:] = [0.0, 0.0, 0.0]
+    mesh = Mesh.curved(degree, mapping_degree, V, cells, nbr, bid, md,
+                       transpose_y_face_normals=transpose_y_face_normals)
+    ph = make_physics(mu0=u_inf / reynolds, T0=T_inf, S=SUTH_RATIO * T_inf, flux=flux, wall_blender_limit=0.0)
+    body = make_bc(L.BC_UNIFORM_TEMP_WALL, T=1.5 * T_inf, vel=(0, 0, 0)) if wall else make_bc(L.BC_SYMMETRY)
+    bcs = [make_bc(L.BC_UNIFORM_INFLOW, p=p_inf, T=T_inf, vel=(u_inf, 0, 0)), make_bc(L.BC_OUTFLOW, p=p_inf),
+           body, make_bc(L.BC_SYMMETRY)]
+    Q = _smooth_state(mesh, g, 1.0, (u_inf, 0.0, 0.0), p_inf, 0.05, (2 * math.pi / length, 0.0, 0.0))
+    return Case("cylinder_flare", mesh, ph, bcs, Q, 0.4, steps,
+                {"degree": degree, "mapping_degree": mapping_degree, "cells": (n_axial, n_normal, n_azim)})
+
+
+def nose_cone(degree=3, mapping_degree=2, n_sphere=4, n_cone=6, n_normal=3, n_azim=2, mach=3.0, reynolds=1e4,
+              radius=1.0, thickness=0.25, cone_length=1.5, cone_deg=20.0, theta0_deg=30.0, wedge_deg=10.0,
+              flux="HLLC", wall=True, steps=20, transpose_y_face_normals=True):
+    """Curved mesh on the "nose cylinder" manifold description: an azimuthal wedge of the layer over
+    a blunted cone (spherical nose of the given radius about the origin, axis pointing to -x out of
+    the nose, cone of half-angle `cone_deg` tangent to the sphere, separation plane through the
+    tangency circle). Along the body the mesh runs from polar angle `theta0_deg` (the tip is left
+    out) over the sphere (n_sphere cells), then `cone_length` along the cone (n_cone cells); spherical
+    manifold on the nose cells, cylindrical on the cone cells. Free stream Mach `mach` along +x
+    (rho = 1, p = 1 / gamma) prescribed on the outer surface and at the cut near the tip, outflow at
+    the cone's end, isothermal wall at 1.5 T_inf (symmetry with `wall=False`) on the body, symmetry
+    on both wedge planes; a 5% smooth density wave is added."""
+    g = 1.4
+    p_inf, T_inf = 1.0 / g, 1.0 / g
+    u_inf = mach
+    a = np.array([-1.0, 0.0, 0.0])
+    tt = math.radians(90.0 - cone_deg)
+    th = np.linspace(math.radians(theta0_deg), tt, n_sphere + 1)
+    ls = np.linspace(0.0, cone_length, n_cone + 1)[1:]
+    tcone = -math.cos(math.radians(cone_deg)) * a
+    n0 = n_sphere + n_cone
+    X = np.zeros((n_azim + 1, n_normal + 1, n0 + 1, 3))
+    for k in range(n_azim + 1):
+        ph = math.radians(wedge_deg) * k / n_azim
+        er = np.array([0.0, math.cos(ph), math.sin(ph)])
+        t_dir = tcone + math.sin(math.radians(cone_deg)) * er
+        for j in range(n_normal + 1):
+            rr = radius + thickness * j / n_normal
+            for i, t in enumerate(th):
+                X[k, j, i] = rr * (math.cos(t) * a + math.sin(t) * er)
+            base = rr * (math.cos(tt) * a + math.sin(tt) * er)
+            for i, l in enumerate(ls):
+                X[k, j, n_sphere + 1 + i] = base + l * t_dir
+    V, cells, nbr, bid = _structured_cells(X, [0, 1, 2, 0, 3, 3])
+    md = L.ManifoldDesc()
+    md.type = L.MANIFOLD_NOSE_CYLINDER
+    md.axis_dir[:] = list(a)
+    md.axis_point[:] = list(radius * math.cos(tt) * a)
+    md.nose_center[:] = [0.0, 0.0, 0.0]
+    # the separation plane must put exactly the nose cells on the spherical side
+    ctr = V[cells].mean(axis=1)
+    sph = (ctr - np.array(md.axis_point[:])) @ a > 0
+    expect = (np.arange(len(cells)) % n0) < n_sphere
+    if not np.array_equal(sph, expect):
+        raise ValueError("nose_cone: cone too short or layer too thick for the separation plane")
+    mesh = Mesh.curved(degree, mapping_degree, V, cells, nbr, bid, md,
+                       transpose_y_face_normals=transpose_y_face_normals)
+    ph = make_physics(mu0=u_inf / reynolds, T0=T_inf, S=SUTH_RATIO * T_inf, flux=flux, wall_blender_limit=0.0)
+    body = make_bc(L.BC_UNIFORM_TEMP_WALL, T=1.5 * T_inf, vel=(0, 0, 0)) if wall else make_bc(L.BC_SYMMETRY)
+    bcs = [make_bc(L.BC_UNIFORM_INFLOW, p=p_inf, T=T_inf, vel=(u_inf, 0, 0)), make_bc(L.BC_OUTFLOW, p=p_inf),
+           body, make_bc(L.BC_SYMMETRY)]
+    Q = _smooth_state(mesh, g, 1.0, (u_inf, 0.0, 0.0), p_inf, 0.05, (2.0, 1.0, 0.5))
+    return Case("nose_cone", mesh, ph, bcs, Q, 0.4, steps,
+                {"degree": degree, "mapping_degree": mapping_degree, "cells": (n0, n_normal, n_azim)})
 
 
 def batch_instance(i, seed0=100, min_n=4, max_n=32, flux="HLLC"):

@@ -4,7 +4,9 @@
 #include "plens_host.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
+#include <map>
 #include <cstring>
 #include <new>
 #include <random>
@@ -140,7 +142,9 @@ extern "C" void plens_prim_to_cons(double gamma, double rho, const double vel[3]
 struct plens_mesh {
     int32_t N = 0, n = 0, nf = 0, n_cells = 0;
     int32_t nx = 0, ny = 0, nz = 0;
+    int32_t q = 1;                 // mapping degree
     std::vector<double> vertices, nodes, Ja, detJ, normals, hmin;
+    std::vector<double> support;   // [n_cells][(q+1)^3][3]
     std::vector<int32_t> face_nbr, face_bid;
     std::vector<double> x1d, w1d;
 };
@@ -358,9 +362,440 @@ extern "C" int plens_mesh_build_box(const plens_box_params* p, plens_mesh** out)
                     (void)I;
                 }
             }
+    m->q = 1;
+    m->support = m->vertices;
     *out = m;
     return PLENS_OK;
 }
+
+// ---------------------------------------------------------------------------------------------
+// curved meshes: manifolds and MappingQGeneric support points
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+struct P3 {
+    double v[3] = {0, 0, 0};
+    double& operator[](int i) { return v[i]; }
+    double operator[](int i) const { return v[i]; }
+};
+inline P3 mk(double a, double b, double c) { P3 p; p[0] = a; p[1] = b; p[2] = c; return p; }
+inline double dot3(const P3& a, const P3& b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline P3 cross3(const P3& a, const P3& b) {
+    return mk(a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]);
+}
+inline P3 axpy(double a, const P3& x, const P3& y) { return mk(a * x[0] + y[0], a * x[1] + y[1], a * x[2] + y[2]); }
+inline P3 sub3(const P3& a, const P3& b) { return mk(a[0] - b[0], a[1] - b[1], a[2] - b[2]); }
+inline double norm3(const P3& a) { return std::sqrt(dot3(a, a)); }
+
+// One manifold: CylindricalManifold (chart r, phi, z about a unit axis; phi measured from a unit
+// normal of the axis) or SphericalManifold (center). deal.II's grid/manifold_lib.cc documents both.
+struct Manifold {
+    int kind = 0;          // 1 cylinder, 2 sphere
+    P3 dir, pt, nrm, dxn;  // cylinder
+    P3 c;                  // sphere
+
+    P3 pull_back(const P3& x) const {  // cylinder chart (r, phi, z)
+        const P3 d = sub3(x, pt);
+        const double lam = dot3(d, dir);
+        const P3 pd = axpy(-lam, dir, d);
+        const double phi = std::atan2(dot3(pd, dxn), dot3(pd, nrm));
+        return mk(norm3(pd), phi, lam);
+    }
+    P3 push_forward(const P3& ch) const {
+        const double cr = std::cos(ch[1]) * ch[0], sr = std::sin(ch[1]) * ch[0];
+        P3 x = axpy(ch[2], dir, pt);
+        x = axpy(cr, nrm, x);
+        return axpy(sr, dxn, x);
+    }
+    // new point from surrounding points and weights (rows of a weight table share the points)
+    P3 new_point(const std::vector<P3>& pts, const double* w) const {
+        const size_t np = pts.size();
+        if (kind == 1) {
+            // ChartManifold: weighted mean in the chart, phi with period 2 pi handled as the
+            // periodic FlatManifold does (shift by -2 pi what lies more than pi above the minimum)
+            std::vector<P3> ch(np);
+            double pmin = 1e300;
+            for (size_t i = 0; i < np; ++i) {
+                ch[i] = pull_back(pts[i]);
+                pmin = std::min(pmin, ch[i][1]);
+            }
+            P3 a;
+            for (size_t i = 0; i < np; ++i) {
+                P3 q = ch[i];
+                if (q[1] - pmin > M_PI) q[1] -= 2.0 * M_PI;
+                for (int d = 0; d < 3; ++d) a[d] += w[i] * q[d];
+            }
+            return push_forward(a);
+        }
+        // sphere: radius = weighted mean distance; direction = normalised weighted mean of the unit
+        // directions when all directions lie close together (squared chord < 2e-2), else the
+        // stationary point of sum_i w_i theta_i^2 on the unit sphere (weighted spherical mean,
+        // Buss & Fillmore 2001), iterated from that guess
+        std::vector<P3> dirs(np);
+        std::vector<double> dist(np);
+        double maxd = 0;
+        for (size_t i = 0; i < np; ++i) {
+            dirs[i] = sub3(pts[i], c);
+            dist[i] = norm3(dirs[i]);
+            for (int d = 0; d < 3; ++d) dirs[i][d] /= dist[i];
+            for (size_t k = 0; k < i; ++k) {
+                const P3 t = sub3(dirs[i], dirs[k]);
+                maxd = std::max(maxd, dot3(t, t));
+            }
+        }
+        double rho = 0;
+        P3 cand;
+        for (size_t i = 0; i < np; ++i) {
+            rho += w[i] * dist[i];
+            for (int d = 0; d < 3; ++d) cand[d] += w[i] * dirs[i][d];
+        }
+        const double cn = norm3(cand);
+        if (cn == 0.0) return c;
+        for (int d = 0; d < 3; ++d) cand[d] /= cn;
+        if (maxd >= 2e-2) {
+            for (int it = 0; it < 100; ++it) {
+                P3 g;  // sum_i w_i log_cand(d_i)
+                for (size_t i = 0; i < np; ++i) {
+                    if (w[i] == 0.0) continue;
+                    const double ct = std::max(-1.0, std::min(1.0, dot3(cand, dirs[i])));
+                    const P3 t = axpy(-ct, cand, dirs[i]);
+                    const double tn = norm3(t);
+                    if (tn < 1e-300) continue;
+                    const double th = std::atan2(tn, ct);
+                    for (int d = 0; d < 3; ++d) g[d] += w[i] * th * t[d] / tn;
+                }
+                const double gn = norm3(g);
+                if (gn < 1e-16) break;
+                P3 nc;
+                for (int d = 0; d < 3; ++d) nc[d] = std::cos(gn) * cand[d] + std::sin(gn) * g[d] / gn;
+                const double nn = norm3(nc);
+                for (int d = 0; d < 3; ++d) cand[d] = nc[d] / nn;
+            }
+        }
+        return axpy(rho, cand, c);
+    }
+};
+
+Manifold make_cylinder(const double* axis, const double* point) {
+    Manifold m;
+    m.kind = 1;
+    const double an = std::sqrt(axis[0] * axis[0] + axis[1] * axis[1] + axis[2] * axis[2]);
+    m.dir = mk(axis[0] / an, axis[1] / an, axis[2] / an);
+    m.pt = mk(point[0], point[1], point[2]);
+    // unit normal of the axis: cross with the coordinate axis least aligned with it
+    int k = 0;
+    for (int d = 1; d < 3; ++d)
+        if (std::fabs(m.dir[d]) < std::fabs(m.dir[k])) k = d;
+    P3 e;
+    e[k] = 1;
+    m.nrm = cross3(m.dir, e);
+    const double nn = norm3(m.nrm);
+    for (int d = 0; d < 3; ++d) m.nrm[d] /= nn;
+    m.dxn = cross3(m.dir, m.nrm);
+    return m;
+}
+Manifold make_sphere(const double* center) {
+    Manifold m;
+    m.kind = 2;
+    m.c = mk(center[0], center[1], center[2]);
+    return m;
+}
+
+// Lagrange basis values and derivatives of the nodes xn[0..q] at x
+void lagrange_all(int q, const double* xn, double x, double* L, double* dL) {
+    for (int j = 0; j <= q; ++j) {
+        double v = 1, dv = 0;
+        for (int k = 0; k <= q; ++k) {
+            if (k == j) continue;
+            const double den = xn[j] - xn[k];
+            dv = dv * (x - xn[k]) / den + v / den;
+            v *= (x - xn[k]) / den;
+        }
+        L[j] = v;
+        if (dL) dL[j] = dv;
+    }
+}
+
+// Support points of one cell (lexicographic over the GLL(q) lattice t[0..q]).
+void cell_support_points(int q, const double* t, const P3 V[8], const Manifold* line_m[12],
+                         const Manifold* face_m[6], const Manifold& cell_m, std::vector<P3>& S) {
+    const int qp = q + 1;
+    S.assign((size_t)qp * qp * qp, P3());
+    auto at = [&](int i, int j, int k) -> P3& { return S[(size_t)i + qp * ((size_t)j + (size_t)qp * k)]; };
+    for (int v = 0; v < 8; ++v) at((v & 1) * q, ((v >> 1) & 1) * q, ((v >> 2) & 1) * q) = V[v];
+    if (q == 1) return;
+    // lines: direction d, the other two lattice indices at 0 or q; deal.II line order is
+    // irrelevant here, each line is identified by (d, a, b)
+    int li = 0;
+    for (int d = 0; d < 3; ++d) {
+        const int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+        for (int b = 0; b < 2; ++b)
+            for (int a = 0; a < 2; ++a, ++li) {
+                int I0[3], I1[3];
+                I0[d] = 0; I1[d] = q;
+                I0[d1] = I1[d1] = a * q;
+                I0[d2] = I1[d2] = b * q;
+                const std::vector<P3> ends = {at(I0[0], I0[1], I0[2]), at(I1[0], I1[1], I1[2])};
+                for (int m = 1; m < q; ++m) {
+                    const double w[2] = {1.0 - t[m], t[m]};
+                    int I[3] = {I0[0], I0[1], I0[2]};
+                    I[d] = m;
+                    at(I[0], I[1], I[2]) = line_m[li]->new_point(ends, w);
+                }
+            }
+    }
+    // faces: normal direction d at index e*q; face coordinates (u, v) = the other directions in
+    // increasing order; perimeter = 4 vertices then the 4 lines' interior points; weights of the
+    // 2D transfinite interpolation
+    for (int f = 0; f < 6; ++f) {
+        const int d = f / 2, e = f % 2;
+        const int du = d == 0 ? 1 : 0, dvv = d == 2 ? 1 : 2;
+        auto idx = [&](int u, int v, int* I) { I[d] = e * q; I[du] = u; I[dvv] = v; };
+        std::vector<P3> per;
+        std::vector<std::pair<int, int>> pid;  // lattice (u, v) of each perimeter point
+        for (int v = 0; v < 2; ++v)
+            for (int u = 0; u < 2; ++u) pid.push_back({u * q, v * q});
+        for (int u = 0; u < 2; ++u)  // lines u = 0, u = q (along v)
+            for (int m = 1; m < q; ++m) pid.push_back({u * q, m});
+        for (int v = 0; v < 2; ++v)  // lines v = 0, v = q (along u)
+            for (int m = 1; m < q; ++m) pid.push_back({m, v * q});
+        for (auto& pr : pid) {
+            int I[3];
+            idx(pr.first, pr.second, I);
+            per.push_back(at(I[0], I[1], I[2]));
+        }
+        std::vector<double> w(per.size());
+        for (int mv = 1; mv < q; ++mv)
+            for (int mu = 1; mu < q; ++mu) {
+                const double su = t[mu], sv = t[mv];
+                for (size_t i = 0; i < pid.size(); ++i) {
+                    const int pu = pid[i].first, pv = pid[i].second;
+                    const bool uc = pu == 0 || pu == q, vc = pv == 0 || pv == q;
+                    double wi = 0;
+                    if (uc && vc) {  // vertex: -bilinear
+                        wi = -((pu ? su : 1 - su) * (pv ? sv : 1 - sv));
+                    } else if (uc && pv == mv) {  // line u = const at the point's v
+                        wi = pu ? su : 1 - su;
+                    } else if (vc && pu == mu) {  // line v = const at the point's u
+                        wi = pv ? sv : 1 - sv;
+                    }
+                    w[i] = wi;
+                }
+                int I[3];
+                idx(mu, mv, I);
+                at(I[0], I[1], I[2]) = face_m[f]->new_point(per, w.data());
+            }
+    }
+    // interior: 3D transfinite interpolation, faces + , lines - , vertices +
+    std::vector<P3> per;
+    std::vector<std::array<int, 3>> pid;
+    for (int k = 0; k <= q; ++k)
+        for (int j = 0; j <= q; ++j)
+            for (int i = 0; i <= q; ++i) {
+                const int nb = (i == 0 || i == q) + (j == 0 || j == q) + (k == 0 || k == q);
+                if (nb == 0) continue;
+                pid.push_back({i, j, k});
+                per.push_back(at(i, j, k));
+            }
+    std::vector<double> w(per.size());
+    for (int k = 1; k < q; ++k)
+        for (int j = 1; j < q; ++j)
+            for (int i = 1; i < q; ++i) {
+                const int me[3] = {i, j, k};
+                for (size_t p = 0; p < pid.size(); ++p) {
+                    double wi = 1;
+                    int nb = 0;
+                    bool match = true;
+                    for (int d = 0; d < 3; ++d) {
+                        const int c = pid[p][d];
+                        if (c == 0 || c == q) {
+                            ++nb;
+                            wi *= c ? t[me[d]] : 1 - t[me[d]];
+                        } else if (c != me[d]) {
+                            match = false;
+                        }
+                    }
+                    w[p] = match ? (nb == 2 ? -wi : wi) : 0.0;
+                }
+                at(i, j, k) = cell_m.new_point(per, w.data());
+            }
+}
+
+}  // namespace
+
+extern "C" int plens_mesh_build_curved(int32_t N, int32_t q, int32_t nc, int32_t nv, const double* X,
+                                       const int32_t* cv, const int32_t* nbr, const int32_t* bid,
+                                       const plens_manifold_desc* md, int32_t transpose_y, plens_mesh** out) {
+    if (!out || !X || !cv || !nbr || !bid || !md) return PLENS_E_INVALID;
+    if (N < 1 || N > 5 || q < 1 || q > 8 || nc < 1 || nv < 8) return PLENS_E_INVALID;
+    if (md->type != PLENS_MANIFOLD_CYLINDER && md->type != PLENS_MANIFOLD_NOSE_CYLINDER) return PLENS_E_INVALID;
+    for (size_t i = 0; i < (size_t)nc * 8; ++i)
+        if (cv[i] < 0 || cv[i] >= nv) return PLENS_E_INVALID;
+    for (size_t i = 0; i < (size_t)nc * 6; ++i)
+        if (nbr[i] < -1 || nbr[i] >= nc) return PLENS_E_INVALID;
+    const double an = std::sqrt(md->axis_dir[0] * md->axis_dir[0] + md->axis_dir[1] * md->axis_dir[1] +
+                                md->axis_dir[2] * md->axis_dir[2]);
+    if (!(an > 0)) return PLENS_E_INVALID;
+    const Manifold cyl = make_cylinder(md->axis_dir, md->axis_point);
+    const Manifold sph = make_sphere(md->nose_center);
+    // manifold of every cell (NoseCylinder::set, manifolds/nose_cylinder.cc:45-78)
+    std::vector<int> cman(nc, 1);
+    if (md->type == PLENS_MANIFOLD_NOSE_CYLINDER) {
+        double sep = 0;
+        for (int d = 0; d < 3; ++d) sep += md->axis_dir[d] * (md->nose_center[d] - md->axis_point[d]);
+        if (!(sep < 0)) return PLENS_E_INVALID;  // nose center on the spherical side
+        for (int c = 0; c < nc; ++c) {
+            double dp = 0;
+            for (int d = 0; d < 3; ++d) {
+                double ctr = 0;
+                for (int v = 0; v < 8; ++v) ctr += X[3 * (size_t)cv[8 * (size_t)c + v] + d];
+                dp += md->axis_dir[d] * (ctr / 8.0 - md->axis_point[d]);
+            }
+            if (dp == 0.0) return PLENS_E_INVALID;
+            cman[c] = dp > 0 ? 2 : 1;
+        }
+    }
+    // a line or face takes the manifold of the last (highest-index) cell that contains it
+    std::map<std::array<int, 2>, int> line_owner;
+    std::map<std::array<int, 4>, int> face_owner;
+    auto line_key = [&](int c, int d, int a, int b) {
+        const int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+        int v0 = 0, v1 = 0;
+        int I0[3], I1[3];
+        I0[d] = 0; I1[d] = 1;
+        I0[d1] = I1[d1] = a;
+        I0[d2] = I1[d2] = b;
+        v0 = cv[8 * (size_t)c + I0[0] + 2 * I0[1] + 4 * I0[2]];
+        v1 = cv[8 * (size_t)c + I1[0] + 2 * I1[1] + 4 * I1[2]];
+        return std::array<int, 2>{std::min(v0, v1), std::max(v0, v1)};
+    };
+    auto face_key = [&](int c, int f) {
+        const int d = f / 2, e = f % 2;
+        std::array<int, 4> k;
+        int n = 0;
+        for (int v = 0; v < 8; ++v) {
+            const int b[3] = {v & 1, (v >> 1) & 1, (v >> 2) & 1};
+            if (b[d] == e) k[n++] = cv[8 * (size_t)c + v];
+        }
+        std::sort(k.begin(), k.end());
+        return k;
+    };
+    for (int c = 0; c < nc; ++c) {
+        for (int d = 0; d < 3; ++d)
+            for (int b = 0; b < 2; ++b)
+                for (int a = 0; a < 2; ++a) line_owner[line_key(c, d, a, b)] = c;
+        for (int f = 0; f < 6; ++f) face_owner[face_key(c, f)] = c;
+    }
+    plens_mesh* m = new (std::nothrow) plens_mesh;
+    if (!m) return PLENS_E_NOMEM;
+    const int np = N + 1, n = np * np * np, nf = np * np, qp = q + 1, nsp = qp * qp * qp;
+    m->N = N; m->n = n; m->nf = nf; m->n_cells = nc; m->q = q;
+    m->nx = nc; m->ny = 1; m->nz = 1;
+    m->x1d.resize(np); m->w1d.resize(np);
+    plens_basis_1d(N, m->x1d.data(), m->w1d.data(), nullptr, nullptr);
+    std::vector<double> tq(qp), wq(qp);
+    if (q == 1) { tq[0] = 0; tq[1] = 1; }
+    else gll_01(q, tq.data(), wq.data());
+    m->vertices.resize((size_t)nc * 24);
+    m->support.resize((size_t)nc * nsp * 3);
+    m->nodes.resize((size_t)nc * 3 * n);
+    m->Ja.resize((size_t)nc * 9 * n);
+    m->detJ.resize((size_t)nc * n);
+    m->normals.resize((size_t)nc * 18 * nf);
+    m->hmin.resize(nc);
+    m->face_nbr.assign(nbr, nbr + (size_t)nc * 6);
+    m->face_bid.assign(bid, bid + (size_t)nc * 6);
+    // 1D Lagrange values/derivatives of the GLL(q) support lattice at the GLL(N) nodes
+    std::vector<double> Lq((size_t)np * qp), dLq((size_t)np * qp);
+    for (int i = 0; i < np; ++i) lagrange_all(q, tq.data(), m->x1d[i], &Lq[(size_t)i * qp], &dLq[(size_t)i * qp]);
+    std::vector<P3> S;
+    for (int c = 0; c < nc; ++c) {
+        P3 V[8];
+        for (int v = 0; v < 8; ++v) {
+            const double* src = X + 3 * (size_t)cv[8 * (size_t)c + v];
+            V[v] = mk(src[0], src[1], src[2]);
+            for (int d = 0; d < 3; ++d) m->vertices[(size_t)c * 24 + 3 * v + d] = src[d];
+        }
+        const Manifold* lm[12];
+        int li = 0;
+        for (int d = 0; d < 3; ++d)
+            for (int b = 0; b < 2; ++b)
+                for (int a = 0; a < 2; ++a, ++li) lm[li] = cman[line_owner[line_key(c, d, a, b)]] == 2 ? &sph : &cyl;
+        const Manifold* fm[6];
+        for (int f = 0; f < 6; ++f) fm[f] = cman[face_owner[face_key(c, f)]] == 2 ? &sph : &cyl;
+        cell_support_points(q, tq.data(), V, lm, fm, cman[c] == 2 ? sph : cyl, S);
+        for (int s = 0; s < nsp; ++s)
+            for (int d = 0; d < 3; ++d) m->support[((size_t)c * nsp + s) * 3 + d] = S[s][d];
+        // minimum vertex distance over the 28 vertex pairs
+        double d2min = 1e300;
+        for (int a = 0; a < 8; ++a)
+            for (int b = a + 1; b < 8; ++b) {
+                const P3 t = sub3(V[a], V[b]);
+                d2min = std::min(d2min, dot3(t, t));
+            }
+        m->hmin[c] = std::sqrt(d2min);
+        // metric terms at the GLL(N) nodes from the degree-q map (MetricTerms::reinit,
+        // dgsem/metric_terms.cc:82-102): J = sum_s S_s grad L_s, Ja^d = a_{d+1} x a_{d+2}
+        double* Ja = &m->Ja[(size_t)c * 9 * n];
+        double* dJ = &m->detJ[(size_t)c * n];
+        double* XN = &m->nodes[(size_t)c * 3 * n];
+        for (int nk = 0; nk < np; ++nk)
+            for (int nj = 0; nj < np; ++nj)
+                for (int ni = 0; ni < np; ++ni) {
+                    const int node = ni + np * (nj + np * nk);
+                    const double* Li = &Lq[(size_t)ni * qp];
+                    const double* Lj = &Lq[(size_t)nj * qp];
+                    const double* Lk = &Lq[(size_t)nk * qp];
+                    const double* Di = &dLq[(size_t)ni * qp];
+                    const double* Dj = &dLq[(size_t)nj * qp];
+                    const double* Dk = &dLq[(size_t)nk * qp];
+                    Vec3 a[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                    double Xn[3] = {0, 0, 0};
+                    for (int k = 0; k < qp; ++k)
+                        for (int j = 0; j < qp; ++j)
+                            for (int i = 0; i < qp; ++i) {
+                                const P3& sp = S[(size_t)i + qp * ((size_t)j + (size_t)qp * k)];
+                                const double l = Li[i] * Lj[j] * Lk[k];
+                                const double g[3] = {Di[i] * Lj[j] * Lk[k], Li[i] * Dj[j] * Lk[k], Li[i] * Lj[j] * Dk[k]};
+                                for (int d = 0; d < 3; ++d) {
+                                    a[d].x += g[d] * sp[0];
+                                    a[d].y += g[d] * sp[1];
+                                    a[d].z += g[d] * sp[2];
+                                }
+                                for (int d = 0; d < 3; ++d) Xn[d] += l * sp[d];
+                            }
+                    for (int d = 0; d < 3; ++d) XN[d * n + node] = Xn[d];
+                    const Vec3 cc[3] = {cross(a[1], a[2]), cross(a[2], a[0]), cross(a[0], a[1])};
+                    for (int d = 0; d < 3; ++d) {
+                        Ja[(3 * d + 0) * n + node] = cc[d].x;
+                        Ja[(3 * d + 1) * n + node] = cc[d].y;
+                        Ja[(3 * d + 2) * n + node] = cc[d].z;
+                    }
+                    dJ[node] = a[0].x * cc[0].x + a[0].y * cc[0].y + a[0].z * cc[0].z;
+                }
+        double* Nrm = &m->normals[(size_t)c * 18 * nf];
+        for (int f = 0; f < 6; ++f) {
+            const int dir = f / 2;
+            const double sgn = (f % 2) ? 1.0 : -1.0;
+            for (int qq0 = 0; qq0 < nf; ++qq0) {
+                int qq = qq0;
+                if ((f == 2 || f == 3) && transpose_y) qq = (qq0 % np) * np + qq0 / np;
+                const int node = face_node(N, f, qq);
+                const double v0 = Ja[(3 * dir + 0) * n + node], v1 = Ja[(3 * dir + 1) * n + node],
+                             v2 = Ja[(3 * dir + 2) * n + node];
+                const double nrm = std::sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+                Nrm[(f * 3 + 0) * nf + qq0] = sgn * v0 / nrm;
+                Nrm[(f * 3 + 1) * nf + qq0] = sgn * v1 / nrm;
+                Nrm[(f * 3 + 2) * nf + qq0] = sgn * v2 / nrm;
+            }
+        }
+    }
+    *out = m;
+    return PLENS_OK;
+}
+
+extern "C" int32_t plens_mesh_mapping_degree(const plens_mesh* m) { return m ? m->q : 0; }
+extern "C" const double* plens_mesh_support_points(const plens_mesh* m) { return m->support.data(); }
 
 extern "C" int32_t plens_face_node(int32_t degree, int32_t f, int32_t q) {
     if (degree < 1 || f < 0 || f > 5 || q < 0 || q >= (degree + 1) * (degree + 1)) return -1;

@@ -100,6 +100,41 @@ def test_steps_remaining_fluxes(name, surface, volume):
     assert err <= RUN_TOL
 
 
+def _with_jump(case, x0):
+    """density doubled beyond x = x0 at the same velocity and pressure: blended cells at the jump"""
+    Q = case.Q0.copy()
+    x = case.mesh.node_coords[:, 0, :].reshape(-1)
+    f = np.where(x > x0, 2.0, 1.0)
+    v = Q[1:4] / Q[0]
+    p = 0.4 * (Q[4] - 0.5 * Q[0] * (v * v).sum(0))
+    Q[0] *= f
+    Q[1:4] = Q[0] * v
+    Q[4] = p / 0.4 + 0.5 * Q[0] * (v * v).sum(0)
+    case.Q0 = Q
+    return case
+
+
+@pytest.mark.parametrize("geometry,N,q", [("flare", 3, 2), ("flare", 4, 3), ("flare-perturbed", 4, 2),
+                                          ("nose", 3, 2), ("nose", 4, 3)])
+def test_curved_meshes(geometry, N, q):
+    """MappingQ(q) meshes on the cylinder and nose-cylinder manifold descriptions, metric terms
+    handed over stored: RHS (smooth, and with a density jump that blends cells), 10 steps."""
+    from plens_amd import cylinder_flare, nose_cone
+    make = (lambda: nose_cone(degree=N, mapping_degree=q)) if geometry == "nose" else \
+        (lambda: cylinder_flare(degree=N, mapping_degree=q, perturb=0.2 if "perturbed" in geometry else 0.0))
+    _rhs_parity(make(), f"{geometry}-N{N}-q{q}")
+    x0 = -0.2 if geometry == "nose" else 1.1
+    _, o = _rhs_parity(_with_jump(make(), x0), f"{geometry}-N{N}-q{q}-jump")
+    assert (o.calc_rhs()[1] > 0).any()
+    case = make()
+    g, o = _pair(case)
+    g.run(10, case.courant)
+    o.run(10, case.courant)
+    err = rel_l2(g.get_state(), o.get_state())
+    print(f"{geometry} N{N} q{q} 10 steps rel L2 {err:.3e}", flush=True)
+    assert err <= RUN_TOL
+
+
 def test_dt_parity():
     from plens_amd import wall_flow
     case = wall_flow(nx=6, ny=4, nz=2)
