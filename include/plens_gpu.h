@@ -186,6 +186,38 @@ int plens_gpu_set_local_stepping(plens_gpu* h, int32_t request, double start_tim
  * before the first step. */
 int plens_gpu_ss_error(plens_gpu* h, double* err);
 
+/* Data output of PLENS::write() (dgsem/plens.cc:4170-4322), the fields only (files: plens_host.h).
+ * write() runs inside the first RK stage (update_rk4, 4341-4350), after the stage-1 auxiliary
+ * variables, blender and calc_time_step of the current state; this call computes those for the
+ * current state (local steps switched on by the solution time t; rho E at the step start is not
+ * recorded) and post_process (3950-3963), and returns, interpolated from the GLL nodes to the
+ * equispaced (N+1)^3 lattice of every cell that DataOut::build_patches(mapping, N) evaluates
+ * (lexicographic, x fastest):
+ *   nodal[PLENS_OUT_NODAL][n_cells][(N+1)^3]: txx txy txz tyy tyz tzz qx qy qz rho p T u v w
+ *   cell[PLENS_OUT_CELL][n_cells]: alpha, alpha_d, loc_dt (the cell's step).
+ * The state is not changed; the time step and blender buffers hold the new values afterwards, so
+ * a following plens_gpu_ss_error divides by the new steps as write() does. Single handles only.
+ * The first call allocates the device buffer of the fields. */
+#define PLENS_OUT_NODAL 15
+#define PLENS_OUT_CELL 3
+int plens_gpu_output_fields(plens_gpu* h, double courant, double t, double* nodal, double* cell);
+
+/* Checkpoint and restart of the solution: the role of the ".ar" archive written by write()
+ * (do_solution_transfer, 4117-4127, called at 4304-4308) and read by ICs::FromArchiveRestart
+ * (IBCs/from_archive_restart.cc), whose p4est format is deal.II's own. Little-endian binary file:
+ * header {char magic[8] = "PLENSGPU"; uint32 version = 1; int32 degree; int64 n_cells, n_dof;
+ * double time; int64 step, output_counter; uint64 mesh_fingerprint, state_checksum} followed by
+ * the state Q[5][n_dof] in FP64; checksum and fingerprint are FNV-1a 64 (of the state bytes; of the
+ * degree, face_nbr, face_bid and min_vertex_dist of the mesh). Written through a temporary file
+ * and renamed. Reading restores the state and returns time and counters (each nullable); a file
+ * of another degree, size or mesh, a wrong checksum or trailing bytes give PLENS_E_INVALID and
+ * leave the state as it was. As the reference's restart, only the conservative variables are
+ * restored: the next step recomputes everything else. Single handles only. */
+int plens_gpu_write_checkpoint(plens_gpu* h, const char* path, double time, int64_t step,
+                               int64_t output_counter);
+int plens_gpu_read_checkpoint(plens_gpu* h, const char* path, double* time, int64_t* step,
+                              int64_t* output_counter);
+
 /* Kernel timing with no counterpart in the reference: after one RHS evaluation of the current
  * state, launches kernel class `kernel` (0 gradient/blender, 2 face flux, 4 volume + RK update)
  * back to back on the handle's stream, `reps` times with the parameters of each of the 5 RK stages

@@ -81,6 +81,26 @@ int plens_mesh_build_curved(int32_t degree, int32_t mapping_degree, int32_t n_ce
 /* Mapping degree q (1 for boxes) and the support points [n_cells][(q+1)^3][3], lexicographic over
  * the GLL(q) lattice of the reference cell (for q = 1: the 8 vertices). */
 int32_t plens_mesh_mapping_degree(const plens_mesh* m);
+/* Coordinates of the equispaced (N+1)^3 output lattice of every cell through its map:
+ * xyz[n_cells][(N+1)^3][3], lattice points lexicographic (x fastest). */
+int plens_mesh_output_points(const plens_mesh* m, double* xyz);
+
+/* Data output files of PLENS::write() (dgsem/plens.cc:4170-4322) from the fields of
+ * plens_gpu_output_fields (nodal [15][n_cells][(N+1)^3], cell [3][n_cells]) for one process:
+ *   <dir>/<base>_<counter, 6 digits>.0.vtu  one VTK Lagrange hexahedron (type 72, order N, VTK
+ *       XML version 2.2 node order) per cell on the equispaced lattice (write_higher_order_cells,
+ *       build_patches(mapping, N)); point data in the reference's order: txx txy txz tyy tyz tzz
+ *       qx qy qz rho p T u v w subdomain alpha alpha_d loc_dt global_active_cell_index (the cell
+ *       values constant over the cell's points); inline base64 Float64/Int64/UInt8 arrays with
+ *       UInt64 byte-count headers;
+ *   <dir>/<base>_<counter, 6 digits>.pvtu  the parallel record of that piece;
+ *   <dir>/<base>.pvd  rewritten with one data set per output 0..n_records-1 at record_times[i]
+ *       (the times_and_names of write_pvd_record);
+ *   <dir>/<base>.times  "counter time wall_time" appended.
+ * The directory must exist. Returns 0 or PLENS_E_INVALID (bad argument, file not writable). */
+int plens_output_write(const plens_mesh* m, const char* directory, const char* base, int32_t counter,
+                       double time, double wall_time, const double* nodal, const double* cell,
+                       const double* record_times, int32_t n_records);
 const double* plens_mesh_support_points(const plens_mesh* m);
 void plens_mesh_destroy(plens_mesh* m);
 

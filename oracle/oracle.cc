@@ -1673,6 +1673,51 @@ int oracle_calc_dt(oracle* o, double courant, double* dt) {
 
 int oracle_rk4_step(oracle* o, double courant, double* t) { return o->rk4_step(courant, t); }
 
+// PLENS::write() fields (dgsem/plens.cc:4170-4322): write() follows calc_rhs(true) and
+// calc_time_step() inside the first RK stage (4341-4350); post_process (3950-3963) gives p, T and
+// the velocity at the DoFs; DataOut evaluates every field's FE interpolant at the equispaced
+// (N+1)^3 points of each cell (build_patches with N subdivisions), here by direct summation over
+// all nodal basis functions.
+int oracle_output_fields(oracle* o, double courant, double t, double* nodal, double* cell) {
+    int e = o->calc_rhs();
+    if (e) return e;
+    o->calc_time_step(courant, t);
+    const int np = o->np, n = o->n, N = o->N;
+    const size_t nd = o->ndof;
+    std::vector<double> L((size_t)np * np);  // L[e][i] = l_i(e/N)
+    for (int ee = 0; ee < np; ++ee)
+        for (int i = 0; i < np; ++i) L[ee * np + i] = (double)lagrange_val(o->x1, i, (long double)ee / N);
+    const double cv = o->ns.cv();
+    std::vector<double> f((size_t)15 * n);
+    for (int c = 0; c < o->nc; ++c) {
+        for (int i = 0; i < n; ++i) {
+            const State s = o->cons(c, i);
+            for (int a = 0; a < 9; ++a) f[(size_t)a * n + i] = o->avars[a * nd + o->gid(c, i)];
+            f[(size_t)9 * n + i] = s[0];
+            f[(size_t)10 * n + i] = o->ns.get_p(s);
+            f[(size_t)11 * n + i] = NS::get_e(s) / cv;
+            for (int d = 0; d < 3; ++d) f[(size_t)(12 + d) * n + i] = s[1 + d] / s[0];
+        }
+        for (int ek = 0; ek < np; ++ek)
+            for (int ej = 0; ej < np; ++ej)
+                for (int ei = 0; ei < np; ++ei) {
+                    const int pt = ei + np * (ej + np * ek);
+                    for (int q = 0; q < 15; ++q) {
+                        double v = 0;
+                        for (int k = 0; k < np; ++k)
+                            for (int j = 0; j < np; ++j)
+                                for (int i = 0; i < np; ++i)
+                                    v += f[(size_t)q * n + o->tl(i, j, k)] * L[ei * np + i] * L[ej * np + j] * L[ek * np + k];
+                        nodal[(size_t)q * nd + (size_t)c * n + pt] = v;
+                    }
+                }
+        cell[c] = o->alpha[c];
+        cell[o->nc + c] = o->alpha_d[c];
+        cell[2 * (size_t)o->nc + c] = o->loc_dt[c];
+    }
+    return 0;
+}
+
 int oracle_rk4_step_fixed_dt(oracle* o, double dt, double* t) { return o->rk4_step(0.0, t, dt); }
 
 int oracle_run(oracle* o, int32_t n_steps, double courant, double* t) {

@@ -50,6 +50,9 @@ int oracle_create_mapped(int32_t n_cells, int32_t degree, int32_t map_degree, co
                          const int32_t* face_nbr, const int32_t* face_bid, const oracle_physics* phys,
                          const oracle_bc* bcs, int32_t n_bcs, int32_t transpose_y, oracle** out);
 void oracle_destroy(oracle* o);
+/* fields of PLENS::write() for the current state: nodal [15][n_cells][(N+1)^3] on the equispaced
+ * lattice (txx..qz, rho, p, T, u, v, w), cell [3][n_cells] (alpha, alpha_d, loc_dt) */
+int oracle_output_fields(oracle* o, double courant, double t, double* nodal, double* cell);
 int oracle_set_threads(int32_t n);
 int oracle_set_state(oracle* o, const double* Q);
 int oracle_get_state(const oracle* o, double* Q);

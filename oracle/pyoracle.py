@@ -45,6 +45,7 @@ def load():
         "oracle_create_mapped": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _dp, ip, ip, C.POINTER(OPhys),
                                            C.POINTER(OBC), C.c_int32, C.c_int32, C.POINTER(vp)]),
         "oracle_destroy": (None, [vp]),
+        "oracle_output_fields": (C.c_int, [vp, C.c_double, C.c_double, _dp, _dp]),
         "oracle_set_threads": (C.c_int, [C.c_int32]),
         "oracle_set_state": (C.c_int, [vp, _dp]),
         "oracle_get_state": (C.c_int, [vp, _dp]),
@@ -191,6 +192,14 @@ class OraclePlens:
         if self.lib.oracle_loc_dt(self._h, out.ctypes.data_as(_dp)):
             raise RuntimeError("no step taken yet")
         return out
+
+    def output_fields(self, courant, t=0.0):
+        nodal = np.empty((15, self.n_dof))
+        cell = np.empty((3, self.n_cells))
+        rc = self.lib.oracle_output_fields(self._h, courant, t, nodal.ctypes.data_as(_dp), cell.ctypes.data_as(_dp))
+        if rc:
+            self._fail(rc)
+        return nodal, cell
 
     def ss_error(self):
         e = C.c_double()

@@ -71,7 +71,8 @@ GPU_SYMBOLS = [
     "plens_gpu_calc_rhs", "plens_gpu_calc_dt", "plens_gpu_rk4_step", "plens_gpu_run",
     "plens_gpu_last_failure", "plens_gpu_set_tuning", "plens_gpu_kernel_times", "plens_gpu_strerror",
     "plens_gpu_last_run_ms", "plens_gpu_run_profiled", "plens_gpu_stream_copy", "plens_gpu_time_kernel",
-    "plens_gpu_set_local_stepping", "plens_gpu_ss_error",
+    "plens_gpu_set_local_stepping", "plens_gpu_ss_error", "plens_gpu_output_fields", "plens_gpu_write_checkpoint",
+    "plens_gpu_read_checkpoint",
     "plens_gpu_batch_create", "plens_gpu_batch_destroy", "plens_gpu_batch_run", "plens_gpu_batch_get_state",
     "plens_gpu_batch_last_run_ms", "plens_gpu_batch_last_failure", "plens_gpu_batch_n_groups",
     "plens_gpu_batch_group_info",
@@ -82,7 +83,7 @@ HOST_SYMBOLS = [
     "plens_mesh_detJ", "plens_mesh_face_normals", "plens_mesh_face_nbr", "plens_mesh_face_bid",
     "plens_mesh_min_vertex_dist", "plens_basis_1d", "plens_prim_to_cons", "plens_ic_density_wave",
     "plens_ic_normal_shock", "plens_ic_uniform", "plens_face_node", "plens_mesh_build_curved",
-    "plens_mesh_mapping_degree", "plens_mesh_support_points",
+    "plens_mesh_mapping_degree", "plens_mesh_support_points", "plens_mesh_output_points", "plens_output_write",
 ]
 MLP_SYMBOLS = [
     "plens_mlp_n_params", "plens_mlp_create", "plens_mlp_destroy", "plens_mlp_get_params",
@@ -123,6 +124,9 @@ def load():
         "plens_gpu_time_kernel": (C.c_int, [vp, C.c_int32, C.c_int32, _dp]),
         "plens_gpu_set_local_stepping": (C.c_int, [vp, C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_double]),
         "plens_gpu_ss_error": (C.c_int, [vp, _dp]),
+        "plens_gpu_output_fields": (C.c_int, [vp, C.c_double, C.c_double, _dp, _dp]),
+        "plens_gpu_write_checkpoint": (C.c_int, [vp, C.c_char_p, C.c_double, C.c_int64, C.c_int64]),
+        "plens_gpu_read_checkpoint": (C.c_int, [vp, C.c_char_p, _dp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "plens_gpu_batch_create": (C.c_int, [C.POINTER(vp), C.c_int32, C.c_char_p, C.POINTER(vp)]),
         "plens_gpu_batch_destroy": (None, [vp]),
         "plens_gpu_batch_run": (C.c_int, [vp, C.c_int32, _dp, _dp]),
@@ -148,6 +152,9 @@ def load():
                                               C.POINTER(ManifoldDesc), C.c_int32, C.POINTER(vp)]),
         "plens_mesh_mapping_degree": (C.c_int32, [vp]),
         "plens_mesh_support_points": (_dp, [vp]),
+        "plens_mesh_output_points": (C.c_int, [vp, _dp]),
+        "plens_output_write": (C.c_int, [vp, C.c_char_p, C.c_char_p, C.c_int32, C.c_double, C.c_double, _dp, _dp,
+                                         _dp, C.c_int32]),
         "plens_basis_1d": (C.c_int, [C.c_int32, _dp, _dp, _dp, _dp]),
         "plens_prim_to_cons": (None, [C.c_double, C.c_double, _dp, C.c_double, _dp]),
         "plens_ic_density_wave": (C.c_int, [vp, C.c_double, C.c_double, _dp, C.c_double, C.c_double,

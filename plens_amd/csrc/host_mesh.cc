@@ -794,6 +794,39 @@ extern "C" int plens_mesh_build_curved(int32_t N, int32_t q, int32_t nc, int32_t
     return PLENS_OK;
 }
 
+extern "C" int plens_mesh_output_points(const plens_mesh* m, double* xyz) {
+    if (!m || !xyz) return PLENS_E_INVALID;
+    const int N = m->N, np = N + 1, n = m->n, q = m->q, qp = q + 1, nsp = qp * qp * qp;
+    std::vector<double> tq(qp), wq(qp), L((size_t)np * qp);
+    if (q == 1) { tq[0] = 0; tq[1] = 1; }
+    else gll_01(q, tq.data(), wq.data());
+    for (int e = 0; e < np; ++e) lagrange_all(q, tq.data(), (double)e / N, &L[(size_t)e * qp], nullptr);
+    for (int c = 0; c < m->n_cells; ++c) {
+        const double* S = &m->support[(size_t)c * nsp * 3];
+        for (int ek = 0; ek < np; ++ek)
+            for (int ej = 0; ej < np; ++ej)
+                for (int ei = 0; ei < np; ++ei) {
+                    double X[3] = {0, 0, 0};
+                    if (q == 1) {  // trilinear map of the vertices
+                        const double xi[3] = {(double)ei / N, (double)ej / N, (double)ek / N};
+                        Vec3 a[3];
+                        trilinear_jac(S, xi, a, X);
+                    } else {
+                        for (int k = 0; k < qp; ++k)
+                            for (int j = 0; j < qp; ++j)
+                                for (int i = 0; i < qp; ++i) {
+                                    const double l = L[(size_t)ei * qp + i] * L[(size_t)ej * qp + j] * L[(size_t)ek * qp + k];
+                                    const double* sp = S + 3 * ((size_t)i + qp * ((size_t)j + (size_t)qp * k));
+                                    for (int d = 0; d < 3; ++d) X[d] += l * sp[d];
+                                }
+                    }
+                    double* dst = xyz + 3 * ((size_t)c * n + ei + np * (ej + np * ek));
+                    for (int d = 0; d < 3; ++d) dst[d] = X[d];
+                }
+    }
+    return PLENS_OK;
+}
+
 extern "C" int32_t plens_mesh_mapping_degree(const plens_mesh* m) { return m ? m->q : 0; }
 extern "C" const double* plens_mesh_support_points(const plens_mesh* m) { return m->support.data(); }
 

@@ -60,6 +60,7 @@ struct Basis {
     double gw[kMaxNP];             // QGauss(N+1) weights on [0,1]
     double GL[kMaxNP * kMaxNP];    // GL[q][i] = l_i(xg_q): GLL Lagrange basis at the Gauss points
     double gx[kMaxNP];             // QGauss(N+1) points on [0,1]
+    double EQ[kMaxNP * kMaxNP];    // EQ[e][i] = l_i(e/N): GLL Lagrange basis at the equispaced points
 };
 
 struct KP {
@@ -1372,11 +1373,79 @@ __global__ void __launch_bounds__(256) k_stream_copy(const double2* __restrict__
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) dst[i] = src[i];
 }
 
+// =============================================================================================
+// k_post: output fields of PLENS::write() (dgsem/plens.cc:4170-4322) for one cell per workgroup,
+// one thread per node: the stage-1 avars and post_process (3950-3963: p, T = e / c_v, velocity =
+// momentum / rho) with rho, interpolated from the GLL nodes to the equispaced (N+1)^3 lattice that
+// DataOut::build_patches(mapping, N) evaluates (three sum-factorised passes of EQ). Layout
+// out[f][c][lattice point] for the 15 nodal fields, then alpha, alpha_d and the cell's step.
+// =============================================================================================
+constexpr int kOutNodal = 15, kOutCell = 3;
+template <int N>
+__global__ void __launch_bounds__(256) k_post(KP p, double* out) {
+    constexpr int NP = N + 1, NN = NP * NP * NP;
+    extern __shared__ double sm[];  // 2 * kOutNodal * NN
+    __shared__ double sE[NP * NP];
+    double* s0 = sm;
+    double* s1 = sm + kOutNodal * NN;
+    const int c = blockIdx.x, node = threadIdx.x;  // blockDim.x == NN
+    for (int t = node; t < NP * NP; t += blockDim.x) sE[t] = p.basis->EQ[(t / NP) * kMaxNP + t % NP];
+    const long long g = (long long)c * NN + node;
+    {
+        double u[5];
+        load_state(p.Q, p.ndof, g, u);
+        for (int a = 0; a < 9; ++a)
+            s0[a * NN + node] = p.avars ? p.avars[a * p.ndof + g] : (double)p.avars_f[a * p.ndof + g];
+        s0[9 * NN + node] = u[0];
+        s0[10 * NN + node] = get_p(u, p.gas.gma);
+        s0[11 * NN + node] = get_e(u) / p.gas.cv;
+        for (int d = 0; d < 3; ++d) s0[(12 + d) * NN + node] = u[1 + d] / u[0];
+    }
+    const int ti = node % NP, tj = (node / NP) % NP, tk = node / (NP * NP);
+    __syncthreads();
+    for (int f = 0; f < kOutNodal; ++f) {  // x pass: s1[e_x, j, k]
+        double a = 0;
+        for (int m = 0; m < NP; ++m) a += sE[ti * NP + m] * s0[f * NN + m + NP * (tj + NP * tk)];
+        s1[f * NN + node] = a;
+    }
+    __syncthreads();
+    for (int f = 0; f < kOutNodal; ++f) {  // y pass
+        double a = 0;
+        for (int m = 0; m < NP; ++m) a += sE[tj * NP + m] * s1[f * NN + ti + NP * (m + NP * tk)];
+        s0[f * NN + node] = a;
+    }
+    __syncthreads();
+    for (int f = 0; f < kOutNodal; ++f) {  // z pass, to HBM
+        double a = 0;
+        for (int m = 0; m < NP; ++m) a += sE[tk * NP + m] * s0[f * NN + ti + NP * (tj + NP * m)];
+        out[(long long)f * p.ndof + g] = a;
+    }
+    if (node == 0) {
+        double* oc = out + (long long)kOutNodal * p.ndof;
+        oc[c] = p.alpha[c];
+        oc[p.nc + c] = p.alpha_d[c];
+        oc[2 * (long long)p.nc + c] = p.dt_lts ? p.dt_lts[c] : p.dt[p.cinst[c]];
+    }
+}
+
 }  // namespace
 
 // =============================================================================================
 // host side
 // =============================================================================================
+namespace {
+constexpr uint64_t kFnvOffset = 1469598103934665603ull;
+// FNV-1a 64 over bytes, continuing from h
+uint64_t fnv1a(const void* data, size_t n, uint64_t h) {
+    const unsigned char* b = static_cast<const unsigned char*>(data);
+    for (size_t i = 0; i < n; ++i) {
+        h ^= b[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+}  // namespace
+
 struct plens_gpu {
     int N = 0, NN = 0, NF = 0, nc = 0, cpb = 1, n_bcs = 0, ntasks = 0;
     long long ndof = 0;
@@ -1391,6 +1460,8 @@ struct plens_gpu {
     double* rhoE_old = nullptr;  // rho E at the start of the last step
     double* dt_lts = nullptr;    // local time stepping: per-cell steps (allocated on request)
     double* ss_part = nullptr;   // steady-state error: per-cell integrals, then the two sums
+    double* post = nullptr;      // output fields (allocated by the first output call)
+    uint64_t mesh_fp = 0;        // fingerprint of the mesh (checkpoints)
     long long steps_taken = 0;
     bool use_cmap = false;
     int *nbr = nullptr, *bid = nullptr, *wall = nullptr, *tasks = nullptr;
@@ -1885,6 +1956,14 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
                     if (m2 != i) l *= (gx[q] - x[m2]) / (x[i] - x[m2]);
                 hbas.GL[q * kMaxNP + i] = l;
             }
+        for (int e2 = 0; e2 < NP; ++e2)  // GLL Lagrange basis at the equispaced points e/N
+            for (int i = 0; i < NP; ++i) {
+                const double xe = (double)e2 / N;
+                double l = 1.0;
+                for (int m2 = 0; m2 < NP; ++m2)
+                    if (m2 != i) l *= (xe - x[m2]) / (x[i] - x[m2]);
+                hbas.EQ[e2 * kMaxNP + i] = l;
+            }
         for (int i = 0; i < NP; ++i) {
             hbas.x[i] = x[i];
             hbas.w[i] = w[i];
@@ -1988,6 +2067,12 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
     k.wlimit = ph->wall_blender_limit;
     k.iw0 = 1.0 / hbas.w[0];
     k.iwN = 1.0 / hbas.w[N];
+    {  // mesh fingerprint (checkpoints): degree, connectivity, boundary indices, minimum vertex distances
+        uint64_t fp = fnv1a(&N, sizeof(N), kFnvOffset);
+        fp = fnv1a(m->face_nbr, sizeof(int32_t) * 6 * (size_t)nc, fp);
+        fp = fnv1a(m->face_bid, sizeof(int32_t) * 6 * (size_t)nc, fp);
+        h->mesh_fp = fnv1a(m->min_vertex_dist, sizeof(double) * (size_t)nc, fp);
+    }
     *out = h;
     return PLENS_OK;
 }
@@ -2002,7 +2087,7 @@ void plens_gpu_destroy(plens_gpu* h) {
                     h->bid, h->wall, h->tasks, h->bcs, h->basis, h->avars, h->araw, h->alpha, h->alpha_d,
                     h->sinv, h->sface, h->rhs, h->rlo, h->dt_cell, h->scal, h->fail, h->blist, h->bcount, h->mlp_stats, h->feat,
                     h->avars_f, h->sinv_f, h->sface_f, h->cinst, h->ioff, h->visc, h->cfl, h->cmap, h->fmet,
-                    h->tabs, h->rhoE_old, h->dt_lts, h->ss_part};
+                    h->tabs, h->rhoE_old, h->dt_lts, h->ss_part, h->post};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
@@ -2222,6 +2307,112 @@ int plens_gpu_ss_error(plens_gpu* h, double* err) {
     HIPCHK(hipMemcpyAsync(nd, out, sizeof(nd), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     *err = std::sqrt(nd[0]) / std::sqrt(nd[1]);
+    return PLENS_OK;
+}
+
+int plens_gpu_output_fields(plens_gpu* h, double courant, double t, double* nodal, double* cell) {
+    if (!h || !nodal || !cell || h->ninst != 1) return PLENS_E_INVALID;
+    int e = reset_failure(h);
+    if (e) return e;
+    const size_t nout = (size_t)kOutNodal * h->ndof + (size_t)kOutCell * h->nc;
+    if (!h->post && hipMalloc((void**)&h->post, sizeof(double) * nout) != hipSuccess) return PLENS_E_NOMEM;
+    KP kp = h->base;
+    kp.Q = h->state_in_b ? h->Qb : h->Qa;
+    kp.want_dt = 1;
+    kp.courant = courant;
+    kp.t_acc = nullptr;
+    kp.save_E = 0;
+    // stage-1 quantities of the current state as write() sees them (update_rk4, 4341-4350): avars,
+    // blender and time step (calc_time_step; local steps switched on by t as in a step from t)
+    HIPCHK(hipMemcpyAsync(h->tabs + h->ninst, &t, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if ((e = launch_grad(h, kp))) return e;
+    hipLaunchKernelGGL(k_alpha, dim3((kp.nc + kThreadsAlpha - 1) / kThreadsAlpha), dim3(kThreadsAlpha), 0,
+                       h->stream, kp);
+    HIPCHK(hipGetLastError());
+    if (kp.dt_lts) {
+        hipLaunchKernelGGL(k_lts, dim3((kp.nc + kThreadsAlpha - 1) / kThreadsAlpha), dim3(kThreadsAlpha), 0,
+                           h->stream, kp);
+        HIPCHK(hipGetLastError());
+    }
+    if ((e = read_failure(h))) return e;
+    const size_t lds = sizeof(double) * 2 * kOutNodal * h->NN;
+    switch (h->N) {
+        case 1: hipLaunchKernelGGL(k_post<1>, dim3(h->nc), dim3(h->NN), lds, h->stream, kp, h->post); break;
+        case 2: hipLaunchKernelGGL(k_post<2>, dim3(h->nc), dim3(h->NN), lds, h->stream, kp, h->post); break;
+        case 3: hipLaunchKernelGGL(k_post<3>, dim3(h->nc), dim3(h->NN), lds, h->stream, kp, h->post); break;
+        case 4: hipLaunchKernelGGL(k_post<4>, dim3(h->nc), dim3(h->NN), lds, h->stream, kp, h->post); break;
+        default: hipLaunchKernelGGL(k_post<5>, dim3(h->nc), dim3(h->NN), lds, h->stream, kp, h->post); break;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(nodal, h->post, sizeof(double) * kOutNodal * h->ndof, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(cell, h->post + (size_t)kOutNodal * h->ndof, sizeof(double) * kOutCell * h->nc,
+                          hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return PLENS_OK;
+}
+
+namespace {
+struct CheckpointHeader {
+    char magic[8];
+    uint32_t version;
+    int32_t degree;
+    int64_t n_cells, n_dof;
+    double time;
+    int64_t step, output_counter;
+    uint64_t mesh_fingerprint, state_checksum;
+};
+constexpr char kCkptMagic[8] = {'P', 'L', 'E', 'N', 'S', 'G', 'P', 'U'};
+constexpr uint32_t kCkptVersion = 1;
+}  // namespace
+
+int plens_gpu_write_checkpoint(plens_gpu* h, const char* path, double time, int64_t step, int64_t output_counter) {
+    if (!h || !path || h->ninst != 1) return PLENS_E_INVALID;
+    std::vector<double> Q((size_t)5 * h->ndof);
+    int e = plens_gpu_get_state(h, Q.data());
+    if (e) return e;
+    CheckpointHeader hd{};
+    std::memcpy(hd.magic, kCkptMagic, 8);
+    hd.version = kCkptVersion;
+    hd.degree = h->N;
+    hd.n_cells = h->nc;
+    hd.n_dof = h->ndof;
+    hd.time = time;
+    hd.step = step;
+    hd.output_counter = output_counter;
+    hd.mesh_fingerprint = h->mesh_fp;
+    hd.state_checksum = fnv1a(Q.data(), sizeof(double) * Q.size(), kFnvOffset);
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return PLENS_E_INVALID;
+    const bool ok = std::fwrite(&hd, sizeof(hd), 1, f) == 1 &&
+                    std::fwrite(Q.data(), sizeof(double), Q.size(), f) == Q.size();
+    if (std::fclose(f) != 0 || !ok || std::rename(tmp.c_str(), path) != 0) {
+        std::remove(tmp.c_str());
+        return PLENS_E_INVALID;
+    }
+    return PLENS_OK;
+}
+
+int plens_gpu_read_checkpoint(plens_gpu* h, const char* path, double* time, int64_t* step, int64_t* output_counter) {
+    if (!h || !path || h->ninst != 1) return PLENS_E_INVALID;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return PLENS_E_INVALID;
+    CheckpointHeader hd{};
+    std::vector<double> Q((size_t)5 * h->ndof);
+    bool ok = std::fread(&hd, sizeof(hd), 1, f) == 1 && std::memcmp(hd.magic, kCkptMagic, 8) == 0 &&
+              hd.version == kCkptVersion && hd.degree == h->N && hd.n_cells == h->nc && hd.n_dof == h->ndof &&
+              hd.mesh_fingerprint == h->mesh_fp && std::fread(Q.data(), sizeof(double), Q.size(), f) == Q.size();
+    if (ok) {  // nothing may follow the state
+        char extra;
+        ok = std::fread(&extra, 1, 1, f) == 0;
+    }
+    std::fclose(f);
+    if (!ok || fnv1a(Q.data(), sizeof(double) * Q.size(), kFnvOffset) != hd.state_checksum) return PLENS_E_INVALID;
+    const int e = plens_gpu_set_state(h, Q.data());
+    if (e) return e;
+    if (time) *time = hd.time;
+    if (step) *step = hd.step;
+    if (output_counter) *output_counter = hd.output_counter;
     return PLENS_OK;
 }
 

@@ -26,6 +26,16 @@ def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
 
 
+def write_output_files(mesh, directory, base, counter, time, nodal, cell, record_times, wall_time=0.0):
+    """plens_output_write: vtu piece, pvtu record, pvd collection and .times log of one output"""
+    nodal = np.ascontiguousarray(nodal, dtype=np.float64)
+    cell = np.ascontiguousarray(cell, dtype=np.float64)
+    rt = np.ascontiguousarray(record_times, dtype=np.float64)
+    L.check(L.load().plens_output_write(mesh.handle, str(directory).encode(), str(base).encode(), counter, time,
+                                        wall_time, _dptr(nodal), _dptr(cell), _dptr(rt), len(rt)),
+            "plens_output_write")
+
+
 class GpuPlens:
     def __init__(self, case=None, mesh=None, physics=None, bcs=None, cells_per_block=0, metrics="map"):
         """metrics: "map" derives the metric terms in the kernels from each cell's trilinear map
@@ -110,6 +120,32 @@ class GpuPlens:
         e = C.c_double()
         self._check(self.lib.plens_gpu_ss_error(self._h, C.byref(e)), "ss_error")
         return e.value
+
+    def output_fields(self, courant, t=0.0):
+        """fields of PLENS::write() for the current state: nodal [15, n_dof] on each cell's
+        equispaced lattice (txx..qz, rho, p, T, u, v, w), cell [3, n_cells] (alpha, alpha_d, loc_dt)"""
+        nodal = np.empty((15, self.n_dof))
+        cell = np.empty((3, self.n_cells))
+        self._check(self.lib.plens_gpu_output_fields(self._h, courant, t, _dptr(nodal), _dptr(cell)), "output_fields")
+        return nodal, cell
+
+    def write_checkpoint(self, path, time, step=0, output_counter=0):
+        self._check(self.lib.plens_gpu_write_checkpoint(self._h, str(path).encode(), time, step, output_counter),
+                    "write_checkpoint")
+
+    def read_checkpoint(self, path):
+        """restore the state from a checkpoint; returns (time, step, output_counter)"""
+        t, st, oc = C.c_double(), C.c_int64(), C.c_int64()
+        self._check(self.lib.plens_gpu_read_checkpoint(self._h, str(path).encode(), C.byref(t), C.byref(st),
+                                                       C.byref(oc)), "read_checkpoint")
+        return t.value, st.value, oc.value
+
+    def write_output(self, directory, base, counter, courant, time, record_times, wall_time=0.0):
+        """PLENS::write(): the fields of the current state and the vtu/pvtu/pvd/times files;
+        record_times = solution times of outputs 0..counter"""
+        nodal, cell = self.output_fields(courant, time)
+        write_output_files(self.mesh, directory, base, counter, time, nodal, cell, record_times, wall_time)
+        return nodal, cell
 
     def time_kernel(self, kernel, reps=20):
         """average duration (ms) of one launch of kernel class `kernel` ("grad", "face" or "vol"),
