@@ -71,7 +71,12 @@ enum {
     PLENS_BC_INSULATED_WALL = 7,    /* IBCs/insulated_wall.cc, (vel); the stage-3 heat flux has its
                                      * normal component reversed as documented there (the
                                      * reference's copy loop at line 99 indexes past the 3-vector) */
-    PLENS_BC_ZPG_INFLOW = 8         /* IBCs/zpg_inflow.cc, (vel) with inner density and pressure */
+    PLENS_BC_ZPG_INFLOW = 8,        /* IBCs/zpg_inflow.cc, (vel) with inner density and pressure */
+    PLENS_BC_VARYING_INFLOW = 9     /* IBCs/varying_inflow.cc: p_expr, T_expr, vel_expr (deal.II
+                                     * FunctionParser syntax in x, y, z, t; velocity components
+                                     * separated by ';'), evaluated at the face DoF's location and
+                                     * the solution time at the start of the step; needs
+                                     * node_coords or vertices in the mesh; not in batches */
 };
 
 /* Arithmetic of the flux evaluations. FP64 registers/residual accumulation in both. */
@@ -106,6 +111,9 @@ typedef struct {
                                      * every node from each cell's map instead of reading stored arrays
                                      * (Ja and detJ may then be NULL; if given, create() checks that the
                                      * map reproduces them to 1e-10 relative, else PLENS_E_INVALID). */
+    const double* node_coords;      /* [n_cells][3][n]: DoF locations (dof_locations), nullable; read
+                                     * only for the face nodes of varying-inflow faces (vertices, if
+                                     * given, are used in their absence) */
 } plens_mesh_desc;
 
 /* Gas model ("Navier-Stokes" subsection, custom gas, dgsem/plens.cc:185-223) and blender
@@ -128,6 +136,9 @@ typedef struct {
     int32_t type;      /* PLENS_BC_* */
     double p, T;       /* prescribed pressure / temperature where the type uses them */
     double vel[3];     /* prescribed (inflow) or wall velocity */
+    const char* p_expr;    /* varying inflow: "prescribed p", "prescribed T", "prescribed velocity" */
+    const char* T_expr;    /* (the prm strings; NULL for the other types) */
+    const char* vel_expr;
 } plens_bc;
 
 /* Create a solver handle on the current HIP device. */
@@ -198,6 +209,10 @@ int plens_gpu_ss_error(plens_gpu* h, double* err);
  * The state is not changed; the time step and blender buffers hold the new values afterwards, so
  * a following plens_gpu_ss_error divides by the new steps as write() does. Single handles only.
  * The first call allocates the device buffer of the fields. */
+/* Solution time of the handle (the BCs' cur_time for plens_gpu_calc_rhs; runs and steps set it
+ * from their *t_inout and advance it). */
+int plens_gpu_set_time(plens_gpu* h, double t);
+
 #define PLENS_OUT_NODAL 15
 #define PLENS_OUT_CELL 3
 int plens_gpu_output_fields(plens_gpu* h, double courant, double t, double* nodal, double* cell);

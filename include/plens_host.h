@@ -81,6 +81,12 @@ int plens_mesh_build_curved(int32_t degree, int32_t mapping_degree, int32_t n_ce
 /* Mapping degree q (1 for boxes) and the support points [n_cells][(q+1)^3][3], lexicographic over
  * the GLL(q) lattice of the reference cell (for q = 1: the 8 vertices). */
 int32_t plens_mesh_mapping_degree(const plens_mesh* m);
+/* Evaluate an expression of the varying inflow BC (the same compiler and evaluator the device
+ * uses): n_components expressions separated by ';' at (x, y, z, t) into values; -1 on a syntax
+ * error or another number of components. */
+int plens_expr_eval(const char* expr, int32_t n_components, double x, double y, double z, double t,
+                    double* values);
+
 /* Coordinates of the equispaced (N+1)^3 output lattice of every cell through its map:
  * xyz[n_cells][(N+1)^3][3], lattice points lexicographic (x fastest). */
 int plens_mesh_output_points(const plens_mesh* m, double* xyz);

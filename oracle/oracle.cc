@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
+#include <cctype>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -35,6 +36,170 @@ inline T1 cross(const T1& a, const T1& b) {
 }
 
 constexpr double R0 = 8.314462618;  // modelling/navier_stokes.h:149
+
+// ---------------------------------------------------------------------------------------------
+// Expressions of the varying inflow BC (IBCs/varying_inflow.cc): deal.II's FunctionParser over
+// muParser, variables x, y, z, t; evaluated directly while parsing. muParser's precedence, lowest
+// first: ?: (right associative), ||, &&, comparisons, + -, * /, unary signs, ^ (right
+// associative, above the signs: -2^2 = -4). Functions of muParser (sin ... abs, log = ln, min,
+// max, sum, avg, atan2) and deal.II's additions (if, pow, floor, ceil, int, cot, csc, sec, erfc);
+// constants _pi and _e.
+// ---------------------------------------------------------------------------------------------
+struct ExprEval {
+    const std::string& s;
+    size_t i;
+    double x, y, z, t;
+    bool ok = true;
+    void ws() { while (i < s.size() && std::isspace((unsigned char)s[i])) ++i; }
+    bool eat(const char* tok) {
+        ws();
+        const size_t n = std::strlen(tok);
+        if (s.compare(i, n, tok) == 0) { i += n; return true; }
+        return false;
+    }
+    double expr() {  // ternary
+        const double c = lor();
+        if (eat("?")) {
+            const double a = expr();
+            if (!eat(":")) ok = false;
+            const double b = expr();
+            return c != 0 ? a : b;
+        }
+        return c;
+    }
+    double lor() { double a = land(); while (eat("||")) { const double b = land(); a = (a != 0 || b != 0); } return a; }
+    double land() { double a = cmp(); while (eat("&&")) { const double b = cmp(); a = (a != 0 && b != 0); } return a; }
+    double cmp() {
+        double a = sum();
+        for (;;) {
+            if (eat("<=")) { const double b = sum(); a = a <= b; }
+            else if (eat(">=")) { const double b = sum(); a = a >= b; }
+            else if (eat("==")) { const double b = sum(); a = a == b; }
+            else if (eat("!=")) { const double b = sum(); a = a != b; }
+            else if (eat("<")) { const double b = sum(); a = a < b; }
+            else if (eat(">")) { const double b = sum(); a = a > b; }
+            else return a;
+        }
+    }
+    double sum() {
+        double a = prod();
+        for (;;) {
+            if (eat("+")) a = a + prod();
+            else if (eat("-")) a = a - prod();
+            else return a;
+        }
+    }
+    double prod() {
+        double a = sign();
+        for (;;) {
+            if (eat("*")) a = a * sign();
+            else if (eat("/")) a = a / sign();
+            else return a;
+        }
+    }
+    double sign() {
+        if (eat("-")) return -sign();
+        if (eat("+")) return sign();
+        const double b = atom();
+        if (eat("^")) return std::pow(b, sign());
+        return b;
+    }
+    double atom() {
+        ws();
+        if (eat("(")) { const double v = expr(); if (!eat(")")) ok = false; return v; }
+        if (i < s.size() && (std::isdigit((unsigned char)s[i]) || s[i] == '.')) {
+            size_t used = 0;
+            const double v = std::stod(s.substr(i), &used);
+            i += used;
+            return v;
+        }
+        size_t b = i;
+        while (i < s.size() && (std::isalnum((unsigned char)s[i]) || s[i] == '_')) ++i;
+        const std::string id = s.substr(b, i - b);
+        if (id.empty()) { ok = false; return 0; }
+        if (eat("(")) {
+            std::vector<double> a;
+            ws();
+            if (!eat(")")) {
+                do a.push_back(expr()); while (eat(","));
+                if (!eat(")")) ok = false;
+            }
+            return call(id, a);
+        }
+        if (id == "x") return x;
+        if (id == "y") return y;
+        if (id == "z") return z;
+        if (id == "t") return t;
+        if (id == "_pi") return M_PI;
+        if (id == "_e") return M_E;
+        ok = false;
+        return 0;
+    }
+    double call(const std::string& f, const std::vector<double>& a) {
+        const size_t n = a.size();
+        auto one = [&](double v) { if (n != 1) ok = false; return v; };
+        const double a0 = n ? a[0] : 0;
+        if (f == "sin") return one(std::sin(a0));
+        if (f == "cos") return one(std::cos(a0));
+        if (f == "tan") return one(std::tan(a0));
+        if (f == "asin") return one(std::asin(a0));
+        if (f == "acos") return one(std::acos(a0));
+        if (f == "atan") return one(std::atan(a0));
+        if (f == "sinh") return one(std::sinh(a0));
+        if (f == "cosh") return one(std::cosh(a0));
+        if (f == "tanh") return one(std::tanh(a0));
+        if (f == "asinh") return one(std::asinh(a0));
+        if (f == "acosh") return one(std::acosh(a0));
+        if (f == "atanh") return one(std::atanh(a0));
+        if (f == "log2") return one(std::log2(a0));
+        if (f == "log10") return one(std::log10(a0));
+        if (f == "log" || f == "ln") return one(std::log(a0));
+        if (f == "exp") return one(std::exp(a0));
+        if (f == "sqrt") return one(std::sqrt(a0));
+        if (f == "sign") return one(a0 > 0 ? 1.0 : (a0 < 0 ? -1.0 : 0.0));
+        if (f == "rint") return one(std::rint(a0));
+        if (f == "abs") return one(std::fabs(a0));
+        if (f == "floor") return one(std::floor(a0));
+        if (f == "ceil") return one(std::ceil(a0));
+        if (f == "int") return one((double)(long long)a0);
+        if (f == "cot") return one(1 / std::tan(a0));
+        if (f == "csc") return one(1 / std::sin(a0));
+        if (f == "sec") return one(1 / std::cos(a0));
+        if (f == "erfc") return one(std::erfc(a0));
+        if (f == "pow" && n == 2) return std::pow(a[0], a[1]);
+        if (f == "atan2" && n == 2) return std::atan2(a[0], a[1]);
+        if (f == "if" && n == 3) return a[0] != 0 ? a[1] : a[2];
+        if (n >= 1 && (f == "min" || f == "max" || f == "sum" || f == "avg")) {
+            double r = a[0];
+            for (size_t k = 1; k < n; ++k) {
+                if (f == "min") r = std::min(r, a[k]);
+                else if (f == "max") r = std::max(r, a[k]);
+                else r += a[k];
+            }
+            return f == "avg" ? r / n : r;
+        }
+        ok = false;
+        return 0;
+    }
+};
+
+// value of expression `src` at (x, y, z, t); NaN on a syntax error
+double eval_expression(const std::string& src, double x, double y, double z, double t) {
+    ExprEval e{src, 0, x, y, z, t};
+    const double v = e.expr();
+    e.ws();
+    return (e.ok && e.i == src.size()) ? v : std::nan("");
+}
+
+// components of "a; b; c"
+std::vector<std::string> split_components(const char* s) {
+    std::vector<std::string> out(1);
+    for (; s && *s; ++s) {
+        if (*s == ';') out.emplace_back();
+        else out.back() += *s;
+    }
+    return out;
+}
 
 // ---------------------------------------------------------------------------------------------
 // NavierStokes restatement (modelling/navier_stokes.{h,cc})
@@ -710,6 +875,8 @@ struct oracle {
     std::vector<double> s1, s2, s3;                     // surface terms [var][c][f][q]
     std::vector<double> ws1;                            // entropy variables of s1 (default build)
     std::vector<double> verts;                          // cell vertices [c][8][3]
+    double bc_time = 0;                                 // BC::time_ = cur_time of the step
+    std::vector<std::vector<std::string>> bexpr;       // varying inflow: p, T, u, v, w per bc
     int mq = 1;                                         // degree of the MappingQGeneric
     std::vector<double> sup, tq;                        // support points [c][(mq+1)^3][3], GLL(mq)
     std::vector<double> rhoE_old;                       // rho E at the start of the last step
@@ -739,6 +906,31 @@ struct oracle {
         T1 t;
         for (int d = 0; d < 3; ++d) t[d] = normals[(((size_t)c * 6 + f) * nf + q) * 3 + d];
         return t;
+    }
+
+    // location of node `node` of cell c (the DoF location): the map at the GLL point
+    T1 node_location(int c, int node) const {
+        const double xi[3] = {x1[node % np], x1[(node / np) % np], x1[node / (np * np)]};
+        T1 X;
+        if (mq == 1) {
+            const double* cv = &verts[(size_t)c * 24];
+            for (int v = 0; v < 8; ++v) {
+                double w = 1;
+                for (int a = 0; a < 3; ++a) w *= ((v >> a) & 1) ? xi[a] : 1.0 - xi[a];
+                for (int d = 0; d < 3; ++d) X[d] += w * cv[3 * v + d];
+            }
+            return X;
+        }
+        const int qp = mq + 1;
+        const double* S = &sup[(size_t)c * qp * qp * qp * 3];
+        for (int k = 0; k < qp; ++k)
+            for (int j = 0; j < qp; ++j)
+                for (int i = 0; i < qp; ++i) {
+                    const double l = (double)(lagrange_val(tq, i, xi[0]) * lagrange_val(tq, j, xi[1]) *
+                                              lagrange_val(tq, k, xi[2]));
+                    for (int d = 0; d < 3; ++d) X[d] += l * S[3 * ((size_t)i + qp * ((size_t)j + (size_t)qp * k)) + d];
+                }
+        return X;
     }
 
     // Jacobian J[a][b] = dx_a/dxi_b of cell c's map at reference point xi: trilinear from the 8
@@ -925,6 +1117,19 @@ struct oracle {
                     g[4] = in[4];
                     for (int d = 0; d < 3; ++d) g[1 + d] = sm[d] - 2 * smn * nrm[d];
                 }
+                break;
+            }
+            case 9: {  // varying inflow (varying_inflow.cc): uniform inflow's ghosts, state prescribed
+                       // by the expressions at the DoF location and BC::time_
+                const std::vector<std::string>& ex = bexpr[bid[(size_t)c * 6 + f]];
+                const T1 X = node_location(c, face_map(N, f, q));
+                double val[5];
+                for (int k = 0; k < 5; ++k) val[k] = eval_expression(ex[k], X[0], X[1], X[2], bc_time);
+                T1 v; v[0] = val[2]; v[1] = val[3]; v[2] = val[4];
+                State pr;
+                ns.prim_to_cons(val[0] / (ns.R() * val[1]), v, val[0], pr);
+                if (stage == 2) g = pr;
+                else for (int k = 0; k < 5; ++k) g[k] = 2 * pr[k] - in[k];
                 break;
             }
             case 8: {  // ZPG inflow (zpg_inflow.cc): prescribed velocity, inner density and pressure
@@ -1516,6 +1721,7 @@ struct oracle {
 
     // ---- PLENS::update + update_rk4, dgsem/plens.cc:4334-4403, 4484-4518 --------------------------
     int rk4_step(double Co, double* t, double fixed_dt = 0) {
+        if (t) bc_time = *t;  // BC::set_time(cur_time), dgsem/plens.cc:1798
         // gcrk_cvars = g_cvars (the state lives in Q throughout)
         int e = calc_rhs();
         if (e) return e;
@@ -1581,6 +1787,18 @@ int oracle_create(int32_t n_cells, int32_t degree, const double* vertices, const
                phys->inv_surf_flux};
     o->ns.vscheme = phys->inv_vol_flux;
     o->bcs.assign(bcs, bcs + n_bcs);
+    o->bexpr.assign(std::max(n_bcs, 0), {});
+    for (int i = 0; i < n_bcs; ++i) {
+        if (bcs[i].type != 9) continue;
+        if (!bcs[i].p_expr || !bcs[i].T_expr || !bcs[i].vel_expr) { delete o; return -1; }
+        std::vector<std::string> e = {bcs[i].p_expr, bcs[i].T_expr};
+        for (const auto& v : split_components(bcs[i].vel_expr)) e.push_back(v);
+        if (e.size() != 5) { delete o; return -1; }
+        o->bexpr[i] = e;
+    }
+    for (size_t k = 0; k < o->bcs.size(); ++k) {  // the pointers are the caller's; keep copies only
+        o->bcs[k].p_expr = o->bcs[k].T_expr = o->bcs[k].vel_expr = nullptr;
+    }
     o->nbr.assign(face_nbr, face_nbr + (size_t)n_cells * 6);
     o->bid.assign(face_bid, face_bid + (size_t)n_cells * 6);
     gll(degree, o->x1, o->w);
@@ -1673,12 +1891,29 @@ int oracle_calc_dt(oracle* o, double courant, double* dt) {
 
 int oracle_rk4_step(oracle* o, double courant, double* t) { return o->rk4_step(courant, t); }
 
+int oracle_node_location(const oracle* o, int32_t c, int32_t node, double* xyz) {
+    if (c < 0 || c >= o->nc || node < 0 || node >= o->n) return -1;
+    const T1 X = o->node_location(c, node);
+    for (int d = 0; d < 3; ++d) xyz[d] = X[d];
+    return 0;
+}
+
+int oracle_set_time(oracle* o, double t) {
+    o->bc_time = t;
+    return 0;
+}
+
+double oracle_eval_expression(const char* expr, double x, double y, double z, double t) {
+    return eval_expression(expr ? expr : "", x, y, z, t);
+}
+
 // PLENS::write() fields (dgsem/plens.cc:4170-4322): write() follows calc_rhs(true) and
 // calc_time_step() inside the first RK stage (4341-4350); post_process (3950-3963) gives p, T and
 // the velocity at the DoFs; DataOut evaluates every field's FE interpolant at the equispaced
 // (N+1)^3 points of each cell (build_patches with N subdivisions), here by direct summation over
 // all nodal basis functions.
 int oracle_output_fields(oracle* o, double courant, double t, double* nodal, double* cell) {
+    o->bc_time = t;
     int e = o->calc_rhs();
     if (e) return e;
     o->calc_time_step(courant, t);

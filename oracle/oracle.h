@@ -38,7 +38,8 @@ typedef struct {
     int32_t inv_vol_flux;        /* 0 Chandrashekhar, 1 Kennedy-Gruber, 2 Ismail-Roe */
 } oracle_physics;
 
-typedef struct { int32_t type; double p, T, vel[3]; } oracle_bc; /* type as PLENS_BC_* */
+/* type as PLENS_BC_*; the expressions of a varying inflow (9) as plens_bc, copied at create */
+typedef struct { int32_t type; double p, T, vel[3]; const char *p_expr, *T_expr, *vel_expr; } oracle_bc;
 
 /* vertices [n_cells][8][3] (deal.II order), face_nbr/face_bid [n_cells][6] as plens_mesh_desc */
 int oracle_create(int32_t n_cells, int32_t degree, const double* vertices, const int32_t* face_nbr,
@@ -50,6 +51,12 @@ int oracle_create_mapped(int32_t n_cells, int32_t degree, int32_t map_degree, co
                          const int32_t* face_nbr, const int32_t* face_bid, const oracle_physics* phys,
                          const oracle_bc* bcs, int32_t n_bcs, int32_t transpose_y, oracle** out);
 void oracle_destroy(oracle* o);
+/* BC time (cur_time) of calc_rhs; steps set it from their time argument */
+int oracle_set_time(oracle* o, double t);
+/* location of node `node` of cell c (through the cell's map) */
+int oracle_node_location(const oracle* o, int32_t c, int32_t node, double* xyz);
+/* value of one expression of the varying inflow BC; NaN on a syntax error */
+double oracle_eval_expression(const char* expr, double x, double y, double z, double t);
 /* fields of PLENS::write() for the current state: nodal [15][n_cells][(N+1)^3] on the equispaced
  * lattice (txx..qz, rho, p, T, u, v, w), cell [3][n_cells] (alpha, alpha_d, loc_dt) */
 int oracle_output_fields(oracle* o, double courant, double t, double* nodal, double* cell);

@@ -23,7 +23,8 @@ class OPhys(C.Structure):
 
 
 class OBC(C.Structure):
-    _fields_ = [("type", C.c_int32), ("p", C.c_double), ("T", C.c_double), ("vel", C.c_double * 3)]
+    _fields_ = [("type", C.c_int32), ("p", C.c_double), ("T", C.c_double), ("vel", C.c_double * 3),
+                ("p_expr", C.c_char_p), ("T_expr", C.c_char_p), ("vel_expr", C.c_char_p)]
 
 
 _lib = None
@@ -46,6 +47,9 @@ def load():
                                            C.POINTER(OBC), C.c_int32, C.c_int32, C.POINTER(vp)]),
         "oracle_destroy": (None, [vp]),
         "oracle_output_fields": (C.c_int, [vp, C.c_double, C.c_double, _dp, _dp]),
+        "oracle_set_time": (C.c_int, [vp, C.c_double]),
+        "oracle_node_location": (C.c_int, [vp, C.c_int32, C.c_int32, _dp]),
+        "oracle_eval_expression": (C.c_double, [C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_double]),
         "oracle_set_threads": (C.c_int, [C.c_int32]),
         "oracle_set_state": (C.c_int, [vp, _dp]),
         "oracle_get_state": (C.c_int, [vp, _dp]),
@@ -104,6 +108,7 @@ class OraclePlens:
         arr = (OBC * max(len(bcs), 1))()
         for i, b in enumerate(bcs):
             arr[i].type, arr[i].p, arr[i].T = b.type, b.p, b.T
+            arr[i].p_expr, arr[i].T_expr, arr[i].vel_expr = b.p_expr, b.T_expr, b.vel_expr
             for d in range(3):
                 arr[i].vel[d] = b.vel[d]
         self._bcs = arr
@@ -192,6 +197,9 @@ class OraclePlens:
         if self.lib.oracle_loc_dt(self._h, out.ctypes.data_as(_dp)):
             raise RuntimeError("no step taken yet")
         return out
+
+    def set_time(self, t):
+        self.lib.oracle_set_time(self._h, t)
 
     def output_fields(self, courant, t=0.0):
         nodal = np.empty((15, self.n_dof))

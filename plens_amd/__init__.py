@@ -5,5 +5,5 @@ of include/plens_gpu.h and the host setup of include/plens_host.h.
 """
 from . import _lib  # noqa: F401
 from .batch import GpuBatch  # noqa: F401
-from .cases import CONFIGS, Mesh, batch_instance, cylinder_flare, nose_cone, periodic_cube, shock_tube, wall_flow, with_fluxes, with_gradient_variables, with_precision  # noqa: F401
+from .cases import CONFIGS, Mesh, batch_instance, cylinder_flare, nose_cone, periodic_cube, shock_tube, varying_inflow_tube, wall_flow, with_fluxes, with_gradient_variables, with_precision  # noqa: F401
 from .solver import GpuPlens, PositivityError  # noqa: F401

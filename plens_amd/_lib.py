@@ -20,7 +20,7 @@ FLUX = {"HLLC": 0, "Rusanov": 1, "AUSM+-up": 2, "Chandrashekhar": 3, "Rusanov-HL
         "Rusanov-AUSM+-up": 5, "Modified SW": 6, "KG": 7, "Rusanov-KG": 8, "IR": 9}
 VOL_FLUX = {"Chandrashekhar": 0, "KG": 1, "IR": 2}
 BC_PERIODIC, BC_UNIFORM_INFLOW, BC_OUTFLOW, BC_UNIFORM_TEMP_WALL = 0, 1, 2, 3
-BC_FREE, BC_SYMMETRY, BC_EMPTY, BC_INSULATED_WALL, BC_ZPG_INFLOW = 4, 5, 6, 7, 8
+BC_FREE, BC_SYMMETRY, BC_EMPTY, BC_INSULATED_WALL, BC_ZPG_INFLOW, BC_VARYING_INFLOW = 4, 5, 6, 7, 8, 9
 PREC_FP64, PREC_FP32_BULK = 0, 1
 GRAD_CONSERVATIVE, GRAD_ENTROPY = 0, 1
 
@@ -32,7 +32,7 @@ _ip = C.POINTER(C.c_int32)
 class MeshDesc(C.Structure):
     _fields_ = [("n_cells", C.c_int32), ("degree", C.c_int32), ("Ja", _dp), ("detJ", _dp),
                 ("face_normals", _dp), ("min_vertex_dist", _dp), ("face_nbr", _ip), ("face_bid", _ip),
-                ("vertices", _dp)]
+                ("vertices", _dp), ("node_coords", _dp)]
 
 
 class Physics(C.Structure):
@@ -46,7 +46,8 @@ class Physics(C.Structure):
 
 
 class BC(C.Structure):
-    _fields_ = [("type", C.c_int32), ("p", C.c_double), ("T", C.c_double), ("vel", C.c_double * 3)]
+    _fields_ = [("type", C.c_int32), ("p", C.c_double), ("T", C.c_double), ("vel", C.c_double * 3),
+                ("p_expr", C.c_char_p), ("T_expr", C.c_char_p), ("vel_expr", C.c_char_p)]
 
 
 class BoxParams(C.Structure):
@@ -72,7 +73,7 @@ GPU_SYMBOLS = [
     "plens_gpu_last_failure", "plens_gpu_set_tuning", "plens_gpu_kernel_times", "plens_gpu_strerror",
     "plens_gpu_last_run_ms", "plens_gpu_run_profiled", "plens_gpu_stream_copy", "plens_gpu_time_kernel",
     "plens_gpu_set_local_stepping", "plens_gpu_ss_error", "plens_gpu_output_fields", "plens_gpu_write_checkpoint",
-    "plens_gpu_read_checkpoint",
+    "plens_gpu_read_checkpoint", "plens_gpu_set_time",
     "plens_gpu_batch_create", "plens_gpu_batch_destroy", "plens_gpu_batch_run", "plens_gpu_batch_get_state",
     "plens_gpu_batch_last_run_ms", "plens_gpu_batch_last_failure", "plens_gpu_batch_n_groups",
     "plens_gpu_batch_group_info",
@@ -84,6 +85,7 @@ HOST_SYMBOLS = [
     "plens_mesh_min_vertex_dist", "plens_basis_1d", "plens_prim_to_cons", "plens_ic_density_wave",
     "plens_ic_normal_shock", "plens_ic_uniform", "plens_face_node", "plens_mesh_build_curved",
     "plens_mesh_mapping_degree", "plens_mesh_support_points", "plens_mesh_output_points", "plens_output_write",
+    "plens_expr_eval",
 ]
 MLP_SYMBOLS = [
     "plens_mlp_n_params", "plens_mlp_create", "plens_mlp_destroy", "plens_mlp_get_params",
@@ -125,6 +127,8 @@ def load():
         "plens_gpu_set_local_stepping": (C.c_int, [vp, C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_double]),
         "plens_gpu_ss_error": (C.c_int, [vp, _dp]),
         "plens_gpu_output_fields": (C.c_int, [vp, C.c_double, C.c_double, _dp, _dp]),
+        "plens_gpu_set_time": (C.c_int, [vp, C.c_double]),
+        "plens_expr_eval": (C.c_int, [C.c_char_p, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
         "plens_gpu_write_checkpoint": (C.c_int, [vp, C.c_char_p, C.c_double, C.c_int64, C.c_int64]),
         "plens_gpu_read_checkpoint": (C.c_int, [vp, C.c_char_p, _dp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "plens_gpu_batch_create": (C.c_int, [C.POINTER(vp), C.c_int32, C.c_char_p, C.POINTER(vp)]),

@@ -37,16 +37,18 @@ def build(verbose=False):
     """Compile libplens_gpu.so; raises on failure."""
     obj_host = os.path.join(CSRC, "host_mesh.o")
     obj_out = os.path.join(CSRC, "host_output.o")
+    obj_expr = os.path.join(CSRC, "expr.o")
     obj_dev = os.path.join(CSRC, "plens_gpu.o")
     obj_mlp = os.path.join(CSRC, "mlp.o")
     cmds = [
         ["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", f"-I{INCLUDE}", "-c",
          os.path.join(CSRC, "host_mesh.cc"), "-o", obj_host],
         ["g++", "-O2", "-std=c++17", "-fPIC", f"-I{INCLUDE}", "-c", os.path.join(CSRC, "host_output.cc"), "-o", obj_out],
+        ["g++", "-O2", "-std=c++17", "-fPIC", f"-I{INCLUDE}", "-c", os.path.join(CSRC, "expr.cc"), "-o", obj_expr],
         [hipcc(), *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c", os.path.join(CSRC, "plens_gpu.hip"),
          "-o", obj_dev],
         [hipcc(), *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c", os.path.join(CSRC, "mlp.hip"), "-o", obj_mlp],
-        [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", obj_dev, obj_mlp, obj_host, obj_out, "-o", LIB + ".tmp"],
+        [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", obj_dev, obj_mlp, obj_host, obj_out, obj_expr, "-o", LIB + ".tmp"],
     ]
     for c in cmds:
         if verbose:
@@ -57,7 +59,7 @@ def build(verbose=False):
         if verbose and r.stderr.strip():
             print(r.stderr, file=sys.stderr)
     os.replace(LIB + ".tmp", LIB)
-    for o in (obj_host, obj_out, obj_dev, obj_mlp):
+    for o in (obj_host, obj_out, obj_expr, obj_dev, obj_mlp):
         if os.path.exists(o):
             os.remove(o)
     return LIB

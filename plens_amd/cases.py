@@ -116,6 +116,7 @@ class Mesh:
         d.face_bid = self.face_bid.ctypes.data_as(ip)
         # a curved (q > 1) map is not trilinear: its metric terms are always handed over stored
         d.vertices = self.vertices.ctypes.data_as(dp) if (with_vertices and self.mapping_degree == 1) else None
+        d.node_coords = self.node_coords.ctypes.data_as(dp)
         return d
 
     def close(self):
@@ -147,11 +148,15 @@ def make_physics(gamma=1.4, prandtl=0.72, mu0=1e-3, T0=1.0, S=SUTH_RATIO, flux="
     return ph
 
 
-def make_bc(type_, p=0.0, T=0.0, vel=(0.0, 0.0, 0.0)):
+def make_bc(type_, p=0.0, T=0.0, vel=(0.0, 0.0, 0.0), p_expr=None, T_expr=None, vel_expr=None):
+    """plens_bc; a varying inflow takes the prm strings "prescribed p", "prescribed T" and
+    "prescribed velocity" (components separated by ';')"""
     b = L.BC()
     b.type, b.p, b.T = type_, p, T
     for d in range(3):
         b.vel[d] = vel[d]
+    for name, v in (("p_expr", p_expr), ("T_expr", T_expr), ("vel_expr", vel_expr)):
+        setattr(b, name, v.encode() if v is not None else None)
     return b
 
 
@@ -385,6 +390,24 @@ def nose_cone(degree=3, mapping_degree=2, n_sphere=4, n_cone=6, n_normal=3, n_az
     Q = _smooth_state(mesh, g, 1.0, (u_inf, 0.0, 0.0), p_inf, 0.05, (2.0, 1.0, 0.5))
     return Case("nose_cone", mesh, ph, bcs, Q, 0.4, steps,
                 {"degree": degree, "mapping_degree": mapping_degree, "cells": (n0, n_normal, n_azim)})
+
+
+def varying_inflow_tube(degree=3, nx=12, ny=2, nz=2, perturb=0.1, seed=4, flux="HLLC", steps=20,
+                        p_expr="1 + 0.1*sin(6.283185307179586*y)*cos(3*t)",
+                        T_expr="(x < 1e-12 && z > 0.1) ? 1.05 : 1 + 0.02*t",
+                        vel_expr="2*(1 + 0.05*cos(6.283185307179586*(y + z)) + 0.5*t^2); 0.1*sin(t); _pi/100"):
+    """Box [0,1]x[0,1/4]^2 (perturbed vertices) with a varying inflow at x = 0 whose p, T and
+    velocity depend on y, z and t (the defaults exercise functions, the ternary, the logical and
+    power operators and a constant), outflow at x = 1, periodic in y and z; the initial state is
+    the uniform flow rho = 1, p = 1, u = 2 (Re 1e3 per length)."""
+    g = 1.4
+    mesh = Mesh(degree, nx, ny, nz, (0, 0, 0), (1.0, 0.25, 0.25), (0, 1, 1), [0, 1, -1, -1, -1, -1], (-1, 2, 2),
+                perturb_frac=perturb, seed=seed)
+    ph = make_physics(mu0=2.0 / 1e3, T0=1.0, S=SUTH_RATIO, flux=flux)
+    bcs = [make_bc(L.BC_VARYING_INFLOW, p_expr=p_expr, T_expr=T_expr, vel_expr=vel_expr),
+           make_bc(L.BC_OUTFLOW, p=1.0), make_bc(L.BC_PERIODIC)]
+    Q = _smooth_state(mesh, g, 1.0, (2.0, 0.0, 0.0), 1.0, 0.0, (0.0, 0.0, 0.0))
+    return Case("varying_inflow_tube", mesh, ph, bcs, Q, 0.4, steps, {"degree": degree, "cells": (nx, ny, nz)})
 
 
 def batch_instance(i, seed0=100, min_n=4, max_n=32, flux="HLLC"):

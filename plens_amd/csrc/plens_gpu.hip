@@ -29,6 +29,7 @@
 #include <string>
 #include <vector>
 
+#include "expr.h"
 #include "flux.h"
 #include "plens_gpu.h"
 #include "plens_host.h"
@@ -121,6 +122,14 @@ struct KP {
     Gas gas;
     int scheme;              // PLENS_FLUX_*
     int vscheme;             // PLENS_VOL_FLUX_*
+    // varying inflow: per face slot (-1: none), prescribed states [slot][n_f][5] of the current
+    // step, face-node coordinates [slot][3][n_f], bc of every slot, 5 programs per bc
+    const int* vslot;
+    double* vin;
+    const double* vxyz;
+    const int* vbc;
+    const ExprCode* vcode;
+    int nvslot;
     double thr, sharp, amin, amax, wlimit, courant;
     const double* cfl;       // Courant number of every instance (batches; nullptr: courant)
     double iw0, iwN;  // 1 / w_0, 1 / w_N of the GLL rule
@@ -248,13 +257,16 @@ __device__ __forceinline__ void cons_to_entropy(const double u[5], double gma, d
 
 // BC ghost state (IBCs/*.cc): stage 1 and 3 share the state rule, stage 2 has its own; n is the
 // outward unit normal of the face node
+// outward unit normal of the face node; vin: the node's prescribed state of a varying inflow
 __device__ void bc_ghost_state(const BCDev& b, int stage, const Gas& g, const double u[5], const double n[3],
-                               double ug[5]) {
-    if (b.type == PLENS_BC_UNIFORM_INFLOW) {
+                               double ug[5], const double* vin = nullptr) {
+    if (b.type == PLENS_BC_UNIFORM_INFLOW || b.type == PLENS_BC_VARYING_INFLOW) {
+        // varying_inflow.cc: uniform inflow's ghosts with the state prescribed at the node
+        const double* pr = b.type == PLENS_BC_VARYING_INFLOW ? vin : b.cons_pr;
         if (stage == 2) {
-            for (int v = 0; v < 5; ++v) ug[v] = b.cons_pr[v];
+            for (int v = 0; v < 5; ++v) ug[v] = pr[v];
         } else {
-            for (int v = 0; v < 5; ++v) ug[v] = 2.0 * b.cons_pr[v] - u[v];
+            for (int v = 0; v < 5; ++v) ug[v] = 2.0 * pr[v] - u[v];
         }
     } else if (b.type == PLENS_BC_OUTFLOW) {
         for (int v = 0; v < 5; ++v) ug[v] = u[v];
@@ -315,6 +327,33 @@ __device__ void bc_ghost_state(const BCDev& b, int stage, const Gas& g, const do
             for (int v = 0; v < 5; ++v) ug[v] = 2.0 * pr[v] - u[v];
         }
     }
+}
+
+// prescribed state of face node q of face task cf when its BC is a varying inflow
+__device__ __forceinline__ const double* vin_node(const KP& p, int b, long long cf, int q, int NF) {
+    return p.bcs[b].type == PLENS_BC_VARYING_INFLOW ? p.vin + ((long long)p.vslot[cf] * NF + q) * 5 : nullptr;
+}
+
+// k_vinflow: prescribed states of the varying-inflow face nodes for the current step
+// (VaryingInflow::calculate_prescribed_state, IBCs/varying_inflow.cc): p, T and velocity from the
+// compiled expressions at the node's location and the solution time at the start of the step (the
+// BCs' time is cur_time for all stages, dgsem/plens.cc:1798), rho = p / (R T).
+__global__ void __launch_bounds__(256) k_vinflow(KP p, int NF) {
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (long long)p.nvslot * NF) return;
+    const int slot = (int)(gid / NF), q = (int)(gid - (long long)slot * NF);
+    const double x = p.vxyz[((long long)slot * 3 + 0) * NF + q];
+    const double y = p.vxyz[((long long)slot * 3 + 1) * NF + q];
+    const double z = p.vxyz[((long long)slot * 3 + 2) * NF + q];
+    const double t = p.tabs[0];
+    const ExprCode* c = p.vcode + 5 * p.vbc[slot];
+    const double pr = expr_eval(c[0], x, y, z, t);
+    const double T = expr_eval(c[1], x, y, z, t);
+    const double vel[3] = {expr_eval(c[2], x, y, z, t), expr_eval(c[3], x, y, z, t), expr_eval(c[4], x, y, z, t)};
+    double cons[5];
+    prim_to_cons(p.gas.gma, pr / (p.gas.R * T), vel, pr, cons);
+    double* dst = p.vin + gid * 5;
+    for (int v = 0; v < 5; ++v) dst[v] = cons[v];
 }
 
 // basis tables into LDS; sW holds the reciprocal GLL weights 1/w_i (every use divides by w_i)
@@ -524,7 +563,7 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
         } else {
             double ug[5], nf[3];
             for (int d = 0; d < 3; ++d) nf[d] = p.nrm[((long long)(c * 6 + f) * 3 + d) * (NP * NP) + q];
-            bc_ghost_state(p.bcs[b], 1, gs, u, nf, ug);
+            bc_ghost_state(p.bcs[b], 1, gs, u, nf, ug, vin_node(p, b, (long long)c * 6 + f, q, NP * NP));
             for (int v = 0; v < 5; ++v) us[v] = 0.5 * (u[v] + ug[v]);
         }
         // g -= s (u* Ja - u Ja)[gd] / w_end, evaluated as (u* - u) (s Ja[gd] / w_end); in entropy
@@ -864,7 +903,7 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
         load_state(p.Q, p.ndof, gn, ug);
         bl = interior ? 0.5 * (p.alpha[c] + p.alpha[nb]) : p.alpha[c];
     } else {
-        bc_ghost_state(p.bcs[b], 2, gs, u, n, ug);
+        bc_ghost_state(p.bcs[b], 2, gs, u, n, ug, vin_node(p, b, cf, q, NF));
         bl = p.alpha[c];
     }
     bl = bl <= 0.0 ? 0.0 : (bl <= 1.0 ? bl : 1.0);  // set_flux_blender_value
@@ -940,7 +979,7 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
             const TF qn = avg[6] * nt[0] + avg[7] * nt[1] + avg[8] * nt[2];
             for (int d = 0; d < 3; ++d) avg[6 + d] = av[6 + d] - TF(2) * qn * nt[d];
         }
-        if (p.bcs[b].type != PLENS_BC_OUTFLOW) bc_ghost_state(p.bcs[b], 3, gs, u, n, ug);
+        if (p.bcs[b].type != PLENS_BC_OUTFLOW) bc_ghost_state(p.bcs[b], 3, gs, u, n, ug, vin_node(p, b, cf, q, NF));
         else for (int v = 0; v < 5; ++v) ug[v] = u[v];
         for (int v = 0; v < 5; ++v) ugt[v] = (TF)ug[v];
     }
@@ -1461,6 +1500,11 @@ struct plens_gpu {
     double* dt_lts = nullptr;    // local time stepping: per-cell steps (allocated on request)
     double* ss_part = nullptr;   // steady-state error: per-cell integrals, then the two sums
     double* post = nullptr;      // output fields (allocated by the first output call)
+    int* vslot = nullptr;        // varying inflow: face slots, prescribed states, coordinates,
+    double *vin = nullptr, *vxyz = nullptr;  // bc of every slot, compiled expressions
+    int* vbc = nullptr;
+    ExprCode* vcode = nullptr;
+    int nvslot = 0;
     uint64_t mesh_fp = 0;        // fingerprint of the mesh (checkpoints)
     long long steps_taken = 0;
     bool use_cmap = false;
@@ -1687,9 +1731,22 @@ KP stage_params(plens_gpu* h, int s, bool from_b, double courant) {
     return kp;
 }
 
+// prescribed varying-inflow states for the solution time held by the device
+int launch_vinflow(plens_gpu* h, const KP& kp) {
+    if (kp.nvslot <= 0) return PLENS_OK;
+    const long long nt = (long long)kp.nvslot * h->NF;
+    hipLaunchKernelGGL(k_vinflow, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, h->stream, kp, h->NF);
+    HIPCHK(hipGetLastError());
+    return PLENS_OK;
+}
+
 int launch_step(plens_gpu* h, bool from_b, double courant, hipEvent_t* ev = nullptr) {
     for (int s = 1; s <= 5; ++s) {
         KP kp = stage_params(h, s, from_b, courant);
+        if (s == 1) {  // the BCs' time is the step's start time for all stages
+            const int e = launch_vinflow(h, kp);
+            if (e) return e;
+        }
         const int e = launch_stage(h, kp, ev ? ev + 6 * (s - 1) : nullptr);
         if (e) return e;
     }
@@ -1898,7 +1955,7 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
             if (b >= n_bcs) return PLENS_E_INVALID;
             const bool needs_nbr = b < 0 || bcs[b].type == PLENS_BC_PERIODIC;
             if (needs_nbr && (nb < 0 || nb >= nc)) return PLENS_E_INVALID;
-            if (b >= 0 && (bcs[b].type < 0 || bcs[b].type > PLENS_BC_ZPG_INFLOW)) return PLENS_E_INVALID;
+            if (b >= 0 && (bcs[b].type < 0 || bcs[b].type > PLENS_BC_VARYING_INFLOW)) return PLENS_E_INVALID;
         }
     plens_gpu* h = new (std::nothrow) plens_gpu;
     if (!h) return PLENS_E_NOMEM;
@@ -1974,6 +2031,63 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
                 hbas.Q[i * kMaxNP + j] = w[i] * D[i * NP + j];  // ref_Q_1d = w_i D_ij
                 hbas.C1[i * kMaxNP + j] = C1[i * NP + j];
             }
+        }
+    }
+    // varying inflow (set_BC, dgsem/plens.cc:1392-1408): compiled p, T and velocity expressions of
+    // every such BC, one slot per boundary face with it, and the locations of its face nodes (the
+    // DoF locations: node_coords, else the trilinear map of the vertices)
+    {
+        std::vector<ExprCode> codes((size_t)std::max(n_bcs, 1) * 5);
+        bool any = false;
+        for (int i = 0; i < n_bcs; ++i) {
+            if (bcs[i].type != PLENS_BC_VARYING_INFLOW) continue;
+            any = true;
+            if (!bcs[i].p_expr || !bcs[i].T_expr || !bcs[i].vel_expr) return fail(PLENS_E_INVALID);
+            if (!expr_compile(bcs[i].p_expr, &codes[5 * i], nullptr, 0) ||
+                !expr_compile(bcs[i].T_expr, &codes[5 * i + 1], nullptr, 0) ||
+                !expr_compile_components(bcs[i].vel_expr, 3, &codes[5 * i + 2], nullptr, 0))
+                return fail(PLENS_E_INVALID);
+        }
+        if (any) {
+            if (!m->node_coords && !m->vertices) return fail(PLENS_E_INVALID);
+            std::vector<int> slot((size_t)nc * 6, -1), sbc;
+            std::vector<double> xyz;
+            for (int c = 0; c < nc; ++c)
+                for (int f = 0; f < 6; ++f) {
+                    const int b = m->face_bid[c * 6 + f];
+                    if (b < 0 || bcs[b].type != PLENS_BC_VARYING_INFLOW) continue;
+                    slot[(size_t)c * 6 + f] = (int)sbc.size();
+                    sbc.push_back(b);
+                    const size_t base = xyz.size();
+                    xyz.resize(base + 3 * (size_t)NF);
+                    for (int q = 0; q < NF; ++q) {
+                        const int node = plens_face_node(N, f, q);
+                        double X[3];
+                        if (m->node_coords) {
+                            for (int d = 0; d < 3; ++d) X[d] = m->node_coords[((size_t)c * 3 + d) * NN + node];
+                        } else {  // trilinear map of the vertices at the node
+                            const double xi[3] = {hbas.x[node % NP], hbas.x[(node / NP) % NP], hbas.x[node / (NP * NP)]};
+                            const double* V = m->vertices + (size_t)c * 24;
+                            for (int d = 0; d < 3; ++d) X[d] = 0;
+                            for (int v = 0; v < 8; ++v) {
+                                double w = 1;
+                                for (int a = 0; a < 3; ++a) w *= ((v >> a) & 1) ? xi[a] : 1.0 - xi[a];
+                                for (int d = 0; d < 3; ++d) X[d] += w * V[3 * v + d];
+                            }
+                        }
+                        for (int d = 0; d < 3; ++d) xyz[base + (size_t)d * NF + q] = X[d];
+                    }
+                }
+            h->nvslot = (int)sbc.size();
+            if ((rc = dmalloc(&h->vslot, (size_t)nc * 6)) || (rc = dmalloc(&h->vin, (size_t)h->nvslot * NF * 5)) ||
+                (rc = dmalloc(&h->vxyz, xyz.size())) || (rc = dmalloc(&h->vbc, sbc.size())) ||
+                (rc = dmalloc(&h->vcode, codes.size())))
+                return fail(rc);
+            if (hipMemcpy(h->vslot, slot.data(), sizeof(int) * slot.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(h->vxyz, xyz.data(), sizeof(double) * xyz.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(h->vbc, sbc.data(), sizeof(int) * sbc.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(h->vcode, codes.data(), sizeof(ExprCode) * codes.size(), hipMemcpyHostToDevice) != hipSuccess)
+                return fail(PLENS_E_HIP);
         }
     }
     const int ioff_h[2] = {0, nc};
@@ -2061,6 +2175,7 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
     k.gas.S = ph->S; k.gas.Pr = ph->prandtl; k.gas.inviscid = ph->mu0 < 1e-16;
     k.scheme = ph->inv_surf_flux;
     k.vscheme = ph->inv_vol_flux;
+    k.vslot = h->vslot; k.vin = h->vin; k.vxyz = h->vxyz; k.vbc = h->vbc; k.vcode = h->vcode; k.nvslot = h->nvslot;
     k.evar = ph->gradient_variables == PLENS_GRAD_ENTROPY;
     k.thr = ph->threshold_factor * pow(10.0, -ph->threshold_exp_factor * (pow(N + 1, 0.25)));
     k.sharp = ph->sharpness_factor; k.amin = ph->alpha_min; k.amax = ph->alpha_max;
@@ -2087,7 +2202,8 @@ void plens_gpu_destroy(plens_gpu* h) {
                     h->bid, h->wall, h->tasks, h->bcs, h->basis, h->avars, h->araw, h->alpha, h->alpha_d,
                     h->sinv, h->sface, h->rhs, h->rlo, h->dt_cell, h->scal, h->fail, h->blist, h->bcount, h->mlp_stats, h->feat,
                     h->avars_f, h->sinv_f, h->sface_f, h->cinst, h->ioff, h->visc, h->cfl, h->cmap, h->fmet,
-                    h->tabs, h->rhoE_old, h->dt_lts, h->ss_part, h->post};
+                    h->tabs, h->rhoE_old, h->dt_lts, h->ss_part, h->post, h->vslot, h->vin, h->vxyz,
+                    h->vbc, h->vcode};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
@@ -2130,6 +2246,7 @@ int plens_gpu_calc_rhs(plens_gpu* h, double* rhs_out, double* alpha_out, double*
     kp.stage = 0;
     kp.want_dt = 0;
     kp.t_acc = nullptr;
+    if ((e = launch_vinflow(h, kp))) return e;
     if ((e = launch_stage(h, kp))) return e;
     if ((e = read_failure(h))) return e;
     if (rhs_out)
@@ -2152,6 +2269,7 @@ int plens_gpu_calc_dt(plens_gpu* h, double courant, double* dt) {
     kp.courant = courant;
     kp.t_acc = nullptr;
     // gradient kernel with the time-step candidates (it also refreshes avars), then the reduction
+    if ((e = launch_vinflow(h, kp))) return e;
     if ((e = launch_grad(h, kp))) return e;
     hipLaunchKernelGGL(k_alpha, dim3((kp.nc + kThreadsAlpha - 1) / kThreadsAlpha), dim3(kThreadsAlpha), 0,
                        h->stream, kp);
@@ -2310,6 +2428,13 @@ int plens_gpu_ss_error(plens_gpu* h, double* err) {
     return PLENS_OK;
 }
 
+int plens_gpu_set_time(plens_gpu* h, double t) {
+    if (!h || h->ninst != 1) return PLENS_E_INVALID;
+    HIPCHK(hipMemcpyAsync(h->tabs, &t, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return PLENS_OK;
+}
+
 int plens_gpu_output_fields(plens_gpu* h, double courant, double t, double* nodal, double* cell) {
     if (!h || !nodal || !cell || h->ninst != 1) return PLENS_E_INVALID;
     int e = reset_failure(h);
@@ -2324,7 +2449,10 @@ int plens_gpu_output_fields(plens_gpu* h, double courant, double t, double* noda
     kp.save_E = 0;
     // stage-1 quantities of the current state as write() sees them (update_rk4, 4341-4350): avars,
     // blender and time step (calc_time_step; local steps switched on by t as in a step from t)
-    HIPCHK(hipMemcpyAsync(h->tabs + h->ninst, &t, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    const double tt[2] = {t, t};  // solution time (BCs) and the step's start (local stepping switch)
+    HIPCHK(hipMemcpyAsync(h->tabs, &tt[0], sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->tabs + h->ninst, &tt[1], sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if ((e = launch_vinflow(h, kp))) return e;
     if ((e = launch_grad(h, kp))) return e;
     hipLaunchKernelGGL(k_alpha, dim3((kp.nc + kThreadsAlpha - 1) / kThreadsAlpha), dim3(kThreadsAlpha), 0,
                        h->stream, kp);
@@ -2791,8 +2919,8 @@ extern "C" {
 
 int plens_gpu_batch_create(plens_gpu* const* instances, int32_t n, const char* tuning_db, plens_batch** out) {
     if (!instances || n < 1 || !out) return PLENS_E_INVALID;
-    for (int i = 0; i < n; ++i)
-        if (!instances[i] || instances[i]->ninst != 1) return PLENS_E_INVALID;
+    for (int i = 0; i < n; ++i)  // single handles; no varying inflow (its tables are per handle)
+        if (!instances[i] || instances[i]->ninst != 1 || instances[i]->nvslot > 0) return PLENS_E_INVALID;
     plens_batch* b = new (std::nothrow) plens_batch;
     if (!b) return PLENS_E_NOMEM;
     auto fail = [&](int code) {

@@ -121,6 +121,10 @@ class GpuPlens:
         self._check(self.lib.plens_gpu_ss_error(self._h, C.byref(e)), "ss_error")
         return e.value
 
+    def set_time(self, t):
+        """solution time of the handle (the BCs' time of calc_rhs)"""
+        self._check(self.lib.plens_gpu_set_time(self._h, t), "set_time")
+
     def output_fields(self, courant, t=0.0):
         """fields of PLENS::write() for the current state: nodal [15, n_dof] on each cell's
         equispaced lattice (txx..qz, rho, p, T, u, v, w), cell [3, n_cells] (alpha, alpha_d, loc_dt)"""

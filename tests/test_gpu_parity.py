@@ -135,6 +135,62 @@ def test_curved_meshes(geometry, N, q):
     assert err <= RUN_TOL
 
 
+@pytest.mark.parametrize("exprs", ["default", "front"])
+def test_varying_inflow(exprs):
+    """Varying inflow: prescribed p, T, velocity from expressions of the face DoF location and the
+    step's start time; RHS at two BC times and 10 steps from t = 0.3. "front": a moving
+    discontinuity written as in the double-Mach-reflection input (ternary in y and t)."""
+    from plens_amd import varying_inflow_tube
+    kw = {} if exprs == "default" else dict(
+        p_expr="( y - 0.05 - 0.5*t > 0 ? 1 : 1.3 )", T_expr="( y - 0.05 - 0.5*t > 0 ? 1 : 1.1 )",
+        vel_expr="( y - 0.05 - 0.5*t > 0 ? 2 : 2.2 ); ( y - 0.05 - 0.5*t > 0 ? 0 : -0.1 ); 0")
+    case = varying_inflow_tube(**kw)
+    g, o = _rhs_parity(case, f"vinflow-{exprs}-t0")
+    for h in (g, o):
+        h.set_time(0.2)
+    rg, _, _ = g.calc_rhs()
+    ro, _, _ = o.calc_rhs()
+    err = rel_l2(rg, ro)
+    print(f"vinflow-{exprs} t=0.2 rhs rel L2 {err:.3e}", flush=True)
+    assert err <= RHS_TOL
+    g, o = _pair(case)
+    g.run(10, case.courant, t0=0.3)
+    o.run(10, case.courant, t0=0.3)
+    err = rel_l2(g.get_state(), o.get_state())
+    print(f"vinflow-{exprs} 10 steps rel L2 {err:.3e}", flush=True)
+    assert err <= RUN_TOL
+
+
+def test_constant_varying_inflow_is_bitwise_the_uniform_inflow():
+    from plens_amd import GpuPlens, varying_inflow_tube
+    from plens_amd import _lib as L
+    from plens_amd.cases import make_bc
+    case = varying_inflow_tube()
+    uni = make_bc(L.BC_UNIFORM_INFLOW, p=1.2, T=0.9, vel=(2.0, 0.1, -0.05))
+    var = make_bc(L.BC_VARYING_INFLOW, p_expr="1.2", T_expr="0.9", vel_expr="2; 0.1; -0.05")
+    a = GpuPlens(mesh=case.mesh, physics=case.physics, bcs=[uni] + case.bcs[1:])
+    b = GpuPlens(mesh=case.mesh, physics=case.physics, bcs=[var] + case.bcs[1:])
+    for h in (a, b):
+        h.set_state(case.Q0)
+        h.run(5, case.courant)
+    assert np.array_equal(a.get_state(), b.get_state())
+
+
+def test_varying_inflow_rejections():
+    from plens_amd import GpuBatch, GpuPlens, varying_inflow_tube
+    from plens_amd import _lib as L
+    from plens_amd.cases import make_bc
+    case = varying_inflow_tube()
+    with pytest.raises(L.PlensError):
+        GpuPlens(mesh=case.mesh, physics=case.physics,
+                 bcs=[make_bc(L.BC_VARYING_INFLOW, p_expr="1 +", T_expr="1", vel_expr="1; 0; 0")] + case.bcs[1:])
+    with pytest.raises(L.PlensError):
+        GpuPlens(mesh=case.mesh, physics=case.physics,
+                 bcs=[make_bc(L.BC_VARYING_INFLOW, p_expr="1", T_expr="1", vel_expr="1; 0")] + case.bcs[1:])
+    with pytest.raises(L.PlensError):
+        GpuBatch([GpuPlens(case)])
+
+
 def test_dt_parity():
     from plens_amd import wall_flow
     case = wall_flow(nx=6, ny=4, nz=2)
