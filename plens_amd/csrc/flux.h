@@ -17,6 +17,7 @@ namespace plens_dev {
 
 struct Gas {
     double gma, R, cv, mu0, T0, S, Pr;
+    double icv, iR, kfac;  // 1/c_v, 1/R, R gamma / ((gamma - 1) Pr)
     int inviscid;
 };
 
@@ -68,12 +69,10 @@ __device__ __forceinline__ T get_a(const T u[5], T gma) {
 
 // Sutherland viscosity and conductivity, navier_stokes.h:348-360
 __device__ __forceinline__ double sutherland(const Gas& g, double T) {
-    const double r = T / g.T0;
-    return g.mu0 * (r * sqrt(r)) * (g.T0 + g.S) / (T + g.S);
+    const double r = dv(T, g.T0);
+    return dv(g.mu0 * (r * sqrt(r)) * (g.T0 + g.S), T + g.S);
 }
-__device__ __forceinline__ double conductivity(const Gas& g, double mu) {
-    return mu * g.R * g.gma / ((g.gma - 1.0) * g.Pr);
-}
+__device__ __forceinline__ double conductivity(const Gas& g, double mu) { return mu * g.kfac; }
 
 // exact Euler flux along dir (not necessarily unit), get_inv_flux (navier_stokes.cc:424-442)
 template <typename T>

@@ -257,16 +257,21 @@ __device__ __forceinline__ void cons_to_entropy(const double u[5], double gma, d
 
 // BC ghost state (IBCs/*.cc): stage 1 and 3 share the state rule, stage 2 has its own; n is the
 // outward unit normal of the face node
-// outward unit normal of the face node; vin: the node's prescribed state of a varying inflow
+// outward unit normal of the face node; vin: where a varying inflow finds the node's prescribed
+// state (read only for that type)
+struct VinRef {
+    const int* slot;
+    const double* vin;
+    long long cf;
+    int q, NF;
+};
 __device__ void bc_ghost_state(const BCDev& b, int stage, const Gas& g, const double u[5], const double n[3],
-                               double ug[5], const double* vin = nullptr) {
-    if (b.type == PLENS_BC_UNIFORM_INFLOW || b.type == PLENS_BC_VARYING_INFLOW) {
-        // varying_inflow.cc: uniform inflow's ghosts with the state prescribed at the node
-        const double* pr = b.type == PLENS_BC_VARYING_INFLOW ? vin : b.cons_pr;
+                               double ug[5], const VinRef& vin = VinRef{nullptr, nullptr, 0, 0, 0}) {
+    if (b.type == PLENS_BC_UNIFORM_INFLOW) {
         if (stage == 2) {
-            for (int v = 0; v < 5; ++v) ug[v] = pr[v];
+            for (int v = 0; v < 5; ++v) ug[v] = b.cons_pr[v];
         } else {
-            for (int v = 0; v < 5; ++v) ug[v] = 2.0 * pr[v] - u[v];
+            for (int v = 0; v < 5; ++v) ug[v] = 2.0 * b.cons_pr[v] - u[v];
         }
     } else if (b.type == PLENS_BC_OUTFLOW) {
         for (int v = 0; v < 5; ++v) ug[v] = u[v];
@@ -308,6 +313,14 @@ __device__ void bc_ghost_state(const BCDev& b, int stage, const Gas& g, const do
         } else {
             for (int v = 0; v < 5; ++v) ug[v] = 2.0 * pr[v] - u[v];
         }
+    } else if (b.type == PLENS_BC_VARYING_INFLOW) {
+        // varying_inflow.cc: uniform inflow's ghosts with the state prescribed at the node
+        const double* pr = vin.vin + ((long long)vin.slot[vin.cf] * vin.NF + vin.q) * 5;
+        if (stage == 2) {
+            for (int v = 0; v < 5; ++v) ug[v] = pr[v];
+        } else {
+            for (int v = 0; v < 5; ++v) ug[v] = 2.0 * pr[v] - u[v];
+        }
     } else {  // uniform temperature wall (uniform_temp_wall.cc), insulated wall (insulated_wall.cc)
         if (stage == 2) {
             ug[0] = u[0];
@@ -329,9 +342,9 @@ __device__ void bc_ghost_state(const BCDev& b, int stage, const Gas& g, const do
     }
 }
 
-// prescribed state of face node q of face task cf when its BC is a varying inflow
-__device__ __forceinline__ const double* vin_node(const KP& p, int b, long long cf, int q, int NF) {
-    return p.bcs[b].type == PLENS_BC_VARYING_INFLOW ? p.vin + ((long long)p.vslot[cf] * NF + q) * 5 : nullptr;
+// where a varying inflow finds the prescribed state of face node q of face cf
+__device__ __forceinline__ VinRef vin_node(const KP& p, long long cf, int q, int NF) {
+    return VinRef{p.vslot, p.vin, cf, q, NF};
 }
 
 // k_vinflow: prescribed states of the varying-inflow face nodes for the current step
@@ -422,7 +435,7 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
         gi.mu0 = p.visc[3 * inst];
         gi.T0 = p.visc[3 * inst + 1];
         gi.S = p.visc[3 * inst + 2];
-        mu = sutherland(gi, e / gs.cv);
+        mu = sutherland(gi, e * gs.icv);
     }
     const double kap = conductivity(gs, mu);
 
@@ -472,7 +485,7 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
     if (p.feat && act) {
         const double m0 = sT[lc * NN];
         for (int k = node; k < kMlpIn; k += NN)
-            p.feat[(long long)c * kMlpIn + k] = (k < NN) ? (float)(sT[lc * NN + k] / m0) : 0.0f;
+            p.feat[(long long)c * kMlpIn + k] = (k < NN) ? (float)(sT[lc * NN + k] * rcp64(m0)) : 0.0f;
     }
 
     // ---- stage-1 time step candidates (PLENS::calc_time_step, dgsem/plens.cc:3836-3866)
@@ -483,13 +496,14 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
             if (act) {
                 const double h = p.hmin[c];
                 const double a = get_a(u, gs.gma);
-                const double vel[3] = {u[1] / u[0], u[2] / u[0], u[3] / u[0]};
+                const double irho = rcp64(u[0]), idet = rcp64(detJ);
+                const double vel[3] = {u[1] * irho, u[2] * irho, u[3] * irho};
                 double madv = -1e300;
                 double ja[9], dJ;
                 node_metrics<N>(p, c, node, ja, dJ);
                 for (int r = 0; r < 3; ++r) {  // (J^-T v)_r with J^-1 rows = Ja^i / detJ
                     double s = 0;
-                    for (int cc = 0; cc < 3; ++cc) s += (ja[3 * cc + r] / detJ) * vel[cc];
+                    for (int cc = 0; cc < 3; ++cc) s += (ja[3 * cc + r] * idet) * vel[cc];
                     madv = fmax(madv, s);
                 }
                 const double cr = p.cfl ? p.cfl[p.cinst[c]] : p.courant;
@@ -554,16 +568,22 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
         const int q = face_dof<N>(f, ti, tj, tk);
         const int sd = f >> 1, lr = f & 1;
         const int b = p.bid[c * 6 + f];
+        const int bt = b < 0 ? -1 : p.bcs[b].type;
         double us[5];
-        if (b < 0 || p.bcs[b].type == PLENS_BC_PERIODIC) {
-            const int nb = p.nbr[c * 6 + f];
+        if (b < 0 || bt == PLENS_BC_PERIODIC || bt == PLENS_BC_VARYING_INFLOW) {
+            // neighbour's state, or the prescribed state of a varying inflow (stage-1 ghost
+            // 2 u_pr - u, averaged as every ghost is)
+            const bool vary = bt == PLENS_BC_VARYING_INFLOW;
+            const double* src = vary ? p.vin + ((long long)p.vslot[c * 6 + f] * (NP * NP) + q) * 5 : p.Q;
+            const long long stride = vary ? 1 : p.ndof;
+            const long long idx = vary ? 0 : (long long)p.nbr[c * 6 + f] * NN + face_node<N>(f ^ 1, q);
             double un[5];
-            load_state(p.Q, p.ndof, (long long)nb * NN + face_node<N>(f ^ 1, q), un);
-            for (int v = 0; v < 5; ++v) us[v] = 0.5 * (u[v] + un[v]);
+            load_state(src, stride, idx, un);
+            for (int v = 0; v < 5; ++v) us[v] = 0.5 * (u[v] + (vary ? 2.0 * un[v] - u[v] : un[v]));
         } else {
             double ug[5], nf[3];
             for (int d = 0; d < 3; ++d) nf[d] = p.nrm[((long long)(c * 6 + f) * 3 + d) * (NP * NP) + q];
-            bc_ghost_state(p.bcs[b], 1, gs, u, nf, ug, vin_node(p, b, (long long)c * 6 + f, q, NP * NP));
+            bc_ghost_state(p.bcs[b], 1, gs, u, nf, ug);
             for (int v = 0; v < 5; ++v) us[v] = 0.5 * (u[v] + ug[v]);
         }
         // g -= s (u* Ja - u Ja)[gd] / w_end, evaluated as (u* - u) (s Ja[gd] / w_end); in entropy
@@ -583,7 +603,7 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
             for (int v = 0; v < 5; ++v) gr[gd][v] -= du[v] * jw;
         }
     }
-    const TF iJ = (TF)(1.0 / detJ);
+    const TF iJ = (TF)rcp64(detJ);
     for (int gd = 0; gd < 3; ++gd)
         for (int v = 0; v < 5; ++v) gr[gd][v] *= iJ;
 
@@ -592,19 +612,22 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
     if (p.evar) {  // velocity and temperature gradients from the entropy variables
         TF wt[5];
         for (int v = 0; v < 5; ++v) wt[v] = sQ[v * B + li];
+        const TF iw = rcpt(wt[4]);
         for (int vd = 0; vd < 3; ++vd)
-            for (int gd = 0; gd < 3; ++gd) vg[vd][gd] = (wt[1 + vd] * gr[gd][4] / wt[4] - gr[gd][1 + vd]) / wt[4];
-        const TF iR = (TF)(1.0 / gs.R);
-        for (int gd = 0; gd < 3; ++gd) av[6 + gd] = -(gr[gd][4] / (wt[4] * wt[4]) * iR);
+            for (int gd = 0; gd < 3; ++gd) vg[vd][gd] = (wt[1 + vd] * gr[gd][4] * iw - gr[gd][1 + vd]) * iw;
+        const TF iR = (TF)gs.iR;
+        for (int gd = 0; gd < 3; ++gd) av[6 + gd] = -(gr[gd][4] * (iw * iw) * iR);
     } else {
-        const TF vel[3] = {(TF)(u[1] / u[0]), (TF)(u[2] / u[0]), (TF)(u[3] / u[0])};
+        const double irho = rcp64(u[0]);
+        const TF ir = (TF)irho;
+        const TF vel[3] = {(TF)(u[1] * irho), (TF)(u[2] * irho), (TF)(u[3] * irho)};
         for (int vd = 0; vd < 3; ++vd)
-            for (int gd = 0; gd < 3; ++gd) vg[vd][gd] = (gr[gd][1 + vd] - gr[gd][0] * vel[vd]) / ut[0];
-        const TF et = (TF)e;
+            for (int gd = 0; gd < 3; ++gd) vg[vd][gd] = (gr[gd][1 + vd] - gr[gd][0] * vel[vd]) * ir;
+        const TF et = (TF)e, icv = (TF)gs.icv;
         for (int gd = 0; gd < 3; ++gd) {
-            TF eg = (gr[gd][4] - gr[gd][0] * et) / ut[0];
-            for (int vd = 0; vd < 3; ++vd) eg -= (TF(0.5) * gr[gd][0] * vel[vd] * vel[vd] + ut[1 + vd] * vg[vd][gd]) / ut[0];
-            av[6 + gd] = -eg / (TF)gs.cv;
+            TF eg = (gr[gd][4] - gr[gd][0] * et) * ir;
+            for (int vd = 0; vd < 3; ++vd) eg -= (TF(0.5) * gr[gd][0] * vel[vd] * vel[vd] + ut[1 + vd] * vg[vd][gd]) * ir;
+            av[6 + gd] = -eg * icv;
         }
     }
     const TF tr = vg[0][0] + vg[1][1] + vg[2][2];
@@ -903,7 +926,7 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
         load_state(p.Q, p.ndof, gn, ug);
         bl = interior ? 0.5 * (p.alpha[c] + p.alpha[nb]) : p.alpha[c];
     } else {
-        bc_ghost_state(p.bcs[b], 2, gs, u, n, ug, vin_node(p, b, cf, q, NF));
+        bc_ghost_state(p.bcs[b], 2, gs, u, n, ug, vin_node(p, cf, q, NF));
         bl = p.alpha[c];
     }
     bl = bl <= 0.0 ? 0.0 : (bl <= 1.0 ? bl : 1.0);  // set_flux_blender_value
@@ -955,7 +978,7 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     // each side's contribution to its blended RHS (calc_rhs, dgsem/plens.cc:3753-3771): the
     // inviscid term scaled by (1 - alpha)/(-J), the viscous one by (1 - alpha_d)/J (the residual
     // scalings of calc_cell_ho_inv/dif_residual, 3271-3274, 3409-3412)
-    const TF wic = (TF)((1.0 - ac) / (-1.0 * Jc)), win = (TF)((1.0 - an) / (-1.0 * Jn));
+    const TF wic = (TF)((1.0 - ac) * -rcp64(Jc)), win = (TF)((1.0 - an) * -rcp64(Jn));
     TF* sface = tsel<TF>(p.sface, p.sface_f);
     if (gs.inviscid) {
         for (int v = 0; v < 5; ++v) sface[o + v * NF] = wic * tc[v];
@@ -963,8 +986,8 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
             for (int v = 0; v < 5; ++v) sface[on + v * NF] = win * tn[v];
         return;
     }
-    const TF wvc = (TF)((1.0 - p.alpha_d[c]) / (1.0 * Jc));
-    const TF wvn = interior ? (TF)((1.0 - p.alpha_d[nb]) / (1.0 * Jn)) : TF(0);
+    const TF wvc = (TF)((1.0 - p.alpha_d[c]) * rcp64(Jc));
+    const TF wvn = interior ? (TF)((1.0 - p.alpha_d[nb]) * rcp64(Jn)) : TF(0);
     // stage 3: BR1 average of the viscous fluxes (navier_stokes.cc:1644-1652)
     const TF* avars = tsel<TF>(p.avars, p.avars_f);
     TF av[9], avg[9];
@@ -979,7 +1002,7 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
             const TF qn = avg[6] * nt[0] + avg[7] * nt[1] + avg[8] * nt[2];
             for (int d = 0; d < 3; ++d) avg[6 + d] = av[6 + d] - TF(2) * qn * nt[d];
         }
-        if (p.bcs[b].type != PLENS_BC_OUTFLOW) bc_ghost_state(p.bcs[b], 3, gs, u, n, ug, vin_node(p, b, cf, q, NF));
+        if (p.bcs[b].type != PLENS_BC_OUTFLOW) bc_ghost_state(p.bcs[b], 3, gs, u, n, ug, vin_node(p, cf, q, NF));
         else for (int v = 0; v < 5; ++v) ug[v] = u[v];
         for (int v = 0; v < 5; ++v) ugt[v] = (TF)ug[v];
     }
@@ -2173,6 +2196,7 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
     k.mlp = nullptr; k.mlp_tol = 0.05; k.mlp_floor = ph->alpha_min;
     k.gas.gma = ph->gamma; k.gas.R = R; k.gas.cv = R / (ph->gamma - 1); k.gas.mu0 = ph->mu0; k.gas.T0 = ph->T0;
     k.gas.S = ph->S; k.gas.Pr = ph->prandtl; k.gas.inviscid = ph->mu0 < 1e-16;
+    k.gas.icv = 1.0 / k.gas.cv; k.gas.iR = 1.0 / R; k.gas.kfac = R * ph->gamma / ((ph->gamma - 1.0) * ph->prandtl);
     k.scheme = ph->inv_surf_flux;
     k.vscheme = ph->inv_vol_flux;
     k.vslot = h->vslot; k.vin = h->vin; k.vxyz = h->vxyz; k.vbc = h->vbc; k.vcode = h->vcode; k.nvslot = h->nvslot;
