@@ -1628,6 +1628,10 @@ int launch_grad(plens_gpu* h, const KP& kp) {
     return PLENS_OK;
 }
 
+// Workgroups of the persistent blended-cell kernel: enough to fill every SIMD of the device at the
+// kernel's register-limited occupancy (idle groups exit at once when few cells are blended)
+constexpr int kLoGrid = 2048;
+
 // Kernel classes of a stage, in launch order: 0 k_grad (+ k_mlp with a learned indicator),
 // 1 k_alpha, 2 k_face, 3 k_lo, 4 k_vol.
 template <int N, typename TF>
@@ -1672,10 +1676,10 @@ int launch_one_t(plens_gpu* h, const KP& kp, int kernel) {
         }
         case 3:
             if (ext)
-                hipLaunchKernelGGL((k_lo<N, TF, true>), dim3(std::min(blocks, 256)), dim3(thr), sizeof(TF) * 19 * B,
+                hipLaunchKernelGGL((k_lo<N, TF, true>), dim3(std::min(blocks, kLoGrid)), dim3(thr), sizeof(TF) * 19 * B,
                                    h->stream, kp);
             else
-                hipLaunchKernelGGL((k_lo<N, TF, false>), dim3(std::min(blocks, 256)), dim3(thr), sizeof(TF) * 19 * B,
+                hipLaunchKernelGGL((k_lo<N, TF, false>), dim3(std::min(blocks, kLoGrid)), dim3(thr), sizeof(TF) * 19 * B,
                                    h->stream, kp);
             break;
         case 4: {
