@@ -553,12 +553,13 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
                 int to[3] = {ti, tj, tk};
                 to[md] = m;
                 const int jo = lc * NN + to[0] + NP * (to[1] + NP * to[2]);
-                TF fl[5];
-                for (int v = 0; v < 5; ++v) fl[v] = TF(0.5) * (ut[v] + sQ[v * B + jo]);
-                const TF Dm2 = (TF)(2.0 * sD[tt[md] * NP + m]);
+                // 2 D (u_i + u_m)/2 (Ja_i + Ja_m)/2 grouped as (u_i + u_m) (D/2 (Ja_i + Ja_m))
+                TF su[5];
+                for (int v = 0; v < 5; ++v) su[v] = ut[v] + sQ[v * B + jo];
+                const TF Dh = (TF)(0.5 * sD[tt[md] * NP + m]);
                 for (int gd = 0; gd < 3; ++gd) {
-                    const TF jav = TF(0.5) * (sJa[(3 * md + gd) * B + li] + sJa[(3 * md + gd) * B + jo]);
-                    for (int v = 0; v < 5; ++v) gr[gd][v] += Dm2 * fl[v] * jav;
+                    const TF jav = Dh * (sJa[(3 * md + gd) * B + li] + sJa[(3 * md + gd) * B + jo]);
+                    for (int v = 0; v < 5; ++v) gr[gd][v] += su[v] * jav;
                 }
             }
         }
@@ -1165,18 +1166,18 @@ __device__ __forceinline__ LogMean<TF> log_mean(double xa, double xb, TF lxa, TF
     return m;
 }
 // Chandrashekhar (chandrashekhar_vol_flux, navier_stokes.cc:1510-1542). sR: rho, beta = rho/(2p);
-// sA: ln rho, ln beta, v0, v1, v2, |v|^2
+// sA: ln rho, ln beta, v0, v1, v2
 template <typename TF>
 __device__ __forceinline__ void pair_chandrashekhar(const double* sR, const TF* sA, int B, int ja, int jb,
                                                     const TF dir[3], TF half_gm1_inv, TF f[5]) {
     const LogMean<TF> mr = log_mean<TF>(sR[ja], sR[jb], sA[ja], sA[jb]);
     const LogMean<TF> mb = log_mean<TF>(sR[B + ja], sR[B + jb], sA[B + ja], sA[B + jb]);
     const TF p_hat = (TF)(0.5 * (sR[ja] + sR[jb]) * mb.isum);
-    const TF va0 = TF(0.5) * (sA[2 * B + ja] + sA[2 * B + jb]);
-    const TF va1 = TF(0.5) * (sA[3 * B + ja] + sA[3 * B + jb]);
-    const TF va2 = TF(0.5) * (sA[4 * B + ja] + sA[4 * B + jb]);
+    const TF a0 = sA[2 * B + ja], a1 = sA[3 * B + ja], a2 = sA[4 * B + ja];
+    const TF b0 = sA[2 * B + jb], b1 = sA[3 * B + jb], b2 = sA[4 * B + jb];
+    const TF va0 = TF(0.5) * (a0 + b0), va1 = TF(0.5) * (a1 + b1), va2 = TF(0.5) * (a2 + b2);
     const TF H = half_gm1_inv * mb.inv + p_hat * mr.inv + (va0 * va0 + va1 * va1 + va2 * va2) -
-                 TF(0.25) * (sA[5 * B + ja] + sA[5 * B + jb]);
+                 TF(0.25) * ((a0 * a0 + a1 * a1 + a2 * a2) + (b0 * b0 + b1 * b1 + b2 * b2));
     const TF rv = mr.xln * (va0 * dir[0] + va1 * dir[1] + va2 * dir[2]);
     f[0] = rv;
     f[1] = rv * va0 + p_hat * dir[0];
@@ -1185,16 +1186,16 @@ __device__ __forceinline__ void pair_chandrashekhar(const double* sR, const TF* 
     f[4] = rv * H;
 }
 // Kennedy-Gruber (kennedy_gruber_vol_flux, navier_stokes.cc:1549-1575). sR: rho, p; sA: e, -, v0,
-// v1, v2, |v|^2
+// v1, v2
 template <typename TF>
 __device__ __forceinline__ void pair_kennedy_gruber(const double* sR, const TF* sA, int B, int ja, int jb,
                                                     const TF dir[3], TF f[5]) {
     const TF ra = (TF)(0.5 * (sR[ja] + sR[jb]));
     const TF pa = (TF)(0.5 * (sR[B + ja] + sR[B + jb]));
-    const TF va0 = TF(0.5) * (sA[2 * B + ja] + sA[2 * B + jb]);
-    const TF va1 = TF(0.5) * (sA[3 * B + ja] + sA[3 * B + jb]);
-    const TF va2 = TF(0.5) * (sA[4 * B + ja] + sA[4 * B + jb]);
-    const TF Ea = TF(0.5) * (sA[ja] + sA[jb] + TF(0.5) * (sA[5 * B + ja] + sA[5 * B + jb]));
+    const TF a0 = sA[2 * B + ja], a1 = sA[3 * B + ja], a2 = sA[4 * B + ja];
+    const TF b0 = sA[2 * B + jb], b1 = sA[3 * B + jb], b2 = sA[4 * B + jb];
+    const TF va0 = TF(0.5) * (a0 + b0), va1 = TF(0.5) * (a1 + b1), va2 = TF(0.5) * (a2 + b2);
+    const TF Ea = TF(0.5) * (sA[ja] + sA[jb] + TF(0.5) * ((a0 * a0 + a1 * a1 + a2 * a2) + (b0 * b0 + b1 * b1 + b2 * b2)));
     const TF rv = ra * (va0 * dir[0] + va1 * dir[1] + va2 * dir[2]);
     f[0] = rv;
     f[1] = rv * va0 + pa * dir[0];
@@ -1203,7 +1204,7 @@ __device__ __forceinline__ void pair_kennedy_gruber(const double* sR, const TF* 
     f[4] = rv * Ea + pa * (va0 * dir[0] + va1 * dir[1] + va2 * dir[2]);
 }
 // Ismail-Roe (ismail_roe_vol_flux, navier_stokes.cc:1590-1622) through z1 = sqrt(rho/p),
-// z5 = sqrt(rho p). sR: z5, z1; sA: ln z5, ln z1, v0, v1, v2, -
+// z5 = sqrt(rho p). sR: z5, z1; sA: ln z5, ln z1, v0, v1, v2
 template <typename TF>
 __device__ __forceinline__ void pair_ismail_roe(const double* sR, const TF* sA, int B, int ja, int jb,
                                                 const TF dir[3], TF gma, TF f[5]) {
@@ -1242,8 +1243,8 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
     const int B = p.cpb * NN;
     const int BP = p.cpb * NITEM;
     double* sR = smem;                               // 2B: per-node FP64 data of the pair flux
-    TF* sA = reinterpret_cast<TF*>(sR + 2 * B);      // 6B: per-node flux-type data of the pair flux
-    TF* sJa = sA + 6 * B;                            // 3B: Ja^md of the direction being processed
+    TF* sA = reinterpret_cast<TF*>(sR + 2 * B);      // 5B: per-node flux-type data of the pair flux
+    TF* sJa = sA + 5 * B;                            // 3B: Ja^md of the direction being processed
     TF* sF = sJa + 3 * B;                            // 5*max(BP,B) pair fluxes / viscous scratch
     load_basis<N>(p.basis, sD, sW, sQm, sC);
     for (int t = threadIdx.x; t < NP * NP; t += blockDim.x) {
@@ -1293,7 +1294,6 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
         sA[2 * B + li] = (TF)v0;
         sA[3 * B + li] = (TF)v1;
         sA[4 * B + li] = (TF)v2;
-        sA[5 * B + li] = (TF)(v0 * v0 + v1 * v1 + v2 * v2);
     }
     TF ja_own[9], ut[5];
     for (int a = 0; a < 9; ++a) ja_own[a] = (TF)jm[a];
@@ -1685,7 +1685,7 @@ int launch_one_t(plens_gpu* h, const KP& kp, int kernel) {
         case 4: {
             const size_t nitem = (size_t)NP * NP * (NP * (NP - 1) / 2);
             const size_t lds_vol =
-                sizeof(double) * 2 * B + sizeof(TF) * (9 * B + 5 * std::max((size_t)kp.cpb * nitem, B));
+                sizeof(double) * 2 * B + sizeof(TF) * (8 * B + 5 * std::max((size_t)kp.cpb * nitem, B));
             if (ext)
                 hipLaunchKernelGGL((k_vol<N, TF, true>), dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
             else
