@@ -184,11 +184,6 @@ __device__ __forceinline__ int face_dof(int f, int i, int j, int k) {
     constexpr int NP = N + 1;
     return (f < 2) ? j + NP * k : (f < 4) ? i + NP * k : i + NP * j;
 }
-__device__ __forceinline__ bool on_face(int f, int i, int j, int k, int N) {
-    const int t = (f < 2) ? i : (f < 4) ? j : k;
-    return (f & 1) ? (t == N) : (t == 0);
-}
-
 __device__ __forceinline__ void load_state(const double* Q, long long ndof, long long g, double u[5]) {
 #pragma unroll
     for (int v = 0; v < 5; ++v) u[v] = Q[v * ndof + g];
@@ -386,7 +381,7 @@ __device__ __forceinline__ void load_basis(const Basis* b, double* sD, double* s
 // k_grad
 // =============================================================================================
 template <int N, typename TF>
-__global__ void __launch_bounds__(256) k_grad(KP p) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_grad(KP p) {
     constexpr int NP = N + 1, NN = NP * NP * NP;
     extern __shared__ double smem[];
     __shared__ double sD[NP * NP], sW[NP], sQm[NP * NP], sC[NP * NP];
@@ -396,10 +391,16 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
     double* sE = sT + B;   // 3B modal-energy partial sums
     TF* sQ = reinterpret_cast<TF*>(sE + 3 * B);  // 5B states in the flux arithmetic
     TF* sJa = sQ + 5 * B;                        // 9B contravariant vectors
+    int* sFn = reinterpret_cast<int*>(sJa + 9 * B);  // per cell: boundary ids, then neighbours of the 6 faces
     load_basis<N>(p.basis, sD, sW, sQm, sC);
 
     const int tid = threadIdx.x, lc = tid / NN, node = tid - lc * NN;
-    const int c = xcd_block(blockIdx.x, gridDim.x) * p.cpb + lc;
+    const int c0 = xcd_block(blockIdx.x, gridDim.x) * p.cpb;
+    const int c = c0 + lc;
+    for (int t = tid; t < 12 * p.cpb; t += blockDim.x) {
+        const int cl = t / 12, r = t - 12 * cl, cc = c0 + cl;
+        sFn[t] = cc < p.nc ? (r < 6 ? p.bid[cc * 6 + r] : p.nbr[cc * 6 + r - 6]) : -1;
+    }
     const bool inblk = lc < p.cpb;
     const bool act = inblk && c < p.nc;
     const int li = lc * NN + node;
@@ -564,11 +565,11 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
             }
         }
     }
-    for (int f = 0; f < 6; ++f) {
-        if (!on_face(f, ti, tj, tk, N)) continue;
+    for (int sd = 0; sd < 3; ++sd) {  // a node lies on at most one face of each direction
+        if (tt[sd] != 0 && tt[sd] != N) continue;
+        const int lr = tt[sd] == N ? 1 : 0, f = 2 * sd + lr;
         const int q = face_dof<N>(f, ti, tj, tk);
-        const int sd = f >> 1, lr = f & 1;
-        const int b = p.bid[c * 6 + f];
+        const int b = sFn[lc * 12 + f];
         const int bt = b < 0 ? -1 : p.bcs[b].type;
         double us[5];
         if (b < 0 || bt == PLENS_BC_PERIODIC || bt == PLENS_BC_VARYING_INFLOW) {
@@ -577,7 +578,7 @@ __global__ void __launch_bounds__(256) k_grad(KP p) {
             const bool vary = bt == PLENS_BC_VARYING_INFLOW;
             const double* src = vary ? p.vin + ((long long)p.vslot[c * 6 + f] * (NP * NP) + q) * 5 : p.Q;
             const long long stride = vary ? 1 : p.ndof;
-            const long long idx = vary ? 0 : (long long)p.nbr[c * 6 + f] * NN + face_node<N>(f ^ 1, q);
+            const long long idx = vary ? 0 : (long long)sFn[lc * 12 + 6 + f] * NN + face_node<N>(f ^ 1, q);
             double un[5];
             load_state(src, stride, idx, un);
             for (int v = 0; v < 5; ++v) us[v] = 0.5 * (u[v] + (vary ? 2.0 * un[v] - u[v] : un[v]));
@@ -1106,10 +1107,10 @@ __global__ void __launch_bounds__(256) k_lo(KP p) {
         if (act) {
             // boundary subcells use the stage-2 face flux (dgsem/plens.cc:3544-3575): F*|Ja|/w_end,
             // recovered from the stored surface term S = (F* + s f_in) |Ja| / w_end
-            for (int f = 0; f < 6; ++f) {
-                if (!on_face(f, ti, tj, tk, N)) continue;
+            for (int sd = 0; sd < 3; ++sd) {  // a node lies on at most one face of each direction
+                if (tt[sd] != 0 && tt[sd] != N) continue;
+                const int lr = tt[sd] == N ? 1 : 0, f = 2 * sd + lr;
                 const int q = face_dof<N>(f, ti, tj, tk);
-                const int sd = f >> 1, lr = f & 1;
                 const TF sg = lr ? TF(-1) : TF(1);
                 TF dir[3];
                 for (int a = 0; a < 3; ++a) dir[a] = sJa[(3 * sd + a) * B + li];
@@ -1385,8 +1386,9 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
     // ---- surface terms of both high-order residuals, finished and weighted per face node by k_face
     const TF* sface = tsel<TF>(p.sface, p.sface_f);
     double sf[5] = {0, 0, 0, 0, 0};
-    for (int f = 0; f < 6; ++f) {
-        if (!on_face(f, ti, tj, tk, N)) continue;
+    for (int sd = 0; sd < 3; ++sd) {  // a node lies on at most one face of each direction
+        if (tt[sd] != 0 && tt[sd] != N) continue;
+        const int f = 2 * sd + (tt[sd] == N ? 1 : 0);
         const long long o = ((long long)c * 6 + f) * 5 * NF + face_dof<N>(f, ti, tj, tk);
         for (int v = 0; v < 5; ++v) sf[v] += (double)sface[o + v * NF];
     }
@@ -1596,10 +1598,11 @@ const double kRkAo[5] = {4745337637855.0 / 22386579876409, 6808157035527.0 / 131
 const double kRkAi[4] = {546509042554.0 / 9152262712923, 625707605167.0 / 5316659119056,
                          582400652113.0 / 7078426004906, 5010207514426.0 / 21876007855139};
 
-// dynamic LDS of k_grad: FP64 blender/modal arrays (5B) + flux-type states and metrics (14B)
+// dynamic LDS of k_grad: FP64 blender/modal arrays (5B) + flux-type states and metrics (14B) + face
+// boundary ids and neighbours (12 per cell)
 template <typename TF>
 size_t lds_grad_bytes(int cpb, int NN) {
-    return sizeof(double) * 5 * (size_t)cpb * NN + sizeof(TF) * 14 * (size_t)cpb * NN;
+    return sizeof(double) * 5 * (size_t)cpb * NN + sizeof(TF) * 14 * (size_t)cpb * NN + sizeof(int) * 12 * (size_t)cpb;
 }
 
 template <typename TF>
