@@ -74,12 +74,14 @@ struct KP {
     const double* cmap;      // trilinear map of every cell: [3][8] monomial coefficients (nullable)
     const double* fmet;      // per face task and face node: Ja/|Ja|, |Ja|/w_end, detJ of the cell side,
                              // then the same of the neighbour side ([task][10][n_f])
+    const float* fmet_f;     // the same in FP32 (FP32-bulk)
     const double* nrm;
     const double* hmin;
     const int* nbr;
     const int* bid;
     const int* wall;
     const int* tasks;
+    const int4* tdesc;       // per face task: cell*6+face, neighbour, boundary id, boundary type (-1 interior)
     const BCDev* bcs;
     const Basis* basis;
     double* avars;
@@ -145,6 +147,12 @@ template <>
 __device__ __forceinline__ double* tsel<double>(double* d, float*) { return d; }
 template <>
 __device__ __forceinline__ float* tsel<float>(double*, float* f) { return f; }
+template <typename T>
+__device__ __forceinline__ const T* tsel(const double* d, const float* f);
+template <>
+__device__ __forceinline__ const double* tsel<double>(const double* d, const float*) { return d; }
+template <>
+__device__ __forceinline__ const float* tsel<float>(const double*, const float* f) { return f; }
 __device__ __forceinline__ double rcpt(double x) { return rcp64(x); }
 __device__ __forceinline__ float rcpt(float x) { return __builtin_amdgcn_rcpf(x); }
 
@@ -910,9 +918,10 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     const long long gid = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (gid >= (long long)p.ntasks * NF) return;
     const int t = (int)(gid / NF), q = (int)(gid - (long long)t * NF);
-    const int cf = p.tasks[t];
+    const int4 td = p.tdesc[t];
+    const int cf = td.x;
     const int c = cf / 6, f = cf - 6 * c;
-    const int b = p.bid[cf], nb = p.nbr[cf];
+    const int b = td.z, nb = td.y;
     const Gas& gs = p.gas;
     const int sd = f >> 1;
     const long long g = (long long)c * NN + face_node<N>(f, q);
@@ -920,7 +929,7 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     load_state(p.Q, p.ndof, g, u);
     for (int d = 0; d < 3; ++d) n[d] = p.nrm[((long long)cf * 3 + d) * NF + q];
     const bool interior = b < 0;
-    const bool periodic = !interior && p.bcs[b].type == PLENS_BC_PERIODIC;
+    const bool periodic = td.w == PLENS_BC_PERIODIC;
     long long gn = 0;
     double bl;
     if (interior || periodic) {
@@ -946,16 +955,16 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     // F*_side the outward flux of that side (-F for the neighbour of an interior face) and
     // s = +1 / -1 for left / right faces.
     // face-node metric terms of both sides (direction Ja/|Ja|, weight |Ja|/w_end, detJ)
-    const double* fm = p.fmet + (long long)t * 10 * NF + q;
-    const TF dirc[3] = {(TF)fm[0], (TF)fm[NF], (TF)fm[2 * NF]};
-    const TF nwc = (TF)fm[3 * NF];
-    const double Jc = fm[4 * NF];
+    const TF* fm = tsel<TF>(p.fmet, p.fmet_f) + (long long)t * 10 * NF + q;
+    const TF dirc[3] = {fm[0], fm[NF], fm[2 * NF]};
+    const TF nwc = fm[3 * NF];
+    const double Jc = (double)fm[4 * NF];
     TF dirn[3] = {TF(0), TF(0), TF(0)}, nwn = TF(0);
     double Jn = 1.0;
     if (interior) {
-        for (int a = 0; a < 3; ++a) dirn[a] = (TF)fm[(5 + a) * NF];
-        nwn = (TF)fm[8 * NF];
-        Jn = fm[9 * NF];
+        for (int a = 0; a < 3; ++a) dirn[a] = fm[(5 + a) * NF];
+        nwn = fm[8 * NF];
+        Jn = (double)fm[9 * NF];
     }
     const TF sgc = (f & 1) ? TF(-1) : TF(1), sgn = -sgc;
     const long long o = (long long)cf * 5 * NF + q;
@@ -998,13 +1007,13 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
         for (int a = 0; a < 9; ++a) avg[a] = avars[a * p.ndof + gn];
     } else {
         for (int a = 0; a < 9; ++a) avg[a] = av[a];
-        if (p.bcs[b].type == PLENS_BC_INSULATED_WALL) {
+        if (td.w == PLENS_BC_INSULATED_WALL) {
             // heat flux with its normal component reversed, shear stresses kept (insulated_wall.cc,
             // as documented there; its copy loop at line 99 indexes past the 3-vector)
             const TF qn = avg[6] * nt[0] + avg[7] * nt[1] + avg[8] * nt[2];
             for (int d = 0; d < 3; ++d) avg[6 + d] = av[6 + d] - TF(2) * qn * nt[d];
         }
-        if (p.bcs[b].type != PLENS_BC_OUTFLOW) bc_ghost_state(p.bcs[b], 3, gs, u, n, ug, vin_node(p, cf, q, NF));
+        if (td.w != PLENS_BC_OUTFLOW) bc_ghost_state(p.bcs[b], 3, gs, u, n, ug, vin_node(p, cf, q, NF));
         else for (int v = 0; v < 5; ++v) ug[v] = u[v];
         for (int v = 0; v < 5; ++v) ugt[v] = (TF)ug[v];
     }
@@ -1437,6 +1446,16 @@ __global__ void __launch_bounds__(256) k_stream_copy(const double2* __restrict__
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) dst[i] = src[i];
 }
 
+// Face-task descriptors (cell*6+face, neighbour, boundary id, boundary type) from the task list,
+// the face tables and the boundary conditions, so that k_face reads one 16-byte record per task.
+__global__ void __launch_bounds__(256) k_task_desc(const int* tasks, const int* nbr, const int* bid,
+                                                   const BCDev* bcs, int ntasks, int4* out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntasks) return;
+    const int cf = tasks[t], b = bid[cf];
+    out[t] = make_int4(cf, nbr[cf], b, b < 0 ? -1 : bcs[b].type);
+}
+
 // =============================================================================================
 // k_post: output fields of PLENS::write() (dgsem/plens.cc:4170-4322) for one cell per workgroup,
 // one thread per node: the stage-1 avars and post_process (3950-3963: p, T = e / c_v, velocity =
@@ -1520,6 +1539,7 @@ struct plens_gpu {
     double *Ja = nullptr, *detJ = nullptr, *nrm = nullptr, *hmin = nullptr;
     double* cmap = nullptr;  // trilinear maps of the cells; the stored metric arrays are not kept then
     double* fmet = nullptr;  // face-node metric terms of the face tasks
+    float* fmet_f = nullptr; // the same in FP32 (FP32-bulk)
     double* tabs = nullptr;      // solution time per instance, and at the start of the current step
     double* rhoE_old = nullptr;  // rho E at the start of the last step
     double* dt_lts = nullptr;    // local time stepping: per-cell steps (allocated on request)
@@ -1534,6 +1554,7 @@ struct plens_gpu {
     long long steps_taken = 0;
     bool use_cmap = false;
     int *nbr = nullptr, *bid = nullptr, *wall = nullptr, *tasks = nullptr;
+    int4* tdesc = nullptr;
     BCDev* bcs = nullptr;
     Basis* basis = nullptr;
     double *avars = nullptr, *araw = nullptr, *alpha = nullptr, *alpha_d = nullptr;
@@ -1878,9 +1899,12 @@ int alloc_buffers(plens_gpu* h) {
     int rc;
     if ((rc = dmalloc(&h->Qa, 5 * nd)) || (rc = dmalloc(&h->Qb, 5 * nd)) || (rc = dmalloc(&h->R[0], 5 * nd)) ||
         (rc = dmalloc(&h->R[1], 5 * nd)) || (rc = dmalloc(&h->rhs, 5 * nd)) || (rc = dmalloc(&h->rlo, 5 * nd)) ||
-        (rc = dmalloc(&h->nrm, nc * 18 * NF)) || (rc = dmalloc(&h->fmet, (size_t)h->ntasks * 10 * NF)) ||
+        (rc = dmalloc(&h->nrm, nc * 18 * NF)) ||
+        (rc = (h->precision == PLENS_PREC_FP64 ? dmalloc(&h->fmet, (size_t)h->ntasks * 10 * NF)
+                                               : dmalloc(&h->fmet_f, (size_t)h->ntasks * 10 * NF))) ||
         (rc = dmalloc(&h->hmin, nc)) || (rc = dmalloc(&h->nbr, nc * 6)) || (rc = dmalloc(&h->bid, nc * 6)) ||
         (rc = dmalloc(&h->wall, nc)) || (rc = dmalloc(&h->tasks, (size_t)h->ntasks)) ||
+        (rc = dmalloc(&h->tdesc, (size_t)h->ntasks)) ||
         (rc = dmalloc(&h->bcs, (size_t)std::max(h->n_bcs, 1))) || (rc = dmalloc(&h->basis, 1)) ||
         (rc = dmalloc(&h->araw, nc)) || (rc = dmalloc(&h->alpha, nc)) || (rc = dmalloc(&h->alpha_d, nc)) ||
         (rc = dmalloc(&h->dt_cell, nc)) || (rc = dmalloc(&h->cinst, nc)) ||
@@ -1924,8 +1948,8 @@ int alloc_buffers(plens_gpu* h) {
 void bind_buffers(plens_gpu* h) {
     KP& k = h->base;
     k.nc = h->nc; k.cpb = h->cpb; k.ntasks = h->ntasks; k.ndof = h->ndof; k.ninst = h->ninst;
-    k.Ja = h->Ja; k.detJ = h->detJ; k.cmap = h->cmap; k.fmet = h->fmet; k.nrm = h->nrm; k.hmin = h->hmin;
-    k.nbr = h->nbr; k.bid = h->bid; k.wall = h->wall; k.tasks = h->tasks; k.bcs = h->bcs; k.basis = h->basis;
+    k.Ja = h->Ja; k.detJ = h->detJ; k.cmap = h->cmap; k.fmet = h->fmet; k.fmet_f = h->fmet_f; k.nrm = h->nrm; k.hmin = h->hmin;
+    k.nbr = h->nbr; k.bid = h->bid; k.wall = h->wall; k.tasks = h->tasks; k.tdesc = h->tdesc; k.bcs = h->bcs; k.basis = h->basis;
     k.avars = h->avars; k.araw = h->araw; k.alpha = h->alpha; k.alpha_d = h->alpha_d;
     k.sinv = h->sinv; k.sface = h->sface; k.avars_f = h->avars_f; k.sinv_f = h->sinv_f; k.sface_f = h->sface_f;
     k.rhs_out = h->rhs; k.rlo = h->rlo;
@@ -1938,6 +1962,14 @@ void bind_buffers(plens_gpu* h) {
     k.rhoE_old = h->rhoE_old;
     k.dt_lts = h->dt_lts;
     k.save_E = 0;
+}
+
+// fills h->tdesc from the uploaded task list, face tables and boundary conditions
+int build_task_desc(plens_gpu* h) {
+    if (h->ntasks == 0) return PLENS_OK;
+    hipLaunchKernelGGL(k_task_desc, dim3((unsigned)((h->ntasks + 255) / 256)), dim3(256), 0, h->stream, h->tasks,
+                       h->nbr, h->bid, h->bcs, h->ntasks, h->tdesc);
+    return (hipGetLastError() == hipSuccess && hipStreamSynchronize(h->stream) == hipSuccess) ? PLENS_OK : PLENS_E_HIP;
 }
 
 void destroy_graphs(plens_gpu* h) {
@@ -2182,8 +2214,14 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
                 if (!interior) o[9 * NF] = 1.0;
             }
         }
-        if (hipMemcpy(h->fmet, fm.data(), sizeof(double) * fm.size(), hipMemcpyHostToDevice) != hipSuccess)
-            return fail(PLENS_E_HIP);
+        if (h->precision == PLENS_PREC_FP64) {
+            if (hipMemcpy(h->fmet, fm.data(), sizeof(double) * fm.size(), hipMemcpyHostToDevice) != hipSuccess)
+                return fail(PLENS_E_HIP);
+        } else {
+            const std::vector<float> ff(fm.begin(), fm.end());
+            if (hipMemcpy(h->fmet_f, ff.data(), sizeof(float) * ff.size(), hipMemcpyHostToDevice) != hipSuccess)
+                return fail(PLENS_E_HIP);
+        }
     }
     if (hipMemcpy(h->nrm, m->face_normals, sizeof(double) * nc * 18 * NF, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->hmin, m->min_vertex_dist, sizeof(double) * nc, hipMemcpyHostToDevice) != hipSuccess ||
@@ -2197,6 +2235,7 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
         hipMemcpy(h->ioff, ioff_h, sizeof(ioff_h), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->visc, visc_h, sizeof(visc_h), hipMemcpyHostToDevice) != hipSuccess)
         return fail(PLENS_E_HIP);
+    if ((rc = build_task_desc(h))) return fail(rc);
 
     KP& k = h->base;
     bind_buffers(h);
@@ -2230,9 +2269,9 @@ void plens_gpu_destroy(plens_gpu* h) {
     for (auto& x : h->ev_run)
         if (x) (void)hipEventDestroy(x);
     void* ptrs[] = {h->Qa, h->Qb, h->R[0], h->R[1], h->Ja, h->detJ, h->nrm, h->hmin, h->nbr,
-                    h->bid, h->wall, h->tasks, h->bcs, h->basis, h->avars, h->araw, h->alpha, h->alpha_d,
+                    h->bid, h->wall, h->tasks, h->tdesc, h->bcs, h->basis, h->avars, h->araw, h->alpha, h->alpha_d,
                     h->sinv, h->sface, h->rhs, h->rlo, h->dt_cell, h->scal, h->fail, h->blist, h->bcount, h->mlp_stats, h->feat,
-                    h->avars_f, h->sinv_f, h->sface_f, h->cinst, h->ioff, h->visc, h->cfl, h->cmap, h->fmet,
+                    h->avars_f, h->sinv_f, h->sface_f, h->cinst, h->ioff, h->visc, h->cfl, h->cmap, h->fmet, h->fmet_f,
                     h->tabs, h->rhoE_old, h->dt_lts, h->ss_part, h->post, h->vslot, h->vin, h->vxyz,
                     h->vbc, h->vcode};
     for (void* p : ptrs)
@@ -2787,8 +2826,12 @@ int merge_group(const std::vector<plens_gpu*>& mem, hipStream_t stream, plens_gp
         HIPCHK(hipMemcpy(bid.data() + (size_t)co * 6, m->bid, sizeof(int) * m->nc * 6, hipMemcpyDeviceToHost));
         if (m->ntasks > 0) {
             HIPCHK(hipMemcpy(tasks.data() + to, m->tasks, sizeof(int) * m->ntasks, hipMemcpyDeviceToHost));
-            HIPCHK(hipMemcpy(h->fmet + (size_t)to * 10 * NF, m->fmet, sizeof(double) * m->ntasks * 10 * NF,
-                             hipMemcpyDeviceToDevice));
+            if (h->precision == PLENS_PREC_FP64)
+                HIPCHK(hipMemcpy(h->fmet + (size_t)to * 10 * NF, m->fmet, sizeof(double) * m->ntasks * 10 * NF,
+                                 hipMemcpyDeviceToDevice));
+            else
+                HIPCHK(hipMemcpy(h->fmet_f + (size_t)to * 10 * NF, m->fmet_f, sizeof(float) * m->ntasks * 10 * NF,
+                                 hipMemcpyDeviceToDevice));
         }
         for (int k = co * 6; k < (co + m->nc) * 6; ++k) {
             if (nbr[k] >= 0) nbr[k] += co;
@@ -2813,6 +2856,7 @@ int merge_group(const std::vector<plens_gpu*>& mem, hipStream_t stream, plens_gp
         hipMemcpy(h->visc, visc.data(), sizeof(double) * visc.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->basis, m0->basis, sizeof(Basis), hipMemcpyDeviceToDevice) != hipSuccess)
         return fail(PLENS_E_HIP);
+    if ((rc = build_task_desc(h))) return fail(rc);
     if ((rc = dmalloc(&h->cfl, (size_t)h->ninst))) return fail(rc);
     h->base = m0->base;
     bind_buffers(h);
