@@ -463,28 +463,61 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         sV[li] = s;
     }
     __syncthreads();
-    if (inblk) {  // z pass: Legendre mode of this node's index; energies E, E_{N-1}, E_{N-2} summands
+    double e0 = 0.0, e1 = 0.0, e2 = 0.0;  // this node's summands of E, E_{N-1}, E_{N-2}
+    if (inblk) {  // z pass: Legendre mode of this node's index
         double m = 0;
         for (int cc = 0; cc < NP; ++cc) m += sC[tk * NP + cc] * sV[lc * NN + ti + NP * (tj + NP * cc)];
         sT[li] = m;
-        const double m2 = m * m;
-        sE[li] = m2;
-        sE[B + li] = (ti < N && tj < N && tk < N) ? m2 : 0.0;
-        sE[2 * B + li] = (ti < N - 1 && tj < N - 1 && tk < N - 1) ? m2 : 0.0;
+        e0 = m * m;
+        e1 = (ti < N && tj < N && tk < N) ? e0 : 0.0;
+        e2 = (ti < N - 1 && tj < N - 1 && tk < N - 1) ? e0 : 0.0;
     }
-    // per-cell tree sums over the NN modes
-    for (int st = kPow2Half<NN>(); st > 0; st >>= 1) {
+    // per-cell sums over the NN modes: butterfly sums within each wavefront when the cell's nodes fill
+    // whole wavefronts (one cell per workgroup, or 64 nodes per cell), else an LDS tree
+    const bool wave_sum = (NN % 64 == 0) || p.cpb == 1;
+    if (wave_sum) {
+        for (int o = 32; o > 0; o >>= 1) {
+            e0 += __shfl_xor(e0, o);
+            e1 += __shfl_xor(e1, o);
+            e2 += __shfl_xor(e2, o);
+        }
+        if ((tid & 63) == 0) {
+            sE[3 * (tid >> 6)] = e0;
+            sE[3 * (tid >> 6) + 1] = e1;
+            sE[3 * (tid >> 6) + 2] = e2;
+        }
         __syncthreads();
-        if (inblk && node < st && node + st < NN) {
-            sE[li] += sE[li + st];
-            sE[B + li] += sE[B + li + st];
-            sE[2 * B + li] += sE[2 * B + li + st];
+        if (act && node == 0) {
+            const int w0 = (lc * NN) >> 6, nw = (NN + 63) >> 6;
+            e0 = sE[3 * w0]; e1 = sE[3 * w0 + 1]; e2 = sE[3 * w0 + 2];
+            for (int w = 1; w < nw; ++w) {
+                e0 += sE[3 * (w0 + w)];
+                e1 += sE[3 * (w0 + w) + 1];
+                e2 += sE[3 * (w0 + w) + 2];
+            }
+        }
+    } else {
+        if (inblk) {
+            sE[li] = e0;
+            sE[B + li] = e1;
+            sE[2 * B + li] = e2;
+        }
+        for (int st = kPow2Half<NN>(); st > 0; st >>= 1) {
+            __syncthreads();
+            if (inblk && node < st && node + st < NN) {
+                sE[li] += sE[li + st];
+                sE[B + li] += sE[B + li + st];
+                sE[2 * B + li] += sE[2 * B + li + st];
+            }
+        }
+        __syncthreads();
+        if (act && node == 0) {
+            e0 = sE[li]; e1 = sE[B + li]; e2 = sE[2 * B + li];
         }
     }
-    __syncthreads();
     double a_plain = 0.0;
     if (act && node == 0) {  // get_trouble (blender_calculator.cc:277-303), Hennemann modification for N>2
-        const double et = sE[li], e1 = sE[B + li], e2 = sE[2 * B + li];
+        const double et = e0;
         const double trouble = (N > 2) ? fmax(1.0 - e1 / et, 1.0 - e2 / e1) : 1.0 - e1 / et;
         const double at = 1.0 / (1.0 + exp(-p.sharp * (trouble / p.thr - 1.0)));
         a_plain = (at < p.amin) ? 0.0 : (at < p.amax ? at : p.amax);
@@ -541,13 +574,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     TF ut[5];
     for (int v = 0; v < 5; ++v) ut[v] = (TF)u[v];
     const int tt[3] = {ti, tj, tk};
+    // direction outer, line node inner: the node's own metric row is read once per direction
     if (p.evar) {
+        for (int md = 0; md < 3; ++md) {
+            const int st = md == 0 ? 1 : (md == 1 ? NP : NP * NP);  // node stride along direction md
+            const int j0 = li - tt[md] * st;                       // first node of this node's line
 #pragma unroll 1
-        for (int m = 0; m <= N; ++m) {
-            for (int md = 0; md < 3; ++md) {
-                int to[3] = {ti, tj, tk};
-                to[md] = m;
-                const int jo = lc * NN + to[0] + NP * (to[1] + NP * to[2]);
+            for (int m = 0; m <= N; ++m) {
+                const int jo = j0 + m * st;
                 const TF Dv = (TF)sD[tt[md] * NP + m];
                 for (int gd = 0; gd < 3; ++gd) {
                     const TF jw = Dv * sJa[(3 * md + gd) * B + jo];
@@ -556,18 +590,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             }
         }
     } else {
+        for (int md = 0; md < 3; ++md) {
+            const int st = md == 0 ? 1 : (md == 1 ? NP : NP * NP);
+            const int j0 = li - tt[md] * st;
+            TF jown[3];
+            for (int gd = 0; gd < 3; ++gd) jown[gd] = sJa[(3 * md + gd) * B + li];
 #pragma unroll 1
-        for (int m = 0; m <= N; ++m) {
-            for (int md = 0; md < 3; ++md) {
-                int to[3] = {ti, tj, tk};
-                to[md] = m;
-                const int jo = lc * NN + to[0] + NP * (to[1] + NP * to[2]);
+            for (int m = 0; m <= N; ++m) {
+                const int jo = j0 + m * st;
                 // 2 D (u_i + u_m)/2 (Ja_i + Ja_m)/2 grouped as (u_i + u_m) (D/2 (Ja_i + Ja_m))
                 TF su[5];
                 for (int v = 0; v < 5; ++v) su[v] = ut[v] + sQ[v * B + jo];
                 const TF Dh = (TF)(0.5 * sD[tt[md] * NP + m]);
                 for (int gd = 0; gd < 3; ++gd) {
-                    const TF jav = Dh * (sJa[(3 * md + gd) * B + li] + sJa[(3 * md + gd) * B + jo]);
+                    const TF jav = Dh * (jown[gd] + sJa[(3 * md + gd) * B + jo]);
                     for (int v = 0; v < 5; ++v) gr[gd][v] += su[v] * jav;
                 }
             }
