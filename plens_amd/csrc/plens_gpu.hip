@@ -82,6 +82,8 @@ struct KP {
     const int* wall;
     const int* tasks;
     const int4* tdesc;       // per face task: cell*6+face, neighbour, boundary id, boundary type (-1 interior)
+    unsigned long long* dtmin;  // per instance: running minimum of the cell steps' bit patterns (all ones = none)
+    unsigned int* dtcount;      // blocks of k_alpha done with their minimum
     const BCDev* bcs;
     const Basis* basis;
     double* avars;
@@ -533,6 +535,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     // ---- stage-1 time step candidates (PLENS::calc_time_step, dgsem/plens.cc:3836-3866)
     if (p.want_dt) {
         __syncthreads();
+        double dmin = 1e300;  // this node's candidate (none outside the cells)
         if (inblk) {
             double cur = 1e300;
             if (act) {
@@ -542,7 +545,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                 const double vel[3] = {u[1] * irho, u[2] * irho, u[3] * irho};
                 double madv = -1e300;
                 double ja[9], dJ;
-                node_metrics<N>(p, c, node, ja, dJ);
+                if (sizeof(TF) == sizeof(double))  // the stored metric terms are exact
+                    for (int q = 0; q < 9; ++q) ja[q] = (double)sJa[q * B + li];
+                else
+                    node_metrics<N>(p, c, node, ja, dJ);
                 for (int r = 0; r < 3; ++r) {  // (J^-T v)_r with J^-1 rows = Ja^i / detJ
                     double s = 0;
                     for (int cc = 0; cc < 3; ++cc) s += (ja[3 * cc + r] * idet) * vel[cc];
@@ -552,14 +558,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                 const double co = cr >= 1.0 ? 0.9 : cr;
                 cur = co / (pow((double)N, 1.5) * (madv + a / h + (double)(N * N) * mu / (h * h * u[0])));
             }
-            sE[li] = cur;
+            if (!wave_sum) sE[li] = cur;
+            dmin = cur;
         }
-        for (int st = kPow2Half<NN>(); st > 0; st >>= 1) {  // per-cell tree minimum
+        if (wave_sum) {  // per-cell minimum (exact in any order): wavefront butterflies, then the waves
+            for (int o = 32; o > 0; o >>= 1) dmin = fmin(dmin, __shfl_xor(dmin, o));
+            if ((tid & 63) == 0) sE[tid >> 6] = dmin;
             __syncthreads();
-            if (inblk && node < st && node + st < NN) sE[li] = fmin(sE[li], sE[li + st]);
+            if (act && node == 0) {
+                const int w0 = (lc * NN) >> 6, nw = (NN + 63) >> 6;
+                for (int w = 0; w < nw; ++w) dmin = fmin(dmin, sE[w0 + w]);
+                p.dt_cell[c] = dmin;
+            }
+        } else {
+            for (int st = kPow2Half<NN>(); st > 0; st >>= 1) {  // per-cell tree minimum
+                __syncthreads();
+                if (inblk && node < st && node + st < NN) sE[li] = fmin(sE[li], sE[li + st]);
+            }
+            __syncthreads();
+            if (act && node == 0) p.dt_cell[c] = sE[li];
         }
-        __syncthreads();
-        if (act && node == 0) p.dt_cell[c] = sE[li];
         if (act && p.save_E) p.rhoE_old[g] = u[4];  // update(), dgsem/plens.cc:4507-4508
     }
     if (gs.inviscid) return;
@@ -791,27 +809,50 @@ __global__ void __launch_bounds__(kThreadsAlpha) k_alpha(KP p) {
         if (f != ~0ull && p.fail[1] == ~0ull) p.fail[1] = f;
     }
     if (!p.want_dt) return;
-    // global time step of every instance: minimum over its cells (dgsem/plens.cc:3872-3875, 3905-3910)
-    __shared__ double red[kThreadsAlpha];
-    for (int i = blockIdx.x; i < p.ninst; i += gridDim.x) {
-        double m = 1e300;
-        for (int t = p.ioff[i] + threadIdx.x; t < p.ioff[i + 1]; t += blockDim.x) m = fmin(m, p.dt_cell[t]);
-        red[threadIdx.x] = m;
-        __syncthreads();
-        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + s]);
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            p.dt[i] = red[0];
-            if (p.t_acc) {
-                p.t_acc[i] += red[0];
-                p.tabs[p.ninst + i] = p.tabs[i];  // solution time at the start of this step
-                p.tabs[i] += red[0];
-            }
-        }
-        __syncthreads();
+    // global time step of every instance: minimum over its cells (dgsem/plens.cc:3872-3875, 3905-3910).
+    // The cell steps are positive, so their bit patterns order as the values: every block folds its
+    // cells into the instance's slot with one atomic minimum (per cell where the block holds cells of
+    // two instances), and the last block to finish publishes the steps and resets the slots. The
+    // minimum does not depend on the order, so the result is deterministic.
+    __shared__ unsigned long long red[kThreadsAlpha / 64];
+    __shared__ bool last;
+    unsigned long long key = ~0ull;
+    if (c < p.nc) {
+        const double m = p.dt_cell[c];
+        key = m > 0.0 ? (unsigned long long)__double_as_longlong(m) : 0ull;  // a failed state: step 0
     }
+    const int cf = blockIdx.x * blockDim.x, cl = min(p.nc, cf + (int)blockDim.x) - 1;
+    const int i0 = p.cinst[cf];
+    if (i0 == p.cinst[cl]) {
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long k2 = __shfl_xor(key, o);
+            key = k2 < key ? k2 : key;
+        }
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = key;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < (int)(blockDim.x >> 6); ++w) key = red[w] < key ? red[w] : key;
+            atomicMin(p.dtmin + i0, key);
+        }
+    } else if (c < p.nc) {
+        atomicMin(p.dtmin + p.cinst[c], key);
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(p.dtcount, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    for (int i = threadIdx.x; i < p.ninst; i += blockDim.x) {
+        const double dt = __longlong_as_double((long long)atomicExch(p.dtmin + i, ~0ull));
+        p.dt[i] = dt;
+        if (p.t_acc) {
+            p.t_acc[i] += dt;
+            p.tabs[p.ninst + i] = p.tabs[i];  // solution time at the start of this step
+            p.tabs[i] += dt;
+        }
+    }
+    if (threadIdx.x == 0) *p.dtcount = 0u;
 }
 
 // =============================================================================================
@@ -1591,6 +1632,8 @@ struct plens_gpu {
     bool use_cmap = false;
     int *nbr = nullptr, *bid = nullptr, *wall = nullptr, *tasks = nullptr;
     int4* tdesc = nullptr;
+    unsigned long long* dtmin = nullptr;  // step reduction slots (per instance) and block counter
+    unsigned int* dtcount = nullptr;
     BCDev* bcs = nullptr;
     Basis* basis = nullptr;
     double *avars = nullptr, *araw = nullptr, *alpha = nullptr, *alpha_d = nullptr;
@@ -1940,7 +1983,8 @@ int alloc_buffers(plens_gpu* h) {
                                                : dmalloc(&h->fmet_f, (size_t)h->ntasks * 10 * NF))) ||
         (rc = dmalloc(&h->hmin, nc)) || (rc = dmalloc(&h->nbr, nc * 6)) || (rc = dmalloc(&h->bid, nc * 6)) ||
         (rc = dmalloc(&h->wall, nc)) || (rc = dmalloc(&h->tasks, (size_t)h->ntasks)) ||
-        (rc = dmalloc(&h->tdesc, (size_t)h->ntasks)) ||
+        (rc = dmalloc(&h->tdesc, (size_t)h->ntasks)) || (rc = dmalloc(&h->dtmin, (size_t)h->ninst)) ||
+        (rc = dmalloc(&h->dtcount, 1)) ||
         (rc = dmalloc(&h->bcs, (size_t)std::max(h->n_bcs, 1))) || (rc = dmalloc(&h->basis, 1)) ||
         (rc = dmalloc(&h->araw, nc)) || (rc = dmalloc(&h->alpha, nc)) || (rc = dmalloc(&h->alpha_d, nc)) ||
         (rc = dmalloc(&h->dt_cell, nc)) || (rc = dmalloc(&h->cinst, nc)) ||
@@ -1950,7 +1994,10 @@ int alloc_buffers(plens_gpu* h) {
         (rc = dmalloc(&h->tabs, 2 * (size_t)h->ninst)) || (rc = dmalloc(&h->rhoE_old, nd)) ||
         (rc = dmalloc(&h->ss_part, 2 * nc + 2)))
         return rc;
-    if (hipMemset(h->tabs, 0, sizeof(double) * 2 * h->ninst) != hipSuccess) return PLENS_E_HIP;
+    if (hipMemset(h->tabs, 0, sizeof(double) * 2 * h->ninst) != hipSuccess ||
+        hipMemset(h->dtmin, 0xff, sizeof(unsigned long long) * h->ninst) != hipSuccess ||
+        hipMemset(h->dtcount, 0, sizeof(unsigned int)) != hipSuccess)
+        return PLENS_E_HIP;
     if (h->use_cmap) {
         if ((rc = dmalloc(&h->cmap, nc * 24))) return rc;
     } else if ((rc = dmalloc(&h->Ja, 9 * nd)) || (rc = dmalloc(&h->detJ, nd))) {
@@ -1985,7 +2032,8 @@ void bind_buffers(plens_gpu* h) {
     KP& k = h->base;
     k.nc = h->nc; k.cpb = h->cpb; k.ntasks = h->ntasks; k.ndof = h->ndof; k.ninst = h->ninst;
     k.Ja = h->Ja; k.detJ = h->detJ; k.cmap = h->cmap; k.fmet = h->fmet; k.fmet_f = h->fmet_f; k.nrm = h->nrm; k.hmin = h->hmin;
-    k.nbr = h->nbr; k.bid = h->bid; k.wall = h->wall; k.tasks = h->tasks; k.tdesc = h->tdesc; k.bcs = h->bcs; k.basis = h->basis;
+    k.nbr = h->nbr; k.bid = h->bid; k.wall = h->wall; k.tasks = h->tasks; k.tdesc = h->tdesc; k.bcs = h->bcs;
+    k.dtmin = h->dtmin; k.dtcount = h->dtcount; k.basis = h->basis;
     k.avars = h->avars; k.araw = h->araw; k.alpha = h->alpha; k.alpha_d = h->alpha_d;
     k.sinv = h->sinv; k.sface = h->sface; k.avars_f = h->avars_f; k.sinv_f = h->sinv_f; k.sface_f = h->sface_f;
     k.rhs_out = h->rhs; k.rlo = h->rlo;
@@ -2305,7 +2353,7 @@ void plens_gpu_destroy(plens_gpu* h) {
     for (auto& x : h->ev_run)
         if (x) (void)hipEventDestroy(x);
     void* ptrs[] = {h->Qa, h->Qb, h->R[0], h->R[1], h->Ja, h->detJ, h->nrm, h->hmin, h->nbr,
-                    h->bid, h->wall, h->tasks, h->tdesc, h->bcs, h->basis, h->avars, h->araw, h->alpha, h->alpha_d,
+                    h->bid, h->wall, h->tasks, h->tdesc, h->dtmin, h->dtcount, h->bcs, h->basis, h->avars, h->araw, h->alpha, h->alpha_d,
                     h->sinv, h->sface, h->rhs, h->rlo, h->dt_cell, h->scal, h->fail, h->blist, h->bcount, h->mlp_stats, h->feat,
                     h->avars_f, h->sinv_f, h->sface_f, h->cinst, h->ioff, h->visc, h->cfl, h->cmap, h->fmet, h->fmet_f,
                     h->tabs, h->rhoE_old, h->dt_lts, h->ss_part, h->post, h->vslot, h->vin, h->vxyz,
