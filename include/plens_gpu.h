@@ -234,7 +234,8 @@ int plens_gpu_read_checkpoint(plens_gpu* h, const char* path, double* time, int6
                               int64_t* output_counter);
 
 /* Kernel timing with no counterpart in the reference: after one RHS evaluation of the current
- * state, launches kernel class `kernel` (0 gradient/blender, 2 face flux, 4 volume + RK update)
+ * state, launches kernel class `kernel` (0 gradient/blender, 2 face flux, 3 subcell FV residual of the
+ * blended cells, 4 volume + RK update)
  * back to back on the handle's stream, `reps` times with the parameters of each of the 5 RK stages
  * in turn (new states go to a scratch buffer), between two HIP events, and returns the average
  * duration of one launch. The state is not modified; the residual registers are (they are dead

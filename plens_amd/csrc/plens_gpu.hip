@@ -2699,7 +2699,8 @@ int plens_gpu_read_checkpoint(plens_gpu* h, const char* path, double* time, int6
 }
 
 int plens_gpu_time_kernel(plens_gpu* h, int32_t kernel, int32_t reps, double* avg_ms) {
-    if (!h || !avg_ms || reps < 1 || (kernel != 0 && kernel != 2 && kernel != 4)) return PLENS_E_INVALID;
+    // k_alpha (1) appends to the blended-cell list, so it is not launched twice without k_grad
+    if (!h || !avg_ms || reps < 1 || kernel < 0 || kernel > 4 || kernel == 1) return PLENS_E_INVALID;
     int e = plens_gpu_calc_rhs(h, nullptr, nullptr, nullptr);  // every input of the kernel current
     if (e) return e;
     // the kernel with the parameters of RK stages 1..5 in turn (the mix of a time step), reading the

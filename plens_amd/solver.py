@@ -152,9 +152,9 @@ class GpuPlens:
         return nodal, cell
 
     def time_kernel(self, kernel, reps=20):
-        """average duration (ms) of one launch of kernel class `kernel` ("grad", "face" or "vol"),
+        """average duration (ms) of one launch of kernel class `kernel` ("grad", "face", "lo" or "vol"),
         launched `reps` times back to back after an RHS evaluation; the state is unchanged"""
-        k = {"grad": 0, "face": 2, "vol": 4}[kernel]
+        k = {"grad": 0, "face": 2, "lo": 3, "vol": 4}[kernel]
         ms = C.c_double()
         self._check(self.lib.plens_gpu_time_kernel(self._h, k, reps, C.byref(ms)), "time_kernel")
         return ms.value

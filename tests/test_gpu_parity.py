@@ -400,6 +400,7 @@ def test_ss_error_global_stepping_and_measurement_calls():
     e0 = g.ss_error()
     g.calc_rhs()
     g.time_kernel("grad", reps=2)
+    g.time_kernel("lo", reps=2)
     g.time_kernel("vol", reps=2)
     e1 = g.ss_error()
     print(f"ss error gpu {e0:.10e} cpu {o.ss_error():.10e}", flush=True)
