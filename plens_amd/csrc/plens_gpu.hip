@@ -2080,6 +2080,60 @@ const char* plens_gpu_strerror(int code) {
     }
 }
 
+namespace {
+// 21-bit spread for a 63-bit Morton code
+uint64_t morton_spread(uint64_t v) {
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffull;
+    v = (v | v << 16) & 0x1f0000ff0000ffull;
+    v = (v | v << 8) & 0x100f00f00f00f00full;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+    v = (v | v << 2) & 0x1249249249249249ull;
+    return v;
+}
+
+// Face tasks in the Morton order of their cells' centroids (from the vertices, else the DoF
+// locations; unchanged without either): the cells a task gathers from were then gathered by tasks
+// shortly before it, on the same XCD, and are still in its L2. Ties keep the cell order, and a
+// cell's tasks keep their face order; the tasks write disjoint outputs, so the order does not
+// change any result.
+void order_tasks(const plens_mesh_desc* m, int N, std::vector<int>& tasks) {
+    const int nc = m->n_cells, NN = (N + 1) * (N + 1) * (N + 1);
+    if (nc <= 1 || (!m->vertices && !m->node_coords)) return;
+    std::vector<double> cen((size_t)nc * 3, 0.0);
+    for (int c = 0; c < nc; ++c)
+        for (int d = 0; d < 3; ++d) {
+            double a = 0;
+            if (m->vertices) {
+                for (int v = 0; v < 8; ++v) a += m->vertices[((size_t)c * 8 + v) * 3 + d];
+                a /= 8;
+            } else {
+                for (int k = 0; k < NN; ++k) a += m->node_coords[((size_t)c * 3 + d) * NN + k];
+                a /= NN;
+            }
+            cen[(size_t)c * 3 + d] = a;
+        }
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) lo[d] = hi[d] = cen[d];
+    for (int c = 1; c < nc; ++c)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], cen[(size_t)c * 3 + d]);
+            hi[d] = std::max(hi[d], cen[(size_t)c * 3 + d]);
+        }
+    const double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    if (!(ext > 0)) return;
+    const double sc = 2097151.0 / ext;  // one scale for the three axes
+    std::vector<uint64_t> code(nc);
+    for (int c = 0; c < nc; ++c) {
+        uint64_t k = 0;
+        for (int d = 0; d < 3; ++d)
+            k |= morton_spread((uint64_t)((cen[(size_t)c * 3 + d] - lo[d]) * sc)) << d;
+        code[c] = k;
+    }
+    std::stable_sort(tasks.begin(), tasks.end(), [&](int a, int b) { return code[a / 6] < code[b / 6]; });
+}
+}  // namespace
+
 int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const plens_bc* bcs, int32_t n_bcs,
                      plens_gpu** out) {
     if (!m || !ph || !out || (n_bcs > 0 && !bcs)) return PLENS_E_INVALID;
@@ -2131,6 +2185,7 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
                 tasks.push_back(c * 6 + f);
             }
         }
+    order_tasks(m, N, tasks);
     h->ntasks = (int)tasks.size();
     if ((rc = alloc_buffers(h))) return fail(rc);
     // host-side tables
