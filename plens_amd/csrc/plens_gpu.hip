@@ -1,22 +1,24 @@
 // MI355X (gfx950) DGSEM right-hand side + RK4(3)5[3R+]N update behind the C ABI of plens_gpu.h.
 //
-// Per RK stage four kernels run on the handle's stream:
+// Per RK stage five kernels run on the handle's stream:
 //   k_grad   one workgroup per batch of cells, one thread per GLL node: positivity check,
 //            Sutherland mu/k, Hennemann blender (sum-factorised modal transform), the stage-1
-//            time-step candidates, BR1 conservative gradients with stage-1 face values gathered
-//            from the neighbours, and the auxiliary variables (stress, heat flux)
-//            (PLENS::calc_aux_vars / calc_cell_cons_grad / calc_blender, dgsem/plens.cc:2593-2748,
-//            2174-2285, 2858-2962; calc_time_step 3810-3912)
+//            time-step candidates, BR1 gradients of the conservative or entropy variables with the
+//            stage-1 face values of the neighbours, and the auxiliary variables (stress, heat flux)
+//            (PLENS::calc_aux_vars / calc_cell_cons_grad / calc_cell_evar_grad / calc_blender,
+//            dgsem/plens.cc:2593-2748, 2174-2405, 2858-2962; calc_time_step 3810-3912)
 //   k_alpha  one thread per cell: wall override, face-neighbour diffusion and alpha_d of the
-//            blender; block 0 also reduces the time step (dgsem/plens.cc:2905-2960, 3872-3875)
-//   k_face   one thread per face node: stage-2 inviscid numerical flux and stage-3 BR1 viscous
-//            flux, computed once by the lower-index cell of an interior face and stored for both
-//            sides, each side on its own for boundary (incl. periodic) faces
-//            (PLENS::calc_surf_flux, dgsem/plens.cc:1717-1913)
+//            blender, the list of blended cells; at stage 1 the time step of every instance
+//            (dgsem/plens.cc:2905-2960, 3872-3875)
+//   k_face   one thread per face node of a face task (tasks in the Morton order of their cells):
+//            stage-2 inviscid numerical flux and stage-3 BR1 viscous flux, computed once by the
+//            lower-index cell of an interior face and stored for both sides, each side on its own
+//            for boundary (incl. periodic) faces (PLENS::calc_surf_flux, dgsem/plens.cc:1717-1913)
+//   k_lo     persistent over the blended cells: first-order subcell FV residual
+//            (calc_cell_lo_inv_residual, dgsem/plens.cc:3469-3583)
 //   k_vol    one workgroup per batch of cells, one thread per node: split-form inviscid volume
-//            term, viscous volume term, surface terms, first-order subcell FV for blended cells,
-//            the alpha blend and the fused low-storage RK epilogue (dgsem/plens.cc:3153-3583,
-//            3688-3779, 4334-4403)
+//            term, viscous volume term, surface terms, the alpha blend and the fused low-storage RK
+//            epilogue (dgsem/plens.cc:3153-3413, 3688-3779, 4334-4403)
 // A time step (5 stages) is captured once into a HIP graph and replayed.
 #include <hip/hip_runtime.h>
 
