@@ -872,6 +872,7 @@ struct oracle {
     std::vector<T1> sub;                                // subcell normals [c][dir][(N+2)(N+1)^2]
     // solution vectors
     std::vector<double> Q, avars, mu, kap, alpha, alpha_d, rhs, rhs1, rhs2, Qn, loc_dt;
+    std::vector<double> alpha_given, alpha_d_given;  // checker hook: blender values to use, when set
     std::vector<double> s1, s2, s3;                     // surface terms [var][c][f][q]
     std::vector<double> ws1;                            // entropy variables of s1 (default build)
     std::vector<double> verts;                          // cell vertices [c][8][3]
@@ -1607,7 +1608,12 @@ struct oracle {
         lap(0);
         if (!ns.inviscid()) calc_aux_vars();
         lap(1);
-        calc_blender();
+        if (!alpha_given.empty()) {
+            alpha = alpha_given;
+            alpha_d = alpha_d_given;
+        } else {
+            calc_blender();
+        }
         lap(2);
         calc_surf_flux(2, s2);
         if (!ns.inviscid()) calc_surf_flux(3, s3);
@@ -1900,6 +1906,17 @@ int oracle_node_location(const oracle* o, int32_t c, int32_t node, double* xyz) 
 
 int oracle_set_time(oracle* o, double t) {
     o->bc_time = t;
+    return 0;
+}
+
+int oracle_set_alpha(oracle* o, const double* alpha, const double* alpha_d) {
+    if (!alpha || !alpha_d) {
+        o->alpha_given.clear();
+        o->alpha_d_given.clear();
+        return 0;
+    }
+    o->alpha_given.assign(alpha, alpha + o->nc);
+    o->alpha_d_given.assign(alpha_d, alpha_d + o->nc);
     return 0;
 }
 

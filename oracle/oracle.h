@@ -53,6 +53,10 @@ int oracle_create_mapped(int32_t n_cells, int32_t degree, int32_t map_degree, co
 void oracle_destroy(oracle* o);
 /* BC time (cur_time) of calc_rhs; steps set it from their time argument */
 int oracle_set_time(oracle* o, double t);
+/* Checker hook (no counterpart in the reference): with both arrays given ([n_cells] each), later RHS
+ * evaluations use these blender values alpha and alpha_d in place of calc_blender's, which separates
+ * the arithmetic of the DG operator from the conditioning of the blender; NULL restores calc_blender. */
+int oracle_set_alpha(oracle* o, const double* alpha, const double* alpha_d);
 /* location of node `node` of cell c (through the cell's map) */
 int oracle_node_location(const oracle* o, int32_t c, int32_t node, double* xyz);
 /* value of one expression of the varying inflow BC; NaN on a syntax error */

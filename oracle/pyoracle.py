@@ -48,6 +48,7 @@ def load():
         "oracle_destroy": (None, [vp]),
         "oracle_output_fields": (C.c_int, [vp, C.c_double, C.c_double, _dp, _dp]),
         "oracle_set_time": (C.c_int, [vp, C.c_double]),
+        "oracle_set_alpha": (C.c_int, [vp, _dp, _dp]),
         "oracle_node_location": (C.c_int, [vp, C.c_int32, C.c_int32, _dp]),
         "oracle_eval_expression": (C.c_double, [C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_double]),
         "oracle_set_threads": (C.c_int, [C.c_int32]),
@@ -200,6 +201,17 @@ class OraclePlens:
 
     def set_time(self, t):
         self.lib.oracle_set_time(self._h, t)
+
+    def set_alpha(self, alpha=None, alpha_d=None):
+        """blender values (alpha, alpha_d per cell) for later RHS evaluations in place of the
+        computed ones; no arguments restore the computed ones"""
+        if alpha is None:
+            self.lib.oracle_set_alpha(self._h, None, None)
+            return
+        self._alpha_given = (np.ascontiguousarray(alpha, dtype=np.float64),
+                             np.ascontiguousarray(alpha_d, dtype=np.float64))
+        self.lib.oracle_set_alpha(self._h, self._alpha_given[0].ctypes.data_as(_dp),
+                                  self._alpha_given[1].ctypes.data_as(_dp))
 
     def output_fields(self, courant, t=0.0):
         nodal = np.empty((15, self.n_dof))

@@ -516,3 +516,24 @@ def test_rk_order_conditions():
     for got, exp in conds:
         assert abs(got - exp) < 1e-14
     assert ao[4] == b[4] and ai[3] == b[3]
+
+
+# ------------------------------------------------------------------ checker hook: given blender values
+def test_given_alpha_hook():
+    """the oracle's own blender values handed back reproduce its RHS bit for bit; other values change
+    only the blended terms (a cell with alpha = 0 everywhere gives the pure high-order RHS)"""
+    from plens_amd import shock_tube
+    case = shock_tube(nx=16, ny=2, nz=2)
+    o = PO.OraclePlens(case)
+    r0, a0, ad0 = o.calc_rhs()
+    assert (a0 > 0).any()
+    o.set_alpha(a0, ad0)
+    r1, a1, ad1 = o.calc_rhs()
+    assert np.array_equal(r0, r1) and np.array_equal(a0, a1) and np.array_equal(ad0, ad1)
+    z = np.zeros_like(a0)
+    o.set_alpha(z, z)
+    r2, a2, _ = o.calc_rhs()
+    assert np.array_equal(a2, z) and not np.array_equal(r2, r0)
+    o.set_alpha()
+    r3, a3, _ = o.calc_rhs()
+    assert np.array_equal(r3, r0) and np.array_equal(a3, a0)
