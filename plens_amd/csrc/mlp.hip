@@ -1,7 +1,7 @@
 // Learned troubled-cell indicator: a 125-64-64-1 MLP (GELU hidden layers, 0.5*sigmoid output)
 // trained on the GPU by hand-written forward / backward / Adam kernels. Weights are held in FP32
 // (master copy, Adam moments) and rounded to BF16 for every forward pass; all sums are FP32.
-// Inference for the solver runs fused in the gradient/blender kernel (plens_gpu.hip) on the BF16
+// Inference for the solver runs in k_mlp (plens_gpu.hip, on the matrix cores) on the BF16-rounded
 // copy exported by plens_mlp_device_bf16_params().
 //
 // There is no counterpart in the reference: it replaces BlenderCalculator::get_blender

@@ -107,7 +107,7 @@ struct KP {
     const float* mlp;        // BF16-rounded MLP parameters (nullptr: plain indicator)
     double mlp_tol;          // fallback threshold |alpha_mlp - alpha_plain|
     double mlp_floor;        // network values below this are taken as 0
-    unsigned long long* mlp_stats;  // [0] cells evaluated, [1] fallbacks
+    unsigned long long* mlp_stats;  // per k_mlp workgroup: cells evaluated, fallbacks
     float* feat;             // optional features [n_cells][125] of the indicator
     double* dt_cell;         // stage-1 time-step candidate of every cell
     const int* cinst;        // instance of every cell (batched solves; all 0 for one instance)
@@ -714,66 +714,122 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 // =============================================================================================
 // =============================================================================================
 // k_mlp: learned troubled-cell indicator on kMlpCells cells per workgroup. MLP 125-64-64-1, GELU,
-// 0.5 sigmoid, BF16-rounded weights, FP32 accumulation in input order; the network value (zero
-// below mlp_floor, capped at alpha_max) replaces the plain blender value unless they differ by more
-// than mlp_tol. Transposed tiles in LDS: X^T[k][cell], W1^T[k][h], H1^T[k][cell], W2^T[k][h]; each
-// thread accumulates 4 cells x 4 hidden units.
+// 0.5 sigmoid, BF16-rounded weights; the two hidden layers are FP32 matrix products on the matrix
+// cores (v_mfma_f32_16x16x4f32, FP32 accumulation; K padded to 128 with zeros), the output layer an
+// FP32 dot product in input order. The network value (zero below mlp_floor, capped at alpha_max)
+// replaces the plain blender value unless they differ by more than mlp_tol.
+// Tiles in LDS, row-major as in HBM: X[cell][k], W1[hidden][k], H1[cell][hidden], W2[out][in],
+// H2[cell][hidden]; row strides of 132 and 68 floats put the 64 lanes of an operand read on every
+// bank exactly twice. Wavefront w owns 16 cells (row block w/2) x 32 hidden units (column tiles
+// 2(w%2), 2(w%2)+1). MFMA 16x16x4 operand layout: lane l holds A[l%16][l/16] and B[l/16][l%16];
+// result register r of lane l is D[4(l/16)+r][l%16].
 // =============================================================================================
-constexpr int kMlpCells = 64;
+constexpr int kMlpCells = 32;   // cells per workgroup
+constexpr int kMlpKS = 132;     // LDS row stride of the 128-wide (padded input) tiles
+constexpr int kMlpHS = 68;      // LDS row stride of the 64-wide tiles
+typedef float mlp_f4 __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(256) k_mlp(KP p) {
-    __shared__ float sX[kMlpIn * kMlpCells];   // X^T, later H2^T (64 x 64)
-    __shared__ float sW[kMlpIn * kMlpH];       // W1^T, later W2^T
-    __shared__ float sH[kMlpH * kMlpCells];    // H1^T
+    __shared__ float sA[kMlpCells * kMlpKS];  // X, later H2 (stride kMlpHS)
+    __shared__ float sB[kMlpH * kMlpKS];      // W1, later W2 (stride kMlpHS)
+    __shared__ float sH[kMlpCells * kMlpHS];  // H1
     __shared__ unsigned long long sCnt[2];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int c0 = blockIdx.x * kMlpCells;
     const int ncell = min(kMlpCells, p.nc - c0);
     const float* W = p.mlp;
     if (tid < 2) sCnt[tid] = 0;
-    for (int e = tid; e < kMlpCells * kMlpIn; e += 256) {
-        const int cl = e / kMlpIn, k = e - cl * kMlpIn;
-        sX[k * kMlpCells + cl] = cl < ncell ? p.feat[(long long)(c0 + cl) * kMlpIn + k] : 0.0f;
+    // every global read of the workgroup issued first, as 16-byte loads of contiguous ranges: the
+    // features of its cells (ncell x 125 floats), W1 (64 x 125) and W2 (64 x 64, held until layer 2)
+    constexpr int nw14 = kMlpH * kMlpIn / 4;  // float4 of W1
+    constexpr int kX4 = (kMlpCells * kMlpIn + 1023) / 1024, kW14 = (nw14 + 255) / 256, kW24 = kMlpH * kMlpH / 1024;
+    static_assert(kMlpH * kMlpIn % 4 == 0 && kMlpW2 % 4 == 0 && kMlpH * kMlpH % 1024 == 0, "float4 ranges");
+    const int nx = ncell * kMlpIn, nx4 = nx >> 2;
+    const float4* X4 = reinterpret_cast<const float4*>(p.feat + (long long)c0 * kMlpIn);
+    const float4* W14 = reinterpret_cast<const float4*>(W);
+    const float4* W24 = reinterpret_cast<const float4*>(W + kMlpW2);
+    float4 xr[kX4], w1r[kW14], w2r[kW24];
+    for (int i = 0; i < kX4; ++i) {
+        const int q = tid + 256 * i;
+        xr[i] = q < nx4 ? X4[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    for (int e = tid; e < kMlpH * kMlpIn; e += 256) {
-        const int h = e / kMlpIn, k = e - h * kMlpIn;
-        sW[k * kMlpH + h] = W[e];
+    for (int i = 0; i < kW14; ++i) {
+        const int q = tid + 256 * i;
+        w1r[i] = q < nw14 ? W14[q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    for (int i = 0; i < kW24; ++i) w2r[i] = W24[tid + 256 * i];
+    // zero padding: input columns 125..127, and the rows of absent cells
+    for (int e = tid; e < kMlpCells * 128; e += 256) {
+        const int cl = e >> 7, k = e & 127;
+        if (k >= kMlpIn || cl >= ncell) sA[cl * kMlpKS + k] = 0.0f;
+    }
+    for (int e = tid; e < kMlpH * (128 - kMlpIn); e += 256)
+        sB[(e / (128 - kMlpIn)) * kMlpKS + kMlpIn + e % (128 - kMlpIn)] = 0.0f;
+    for (int i = 0; i < kX4; ++i) {
+        const int q = tid + 256 * i;
+        if (q < nx4) {
+            const float v4[4] = {xr[i].x, xr[i].y, xr[i].z, xr[i].w};
+            for (int j = 0; j < 4; ++j) {
+                const int f = 4 * q + j, cl = f / kMlpIn;
+                sA[cl * kMlpKS + f - cl * kMlpIn] = v4[j];
+            }
+        }
+    }
+    for (int f = 4 * nx4 + tid; f < nx; f += 256) {  // features past the last whole float4
+        const int cl = f / kMlpIn;
+        sA[cl * kMlpKS + f - cl * kMlpIn] = p.feat[(long long)c0 * kMlpIn + f];
+    }
+    for (int i = 0; i < kW14; ++i) {
+        const int q = tid + 256 * i;
+        if (q >= nw14) break;
+        const float v4[4] = {w1r[i].x, w1r[i].y, w1r[i].z, w1r[i].w};
+        for (int j = 0; j < 4; ++j) {
+            const int f = 4 * q + j, h = f / kMlpIn;
+            sB[h * kMlpKS + f - h * kMlpIn] = v4[j];
+        }
     }
     __syncthreads();
-    const int hg = (tid & 15) * 4, cg = (tid >> 4) * 4;
-    float acc[4][4];
-    for (int a = 0; a < 4; ++a)
-        for (int b = 0; b < 4; ++b) acc[a][b] = W[kMlpB1 + hg + b];
-    for (int k = 0; k < kMlpIn; ++k) {
-        const float4 x = *reinterpret_cast<const float4*>(&sX[k * kMlpCells + cg]);
-        const float4 w = *reinterpret_cast<const float4*>(&sW[k * kMlpH + hg]);
-        const float xv[4] = {x.x, x.y, x.z, x.w}, wv[4] = {w.x, w.y, w.z, w.w};
-        for (int a = 0; a < 4; ++a)
-            for (int b = 0; b < 4; ++b) acc[a][b] = fmaf(xv[a], wv[b], acc[a][b]);
+    const int rb = wv >> 1, ct0 = 2 * (wv & 1);
+    const int li = lane & 15, lk = lane >> 4;
+    mlp_f4 acc[2];
+    // layer 1: H1 = gelu(X W1^T + b1)
+    for (int t = 0; t < 2; ++t) {
+        const float b = W[kMlpB1 + 16 * (ct0 + t) + li];
+        acc[t] = mlp_f4{b, b, b, b};
     }
-    for (int a = 0; a < 4; ++a)
-        for (int b = 0; b < 4; ++b) sH[(hg + b) * kMlpCells + cg + a] = mlp_gelu(acc[a][b]);
+    for (int s = 0; s < 32; ++s) {
+        const float a = sA[(16 * rb + li) * kMlpKS + 4 * s + lk];
+        for (int t = 0; t < 2; ++t) {
+            const float b = sB[(16 * (ct0 + t) + li) * kMlpKS + 4 * s + lk];
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[t], 0, 0, 0);
+        }
+    }
+    for (int t = 0; t < 2; ++t)
+        for (int r = 0; r < 4; ++r) sH[(16 * rb + 4 * lk + r) * kMlpHS + 16 * (ct0 + t) + li] = mlp_gelu(acc[t][r]);
+    __syncthreads();  // H1 complete; X and W1 no longer read
+    for (int i = 0; i < kW24; ++i) {  // 4 floats of one row of W2
+        const int f = 4 * (tid + 256 * i);
+        *reinterpret_cast<float4*>(&sB[(f >> 6) * kMlpHS + (f & 63)]) = w2r[i];
+    }
     __syncthreads();
-    for (int e = tid; e < kMlpH * kMlpH; e += 256) {
-        const int h = e / kMlpH, k = e - h * kMlpH;
-        sW[k * kMlpH + h] = W[kMlpW2 + e];
+    // layer 2: H2 = gelu(H1 W2^T + b2)
+    for (int t = 0; t < 2; ++t) {
+        const float b = W[kMlpB2 + 16 * (ct0 + t) + li];
+        acc[t] = mlp_f4{b, b, b, b};
     }
-    __syncthreads();
-    for (int a = 0; a < 4; ++a)
-        for (int b = 0; b < 4; ++b) acc[a][b] = W[kMlpB2 + hg + b];
-    for (int k = 0; k < kMlpH; ++k) {
-        const float4 x = *reinterpret_cast<const float4*>(&sH[k * kMlpCells + cg]);
-        const float4 w = *reinterpret_cast<const float4*>(&sW[k * kMlpH + hg]);
-        const float xv[4] = {x.x, x.y, x.z, x.w}, wv[4] = {w.x, w.y, w.z, w.w};
-        for (int a = 0; a < 4; ++a)
-            for (int b = 0; b < 4; ++b) acc[a][b] = fmaf(xv[a], wv[b], acc[a][b]);
+    for (int s = 0; s < 16; ++s) {
+        const float a = sH[(16 * rb + li) * kMlpHS + 4 * s + lk];
+        for (int t = 0; t < 2; ++t) {
+            const float b = sB[(16 * (ct0 + t) + li) * kMlpHS + 4 * s + lk];
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[t], 0, 0, 0);
+        }
     }
-    for (int a = 0; a < 4; ++a)
-        for (int b = 0; b < 4; ++b) sX[(hg + b) * kMlpCells + cg + a] = mlp_gelu(acc[a][b]);
+    for (int t = 0; t < 2; ++t)
+        for (int r = 0; r < 4; ++r) sA[(16 * rb + 4 * lk + r) * kMlpHS + 16 * (ct0 + t) + li] = mlp_gelu(acc[t][r]);
     __syncthreads();
     if (tid < ncell) {
         const int c = c0 + tid;
         float z = W[kMlpB3];
-        for (int k = 0; k < kMlpH; ++k) z = fmaf(sX[k * kMlpCells + tid], W[kMlpW3 + k], z);
+        for (int k = 0; k < kMlpH; ++k) z = fmaf(sA[tid * kMlpHS + k], W[kMlpW3 + k], z);
         double a = (double)(0.5f / (1.0f + expf(-z)));
         a = (a < p.mlp_floor) ? 0.0 : (a < p.amax ? a : p.amax);
         const double a_plain = p.araw[c];
@@ -783,9 +839,9 @@ __global__ void __launch_bounds__(256) k_mlp(KP p) {
         if (fb) atomicAdd(&sCnt[1], 1ull);
     }
     __syncthreads();
-    if (tid == 0) {
-        atomicAdd(&p.mlp_stats[0], sCnt[0]);
-        if (sCnt[1]) atomicAdd(&p.mlp_stats[1], sCnt[1]);
+    if (tid == 0) {  // this workgroup's own counters (summed on the host)
+        p.mlp_stats[2 * blockIdx.x] += sCnt[0];
+        p.mlp_stats[2 * blockIdx.x + 1] += sCnt[1];
     }
 }
 
@@ -1683,6 +1739,8 @@ int default_cpb(int N) {
     }
 }
 int threads_for(int cpb, int NN) { return ((cpb * NN + 63) / 64) * 64; }
+// counters of the learned indicator: two per k_mlp workgroup
+size_t mlp_stat_slots(const plens_gpu* h) { return 2 * (size_t)((h->nc + kMlpCells - 1) / kMlpCells); }
 
 template <typename T>
 int dmalloc(T** p, size_t count) {
@@ -1992,7 +2050,7 @@ int alloc_buffers(plens_gpu* h) {
         (rc = dmalloc(&h->dt_cell, nc)) || (rc = dmalloc(&h->cinst, nc)) ||
         (rc = dmalloc(&h->ioff, (size_t)h->ninst + 1)) || (rc = dmalloc(&h->visc, 3 * (size_t)h->ninst)) ||
         (rc = dmalloc(&h->scal, 2 * (size_t)h->ninst)) || (rc = dmalloc(&h->fail, 2)) ||
-        (rc = dmalloc(&h->blist, nc)) || (rc = dmalloc(&h->bcount, 1)) || (rc = dmalloc(&h->mlp_stats, 2)) ||
+        (rc = dmalloc(&h->blist, nc)) || (rc = dmalloc(&h->bcount, 1)) || (rc = dmalloc(&h->mlp_stats, mlp_stat_slots(h))) ||
         (rc = dmalloc(&h->tabs, 2 * (size_t)h->ninst)) || (rc = dmalloc(&h->rhoE_old, nd)) ||
         (rc = dmalloc(&h->ss_part, 2 * nc + 2)))
         return rc;
@@ -2022,7 +2080,7 @@ int alloc_buffers(plens_gpu* h) {
         hipMemset(h->alpha, 0, sizeof(double) * nc) != hipSuccess ||
         hipMemset(h->scal, 0, sizeof(double) * 2 * h->ninst) != hipSuccess ||
         hipMemset(h->bcount, 0, sizeof(int)) != hipSuccess ||
-        hipMemset(h->mlp_stats, 0, 2 * sizeof(unsigned long long)) != hipSuccess)
+        hipMemset(h->mlp_stats, 0, mlp_stat_slots(h) * sizeof(unsigned long long)) != hipSuccess)
         return PLENS_E_HIP;
     const unsigned long long init[2] = {~0ull, ~0ull};
     if (hipMemcpy(h->fail, init, sizeof(init), hipMemcpyHostToDevice) != hipSuccess) return PLENS_E_HIP;
@@ -2840,12 +2898,18 @@ int plens_gpu_set_learned_indicator(plens_gpu* h, const float* mlp_params_device
 
 int plens_gpu_indicator_stats(plens_gpu* h, int64_t* evaluated, int64_t* fallbacks, int32_t reset) {
     if (!h) return PLENS_E_INVALID;
-    unsigned long long st[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(st, h->mlp_stats, sizeof(st), hipMemcpyDeviceToHost, h->stream));
+    std::vector<unsigned long long> st(mlp_stat_slots(h));
+    HIPCHK(hipMemcpyAsync(st.data(), h->mlp_stats, sizeof(unsigned long long) * st.size(), hipMemcpyDeviceToHost,
+                          h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    if (evaluated) *evaluated = (int64_t)st[0];
-    if (fallbacks) *fallbacks = (int64_t)st[1];
-    if (reset) HIPCHK(hipMemsetAsync(h->mlp_stats, 0, sizeof(st), h->stream));
+    unsigned long long ev = 0, fb = 0;
+    for (size_t i = 0; i + 1 < st.size(); i += 2) {
+        ev += st[i];
+        fb += st[i + 1];
+    }
+    if (evaluated) *evaluated = (int64_t)ev;
+    if (fallbacks) *fallbacks = (int64_t)fb;
+    if (reset) HIPCHK(hipMemsetAsync(h->mlp_stats, 0, sizeof(unsigned long long) * st.size(), h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return PLENS_OK;
 }
