@@ -31,8 +31,9 @@ HBM_PEAK_GBS = 8000.0
 
 def kernel_bytes_per_dof(N, viscous=True, fb=8):
     """Algorithmic HBM bytes per DoF of one launch of each kernel class (compulsory traffic of this
-    design); fb is the byte size of the flux-arithmetic arrays (avars and the face terms: 8 for
-    FP64, 4 for FP32-bulk); states, residual registers and metric data are FP64 in every variant.
+    design); fb is the byte size of the flux-arithmetic arrays (avars, the face terms and the RK
+    residual registers: 8 for FP64, 4 for FP32-bulk); states and metric data are FP64 in every
+    variant.
     Metric terms are derived from each cell's trilinear map: 24 doubles per cell (map_b per DoF)."""
     n, nf = (N + 1) ** 3, (N + 1) ** 2
     face_per_dof = 6.0 * nf / n  # face nodes per cell node
@@ -50,15 +51,16 @@ def kernel_bytes_per_dof(N, viscous=True, fb=8):
     b["face"] = face_per_dof / 2 * (8 * (10 + 3 + 10) + fb * (18 * v + 10))
     # k_vol: read Q (5), the map, avars (9, viscous), the weighted surface terms of its face nodes
     # (5 per face node), R' and R'' (stage averages 4/5 and 3/5 of 5); write Q (5), R (5)
-    b["vol"] = 8 * (5 + 5 * 4 / 5 + 5 * 3 / 5 + 10) + map_b + fb * (9 * v + face_per_dof * 5)
+    b["vol"] = 8 * (5 + 5) + map_b + fb * (5 * 4 / 5 + 5 * 3 / 5 + 5 + 9 * v + face_per_dof * 5)
     return b
 
 
 def stage_bytes_per_dof(N, fb=8):
     """compulsory bytes per DoF-stage of the two-pass algorithm (lift, then residual) with the fused
     RK epilogue: Q read + write, R write, R' and R'' reads (stage averages), avars write + read, and
-    the cells' trilinear maps (the metric terms are not read)"""
-    return 8 * (5 + 5 + 5 + 4 + 3) + fb * 18 + 24 * 8 / (N + 1) ** 3
+    the cells' trilinear maps (the metric terms are not read); R and avars in the flux-arithmetic
+    type"""
+    return 8 * (5 + 5) + fb * (5 + 4 + 3) + fb * 18 + 24 * 8 / (N + 1) ** 3
 
 
 def measured_traffic(precision, kernel):

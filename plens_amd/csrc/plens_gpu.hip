@@ -97,9 +97,12 @@ struct KP {
     float* avars_f;  // FP32-bulk storage of avars and the face terms
     float* sinv_f;
     float* sface_f;
-    double* Rnew;
+    double* Rnew;            // RK stage registers: the new one, R', R'' (FP64 handles)
     const double* R1;
     const double* R2;
+    float* Rnew_f;           // the same in FP32 (FP32-bulk: increments stored in FP32, state in FP64)
+    const float* R1_f;
+    const float* R2_f;
     double* rhs_out;
     double* rlo;
     int* blist;
@@ -1556,7 +1559,12 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
         for (int v = 0; v < 5; ++v) p.rhs_out[v * p.ndof + g] = rhs[v];
         return;
     }
+    // the stage's own increment enters in FP64; the registers kept for later stages are in the
+    // flux-arithmetic type (FP32 for FP32-bulk)
     const double dt = p.dt_lts ? p.dt_lts[c] : p.dt[p.cinst[c]];
+    const TF* R1 = tsel<TF>(p.R1, p.R1_f);
+    const TF* R2 = tsel<TF>(p.R2, p.R2_f);
+    TF* Rn = tsel<TF>(p.Rnew, p.Rnew_f);
     for (int v = 0; v < 5; ++v) {
         const long long k = v * p.ndof + g;
         const double R = rhs[v] * dt;
@@ -1564,12 +1572,12 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
         if (p.stage == 1) {
             qn = u[v] + p.c3 * R;
         } else if (p.stage == 2) {
-            qn = u[v] + (p.c2 * p.R1[k] + p.c3 * R);
+            qn = u[v] + (p.c2 * (double)R1[k] + p.c3 * R);
         } else {
-            qn = u[v] + (p.c1 * p.R2[k] + p.c2 * p.R1[k] + p.c3 * R);
+            qn = u[v] + (p.c1 * (double)R2[k] + p.c2 * (double)R1[k] + p.c3 * R);
         }
         p.Qout[k] = qn;
-        p.Rnew[k] = R;
+        Rn[k] = (TF)R;
     }
 }
 
@@ -1671,6 +1679,7 @@ struct plens_gpu {
     int device = 0;
     // device buffers
     double *Qa = nullptr, *Qb = nullptr, *R[2] = {nullptr, nullptr};
+    float* Rf[2] = {nullptr, nullptr};            // RK registers of FP32-bulk handles
     double *Ja = nullptr, *detJ = nullptr, *nrm = nullptr, *hmin = nullptr;
     double* cmap = nullptr;  // trilinear maps of the cells; the stored metric arrays are not kept then
     double* fmet = nullptr;  // face-node metric terms of the face tasks
@@ -1905,6 +1914,9 @@ KP stage_params(plens_gpu* h, int s, bool from_b, double courant) {
     kp.Rnew = h->R[wr[s]];
     kp.R1 = h->R[r1[s]];
     kp.R2 = h->R[r2[s]];
+    kp.Rnew_f = h->Rf[wr[s]];
+    kp.R1_f = h->Rf[r1[s]];
+    kp.R2_f = h->Rf[r2[s]];
     kp.stage = s;
     kp.want_dt = (s == 1);
     kp.save_E = (s == 1);
@@ -2036,8 +2048,11 @@ void host_node_metrics(const double* k, double x0, double x1, double x2, double 
 int alloc_buffers(plens_gpu* h) {
     const size_t nd = (size_t)h->ndof, nc = (size_t)h->nc, NF = (size_t)h->NF, nface = nc * 30 * NF;
     int rc;
-    if ((rc = dmalloc(&h->Qa, 5 * nd)) || (rc = dmalloc(&h->Qb, 5 * nd)) || (rc = dmalloc(&h->R[0], 5 * nd)) ||
-        (rc = dmalloc(&h->R[1], 5 * nd)) || (rc = dmalloc(&h->rhs, 5 * nd)) || (rc = dmalloc(&h->rlo, 5 * nd)) ||
+    const bool fp64 = h->precision == PLENS_PREC_FP64;
+    if ((rc = dmalloc(&h->Qa, 5 * nd)) || (rc = dmalloc(&h->Qb, 5 * nd)) ||
+        (rc = fp64 ? dmalloc(&h->R[0], 5 * nd) : dmalloc(&h->Rf[0], 5 * nd)) ||
+        (rc = fp64 ? dmalloc(&h->R[1], 5 * nd) : dmalloc(&h->Rf[1], 5 * nd)) ||
+        (rc = dmalloc(&h->rhs, 5 * nd)) || (rc = dmalloc(&h->rlo, 5 * nd)) ||
         (rc = dmalloc(&h->nrm, nc * 18 * NF)) ||
         (rc = (h->precision == PLENS_PREC_FP64 ? dmalloc(&h->fmet, (size_t)h->ntasks * 10 * NF)
                                                : dmalloc(&h->fmet_f, (size_t)h->ntasks * 10 * NF))) ||
@@ -2074,8 +2089,9 @@ int alloc_buffers(plens_gpu* h) {
             return rc;
         if (hipMemset(h->avars_f, 0, sizeof(float) * 9 * nd) != hipSuccess) return PLENS_E_HIP;
     }
-    if (hipMemset(h->R[0], 0, sizeof(double) * 5 * nd) != hipSuccess ||
-        hipMemset(h->R[1], 0, sizeof(double) * 5 * nd) != hipSuccess ||
+    const size_t rbytes = (h->precision == PLENS_PREC_FP64 ? sizeof(double) : sizeof(float)) * 5 * nd;
+    if (hipMemset(h->R[0] ? (void*)h->R[0] : (void*)h->Rf[0], 0, rbytes) != hipSuccess ||
+        hipMemset(h->R[1] ? (void*)h->R[1] : (void*)h->Rf[1], 0, rbytes) != hipSuccess ||
         hipMemset(h->rlo, 0, sizeof(double) * 5 * nd) != hipSuccess ||
         hipMemset(h->alpha, 0, sizeof(double) * nc) != hipSuccess ||
         hipMemset(h->scal, 0, sizeof(double) * 2 * h->ninst) != hipSuccess ||
@@ -2467,7 +2483,7 @@ void plens_gpu_destroy(plens_gpu* h) {
     destroy_graphs(h);
     for (auto& x : h->ev_run)
         if (x) (void)hipEventDestroy(x);
-    void* ptrs[] = {h->Qa, h->Qb, h->R[0], h->R[1], h->Ja, h->detJ, h->nrm, h->hmin, h->nbr,
+    void* ptrs[] = {h->Qa, h->Qb, h->R[0], h->R[1], h->Rf[0], h->Rf[1], h->Ja, h->detJ, h->nrm, h->hmin, h->nbr,
                     h->bid, h->wall, h->tasks, h->tdesc, h->dtmin, h->dtcount, h->bcs, h->basis, h->avars, h->araw, h->alpha, h->alpha_d,
                     h->sinv, h->sface, h->rhs, h->rlo, h->dt_cell, h->scal, h->fail, h->blist, h->bcount, h->mlp_stats, h->feat,
                     h->avars_f, h->sinv_f, h->sface_f, h->cinst, h->ioff, h->visc, h->cfl, h->cmap, h->fmet, h->fmet_f,
