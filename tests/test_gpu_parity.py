@@ -421,3 +421,18 @@ def test_batch_rejects_local_stepping():
     with pytest.raises(PlensError) as ei:
         GpuBatch([a, b])
     assert ei.value.code == PLENS_E_INVALID
+
+
+@pytest.mark.parametrize("label", ["cell-N3", "cell-N5", "shock-2cells"])
+def test_smallest_meshes(label):
+    """one fully periodic cell (every face its own neighbour) and a two-cell blended shock tube:
+    the smallest grids of every kernel (one face-task group, one k_alpha block, one blended cell)"""
+    from plens_amd import periodic_cube, shock_tube
+    case = {"cell-N3": lambda: periodic_cube(n=1, degree=3), "cell-N5": lambda: periodic_cube(n=1, degree=5),
+            "shock-2cells": lambda: shock_tube(nx=2, ny=1, nz=1)}[label]()
+    g, o = _rhs_parity(case, label)
+    g.run(10, case.courant)
+    o.run(10, case.courant)
+    err = rel_l2(g.get_state(), o.get_state())
+    print(f"[{label}] 10 steps rel L2 {err:.3e}", flush=True)
+    assert err <= RUN_TOL
