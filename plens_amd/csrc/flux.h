@@ -143,6 +143,57 @@ __device__ void hllc_xflux(const T l[5], const T r[5], T gma, T f[5]) {
     }
 }
 
+// HLLC flux along the unit normal n in the original frame: get_inv_surf_flux's rotation (momentum
+// along n and two tangents), hllc_xflux and the rotation back, written out. The rotation is
+// linear, so the rotated-back Euler x-flux is the normal flux F_n(u), the star momentum is
+// t (rho s n + m - m_n n) with m_n = n . m, and mass and energy are those of hllc_xflux; the same
+// star-pressure choice (PVRS / TRRS / TSRS) and wave speeds from the normal velocities. Neither the
+// rotation nor the rotated states are formed (fewer registers); equal to the rotated form up to
+// rounding.
+template <typename T>
+__device__ void hllc_nflux(const T l[5], const T r[5], const T n[3], T gma, T f[5]) {
+    const T pl = get_p(l, gma), pr = get_p(r, gma);
+    const T pmin = mn(pl, pr), pmax = mx(pl, pr);
+    const T mnl = l[1] * n[0] + l[2] * n[1] + l[3] * n[2], mnr = r[1] * n[0] + r[2] * n[1] + r[3] * n[2];
+    const T ul = dv(mnl, l[0]), ur = dv(mnr, r[0]);
+    const T al = sq(dv(gma * pl, l[0])), ar = sq(dv(gma * pr, r[0]));
+    const T p_pvrs = T(0.5) * (pl + pr) - T(0.5) * (ur - ul) * T(0.5) * (l[0] + r[0]) * T(0.5) * (al + ar);
+    T ps;
+    if (dv(pmax, pmin) < T(2) && pmin <= p_pvrs && p_pvrs <= pmax) {
+        ps = p_pvrs;
+    } else if (p_pvrs < pmin) {
+        const T z = dv(gma - T(1), T(2) * gma);
+        ps = pw(dv((al + ar) - dv(gma - T(1), T(2)) * (ur - ul), dv(al, pw(pl, z)) + dv(ar, pw(pr, z))), dv(T(1), z));
+    } else {
+        const T p0 = mx(T(0), p_pvrs);
+        const T gl = sq(dv(dv(T(2), (gma + T(1)) * l[0]), p0 + dv(gma - T(1), gma + T(1)) * pl));
+        const T gr = sq(dv(dv(T(2), (gma + T(1)) * r[0]), p0 + dv(gma - T(1), gma + T(1)) * pr));
+        ps = dv(gl * pl + gr * pr - (ur - ul), gl + gr);
+    }
+    const T c1 = dv(gma + T(1), T(2) * gma);
+    const T ql = (ps <= pl) ? T(1) : sq(T(1) + c1 * (dv(ps, pl) - T(1)));
+    const T qr = (ps <= pr) ? T(1) : sq(T(1) + c1 * (dv(ps, pr) - T(1)));
+    const T sl = ul - al * ql, sr = ur + ar * qr;
+    const T s = dv(pr - pl + mnl * (sl - ul) - mnr * (sr - ur), l[0] * (sl - ul) - r[0] * (sr - ur));
+    if (sl > T(0)) {
+        inv_flux(l, n, gma, f);
+    } else if (s > T(0)) {
+        const T t = dv(sl - ul, sl - s);
+        inv_flux(l, n, gma, f);
+        f[0] += sl * (t * l[0] - l[0]);
+        for (int a = 0; a < 3; ++a) f[1 + a] += sl * (t * (l[0] * s * n[a] + l[1 + a] - mnl * n[a]) - l[1 + a]);
+        f[4] += sl * (t * (l[4] + (s - ul) * (l[0] * s + dv(pl, sl - ul))) - l[4]);
+    } else if (sr > T(0)) {
+        const T t = dv(sr - ur, sr - s);
+        inv_flux(r, n, gma, f);
+        f[0] += sr * (t * r[0] - r[0]);
+        for (int a = 0; a < 3; ++a) f[1 + a] += sr * (t * (r[0] * s * n[a] + r[1 + a] - mnr * n[a]) - r[1 + a]);
+        f[4] += sr * (t * (r[4] + (s - ur) * (r[0] * s + dv(pr, sr - ur))) - r[4]);
+    } else {
+        inv_flux(r, n, gma, f);
+    }
+}
+
 // Rusanov x-flux (navier_stokes.cc:1044-1059)
 template <typename T>
 __device__ void rusanov_xflux(const T l[5], const T r[5], T gma, T f[5]) {

@@ -1050,7 +1050,9 @@ __global__ void __launch_bounds__(256) k_ss_reduce(const double* part, int nc, d
 // =============================================================================================
 // k_face
 // =============================================================================================
-template <int N, typename TF, bool EXT>
+// HN: HLLC (the scheme of the handle) in the normal frame (hllc_nflux), an instantiation that
+// leaves out the rotation of the other schemes and so needs fewer registers
+template <int N, typename TF, bool EXT, bool HN>
 __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
     const long long gid = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -1087,7 +1089,8 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     for (int d = 0; d < 3; ++d) nt[d] = (TF)n[d];
     const TF gma = (TF)gs.gma;
     TF F[5];
-    inv_surf_flux<EXT>(p.scheme, (TF)bl, ut, ugt, nt, gma, F);
+    if constexpr (HN) hllc_nflux(ut, ugt, nt, gma, F);
+    else inv_surf_flux<EXT>(p.scheme, (TF)bl, ut, ugt, nt, gma, F);
     // Each side stores its finished surface term of calc_cell_ho_inv/dif_residual
     // (dgsem/plens.cc:3237-3268, 3373-3406): -s (-s F*_side - f(u; Ja/|Ja|)) |Ja| / w_end, with
     // F*_side the outward flux of that side (-F for the neighbour of an interior face) and
@@ -1838,12 +1841,13 @@ int launch_one_t(plens_gpu* h, const KP& kp, int kernel) {
         case 2: {
             const long long fthreads = (long long)kp.ntasks * NF;
             const int ft = kp.face_threads;
+            const dim3 fgrid((unsigned)((fthreads + ft - 1) / ft));
             if (ext)
-                hipLaunchKernelGGL((k_face<N, TF, true>), dim3((unsigned)((fthreads + ft - 1) / ft)), dim3(ft), 0,
-                                   h->stream, kp);
+                hipLaunchKernelGGL((k_face<N, TF, true, false>), fgrid, dim3(ft), 0, h->stream, kp);
+            else if (kp.scheme == PLENS_FLUX_HLLC)
+                hipLaunchKernelGGL((k_face<N, TF, false, true>), fgrid, dim3(ft), 0, h->stream, kp);
             else
-                hipLaunchKernelGGL((k_face<N, TF, false>), dim3((unsigned)((fthreads + ft - 1) / ft)), dim3(ft), 0,
-                                   h->stream, kp);
+                hipLaunchKernelGGL((k_face<N, TF, false, false>), fgrid, dim3(ft), 0, h->stream, kp);
             break;
         }
         case 3:
