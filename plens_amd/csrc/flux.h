@@ -149,9 +149,16 @@ __device__ void hllc_xflux(const T l[5], const T r[5], T gma, T f[5]) {
 // t (rho s n + m - m_n n) with m_n = n . m, and mass and energy are those of hllc_xflux; the same
 // star-pressure choice (PVRS / TRRS / TSRS) and wave speeds from the normal velocities. Neither the
 // rotation nor the rotated states are formed (fewer registers); equal to the rotated form up to
-// rounding.
+// rounding. Like that rotation, a normal within 1e-3 of the x axis (|n_y, n_z| <= 1e-3) is taken
+// as (sign n_x, 0, 0): the rotation is then sign(n_x) I, and the flux is that along this axis.
 template <typename T>
-__device__ void hllc_nflux(const T l[5], const T r[5], const T n[3], T gma, T f[5]) {
+__device__ void hllc_nflux(const T l[5], const T r[5], const T nin[3], T gma, T f[5]) {
+    T n[3] = {nin[0], nin[1], nin[2]};
+    if (!(sq(nin[2] * nin[2] + nin[1] * nin[1]) > T(1e-3))) {
+        n[0] = nin[0] < T(0) ? T(-1) : T(1);
+        n[1] = T(0);
+        n[2] = T(0);
+    }
     const T pl = get_p(l, gma), pr = get_p(r, gma);
     const T pmin = mn(pl, pr), pmax = mx(pl, pr);
     const T mnl = l[1] * n[0] + l[2] * n[1] + l[3] * n[2], mnr = r[1] * n[0] + r[2] * n[1] + r[3] * n[2];

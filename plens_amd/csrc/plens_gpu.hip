@@ -1174,7 +1174,7 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
 // =============================================================================================
 // k_lo: first-order subcell FV residual of blended cells (calc_cell_lo_inv_residual, 3469-3583)
 // =============================================================================================
-template <int N, typename TF, bool EXT>
+template <int N, typename TF, bool EXT, bool HN>  // HN: as in k_face
 __global__ void __launch_bounds__(256) k_lo(KP p) {
     constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
     extern __shared__ double smem[];
@@ -1239,7 +1239,8 @@ __global__ void __launch_bounds__(256) k_lo(KP p) {
                 const int jr = lc * NN + tr[0] + NP * (tr[1] + NP * tr[2]);
                 TF ur[5], fl[5];
                 for (int v = 0; v < 5; ++v) ur[v] = sQ[v * B + jr];
-                inv_surf_flux<EXT>(p.scheme, bl, u, ur, sn, gma, fl);
+                if constexpr (HN) hllc_nflux(u, ur, sn, gma, fl);
+                else inv_surf_flux<EXT>(p.scheme, bl, u, ur, sn, gma, fl);
                 for (int v = 0; v < 5; ++v) sX[v * B + li] = fl[v] * nn;
             }
             __syncthreads();
@@ -1852,11 +1853,14 @@ int launch_one_t(plens_gpu* h, const KP& kp, int kernel) {
         }
         case 3:
             if (ext)
-                hipLaunchKernelGGL((k_lo<N, TF, true>), dim3(std::min(blocks, kLoGrid)), dim3(thr), sizeof(TF) * 19 * B,
-                                   h->stream, kp);
+                hipLaunchKernelGGL((k_lo<N, TF, true, false>), dim3(std::min(blocks, kLoGrid)), dim3(thr),
+                                   sizeof(TF) * 19 * B, h->stream, kp);
+            else if (kp.scheme == PLENS_FLUX_HLLC)
+                hipLaunchKernelGGL((k_lo<N, TF, false, true>), dim3(std::min(blocks, kLoGrid)), dim3(thr),
+                                   sizeof(TF) * 19 * B, h->stream, kp);
             else
-                hipLaunchKernelGGL((k_lo<N, TF, false>), dim3(std::min(blocks, kLoGrid)), dim3(thr), sizeof(TF) * 19 * B,
-                                   h->stream, kp);
+                hipLaunchKernelGGL((k_lo<N, TF, false, false>), dim3(std::min(blocks, kLoGrid)), dim3(thr),
+                                   sizeof(TF) * 19 * B, h->stream, kp);
             break;
         case 4: {
             const size_t nitem = (size_t)NP * NP * (NP * (NP - 1) / 2);
