@@ -228,12 +228,12 @@ def main():
     stage_ms = float(sum(kernels[k]["avg_ms"] for k in names))
     fb = 8 if args.precision == "fp64" else 4
     stage_bytes = stage_bytes_per_dof(N, fb)
-    # kernel name as rocprofv3 prints it: k_grad<N, type>, the others k_*<N, type, extended scheme>
-    # (false here: HLLC surface flux and Chandrashekhar volume flux)
-    tname = f"k_{dom}<{N},{'double' if args.precision == 'fp64' else 'float'}" + (">" if dom in ("grad", "alpha")
-                                                                                  else ",false>")
-    if dom == "alpha":
-        tname = "k_alpha"
+    # kernel name as rocprofv3 prints it: k_grad<N, type>, k_vol<N, type, extended scheme>, k_face and
+    # k_lo <N, type, extended scheme, HLLC in the normal frame> (here: HLLC surface flux and
+    # Chandrashekhar volume flux)
+    tail = {"grad": ">", "vol": ",false>", "face": ",false,true>", "lo": ",false,true>"}
+    tname = ("k_alpha" if dom == "alpha" else
+             f"k_{dom}<{N},{'double' if args.precision == 'fp64' else 'float'}" + tail[dom])
     traffic, tsrc = measured_traffic(args.precision, tname)
     roofline = {"bound": "hbm", "kernel": tname, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": tsrc,

@@ -82,7 +82,6 @@ struct KP {
     const int* nbr;
     const int* bid;
     const int* wall;
-    const int* tasks;
     const int4* tdesc;       // per face task: cell*6+face, neighbour, boundary id, boundary type (-1 interior)
     unsigned long long* dtmin;  // per instance: running minimum of the cell steps' bit patterns (all ones = none)
     unsigned int* dtcount;      // blocks of k_alpha done with their minimum
@@ -2116,7 +2115,7 @@ void bind_buffers(plens_gpu* h) {
     KP& k = h->base;
     k.nc = h->nc; k.cpb = h->cpb; k.ntasks = h->ntasks; k.ndof = h->ndof; k.ninst = h->ninst;
     k.Ja = h->Ja; k.detJ = h->detJ; k.cmap = h->cmap; k.fmet = h->fmet; k.fmet_f = h->fmet_f; k.nrm = h->nrm; k.hmin = h->hmin;
-    k.nbr = h->nbr; k.bid = h->bid; k.wall = h->wall; k.tasks = h->tasks; k.tdesc = h->tdesc; k.bcs = h->bcs;
+    k.nbr = h->nbr; k.bid = h->bid; k.wall = h->wall; k.tdesc = h->tdesc; k.bcs = h->bcs;
     k.dtmin = h->dtmin; k.dtcount = h->dtcount; k.basis = h->basis;
     k.avars = h->avars; k.araw = h->araw; k.alpha = h->alpha; k.alpha_d = h->alpha_d;
     k.sinv = h->sinv; k.sface = h->sface; k.avars_f = h->avars_f; k.sinv_f = h->sinv_f; k.sface_f = h->sface_f;
