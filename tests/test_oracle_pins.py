@@ -518,6 +518,47 @@ def test_rk_order_conditions():
     assert ao[4] == b[4] and ai[3] == b[3]
 
 
+def _butcher():
+    ao, ai, b = np.zeros(5), np.zeros(4), np.zeros(5)
+    PO.load().oracle_rk4_coeffs(ao.ctypes.data_as(dp), ai.ctypes.data_as(dp), b.ctypes.data_as(dp))
+    A = np.zeros((5, 5))
+    A[1, 0] = ao[0]
+    A[2, 0], A[2, 1] = ai[0], ao[1]
+    A[3, 0], A[3, 1], A[3, 2] = b[0], ai[1], ao[2]
+    A[4, 0], A[4, 1], A[4, 2], A[4, 3] = b[0], b[1], ai[2], ao[3]
+    return A, b
+
+
+@pytest.mark.parametrize("which", ["periodic_cube", "shock_tube"])
+def test_register_step_equals_butcher_form(which):
+    """One step of the oracle's three-register update (dgsem/plens.cc:4334-4403) equals the Butcher
+    form y1 = y0 + dt sum b_i k_i, k_i = f(y0 + dt sum_j a_ij k_j) built here from the oracle's
+    right-hand side: a wrong register, stage coefficient or stage order fails. For y' = lambda y the
+    Butcher form is the stability polynomial R(z) = sum_k z^k b^T A^(k-1) 1, whose first five
+    coefficients are 1/k! (fourth order)."""
+    from plens_amd import cases as CS
+    if which == "periodic_cube":
+        case, dt = CS.periodic_cube(n=2, degree=3), 0.004
+    else:
+        case, dt = CS.shock_tube(degree=2, nx=6, ny=2, nz=2), 0.002
+    A, b = _butcher()
+    o = PO.OraclePlens(case)
+    y0 = case.Q0.copy()
+    ks = []
+    for i in range(5):
+        o.set_state(y0 + dt * sum(A[i, j] * ks[j] for j in range(i)) if i else y0)
+        ks.append(o.calc_rhs()[0])
+    y1 = y0 + dt * sum(b[i] * ks[i] for i in range(5))
+    o.set_state(y0)
+    o.update_fixed_dt(dt)
+    got = o.get_state()
+    # relative to the step's increment: about 1e-13 from rounding, 2e-3 with two weights swapped
+    assert np.linalg.norm(got - y1) <= 1e-12 * np.linalg.norm(y1 - y0)
+    one = np.ones(5)
+    coeff = [b @ np.linalg.matrix_power(A, k - 1) @ one for k in range(1, 5)]
+    assert np.allclose(coeff, [1, 1 / 2, 1 / 6, 1 / 24], rtol=0, atol=1e-15)
+
+
 # ------------------------------------------------------------------ checker hook: given blender values
 def test_given_alpha_hook():
     """the oracle's own blender values handed back reproduce its RHS bit for bit; other values change
