@@ -13,7 +13,8 @@
   cases, from a state stored in the file (dgsem/plens.cc:3688-3779, 4334-4403): the periodic
   density wave on 2^3 cells, N=3 (under-resolved, so the blender saturates); the Mach-5 shock on
   6x2x2 cells, N=2, two steps in (blender active); the curvilinear viscous wall flow on 4x2x2
-  cells, N=4, six steps in. Each file names its builder and arguments.
+  cells, N=4, six steps in, with 1% seeded density noise. Each file names its builder and
+  arguments, and holds the oracle's sensitivity to one-unit-roundoff input changes.
 
 The fixtures pin the oracle against later edits (tests/test_golden.py, CPU) and are the expected
 values of the GPU path on the same inputs (tests/test_gpu_golden.py)."""
@@ -35,11 +36,12 @@ from plens_amd import cases  # noqa: E402
 
 _dp = C.POINTER(C.c_double)
 
-# (file stem, builder, keyword arguments, oracle steps run before the stored state)
+# (file stem, builder, keyword arguments, oracle steps run before the stored state, relative
+# amplitude of seeded nodal density noise added after those steps)
 RHS_CASES = [
-    ("periodic_cube_2x2x2_N3", "periodic_cube", {"degree": 3, "n": 2, "seed": 0}, 0),
-    ("shock_tube_6x2x2_N2", "shock_tube", {"degree": 2, "nx": 6, "ny": 2, "nz": 2, "seed": 1}, 2),
-    ("wall_flow_4x2x2_N4", "wall_flow", {"degree": 4, "nx": 4, "ny": 2, "nz": 2, "seed": 2}, 6),
+    ("periodic_cube_2x2x2_N3", "periodic_cube", {"degree": 3, "n": 2, "seed": 0}, 0, 0.0),
+    ("shock_tube_6x2x2_N2", "shock_tube", {"degree": 2, "nx": 6, "ny": 2, "nz": 2, "seed": 1}, 2, 0.0),
+    ("wall_flow_4x2x2_N4", "wall_flow", {"degree": 4, "nx": 4, "ny": 2, "nz": 2, "seed": 2}, 6, 0.01),
 ]
 METRIC_MESH = {"degree": 3, "nx": 2, "ny": 2, "nz": 2, "lo": (0, 0, 0), "hi": (1.0, 0.5, 0.25),
                "periodic": (0, 0, 0), "side_bid": [0, 0, 0, 0, 0, 0], "periodic_bid": (-1, -1, -1),
@@ -109,26 +111,46 @@ def build_case(builder, kwargs):
     return getattr(cases, builder)(**kwargs)
 
 
-def rhs_case(builder, kwargs, warm_steps):
+def rhs_conditioning(o, Q, rhs, n=3, seed=0):
+    """Largest relative L2 change of the oracle's right-hand side when every entry of Q is moved by
+    one unit roundoff (+-2.2e-16 relative, seeded signs): the disagreement that evaluation-order
+    rounding alone can cause. States close to a uniform flow next to walls reach 5e-12, because the
+    right-hand side is then a small difference of differentiated near-constant fields; the stored
+    states are chosen to stay below 5e-13, so that the 1e-12 bar is a test of the arithmetic."""
+    rng = np.random.default_rng(seed)
+    worst = 0.0
+    for _ in range(n):
+        o.set_state(Q * (1 + 2.2e-16 * rng.choice([-1.0, 1.0], Q.shape)))
+        r = o.calc_rhs()[0]
+        worst = max(worst, float(np.linalg.norm(r - rhs) / np.linalg.norm(rhs)))
+    o.set_state(Q)
+    return worst
+
+
+def rhs_case(builder, kwargs, warm_steps, noise, seed=3):
     case = build_case(builder, kwargs)
     o = PO.OraclePlens(case)
     if warm_steps:
         o.run(warm_steps, case.courant)
     Q = o.get_state()
+    if noise:
+        Q[0] *= 1 + noise * np.random.default_rng(seed).uniform(-1.0, 1.0, Q.shape[1])
+        o.set_state(Q)
     rhs, alpha, alpha_d = o.calc_rhs()
+    cond = rhs_conditioning(o, Q, rhs)
     dt = o.calc_time_step(case.courant)
     o.update(case.courant)
     Q1 = o.get_state()
     o.close()
     return dict(builder=builder, kwargs=json.dumps(kwargs), courant=case.courant, Q=Q, rhs=rhs, alpha=alpha,
-                alpha_d=alpha_d, dt=dt, Q1=Q1)
+                alpha_d=alpha_d, dt=dt, Q1=Q1, conditioning=cond)
 
 
 def main():
     np.savez_compressed(os.path.join(HERE, "fluxes.npz"), **fluxes())
     np.savez_compressed(os.path.join(HERE, "metrics_perturbed_2x2x2_N3.npz"), **metrics())
-    for stem, builder, kw, warm in RHS_CASES:
-        np.savez_compressed(os.path.join(HERE, f"rhs_{stem}.npz"), **rhs_case(builder, kw, warm))
+    for stem, builder, kw, warm, noise in RHS_CASES:
+        np.savez_compressed(os.path.join(HERE, f"rhs_{stem}.npz"), **rhs_case(builder, kw, warm, noise))
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -58,8 +58,8 @@ def test_metrics_match_fixture():
         assert np.abs(got[key] - d[key]).max() <= 1e-15 * np.abs(d[key]).max(), key
 
 
-@pytest.mark.parametrize("stem,builder,kwargs,warm", MF.RHS_CASES, ids=[c[0] for c in MF.RHS_CASES])
-def test_rhs_match_fixture(stem, builder, kwargs, warm):
+@pytest.mark.parametrize("stem,builder,kwargs,warm,noise", MF.RHS_CASES, ids=[c[0] for c in MF.RHS_CASES])
+def test_rhs_match_fixture(stem, builder, kwargs, warm, noise):
     d = load(f"rhs_{stem}.npz")
     assert str(d["builder"]) == builder and json.loads(str(d["kwargs"])) == kwargs
     case = MF.build_case(builder, kwargs)
@@ -71,6 +71,10 @@ def test_rhs_match_fixture(stem, builder, kwargs, warm):
     assert rel_l2(rhs, d["rhs"]) <= 1e-14
     assert np.allclose(a, d["alpha"], rtol=1e-13, atol=1e-15)
     assert np.allclose(ad, d["alpha_d"], rtol=1e-13, atol=1e-15)
+    # the stored state is well conditioned: rounding-level input changes move the RHS far less than
+    # the 1e-12 bar the GPU is held to
+    cond = MF.rhs_conditioning(o, d["Q"], d["rhs"])
+    assert cond <= 5e-13 and abs(cond - float(d["conditioning"])) <= 0.1 * cond
     dt = o.calc_time_step(float(d["courant"]))
     assert abs(dt - float(d["dt"])) <= 1e-14 * dt
     o.update(float(d["courant"]))
