@@ -113,6 +113,7 @@ struct KP {
     float* feat;             // optional features [n_cells][125] of the indicator
     double* dt_cell;         // stage-1 time-step candidate of every cell
     const int* cinst;        // instance of every cell (batched solves; all 0 for one instance)
+    const int* corder;       // k_grad's cell order (Morton order of the centroids), or null: index order
     const int* ioff;         // cell range [ioff[i], ioff[i+1]) of instance i
     const double* visc;      // Sutherland mu0, T0, S of every instance
     double* dt;              // time step of every instance
@@ -409,10 +410,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     load_basis<N>(p.basis, sD, sW, sQm, sC);
 
     const int tid = threadIdx.x, lc = tid / NN, node = tid - lc * NN;
+    // cells in the Morton order of their centroids when the handle has one, so that the neighbours
+    // whose face states the surface terms read were read shortly before, on the same XCD
     const int c0 = xcd_block(blockIdx.x, gridDim.x) * p.cpb;
-    const int c = c0 + lc;
+    const int c = (p.corder && c0 + lc < p.nc) ? p.corder[c0 + lc] : c0 + lc;
     for (int t = tid; t < 12 * p.cpb; t += blockDim.x) {
-        const int cl = t / 12, r = t - 12 * cl, cc = c0 + cl;
+        const int cl = t / 12, r = t - 12 * cl;
+        const int cc = (p.corder && c0 + cl < p.nc) ? p.corder[c0 + cl] : c0 + cl;
         sFn[t] = cc < p.nc ? (r < 6 ? p.bid[cc * 6 + r] : p.nbr[cc * 6 + r - 6]) : -1;
     }
     const bool inblk = lc < p.cpb;
@@ -1701,6 +1705,7 @@ struct plens_gpu {
     long long steps_taken = 0;
     bool use_cmap = false;
     int *nbr = nullptr, *bid = nullptr, *wall = nullptr, *tasks = nullptr;
+    int* corder = nullptr;  // cells in the Morton order of their centroids (k_grad)
     int4* tdesc = nullptr;
     unsigned long long* dtmin = nullptr;  // step reduction slots (per instance) and block counter
     unsigned int* dtcount = nullptr;
@@ -2115,7 +2120,7 @@ void bind_buffers(plens_gpu* h) {
     KP& k = h->base;
     k.nc = h->nc; k.cpb = h->cpb; k.ntasks = h->ntasks; k.ndof = h->ndof; k.ninst = h->ninst;
     k.Ja = h->Ja; k.detJ = h->detJ; k.cmap = h->cmap; k.fmet = h->fmet; k.fmet_f = h->fmet_f; k.nrm = h->nrm; k.hmin = h->hmin;
-    k.nbr = h->nbr; k.bid = h->bid; k.wall = h->wall; k.tdesc = h->tdesc; k.bcs = h->bcs;
+    k.nbr = h->nbr; k.bid = h->bid; k.wall = h->wall; k.tdesc = h->tdesc; k.bcs = h->bcs; k.corder = h->corder;
     k.dtmin = h->dtmin; k.dtcount = h->dtcount; k.basis = h->basis;
     k.avars = h->avars; k.araw = h->araw; k.alpha = h->alpha; k.alpha_d = h->alpha_d;
     k.sinv = h->sinv; k.sface = h->sface; k.avars_f = h->avars_f; k.sinv_f = h->sinv_f; k.sface_f = h->sface_f;
@@ -2175,14 +2180,14 @@ uint64_t morton_spread(uint64_t v) {
     return v;
 }
 
-// Face tasks in the Morton order of their cells' centroids (from the vertices, else the DoF
-// locations; unchanged without either): the cells a task gathers from were then gathered by tasks
+// Morton codes of the cells' centroids (from the vertices, else the DoF locations; none without
+// either). Face tasks are sorted by them: the cells a task gathers from were then gathered by tasks
 // shortly before it, on the same XCD, and are still in its L2. Ties keep the cell order, and a
 // cell's tasks keep their face order; the tasks write disjoint outputs, so the order does not
 // change any result.
-void order_tasks(const plens_mesh_desc* m, int N, std::vector<int>& tasks) {
+bool cell_morton_codes(const plens_mesh_desc* m, int N, std::vector<uint64_t>& code) {
     const int nc = m->n_cells, NN = (N + 1) * (N + 1) * (N + 1);
-    if (nc <= 1 || (!m->vertices && !m->node_coords)) return;
+    if (nc <= 1 || (!m->vertices && !m->node_coords)) return false;
     std::vector<double> cen((size_t)nc * 3, 0.0);
     for (int c = 0; c < nc; ++c)
         for (int d = 0; d < 3; ++d) {
@@ -2204,15 +2209,18 @@ void order_tasks(const plens_mesh_desc* m, int N, std::vector<int>& tasks) {
             hi[d] = std::max(hi[d], cen[(size_t)c * 3 + d]);
         }
     const double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
-    if (!(ext > 0)) return;
+    if (!(ext > 0)) return false;
     const double sc = 2097151.0 / ext;  // one scale for the three axes
-    std::vector<uint64_t> code(nc);
+    code.assign(nc, 0);
     for (int c = 0; c < nc; ++c) {
         uint64_t k = 0;
         for (int d = 0; d < 3; ++d)
             k |= morton_spread((uint64_t)((cen[(size_t)c * 3 + d] - lo[d]) * sc)) << d;
         code[c] = k;
     }
+    return true;
+}
+void order_tasks(const std::vector<uint64_t>& code, std::vector<int>& tasks) {
     std::stable_sort(tasks.begin(), tasks.end(), [&](int a, int b) { return code[a / 6] < code[b / 6]; });
 }
 }  // namespace
@@ -2268,9 +2276,19 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
                 tasks.push_back(c * 6 + f);
             }
         }
-    order_tasks(m, N, tasks);
+    std::vector<uint64_t> mcode;
+    const bool have_codes = cell_morton_codes(m, N, mcode);
+    if (have_codes) order_tasks(mcode, tasks);
     h->ntasks = (int)tasks.size();
     if ((rc = alloc_buffers(h))) return fail(rc);
+    if (have_codes) {  // k_grad's cell order; ties keep the index order
+        std::vector<int> order(nc);
+        for (int c = 0; c < nc; ++c) order[c] = c;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return mcode[a] < mcode[b]; });
+        if ((rc = dmalloc(&h->corder, (size_t)nc))) return fail(rc);
+        if (hipMemcpy(h->corder, order.data(), sizeof(int) * (size_t)nc, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(PLENS_E_HIP);
+    }
     // host-side tables
     std::vector<BCDev> hb(std::max(n_bcs, 1));
     const double R = 8.314462618 / ph->molecular_weight;  // NavierStokes::get_R
@@ -2495,7 +2513,7 @@ void plens_gpu_destroy(plens_gpu* h) {
                     h->sinv, h->sface, h->rhs, h->rlo, h->dt_cell, h->scal, h->fail, h->blist, h->bcount, h->mlp_stats, h->feat,
                     h->avars_f, h->sinv_f, h->sface_f, h->cinst, h->ioff, h->visc, h->cfl, h->cmap, h->fmet, h->fmet_f,
                     h->tabs, h->rhoE_old, h->dt_lts, h->ss_part, h->post, h->vslot, h->vin, h->vxyz,
-                    h->vbc, h->vcode};
+                    h->vbc, h->vcode, h->corder};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
