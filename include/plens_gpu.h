@@ -5,6 +5,7 @@
  *   PLENS::calc_rhs()            dgsem/plens.cc:3688-3779   -> plens_gpu_calc_rhs()
  *   PLENS::calc_time_step()      dgsem/plens.cc:3810-3912   -> plens_gpu_calc_dt()   (global stepping)
  *   PLENS::update()+update_rk4() dgsem/plens.cc:4484-4518, 4334-4403 -> plens_gpu_rk4_step()
+ *                                (or, with the step reduced by the caller, plens_gpu_rk4_step_dt())
  *   PLENS::run() time loop       dgsem/plens.cc:1547        -> plens_gpu_run()
  *   NavierStokes::assert_positivity's std::exit (dgsem/plens.cc:2540-2544)
  *                                                            -> PLENS_E_POSITIVITY + plens_gpu_last_failure()
@@ -154,12 +155,25 @@ int plens_gpu_get_state(plens_gpu* h, double* Q);
  * post-diffusion blender gcrk_alpha), alpha_d_out [n_cells] (nullable). */
 int plens_gpu_calc_rhs(plens_gpu* h, double* rhs_out, double* alpha_out, double* alpha_d_out);
 
-/* == PLENS::calc_time_step() with global stepping; needs mu, so it evaluates mu from the state. */
-int plens_gpu_calc_dt(plens_gpu* h, double courant, double* dt_global);
+/* == PLENS::calc_time_step() (dgsem/plens.cc:3810-3912) with global stepping; needs mu, so it
+ * evaluates mu from the state. dt_global = minimum over the cells of the per-node stable step
+ * Co / (N^1.5 (max_d (J^-T v)_d + a/h + N^2 mu / (h^2 rho))), Co >= 1 capped to 0.9; dt_cell
+ * [n_cells] (nullable) = each cell's own minimum (this_cell_step, 3844-3866), the values a caller
+ * that owns several partitions reduces itself (3872-3875: MPI_Reduce + MPI_Bcast of the minimum). */
+int plens_gpu_calc_dt(plens_gpu* h, double courant, double* dt_global, double* dt_cell);
 
 /* One time step == PLENS::update() with RK order 4: dt from the stage-1 state (courant capped at
  * 0.9 when >= 1), 5 stages of calc_rhs + 3-register update. *t_inout += dt. */
 int plens_gpu_rk4_step(plens_gpu* h, double courant, double* t_inout);
+
+/* One time step with a step the caller supplies: update() + update_rk4() (4334-4403, 4484-4518)
+ * with time_step = dt_global, e.g. the minimum of plens_gpu_calc_dt's dt_global over every
+ * partition (the reference's MPI_Reduce/MPI_Bcast at 3872-3875), or that step clipped to
+ * end_time - cur_time (4340). The five stages and the register update are those of
+ * plens_gpu_rk4_step, so calc_dt followed by this call gives the same state bit for bit. t is the
+ * solution time at the start of the step (the BCs' cur_time). Global stepping only: PLENS_E_INVALID
+ * when local time stepping is requested, or for dt_global <= 0 or not finite. */
+int plens_gpu_rk4_step_dt(plens_gpu* h, double t, double dt_global);
 
 /* n_steps time steps replayed from a captured HIP graph; *t_inout += sum of dt. The state stays on
  * the device; positivity failures are reported at the end. */

@@ -20,6 +20,27 @@ namespace {
 
 using State = std::array<double, 5>;
 using Avars = std::array<double, 9>;
+
+// Planted mistakes (checker hook, oracle_set_mutation): each turns one operation of the time step,
+// the inflow/outflow/isothermal-wall ghosts or the blender post-steps into a plausible misreading
+// of the reference, so that the tests can show that their closed forms catch it. 0 = none.
+enum Mutation {
+    kMutNone = 0,
+    kMutDtAbs = 1,          // max_d |(J^-T v)_d| instead of the signed maximum (plens.cc:3856-3858)
+    kMutDtDegreeSq = 2,     // N^2 in place of N^1.5 (3860)
+    kMutDtNoCap = 3,        // Courant >= 1 not capped to 0.9 (3818-3822)
+    kMutDtJinv = 4,         // J^-1 v instead of J^-T v (3850-3855)
+    kMutInflowS1 = 11,      // uniform inflow stage-1 ghost = prescribed state (uniform_inflow.cc:23-32)
+    kMutOutflowSign = 12,   // outflow stage-2 energy correction with p_in - p_pr (outflow.cc:47-55)
+    kMutOutflowInviscid = 13,  // outflow stage-2 correction also for inviscid flow (outflow.cc:48)
+    kMutWallRho = 14,       // isothermal wall prescribed density from the inner temperature (uniform_temp_wall.cc:25-27)
+    kMutWallAbsVel = 15,    // isothermal wall stage-2 reverses the absolute momentum (uniform_temp_wall.cc:67-68)
+    kMutBlendLive = 21,     // diffusion reads the neighbours' diffused values (plens.cc:2926-2936 reads gh)
+    kMutBlendPeriodic = 22, // diffusion across periodic faces (plens.cc:2928 skips every boundary face)
+    kMutBlendOverrideLate = 23,  // wall override after the diffusion (plens.cc:2909-2920 comes first)
+    kMutBlendNoCap = 24     // alpha_d = alpha / alpha_max without min(1, .) (plens.cc:2953-2957)
+};
+int g_mutation = kMutNone;
 struct T1 {  // rank-1 tensor in 3D (deal.II Tensor<1,3>)
     double v[3] = {0, 0, 0};
     double& operator[](int i) { return v[i]; }
@@ -873,6 +894,7 @@ struct oracle {
     // solution vectors
     std::vector<double> Q, avars, mu, kap, alpha, alpha_d, rhs, rhs1, rhs2, Qn, loc_dt;
     std::vector<double> alpha_given, alpha_d_given;  // checker hook: blender values to use, when set
+    std::vector<double> raw_given;                    // checker hook: indicator values (pre-override)
     std::vector<double> s1, s2, s3;                     // surface terms [var][c][f][q]
     std::vector<double> ws1;                            // entropy variables of s1 (default build)
     std::vector<double> verts;                          // cell vertices [c][8][3]
@@ -1078,18 +1100,19 @@ struct oracle {
                 T1 v; v[0] = b.vel[0]; v[1] = b.vel[1]; v[2] = b.vel[2];
                 const double rho = b.p / (ns.R() * b.T);  // set_BC, plens.cc:1268-1270
                 ns.prim_to_cons(rho, v, b.p, pr);
-                if (stage == 2) g = pr;
+                if (stage == 2 || (stage == 1 && g_mutation == kMutInflowS1)) g = pr;
                 else for (int k = 0; k < 5; ++k) g[k] = 2 * pr[k] - in[k];
                 break;
             }
             case 2: {  // outflow (outflow.cc:18-75)
                 if (stage == 2) {
                     const double M = ns.get_M(in);
-                    if (ns.inviscid() || M >= 1) g = in;
+                    if ((ns.inviscid() && g_mutation != kMutOutflowInviscid) || M >= 1) g = in;
                     else {
                         g = in;
                         const double pin = ns.get_p(in);
-                        g[4] += 2 * (b.p - pin) / (ns.gma - 1);
+                        if (g_mutation == kMutOutflowSign) g[4] += 2 * (pin - b.p) / (ns.gma - 1);
+                        else g[4] += 2 * (b.p - pin) / (ns.gma - 1);
                     }
                 } else g = in;
                 break;
@@ -1153,13 +1176,16 @@ struct oracle {
                         for (int d = 0; d < 3; ++d) { sm[d] = in[1 + d]; smr[d] = in[1 + d] - in[0] * vw[d]; }
                         const double nsm = dot(smr, nrm);
                         for (int d = 0; d < 3; ++d) g[1 + d] = sm[d] - 2 * nsm * nrm[d];
+                    } else if (g_mutation == kMutWallAbsVel) {
+                        for (int d = 0; d < 3; ++d) g[1 + d] = -in[1 + d];
                     } else {
                         for (int d = 0; d < 3; ++d) g[1 + d] = 2 * in[0] * vw[d] - in[1 + d];
                     }
                 } else {
                     State pr;
                     const double pin = ns.get_p(in);
-                    const double rpr = (b.type == 7) ? in[0] : pin / (ns.R() * b.T);
+                    const bool inner_rho = b.type == 7 || g_mutation == kMutWallRho;
+                    const double rpr = inner_rho ? in[0] : pin / (ns.R() * b.T);
                     ns.prim_to_cons(rpr, vw, pin, pr);
                     for (int k = 0; k < 5; ++k) g[k] = 2 * pr[k] - in[k];
                     if (b.type == 7 && stage == 3) {
@@ -1447,22 +1473,31 @@ struct oracle {
                 raw[c] = blender_from_values(ns, ph, N, C, vals.data());
             }
         }
-        for (int c = 0; c < nc; ++c) {
-            alpha[c] = raw[c];
-            for (int f = 0; f < 6; ++f) {
-                const int b = bid[(size_t)c * 6 + f];
-                if (b >= 0 && (bcs[b].type == 3 || bcs[b].type == 7))  // uniform temp / insulated wall
-                    alpha[c] = std::max(ph.wall_blender_limit, raw[c]);
+        if (!raw_given.empty()) raw = raw_given;  // checker hook: given indicator values
+        auto wall_override = [&]() {
+            for (int c = 0; c < nc; ++c) {
+                for (int f = 0; f < 6; ++f) {
+                    const int b = bid[(size_t)c * 6 + f];
+                    if (b >= 0 && (bcs[b].type == 3 || bcs[b].type == 7))  // uniform temp / insulated wall
+                        alpha[c] = std::max(ph.wall_blender_limit, g_mutation == kMutBlendOverrideLate ? alpha[c] : raw[c]);
+                }
             }
-        }
+        };
+        alpha = raw;
+        if (g_mutation != kMutBlendOverrideLate) wall_override();
         const std::vector<double> gh = alpha;  // gh_gcrk_alpha = gcrk_alpha
+        const std::vector<double>& nsrc = g_mutation == kMutBlendLive ? alpha : gh;
         for (int c = 0; c < nc; ++c)
             for (int f = 0; f < 6; ++f) {
-                if (bid[(size_t)c * 6 + f] >= 0) continue;  // face->at_boundary() (periodic included)
-                const double cur = alpha[c], nei = gh[nbr[(size_t)c * 6 + f]];
+                // face->at_boundary() (periodic included)
+                const int b = bid[(size_t)c * 6 + f];
+                if (b >= 0 && !(g_mutation == kMutBlendPeriodic && bcs[b].type == 0)) continue;
+                const double cur = alpha[c], nei = nsrc[nbr[(size_t)c * 6 + f]];
                 alpha[c] = std::max(cur, 0.5 * nei);
             }
-        for (int c = 0; c < nc; ++c) alpha_d[c] = std::min(1.0, alpha[c] / ph.alpha_max);  // region "1"
+        if (g_mutation == kMutBlendOverrideLate) wall_override();
+        for (int c = 0; c < nc; ++c)  // region "1"
+            alpha_d[c] = g_mutation == kMutBlendNoCap ? alpha[c] / ph.alpha_max : std::min(1.0, alpha[c] / ph.alpha_max);
     }
 
     // ---- PLENS::calc_cell_ho_inv_residual, dgsem/plens.cc:3153-3275 ----------------------------
@@ -1650,7 +1685,7 @@ struct oracle {
     // ---- PLENS::calc_time_step, dgsem/plens.cc:3810-3912: per-cell steps into loc_dt, then global
     //      stepping (every cell gets the minimum) or, past the start time, local stepping
     double calc_time_step(double Co, double cur_time = 0.0) {
-        if (Co >= 1) Co = 0.9;
+        if (Co >= 1 && g_mutation != kMutDtNoCap) Co = 0.9;
         double proc = 1e6;
         loc_dt.assign(nc, 0.0);
         for (int c = 0; c < nc; ++c) {
@@ -1663,9 +1698,13 @@ struct oracle {
                 const double* Ji = &Jinv[gid(c, i) * 9];
                 double adv[3] = {0, 0, 0};
                 for (int row = 0; row < 3; ++row)
-                    for (int col = 0; col < 3; ++col) adv[row] += Ji[col * 3 + row] * vel[col];  // JinvT
+                    for (int col = 0; col < 3; ++col)  // JinvT
+                        adv[row] += (g_mutation == kMutDtJinv ? Ji[row * 3 + col] : Ji[col * 3 + row]) * vel[col];
+                if (g_mutation == kMutDtAbs)
+                    for (double& x : adv) x = std::fabs(x);
                 const double madv = std::max({adv[0], adv[1], adv[2]});
-                const double cur = Co / (std::pow(N, 1.5) * (madv + a / h + N * N * mu[gid(c, i)] / (h * h * s[0])));
+                const double pw = std::pow(N, g_mutation == kMutDtDegreeSq ? 2.0 : 1.5);
+                const double cur = Co / (pw * (madv + a / h + N * N * mu[gid(c, i)] / (h * h * s[0])));
                 if (cur < cs) cs = cur;
             }
             loc_dt[c] = cs;
@@ -1917,6 +1956,34 @@ int oracle_set_alpha(oracle* o, const double* alpha, const double* alpha_d) {
     }
     o->alpha_given.assign(alpha, alpha + o->nc);
     o->alpha_d_given.assign(alpha_d, alpha_d + o->nc);
+    return 0;
+}
+
+int oracle_set_raw_alpha(oracle* o, const double* raw) {
+    if (!raw) o->raw_given.clear();
+    else o->raw_given.assign(raw, raw + o->nc);
+    return 0;
+}
+
+int oracle_set_mutation(int32_t id) {
+    g_mutation = id;
+    return 0;
+}
+
+int oracle_bc_ghost(const oracle* o, int32_t stage, int32_t cell, int32_t face, int32_t q, const double* in,
+                    const double* ain, const double* nrm, double* g, double* ag) {
+    if (!o || !in || !ain || !nrm || !g || !ag || stage < 1 || stage > 3 || cell < 0 || cell >= o->nc ||
+        face < 0 || face > 5 || q < 0 || q >= o->nf || o->bid[(size_t)cell * 6 + face] < 0)
+        return -1;
+    State s, gs;
+    Avars a, ga;
+    T1 n;
+    for (int v = 0; v < 5; ++v) s[v] = in[v];
+    for (int v = 0; v < 9; ++v) a[v] = ain[v];
+    for (int d = 0; d < 3; ++d) n[d] = nrm[d];
+    o->ghost(stage, cell, face, q, s, a, n, gs, ga);
+    for (int v = 0; v < 5; ++v) g[v] = gs[v];
+    for (int v = 0; v < 9; ++v) ag[v] = ga[v];
     return 0;
 }
 

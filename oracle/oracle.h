@@ -57,6 +57,22 @@ int oracle_set_time(oracle* o, double t);
  * evaluations use these blender values alpha and alpha_d in place of calc_blender's, which separates
  * the arithmetic of the DG operator from the conditioning of the blender; NULL restores calc_blender. */
 int oracle_set_alpha(oracle* o, const double* alpha, const double* alpha_d);
+/* Checker hook: with raw given ([n_cells]), calc_blender takes these indicator values in place of
+ * get_blender's and applies its post-steps to them (wall override, diffusion, alpha_d); NULL clears. */
+int oracle_set_raw_alpha(oracle* o, const double* raw);
+/* Checker hook: the ghost state and auxiliary variables that boundary face `face` of `cell` gives at
+ * face node q and stage 1..3 (the BC getters of IBCs/), for the inner state in[5], inner avars ain[9]
+ * and outward unit normal nrm[3]; -1 for an interior face or bad indices. */
+int oracle_bc_ghost(const oracle* o, int32_t stage, int32_t cell, int32_t face, int32_t q, const double* in,
+                    const double* ain, const double* nrm, double* g, double* ag);
+/* Checker hook: plant mistake `id` (process-wide; 0 = none) in the time step, the inflow/outflow/
+ * isothermal-wall ghosts or the blender post-steps, so that tests can show their pins catch it:
+ * 1 abs in the advection maximum, 2 N^2 for N^1.5, 3 no Courant cap, 4 J^-1 v for J^-T v;
+ * 11 inflow stage-1 ghost = prescribed state, 12 outflow correction sign, 13 outflow correction
+ * also when inviscid, 14 wall density from the inner temperature, 15 wall stage 2 reverses the
+ * absolute momentum; 21 diffusion from diffused neighbours, 22 diffusion across periodic faces,
+ * 23 wall override after diffusion, 24 alpha_d not capped at 1. */
+int oracle_set_mutation(int32_t id);
 /* location of node `node` of cell c (through the cell's map) */
 int oracle_node_location(const oracle* o, int32_t c, int32_t node, double* xyz);
 /* value of one expression of the varying inflow BC; NaN on a syntax error */

@@ -49,6 +49,9 @@ def load():
         "oracle_output_fields": (C.c_int, [vp, C.c_double, C.c_double, _dp, _dp]),
         "oracle_set_time": (C.c_int, [vp, C.c_double]),
         "oracle_set_alpha": (C.c_int, [vp, _dp, _dp]),
+        "oracle_set_raw_alpha": (C.c_int, [vp, _dp]),
+        "oracle_set_mutation": (C.c_int, [C.c_int32]),
+        "oracle_bc_ghost": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp, _dp, _dp]),
         "oracle_node_location": (C.c_int, [vp, C.c_int32, C.c_int32, _dp]),
         "oracle_eval_expression": (C.c_double, [C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_double]),
         "oracle_set_threads": (C.c_int, [C.c_int32]),
@@ -85,6 +88,21 @@ def load():
 
 def p(a):
     return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(_dp)
+
+
+class mutation:
+    """context manager planting oracle mistake `id` (oracle_set_mutation) for the duration of a block"""
+
+    def __init__(self, mid):
+        self.mid = mid
+
+    def __enter__(self):
+        load().oracle_set_mutation(self.mid)
+        return self
+
+    def __exit__(self, *exc):
+        load().oracle_set_mutation(0)
+        return False
 
 
 def ophys_from(ph):
@@ -212,6 +230,24 @@ class OraclePlens:
                              np.ascontiguousarray(alpha_d, dtype=np.float64))
         self.lib.oracle_set_alpha(self._h, self._alpha_given[0].ctypes.data_as(_dp),
                                   self._alpha_given[1].ctypes.data_as(_dp))
+
+    def set_raw_alpha(self, raw=None):
+        """indicator values (per cell, before the wall override) for calc_blender's post-steps in
+        place of get_blender's; None restores get_blender"""
+        if raw is None:
+            self.lib.oracle_set_raw_alpha(self._h, None)
+            return
+        self._raw_given = np.ascontiguousarray(raw, dtype=np.float64)
+        self.lib.oracle_set_raw_alpha(self._h, self._raw_given.ctypes.data_as(_dp))
+
+    def bc_ghost(self, stage, cell, face, q, u, av, nrm):
+        """ghost state and avars of boundary face `face` of `cell` at face node q and stage 1..3"""
+        g, ag = np.empty(5), np.empty(9)
+        rc = self.lib.oracle_bc_ghost(self._h, stage, cell, face, q, p(u), p(av), p(nrm),
+                                      g.ctypes.data_as(_dp), ag.ctypes.data_as(_dp))
+        if rc:
+            raise ValueError("not a boundary face")
+        return g, ag
 
     def output_fields(self, courant, t=0.0):
         nodal = np.empty((15, self.n_dof))

@@ -69,7 +69,7 @@ MANIFOLD_CYLINDER, MANIFOLD_NOSE_CYLINDER = 1, 2
 # exported symbols (checked by the CPU tests against include/*.h)
 GPU_SYMBOLS = [
     "plens_gpu_create", "plens_gpu_destroy", "plens_gpu_set_state", "plens_gpu_get_state",
-    "plens_gpu_calc_rhs", "plens_gpu_calc_dt", "plens_gpu_rk4_step", "plens_gpu_run",
+    "plens_gpu_calc_rhs", "plens_gpu_calc_dt", "plens_gpu_rk4_step", "plens_gpu_rk4_step_dt", "plens_gpu_run",
     "plens_gpu_last_failure", "plens_gpu_set_tuning", "plens_gpu_kernel_times", "plens_gpu_strerror",
     "plens_gpu_last_run_ms", "plens_gpu_run_profiled", "plens_gpu_stream_copy", "plens_gpu_time_kernel",
     "plens_gpu_set_local_stepping", "plens_gpu_ss_error", "plens_gpu_output_fields", "plens_gpu_write_checkpoint",
@@ -113,7 +113,8 @@ def load():
         "plens_gpu_set_state": (C.c_int, [vp, _dp]),
         "plens_gpu_get_state": (C.c_int, [vp, _dp]),
         "plens_gpu_calc_rhs": (C.c_int, [vp, _dp, _dp, _dp]),
-        "plens_gpu_calc_dt": (C.c_int, [vp, C.c_double, _dp]),
+        "plens_gpu_calc_dt": (C.c_int, [vp, C.c_double, _dp, _dp]),
+        "plens_gpu_rk4_step_dt": (C.c_int, [vp, C.c_double, C.c_double]),
         "plens_gpu_rk4_step": (C.c_int, [vp, C.c_double, _dp]),
         "plens_gpu_run": (C.c_int, [vp, C.c_int32, C.c_double, _dp]),
         "plens_gpu_run_profiled": (C.c_int, [vp, C.c_int32, C.c_double, _dp]),

@@ -73,6 +73,19 @@ class Mesh:
         return cls(degree, cv.shape[0], 1, 1, None, None, None, None, None,
                    transpose_y_face_normals=transpose_y_face_normals, _curved=h)
 
+    @classmethod
+    def from_cells(cls, degree, cell_vertices, face_nbr, face_bid, transpose_y_face_normals=True):
+        """Trilinear (MappingQ1) mesh of the given cells [n_c][8][3] (deal.II vertex order, each cell
+        its own vertices) and connectivity: plens_mesh_build_curved at mapping degree 1, whose
+        support points are the vertices, so the manifold plays no part."""
+        V = np.ascontiguousarray(cell_vertices, dtype=np.float64).reshape(-1, 8, 3)
+        md = L.ManifoldDesc()
+        md.type = L.MANIFOLD_CYLINDER
+        md.axis_dir[0] = 1.0
+        cells = np.arange(V.shape[0] * 8, dtype=np.int32).reshape(-1, 8)
+        return cls.curved(degree, 1, V.reshape(-1, 3), cells, face_nbr, face_bid, md,
+                          transpose_y_face_normals=transpose_y_face_normals)
+
     def _load(self, lib):
         degree = self.degree
         self.n = (degree + 1) ** 3

@@ -633,6 +633,20 @@ extern "C" int plens_mesh_build_curved(int32_t N, int32_t q, int32_t nc, int32_t
         if (cv[i] < 0 || cv[i] >= nv) return PLENS_E_INVALID;
     for (size_t i = 0; i < (size_t)nc * 6; ++i)
         if (nbr[i] < -1 || nbr[i] >= nc) return PLENS_E_INVALID;
+    // interior faces: a reciprocal neighbour other than the cell itself whose matching face f^1
+    // has the same vertices in the standard orientation (vertex v of face f is vertex v ^ (1 << d)
+    // of the neighbour), which is what the face-node maps of both sides assume
+    // (form_neighbor_face_matchings, dgsem/plens.cc:1574-1627, for the orientation every mesh of
+    // the reference's structured generators has)
+    for (int c = 0; c < nc; ++c)
+        for (int f = 0; f < 6; ++f) {
+            if (bid[(size_t)c * 6 + f] >= 0) continue;
+            const int nb = nbr[(size_t)c * 6 + f], d = f >> 1;
+            if (nb < 0 || nb == c || nbr[(size_t)nb * 6 + (f ^ 1)] != c) return PLENS_E_INVALID;
+            for (int v = 0; v < 8; ++v)
+                if (((v >> d) & 1) == (f & 1) && cv[8 * (size_t)c + v] != cv[8 * (size_t)nb + (v ^ (1 << d))])
+                    return PLENS_E_INVALID;
+        }
     const double an = std::sqrt(md->axis_dir[0] * md->axis_dir[0] + md->axis_dir[1] * md->axis_dir[1] +
                                 md->axis_dir[2] * md->axis_dir[2]);
     if (!(an > 0)) return PLENS_E_INVALID;

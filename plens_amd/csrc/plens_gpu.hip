@@ -96,12 +96,9 @@ struct KP {
     float* avars_f;  // FP32-bulk storage of avars and the face terms
     float* sinv_f;
     float* sface_f;
-    double* Rnew;            // RK stage registers: the new one, R', R'' (FP64 handles)
+    double* Rnew;            // RK stage registers: the new one, R', R'' (FP64 in every variant)
     const double* R1;
     const double* R2;
-    float* Rnew_f;           // the same in FP32 (FP32-bulk: increments stored in FP32, state in FP64)
-    const float* R1_f;
-    const float* R2_f;
     double* rhs_out;
     double* rlo;
     int* blist;
@@ -144,6 +141,7 @@ struct KP {
     double iw0, iwN;  // 1 / w_0, 1 / w_N of the GLL rule
     int stage;    // 0: write rhs only; 1..5: RK stage
     int want_dt;  // compute the stage-1 time step
+    double dt_given;  // > 0: the caller's global step replaces the reduced one (single instance)
     double c1, c2, c3;
 };
 
@@ -908,7 +906,8 @@ __global__ void __launch_bounds__(kThreadsAlpha) k_alpha(KP p) {
     if (!last) return;
     __threadfence();
     for (int i = threadIdx.x; i < p.ninst; i += blockDim.x) {
-        const double dt = __longlong_as_double((long long)atomicExch(p.dtmin + i, ~0ull));
+        const double red = __longlong_as_double((long long)atomicExch(p.dtmin + i, ~0ull));
+        const double dt = p.dt_given > 0.0 ? p.dt_given : red;
         p.dt[i] = dt;
         if (p.t_acc) {
             p.t_acc[i] += dt;
@@ -1566,12 +1565,8 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
         for (int v = 0; v < 5; ++v) p.rhs_out[v * p.ndof + g] = rhs[v];
         return;
     }
-    // the stage's own increment enters in FP64; the registers kept for later stages are in the
-    // flux-arithmetic type (FP32 for FP32-bulk)
+    // the registers R', R'' and the new increment are FP64 in every variant
     const double dt = p.dt_lts ? p.dt_lts[c] : p.dt[p.cinst[c]];
-    const TF* R1 = tsel<TF>(p.R1, p.R1_f);
-    const TF* R2 = tsel<TF>(p.R2, p.R2_f);
-    TF* Rn = tsel<TF>(p.Rnew, p.Rnew_f);
     for (int v = 0; v < 5; ++v) {
         const long long k = v * p.ndof + g;
         const double R = rhs[v] * dt;
@@ -1579,12 +1574,12 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
         if (p.stage == 1) {
             qn = u[v] + p.c3 * R;
         } else if (p.stage == 2) {
-            qn = u[v] + (p.c2 * (double)R1[k] + p.c3 * R);
+            qn = u[v] + (p.c2 * p.R1[k] + p.c3 * R);
         } else {
-            qn = u[v] + (p.c1 * (double)R2[k] + p.c2 * (double)R1[k] + p.c3 * R);
+            qn = u[v] + (p.c1 * p.R2[k] + p.c2 * p.R1[k] + p.c3 * R);
         }
         p.Qout[k] = qn;
-        Rn[k] = (TF)R;
+        p.Rnew[k] = R;
     }
 }
 
@@ -1686,7 +1681,6 @@ struct plens_gpu {
     int device = 0;
     // device buffers
     double *Qa = nullptr, *Qb = nullptr, *R[2] = {nullptr, nullptr};
-    float* Rf[2] = {nullptr, nullptr};            // RK registers of FP32-bulk handles
     double *Ja = nullptr, *detJ = nullptr, *nrm = nullptr, *hmin = nullptr;
     double* cmap = nullptr;  // trilinear maps of the cells; the stored metric arrays are not kept then
     double* fmet = nullptr;  // face-node metric terms of the face tasks
@@ -1926,9 +1920,6 @@ KP stage_params(plens_gpu* h, int s, bool from_b, double courant) {
     kp.Rnew = h->R[wr[s]];
     kp.R1 = h->R[r1[s]];
     kp.R2 = h->R[r2[s]];
-    kp.Rnew_f = h->Rf[wr[s]];
-    kp.R1_f = h->Rf[r1[s]];
-    kp.R2_f = h->Rf[r2[s]];
     kp.stage = s;
     kp.want_dt = (s == 1);
     kp.save_E = (s == 1);
@@ -1954,9 +1945,10 @@ int launch_vinflow(plens_gpu* h, const KP& kp) {
     return PLENS_OK;
 }
 
-int launch_step(plens_gpu* h, bool from_b, double courant, hipEvent_t* ev = nullptr) {
+int launch_step(plens_gpu* h, bool from_b, double courant, hipEvent_t* ev = nullptr, double dt_given = 0.0) {
     for (int s = 1; s <= 5; ++s) {
         KP kp = stage_params(h, s, from_b, courant);
+        kp.dt_given = dt_given;
         if (s == 1) {  // the BCs' time is the step's start time for all stages
             const int e = launch_vinflow(h, kp);
             if (e) return e;
@@ -2060,10 +2052,8 @@ void host_node_metrics(const double* k, double x0, double x1, double x2, double 
 int alloc_buffers(plens_gpu* h) {
     const size_t nd = (size_t)h->ndof, nc = (size_t)h->nc, NF = (size_t)h->NF, nface = nc * 30 * NF;
     int rc;
-    const bool fp64 = h->precision == PLENS_PREC_FP64;
     if ((rc = dmalloc(&h->Qa, 5 * nd)) || (rc = dmalloc(&h->Qb, 5 * nd)) ||
-        (rc = fp64 ? dmalloc(&h->R[0], 5 * nd) : dmalloc(&h->Rf[0], 5 * nd)) ||
-        (rc = fp64 ? dmalloc(&h->R[1], 5 * nd) : dmalloc(&h->Rf[1], 5 * nd)) ||
+        (rc = dmalloc(&h->R[0], 5 * nd)) || (rc = dmalloc(&h->R[1], 5 * nd)) ||
         (rc = dmalloc(&h->rhs, 5 * nd)) || (rc = dmalloc(&h->rlo, 5 * nd)) ||
         (rc = dmalloc(&h->nrm, nc * 18 * NF)) ||
         (rc = (h->precision == PLENS_PREC_FP64 ? dmalloc(&h->fmet, (size_t)h->ntasks * 10 * NF)
@@ -2101,9 +2091,8 @@ int alloc_buffers(plens_gpu* h) {
             return rc;
         if (hipMemset(h->avars_f, 0, sizeof(float) * 9 * nd) != hipSuccess) return PLENS_E_HIP;
     }
-    const size_t rbytes = (h->precision == PLENS_PREC_FP64 ? sizeof(double) : sizeof(float)) * 5 * nd;
-    if (hipMemset(h->R[0] ? (void*)h->R[0] : (void*)h->Rf[0], 0, rbytes) != hipSuccess ||
-        hipMemset(h->R[1] ? (void*)h->R[1] : (void*)h->Rf[1], 0, rbytes) != hipSuccess ||
+    const size_t rbytes = sizeof(double) * 5 * nd;
+    if (hipMemset(h->R[0], 0, rbytes) != hipSuccess || hipMemset(h->R[1], 0, rbytes) != hipSuccess ||
         hipMemset(h->rlo, 0, sizeof(double) * 5 * nd) != hipSuccess ||
         hipMemset(h->alpha, 0, sizeof(double) * nc) != hipSuccess ||
         hipMemset(h->scal, 0, sizeof(double) * 2 * h->ninst) != hipSuccess ||
@@ -2223,6 +2212,61 @@ bool cell_morton_codes(const plens_mesh_desc* m, int N, std::vector<uint64_t>& c
 void order_tasks(const std::vector<uint64_t>& code, std::vector<int>& tasks) {
     std::stable_sort(tasks.begin(), tasks.end(), [&](int a, int b) { return code[a / 6] < code[b / 6]; });
 }
+
+// Connectivity the kernels rely on: an interior face has a neighbour other than the cell itself, a
+// periodic face a partner, and the neighbour's matching face is f^1, of the same kind, pointing back;
+// the face nodes of both sides coincide in the standard orientation (face node q of face f against
+// face node q of face f^1), checked on the vertices when given, else on the DoF locations when
+// given; periodic partners must differ by one translation. The reference matches neighbour face
+// nodes of any orientation by location (form_neighbor_face_matchings, dgsem/plens.cc:1574-1627) and
+// requires the standard orientation on periodic faces (IBCs/periodic.cc:38-48); this path takes the
+// standard orientation everywhere and rejects other meshes.
+int validate_connectivity(const plens_mesh_desc* m, const plens_bc* bcs, int n_bcs) {
+    const int nc = m->n_cells, N = m->degree, NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
+    for (int c = 0; c < nc; ++c)
+        for (int f = 0; f < 6; ++f) {
+            const int b = m->face_bid[c * 6 + f], nb = m->face_nbr[c * 6 + f];
+            if (b >= n_bcs || b < -1) return PLENS_E_INVALID;
+            if (b >= 0 && (bcs[b].type < 0 || bcs[b].type > PLENS_BC_VARYING_INFLOW)) return PLENS_E_INVALID;
+            const bool periodic = b >= 0 && bcs[b].type == PLENS_BC_PERIODIC;
+            if (b >= 0 && !periodic) continue;
+            if (nb < 0 || nb >= nc || (b < 0 && nb == c)) return PLENS_E_INVALID;
+            const int fo = f ^ 1, bo = m->face_bid[nb * 6 + fo];
+            if (m->face_nbr[nb * 6 + fo] != c) return PLENS_E_INVALID;
+            if (b < 0 ? bo >= 0 : (bo < 0 || bcs[bo].type != PLENS_BC_PERIODIC)) return PLENS_E_INVALID;
+            // coincident (interior) or translated (periodic) face points, standard orientation
+            const double tol = 1e-8 * m->min_vertex_dist[c];
+            double shift[3] = {0, 0, 0};
+            bool first = true;
+            auto check = [&](const double* a, const double* o) {
+                for (int d = 0; d < 3; ++d) {
+                    const double s = a[d] - o[d];
+                    if (b < 0 ? !(std::fabs(s) <= tol) : !(first || std::fabs(s - shift[d]) <= tol)) return false;
+                    if (first) shift[d] = s;
+                }
+                first = false;
+                return true;
+            };
+            const int dd = f >> 1;
+            if (m->vertices) {
+                for (int v = 0; v < 8; ++v) {
+                    if (((v >> dd) & 1) != (f & 1)) continue;
+                    if (!check(m->vertices + ((size_t)c * 8 + v) * 3, m->vertices + ((size_t)nb * 8 + (v ^ (1 << dd))) * 3))
+                        return PLENS_E_INVALID;
+                }
+            } else if (m->node_coords) {
+                for (int q = 0; q < NF; ++q) {
+                    double a[3], o[3];
+                    for (int d = 0; d < 3; ++d) {
+                        a[d] = m->node_coords[((size_t)c * 3 + d) * NN + plens_face_node(N, f, q)];
+                        o[d] = m->node_coords[((size_t)nb * 3 + d) * NN + plens_face_node(N, fo, q)];
+                    }
+                    if (!check(a, o)) return PLENS_E_INVALID;
+                }
+            }
+        }
+    return PLENS_OK;
+}
 }  // namespace
 
 int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const plens_bc* bcs, int32_t n_bcs,
@@ -2236,18 +2280,11 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
     if (ph->precision != PLENS_PREC_FP64 && ph->precision != PLENS_PREC_FP32_BULK) return PLENS_E_INVALID;
     if (ph->gradient_variables != PLENS_GRAD_CONSERVATIVE && ph->gradient_variables != PLENS_GRAD_ENTROPY)
         return PLENS_E_INVALID;
+    // the connectivity is checked before any device call
+    if (validate_connectivity(m, bcs, n_bcs) != PLENS_OK) return PLENS_E_INVALID;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PLENS_E_NODEVICE;
     const int N = m->degree, NP = N + 1, NN = NP * NP * NP, NF = NP * NP, nc = m->n_cells;
-    // validate the connectivity
-    for (int c = 0; c < nc; ++c)
-        for (int f = 0; f < 6; ++f) {
-            const int b = m->face_bid[c * 6 + f], nb = m->face_nbr[c * 6 + f];
-            if (b >= n_bcs) return PLENS_E_INVALID;
-            const bool needs_nbr = b < 0 || bcs[b].type == PLENS_BC_PERIODIC;
-            if (needs_nbr && (nb < 0 || nb >= nc)) return PLENS_E_INVALID;
-            if (b >= 0 && (bcs[b].type < 0 || bcs[b].type > PLENS_BC_VARYING_INFLOW)) return PLENS_E_INVALID;
-        }
     plens_gpu* h = new (std::nothrow) plens_gpu;
     if (!h) return PLENS_E_NOMEM;
     h->N = N; h->NN = NN; h->NF = NF; h->nc = nc; h->n_bcs = n_bcs;
@@ -2508,7 +2545,7 @@ void plens_gpu_destroy(plens_gpu* h) {
     destroy_graphs(h);
     for (auto& x : h->ev_run)
         if (x) (void)hipEventDestroy(x);
-    void* ptrs[] = {h->Qa, h->Qb, h->R[0], h->R[1], h->Rf[0], h->Rf[1], h->Ja, h->detJ, h->nrm, h->hmin, h->nbr,
+    void* ptrs[] = {h->Qa, h->Qb, h->R[0], h->R[1], h->Ja, h->detJ, h->nrm, h->hmin, h->nbr,
                     h->bid, h->wall, h->tasks, h->tdesc, h->dtmin, h->dtcount, h->bcs, h->basis, h->avars, h->araw, h->alpha, h->alpha_d,
                     h->sinv, h->sface, h->rhs, h->rlo, h->dt_cell, h->scal, h->fail, h->blist, h->bcount, h->mlp_stats, h->feat,
                     h->avars_f, h->sinv_f, h->sface_f, h->cinst, h->ioff, h->visc, h->cfl, h->cmap, h->fmet, h->fmet_f,
@@ -2569,7 +2606,7 @@ int plens_gpu_calc_rhs(plens_gpu* h, double* rhs_out, double* alpha_out, double*
     return PLENS_OK;
 }
 
-int plens_gpu_calc_dt(plens_gpu* h, double courant, double* dt) {
+int plens_gpu_calc_dt(plens_gpu* h, double courant, double* dt, double* dt_cell) {
     if (!h || !dt) return PLENS_E_INVALID;
     int e = reset_failure(h);
     if (e) return e;
@@ -2586,6 +2623,8 @@ int plens_gpu_calc_dt(plens_gpu* h, double courant, double* dt) {
     HIPCHK(hipGetLastError());
     if ((e = read_failure(h))) return e;
     HIPCHK(hipMemcpyAsync(dt, h->scal, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (dt_cell)
+        HIPCHK(hipMemcpyAsync(dt_cell, h->dt_cell, sizeof(double) * h->nc, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return PLENS_OK;
 }
@@ -2606,6 +2645,24 @@ int plens_gpu_rk4_step(plens_gpu* h, double courant, double* t_inout) {
     HIPCHK(hipMemcpyAsync(&dt, h->scal + h->ninst, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     if (t_inout) *t_inout += dt;
+    return PLENS_OK;
+}
+
+int plens_gpu_rk4_step_dt(plens_gpu* h, double t, double dt_global) {
+    if (!h || !(dt_global > 0.0) || !std::isfinite(dt_global) || h->ninst != 1 || h->base.dt_lts)
+        return PLENS_E_INVALID;
+    int e = reset_failure(h);
+    if (e) return e;
+    const double zero = 0.0;
+    HIPCHK(hipMemcpyAsync(h->scal + h->ninst, &zero, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->tabs, &t, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    // the stage-1 kernels still form every cell's candidate (dt_cell); the given step replaces the
+    // reduced one where k_alpha publishes it, so the stages scale by it and the time advances by it
+    if ((e = launch_step(h, h->state_in_b, 0.5, nullptr, dt_global))) return e;
+    h->state_in_b = !h->state_in_b;
+    h->steps_taken += 1;
+    if ((e = read_failure(h))) return e;
+    HIPCHK(hipStreamSynchronize(h->stream));
     return PLENS_OK;
 }
 

@@ -82,10 +82,17 @@ class GpuPlens:
         self._check(self.lib.plens_gpu_calc_rhs(self._h, _dptr(rhs), _dptr(a), _dptr(ad)), "calc_rhs")
         return rhs, a, ad
 
-    def calc_time_step(self, courant):
+    def calc_time_step(self, courant, cell_steps=False):
+        """global step (calc_time_step, global stepping); with cell_steps also every cell's own step"""
         dt = C.c_double()
-        self._check(self.lib.plens_gpu_calc_dt(self._h, courant, C.byref(dt)), "calc_dt")
-        return dt.value
+        dc = np.empty(self.n_cells) if cell_steps else None
+        self._check(self.lib.plens_gpu_calc_dt(self._h, courant, C.byref(dt), _dptr(dc) if cell_steps else None),
+                    "calc_dt")
+        return (dt.value, dc) if cell_steps else dt.value
+
+    def update_with_step(self, dt, t0=0.0):
+        """one RK4 step of the given global step dt from solution time t0"""
+        self._check(self.lib.plens_gpu_rk4_step_dt(self._h, t0, dt), "rk4_step_dt")
 
     def update(self, courant, t0=0.0):
         """one RK4 step from solution time t0; returns the step"""

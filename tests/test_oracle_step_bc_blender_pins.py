@@ -1,0 +1,368 @@
+"""Independent pins of three oracle functions: the time step (PLENS::calc_time_step,
+dgsem/plens.cc:3810-3912), the inflow / outflow / isothermal-wall ghost states (IBCs/uniform_inflow.cc,
+IBCs/outflow.cc, IBCs/uniform_temp_wall.cc) and the blender post-steps (wall override, diffusion,
+alpha_d; PLENS::calc_blender, dgsem/plens.cc:2905-2960).
+
+Each expected value comes from the mathematics the reference states, computed here by other means:
+the step from the affine map's matrix (numpy inverse) and a brute-force minimum vertex distance, the
+ghosts through what their BR1 averages or fluxes must equal (prescribed pressure, temperature and
+velocity of the face state), the blender from a hand-evaluated pattern. Every pin is shown to catch
+a planted mistake: the oracle's mistake switch (oracle_set_mutation) turns one operation into a
+plausible misreading and the same check must then fail.
+"""
+import itertools
+import math
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as PO
+from plens_amd import _lib as L
+from plens_amd import cases
+
+GAMMA = 1.4
+R = 1.0  # molecular weight R0: nondimensional gas constant 1
+
+
+# ---- time step -------------------------------------------------------------------------------
+def lattice_mesh(A, N, n=2):
+    """periodic lattice of n^3 parallelepiped cells x = A (ijk + xi), xi in [0,1]^3, deal.II vertex
+    order (x fastest); every face periodic (bid 0)"""
+    A = np.asarray(A, dtype=float)
+    nc = n ** 3
+    V = np.empty((nc, 8, 3))
+    nbr = np.empty((nc, 6), dtype=np.int32)
+    for k, j, i in itertools.product(range(n), repeat=3):
+        c = i + n * (j + n * k)
+        for v in range(8):
+            V[c, v] = A @ np.array([i + (v & 1), j + ((v >> 1) & 1), k + ((v >> 2) & 1)], dtype=float)
+        nbr[c] = [((i - 1) % n) + n * (j + n * k), ((i + 1) % n) + n * (j + n * k),
+                  i + n * (((j - 1) % n) + n * k), i + n * (((j + 1) % n) + n * k),
+                  i + n * (j + n * ((k - 1) % n)), i + n * (j + n * ((k + 1) % n))]
+    bid = np.zeros((nc, 6), dtype=np.int32)
+    np_ = N + 1
+    return SimpleNamespace(n_cells=nc, degree=N, n=np_ ** 3, nf=np_ ** 2, n_dof=nc * np_ ** 3, vertices=V,
+                           face_nbr=nbr, face_bid=bid, transpose_y_face_normals=False, mapping_degree=1)
+
+
+def sutherland(T, mu0, T0, S):
+    return mu0 * (T / T0) ** 1.5 * (T0 + S) / (T + S)
+
+
+def closed_form_dt(A, N, rho, vel, p, courant, mu0, T0=1.0, S=cases.SUTH_RATIO):
+    """Co / (N^1.5 (max_d (J^-T v)_d + a/h + N^2 mu / (h^2 rho))), Co >= 1 capped to 0.9, h the
+    minimum distance over the 28 vertex pairs of a cell (dgsem/plens.cc:3818-3862)"""
+    A = np.asarray(A, dtype=float)
+    corners = [A @ np.array([(v & 1), (v >> 1) & 1, (v >> 2) & 1], dtype=float) for v in range(8)]
+    h = min(np.linalg.norm(a - b) for a, b in itertools.combinations(corners, 2))
+    adv = np.linalg.inv(A).T @ np.asarray(vel, dtype=float)  # J = A on every cell of the lattice
+    a = math.sqrt(GAMMA * p / rho)
+    T = p / (rho * R)
+    mu = sutherland(T, mu0, T0, S) if mu0 > 0 else 0.0
+    co = 0.9 if courant >= 1 else courant
+    return co / (N ** 1.5 * (adv.max() + a / h + N * N * mu / (h * h * rho)))
+
+
+def oracle_dt(A, N, rho, vel, p, courant, mu0):
+    m = lattice_mesh(A, N)
+    o = PO.OraclePlens(mesh=m, physics=cases.make_physics(mu0=mu0), bcs=[cases.make_bc(L.BC_PERIODIC)])
+    E = p / (GAMMA - 1) + 0.5 * rho * float(np.dot(vel, vel))
+    Q = np.tile(np.array([rho, rho * vel[0], rho * vel[1], rho * vel[2], E])[:, None], (1, m.n_dof))
+    o.set_state(Q)
+    dt = o.calc_time_step(courant)
+    o.close()
+    return dt
+
+
+# Each case targets one part of the formula: a negative maximum of the contravariant velocity on an
+# anisotropic box (abs would pick 1.6 instead of -1.0), a sheared cell where J^-1 v and J^-T v differ,
+# the Courant cap (1.2 and exactly 1), and N = 3 / 4 (N^1.5).
+DT_CASES = {
+    "negative_max": (np.diag([0.5, 0.25, 0.125]), 3, 1.0, (-0.5, -0.3, -0.2), 1.0, 0.4, 1e-3),
+    "sheared": ([[0.5, 0.1, 0.0], [0.05, 0.3, 0.1], [0.0, -0.08, 0.2]], 4, 1.3, (0.4, -0.3, 0.25), 0.8, 0.4, 1e-2),
+    "courant_cap": ([[0.4, 0.0, 0.05], [0.0, 0.3, 0.0], [0.02, 0.0, 0.25]], 3, 0.9, (0.2, 0.1, -0.1), 1.1, 1.2, 5e-3),
+    "courant_one": (np.diag([0.3, 0.3, 0.3]), 2, 1.0, (0.1, 0.2, 0.3), 1.0, 1.0, 0.0),
+}
+
+
+@pytest.mark.parametrize("name", list(DT_CASES))
+def test_time_step_closed_form(name):
+    A, N, rho, vel, p, co, mu0 = DT_CASES[name]
+    want = closed_form_dt(A, N, rho, vel, p, co, mu0)
+    got = oracle_dt(A, N, rho, vel, p, co, mu0)
+    assert abs(got - want) <= 1e-13 * want, (got, want)
+
+
+@pytest.mark.parametrize("mutation,case", [(1, "negative_max"), (2, "sheared"), (3, "courant_cap"),
+                                           (3, "courant_one"), (4, "sheared")],
+                         ids=["abs_in_max", "N_squared", "no_cap", "no_cap_at_one", "Jinv_not_transposed"])
+def test_time_step_pin_catches_planted_mistake(mutation, case):
+    A, N, rho, vel, p, co, mu0 = DT_CASES[case]
+    want = closed_form_dt(A, N, rho, vel, p, co, mu0)
+    with PO.mutation(mutation):
+        got = oracle_dt(A, N, rho, vel, p, co, mu0)
+    assert abs(got - want) > 1e-3 * want, (got, want)
+
+
+# ---- ghost states of the uniform inflow, outflow and isothermal wall --------------------------
+def prim(u):
+    rho = u[0]
+    v = np.asarray(u[1:4]) / rho
+    p = (GAMMA - 1) * (u[4] - 0.5 * rho * float(v @ v))
+    return rho, v, p, p / (rho * R)
+
+
+def cons(rho, v, p):
+    v = np.asarray(v, dtype=float)
+    return np.array([rho, *(rho * v), p / (GAMMA - 1) + 0.5 * rho * float(v @ v)])
+
+
+BC_IN = dict(p=1.3, T=0.8, vel=(2.0, 0.3, -0.1))
+BC_OUT = dict(p=0.9)
+BC_WALL = dict(T=2.5, vel=(0.2, 0.0, 0.1))
+
+
+def ghost_oracle(mu0=1e-3):
+    """2x2x2 box, N=2: inflow on x-, outflow on x+ and y+, isothermal wall on y-, periodic in z"""
+    bcs = [cases.make_bc(L.BC_PERIODIC), cases.make_bc(L.BC_UNIFORM_INFLOW, **BC_IN),
+           cases.make_bc(L.BC_OUTFLOW, **BC_OUT), cases.make_bc(L.BC_UNIFORM_TEMP_WALL, **BC_WALL)]
+    m = cases.Mesh(2, 2, 2, 2, (0, 0, 0), (1, 1, 1), (0, 0, 1), [1, 2, 3, 2, -1, -1], (-1, -1, 0))
+    o = PO.OraclePlens(mesh=m, physics=cases.make_physics(mu0=mu0), bcs=bcs)
+    return o, m
+
+
+def random_inner(rng, mach):
+    rho = rng.uniform(0.5, 2.0)
+    p = rng.uniform(0.5, 2.0)
+    d = rng.normal(size=3)
+    v = mach * math.sqrt(GAMMA * p / rho) * d / np.linalg.norm(d)
+    return cons(rho, v, p), rng.normal(size=9)
+
+
+def unit(rng):
+    n = rng.normal(size=3)
+    return n / np.linalg.norm(n)
+
+
+def check_inflow(o, rng):
+    rho_pr = BC_IN["p"] / (R * BC_IN["T"])
+    for _ in range(20):
+        u, av = random_inner(rng, rng.uniform(0.1, 3.0))
+        n = unit(rng)
+        for stage in (1, 2, 3):
+            g, ag = o.bc_ghost(stage, 0, 0, 3, u, av, n)
+            face = g if stage == 2 else 0.5 * (u + g)  # stage 1/3: the BR1 face value
+            r, v, p, T = prim(face)
+            # the face state is the prescribed one (Dirichlet): p, T and velocity as given
+            assert abs(p - BC_IN["p"]) < 1e-12 and abs(T - BC_IN["T"]) < 1e-12 and abs(r - rho_pr) < 1e-12
+            assert np.abs(v - BC_IN["vel"]).max() < 1e-12
+            if stage == 3:
+                assert np.array_equal(ag, av)  # avars copied
+
+
+def check_outflow(o, rng, viscous):
+    for mach in (0.3, 0.8, 1.2, 2.5):
+        u, av = random_inner(rng, mach)
+        n = unit(rng)
+        for stage in (1, 2, 3):
+            g, ag = o.bc_ghost(stage, 1, 1, 2, u, av, n)
+            if stage == 2 and viscous and mach < 1:
+                # subsonic viscous: density and momentum kept, the BR1 average of the pressures is the
+                # prescribed one (Mengaldo et al. 2014, eqs. 40-41)
+                assert np.array_equal(g[:4], u[:4])
+                assert abs(0.5 * (prim(u)[2] + prim(g)[2]) - BC_OUT["p"]) < 1e-12
+            else:
+                assert np.array_equal(g, u)  # extrapolation: no jump
+            if stage == 3:
+                assert np.array_equal(ag, av)
+
+
+def check_wall(o, rng, viscous):
+    vw = np.array(BC_WALL["vel"])
+    for _ in range(20):
+        u, av = random_inner(rng, rng.uniform(0.1, 3.0))
+        n = unit(rng)
+        for stage in (1, 2, 3):
+            g, ag = o.bc_ghost(stage, 0, 2, 4, u, av, n)
+            if stage == 2:
+                assert g[0] == u[0] and g[4] == u[4]  # density and total energy kept
+                m, mg = u[1:4], g[1:4]
+                if viscous:  # no slip: the averaged momentum is that of the wall velocity
+                    assert np.abs(0.5 * (m + mg) - u[0] * vw).max() < 1e-12
+                else:  # slip: the relative normal momentum reversed, the tangential part kept
+                    assert abs((mg - u[0] * vw) @ n + (m - u[0] * vw) @ n) < 1e-12
+                    assert np.abs(np.cross(mg - m, n)).max() < 1e-12
+            else:
+                # the BR1 face value has the wall temperature and velocity and the inner pressure
+                r, v, p, T = prim(0.5 * (u + g))
+                assert abs(T - BC_WALL["T"]) < 1e-12 and abs(p - prim(u)[2]) < 1e-12
+                assert np.abs(v - vw).max() < 1e-12
+                if stage == 3:
+                    assert np.array_equal(ag, av)
+
+
+@pytest.mark.parametrize("viscous", [True, False], ids=["viscous", "inviscid"])
+def test_ghost_states_closed_form(viscous):
+    o, m = ghost_oracle(1e-3 if viscous else 0.0)
+    rng = np.random.default_rng(7)
+    check_inflow(o, rng)
+    check_outflow(o, rng, viscous)
+    check_wall(o, rng, viscous)
+    with pytest.raises(ValueError):  # an interior face has no ghost
+        o.bc_ghost(1, 0, 1, 0, np.ones(5), np.zeros(9), np.array([1.0, 0, 0]))
+    o.close()
+
+
+@pytest.mark.parametrize("mutation,check,viscous", [
+    (11, check_inflow, True), (12, check_outflow, True), (13, check_outflow, False),
+    (14, check_wall, True), (15, check_wall, True)],
+    ids=["inflow_stage1", "outflow_sign", "outflow_inviscid", "wall_density", "wall_absolute_velocity"])
+def test_ghost_pins_catch_planted_mistakes(mutation, check, viscous):
+    o, m = ghost_oracle(1e-3 if viscous else 0.0)
+    rng = np.random.default_rng(7)
+    args = (o, rng) if check is check_inflow else (o, rng, viscous)
+    with PO.mutation(mutation):
+        with pytest.raises(AssertionError):
+            check(*args)
+    o.close()
+
+
+def test_inviscid_inflow_reproduces_periodic():
+    """x-periodic box whose last cell in x holds the prescribed state u_pr: its x+ face nodes are the
+    periodic partners of the x- face of the first cell, so an inflow at x = 0 (stage-2 ghost u_pr)
+    gives the same inviscid fluxes there; the x+ face of the last cell is an outflow in one case and
+    periodic in the other, but the flow crosses it supersonically, so HLLC takes the inner state in
+    both and the RHS agrees in every cell"""
+    N, nx = 3, 4
+    bc_in = dict(p=1.0, T=1.0, vel=(1.6, 0.2, 0.0))  # supersonic: the outflow ghost is the inner state
+    ph = cases.make_physics(mu0=0.0, flux="HLLC", wall_blender_limit=0.0)
+    per = cases.Mesh(N, nx, 2, 2, (0, 0, 0), (1, 1, 1), (1, 1, 1), [-1] * 6, (0, 0, 0))
+    bnd = cases.Mesh(N, nx, 2, 2, (0, 0, 0), (1, 1, 1), (0, 1, 1), [1, 2, -1, -1, -1, -1], (-1, 0, 0))
+    X, Y = (per.node_coords[:, d, :].reshape(-1) for d in (0, 1))
+    last = np.repeat((np.arange(per.n_cells) % nx) == nx - 1, per.n)
+    rho = np.where(last, 1.0, 1.0 + 0.2 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y))
+    vx = np.where(last, 1.6, 1.6 + 0.1 * np.cos(2 * np.pi * Y))
+    Q = np.vstack([rho, rho * vx, rho * 0.2, 0 * X, 1.0 / (GAMMA - 1) + 0.5 * rho * (vx ** 2 + 0.04)])
+    assert np.allclose(Q[:, last], cons(1.0, bc_in["vel"], 1.0)[:, None])
+    bcs_b = [cases.make_bc(L.BC_PERIODIC), cases.make_bc(L.BC_UNIFORM_INFLOW, **bc_in), cases.make_bc(L.BC_OUTFLOW, p=1.0)]
+    rp, _, _ = PO.OraclePlens(cases.Case("per", per, ph, [cases.make_bc(L.BC_PERIODIC)], Q, 0.4, 1)).calc_rhs()
+    rb, _, _ = PO.OraclePlens(cases.Case("bnd", bnd, ph, bcs_b, Q, 0.4, 1)).calc_rhs()
+    assert np.abs(rp[:, ~last]).max() > 1e-2
+    assert np.abs(rb - rp).max() < 1e-12 * np.abs(rp).max()
+    # a different prescribed state changes the first cells
+    bcs_c = list(bcs_b)
+    bcs_c[1] = cases.make_bc(L.BC_UNIFORM_INFLOW, p=1.0, T=1.0, vel=(1.5, 0.2, 0.0))
+    rc, _, _ = PO.OraclePlens(cases.Case("bnd2", bnd, ph, bcs_c, Q, 0.4, 1)).calc_rhs()
+    assert np.abs(rc - rp).max() > 1e-2
+
+
+def isothermal_y_case(bc):
+    """x- and z-periodic box, y-independent flow without y velocity at the uniform temperature 2.5
+    (p = rho R T with rho = 1 + 0.1 sin 2 pi x) and velocity (0.2, 0, 0.1), viscous; `bc` on both y
+    faces, or y-periodic when None"""
+    per = bc is None
+    m = cases.Mesh(3, 3, 3, 3, (0, 0, 0), (1, 1, 1), (1, 1 if per else 0, 1),
+                   [-1, -1, -1 if per else 1, -1 if per else 1, -1, -1], (0, 0 if per else -1, 0))
+    X = m.node_coords[:, 0, :].reshape(-1)
+    rho = 1 + 0.1 * np.sin(2 * np.pi * X)
+    v = np.array(BC_WALL["vel"])
+    p = rho * R * BC_WALL["T"]
+    Q = np.vstack([rho, rho * v[0], 0 * X, rho * v[2], p / (GAMMA - 1) + 0.5 * rho * float(v @ v)])
+    ph = cases.make_physics(mu0=1e-3, flux="HLLC", wall_blender_limit=0.0)
+    bcs = [cases.make_bc(L.BC_PERIODIC)] + ([] if per else [bc])
+    return cases.Case("iso", m, ph, bcs, Q, 0.4, 1)
+
+
+@pytest.mark.parametrize("bc", [cases.make_bc(L.BC_UNIFORM_TEMP_WALL, **BC_WALL), cases.make_bc(L.BC_OUTFLOW, p=3.0)],
+                         ids=["isothermal_wall", "outflow"])
+def test_wall_and_outflow_reproduce_periodic(bc):
+    """the flow of isothermal_y_case at the wall's temperature and velocity: every wall ghost equals
+    the inner state; the outflow's prescribed pressure 3 is not the flow's (2.25-2.75), so its
+    subsonic stage-2 ghost differs and only its stage-1/3 ghosts (the inner state) match: the RHS
+    is compared for the inviscid part separately"""
+    r_p, a_p, _ = PO.OraclePlens(isothermal_y_case(None)).calc_rhs()
+    r_b, a_b, _ = PO.OraclePlens(isothermal_y_case(bc)).calc_rhs()
+    assert np.abs(r_p).max() > 1e-2
+    if bc.type == L.BC_UNIFORM_TEMP_WALL:
+        assert np.abs(r_b - r_p).max() < 1e-11 * np.abs(r_p).max()
+        assert np.array_equal(a_b, a_p)
+    else:  # the energy correction enters through the y-face nodes only
+        case = isothermal_y_case(bc)
+        Y = case.mesh.node_coords[:, 1, :].reshape(-1)
+        interior = (np.abs(Y) > 1e-12) & (np.abs(Y - 1) > 1e-12)
+        assert np.abs(r_b - r_p)[:, ~interior].max() > 1e-3
+        assert np.abs(r_b - r_p)[:, interior].max() < 1e-11 * np.abs(r_p).max()
+
+
+# ---- blender post-steps -----------------------------------------------------------------------
+def blender_oracle(wall_limit):
+    """3x3x1 cells: x and z periodic, isothermal wall at y = 0, outflow at y = 1; cell (i, j) = i + 3j"""
+    m = cases.Mesh(2, 3, 3, 1, (0, 0, 0), (1, 1, 0.3), (1, 0, 1), [-1, -1, 1, 2, -1, -1], (0, -1, 0))
+    bcs = [cases.make_bc(L.BC_PERIODIC), cases.make_bc(L.BC_UNIFORM_TEMP_WALL, T=1.0),
+           cases.make_bc(L.BC_OUTFLOW, p=1.0)]
+    o = PO.OraclePlens(mesh=m, physics=cases.make_physics(mu0=1e-3, wall_blender_limit=wall_limit), bcs=bcs)
+    o.set_state(np.tile(cons(1.0, (0.1, 0.0, 0.0), 1.0)[:, None], (1, m.n_dof)))
+    return o
+
+
+def cell(i, j):
+    return i + 3 * j
+
+
+# Indicator values (before the wall override) and the values after override, diffusion and alpha_d,
+# evaluated by hand: wall cells (j = 0) take max(limit, raw); every cell then takes the maximum of its
+# own value and half of each interior neighbour's overridden, undiffused value; the x faces of i = 0
+# and i = 2 and all z faces are periodic and, like the wall and outflow faces, are not diffused over.
+RAW = {cell(0, 2): 0.40, cell(1, 0): 0.10, cell(2, 0): 0.30}
+ALPHA_LIMIT_02 = {cell(0, 0): 0.2, cell(1, 0): 0.2, cell(2, 0): 0.3,
+                  cell(0, 1): 0.2, cell(1, 1): 0.1, cell(2, 1): 0.15,
+                  cell(0, 2): 0.4, cell(1, 2): 0.2, cell(2, 2): 0.0}
+# with the wall limit 0.6 (> alpha_max = 0.5) the wall cells take 0.6 and alpha_d saturates at 1
+ALPHA_LIMIT_06 = {cell(0, 0): 0.6, cell(1, 0): 0.6, cell(2, 0): 0.6,
+                  cell(0, 1): 0.3, cell(1, 1): 0.3, cell(2, 1): 0.3,
+                  cell(0, 2): 0.4, cell(1, 2): 0.2, cell(2, 2): 0.0}
+
+
+def blender_post_steps(wall_limit):
+    o = blender_oracle(wall_limit)
+    raw = np.zeros(9)
+    for c, v in RAW.items():
+        raw[c] = v
+    o.set_raw_alpha(raw)
+    _, a, ad = o.calc_rhs()
+    o.close()
+    return a, ad
+
+
+def check_blender(wall_limit):
+    want = ALPHA_LIMIT_02 if wall_limit == 0.2 else ALPHA_LIMIT_06
+    a, ad = blender_post_steps(wall_limit)
+    for c in range(9):
+        assert abs(a[c] - want[c]) < 1e-15, (c, a[c], want[c])
+        assert abs(ad[c] - min(1.0, want[c] / 0.5)) < 1e-15, (c, ad[c])
+
+
+@pytest.mark.parametrize("wall_limit", [0.2, 0.6])
+def test_blender_post_steps_hand_pattern(wall_limit):
+    check_blender(wall_limit)
+
+
+def test_blender_hook_clears():
+    """without the given indicator values the uniform state's own indicator (0) is used again"""
+    o = blender_oracle(0.0)
+    o.set_raw_alpha(np.full(9, 0.3))
+    _, a, _ = o.calc_rhs()
+    assert a.min() == 0.3
+    o.set_raw_alpha(None)
+    _, a, _ = o.calc_rhs()
+    assert a.max() == 0.0
+    o.close()
+
+
+@pytest.mark.parametrize("mutation,wall_limit", [(21, 0.2), (22, 0.2), (23, 0.2), (24, 0.6)],
+                         ids=["diffused_neighbours", "periodic_faces", "override_after_diffusion", "alpha_d_uncapped"])
+def test_blender_pin_catches_planted_mistakes(mutation, wall_limit):
+    with PO.mutation(mutation):
+        with pytest.raises(AssertionError):
+            check_blender(wall_limit)
