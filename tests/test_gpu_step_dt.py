@@ -20,9 +20,10 @@ def test_given_step_equals_computed_step_bitwise():
     for _ in range(3):
         dt = a.calc_time_step(case.courant)
         a.update_with_step(dt, t)
-        assert b.update(case.courant, t) == dt
+        tb = b.update(case.courant, t)  # (t + dt) - t: dt up to the rounding of the sum
+        assert abs(tb - dt) <= 4e-16 * (t + dt)
+        assert np.array_equal(a.get_state(), b.get_state())
         t += dt
-    assert np.array_equal(a.get_state(), b.get_state())
 
 
 def test_given_step_scales_the_update_and_rejects_bad_steps():
