@@ -211,6 +211,15 @@ int plens_gpu_set_local_stepping(plens_gpu* h, int32_t request, double start_tim
  * before the first step. */
 int plens_gpu_ss_error(plens_gpu* h, double* err);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): plens_gpu_ss_error with its
+ * per-cell partial integrals in a fixed device array of the library (one per GPU, 2 x 2^18 + 2
+ * doubles = 4 MiB, held until the process ends) instead of the handle's buffer; the call itself
+ * allocates nothing. At most 2^18 cells (PLENS_E_INVALID beyond); the handle's device must be
+ * current. Every call on one GPU uses that GPU's one array, so calls that overlap on a GPU (from
+ * several threads or streams) race; the result equals plens_gpu_ss_error's bit for bit. */
+int plens_gpu_ss_error_noalloc(plens_gpu* h, double* err);
+
 /* Data output of PLENS::write() (dgsem/plens.cc:4170-4322), the fields only (files: plens_host.h).
  * write() runs inside the first RK stage (update_rk4, 4341-4350), after the stage-1 auxiliary
  * variables, blender and calc_time_step of the current state; this call computes those for the

@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -2787,6 +2788,70 @@ int plens_gpu_ss_error(plens_gpu* h, double* err) {
     HIPCHK(hipGetLastError());
     double* out = h->ss_part + 2 * (size_t)h->nc;
     hipLaunchKernelGGL(k_ss_reduce, dim3(1), dim3(256), 0, h->stream, h->ss_part, h->nc, out);
+    HIPCHK(hipGetLastError());
+    double nd[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(nd, out, sizeof(nd), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *err = std::sqrt(nd[0]) / std::sqrt(nd[1]);
+    return PLENS_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// Device scratch of plens_gpu_ss_error_noalloc: the per-cell integrals (2 per cell) and the two sums.
+// One copy per GPU, held from module load to process end; every call on a GPU shares it.
+constexpr int kNoallocMaxCells = 1 << 18;
+constexpr size_t kNoallocScratch = 2 * (size_t)kNoallocMaxCells + 2;
+static __device__ __align__(256) double scratch[kNoallocScratch];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch` on each GPU (set-up once)
+
+// address of this GPU's copy of `scratch`, looked up once per device with that device current
+int scratch_on_current_device(double** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return PLENS_E_INVALID;
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return PLENS_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int plens_gpu_ss_error_noalloc(plens_gpu* h, double* err) {
+    if (!h || !err) return PLENS_E_INVALID;
+    if (h->steps_taken < 1 || h->nc > kNoallocMaxCells) return PLENS_E_INVALID;
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev != h->device) return PLENS_E_INVALID;  // the handle's device must be current
+    double* part = nullptr;
+    const int e = scratch_on_current_device(&part);
+    if (e) return e;
+    // the same kernels as plens_gpu_ss_error on the device scratch: k_ss writes every cell's two
+    // integrals and k_ss_reduce both sums, so nothing of an earlier call is read
+    KP kp = h->base;
+    kp.Q = h->state_in_b ? h->Qb : h->Qa;
+    const int thr = threads_for(kp.cpb, h->NN);
+    const int blocks = (kp.nc + kp.cpb - 1) / kp.cpb;
+    switch (h->N) {
+        case 1: hipLaunchKernelGGL(k_ss<1>, dim3(blocks), dim3(thr), 0, h->stream, kp, part); break;
+        case 2: hipLaunchKernelGGL(k_ss<2>, dim3(blocks), dim3(thr), 0, h->stream, kp, part); break;
+        case 3: hipLaunchKernelGGL(k_ss<3>, dim3(blocks), dim3(thr), 0, h->stream, kp, part); break;
+        case 4: hipLaunchKernelGGL(k_ss<4>, dim3(blocks), dim3(thr), 0, h->stream, kp, part); break;
+        default: hipLaunchKernelGGL(k_ss<5>, dim3(blocks), dim3(thr), 0, h->stream, kp, part); break;
+    }
+    HIPCHK(hipGetLastError());
+    double* out = part + 2 * (size_t)h->nc;
+    hipLaunchKernelGGL(k_ss_reduce, dim3(1), dim3(256), 0, h->stream, part, h->nc, out);
     HIPCHK(hipGetLastError());
     double nd[2] = {0, 0};
     HIPCHK(hipMemcpyAsync(nd, out, sizeof(nd), hipMemcpyDeviceToHost, h->stream));
