@@ -1284,18 +1284,8 @@ __global__ void __launch_bounds__(256) k_lo(KP p) {
 }
 
 
-// Chandrashekhar two-point flux along an unnormalised direction a from per-node precomputed
-// (rho, beta = rho/2p, ln rho, ln beta, v, |v|^2): equal to |a| * vol_flux(u_i, u_j, a/|a|) with the
-// logarithms of the log means taken per node (navier_stokes.cc:1510-1542, navier_stokes.h:494-501)
-__device__ __forceinline__ double ln_mean(double x, double y, double lx, double ly) {
-    const double s = x + y, dl = x - y;
-    const double f = dl / s, u = f * f;
-    if (u < 1e-2) return 0.5 * s / (1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0))));
-    return dl / (lx - ly);
-}
-
-// Pair fluxes of the volume term from per-node data in LDS (sR: 2 FP64 slots, sA: 6 flux-type
-// slots, B entries each). Logarithmic means share reciprocals: f = (x-y)/(x+y); below f^2 = 1e-2
+// Logarithmic means of the pair fluxes of the volume term, from per-node values and their
+// per-node logarithms; they share reciprocals: f = (x-y)/(x+y); below f^2 = 1e-2
 // the series of log_avg (navier_stokes.h:494-501: sums, differences, branch and series in FP64 in
 // every variant), else (x-y)/(ln x - ln y) with the per-node logarithms.
 template <typename TF>
@@ -1320,16 +1310,33 @@ __device__ __forceinline__ LogMean<TF> log_mean(double xa, double xb, TF lxa, TF
     }
     return m;
 }
-// Chandrashekhar (chandrashekhar_vol_flux, navier_stokes.cc:1510-1542). sR: rho, beta = rho/(2p);
-// sA: ln rho, ln beta, v0, v1, v2
+// Per-node data of the pair fluxes: two FP64 values (r0, r1) and five flux-type values (a[0..4]);
+// the meaning depends on the volume flux (below). A node's own record stays in registers, the
+// partner's is read from LDS (sR: 2 FP64 slots, sA: 5 flux-type slots, B entries each).
 template <typename TF>
-__device__ __forceinline__ void pair_chandrashekhar(const double* sR, const TF* sA, int B, int ja, int jb,
-                                                    const TF dir[3], TF half_gm1_inv, TF f[5]) {
-    const LogMean<TF> mr = log_mean<TF>(sR[ja], sR[jb], sA[ja], sA[jb]);
-    const LogMean<TF> mb = log_mean<TF>(sR[B + ja], sR[B + jb], sA[B + ja], sA[B + jb]);
-    const TF p_hat = (TF)(0.5 * (sR[ja] + sR[jb]) * mb.isum);
-    const TF a0 = sA[2 * B + ja], a1 = sA[3 * B + ja], a2 = sA[4 * B + ja];
-    const TF b0 = sA[2 * B + jb], b1 = sA[3 * B + jb], b2 = sA[4 * B + jb];
+struct PairNode {
+    double r0, r1;
+    TF a[5];
+};
+template <typename TF>
+__device__ __forceinline__ PairNode<TF> pair_node(const double* sR, const TF* sA, int B, int j) {
+    PairNode<TF> n;
+    n.r0 = sR[j];
+    n.r1 = sR[B + j];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) n.a[v] = sA[v * B + j];
+    return n;
+}
+// Chandrashekhar (chandrashekhar_vol_flux, navier_stokes.cc:1510-1542): r0 = rho, r1 = beta = rho/(2p);
+// a = ln rho, ln beta, v0, v1, v2
+template <typename TF>
+__device__ __forceinline__ void pair_chandrashekhar(const PairNode<TF>& A, const PairNode<TF>& Bn, const TF dir[3],
+                                                    TF half_gm1_inv, TF f[5]) {
+    const LogMean<TF> mr = log_mean<TF>(A.r0, Bn.r0, A.a[0], Bn.a[0]);
+    const LogMean<TF> mb = log_mean<TF>(A.r1, Bn.r1, A.a[1], Bn.a[1]);
+    const TF p_hat = (TF)(0.5 * (A.r0 + Bn.r0) * mb.isum);
+    const TF a0 = A.a[2], a1 = A.a[3], a2 = A.a[4];
+    const TF b0 = Bn.a[2], b1 = Bn.a[3], b2 = Bn.a[4];
     const TF va0 = TF(0.5) * (a0 + b0), va1 = TF(0.5) * (a1 + b1), va2 = TF(0.5) * (a2 + b2);
     const TF H = half_gm1_inv * mb.inv + p_hat * mr.inv + (va0 * va0 + va1 * va1 + va2 * va2) -
                  TF(0.25) * ((a0 * a0 + a1 * a1 + a2 * a2) + (b0 * b0 + b1 * b1 + b2 * b2));
@@ -1340,17 +1347,17 @@ __device__ __forceinline__ void pair_chandrashekhar(const double* sR, const TF* 
     f[3] = rv * va2 + p_hat * dir[2];
     f[4] = rv * H;
 }
-// Kennedy-Gruber (kennedy_gruber_vol_flux, navier_stokes.cc:1549-1575). sR: rho, p; sA: e, -, v0,
-// v1, v2
+// Kennedy-Gruber (kennedy_gruber_vol_flux, navier_stokes.cc:1549-1575): r0 = rho, r1 = p; a = e, -,
+// v0, v1, v2
 template <typename TF>
-__device__ __forceinline__ void pair_kennedy_gruber(const double* sR, const TF* sA, int B, int ja, int jb,
-                                                    const TF dir[3], TF f[5]) {
-    const TF ra = (TF)(0.5 * (sR[ja] + sR[jb]));
-    const TF pa = (TF)(0.5 * (sR[B + ja] + sR[B + jb]));
-    const TF a0 = sA[2 * B + ja], a1 = sA[3 * B + ja], a2 = sA[4 * B + ja];
-    const TF b0 = sA[2 * B + jb], b1 = sA[3 * B + jb], b2 = sA[4 * B + jb];
+__device__ __forceinline__ void pair_kennedy_gruber(const PairNode<TF>& A, const PairNode<TF>& Bn, const TF dir[3],
+                                                    TF f[5]) {
+    const TF ra = (TF)(0.5 * (A.r0 + Bn.r0));
+    const TF pa = (TF)(0.5 * (A.r1 + Bn.r1));
+    const TF a0 = A.a[2], a1 = A.a[3], a2 = A.a[4];
+    const TF b0 = Bn.a[2], b1 = Bn.a[3], b2 = Bn.a[4];
     const TF va0 = TF(0.5) * (a0 + b0), va1 = TF(0.5) * (a1 + b1), va2 = TF(0.5) * (a2 + b2);
-    const TF Ea = TF(0.5) * (sA[ja] + sA[jb] + TF(0.5) * ((a0 * a0 + a1 * a1 + a2 * a2) + (b0 * b0 + b1 * b1 + b2 * b2)));
+    const TF Ea = TF(0.5) * (A.a[0] + Bn.a[0] + TF(0.5) * ((a0 * a0 + a1 * a1 + a2 * a2) + (b0 * b0 + b1 * b1 + b2 * b2)));
     const TF rv = ra * (va0 * dir[0] + va1 * dir[1] + va2 * dir[2]);
     f[0] = rv;
     f[1] = rv * va0 + pa * dir[0];
@@ -1359,20 +1366,20 @@ __device__ __forceinline__ void pair_kennedy_gruber(const double* sR, const TF* 
     f[4] = rv * Ea + pa * (va0 * dir[0] + va1 * dir[1] + va2 * dir[2]);
 }
 // Ismail-Roe (ismail_roe_vol_flux, navier_stokes.cc:1590-1622) through z1 = sqrt(rho/p),
-// z5 = sqrt(rho p). sR: z5, z1; sA: ln z5, ln z1, v0, v1, v2
+// z5 = sqrt(rho p): r0 = z5, r1 = z1; a = ln z5, ln z1, v0, v1, v2
 template <typename TF>
-__device__ __forceinline__ void pair_ismail_roe(const double* sR, const TF* sA, int B, int ja, int jb,
-                                                const TF dir[3], TF gma, TF f[5]) {
-    const LogMean<TF> m5 = log_mean<TF>(sR[ja], sR[jb], sA[ja], sA[jb]);
-    const LogMean<TF> m1 = log_mean<TF>(sR[B + ja], sR[B + jb], sA[B + ja], sA[B + jb]);
-    const TF za = (TF)sR[B + ja], zb = (TF)sR[B + jb];
+__device__ __forceinline__ void pair_ismail_roe(const PairNode<TF>& A, const PairNode<TF>& Bn, const TF dir[3], TF gma,
+                                                TF f[5]) {
+    const LogMean<TF> m5 = log_mean<TF>(A.r0, Bn.r0, A.a[0], Bn.a[0]);
+    const LogMean<TF> m1 = log_mean<TF>(A.r1, Bn.r1, A.a[1], Bn.a[1]);
+    const TF za = (TF)A.r1, zb = (TF)Bn.r1;
     const TF iz = (TF)m1.isum;                                  // 1/(z1a + z1b)
-    const TF rh = (TF)(0.5 * (sR[B + ja] + sR[B + jb])) * m5.xln;
-    const TF p1 = (TF)((sR[ja] + sR[jb]) * m1.isum);
+    const TF rh = (TF)(0.5 * (A.r1 + Bn.r1)) * m5.xln;
+    const TF p1 = (TF)((A.r0 + Bn.r0) * m1.isum);
     const TF p2 = dv(TF(0.5) * ((gma + TF(1)) * m5.xln * m1.inv + (gma - TF(1)) * p1), gma);
-    const TF vh0 = (za * sA[2 * B + ja] + zb * sA[2 * B + jb]) * iz;
-    const TF vh1 = (za * sA[3 * B + ja] + zb * sA[3 * B + jb]) * iz;
-    const TF vh2 = (za * sA[4 * B + ja] + zb * sA[4 * B + jb]) * iz;
+    const TF vh0 = (za * A.a[2] + zb * Bn.a[2]) * iz;
+    const TF vh1 = (za * A.a[3] + zb * Bn.a[3]) * iz;
+    const TF vh2 = (za * A.a[4] + zb * Bn.a[4]) * iz;
     const TF vn = vh0 * dir[0] + vh1 * dir[1] + vh2 * dir[2];
     const TF rv = rh * vn;
     f[0] = rv;
@@ -1385,30 +1392,27 @@ __device__ __forceinline__ void pair_ismail_roe(const double* sR, const TF* sA, 
 
 // =============================================================================================
 // k_vol: high-order inviscid + viscous residuals, blend, RK epilogue
+//
+// Inviscid volume term: every unordered node pair of a line once, with the cyclic pair mapping:
+// on a line of NP nodes, the node at position a evaluates the pairs (a, a+k mod NP) for
+// k = 1..NP/2 (for even NP the pairs k = NP/2 only at a < NP/2, since a and a+NP/2 generate the
+// same pair), which covers every unordered pair exactly once. The node adds 2 D_am F to its own
+// residual in registers and leaves F in LDS, where the partner picks it up with its own D entry
+// (F#(u_a, u_m) = F#(u_m, u_a)). Own pair data stay in registers; no pair index is decoded.
 // =============================================================================================
 template <int N, typename TF, bool EXT>
 __global__ void __launch_bounds__(256) k_vol(KP p) {
     constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
-    constexpr int NPAIR = NP * (NP - 1) / 2;       // unordered pairs on one line
-    constexpr int NITEM = NP * NP * NPAIR;          // pairs of one direction in one cell
+    constexpr int KP = NP / 2;                     // pair slots per node and direction
+    constexpr bool EVEN = (NP % 2) == 0;
     extern __shared__ double smem[];
     __shared__ double sD[NP * NP], sW[NP], sQm[NP * NP], sC[NP * NP];
-    __shared__ int sPair[NP * NP];                  // pair index of (a,b), a<b, on a line
-    __shared__ int sPA[NPAIR > 0 ? NPAIR : 1], sPB[NPAIR > 0 ? NPAIR : 1];
     const int B = p.cpb * NN;
-    const int BP = p.cpb * NITEM;
     double* sR = smem;                               // 2B: per-node FP64 data of the pair flux
     TF* sA = reinterpret_cast<TF*>(sR + 2 * B);      // 5B: per-node flux-type data of the pair flux
     TF* sJa = sA + 5 * B;                            // 3B: Ja^md of the direction being processed
-    TF* sF = sJa + 3 * B;                            // 5*max(BP,B) pair fluxes / viscous scratch
+    TF* sF = sJa + 3 * B;                            // 5 KP B pair fluxes / 5B viscous scratch
     load_basis<N>(p.basis, sD, sW, sQm, sC);
-    for (int t = threadIdx.x; t < NP * NP; t += blockDim.x) {
-        const int a = t / NP, b = t % NP;
-        const int lo = a < b ? a : b, hi = a < b ? b : a;
-        const int k = (lo == hi) ? -1 : lo * NP - lo * (lo + 1) / 2 + (hi - lo - 1);
-        sPair[t] = k;
-        if (a < b) { sPA[k] = a; sPB[k] = b; }
-    }
 
     const int tid = threadIdx.x, lc = tid / NN, node = tid - lc * NN;
     const int c = blockIdx.x * p.cpb + lc;
@@ -1426,29 +1430,36 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
         node_metrics<N>(p, c, node, jm, detJ);
     }
     const int vs = EXT ? p.vscheme : PLENS_VOL_FLUX_CHANDRASHEKHAR;
-    if (inblk) {
+    PairNode<TF> own;
+    {
+        const double irho = rcp64(u[0]);
         const double pr = get_p(u, gs.gma);
-        const double v0 = u[1] / u[0], v1 = u[2] / u[0], v2 = u[3] / u[0];
         if (vs == PLENS_VOL_FLUX_CHANDRASHEKHAR) {
-            const double beta = 0.5 * u[0] / pr;
-            sR[0 * B + li] = u[0];
-            sR[1 * B + li] = beta;
-            sA[0 * B + li] = lg((TF)u[0]);
-            sA[1 * B + li] = lg((TF)beta);
+            const double beta = 0.5 * u[0] * rcp64(pr);
+            own.r0 = u[0];
+            own.r1 = beta;
+            own.a[0] = lg((TF)u[0]);
+            own.a[1] = lg((TF)beta);
         } else if (vs == PLENS_VOL_FLUX_KG) {
-            sR[0 * B + li] = u[0];
-            sR[1 * B + li] = pr;
-            sA[0 * B + li] = (TF)get_e(u);
+            own.r0 = u[0];
+            own.r1 = pr;
+            own.a[0] = (TF)get_e(u);
+            own.a[1] = TF(0);
         } else {
-            const double z5 = sqrt(u[0] * pr), z1 = sqrt(u[0] / pr);
-            sR[0 * B + li] = z5;
-            sR[1 * B + li] = z1;
-            sA[0 * B + li] = lg((TF)z5);
-            sA[1 * B + li] = lg((TF)z1);
+            const double z5 = sqrt(u[0] * pr), z1 = sqrt(u[0] * rcp64(pr));
+            own.r0 = z5;
+            own.r1 = z1;
+            own.a[0] = lg((TF)z5);
+            own.a[1] = lg((TF)z1);
         }
-        sA[2 * B + li] = (TF)v0;
-        sA[3 * B + li] = (TF)v1;
-        sA[4 * B + li] = (TF)v2;
+        own.a[2] = (TF)(u[1] * irho);
+        own.a[3] = (TF)(u[2] * irho);
+        own.a[4] = (TF)(u[3] * irho);
+    }
+    if (inblk) {
+        sR[li] = own.r0;
+        sR[B + li] = own.r1;
+        for (int v = 0; v < 5; ++v) sA[v * B + li] = own.a[v];
     }
     TF ja_own[9], ut[5];
     for (int a = 0; a < 9; ++a) ja_own[a] = (TF)jm[a];
@@ -1458,51 +1469,52 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
     const TF half_gm1_inv = (TF)(0.5 / (gs.gma - 1.0));
     const TF gma = (TF)gs.gma;
 
-    // ---- high-order inviscid volume term, each unordered pair once (calc_cell_ho_inv_residual,
-    //      dgsem/plens.cc:3202-3234): r_i += 2 D_ij F#(u_i, u_j; (Ja_i+Ja_j)/2), F#_ji = F#_ij
+    // ---- high-order inviscid volume term (calc_cell_ho_inv_residual, dgsem/plens.cc:3202-3234):
+    //      r_i += 2 D_ij F#(u_i, u_j; (Ja_i+Ja_j)/2) over the line through i in each direction
     double ri[5] = {0, 0, 0, 0, 0};
 #pragma unroll
     for (int md = 0; md < 3; ++md) {
-        __syncthreads();  // previous direction done with sJa / sF
+        const int a = tt[md];
+        const int st = md == 0 ? 1 : (md == 1 ? NP : NF);  // node stride along the line
         if (inblk)
             for (int q = 0; q < 3; ++q) sJa[q * B + li] = ja_own[3 * md + q];
+        // all reads of the previous direction's pair fluxes (and of sJa) are done: the barrier
+        // separates them from this direction's writes
         __syncthreads();
-        for (int it = tid; it < BP; it += blockDim.x) {
-            const int cl = it / NITEM, rem = it - cl * NITEM;
-            const int line = rem / NPAIR, pr = rem - line * NPAIR;
-            const int a = sPA[pr], b = sPB[pr];
-            const int l1 = line % NP, l2 = line / NP;   // indices of the other two directions
-            int ia[3], ib[3];
-            ia[md] = a; ib[md] = b;
-            ia[(md + 1) % 3] = l1; ib[(md + 1) % 3] = l1;
-            ia[(md + 2) % 3] = l2; ib[(md + 2) % 3] = l2;
-            const int ja = cl * NN + ia[0] + NP * (ia[1] + NP * ia[2]);
-            const int jb = cl * NN + ib[0] + NP * (ib[1] + NP * ib[2]);
-            TF dir[3];
-            for (int q = 0; q < 3; ++q) dir[q] = TF(0.5) * (sJa[q * B + ja] + sJa[q * B + jb]);
-            TF fp[5];
-            if (vs == PLENS_VOL_FLUX_KG) pair_kennedy_gruber(sR, sA, B, ja, jb, dir, fp);
-            else if (vs == PLENS_VOL_FLUX_IR) pair_ismail_roe(sR, sA, B, ja, jb, dir, gma, fp);
-            else pair_chandrashekhar(sR, sA, B, ja, jb, dir, half_gm1_inv, fp);
-            for (int v = 0; v < 5; ++v) sF[v * BP + it] = fp[v];
+        if (inblk) {
+            TF dj[3], fi[5];  // consistent diagonal term: F#(u,u; Ja) = f(u) . Ja
+            for (int q = 0; q < 3; ++q) dj[q] = ja_own[3 * md + q];
+            inv_flux(ut, dj, gma, fi);
+            const double d2 = 2.0 * sD[a * NP + a];
+            for (int v = 0; v < 5; ++v) ri[v] += d2 * (double)fi[v];
+#pragma unroll 1
+            for (int k = 1; k <= KP; ++k) {
+                if (EVEN && k == NP / 2 && a >= NP / 2) continue;
+                const int m = a + k < NP ? a + k : a + k - NP;
+                const int j = li + (m - a) * st;
+                const PairNode<TF> oth = pair_node<TF>(sR, sA, B, j);
+                TF dir[3];
+                for (int q = 0; q < 3; ++q) dir[q] = TF(0.5) * (ja_own[3 * md + q] + sJa[q * B + j]);
+                TF fp[5];
+                if (vs == PLENS_VOL_FLUX_KG) pair_kennedy_gruber(own, oth, dir, fp);
+                else if (vs == PLENS_VOL_FLUX_IR) pair_ismail_roe(own, oth, dir, gma, fp);
+                else pair_chandrashekhar(own, oth, dir, half_gm1_inv, fp);
+                const double d2m = 2.0 * sD[a * NP + m];
+                for (int v = 0; v < 5; ++v) {
+                    ri[v] += d2m * (double)fp[v];
+                    sF[(v * KP + (k - 1)) * B + li] = fp[v];
+                }
+            }
         }
         __syncthreads();
         if (inblk) {
-            const int me = tt[md];
-            const int l1 = tt[(md + 1) % 3], l2 = tt[(md + 2) % 3];
-            const int pbase = lc * NITEM + (l1 + NP * l2) * NPAIR;
-            {  // consistent diagonal term: F#(u,u; Ja) = f(u) . Ja
-                TF dj[3], fi[5];
-                for (int q = 0; q < 3; ++q) dj[q] = ja_own[3 * md + q];
-                inv_flux(ut, dj, gma, fi);
-                const double d2 = 2.0 * sD[me * NP + me];
-                for (int v = 0; v < 5; ++v) ri[v] += d2 * (double)fi[v];
-            }
-            for (int kk = 0; kk < N; ++kk) {  // the N other nodes of the line
-                const int m = kk < me ? kk : kk + 1;
-                const double d2 = 2.0 * sD[me * NP + m];
-                const int k = pbase + sPair[me * NP + m];
-                for (int v = 0; v < 5; ++v) ri[v] += d2 * (double)sF[v * BP + k];
+#pragma unroll
+            for (int k = 1; k <= KP; ++k) {  // pairs (a-k, a) evaluated by the node at a-k
+                if (EVEN && k == NP / 2 && a < NP / 2) continue;
+                const int m = a - k >= 0 ? a - k : a - k + NP;
+                const int j = li + (m - a) * st;
+                const double d2m = 2.0 * sD[a * NP + m];
+                for (int v = 0; v < 5; ++v) ri[v] += d2m * (double)sF[(v * KP + (k - 1)) * B + j];
             }
         }
     }
@@ -1525,12 +1537,11 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
             }
             __syncthreads();
             if (inblk) {
+                const int st = md == 0 ? 1 : (md == 1 ? NP : NF);
+                const int j0 = li - tt[md] * st;  // first node of the line
                 for (int m = 0; m <= N; ++m) {
-                    int to[3] = {ti, tj, tk};
-                    to[md] = m;
-                    const int jo = lc * NN + to[0] + NP * (to[1] + NP * to[2]);
                     const double Dv = sD[tt[md] * NP + m];
-                    for (int v = 1; v < 5; ++v) rd[v] += Dv * (double)sF[v * B + jo];
+                    for (int v = 1; v < 5; ++v) rd[v] += Dv * (double)sF[v * B + j0 + m * st];
                 }
             }
         }
@@ -1547,9 +1558,8 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
         for (int v = 0; v < 5; ++v) sf[v] += (double)sface[o + v * NF];
     }
     {
-        const double jm = 1.0 / (-1.0 * detJ);
+        const double jp = rcp64(detJ), jm = -jp;
         for (int v = 0; v < 5; ++v) ri[v] *= jm;
-        const double jp = 1.0 / (1.0 * detJ);
         for (int v = 0; v < 5; ++v) rd[v] *= jp;
     }
 
@@ -1862,9 +1872,9 @@ int launch_one_t(plens_gpu* h, const KP& kp, int kernel) {
                                    sizeof(TF) * 19 * B, h->stream, kp);
             break;
         case 4: {
-            const size_t nitem = (size_t)NP * NP * (NP * (NP - 1) / 2);
-            const size_t lds_vol =
-                sizeof(double) * 2 * B + sizeof(TF) * (8 * B + 5 * std::max((size_t)kp.cpb * nitem, B));
+            // FP64 pair-flux data (2B) + flux-type pair data (5B), direction metrics (3B) and the
+            // pair fluxes of NP/2 slots (5 NP/2 B, which also holds the viscous scratch)
+            const size_t lds_vol = sizeof(double) * 2 * B + sizeof(TF) * (8 * B + 5 * (size_t)(NP / 2) * B);
             if (ext)
                 hipLaunchKernelGGL((k_vol<N, TF, true>), dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
             else
