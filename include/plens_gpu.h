@@ -179,6 +179,12 @@ int plens_gpu_rk4_step_dt(plens_gpu* h, double t, double dt_global);
  * the device; positivity failures are reported at the end. */
 int plens_gpu_run(plens_gpu* h, int32_t n_steps, double courant, double* t_inout);
 
+/* Diagnostic with no counterpart in the reference: the number of (cell, stage) pairs with a blender
+ * value alpha > 0 (cells that took the subcell FV blend of calc_rhs, dgsem/plens.cc:3737-3765) over
+ * every RHS evaluation since the handle was created or last reset; reset != 0 clears it after
+ * reading. */
+int plens_gpu_blended_cell_stages(plens_gpu* h, int64_t* count, int32_t reset);
+
 /* Device time (HIP events on the handle's stream) of the graph replays of the last plens_gpu_run. */
 int plens_gpu_last_run_ms(const plens_gpu* h, double* ms);
 

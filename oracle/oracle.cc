@@ -895,6 +895,7 @@ struct oracle {
     std::vector<double> Q, avars, mu, kap, alpha, alpha_d, rhs, rhs1, rhs2, Qn, loc_dt;
     std::vector<double> alpha_given, alpha_d_given;  // checker hook: blender values to use, when set
     std::vector<double> raw_given;                    // checker hook: indicator values (pre-override)
+    int64_t blended_cell_stages = 0;                  // (cell, RHS evaluation) pairs with alpha > 0
     std::vector<double> s1, s2, s3;                     // surface terms [var][c][f][q]
     std::vector<double> ws1;                            // entropy variables of s1 (default build)
     std::vector<double> verts;                          // cell vertices [c][8][3]
@@ -1649,6 +1650,7 @@ struct oracle {
         } else {
             calc_blender();
         }
+        for (int c = 0; c < nc; ++c) blended_cell_stages += alpha[c] > 0.0;
         lap(2);
         calc_surf_flux(2, s2);
         if (!ns.inviscid()) calc_surf_flux(3, s3);
@@ -1962,6 +1964,12 @@ int oracle_set_alpha(oracle* o, const double* alpha, const double* alpha_d) {
 int oracle_set_raw_alpha(oracle* o, const double* raw) {
     if (!raw) o->raw_given.clear();
     else o->raw_given.assign(raw, raw + o->nc);
+    return 0;
+}
+
+int oracle_blended_cell_stages(const oracle* o, int64_t* count) {
+    if (!o || !count) return -1;
+    *count = o->blended_cell_stages;
     return 0;
 }
 

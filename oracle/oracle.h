@@ -73,6 +73,8 @@ int oracle_bc_ghost(const oracle* o, int32_t stage, int32_t cell, int32_t face, 
  * absolute momentum; 21 diffusion from diffused neighbours, 22 diffusion across periodic faces,
  * 23 wall override after diffusion, 24 alpha_d not capped at 1. */
 int oracle_set_mutation(int32_t id);
+/* number of (cell, RHS evaluation) pairs with a blender value alpha > 0 since creation */
+int oracle_blended_cell_stages(const oracle* o, int64_t* count);
 /* location of node `node` of cell c (through the cell's map) */
 int oracle_node_location(const oracle* o, int32_t c, int32_t node, double* xyz);
 /* value of one expression of the varying inflow BC; NaN on a syntax error */

@@ -51,6 +51,7 @@ def load():
         "oracle_set_alpha": (C.c_int, [vp, _dp, _dp]),
         "oracle_set_raw_alpha": (C.c_int, [vp, _dp]),
         "oracle_set_mutation": (C.c_int, [C.c_int32]),
+        "oracle_blended_cell_stages": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "oracle_bc_ghost": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp, _dp, _dp]),
         "oracle_node_location": (C.c_int, [vp, C.c_int32, C.c_int32, _dp]),
         "oracle_eval_expression": (C.c_double, [C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_double]),
@@ -239,6 +240,12 @@ class OraclePlens:
             return
         self._raw_given = np.ascontiguousarray(raw, dtype=np.float64)
         self.lib.oracle_set_raw_alpha(self._h, self._raw_given.ctypes.data_as(_dp))
+
+    def blended_cell_stages(self):
+        """(cell, RHS evaluation) pairs with alpha > 0 since creation"""
+        n = C.c_int64()
+        self.lib.oracle_blended_cell_stages(self._h, C.byref(n))
+        return n.value
 
     def bc_ghost(self, stage, cell, face, q, u, av, nrm):
         """ghost state and avars of boundary face `face` of `cell` at face node q and stage 1..3"""

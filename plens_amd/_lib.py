@@ -71,7 +71,7 @@ GPU_SYMBOLS = [
     "plens_gpu_create", "plens_gpu_destroy", "plens_gpu_set_state", "plens_gpu_get_state",
     "plens_gpu_calc_rhs", "plens_gpu_calc_dt", "plens_gpu_rk4_step", "plens_gpu_rk4_step_dt", "plens_gpu_run",
     "plens_gpu_last_failure", "plens_gpu_set_tuning", "plens_gpu_kernel_times", "plens_gpu_strerror",
-    "plens_gpu_last_run_ms", "plens_gpu_run_profiled", "plens_gpu_stream_copy", "plens_gpu_time_kernel",
+    "plens_gpu_last_run_ms", "plens_gpu_blended_cell_stages", "plens_gpu_run_profiled", "plens_gpu_stream_copy", "plens_gpu_time_kernel",
     "plens_gpu_set_local_stepping", "plens_gpu_ss_error", "plens_gpu_ss_error_noalloc", "plens_gpu_output_fields", "plens_gpu_write_checkpoint",
     "plens_gpu_read_checkpoint", "plens_gpu_set_time",
     "plens_gpu_batch_create", "plens_gpu_batch_destroy", "plens_gpu_batch_run", "plens_gpu_batch_get_state",
@@ -123,6 +123,7 @@ def load():
         "plens_gpu_kernel_times": (C.c_int, [vp, _dp, C.c_int32]),
         "plens_gpu_strerror": (C.c_char_p, [C.c_int]),
         "plens_gpu_last_run_ms": (C.c_int, [vp, _dp]),
+        "plens_gpu_blended_cell_stages": (C.c_int, [vp, C.POINTER(C.c_int64), C.c_int32]),
         "plens_gpu_stream_copy": (C.c_int, [C.c_int64, C.c_int32, _dp]),
         "plens_gpu_time_kernel": (C.c_int, [vp, C.c_int32, C.c_int32, _dp]),
         "plens_gpu_set_local_stepping": (C.c_int, [vp, C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_double]),

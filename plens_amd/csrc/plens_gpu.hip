@@ -104,6 +104,7 @@ struct KP {
     double* rlo;
     int* blist;
     int* bcount;
+    unsigned long long* bstat;  // blended (cell, stage) pairs since the last reset
     const float* mlp;        // BF16-rounded MLP parameters (nullptr: plain indicator)
     double mlp_tol;          // fallback threshold |alpha_mlp - alpha_plain|
     double mlp_floor;        // network values below this are taken as 0
@@ -1183,6 +1184,7 @@ __global__ void __launch_bounds__(256) k_lo(KP p) {
     extern __shared__ double smem[];
     __shared__ double sD[NP * NP], sW[NP], sQm[NP * NP], sC[NP * NP];
     const int count = p.bcount[0];  // blended cells of this stage, listed by k_alpha
+    if (blockIdx.x == 0 && threadIdx.x == 0) p.bstat[0] += (unsigned long long)count;  // one writer
     if ((int)blockIdx.x * p.cpb >= count) return;
     const int B = p.cpb * NN;
     TF* sQ = reinterpret_cast<TF*>(smem);  // 5B
@@ -1721,6 +1723,7 @@ struct plens_gpu {
     float *avars_f = nullptr, *sinv_f = nullptr, *sface_f = nullptr;  // FP32-bulk variant
     int precision = PLENS_PREC_FP64;
     int *blist = nullptr, *bcount = nullptr;
+    unsigned long long* bstat = nullptr;
     unsigned long long* mlp_stats = nullptr;
     float* feat = nullptr;
     int ninst = 1;                                // instances merged into this handle
@@ -2078,7 +2081,7 @@ int alloc_buffers(plens_gpu* h) {
         (rc = dmalloc(&h->dt_cell, nc)) || (rc = dmalloc(&h->cinst, nc)) ||
         (rc = dmalloc(&h->ioff, (size_t)h->ninst + 1)) || (rc = dmalloc(&h->visc, 3 * (size_t)h->ninst)) ||
         (rc = dmalloc(&h->scal, 2 * (size_t)h->ninst)) || (rc = dmalloc(&h->fail, 2)) ||
-        (rc = dmalloc(&h->blist, nc)) || (rc = dmalloc(&h->bcount, 1)) || (rc = dmalloc(&h->mlp_stats, mlp_stat_slots(h))) ||
+        (rc = dmalloc(&h->blist, nc)) || (rc = dmalloc(&h->bcount, 1)) || (rc = dmalloc(&h->bstat, 1)) || (rc = dmalloc(&h->mlp_stats, mlp_stat_slots(h))) ||
         (rc = dmalloc(&h->tabs, 2 * (size_t)h->ninst)) || (rc = dmalloc(&h->rhoE_old, nd)) ||
         (rc = dmalloc(&h->ss_part, 2 * nc + 2)))
         return rc;
@@ -2108,6 +2111,7 @@ int alloc_buffers(plens_gpu* h) {
         hipMemset(h->alpha, 0, sizeof(double) * nc) != hipSuccess ||
         hipMemset(h->scal, 0, sizeof(double) * 2 * h->ninst) != hipSuccess ||
         hipMemset(h->bcount, 0, sizeof(int)) != hipSuccess ||
+        hipMemset(h->bstat, 0, sizeof(unsigned long long)) != hipSuccess ||
         hipMemset(h->mlp_stats, 0, mlp_stat_slots(h) * sizeof(unsigned long long)) != hipSuccess)
         return PLENS_E_HIP;
     const unsigned long long init[2] = {~0ull, ~0ull};
@@ -2125,7 +2129,7 @@ void bind_buffers(plens_gpu* h) {
     k.avars = h->avars; k.araw = h->araw; k.alpha = h->alpha; k.alpha_d = h->alpha_d;
     k.sinv = h->sinv; k.sface = h->sface; k.avars_f = h->avars_f; k.sinv_f = h->sinv_f; k.sface_f = h->sface_f;
     k.rhs_out = h->rhs; k.rlo = h->rlo;
-    k.blist = h->blist; k.bcount = h->bcount;
+    k.blist = h->blist; k.bcount = h->bcount; k.bstat = h->bstat;
     k.mlp_stats = h->mlp_stats; k.feat = nullptr;
     k.dt_cell = h->dt_cell; k.cinst = h->cinst; k.ioff = h->ioff; k.visc = h->visc;
     k.dt = h->scal; k.t_acc = h->scal + h->ninst; k.fail = h->fail;
@@ -2558,7 +2562,7 @@ void plens_gpu_destroy(plens_gpu* h) {
         if (x) (void)hipEventDestroy(x);
     void* ptrs[] = {h->Qa, h->Qb, h->R[0], h->R[1], h->Ja, h->detJ, h->nrm, h->hmin, h->nbr,
                     h->bid, h->wall, h->tasks, h->tdesc, h->dtmin, h->dtcount, h->bcs, h->basis, h->avars, h->araw, h->alpha, h->alpha_d,
-                    h->sinv, h->sface, h->rhs, h->rlo, h->dt_cell, h->scal, h->fail, h->blist, h->bcount, h->mlp_stats, h->feat,
+                    h->sinv, h->sface, h->rhs, h->rlo, h->dt_cell, h->scal, h->fail, h->blist, h->bcount, h->bstat, h->mlp_stats, h->feat,
                     h->avars_f, h->sinv_f, h->sface_f, h->cinst, h->ioff, h->visc, h->cfl, h->cmap, h->fmet, h->fmet_f,
                     h->tabs, h->rhoE_old, h->dt_lts, h->ss_part, h->post, h->vslot, h->vin, h->vxyz,
                     h->vbc, h->vcode, h->corder};
@@ -3105,6 +3109,20 @@ int plens_gpu_indicator_samples(plens_gpu* h, float* feats, double* alpha_raw) {
     if (alpha_raw)
         HIPCHK(hipMemcpyAsync(alpha_raw, h->araw, sizeof(double) * h->nc, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    return PLENS_OK;
+}
+
+int plens_gpu_blended_cell_stages(plens_gpu* h, int64_t* count, int32_t reset) {
+    if (!h || !count) return PLENS_E_INVALID;
+    unsigned long long v = 0;
+    HIPCHK(hipMemcpyAsync(&v, h->bstat, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *count = (int64_t)v;
+    if (reset) {
+        static const unsigned long long zero = 0;
+        HIPCHK(hipMemcpyAsync(h->bstat, &zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
     return PLENS_OK;
 }
 

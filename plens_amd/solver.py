@@ -106,6 +106,12 @@ class GpuPlens:
         self._check(self.lib.plens_gpu_run(self._h, n_steps, courant, C.byref(t)), "run")
         return t.value - t0
 
+    def blended_cell_stages(self, reset=False):
+        """(cell, stage) pairs with alpha > 0 over the RHS evaluations since creation or the last reset"""
+        n = C.c_int64()
+        self._check(self.lib.plens_gpu_blended_cell_stages(self._h, C.byref(n), int(reset)), "blended_cell_stages")
+        return n.value
+
     def last_run_ms(self):
         ms = C.c_double()
         self._check(self.lib.plens_gpu_last_run_ms(self._h, C.byref(ms)), "last_run_ms")
