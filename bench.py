@@ -77,20 +77,35 @@ def measured_traffic(precision, kernel):
     return None, None
 
 
-def cpu_baseline(case, gpu_state_after_1, seconds_target=20.0):
-    """Time the CPU oracle on one RK4 step of the same workload (bounded sample)."""
+# section timers of the oracle, named as the reference's TimerOutput sections (dgsem/plens_test.cc:903-976)
+SECTIONS = ["Assert positivity", "Calculate aux vars", "Calculate blender", "Calculate surface fluxes",
+            "ho inv residual + ho dif residual", "lo inv residual", "Calculating time step", "Updating cvars"]
+
+
+def cpu_baseline(case, gpu_state_after_1):
+    """Time the CPU oracle on one RK4 step of the same workload (bounded sample): on all host threads
+    (the value), then on one core; section timers of the all-threads step."""
     from oracle.pyoracle import OraclePlens
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     o = OraclePlens(case, threads=threads)
     t0 = time.perf_counter()
     o.update(case.courant)
     wall = time.perf_counter() - t0
+    sections = {k: float(v) for k, v in zip(SECTIONS, o.section_times())}
     rate = case.mesh.n_dof * 5 / wall
     err = float(np.linalg.norm(gpu_state_after_1 - o.get_state()) / np.linalg.norm(o.get_state()))
     o.close()
+    o1 = OraclePlens(case, threads=1)
+    t0 = time.perf_counter()
+    o1.update(case.courant)
+    wall1 = time.perf_counter() - t0
+    o1.close()
+    OraclePlens.set_threads(threads)
     return {"value": rate, "unit": "DOF-stage-updates/s", "cores": threads, "kind": "port",
             "sample": f"1 RK4 step (5 stages) of the full workload, {case.mesh.n_dof} DoF, wall {wall:.2f} s, "
-                      f"OpenMP over cells"}, err
+                      f"OpenMP over cells",
+            "one_core": {"value": case.mesh.n_dof * 5 / wall1, "cores": 1, "wall_s": wall1},
+            "sections_s": sections}, err
 
 
 def fp64_sensitivity(case, args, plain_state):
@@ -141,6 +156,35 @@ def run_variants(case, args, plain_value, plain_state):
         print(f"[bench] {label}: {rate:.4e} DOF-stage/s ({rate / plain_value:.3f}x), rel L2 vs FP64 {err:.2e}",
               file=sys.stderr, flush=True)
         g.close()
+    return out
+
+
+def fp32_bulk_errors(case):
+    """FP32-bulk error where the plain path is not chaotic under rounding: the measured wall flow over
+    its first 5 steps (before the blender switches wall cells) against the plain FP64 GPU state, and
+    the periodic cube at full size (8^3, N=3) after its 100 steps against the FP64 CPU oracle"""
+    from oracle.pyoracle import OraclePlens
+    from plens_amd import GpuPlens, periodic_cube, with_precision
+    out = {}
+    a, b = GpuPlens(case), GpuPlens(with_precision(case, "fp32_bulk"))
+    a.run(5, case.courant)
+    b.run(5, case.courant)
+    qa = a.get_state()
+    out["wall_flow_5_steps_rel_l2_vs_fp64"] = float(np.linalg.norm(b.get_state() - qa) / np.linalg.norm(qa))
+    a.close()
+    b.close()
+    cube = periodic_cube()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    o = OraclePlens(cube, threads=threads)
+    o.run(100, cube.courant)
+    qo = o.get_state()
+    o.close()
+    for label, prec in (("fp64", "fp64"), ("fp32_bulk", "fp32_bulk")):
+        g = GpuPlens(with_precision(cube, prec))
+        g.run(100, cube.courant)
+        out[f"cube_8^3_100_steps_{label}_rel_l2_vs_oracle"] = float(np.linalg.norm(g.get_state() - qo) / np.linalg.norm(qo))
+        g.close()
+    print(f"[bench] FP32-bulk errors where rounding is not amplified: {out}", file=sys.stderr, flush=True)
     return out
 
 
@@ -253,6 +297,8 @@ def main():
 
     g.close()
     variants = None if args.no_variants else run_variants(case, args, value, final_plain)
+    if variants is not None:
+        variants["fp32_bulk_errors"] = fp32_bulk_errors(case)
     batched = None if args.no_batch else run_batched()
     cpu, parity = (None, None)
     if not args.no_cpu:

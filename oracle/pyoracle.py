@@ -154,6 +154,11 @@ class OraclePlens:
         if case is not None:
             self.set_state(case.Q0)
 
+    @staticmethod
+    def set_threads(n):
+        """OpenMP threads of the oracle (process-wide); returns the count in effect"""
+        return load().oracle_set_threads(n)
+
     def set_state(self, Q):
         Q = np.ascontiguousarray(Q, dtype=np.float64)
         self.lib.oracle_set_state(self._h, Q.ctypes.data_as(_dp))
