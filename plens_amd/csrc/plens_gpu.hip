@@ -235,6 +235,24 @@ __device__ __forceinline__ void node_metrics(const KP& p, int c, int node, doubl
     }
 }
 
+// Surface metric terms of face node q of face f of cell c from the cell's trilinear map: the unit
+// contravariant direction Ja^d/|Ja^d| (d = f/2), the surface weight |Ja^d|/w_end (w_end the GLL
+// weight of the face's end of the line) and detJ, as the face-node table of the face tasks holds them
+template <int N>
+__device__ __forceinline__ void face_node_metrics(const KP& p, int c, int f, int q, double dir[3], double& nw,
+                                                  double& J) {
+    double ja[9];
+    node_metrics<N>(p, c, face_node<N>(f, q), ja, J);
+    const int sd = f >> 1;
+    const double a0 = ja[3 * sd], a1 = ja[3 * sd + 1], a2 = ja[3 * sd + 2];
+    const double nr = sqrt(a0 * a0 + a1 * a1 + a2 * a2);
+    const double inr = rcp64(nr);
+    dir[0] = a0 * inr;
+    dir[1] = a1 * inr;
+    dir[2] = a2 * inr;
+    nw = nr * ((f & 1) ? p.iwN : p.iw0);
+}
+
 __device__ __forceinline__ float mlp_gelu(float x) {
     const float t = tanhf(0.7978845608028654f * (x + 0.044715f * x * x * x));
     return 0.5f * x * (1.0f + t);
@@ -395,12 +413,13 @@ __device__ __forceinline__ void load_basis(const Basis* b, double* sD, double* s
 // =============================================================================================
 // k_grad
 // =============================================================================================
-template <int N, typename TF>
+// ONE: one cell per workgroup (as k_vol): the cell index and its tables are uniform (scalar)
+template <int N, typename TF, bool ONE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_grad(KP p) {
     constexpr int NP = N + 1, NN = NP * NP * NP;
     extern __shared__ double smem[];
     __shared__ double sD[NP * NP], sW[NP], sQm[NP * NP], sC[NP * NP];
-    const int B = p.cpb * NN;
+    const int B = ONE ? NN : p.cpb * NN;  // a compile-time constant with one cell per workgroup
     double* sV = smem;
     double* sT = sV + B;   // Legendre modes of the blender variable
     double* sE = sT + B;   // 3B modal-energy partial sums
@@ -409,17 +428,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     int* sFn = reinterpret_cast<int*>(sJa + 9 * B);  // per cell: boundary ids, then neighbours of the 6 faces
     load_basis<N>(p.basis, sD, sW, sQm, sC);
 
-    const int tid = threadIdx.x, lc = tid / NN, node = tid - lc * NN;
+    const int tid = threadIdx.x, lc = ONE ? 0 : tid / NN, node = ONE ? tid : tid - lc * NN;
     // cells in the Morton order of their centroids when the handle has one, so that the neighbours
     // whose face states the surface terms read were read shortly before, on the same XCD
-    const int c0 = xcd_block(blockIdx.x, gridDim.x) * p.cpb;
+    const int c0 = xcd_block(blockIdx.x, gridDim.x) * (ONE ? 1 : p.cpb);
     const int c = (p.corder && c0 + lc < p.nc) ? p.corder[c0 + lc] : c0 + lc;
     for (int t = tid; t < 12 * p.cpb; t += blockDim.x) {
         const int cl = t / 12, r = t - 12 * cl;
         const int cc = (p.corder && c0 + cl < p.nc) ? p.corder[c0 + cl] : c0 + cl;
         sFn[t] = cc < p.nc ? (r < 6 ? p.bid[cc * 6 + r] : p.nbr[cc * 6 + r - 6]) : -1;
     }
-    const bool inblk = lc < p.cpb;
+    const bool inblk = ONE ? tid < NN : lc < p.cpb;
     const bool act = inblk && c < p.nc;
     const int li = lc * NN + node;
     const int ti = node % NP, tj = (node / NP) % NP, tk = node / (NP * NP);
@@ -1067,12 +1086,38 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     const int c = cf / 6, f = cf - 6 * c;
     const int b = td.z, nb = td.y;
     const Gas& gs = p.gas;
-    const int sd = f >> 1;
+    const bool interior = b < 0;
+    // face-node metric terms of both sides (direction Ja/|Ja|, weight |Ja|/w_end, detJ): in the
+    // HLLC instantiation from the cells' trilinear maps when the handle has them (before the states,
+    // so that only the results stay live), else from the face-node table of the task (the other
+    // schemes' instantiation keeps its registers for the rotation)
+    TF dirc[3], nwc, dirn[3] = {TF(0), TF(0), TF(0)}, nwn = TF(0);
+    double Jc, Jn = 1.0;
+    if (HN && p.cmap) {
+        double d[3], w;
+        face_node_metrics<N>(p, c, f, q, d, w, Jc);
+        for (int a = 0; a < 3; ++a) dirc[a] = (TF)d[a];
+        nwc = (TF)w;
+        if (interior) {
+            face_node_metrics<N>(p, nb, f ^ 1, q, d, w, Jn);
+            for (int a = 0; a < 3; ++a) dirn[a] = (TF)d[a];
+            nwn = (TF)w;
+        }
+    } else {
+        const TF* fm = tsel<TF>(p.fmet, p.fmet_f) + (long long)t * 10 * NF + q;
+        for (int a = 0; a < 3; ++a) dirc[a] = fm[a * NF];
+        nwc = fm[3 * NF];
+        Jc = (double)fm[4 * NF];
+        if (interior) {
+            for (int a = 0; a < 3; ++a) dirn[a] = fm[(5 + a) * NF];
+            nwn = fm[8 * NF];
+            Jn = (double)fm[9 * NF];
+        }
+    }
     const long long g = (long long)c * NN + face_node<N>(f, q);
     double u[5], ug[5], n[3];
     load_state(p.Q, p.ndof, g, u);
     for (int d = 0; d < 3; ++d) n[d] = p.nrm[((long long)cf * 3 + d) * NF + q];
-    const bool interior = b < 0;
     const bool periodic = td.w == PLENS_BC_PERIODIC;
     long long gn = 0;
     double bl;
@@ -1099,18 +1144,6 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     // (dgsem/plens.cc:3237-3268, 3373-3406): -s (-s F*_side - f(u; Ja/|Ja|)) |Ja| / w_end, with
     // F*_side the outward flux of that side (-F for the neighbour of an interior face) and
     // s = +1 / -1 for left / right faces.
-    // face-node metric terms of both sides (direction Ja/|Ja|, weight |Ja|/w_end, detJ)
-    const TF* fm = tsel<TF>(p.fmet, p.fmet_f) + (long long)t * 10 * NF + q;
-    const TF dirc[3] = {fm[0], fm[NF], fm[2 * NF]};
-    const TF nwc = fm[3 * NF];
-    const double Jc = (double)fm[4 * NF];
-    TF dirn[3] = {TF(0), TF(0), TF(0)}, nwn = TF(0);
-    double Jn = 1.0;
-    if (interior) {
-        for (int a = 0; a < 3; ++a) dirn[a] = fm[(5 + a) * NF];
-        nwn = fm[8 * NF];
-        Jn = (double)fm[9 * NF];
-    }
     const TF sgc = (f & 1) ? TF(-1) : TF(1), sgn = -sgc;
     const long long o = (long long)cf * 5 * NF + q;
     const long long on = ((long long)nb * 6 + (f ^ 1)) * 5 * NF + q;
@@ -1402,23 +1435,26 @@ __device__ __forceinline__ void pair_ismail_roe(const PairNode<TF>& A, const Pai
 // residual in registers and leaves F in LDS, where the partner picks it up with its own D entry
 // (F#(u_a, u_m) = F#(u_m, u_a)). Own pair data stay in registers; no pair index is decoded.
 // =============================================================================================
-template <int N, typename TF, bool EXT>
+// ONE: one cell per workgroup (cells_per_block 1, the default for N >= 4): the cell index is the
+// workgroup index, uniform over the workgroup, so per-cell data (map coefficients, blender values,
+// the step) are loaded once into scalar registers
+template <int N, typename TF, bool EXT, bool ONE>
 __global__ void __launch_bounds__(256) k_vol(KP p) {
     constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
     constexpr int KP = NP / 2;                     // pair slots per node and direction
     constexpr bool EVEN = (NP % 2) == 0;
     extern __shared__ double smem[];
     __shared__ double sD[NP * NP], sW[NP], sQm[NP * NP], sC[NP * NP];
-    const int B = p.cpb * NN;
+    const int B = ONE ? NN : p.cpb * NN;  // a compile-time constant with one cell per workgroup
     double* sR = smem;                               // 2B: per-node FP64 data of the pair flux
     TF* sA = reinterpret_cast<TF*>(sR + 2 * B);      // 5B: per-node flux-type data of the pair flux
     TF* sJa = sA + 5 * B;                            // 3B: Ja^md of the direction being processed
     TF* sF = sJa + 3 * B;                            // 5 KP B pair fluxes / 5B viscous scratch
     load_basis<N>(p.basis, sD, sW, sQm, sC);
 
-    const int tid = threadIdx.x, lc = tid / NN, node = tid - lc * NN;
-    const int c = blockIdx.x * p.cpb + lc;
-    const bool inblk = lc < p.cpb;
+    const int tid = threadIdx.x, lc = ONE ? 0 : tid / NN, node = ONE ? tid : tid - lc * NN;
+    const int c = ONE ? (int)blockIdx.x : (int)blockIdx.x * p.cpb + lc;
+    const bool inblk = ONE ? tid < NN : lc < p.cpb;
     const bool act = inblk && c < p.nc;
     const int li = lc * NN + node;
     const int ti = node % NP, tj = (node / NP) % NP, tk = node / (NP * NP);
@@ -1805,12 +1841,19 @@ int launch_grad_p(plens_gpu* h, const KP& kp) {
     const int thr = threads_for(kp.cpb, h->NN);
     const int blocks = (kp.nc + kp.cpb - 1) / kp.cpb;
     const size_t lds = lds_grad_bytes<TF>(kp.cpb, h->NN);
+    const bool one = kp.cpb == 1;
     switch (h->N) {
-        case 1: hipLaunchKernelGGL((k_grad<1, TF>), dim3(blocks), dim3(thr), lds, h->stream, kp); break;
-        case 2: hipLaunchKernelGGL((k_grad<2, TF>), dim3(blocks), dim3(thr), lds, h->stream, kp); break;
-        case 3: hipLaunchKernelGGL((k_grad<3, TF>), dim3(blocks), dim3(thr), lds, h->stream, kp); break;
-        case 4: hipLaunchKernelGGL((k_grad<4, TF>), dim3(blocks), dim3(thr), lds, h->stream, kp); break;
-        default: hipLaunchKernelGGL((k_grad<5, TF>), dim3(blocks), dim3(thr), lds, h->stream, kp); break;
+        case 1: hipLaunchKernelGGL((k_grad<1, TF, false>), dim3(blocks), dim3(thr), lds, h->stream, kp); break;
+        case 2: hipLaunchKernelGGL((k_grad<2, TF, false>), dim3(blocks), dim3(thr), lds, h->stream, kp); break;
+        case 3: hipLaunchKernelGGL((k_grad<3, TF, false>), dim3(blocks), dim3(thr), lds, h->stream, kp); break;
+        case 4:
+            if (one) hipLaunchKernelGGL((k_grad<4, TF, true>), dim3(blocks), dim3(thr), lds, h->stream, kp);
+            else hipLaunchKernelGGL((k_grad<4, TF, false>), dim3(blocks), dim3(thr), lds, h->stream, kp);
+            break;
+        default:
+            if (one) hipLaunchKernelGGL((k_grad<5, TF, true>), dim3(blocks), dim3(thr), lds, h->stream, kp);
+            else hipLaunchKernelGGL((k_grad<5, TF, false>), dim3(blocks), dim3(thr), lds, h->stream, kp);
+            break;
     }
     HIPCHK(hipGetLastError());
     return PLENS_OK;
@@ -1846,8 +1889,12 @@ int launch_one_t(plens_gpu* h, const KP& kp, int kernel) {
     // pair fluxes (k_vol)
     switch (kernel) {
         case 0:
-            hipLaunchKernelGGL((k_grad<N, TF>), dim3(blocks), dim3(thr), lds_grad_bytes<TF>(kp.cpb, NN), h->stream,
-                               kp);
+            if (N >= 4 && kp.cpb == 1)
+                hipLaunchKernelGGL((k_grad<N, TF, (N >= 4)>), dim3(blocks), dim3(thr), lds_grad_bytes<TF>(kp.cpb, NN),
+                                   h->stream, kp);
+            else
+                hipLaunchKernelGGL((k_grad<N, TF, false>), dim3(blocks), dim3(thr), lds_grad_bytes<TF>(kp.cpb, NN),
+                                   h->stream, kp);
             HIPCHK(hipGetLastError());
             if (kp.mlp && kp.feat)
                 hipLaunchKernelGGL(k_mlp, dim3((kp.nc + kMlpCells - 1) / kMlpCells), dim3(256), 0, h->stream, kp);
@@ -1888,10 +1935,17 @@ int launch_one_t(plens_gpu* h, const KP& kp, int kernel) {
             // FP64 pair-flux data (2B) + flux-type pair data (5B), direction metrics (3B) and the
             // pair fluxes of NP/2 slots (5 NP/2 B, which also holds the viscous scratch)
             const size_t lds_vol = sizeof(double) * 2 * B + sizeof(TF) * (8 * B + 5 * (size_t)(NP / 2) * B);
-            if (ext)
-                hipLaunchKernelGGL((k_vol<N, TF, true>), dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
-            else
-                hipLaunchKernelGGL((k_vol<N, TF, false>), dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
+            const bool one = N >= 4 && kp.cpb == 1;
+            if constexpr (N >= 4) {
+                if (one && ext)
+                    hipLaunchKernelGGL((k_vol<N, TF, true, true>), dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
+                else if (one)
+                    hipLaunchKernelGGL((k_vol<N, TF, false, true>), dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
+            }
+            if (!one && ext)
+                hipLaunchKernelGGL((k_vol<N, TF, true, false>), dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
+            else if (!one)
+                hipLaunchKernelGGL((k_vol<N, TF, false, false>), dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
             break;
         }
         default:
