@@ -238,6 +238,23 @@ int plens_gpu_ss_error_noalloc(plens_gpu* h, double* err);
  * The state is not changed; the time step and blender buffers hold the new values afterwards, so
  * a following plens_gpu_ss_error divides by the new steps as write() does. Single handles only.
  * The first call allocates the device buffer of the fields. */
+/* Expression-valued parameters (deal.II FunctionParser syntax in x, y, z, t, as the prm strings):
+ *   courant                 "Courant number" (dgsem/plens.cc:1484-1488), evaluated at the origin and
+ *                           the step's start time (3816-3817), capped at 0.9 when >= 1; replaces the
+ *                           courant argument of the run, step and calc_dt calls;
+ *   wall_blender_limit      "wall blender limit" (2884-2898), at the origin and that time, in place of
+ *                           plens_physics.wall_blender_limit;
+ *   viscous_blending_region "viscous blending region" (2886-2889, 2953): alpha_d = min(1, alpha /
+ *                           alpha_max) in cells whose center (the mean of the vertices, else of the
+ *                           DoF locations) gives a value > 0, and 0 elsewhere (the default "1" is
+ *                           every cell).
+ * Each is nullable (NULL: the numeric parameter / every cell). PLENS_E_INVALID for a syntax error,
+ * for a region without vertices or DoF locations in the mesh, or for a batch-merged handle; batches
+ * reject handles with expressions. The solution time is the handle's (plens_gpu_set_time, and the
+ * t of runs and steps). */
+int plens_gpu_set_expressions(plens_gpu* h, const char* courant, const char* wall_blender_limit,
+                              const char* viscous_blending_region);
+
 /* Solution time of the handle (the BCs' cur_time for plens_gpu_calc_rhs; runs and steps set it
  * from their *t_inout and advance it). */
 int plens_gpu_set_time(plens_gpu* h, double t);

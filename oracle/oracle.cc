@@ -895,6 +895,8 @@ struct oracle {
     std::vector<double> Q, avars, mu, kap, alpha, alpha_d, rhs, rhs1, rhs2, Qn, loc_dt;
     std::vector<double> alpha_given, alpha_d_given;  // checker hook: blender values to use, when set
     std::vector<double> raw_given;                    // checker hook: indicator values (pre-override)
+    std::string x_courant, x_wall, x_region;          // expression-valued Courant number, wall blender
+                                                      // limit, viscous blending region (empty: unset)
     int64_t blended_cell_stages = 0;                  // (cell, RHS evaluation) pairs with alpha > 0
     std::vector<double> s1, s2, s3;                     // surface terms [var][c][f][q]
     std::vector<double> ws1;                            // entropy variables of s1 (default build)
@@ -1475,12 +1477,14 @@ struct oracle {
             }
         }
         if (!raw_given.empty()) raw = raw_given;  // checker hook: given indicator values
+        // wall_blender_limit_function.value(Point<dim>()) at cur_time (dgsem/plens.cc:2884-2898)
+        const double wlim = x_wall.empty() ? ph.wall_blender_limit : eval_expression(x_wall, 0.0, 0.0, 0.0, bc_time);
         auto wall_override = [&]() {
             for (int c = 0; c < nc; ++c) {
                 for (int f = 0; f < 6; ++f) {
                     const int b = bid[(size_t)c * 6 + f];
                     if (b >= 0 && (bcs[b].type == 3 || bcs[b].type == 7))  // uniform temp / insulated wall
-                        alpha[c] = std::max(ph.wall_blender_limit, g_mutation == kMutBlendOverrideLate ? alpha[c] : raw[c]);
+                        alpha[c] = std::max(wlim, g_mutation == kMutBlendOverrideLate ? alpha[c] : raw[c]);
                 }
             }
         };
@@ -1497,8 +1501,18 @@ struct oracle {
                 alpha[c] = std::max(cur, 0.5 * nei);
             }
         if (g_mutation == kMutBlendOverrideLate) wall_override();
-        for (int c = 0; c < nc; ++c)  // region "1"
-            alpha_d[c] = g_mutation == kMutBlendNoCap ? alpha[c] / ph.alpha_max : std::min(1.0, alpha[c] / ph.alpha_max);
+        for (int c = 0; c < nc; ++c) {  // vis_blending_region.value(cell->center()) > 0 (2950-2960)
+            bool in_region = true;
+            if (!x_region.empty()) {
+                double ctr[3] = {0, 0, 0};  // TriaAccessor::center(): the mean of the vertices
+                for (int v = 0; v < 8; ++v)
+                    for (int d = 0; d < 3; ++d) ctr[d] += verts[(size_t)c * 24 + 3 * v + d] / 8.0;
+                in_region = eval_expression(x_region, ctr[0], ctr[1], ctr[2], bc_time) > 0;
+            }
+            alpha_d[c] = !in_region ? 0.0
+                         : g_mutation == kMutBlendNoCap ? alpha[c] / ph.alpha_max
+                                                        : std::min(1.0, alpha[c] / ph.alpha_max);
+        }
     }
 
     // ---- PLENS::calc_cell_ho_inv_residual, dgsem/plens.cc:3153-3275 ----------------------------
@@ -1692,6 +1706,8 @@ struct oracle {
     // ---- PLENS::calc_time_step, dgsem/plens.cc:3810-3912: per-cell steps into loc_dt, then global
     //      stepping (every cell gets the minimum) or, past the start time, local stepping
     double calc_time_step(double Co, double cur_time = 0.0) {
+        // courant_function.value(Point<dim>()) at cur_time (dgsem/plens.cc:3816-3817)
+        if (!x_courant.empty()) Co = eval_expression(x_courant, 0.0, 0.0, 0.0, cur_time);
         if (Co >= 1 && g_mutation != kMutDtNoCap) Co = 0.9;
         double proc = 1e6;
         loc_dt.assign(nc, 0.0);
@@ -1937,7 +1953,7 @@ int oracle_calc_dt(oracle* o, double courant, double* dt) {
         for (int v = 0; v < 5; ++v) s[v] = o->Q[v * o->ndof + i];
         o->mu[i] = o->ns.inviscid() ? 0.0 : o->ns.mu(NS::get_e(s) / cvv);
     }
-    *dt = o->calc_time_step(courant);
+    *dt = o->calc_time_step(courant, o->bc_time);
     return 0;
 }
 
@@ -1969,6 +1985,15 @@ int oracle_set_alpha(oracle* o, const double* alpha, const double* alpha_d) {
 int oracle_set_raw_alpha(oracle* o, const double* raw) {
     if (!raw) o->raw_given.clear();
     else o->raw_given.assign(raw, raw + o->nc);
+    return 0;
+}
+
+int oracle_set_expressions(oracle* o, const char* courant, const char* wall_blender_limit,
+                           const char* viscous_blending_region) {
+    if (!o) return -1;
+    o->x_courant = courant ? courant : "";
+    o->x_wall = wall_blender_limit ? wall_blender_limit : "";
+    o->x_region = viscous_blending_region ? viscous_blending_region : "";
     return 0;
 }
 

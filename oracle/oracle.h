@@ -73,6 +73,11 @@ int oracle_bc_ghost(const oracle* o, int32_t stage, int32_t cell, int32_t face, 
  * absolute momentum; 21 diffusion from diffused neighbours, 22 diffusion across periodic faces,
  * 23 wall override after diffusion, 24 alpha_d not capped at 1. */
 int oracle_set_mutation(int32_t id);
+/* expression-valued "Courant number" (origin, cur_time), "wall blender limit" (origin, cur_time) and
+ * "viscous blending region" (cell center = vertex mean, cur_time; alpha_d = 0 where not > 0), each
+ * nullable (NULL: the numeric value / every cell) */
+int oracle_set_expressions(oracle* o, const char* courant, const char* wall_blender_limit,
+                           const char* viscous_blending_region);
 /* number of (cell, RHS evaluation) pairs with a blender value alpha > 0 since creation */
 int oracle_blended_cell_stages(const oracle* o, int64_t* count);
 /* location of node `node` of cell c (through the cell's map) */

@@ -52,6 +52,7 @@ def load():
         "oracle_set_raw_alpha": (C.c_int, [vp, _dp]),
         "oracle_set_mutation": (C.c_int, [C.c_int32]),
         "oracle_blended_cell_stages": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+        "oracle_set_expressions": (C.c_int, [vp, C.c_char_p, C.c_char_p, C.c_char_p]),
         "oracle_bc_ghost": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _dp, _dp, _dp, _dp, _dp]),
         "oracle_node_location": (C.c_int, [vp, C.c_int32, C.c_int32, _dp]),
         "oracle_eval_expression": (C.c_double, [C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_double]),
@@ -245,6 +246,11 @@ class OraclePlens:
             return
         self._raw_given = np.ascontiguousarray(raw, dtype=np.float64)
         self.lib.oracle_set_raw_alpha(self._h, self._raw_given.ctypes.data_as(_dp))
+
+    def set_expressions(self, courant=None, wall_blender_limit=None, viscous_blending_region=None):
+        """expression-valued Courant number, wall blender limit and viscous blending region"""
+        enc = lambda x: x.encode() if x is not None else None
+        self.lib.oracle_set_expressions(self._h, enc(courant), enc(wall_blender_limit), enc(viscous_blending_region))
 
     def blended_cell_stages(self):
         """(cell, RHS evaluation) pairs with alpha > 0 since creation"""

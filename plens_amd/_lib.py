@@ -73,7 +73,7 @@ GPU_SYMBOLS = [
     "plens_gpu_last_failure", "plens_gpu_set_tuning", "plens_gpu_kernel_times", "plens_gpu_strerror",
     "plens_gpu_last_run_ms", "plens_gpu_blended_cell_stages", "plens_gpu_run_profiled", "plens_gpu_stream_copy", "plens_gpu_time_kernel",
     "plens_gpu_set_local_stepping", "plens_gpu_ss_error", "plens_gpu_ss_error_noalloc", "plens_gpu_output_fields", "plens_gpu_write_checkpoint",
-    "plens_gpu_read_checkpoint", "plens_gpu_set_time",
+    "plens_gpu_read_checkpoint", "plens_gpu_set_time", "plens_gpu_set_expressions",
     "plens_gpu_batch_create", "plens_gpu_batch_destroy", "plens_gpu_batch_run", "plens_gpu_batch_get_state",
     "plens_gpu_batch_last_run_ms", "plens_gpu_batch_last_failure", "plens_gpu_batch_n_groups",
     "plens_gpu_batch_group_info",
@@ -131,6 +131,7 @@ def load():
         "plens_gpu_ss_error_noalloc": (C.c_int, [vp, _dp]),
         "plens_gpu_output_fields": (C.c_int, [vp, C.c_double, C.c_double, _dp, _dp]),
         "plens_gpu_set_time": (C.c_int, [vp, C.c_double]),
+        "plens_gpu_set_expressions": (C.c_int, [vp, C.c_char_p, C.c_char_p, C.c_char_p]),
         "plens_expr_eval": (C.c_int, [C.c_char_p, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, _dp]),
         "plens_gpu_write_checkpoint": (C.c_int, [vp, C.c_char_p, C.c_double, C.c_int64, C.c_int64]),
         "plens_gpu_read_checkpoint": (C.c_int, [vp, C.c_char_p, _dp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),

@@ -138,6 +138,14 @@ struct KP {
     const int* vbc;
     const ExprCode* vcode;
     int nvslot;
+    // expression-valued parameters of the step (plens_gpu_set_expressions): xset bit 0 Courant
+    // number, bit 1 wall blender limit, bit 2 viscous blending region; programs xcode[3], their
+    // values for the current step xval[2] (Courant, wall limit) and the region mask of every cell
+    int xset;
+    const ExprCode* xcode;
+    double* xval;
+    double* xreg;
+    const double* ccen;      // cell centers [n_cells][3] (vertex means)
     double thr, sharp, amin, amax, wlimit, courant;
     const double* cfl;       // Courant number of every instance (batches; nullptr: courant)
     double iw0, iwN;  // 1 / w_0, 1 / w_N of the GLL rule
@@ -397,6 +405,21 @@ __global__ void __launch_bounds__(256) k_vinflow(KP p, int NF) {
     for (int v = 0; v < 5; ++v) dst[v] = cons[v];
 }
 
+// k_xstep: the expression-valued parameters for the current step, at the solution time at its start
+// (cur_time): the Courant number and the wall blender limit at the origin (courant_function /
+// wall_blender_limit_function .value(Point<dim>()), dgsem/plens.cc:3816-3817, 2896-2898), the viscous
+// blending region at every cell's center (vis_blending_region.value(cell->center()) > 0, 2953)
+__global__ void __launch_bounds__(256) k_xstep(KP p) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const double t = p.tabs[0];
+    if (c == 0) {
+        if (p.xset & 1) p.xval[0] = expr_eval(p.xcode[0], 0.0, 0.0, 0.0, t);
+        if (p.xset & 2) p.xval[1] = expr_eval(p.xcode[1], 0.0, 0.0, 0.0, t);
+    }
+    if ((p.xset & 4) && c < p.nc)
+        p.xreg[c] = expr_eval(p.xcode[2], p.ccen[3 * c], p.ccen[3 * c + 1], p.ccen[3 * c + 2], t) > 0.0 ? 1.0 : 0.0;
+}
+
 // basis tables into LDS; sW holds the reciprocal GLL weights 1/w_i (every use divides by w_i)
 template <int N>
 __device__ __forceinline__ void load_basis(const Basis* b, double* sD, double* sW, double* sQm, double* sC) {
@@ -581,7 +604,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                     for (int cc = 0; cc < 3; ++cc) s += (ja[3 * cc + r] * idet) * vel[cc];
                     madv = fmax(madv, s);
                 }
-                const double cr = p.cfl ? p.cfl[p.cinst[c]] : p.courant;
+                const double cr = p.cfl ? p.cfl[p.cinst[c]] : ((p.xset & 1) ? p.xval[0] : p.courant);
                 const double co = cr >= 1.0 ? 0.9 : cr;
                 cur = co / (pow((double)N, 1.5) * (madv + a / h + (double)(N * N) * mu / (h * h * u[0])));
             }
@@ -874,17 +897,19 @@ __global__ void __launch_bounds__(kThreadsAlpha) k_alpha(KP p) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < p.nc) {
         // wall override (dgsem/plens.cc:2909-2920) of this cell and of the neighbours' pre-diffusion values
+        const double wl = (p.xset & 2) ? p.xval[1] : p.wlimit;
         double a = p.araw[c];
-        if (p.wall[c]) a = fmax(p.wlimit, a);
+        if (p.wall[c]) a = fmax(wl, a);
         for (int f = 0; f < 6; ++f) {
             if (p.bid[c * 6 + f] >= 0) continue;  // boundary faces (periodic included) are skipped
             const int nb = p.nbr[c * 6 + f];
             double an = p.araw[nb];
-            if (p.wall[nb]) an = fmax(p.wlimit, an);
+            if (p.wall[nb]) an = fmax(wl, an);
             a = fmax(a, 0.5 * an);
         }
         p.alpha[c] = a;
-        p.alpha_d[c] = fmin(1.0, a / p.amax);
+        // alpha_d in the viscous blending region (dgsem/plens.cc:2950-2960), 0 outside it
+        p.alpha_d[c] = ((p.xset & 4) && p.xreg[c] == 0.0) ? 0.0 : fmin(1.0, a / p.amax);
         if (a > 0.0) p.blist[atomicAdd(p.bcount, 1)] = c;  // order irrelevant: cells are independent
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {  // latch the failure of the first failing stage
@@ -1749,6 +1774,9 @@ struct plens_gpu {
     double* dt_lts = nullptr;    // local time stepping: per-cell steps (allocated on request)
     double* ss_part = nullptr;   // steady-state error: per-cell integrals, then the two sums
     double* post = nullptr;      // output fields (allocated by the first output call)
+    ExprCode* xcode = nullptr;   // expression-valued Courant number, wall blender limit, viscous
+    double *xval = nullptr, *xreg = nullptr, *ccen = nullptr;  // blending region (set_expressions)
+    std::vector<double> centers; // cell centers (vertex means, else DoF-location means), host copy
     int* vslot = nullptr;        // varying inflow: face slots, prescribed states, coordinates,
     double *vin = nullptr, *vxyz = nullptr;  // bc of every slot, compiled expressions
     int* vbc = nullptr;
@@ -2016,6 +2044,10 @@ KP stage_params(plens_gpu* h, int s, bool from_b, double courant) {
 
 // prescribed varying-inflow states for the solution time held by the device
 int launch_vinflow(plens_gpu* h, const KP& kp) {
+    if (kp.xset) {  // expression-valued parameters of this step
+        hipLaunchKernelGGL(k_xstep, dim3((kp.nc + 255) / 256), dim3(256), 0, h->stream, kp);
+        HIPCHK(hipGetLastError());
+    }
     if (kp.nvslot <= 0) return PLENS_OK;
     const long long nt = (long long)kp.nvslot * h->NF;
     hipLaunchKernelGGL(k_vinflow, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, h->stream, kp, h->NF);
@@ -2508,6 +2540,23 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
                 return fail(PLENS_E_HIP);
         }
     }
+    // cell centers (TriaAccessor::center: the mean of the vertices; without vertices the mean of the
+    // DoF locations, which is the same point for trilinear cells) for the viscous blending region
+    if (m->vertices || m->node_coords) {
+        h->centers.assign((size_t)nc * 3, 0.0);
+        for (int c = 0; c < nc; ++c)
+            for (int d = 0; d < 3; ++d) {
+                double a = 0;
+                if (m->vertices) {
+                    for (int v = 0; v < 8; ++v) a += m->vertices[((size_t)c * 8 + v) * 3 + d];
+                    a /= 8;
+                } else {
+                    for (int k = 0; k < NN; ++k) a += m->node_coords[((size_t)c * 3 + d) * NN + k];
+                    a /= NN;
+                }
+                h->centers[(size_t)c * 3 + d] = a;
+            }
+    }
     const int ioff_h[2] = {0, nc};
     const double visc_h[3] = {ph->mu0, ph->T0, ph->S};
     std::vector<double> cm;
@@ -2629,7 +2678,7 @@ void plens_gpu_destroy(plens_gpu* h) {
                     h->sinv, h->sface, h->rhs, h->rlo, h->dt_cell, h->scal, h->fail, h->blist, h->bcount, h->bstat, h->mlp_stats, h->feat,
                     h->avars_f, h->sinv_f, h->sface_f, h->cinst, h->ioff, h->visc, h->cfl, h->cmap, h->fmet, h->fmet_f,
                     h->tabs, h->rhoE_old, h->dt_lts, h->ss_part, h->post, h->vslot, h->vin, h->vxyz,
-                    h->vbc, h->vcode, h->corder};
+                    h->vbc, h->vcode, h->corder, h->xcode, h->xval, h->xreg, h->ccen};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->stream && h->own_stream) (void)hipStreamDestroy(h->stream);
@@ -2935,6 +2984,35 @@ int plens_gpu_ss_error_noalloc(plens_gpu* h, double* err) {
     HIPCHK(hipMemcpyAsync(nd, out, sizeof(nd), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     *err = std::sqrt(nd[0]) / std::sqrt(nd[1]);
+    return PLENS_OK;
+}
+
+int plens_gpu_set_expressions(plens_gpu* h, const char* courant, const char* wall_blender_limit,
+                              const char* viscous_blending_region) {
+    if (!h || h->ninst != 1) return PLENS_E_INVALID;
+    if (viscous_blending_region && h->centers.empty()) return PLENS_E_INVALID;
+    ExprCode codes[3];
+    const char* src[3] = {courant, wall_blender_limit, viscous_blending_region};
+    int set = 0;
+    for (int i = 0; i < 3; ++i) {
+        if (!src[i]) continue;
+        if (!expr_compile(src[i], &codes[i], nullptr, 0)) return PLENS_E_INVALID;
+        set |= 1 << i;
+    }
+    if (set && !h->xcode) {
+        if (dmalloc(&h->xcode, 3) || dmalloc(&h->xval, 2) || dmalloc(&h->xreg, (size_t)h->nc)) return PLENS_E_NOMEM;
+        if (!h->centers.empty()) {
+            if (dmalloc(&h->ccen, h->centers.size())) return PLENS_E_NOMEM;
+            HIPCHK(hipMemcpy(h->ccen, h->centers.data(), sizeof(double) * h->centers.size(), hipMemcpyHostToDevice));
+        }
+    }
+    if (set) HIPCHK(hipMemcpy(h->xcode, codes, sizeof(codes), hipMemcpyHostToDevice));
+    h->base.xset = set;
+    h->base.xcode = h->xcode;
+    h->base.xval = h->xval;
+    h->base.xreg = h->xreg;
+    h->base.ccen = h->ccen;
+    destroy_graphs(h);
     return PLENS_OK;
 }
 
@@ -3456,7 +3534,8 @@ extern "C" {
 int plens_gpu_batch_create(plens_gpu* const* instances, int32_t n, const char* tuning_db, plens_batch** out) {
     if (!instances || n < 1 || !out) return PLENS_E_INVALID;
     for (int i = 0; i < n; ++i)  // single handles; no varying inflow (its tables are per handle)
-        if (!instances[i] || instances[i]->ninst != 1 || instances[i]->nvslot > 0) return PLENS_E_INVALID;
+        if (!instances[i] || instances[i]->ninst != 1 || instances[i]->nvslot > 0 || instances[i]->base.xset)
+            return PLENS_E_INVALID;
     plens_batch* b = new (std::nothrow) plens_batch;
     if (!b) return PLENS_E_NOMEM;
     auto fail = [&](int code) {

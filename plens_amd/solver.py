@@ -134,6 +134,13 @@ class GpuPlens:
         self._check(self.lib.plens_gpu_ss_error(self._h, C.byref(e)), "ss_error")
         return e.value
 
+    def set_expressions(self, courant=None, wall_blender_limit=None, viscous_blending_region=None):
+        """expression-valued Courant number, wall blender limit and viscous blending region (prm
+        strings in x, y, z, t; None keeps the numeric parameter / every cell)"""
+        enc = lambda x: x.encode() if x is not None else None
+        self._check(self.lib.plens_gpu_set_expressions(self._h, enc(courant), enc(wall_blender_limit),
+                                                       enc(viscous_blending_region)), "set_expressions")
+
     def set_time(self, t):
         """solution time of the handle (the BCs' time of calc_rhs)"""
         self._check(self.lib.plens_gpu_set_time(self._h, t), "set_time")

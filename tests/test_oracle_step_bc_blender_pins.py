@@ -366,3 +366,54 @@ def test_blender_pin_catches_planted_mistakes(mutation, wall_limit):
     with PO.mutation(mutation):
         with pytest.raises(AssertionError):
             check_blender(wall_limit)
+
+
+# ---- expression-valued parameters ("Courant number", "wall blender limit", "viscous blending
+#      region"; dgsem/plens.cc:1484-1488, 2884-2898, 2950-2960, 3816-3817) ------------------------
+def test_courant_expression_at_the_solution_time():
+    """Co(t) is evaluated at the origin and the solution time, then capped at 0.9 when >= 1"""
+    A, N, rho, vel, p, co, mu0 = DT_CASES["sheared"]
+    m = lattice_mesh(A, N)
+    o = PO.OraclePlens(mesh=m, physics=cases.make_physics(mu0=mu0), bcs=[cases.make_bc(L.BC_PERIODIC)])
+    E = p / (GAMMA - 1) + 0.5 * rho * float(np.dot(vel, vel))
+    o.set_state(np.tile(np.array([rho, rho * vel[0], rho * vel[1], rho * vel[2], E])[:, None], (1, m.n_dof)))
+    o.set_expressions(courant="0.2*(1+t) + 0*x*y*z")
+    for t, want_co in ((0.5, 0.3), (0.0, 0.2), (5.0, 0.9)):  # 0.2*6 = 1.2 -> 0.9
+        o.set_time(t)
+        got = o.calc_time_step(0.77)  # the numeric Courant number is ignored
+        assert abs(got - closed_form_dt(A, N, rho, vel, p, want_co, mu0)) <= 1e-13 * got
+    o.close()
+
+
+def test_wall_blender_limit_expression():
+    """the hand pattern of the blender pins with the limit 0.2 given as 0.1 + t at t = 0.1"""
+    o = blender_oracle(0.6)  # the numeric limit must be ignored
+    raw = np.zeros(9)
+    for c, v in RAW.items():
+        raw[c] = v
+    o.set_raw_alpha(raw)
+    o.set_expressions(wall_blender_limit="0.1 + t")
+    o.set_time(0.1)
+    _, a, _ = o.calc_rhs()
+    o.close()
+    for c in range(9):
+        assert abs(a[c] - ALPHA_LIMIT_02[c]) < 1e-15
+
+
+def test_viscous_blending_region_expression():
+    """alpha_d = min(1, alpha/alpha_max) only in cells whose center satisfies the region (x < 0.4:
+    the column i = 0, centers at x = 1/6), 0 elsewhere; alpha itself unchanged"""
+    o = blender_oracle(0.2)
+    raw = np.zeros(9)
+    for c, v in RAW.items():
+        raw[c] = v
+    o.set_raw_alpha(raw)
+    o.set_expressions(viscous_blending_region="x < 0.4")
+    _, a, ad = o.calc_rhs()
+    o.close()
+    for j in range(3):
+        for i in range(3):
+            c = cell(i, j)
+            assert abs(a[c] - ALPHA_LIMIT_02[c]) < 1e-15
+            want = min(1.0, ALPHA_LIMIT_02[c] / 0.5) if i == 0 else 0.0
+            assert abs(ad[c] - want) < 1e-15, (i, j, ad[c], want)
