@@ -125,3 +125,26 @@ def test_trained_network_gains_over_untrained():
     print(f"learned vs plain: density rel L2 {drho:.3e}, shock position {ds:.4f} cells, "
           f"fallbacks {fb}/{ev}", flush=True)
     assert drho < 1e-2 and ds < 0.1
+
+
+def test_trained_network_blends_fewer_cells_than_plain_indicator():
+    """the gain over the plain method: on the held-out shock tube (seed 1001, 128x8x8, N=3, its 200
+    steps) the trained indicator (network of plens_amd/indicator_mlp.npy, plain fallback at 0.05)
+    sends fewer (cell, stage) pairs through the first-order subcell blend than the plain indicator
+    while the shock stays where the plain run puts it"""
+    from plens_amd import GpuPlens, shock_tube
+    from plens_amd.indicator_study import HELD_OUT_SEED, load_trained, shock_position
+    from plens_amd.learned import use_network
+    case = shock_tube(seed=HELD_OUT_SEED)
+    plain, learned = GpuPlens(case), GpuPlens(case)
+    use_network(learned, load_trained(), 0.05)
+    plain.run(200, case.courant)
+    learned.run(200, case.courant)
+    bp, bl = plain.blended_cell_stages(), learned.blended_cell_stages()
+    Qp, Ql = plain.get_state(), learned.get_state()
+    ds = abs(shock_position(case, Ql) - shock_position(case, Qp)) / (4.0 / 128)
+    drho = np.linalg.norm(Ql[0] - Qp[0]) / np.linalg.norm(Qp[0])
+    print(f"blended cell-stages plain {bp}, learned {bl} ({bl / bp:.3f}); shock position difference "
+          f"{ds:.2e} cells, density rel L2 {drho:.2e}", flush=True)
+    assert bl < 0.9 * bp
+    assert ds < 0.01 and drho < 2e-3
