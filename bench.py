@@ -272,10 +272,12 @@ def main():
     stage_ms = float(sum(kernels[k]["avg_ms"] for k in names))
     fb = 8 if args.precision == "fp64" else 4
     stage_bytes = stage_bytes_per_dof(N, fb)
-    # kernel name as rocprofv3 prints it: k_grad<N, type>, k_vol<N, type, extended scheme>, k_face and
-    # k_lo <N, type, extended scheme, HLLC in the normal frame> (here: HLLC surface flux and
-    # Chandrashekhar volume flux)
-    tail = {"grad": ">", "vol": ",false>", "face": ",false,true>", "lo": ",false,true>"}
+    # kernel name as rocprofv3 prints it: k_grad<N, type, one cell per workgroup>, k_vol<N, type,
+    # extended scheme, one cell per workgroup>, k_face and k_lo <N, type, extended scheme, HLLC in the
+    # normal frame> (here: HLLC surface flux, Chandrashekhar volume flux, N = 4 with one cell per
+    # workgroup)
+    one = "true" if (N >= 4 and args.cells_per_block in (0, 1)) else "false"
+    tail = {"grad": f",{one}>", "vol": f",false,{one}>", "face": ",false,true>", "lo": ",false,true>"}
     tname = ("k_alpha" if dom == "alpha" else
              f"k_{dom}<{N},{'double' if args.precision == 'fp64' else 'float'}" + tail[dom])
     traffic, tsrc = measured_traffic(args.precision, tname)

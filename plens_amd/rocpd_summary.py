@@ -8,8 +8,11 @@ per dispatch per kernel. The streaming copy kernel (k_stream_copy, --copy-bytes 
 default) calibrates both counters: factor = expected KiB / measured KiB. The calibrated HBM bytes
 per dispatch of every kernel are also printed as `traffic_bytes_per_launch <kernel> <bytes>` lines.
 --pmc: database of any other `rocprofv3 --pmc` run -> average value per dispatch of every counter.
---flops: with --pmc holding SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 and --trace, the FP64 VALU
-flop rate per kernel (64 lanes per wave instruction, FMA = 2 flops) against the 78.6 TFLOP/s peak.
+--flops: with --pmc holding SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 (and, for the FP32-bulk path,
+the same _F32 counters) and --trace, the FP64 and FP32 VALU flop rates per kernel (64 lanes per wave
+instruction, FMA = 2 flops) against the 78.6 TFLOP/s FP64 and 157.3 TFLOP/s FP32 peaks (the FP32
+peak counts packed FP32 instructions, which this build does not emit: its FP32 issue rate is that
+of FP64, 78.6 TFLOP/s).
 """
 import argparse
 import sqlite3
@@ -75,14 +78,18 @@ def main():
             print(f"{k:36s} " + " ".join(f"{per[n].get(k, 0.0):22.1f}" for n in names))
         if a.flops and a.trace:
             tr = trace_stats(a.trace)
-            print("\nkernel                               FP64 GFLOP/launch   avg_us   FP64 TFLOP/s   frac of 78.6")
-            for k in kernels:
-                fl = 64.0 * (per.get("SQ_INSTS_VALU_ADD_F64", {}).get(k, 0.0) + per.get("SQ_INSTS_VALU_MUL_F64", {}).get(k, 0.0)
-                             + per.get("SQ_INSTS_VALU_TRANS_F64", {}).get(k, 0.0)
-                             + 2.0 * per.get("SQ_INSTS_VALU_FMA_F64", {}).get(k, 0.0))
-                if k in tr and fl > 0:
-                    us = tr[k][1]
-                    print(f"{k:36s} {fl / 1e9:17.3f} {us:8.2f} {fl / (us * 1e-6) / 1e12:14.2f} {fl / (us * 1e-6) / 78.6e12:13.3f}")
+            for prec, peak in (("F64", 78.6e12), ("F32", 157.3e12)):
+                if f"SQ_INSTS_VALU_FMA_{prec}" not in per:
+                    continue
+                print(f"\nkernel                               {prec} GFLOP/launch   avg_us   {prec} TFLOP/s   frac of {peak / 1e12:.1f}")
+                for k in kernels:
+                    fl = 64.0 * (per.get(f"SQ_INSTS_VALU_ADD_{prec}", {}).get(k, 0.0)
+                                 + per.get(f"SQ_INSTS_VALU_MUL_{prec}", {}).get(k, 0.0)
+                                 + per.get(f"SQ_INSTS_VALU_TRANS_{prec}", {}).get(k, 0.0)
+                                 + 2.0 * per.get(f"SQ_INSTS_VALU_FMA_{prec}", {}).get(k, 0.0))
+                    if k in tr and fl > 0:
+                        us = tr[k][1]
+                        print(f"{k:36s} {fl / 1e9:16.3f} {us:8.2f} {fl / (us * 1e-6) / 1e12:13.2f} {fl / (us * 1e-6) / peak:13.3f}")
 
 
 if __name__ == "__main__":
