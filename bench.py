@@ -29,12 +29,14 @@ METRIC = "DOF-stage-updates/s at N=4: plain FP64 vs mixed+learned, at equal L2 e
 HBM_PEAK_GBS = 8000.0
 
 
-def kernel_bytes_per_dof(N, viscous=True, fb=8):
+def kernel_bytes_per_dof(N, viscous=True, fb=8, face_maps=True):
     """Algorithmic HBM bytes per DoF of one launch of each kernel class (compulsory traffic of this
     design); fb is the byte size of the flux-arithmetic arrays (avars, the face terms and the RK
     residual registers: 8 for FP64, 4 for FP32-bulk); states and metric data are FP64 in every
     variant.
-    Metric terms are derived from each cell's trilinear map: 24 doubles per cell (map_b per DoF)."""
+    Metric terms are derived from each cell's trilinear map: 24 doubles per cell (map_b per DoF).
+    face_maps: the face kernel derives its face-node metric terms from the maps of the face's two
+    cells (the HLLC instantiation on trilinear cells) instead of reading the per-face-node table."""
     n, nf = (N + 1) ** 3, (N + 1) ** 2
     face_per_dof = 6.0 * nf / n  # face nodes per cell node
     map_b = 24 * 8 / n
@@ -46,9 +48,10 @@ def kernel_bytes_per_dof(N, viscous=True, fb=8):
     # k_lo: blended cells only; not counted
     b["lo"] = 0.0
     # k_face, per face node (each shared by two cells): read both states (2x5), the normal (3), the
-    # face-node metric terms of both sides (10) and both avars (2x9, viscous); write both sides'
-    # weighted surface terms (2x5)
-    b["face"] = face_per_dof / 2 * (8 * (10 + 3 + 10) + fb * (18 * v + 10))
+    # face-node metric terms of both sides (10; or the two cells' maps, 2x24 per face) and both avars
+    # (2x9, viscous); write both sides' weighted surface terms (2x5)
+    metric = 2 * 24 * 8 / nf if face_maps else 8 * 10
+    b["face"] = face_per_dof / 2 * (8 * (10 + 3) + metric + fb * (18 * v + 10))
     # k_vol: read Q (5), the map, avars (9, viscous), the weighted surface terms of its face nodes
     # (5 per face node), R' and R'' (stage averages 4/5 and 3/5 of 5); write Q (5), R (5)
     b["vol"] = 8 * (5 + 5) + map_b + fb * (5 * 4 / 5 + 5 * 3 / 5 + 5 + 9 * v + face_per_dof * 5)
