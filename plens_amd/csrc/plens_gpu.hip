@@ -88,7 +88,7 @@ struct KP {
     unsigned int* dtcount;      // blocks of k_alpha done with their minimum
     const BCDev* bcs;
     const Basis* basis;
-    double* avars;
+    double* avars;           // stress and heat flux per node, [n_dof][9] (a face node's 9 values are adjacent)
     double* araw;
     double* alpha;
     double* alpha_d;
@@ -526,7 +526,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     }
     // per-cell sums over the NN modes: butterfly sums within each wavefront when the cell's nodes fill
     // whole wavefronts (one cell per workgroup, or 64 nodes per cell), else an LDS tree
-    const bool wave_sum = (NN % 64 == 0) || p.cpb == 1;
+    const bool wave_sum = ONE || (NN % 64 == 0) || p.cpb == 1;
     if (wave_sum) {
         for (int o = 32; o > 0; o >>= 1) {
             e0 += __shfl_xor(e0, o);
@@ -753,8 +753,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     av[4] = vg[1][2] + vg[2][1];
     av[5] = vg[2][2] + vg[2][2] - c23 * tr;
     TF* avo = tsel<TF>(p.avars, p.avars_f);
-    for (int a = 0; a < 6; ++a) avo[a * p.ndof + g] = av[a] * (TF)mu;
-    for (int a = 6; a < 9; ++a) avo[a * p.ndof + g] = av[a] * (TF)kap;
+    for (int a = 0; a < 6; ++a) avo[g * 9 + a] = av[a] * (TF)mu;
+    for (int a = 6; a < 9; ++a) avo[g * 9 + a] = av[a] * (TF)kap;
 }
 
 // =============================================================================================
@@ -1205,9 +1205,9 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     // stage 3: BR1 average of the viscous fluxes (navier_stokes.cc:1644-1652)
     const TF* avars = tsel<TF>(p.avars, p.avars_f);
     TF av[9], avg[9];
-    for (int a = 0; a < 9; ++a) av[a] = avars[a * p.ndof + g];
+    for (int a = 0; a < 9; ++a) av[a] = avars[g * 9 + a];
     if (interior || periodic) {
-        for (int a = 0; a < 9; ++a) avg[a] = avars[a * p.ndof + gn];
+        for (int a = 0; a < 9; ++a) avg[a] = avars[gn * 9 + a];
     } else {
         for (int a = 0; a < 9; ++a) avg[a] = av[a];
         if (td.w == PLENS_BC_INSULATED_WALL) {
@@ -1587,7 +1587,7 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
     TF av[9];
     if (!gs.inviscid) {
         const TF* avars = tsel<TF>(p.avars, p.avars_f);
-        for (int a = 0; a < 9; ++a) av[a] = act ? avars[a * p.ndof + g] : TF(0);
+        for (int a = 0; a < 9; ++a) av[a] = act ? avars[g * 9 + a] : TF(0);
 #pragma unroll
         for (int md = 0; md < 3; ++md) {
             __syncthreads();
@@ -1707,7 +1707,7 @@ __global__ void __launch_bounds__(256) k_post(KP p, double* out) {
         double u[5];
         load_state(p.Q, p.ndof, g, u);
         for (int a = 0; a < 9; ++a)
-            s0[a * NN + node] = p.avars ? p.avars[a * p.ndof + g] : (double)p.avars_f[a * p.ndof + g];
+            s0[a * NN + node] = p.avars ? p.avars[g * 9 + a] : (double)p.avars_f[g * 9 + a];
         s0[9 * NN + node] = u[0];
         s0[10 * NN + node] = get_p(u, p.gas.gma);
         s0[11 * NN + node] = get_e(u) / p.gas.cv;
