@@ -27,12 +27,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "DOF-stage-updates/s at N=4: plain FP64 vs mixed+learned, at equal L2 error"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6   # MI355X vector (and matrix) FP64
+FP32_PEAK_TFS = 157.3  # MI355X vector FP32
 
 
 def kernel_bytes_per_dof(N, viscous=True, fb=8, face_maps=True):
     """Algorithmic HBM bytes per DoF of one launch of each kernel class (compulsory traffic of this
-    design); fb is the byte size of the flux-arithmetic arrays (avars, the face terms and the RK
-    residual registers: 8 for FP64, 4 for FP32-bulk); states and metric data are FP64 in every
+    design); fb is the byte size of the flux-arithmetic arrays (avars and the face terms: 8 for
+    FP64, 4 for FP32-bulk); states, the RK residual registers and metric data are FP64 in every
     variant.
     Metric terms are derived from each cell's trilinear map: 24 doubles per cell (map_b per DoF).
     face_maps: the face kernel derives its face-node metric terms from the maps of the face's two
@@ -54,16 +56,16 @@ def kernel_bytes_per_dof(N, viscous=True, fb=8, face_maps=True):
     b["face"] = face_per_dof / 2 * (8 * (10 + 3) + metric + fb * (18 * v + 10))
     # k_vol: read Q (5), the map, avars (9, viscous), the weighted surface terms of its face nodes
     # (5 per face node), R' and R'' (stage averages 4/5 and 3/5 of 5); write Q (5), R (5)
-    b["vol"] = 8 * (5 + 5) + map_b + fb * (5 * 4 / 5 + 5 * 3 / 5 + 5 + 9 * v + face_per_dof * 5)
+    b["vol"] = 8 * (5 + 5) + map_b + 8 * (5 * 4 / 5 + 5 * 3 / 5 + 5) + fb * (9 * v + face_per_dof * 5)
     return b
 
 
 def stage_bytes_per_dof(N, fb=8):
     """compulsory bytes per DoF-stage of the two-pass algorithm (lift, then residual) with the fused
     RK epilogue: Q read + write, R write, R' and R'' reads (stage averages), avars write + read, and
-    the cells' trilinear maps (the metric terms are not read); R and avars in the flux-arithmetic
-    type"""
-    return 8 * (5 + 5) + fb * (5 + 4 + 3) + fb * 18 + 24 * 8 / (N + 1) ** 3
+    the cells' trilinear maps (the metric terms are not read); the registers in FP64, avars in the
+    flux-arithmetic type"""
+    return 8 * (5 + 5) + 8 * (5 + 4 + 3) + fb * 18 + 24 * 8 / (N + 1) ** 3
 
 
 def measured_traffic(precision, kernel):
@@ -78,6 +80,39 @@ def measured_traffic(precision, kernel):
             if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
                 return float(parts[2]), os.path.relpath(path, ROOT)
     return None, None
+
+
+def measured_compute(precision, kernel):
+    """FP64 and FP32 VALU flops per launch of `kernel` (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS} counter
+    passes, 64 lanes per instruction, FMA = 2) and the fraction of time its SIMDs' VALUs were busy
+    (SQ_ACTIVE_INST_VALU), from the committed counter summaries of this workload; None where absent"""
+    key = kernel.replace(" ", "")
+    flops, src = {}, None
+    path = os.path.join(ROOT, "profiles", f"wall_flow_N4_{precision}_kernels.md")
+    if os.path.exists(path):
+        prec = None
+        with open(path) as f:
+            for line in f:
+                if "GFLOP/launch" in line:
+                    prec = line.split()[1]  # F64 / F32
+                    continue
+                if prec and not line.strip():
+                    prec = None
+                    continue
+                if prec and line.startswith("k_"):
+                    t = line.split()
+                    if "".join(t[:-4]) == key:
+                        flops[prec] = float(t[-4]) * 1e9
+                        src = os.path.relpath(path, ROOT)
+    busy, bsrc = None, None
+    path = os.path.join(ROOT, "profiles", f"wall_flow_N4_{precision}_sq_mix.md")
+    if os.path.exists(path):
+        with open(path) as f:
+            for line in f:
+                t = [x.strip() for x in line.split("|")]
+                if len(t) > 6 and t[1].replace(" ", "") == key:
+                    busy, bsrc = float(t[5]), os.path.relpath(path, ROOT)
+    return flops, src, busy, bsrc
 
 
 # section timers of the oracle, named as the reference's TimerOutput sections (dgsem/plens_test.cc:903-976)
@@ -291,6 +326,19 @@ def main():
                           "achieved_GBs": stage_bytes * n_dof / (stage_ms * 1e-3) / 1e9,
                           "frac": stage_bytes * n_dof / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
                 "kernels": kernels}
+
+    # where the dominant kernel stands against the vector FP64 (and FP32) peaks: flops per launch from
+    # the committed VALU counter passes over its live duration
+    flops, fsrc, busy, bsrc = measured_compute(args.precision, tname)
+    dom_ms = kernels[dom]["avg_ms"]
+    roofline["compute"] = {
+        "unit": "TFLOP/s", "flop_source": fsrc, "valu_busy": busy, "valu_busy_source": bsrc,
+        "f64": ({"flop_per_launch": flops["F64"], "achieved": flops["F64"] / (dom_ms * 1e-3) / 1e12,
+                 "peak": FP64_PEAK_TFS, "frac": flops["F64"] / (dom_ms * 1e-3) / 1e12 / FP64_PEAK_TFS}
+                if "F64" in flops else None),
+        "f32": ({"flop_per_launch": flops["F32"], "achieved": flops["F32"] / (dom_ms * 1e-3) / 1e12,
+                 "peak": FP32_PEAK_TFS, "frac": flops["F32"] / (dom_ms * 1e-3) / 1e12 / FP32_PEAK_TFS}
+                if "F32" in flops else None)}
 
     # achievable HBM bandwidth on this device: streaming copy of 1 GiB (2 GiB of traffic per copy)
     import ctypes as C

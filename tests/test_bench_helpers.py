@@ -1,0 +1,51 @@
+"""CPU checks of the measurement helpers of bench.py: the algorithmic byte counts per DoF, and the
+parsers that read the committed rocprofv3 summaries (calibrated traffic, VALU flops, VALU busy)."""
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+
+def test_bytes_per_dof_at_degree_4():
+    # FP64, viscous: k_grad Q 5 + map 24/125 doubles + avars 9; k_vol reads Q 5, R' 4, R'' 3 (stage
+    # averages), avars 9, face terms 5 x 6 x 25 / 125 and the map, writes Q 5 and R 5; k_face per
+    # face node (0.6 per DoF)
+    # 13 doubles of states and normal, the two cells' maps (48 doubles per 25-node face), 18 avars, 10
+    # written terms
+    b = bench.kernel_bytes_per_dof(4)
+    m = 24 * 8 / 125
+    assert b["grad"] == pytest.approx(8 * 14 + m)
+    assert b["vol"] == pytest.approx(8 * (5 + 4 + 3 + 9 + 6 + 5 + 5) + m)
+    assert b["face"] == pytest.approx(0.6 * (8 * 13 + 2 * 24 * 8 / 25 + 8 * 28))
+    assert b["face"] < bench.kernel_bytes_per_dof(4, face_maps=False)["face"]
+    # inviscid: no avars anywhere
+    bi = bench.kernel_bytes_per_dof(4, viscous=False)
+    assert b["grad"] - bi["grad"] == pytest.approx(72)
+    # the two-pass stage figure: Q in and out, R out, R' and R'' in, avars out and in, the maps
+    assert bench.stage_bytes_per_dof(4) == pytest.approx(8 * 22 + 8 * 18 + m)
+    # FP32-bulk: avars in FP32, the RK registers FP64
+    assert bench.stage_bytes_per_dof(4, fb=4) == pytest.approx(8 * 22 + 4 * 18 + m)
+    assert bench.kernel_bytes_per_dof(4, fb=4)["vol"] == pytest.approx(8 * 22 + 4 * 15 + m)
+
+
+@pytest.mark.parametrize("precision,kernel", [("fp64", "k_vol<4,double,false,true>"),
+                                              ("fp32_bulk", "k_vol<4,float,false,true>")])
+def test_profile_parsers_find_the_dominant_kernel(precision, kernel):
+    traffic, src = bench.measured_traffic(precision, kernel)
+    assert src is not None and traffic > 1e8
+    flops, fsrc, busy, _ = bench.measured_compute(precision, kernel)
+    assert fsrc is not None and flops["F64"] > 1e9
+    if precision == "fp32_bulk":
+        assert flops["F32"] > 1e8
+    else:
+        assert 0.0 < busy <= 1.0
+
+
+def test_profile_parsers_return_none_for_unknown_kernels():
+    assert bench.measured_traffic("fp64", "k_nothing<4>") == (None, None)
+    flops, src, busy, bsrc = bench.measured_compute("fp64", "k_nothing<4>")
+    assert flops == {} and src is None and busy is None and bsrc is None
