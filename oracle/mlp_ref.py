@@ -55,6 +55,17 @@ def forward(p, X):
     return y, (X, z1, a1, z2, a2, z3, y)
 
 
+def infer(p, X):
+    """the solver-side inference arithmetic: as forward(), with the features and the first hidden
+    layer's activations rounded to BF16 before they enter the matrix products (the operands of BF16
+    matrix cores), FP32 accumulation; the output layer in FP32"""
+    Xb = bf16(X.astype(np.float32))
+    a1 = bf16(gelu(Xb @ bf16(p["W1"]).T + p["b1"]))
+    a2 = gelu(a1 @ bf16(p["W2"]).T + p["b2"])
+    z3 = a2 @ bf16(p["w3"]) + p["b3"][0]
+    return np.float32(0.5) / (np.float32(1) + np.exp(-z3))
+
+
 def loss_and_grads(p, X, t):
     y, (X, z1, a1, z2, a2, z3, y) = forward(p, X)
     n = np.float32(X.shape[0])

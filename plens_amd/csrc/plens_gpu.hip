@@ -762,25 +762,30 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 // =============================================================================================
 // =============================================================================================
 // k_mlp: learned troubled-cell indicator on kMlpCells cells per workgroup. MLP 125-64-64-1, GELU,
-// 0.5 sigmoid, BF16-rounded weights; the two hidden layers are FP32 matrix products on the matrix
-// cores (v_mfma_f32_16x16x4f32, FP32 accumulation; K padded to 128 with zeros), the output layer an
+// 0.5 sigmoid, BF16-rounded weights; the two hidden layers run on the BF16 matrix cores
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation; K padded to 128 with zeros): the features and the
+// first hidden layer's activations enter them rounded to BF16 (nearest even), the output layer is an
 // FP32 dot product in input order. The network value (zero below mlp_floor, capped at alpha_max)
 // replaces the plain blender value unless they differ by more than mlp_tol.
-// Tiles in LDS, row-major as in HBM: X[cell][k], W1[hidden][k], H1[cell][hidden], W2[out][in],
-// H2[cell][hidden]; row strides of 132 and 68 floats put the 64 lanes of an operand read on every
-// bank exactly twice. Wavefront w owns 16 cells (row block w/2) x 32 hidden units (column tiles
-// 2(w%2), 2(w%2)+1). MFMA 16x16x4 operand layout: lane l holds A[l%16][l/16] and B[l/16][l%16];
+// Tiles in LDS, row-major as in HBM, BF16: X[cell][k], W1[hidden][k], H1[cell][hidden],
+// W2[out][in]; FP32: H2[cell][hidden]. Row strides of 136 and 72 BF16 (272 and 144 bytes) put the
+// 16 rows that one quarter-wave reads with 16-byte accesses on disjoint banks. Wavefront w owns 16
+// cells (row block w/2) x 32 hidden units (column tiles 2(w%2), 2(w%2)+1). MFMA 16x16x32 operand
+// layout: lane l holds 8 consecutive k of row l%16 of A and of column l%16 of B, k block l/16;
 // result register r of lane l is D[4(l/16)+r][l%16].
 // =============================================================================================
 constexpr int kMlpCells = 32;   // cells per workgroup
-constexpr int kMlpKS = 132;     // LDS row stride of the 128-wide (padded input) tiles
-constexpr int kMlpHS = 68;      // LDS row stride of the 64-wide tiles
+constexpr int kMlpKS = 136;     // LDS row stride (BF16) of the 128-wide (padded input) tiles
+constexpr int kMlpHS = 72;      // LDS row stride (BF16) of the 64-wide tiles
+constexpr int kMlpOS = 68;      // LDS row stride (FP32) of H2
 typedef float mlp_f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 mlp_b8 __attribute__((ext_vector_type(8)));
 __global__ void __launch_bounds__(256) k_mlp(KP p) {
-    __shared__ float sA[kMlpCells * kMlpKS];  // X, later H2 (stride kMlpHS)
-    __shared__ float sB[kMlpH * kMlpKS];      // W1, later W2 (stride kMlpHS)
-    __shared__ float sH[kMlpCells * kMlpHS];  // H1
+    __shared__ __align__(16) __bf16 sX[kMlpCells * kMlpKS];  // X; later H2 as FP32 (stride kMlpOS)
+    __shared__ __align__(16) __bf16 sW[kMlpH * kMlpKS];      // W1, later W2 (stride kMlpHS)
+    __shared__ __align__(16) __bf16 sH[kMlpCells * kMlpHS];  // H1
     __shared__ unsigned long long sCnt[2];
+    static_assert(sizeof(float) * kMlpCells * kMlpOS <= sizeof(__bf16) * kMlpCells * kMlpKS, "H2 over X");
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int c0 = blockIdx.x * kMlpCells;
     const int ncell = min(kMlpCells, p.nc - c0);
@@ -808,76 +813,82 @@ __global__ void __launch_bounds__(256) k_mlp(KP p) {
     // zero padding: input columns 125..127, and the rows of absent cells
     for (int e = tid; e < kMlpCells * 128; e += 256) {
         const int cl = e >> 7, k = e & 127;
-        if (k >= kMlpIn || cl >= ncell) sA[cl * kMlpKS + k] = 0.0f;
+        if (k >= kMlpIn || cl >= ncell) sX[cl * kMlpKS + k] = (__bf16)0.0f;
     }
     for (int e = tid; e < kMlpH * (128 - kMlpIn); e += 256)
-        sB[(e / (128 - kMlpIn)) * kMlpKS + kMlpIn + e % (128 - kMlpIn)] = 0.0f;
+        sW[(e / (128 - kMlpIn)) * kMlpKS + kMlpIn + e % (128 - kMlpIn)] = (__bf16)0.0f;
     for (int i = 0; i < kX4; ++i) {
         const int q = tid + 256 * i;
         if (q < nx4) {
             const float v4[4] = {xr[i].x, xr[i].y, xr[i].z, xr[i].w};
             for (int j = 0; j < 4; ++j) {
                 const int f = 4 * q + j, cl = f / kMlpIn;
-                sA[cl * kMlpKS + f - cl * kMlpIn] = v4[j];
+                sX[cl * kMlpKS + f - cl * kMlpIn] = (__bf16)v4[j];
             }
         }
     }
     for (int f = 4 * nx4 + tid; f < nx; f += 256) {  // features past the last whole float4
         const int cl = f / kMlpIn;
-        sA[cl * kMlpKS + f - cl * kMlpIn] = p.feat[(long long)c0 * kMlpIn + f];
+        sX[cl * kMlpKS + f - cl * kMlpIn] = (__bf16)p.feat[(long long)c0 * kMlpIn + f];
     }
-    for (int i = 0; i < kW14; ++i) {
+    for (int i = 0; i < kW14; ++i) {  // the weights are BF16 values already: exact
         const int q = tid + 256 * i;
         if (q >= nw14) break;
         const float v4[4] = {w1r[i].x, w1r[i].y, w1r[i].z, w1r[i].w};
         for (int j = 0; j < 4; ++j) {
             const int f = 4 * q + j, h = f / kMlpIn;
-            sB[h * kMlpKS + f - h * kMlpIn] = v4[j];
+            sW[h * kMlpKS + f - h * kMlpIn] = (__bf16)v4[j];
         }
     }
     __syncthreads();
     const int rb = wv >> 1, ct0 = 2 * (wv & 1);
     const int li = lane & 15, lk = lane >> 4;
     mlp_f4 acc[2];
-    // layer 1: H1 = gelu(X W1^T + b1)
+    // layer 1: H1 = gelu(X W1^T + b1), K = 128 in four steps of 32
     for (int t = 0; t < 2; ++t) {
         const float b = W[kMlpB1 + 16 * (ct0 + t) + li];
         acc[t] = mlp_f4{b, b, b, b};
     }
-    for (int s = 0; s < 32; ++s) {
-        const float a = sA[(16 * rb + li) * kMlpKS + 4 * s + lk];
+    for (int s = 0; s < 4; ++s) {
+        const mlp_b8 a = *reinterpret_cast<const mlp_b8*>(&sX[(16 * rb + li) * kMlpKS + 32 * s + 8 * lk]);
         for (int t = 0; t < 2; ++t) {
-            const float b = sB[(16 * (ct0 + t) + li) * kMlpKS + 4 * s + lk];
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[t], 0, 0, 0);
+            const mlp_b8 b = *reinterpret_cast<const mlp_b8*>(&sW[(16 * (ct0 + t) + li) * kMlpKS + 32 * s + 8 * lk]);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
         }
     }
     for (int t = 0; t < 2; ++t)
-        for (int r = 0; r < 4; ++r) sH[(16 * rb + 4 * lk + r) * kMlpHS + 16 * (ct0 + t) + li] = mlp_gelu(acc[t][r]);
+        for (int r = 0; r < 4; ++r)
+            sH[(16 * rb + 4 * lk + r) * kMlpHS + 16 * (ct0 + t) + li] = (__bf16)mlp_gelu(acc[t][r]);
     __syncthreads();  // H1 complete; X and W1 no longer read
-    for (int i = 0; i < kW24; ++i) {  // 4 floats of one row of W2
+    for (int i = 0; i < kW24; ++i) {  // 4 values of one row of W2
         const int f = 4 * (tid + 256 * i);
-        *reinterpret_cast<float4*>(&sB[(f >> 6) * kMlpHS + (f & 63)]) = w2r[i];
+        __bf16* d = &sW[(f >> 6) * kMlpHS + (f & 63)];
+        d[0] = (__bf16)w2r[i].x;
+        d[1] = (__bf16)w2r[i].y;
+        d[2] = (__bf16)w2r[i].z;
+        d[3] = (__bf16)w2r[i].w;
     }
     __syncthreads();
-    // layer 2: H2 = gelu(H1 W2^T + b2)
+    // layer 2: H2 = gelu(H1 W2^T + b2), K = 64 in two steps of 32
     for (int t = 0; t < 2; ++t) {
         const float b = W[kMlpB2 + 16 * (ct0 + t) + li];
         acc[t] = mlp_f4{b, b, b, b};
     }
-    for (int s = 0; s < 16; ++s) {
-        const float a = sH[(16 * rb + li) * kMlpHS + 4 * s + lk];
+    for (int s = 0; s < 2; ++s) {
+        const mlp_b8 a = *reinterpret_cast<const mlp_b8*>(&sH[(16 * rb + li) * kMlpHS + 32 * s + 8 * lk]);
         for (int t = 0; t < 2; ++t) {
-            const float b = sB[(16 * (ct0 + t) + li) * kMlpHS + 4 * s + lk];
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[t], 0, 0, 0);
+            const mlp_b8 b = *reinterpret_cast<const mlp_b8*>(&sW[(16 * (ct0 + t) + li) * kMlpHS + 32 * s + 8 * lk]);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
         }
     }
+    float* sO = reinterpret_cast<float*>(sX);  // H2 over X
     for (int t = 0; t < 2; ++t)
-        for (int r = 0; r < 4; ++r) sA[(16 * rb + 4 * lk + r) * kMlpHS + 16 * (ct0 + t) + li] = mlp_gelu(acc[t][r]);
+        for (int r = 0; r < 4; ++r) sO[(16 * rb + 4 * lk + r) * kMlpOS + 16 * (ct0 + t) + li] = mlp_gelu(acc[t][r]);
     __syncthreads();
     if (tid < ncell) {
         const int c = c0 + tid;
         float z = W[kMlpB3];
-        for (int k = 0; k < kMlpH; ++k) z = fmaf(sA[tid * kMlpHS + k], W[kMlpW3 + k], z);
+        for (int k = 0; k < kMlpH; ++k) z = fmaf(sO[tid * kMlpOS + k], W[kMlpW3 + k], z);
         double a = (double)(0.5f / (1.0f + expf(-z)));
         a = (a < p.mlp_floor) ? 0.0 : (a < p.amax ? a : p.amax);
         const double a_plain = p.araw[c];

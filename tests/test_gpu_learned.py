@@ -58,24 +58,37 @@ def test_adam_step_matches_reference():
 
 
 def test_inference_kernel_equals_network_on_features():
+    """the solver's inference kernel (BF16 matrix cores) equals the CPU reference of its arithmetic
+    (features and first-layer activations rounded to BF16, FP32 accumulation); a different summation
+    order can move a first-layer activation across a BF16 rounding boundary, so a few cells may
+    differ by more, and every cell stays within the BF16 rounding of the FP32 network"""
+    from oracle import mlp_ref
     from plens_amd import GpuPlens, shock_tube
-    from plens_amd.learned import MLP, init_params, samples, use_network
+    from plens_amd.learned import MLP, init_params, samples, unpack, use_network
     case = shock_tube(nx=16, ny=2, nz=2)
     g = GpuPlens(case)
     X, a_plain = samples(g)
     m = MLP(init_params(7))
-    y = m.predict(X).astype(np.float64)
     floor = 0.02
-    y[y < floor] = 0.0
-    y = np.minimum(y, case.physics.alpha_max)
+
+    def clip(y):
+        y = y.astype(np.float64)
+        y[y < floor] = 0.0
+        return np.minimum(y, case.physics.alpha_max)
+    y = clip(mlp_ref.infer(unpack(m.params()), X))
+    y32 = clip(m.predict(X))  # the FP32 network of the training kernels
     use_network(g, m, tolerance=1.0, floor=floor)  # never fall back
     X2, a_net = samples(g)
     assert np.array_equal(X, X2)
-    assert np.abs(a_net - y).max() < 1e-6
+    d = np.abs(a_net - y)
+    print(f"inference vs BF16 reference: max {d.max():.2e}, 99th percentile {np.quantile(d, 0.99):.2e}; "
+          f"vs FP32 network max {np.abs(a_net - y32).max():.2e}", flush=True)
+    assert np.quantile(d, 0.99) < 1e-6 and d.max() < 2e-3
+    assert np.abs(a_net - y32).max() < 1e-2
     use_network(g, m, tolerance=0.05, floor=floor)
     _, a_mix = samples(g)
-    fb = np.abs(y - a_plain) > 0.05
-    assert np.allclose(a_mix[fb], a_plain[fb]) and np.allclose(a_mix[~fb], y[~fb], atol=1e-6)
+    fb = np.abs(a_net - a_plain) > 0.05  # the fallback compares the kernel's own value
+    assert np.array_equal(a_mix[fb], a_plain[fb]) and np.array_equal(a_mix[~fb], a_net[~fb])
     use_network(g, None)
     _, a_back = samples(g)
     assert np.array_equal(a_back, a_plain)

@@ -64,3 +64,29 @@ def test_first_adam_step_is_signed_lr():
     # bias-corrected first step: m_hat = g, v_hat = g^2 -> lr * g / (|g| + eps)
     expect = p["a"] - 1e-3 * g["a"] / (np.abs(g["a"]) + 1e-8)
     assert np.allclose(q["a"], expect, rtol=0, atol=1e-7)
+
+
+def test_inference_arithmetic_rounds_the_matrix_operands_only():
+    """infer() is forward() with BF16 operands of the two hidden-layer products: rounding the
+    features first changes nothing, the result moves from the FP32 network by no more than BF16
+    rounding of those operands allows, and it does move (the rounding is applied)"""
+    rng = np.random.default_rng(3)
+    p = M.init_params(5)
+    p["b1"] = rng.normal(0, 0.1, p["b1"].shape).astype(np.float32)
+    X = rng.normal(0, 0.3, (256, M.IN)).astype(np.float32)
+    y = M.infer(p, X)
+    assert np.array_equal(y, M.infer(p, M.bf16(X)))
+    y32, _ = M.forward(p, X)
+    d = np.abs(y - y32)
+    assert d.max() < 5e-3 and d.max() > 0
+    # with a zero first layer the first-layer activations are gelu(b1) for every cell: layer 2 then
+    # sees exactly their BF16 rounding
+    Xb = M.bf16(X)
+    p0 = dict(p)
+    p0["W1"] = np.zeros_like(p["W1"])
+    a1 = M.gelu(p0["b1"])
+    y0 = M.infer(p0, Xb)
+    z2 = M.bf16(a1)[None, :] @ M.bf16(p0["W2"]).T + p0["b2"]
+    z3 = M.gelu(z2) @ M.bf16(p0["w3"]) + p0["b3"][0]
+    ref = np.float32(0.5) / (np.float32(1) + np.exp(-z3))
+    assert np.allclose(y0, ref[0], rtol=0, atol=1e-7)
