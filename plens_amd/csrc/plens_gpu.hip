@@ -3010,12 +3010,24 @@ int plens_gpu_set_expressions(plens_gpu* h, const char* courant, const char* wal
         if (!expr_compile(src[i], &codes[i], nullptr, 0)) return PLENS_E_INVALID;
         set |= 1 << i;
     }
-    if (set && !h->xcode) {
-        if (dmalloc(&h->xcode, 3) || dmalloc(&h->xval, 2) || dmalloc(&h->xreg, (size_t)h->nc)) return PLENS_E_NOMEM;
-        if (!h->centers.empty()) {
-            if (dmalloc(&h->ccen, h->centers.size())) return PLENS_E_NOMEM;
-            HIPCHK(hipMemcpy(h->ccen, h->centers.data(), sizeof(double) * h->centers.size(), hipMemcpyHostToDevice));
+    if (set && !h->xcode) {  // all four buffers or none (a failed allocation leaves none behind)
+        ExprCode* xc = nullptr;
+        double *xv = nullptr, *xr = nullptr, *cc = nullptr;
+        if (dmalloc(&xc, 3) || dmalloc(&xv, 2) || dmalloc(&xr, (size_t)h->nc) ||
+            (!h->centers.empty() && dmalloc(&cc, h->centers.size()))) {
+            for (void* q : {(void*)xc, (void*)xv, (void*)xr, (void*)cc})
+                if (q) (void)hipFree(q);
+            return PLENS_E_NOMEM;
         }
+        if (cc && hipMemcpy(cc, h->centers.data(), sizeof(double) * h->centers.size(), hipMemcpyHostToDevice) !=
+                      hipSuccess) {
+            for (void* q : {(void*)xc, (void*)xv, (void*)xr, (void*)cc}) (void)hipFree(q);
+            return PLENS_E_HIP;
+        }
+        h->xcode = xc;
+        h->xval = xv;
+        h->xreg = xr;
+        h->ccen = cc;
     }
     if (set) HIPCHK(hipMemcpy(h->xcode, codes, sizeof(codes), hipMemcpyHostToDevice));
     h->base.xset = set;
