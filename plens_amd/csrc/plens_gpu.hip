@@ -1114,7 +1114,9 @@ __global__ void __launch_bounds__(256) k_ss_reduce(const double* part, int nc, d
 template <int N, typename TF, bool EXT, bool HN>
 __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
-    const long long gid = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    // the tasks are walked from the last Morton position to the first: the auxiliary variables that
+    // k_grad (first to last) wrote last are read first, while still in the caches
+    const long long gid = (long long)(gridDim.x - 1 - xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
     if (gid >= (long long)p.ntasks * NF) return;
     const int t = (int)(gid / NF), q = (int)(gid - (long long)t * NF);
     const int4 td = p.tdesc[t];
