@@ -27,6 +27,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "DOF-stage-updates/s at N=4: plain FP64 vs mixed+learned, at equal L2 error"
 HBM_PEAK_GBS = 8000.0
+# compulsory bytes per DoF-stage of the two-pass algorithm with the RK epilogue fused (viscous,
+# FP64): Q read and write, R write, R' and R'' reads, the metric terms, 1/J, avars write and read
+STAGE_BYTES_SURVEY = 8 * (5 + 5 + 5 + 5 * 4 / 5 + 5 * 3 / 5 + 9 + 1 + 2 * 9)
 FP64_PEAK_TFS = 78.6   # MI355X vector (and matrix) FP64
 FP32_PEAK_TFS = 157.3  # MI355X vector FP32
 
@@ -144,6 +147,49 @@ def cpu_baseline(case, gpu_state_after_1):
                       f"OpenMP over cells",
             "one_core": {"value": case.mesh.n_dof * 5 / wall1, "cores": 1, "wall_s": wall1},
             "sections_s": sections}, err
+
+
+def algorithmic_flops(case, state):
+    """FP64 operations per DoF-stage of one RK4 step from `state`, counted by the flop-counting
+    build of the CPU oracle (oracle/flopcount.h: add, sub, mul, div, sqrt, log, exp, pow = 1 each;
+    the oracle has no FMA), single-threaded.
+
+    ref: the algorithm as the reference writes it (every ordered node pair of a line and the node
+    itself, one viscous flux per node pair, the dense change of basis, the subcell residual of every
+    cell). min: the minimal algorithm of the roofline, from the same counts: each unordered pair of
+    a line once plus the end-node diagonal terms (3 (N/2 + 2/(N+1)) pair fluxes per node instead of
+    3 (N+1)), priced at the measured flux cost plus the reference's per-pair overhead (direction
+    average, normalisation, weight, accumulation); one viscous flux per node and direction and a
+    (N+1)-point contraction of its 4 non-zero components; the sum-factorised change of basis
+    (3 (N+1) multiply-adds per node); the subcell residual only in blended cells. The BR1 lifting,
+    surface fluxes, blender, time step and update are counted as the reference writes them."""
+    from oracle.pyoracle import FLOP_BUCKETS, FLOPS_LIB_PATH, OraclePlens
+    N, n_dof, n_cells = case.mesh.degree, case.mesh.n_dof, case.mesh.n_cells
+    o = OraclePlens(case, lib_path=FLOPS_LIB_PATH)
+    o.set_state(state)
+    o.flop_counts(reset=True)
+    b0 = o.blended_cell_stages()
+    t0 = time.perf_counter()
+    o.update(case.courant)
+    wall = time.perf_counter() - t0
+    f, k = o.flop_counts(reset=True)
+    blended = (o.blended_cell_stages() - b0) / (5.0 * n_cells)
+    o.close()
+    ds = 5.0 * n_dof  # DoF-stages of the step
+    ref = float(f.sum()) / ds
+    pairs_ref, pairs_min = k[10] / ds, 3 * (N / 2 + 2 / (N + 1))
+    save_pairs = (pairs_ref - pairs_min) * (f[10] + f[4]) / k[10]
+    save_visc = (k[13] / ds - 3) * f[13] / k[13] + (f[5] / ds - 3 * (N + 1) * 4 * 2)
+    save_cob = f[11] / ds - 3 * (N + 1) * 2
+    save_lo = f[6] / ds * (1.0 - blended)
+    fmin = ref - save_pairs - save_visc - save_cob - save_lo
+    return {"ref_flop_per_dof_stage": ref, "min_flop_per_dof_stage": fmin,
+            "sections_ref": {name: float(f[i]) / ds for i, name in enumerate(FLOP_BUCKETS)},
+            "minimal_savings": {"unordered_pairs": save_pairs, "viscous_volume_contraction": save_visc,
+                                "sum_factorised_change_of_basis": save_cob, "subcell_residual_blended_only": save_lo},
+            "blended_cell_fraction": blended, "pair_fluxes_per_node_ref": float(pairs_ref),
+            "pair_fluxes_per_node_min": pairs_min, "count_wall_s": wall,
+            "sample": "1 RK4 step (5 stages) of the full workload from the timed run's final state, single thread"}
 
 
 def fp64_sensitivity(case, args, plain_state):
@@ -356,6 +402,19 @@ def main():
     cpu, parity = (None, None)
     if not args.no_cpu:
         cpu, parity = cpu_baseline(case, state1)
+        # the roofline of the whole path (SURVEY's per-DoF-stage bound): HBM time of the compulsory
+        # bytes against FP64 time of the minimal algorithm's operations, at the timed rate
+        af = algorithmic_flops(case, final_plain)
+        t_hbm = STAGE_BYTES_SURVEY / (HBM_PEAK_GBS * 1e9)
+        t_f64 = af["min_flop_per_dof_stage"] / (FP64_PEAK_TFS * 1e12)
+        t_meas = 1.0 / value
+        roofline["path"] = {"alg_flops": af, "bytes_per_dof_stage": STAGE_BYTES_SURVEY,
+                            "hbm_s_per_dof_stage": t_hbm, "f64_s_per_dof_stage": t_f64,
+                            "bound": "hbm" if t_hbm >= t_f64 else "f64",
+                            "bound_rate": 1.0 / max(t_hbm, t_f64), "frac": max(t_hbm, t_f64) / t_meas}
+        print(f"[bench] algorithmic flops per DoF-stage: reference {af['ref_flop_per_dof_stage']:.0f}, minimal "
+              f"{af['min_flop_per_dof_stage']:.0f}; path at {roofline['path']['frac']:.3f} of its "
+              f"{roofline['path']['bound']} bound", file=sys.stderr, flush=True)
     out = {
         "metric": METRIC, "value": value, "unit": "DOF-stage-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",

@@ -16,10 +16,25 @@
 #include <omp.h>
 #endif
 
+#ifdef ORACLE_COUNT_FLOPS
+#include "flopcount.h"  // Real counts its operations (liboracle_flops.so)
+#define FLOP_SCOPE(b) const flopcount::Scope flop_scope_(b)
+#define FLOP_BUCKET(b) (flopcount::g_bucket = (b))
+#else
+using Real = double;    // the oracle proper
+#define FLOP_SCOPE(b)
+#define FLOP_BUCKET(b)
+#endif
+// Flop buckets of the counting build: 0 positivity, 1 aux vars (lifting), 2 blender, 3 surface
+// fluxes, 4 ho inviscid residual, 5 ho viscous residual, 6 lo inviscid residual, 7 blend of the
+// residuals, 8 time step, 9 RK update; nested: 10 two-point volume fluxes (in 4), 11 dense change
+// of basis (in 2), 12 stage-2 numerical fluxes (in 3), 13 viscous fluxes of the ho viscous volume
+// term (in 5); 14 and 15 the surface parts of the ho inviscid and viscous residuals (after 4 and 5)
+
 namespace {
 
-using State = std::array<double, 5>;
-using Avars = std::array<double, 9>;
+using State = std::array<Real, 5>;
+using Avars = std::array<Real, 9>;
 
 // Planted mistakes (checker hook, oracle_set_mutation): each turns one operation of the time step,
 // the inflow/outflow/isothermal-wall ghosts or the blender post-steps into a plausible misreading
@@ -42,12 +57,12 @@ enum Mutation {
 };
 int g_mutation = kMutNone;
 struct T1 {  // rank-1 tensor in 3D (deal.II Tensor<1,3>)
-    double v[3] = {0, 0, 0};
-    double& operator[](int i) { return v[i]; }
-    double operator[](int i) const { return v[i]; }
-    double norm() const { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+    Real v[3] = {0, 0, 0};
+    Real& operator[](int i) { return v[i]; }
+    Real operator[](int i) const { return v[i]; }
+    Real norm() const { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
 };
-inline double dot(const T1& a, const T1& b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline Real dot(const T1& a, const T1& b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 inline T1 cross(const T1& a, const T1& b) {
     T1 c;
     c[0] = a[1] * b[2] - a[2] * b[1];
@@ -56,7 +71,7 @@ inline T1 cross(const T1& a, const T1& b) {
     return c;
 }
 
-constexpr double R0 = 8.314462618;  // modelling/navier_stokes.h:149
+constexpr Real R0 = 8.314462618;  // modelling/navier_stokes.h:149
 
 // ---------------------------------------------------------------------------------------------
 // Expressions of the varying inflow BC (IBCs/varying_inflow.cc): deal.II's FunctionParser over
@@ -69,7 +84,7 @@ constexpr double R0 = 8.314462618;  // modelling/navier_stokes.h:149
 struct ExprEval {
     const std::string& s;
     size_t i;
-    double x, y, z, t;
+    Real x, y, z, t;
     bool ok = true;
     void ws() { while (i < s.size() && std::isspace((unsigned char)s[i])) ++i; }
     bool eat(const char* tok) {
@@ -78,59 +93,59 @@ struct ExprEval {
         if (s.compare(i, n, tok) == 0) { i += n; return true; }
         return false;
     }
-    double expr() {  // ternary
-        const double c = lor();
+    Real expr() {  // ternary
+        const Real c = lor();
         if (eat("?")) {
-            const double a = expr();
+            const Real a = expr();
             if (!eat(":")) ok = false;
-            const double b = expr();
+            const Real b = expr();
             return c != 0 ? a : b;
         }
         return c;
     }
-    double lor() { double a = land(); while (eat("||")) { const double b = land(); a = (a != 0 || b != 0); } return a; }
-    double land() { double a = cmp(); while (eat("&&")) { const double b = cmp(); a = (a != 0 && b != 0); } return a; }
-    double cmp() {
-        double a = sum();
+    Real lor() { Real a = land(); while (eat("||")) { const Real b = land(); a = (a != 0 || b != 0); } return a; }
+    Real land() { Real a = cmp(); while (eat("&&")) { const Real b = cmp(); a = (a != 0 && b != 0); } return a; }
+    Real cmp() {
+        Real a = sum();
         for (;;) {
-            if (eat("<=")) { const double b = sum(); a = a <= b; }
-            else if (eat(">=")) { const double b = sum(); a = a >= b; }
-            else if (eat("==")) { const double b = sum(); a = a == b; }
-            else if (eat("!=")) { const double b = sum(); a = a != b; }
-            else if (eat("<")) { const double b = sum(); a = a < b; }
-            else if (eat(">")) { const double b = sum(); a = a > b; }
+            if (eat("<=")) { const Real b = sum(); a = a <= b; }
+            else if (eat(">=")) { const Real b = sum(); a = a >= b; }
+            else if (eat("==")) { const Real b = sum(); a = a == b; }
+            else if (eat("!=")) { const Real b = sum(); a = a != b; }
+            else if (eat("<")) { const Real b = sum(); a = a < b; }
+            else if (eat(">")) { const Real b = sum(); a = a > b; }
             else return a;
         }
     }
-    double sum() {
-        double a = prod();
+    Real sum() {
+        Real a = prod();
         for (;;) {
             if (eat("+")) a = a + prod();
             else if (eat("-")) a = a - prod();
             else return a;
         }
     }
-    double prod() {
-        double a = sign();
+    Real prod() {
+        Real a = sign();
         for (;;) {
             if (eat("*")) a = a * sign();
             else if (eat("/")) a = a / sign();
             else return a;
         }
     }
-    double sign() {
+    Real sign() {
         if (eat("-")) return -sign();
         if (eat("+")) return sign();
-        const double b = atom();
+        const Real b = atom();
         if (eat("^")) return std::pow(b, sign());
         return b;
     }
-    double atom() {
+    Real atom() {
         ws();
-        if (eat("(")) { const double v = expr(); if (!eat(")")) ok = false; return v; }
+        if (eat("(")) { const Real v = expr(); if (!eat(")")) ok = false; return v; }
         if (i < s.size() && (std::isdigit((unsigned char)s[i]) || s[i] == '.')) {
             size_t used = 0;
-            const double v = std::stod(s.substr(i), &used);
+            const Real v = std::stod(s.substr(i), &used);
             i += used;
             return v;
         }
@@ -139,7 +154,7 @@ struct ExprEval {
         const std::string id = s.substr(b, i - b);
         if (id.empty()) { ok = false; return 0; }
         if (eat("(")) {
-            std::vector<double> a;
+            std::vector<Real> a;
             ws();
             if (!eat(")")) {
                 do a.push_back(expr()); while (eat(","));
@@ -156,10 +171,10 @@ struct ExprEval {
         ok = false;
         return 0;
     }
-    double call(const std::string& f, const std::vector<double>& a) {
+    Real call(const std::string& f, const std::vector<Real>& a) {
         const size_t n = a.size();
-        auto one = [&](double v) { if (n != 1) ok = false; return v; };
-        const double a0 = n ? a[0] : 0;
+        auto one = [&](Real v) { if (n != 1) ok = false; return v; };
+        const Real a0 = n ? a[0] : Real(0);
         if (f == "sin") return one(std::sin(a0));
         if (f == "cos") return one(std::cos(a0));
         if (f == "tan") return one(std::tan(a0));
@@ -182,7 +197,7 @@ struct ExprEval {
         if (f == "abs") return one(std::fabs(a0));
         if (f == "floor") return one(std::floor(a0));
         if (f == "ceil") return one(std::ceil(a0));
-        if (f == "int") return one((double)(long long)a0);
+        if (f == "int") return one((Real)(long long)a0);
         if (f == "cot") return one(1 / std::tan(a0));
         if (f == "csc") return one(1 / std::sin(a0));
         if (f == "sec") return one(1 / std::cos(a0));
@@ -191,7 +206,7 @@ struct ExprEval {
         if (f == "atan2" && n == 2) return std::atan2(a[0], a[1]);
         if (f == "if" && n == 3) return a[0] != 0 ? a[1] : a[2];
         if (n >= 1 && (f == "min" || f == "max" || f == "sum" || f == "avg")) {
-            double r = a[0];
+            Real r = a[0];
             for (size_t k = 1; k < n; ++k) {
                 if (f == "min") r = std::min(r, a[k]);
                 else if (f == "max") r = std::max(r, a[k]);
@@ -205,11 +220,11 @@ struct ExprEval {
 };
 
 // value of expression `src` at (x, y, z, t); NaN on a syntax error
-double eval_expression(const std::string& src, double x, double y, double z, double t) {
+Real eval_expression(const std::string& src, Real x, Real y, Real z, Real t) {
     ExprEval e{src, 0, x, y, z, t};
-    const double v = e.expr();
+    const Real v = e.expr();
     e.ws();
-    return (e.ok && e.i == src.size()) ? v : std::nan("");
+    return (e.ok && e.i == src.size()) ? v : Real(std::nan(""));
 }
 
 // components of "a; b; c"
@@ -226,43 +241,43 @@ std::vector<std::string> split_components(const char* s) {
 // NavierStokes restatement (modelling/navier_stokes.{h,cc})
 // ---------------------------------------------------------------------------------------------
 struct NS {
-    double gma, M, Pr, mu0, T0, S;
+    Real gma, M, Pr, mu0, T0, S;
     int scheme;
-    double blender = 1;  // flux_blender_value (ctor default 1, navier_stokes.cc:33)
+    Real blender = 1;  // flux_blender_value (ctor default 1, navier_stokes.cc:33)
     int vscheme = 0;     // inv_vol_flux_scheme: 0 Chandrashekhar, 1 Kennedy-Gruber, 2 Ismail-Roe
 
-    double R() const { return R0 / M; }                       // navier_stokes.h:337
-    double cv() const { return R() / (gma - 1); }             // navier_stokes.h:342
-    double mu(double T) const {                               // navier_stokes.h:348-355
+    Real R() const { return R0 / M; }                       // navier_stokes.h:337
+    Real cv() const { return R() / (gma - 1); }             // navier_stokes.h:342
+    Real mu(Real T) const {                               // navier_stokes.h:348-355
         return mu0 * std::pow(T / T0, 1.5) * (T0 + S) / (T + S);
     }
-    double kappa(double m) const { return m * R() * gma / ((gma - 1) * Pr); }  // .h:360
+    Real kappa(Real m) const { return m * R() * gma / ((gma - 1) * Pr); }  // .h:360
     bool inviscid() const { return mu0 < 1e-16; }             // .h:438
-    void set_blender(double a) {                              // .h:443-448
+    void set_blender(Real a) {                              // .h:443-448
         if (a <= 0) blender = 0;
         else if (a <= 1) blender = a;
         else blender = 1;
     }
-    static double get_e(const State& c) {                     // navier_stokes.cc:349-359
-        double ske = 0;
+    static Real get_e(const State& c) {                     // navier_stokes.cc:349-359
+        Real ske = 0;
         for (int d = 0; d < 3; ++d) ske += c[1 + d] * c[1 + d];
-        const double ri = 1 / c[0];
+        const Real ri = 1 / c[0];
         ske *= 0.5 * ri;
         return (c[4] - ske) * ri;
     }
-    double get_p(const State& c) const {                      // navier_stokes.cc:375-384
-        double ske = 0;
+    Real get_p(const State& c) const {                      // navier_stokes.cc:375-384
+        Real ske = 0;
         for (int d = 0; d < 3; ++d) ske += c[1 + d] * c[1 + d];
         ske *= 0.5 / c[0];
         return (gma - 1) * (c[4] - ske);
     }
-    double get_a(const State& c) const { return std::sqrt(gma * get_p(c) / c[0]); }  // .cc:396-399
+    Real get_a(const State& c) const { return std::sqrt(gma * get_p(c) / c[0]); }  // .cc:396-399
     void cons_to_entropy(const State& c, State& w) const {    // navier_stokes.h:404-430
-        const double p = get_p(c);
-        const double s = std::log(p) - gma * std::log(c[0]);
-        const double beta = 0.5 * c[0] / p;
-        const double rhoinv = 1 / c[0];
-        double v[3], vv = 0;
+        const Real p = get_p(c);
+        const Real s = std::log(p) - gma * std::log(c[0]);
+        const Real beta = 0.5 * c[0] / p;
+        const Real rhoinv = 1 / c[0];
+        Real v[3], vv = 0;
         for (int d = 0; d < 3; ++d) {
             v[d] = c[1 + d] * rhoinv;
             vv += v[d] * v[d];
@@ -271,13 +286,13 @@ struct NS {
         for (int d = 0; d < 3; ++d) w[1 + d] = 2 * beta * v[d];
         w[4] = -2 * beta;
     }
-    double get_M(const State& c) const {                      // .cc:408-414
-        double s = 0;
-        const double ri = 1 / c[0];
+    Real get_M(const State& c) const {                      // .cc:408-414
+        Real s = 0;
+        const Real ri = 1 / c[0];
         for (int d = 0; d < 3; ++d) s += (c[1 + d] * c[1 + d]) * (ri * ri);
         return std::sqrt(s) / get_a(c);
     }
-    void prim_to_cons(double rho, const T1& vel, double p, State& c) const {  // .h:365-378
+    void prim_to_cons(Real rho, const T1& vel, Real p, State& c) const {  // .h:365-378
         c[0] = rho;
         c[4] = p / (gma - 1);
         for (int d = 0; d < 3; ++d) {
@@ -287,16 +302,16 @@ struct NS {
     }
     void get_inv_flux(const State& c, const T1& dir, State& f) const {  // .cc:424-442
         T1 vel;
-        const double ri = 1.0 / c[0];
+        const Real ri = 1.0 / c[0];
         for (int d = 0; d < 3; ++d) vel[d] = c[1 + d] * ri;
-        const double p = get_p(c);
-        const double vn = dot(vel, dir);
+        const Real p = get_p(c);
+        const Real vn = dot(vel, dir);
         f[0] = c[0] * vn;
         for (int d = 0; d < 3; ++d) f[1 + d] = vn * c[1 + d] + p * dir[d];
         f[4] = (c[4] + p) * vn;
     }
-    static double log_avg(double xl, double xr) {             // navier_stokes.h:494-501
-        const double xi = xl / xr, f = (xi - 1) / (xi + 1), u = f * f,
+    static Real log_avg(Real xl, Real xr) {             // navier_stokes.h:494-501
+        const Real xi = xl / xr, f = (xi - 1) / (xi + 1), u = f * f,
                      F = (u < 1e-2 ? 1 + u / 3 + u * u / 5 + u * u * u / 7 : 0.5 * std::log(xi) / f);
         return 0.5 * (xl + xr) / F;
     }
@@ -304,40 +319,40 @@ struct NS {
     // HLLC, navier_stokes.cc:936-1033 (pressure-based wave speeds, Toro 10.6)
     void hllc_xflux(const State& l, const State& r, State& f) const {
         T1 x; x[0] = 1;
-        const double pl = get_p(l), pr = get_p(r);
-        const double pmin = std::min(pl, pr), pmax = std::max(pl, pr);
-        const double ul = l[1] / l[0], ur = r[1] / r[0];
-        const double al = std::sqrt(gma * pl / l[0]), ar = std::sqrt(gma * pr / r[0]);
-        const double p_pvrs = 0.5 * (pl + pr) - 0.5 * (ur - ul) * 0.5 * (l[0] + r[0]) * 0.5 * (al + ar);
-        double ps;
+        const Real pl = get_p(l), pr = get_p(r);
+        const Real pmin = std::min(pl, pr), pmax = std::max(pl, pr);
+        const Real ul = l[1] / l[0], ur = r[1] / r[0];
+        const Real al = std::sqrt(gma * pl / l[0]), ar = std::sqrt(gma * pr / r[0]);
+        const Real p_pvrs = 0.5 * (pl + pr) - 0.5 * (ur - ul) * 0.5 * (l[0] + r[0]) * 0.5 * (al + ar);
+        Real ps;
         if (pmax / pmin < 2 && pmin <= p_pvrs && p_pvrs <= pmax) {
             ps = p_pvrs;
         } else if (p_pvrs < pmin) {
-            const double z = (gma - 1) / (2 * gma);
+            const Real z = (gma - 1) / (2 * gma);
             ps = std::pow(((al + ar) - (gma - 1) / 2 * (ur - ul)) /
                               (al / std::pow(pl, z) + ar / std::pow(pr, z)),
                           1 / z);
         } else {
-            const double p0 = std::max(0.0, p_pvrs);
-            const double gl = std::sqrt(2 / ((gma + 1) * l[0]) / (p0 + (gma - 1) / (gma + 1) * pl)),
+            const Real p0 = std::max(Real(0.0), p_pvrs);
+            const Real gl = std::sqrt(2 / ((gma + 1) * l[0]) / (p0 + (gma - 1) / (gma + 1) * pl)),
                          gr = std::sqrt(2 / ((gma + 1) * r[0]) / (p0 + (gma - 1) / (gma + 1) * pr));
             ps = (gl * pl + gr * pr - (ur - ul)) / (gl + gr);
         }
-        const double ql = (ps <= pl ? 1 : std::sqrt(1 + (gma + 1) / (2 * gma) * (ps / pl - 1))),
-                     qr = (ps <= pr ? 1 : std::sqrt(1 + (gma + 1) / (2 * gma) * (ps / pr - 1)));
-        const double sl = ul - al * ql, sr = ur + ar * qr;
-        const double s = (pr - pl + l[1] * (sl - ul) - r[1] * (sr - ur)) / (l[0] * (sl - ul) - r[0] * (sr - ur));
+        const Real ql = (ps <= pl ? Real(1) : std::sqrt(1 + (gma + 1) / (2 * gma) * (ps / pl - 1))),
+                     qr = (ps <= pr ? Real(1) : std::sqrt(1 + (gma + 1) / (2 * gma) * (ps / pr - 1)));
+        const Real sl = ul - al * ql, sr = ur + ar * qr;
+        const Real s = (pr - pl + l[1] * (sl - ul) - r[1] * (sr - ur)) / (l[0] * (sl - ul) - r[0] * (sr - ur));
         if (sl > 0) {
             get_inv_flux(l, x, f);
         } else if (s > 0) {
-            const double t = (sl - ul) / (sl - s);
+            const Real t = (sl - ul) / (sl - s);
             const State ss = {t * l[0], t * l[0] * s, t * l[2], t * l[3],
                               t * (l[4] + (s - ul) * (l[0] * s + pl / (sl - ul)))};
             State lf;
             get_inv_flux(l, x, lf);
             for (int v = 0; v < 5; ++v) f[v] = lf[v] + sl * (ss[v] - l[v]);
         } else if (sr > 0) {
-            const double t = (sr - ur) / (sr - s);
+            const Real t = (sr - ur) / (sr - s);
             const State ss = {t * r[0], t * r[0] * s, t * r[2], t * r[3],
                               t * (r[4] + (s - ur) * (r[0] * s + pr / (sr - ur)))};
             State rf;
@@ -351,52 +366,52 @@ struct NS {
     void rusanov_xflux(const State& l, const State& r, State& f) const {
         T1 x; x[0] = 1;
         State lf, rf;
-        const double al = get_a(l), ar = get_a(r);
-        const double ul = l[1] / l[0], ur = r[1] / r[0];
+        const Real al = get_a(l), ar = get_a(r);
+        const Real ul = l[1] / l[0], ur = r[1] / r[0];
         get_inv_flux(l, x, lf);
         get_inv_flux(r, x, rf);
-        const double S = std::max(std::fabs(ul) + al, std::fabs(ur) + ar);
+        const Real S = std::max(std::fabs(ul) + al, std::fabs(ur) + ar);
         for (int v = 0; v < 5; ++v) f[v] = 0.5 * (lf[v] + rf[v]) - 0.5 * S * (r[v] - l[v]);
     }
     // AUSM+-up, navier_stokes.cc:1070-1121 with modelling/ausm_functions.h; note the
     // integer division 3/16 == 0 of the reference (navier_stokes.cc:1090) is kept
-    static double m1p(double M) { return 0.5 * (M + std::fabs(M)); }
-    static double m1n(double M) { return 0.5 * (M - std::fabs(M)); }
-    static double m2p(double M) { return 0.25 * (M + 1) * (M + 1); }
-    static double m2n(double M) { return -0.25 * (M - 1) * (M - 1); }
-    static double m4p(double M) { return std::fabs(M) >= 1 ? m1p(M) : m2p(M) * (1 - 16 * 0.125 * m2n(M)); }
-    static double m4n(double M) { return std::fabs(M) >= 1 ? m1n(M) : m2n(M) * (1 + 16 * 0.125 * m2p(M)); }
-    static double p5p(double M, double a) {
+    static Real m1p(Real M) { return 0.5 * (M + std::fabs(M)); }
+    static Real m1n(Real M) { return 0.5 * (M - std::fabs(M)); }
+    static Real m2p(Real M) { return 0.25 * (M + 1) * (M + 1); }
+    static Real m2n(Real M) { return -0.25 * (M - 1) * (M - 1); }
+    static Real m4p(Real M) { return std::fabs(M) >= 1 ? m1p(M) : m2p(M) * (1 - 16 * 0.125 * m2n(M)); }
+    static Real m4n(Real M) { return std::fabs(M) >= 1 ? m1n(M) : m2n(M) * (1 + 16 * 0.125 * m2p(M)); }
+    static Real p5p(Real M, Real a) {
         return std::fabs(M) >= 1 ? m1p(M) / M : m2p(M) * ((2 - M) - 16 * a * M * m2n(M));
     }
-    static double p5n(double M, double a) {
+    static Real p5n(Real M, Real a) {
         return std::fabs(M) >= 1 ? m1n(M) / M : m2n(M) * ((-2 - M) + 16 * a * M * m2p(M));
     }
     void ausm_xflux(const State& l, const State& r, State& f) const {
-        const double Minf = 0.3, Kp = 0.25, Ku = 0.75, sigma = 1.0;
-        const double pl = get_p(l), pr = get_p(r);
+        const Real Minf = 0.3, Kp = 0.25, Ku = 0.75, sigma = 1.0;
+        const Real pl = get_p(l), pr = get_p(r);
         T1 vl, vr;
         for (int d = 0; d < 3; ++d) { vl[d] = l[1 + d] / l[0]; vr[d] = r[1 + d] / r[0]; }
-        const double Hl = (l[4] + pl) / l[0], Hr = (r[4] + pr) / r[0];
-        const double a12 = 0.5 * (get_a(l) + get_a(r));
-        const double Ml = vl[0] / a12, Mr = vr[0] / a12;
-        const double Mavgsq = 0.5 * (Ml * Ml + Mr * Mr);
-        const double Mo = std::sqrt(std::min(1.0, std::max(Mavgsq, Minf * Minf)));
-        const double fa = Mo * (2 - Mo), alpha = 3 / 16 * (-4 + 5 * fa * fa);
-        const double M12 = m4p(Ml) + m4n(Mr) -
-                           Kp / fa * std::max(1 - sigma * Mavgsq, 0.0) * 2 * (pr - pl) / (a12 * a12 * (l[0] + r[0]));
-        const double psp = p5p(Ml, alpha), psn = p5n(Mr, alpha);
-        const double p12 = psp * pl + psn * pr - Ku * fa * psp * psn * (l[0] + r[0]) * a12 * (vr[0] - vl[0]);
+        const Real Hl = (l[4] + pl) / l[0], Hr = (r[4] + pr) / r[0];
+        const Real a12 = 0.5 * (get_a(l) + get_a(r));
+        const Real Ml = vl[0] / a12, Mr = vr[0] / a12;
+        const Real Mavgsq = 0.5 * (Ml * Ml + Mr * Mr);
+        const Real Mo = std::sqrt(std::min(Real(1.0), std::max(Mavgsq, Minf * Minf)));
+        const Real fa = Mo * (2 - Mo), alpha = 3 / 16 * (-4 + 5 * fa * fa);
+        const Real M12 = m4p(Ml) + m4n(Mr) -
+                           Kp / fa * std::max(1 - sigma * Mavgsq, Real(0.0)) * 2 * (pr - pl) / (a12 * a12 * (l[0] + r[0]));
+        const Real psp = p5p(Ml, alpha), psn = p5n(Mr, alpha);
+        const Real p12 = psp * pl + psn * pr - Ku * fa * psp * psn * (l[0] + r[0]) * a12 * (vr[0] - vl[0]);
         if (M12 > 0) {
-            const double m12 = a12 * M12 * l[0];
+            const Real m12 = a12 * M12 * l[0];
             f[0] = m12; f[1] = m12 * vl[0] + p12; f[2] = m12 * vl[1]; f[3] = m12 * vl[2]; f[4] = m12 * Hl;
         } else {
-            const double m12 = a12 * M12 * r[0];
+            const Real m12 = a12 * M12 * r[0];
             f[0] = m12; f[1] = m12 * vr[0] + p12; f[2] = m12 * vr[1]; f[3] = m12 * vr[2]; f[4] = m12 * Hr;
         }
     }
     // x eigenvector matrices, navier_stokes.cc:641-678 and 780-825
-    void get_xK(const T1& vel, double a, double H, double K[5][5]) const {
+    void get_xK(const T1& vel, Real a, Real H, Real K[5][5]) const {
         K[0][0] = 1;
         for (int d = 0; d < 3; ++d) K[1 + d][0] = vel[d];
         K[1][0] -= a;
@@ -411,8 +426,8 @@ struct NS {
         K[1][4] += a;
         K[4][4] = H + vel[0] * a;
     }
-    void get_xKinv(const T1& vel, double a, double H, double Ki[5][5]) const {
-        const double t = 1 / (gma - 1);
+    void get_xKinv(const T1& vel, Real a, Real H, Real Ki[5][5]) const {
+        const Real t = 1 / (gma - 1);
         Ki[0][0] = H + a * (vel[0] - a) * t;
         for (int d = 0; d < 3; ++d) Ki[0][1 + d] = -vel[d];
         Ki[0][1] -= a * t;
@@ -426,43 +441,43 @@ struct NS {
         for (int d = 0; d < 3; ++d) Ki[4][1 + d] = -vel[d];
         Ki[4][1] += a * t;
         Ki[4][4] = 1;
-        const double fac = 0.5 * (gma - 1) / (a * a);
+        const Real fac = 0.5 * (gma - 1) / (a * a);
         for (int i = 0; i < 5; ++i)
             for (int j = 0; j < 5; ++j) Ki[i][j] *= fac;
     }
     // Chandrashekhar EC flux with hybrid matrix dissipation, navier_stokes.cc:1250-1318
     void chandrashekhar_xflux(const State& l, const State& r, State& f) const {
-        const double pl = get_p(l), pr = get_p(r);
-        const double bl = 0.5 * l[0] / pl, br = 0.5 * r[0] / pr, bln = log_avg(bl, br);
-        const double p_hat = 0.5 * (l[0] + r[0]) / (bl + br);
-        const double rho_ln = log_avg(l[0], r[0]);
+        const Real pl = get_p(l), pr = get_p(r);
+        const Real bl = 0.5 * l[0] / pl, br = 0.5 * r[0] / pr, bln = log_avg(bl, br);
+        const Real p_hat = 0.5 * (l[0] + r[0]) / (bl + br);
+        const Real rho_ln = log_avg(l[0], r[0]);
         T1 va, vsq, vl, vr;
-        const double ril = 1 / l[0], rir = 1 / r[0];
+        const Real ril = 1 / l[0], rir = 1 / r[0];
         for (int d = 0; d < 3; ++d) {
             vl[d] = l[1 + d] * ril;
             vr[d] = r[1 + d] * rir;
             va[d] = 0.5 * (vl[d] + vr[d]);
             vsq[d] = 0.5 * (vl[d] * vl[d] + vr[d] * vr[d]);
         }
-        double H_hat = 0.5 / ((gma - 1) * bln) + p_hat / rho_ln;
+        Real H_hat = 0.5 / ((gma - 1) * bln) + p_hat / rho_ln;
         for (int d = 0; d < 3; ++d) H_hat += va[d] * va[d] - 0.5 * vsq[d];
         f[0] = rho_ln * va[0];
         f[1] = rho_ln * va[0] * va[0] + p_hat;
         f[2] = rho_ln * va[0] * va[1];
         f[3] = rho_ln * va[0] * va[2];
         f[4] = rho_ln * va[0] * H_hat;
-        const double sql = std::sqrt(l[0]), sqr = std::sqrt(r[0]);
+        const Real sql = std::sqrt(l[0]), sqr = std::sqrt(r[0]);
         T1 vi;
-        const double inv = 1 / (sql + sqr);
+        const Real inv = 1 / (sql + sqr);
         for (int d = 0; d < 3; ++d) vi[d] = inv * (sql * vl[d] + sqr * vr[d]);
-        const double Hi = ((l[4] + pl) / sql + (r[4] + pr) / sqr) / (sql + sqr);
-        const double ai = std::sqrt((gma - 1) * (Hi - 0.5 * dot(vi, vi)));
-        double K[5][5], Ki[5][5], A[5][5];
+        const Real Hi = ((l[4] + pl) / sql + (r[4] + pr) / sqr) / (sql + sqr);
+        const Real ai = std::sqrt((gma - 1) * (Hi - 0.5 * dot(vi, vi)));
+        Real K[5][5], Ki[5][5], A[5][5];
         get_xK(vi, ai, Hi, K);
         get_xKinv(vi, ai, Hi, Ki);
-        const double uabs = std::fabs(vi[0]), lmax = uabs + ai,
+        const Real uabs = std::fabs(vi[0]), lmax = uabs + ai,
                      l2 = blender * lmax + (1 - blender) * uabs;
-        const double eig[5] = {lmax, l2, l2, l2, lmax};
+        const Real eig[5] = {lmax, l2, l2, l2, lmax};
         for (int i = 0; i < 5; ++i)
             for (int j = 0; j < 5; ++j) {
                 A[i][j] = 0;
@@ -482,33 +497,33 @@ struct NS {
     }
     // modified Steger-Warming, navier_stokes.cc:1174-1236 (pos_smooth/neg_smooth, .h:464-476)
     void modified_sw_xflux(const State& l, const State& r, State& f) const {
-        const double pl = get_p(l), pr = get_p(r);
-        const double q = 0.5 * (pr - pl) / std::min(pl, pr);
-        const double om = 0.5 / (1 + q * q);
+        const Real pl = get_p(l), pr = get_p(r);
+        const Real q = 0.5 * (pr - pl) / std::min(pl, pr);
+        const Real om = 0.5 / (1 + q * q);
         T1 vp, vn;
         for (int d = 0; d < 3; ++d) {
             vp[d] = (1 - om) * l[1 + d] / l[0] + om * r[1 + d] / r[0];
             vn[d] = (1 - om) * r[1 + d] / r[0] + om * l[1 + d] / l[0];
         }
-        const double rp = (1 - om) * l[0] + om * r[0], rn = (1 - om) * r[0] + om * l[0];
-        const double pp = (1 - om) * pl + om * pr, pn = (1 - om) * pr + om * pl;
-        const double ap = std::sqrt(gma * pp / rp), an = std::sqrt(gma * pn / rn);
-        const double Hp = gma * pp / ((gma - 1) * rp) + 0.5 * dot(vp, vp),
+        const Real rp = (1 - om) * l[0] + om * r[0], rn = (1 - om) * r[0] + om * l[0];
+        const Real pp = (1 - om) * pl + om * pr, pn = (1 - om) * pr + om * pl;
+        const Real ap = std::sqrt(gma * pp / rp), an = std::sqrt(gma * pn / rn);
+        const Real Hp = gma * pp / ((gma - 1) * rp) + 0.5 * dot(vp, vp),
                      Hn = gma * pn / ((gma - 1) * rn) + 0.5 * dot(vn, vn);
-        double Kp[5][5], Kip[5][5], Kn[5][5], Kin[5][5];
+        Real Kp[5][5], Kip[5][5], Kn[5][5], Kin[5][5];
         get_xK(vp, ap, Hp, Kp);
         get_xK(vn, an, Hn, Kn);
         get_xKinv(vp, ap, Hp, Kip);
         get_xKinv(vn, an, Hn, Kin);
-        auto ps = [](double x, double e) { return 0.5 * (x + std::sqrt(x * x + e * e)); };
-        auto ng = [](double x, double e) { return 0.5 * (x - std::sqrt(x * x + e * e)); };
-        const double ep = 0.3 * ap, en = 0.3 * an;
-        const double lp[5] = {ps(vp[0] - ap, ep), ps(vp[0], ep), ps(vp[0], ep), ps(vp[0], ep), ps(vp[0] + ap, ep)};
-        const double ln[5] = {ng(vn[0] - an, en), ng(vn[0], en), ng(vn[0], en), ng(vn[0], en), ng(vn[0] + an, en)};
+        auto ps = [](Real x, Real e) { return 0.5 * (x + std::sqrt(x * x + e * e)); };
+        auto ng = [](Real x, Real e) { return 0.5 * (x - std::sqrt(x * x + e * e)); };
+        const Real ep = 0.3 * ap, en = 0.3 * an;
+        const Real lp[5] = {ps(vp[0] - ap, ep), ps(vp[0], ep), ps(vp[0], ep), ps(vp[0], ep), ps(vp[0] + ap, ep)};
+        const Real ln[5] = {ng(vn[0] - an, en), ng(vn[0], en), ng(vn[0], en), ng(vn[0], en), ng(vn[0] + an, en)};
         for (int i = 0; i < 5; ++i) {
             f[i] = 0;
             for (int j = 0; j < 5; ++j) {
-                double Ap = 0, An = 0;
+                Real Ap = 0, An = 0;
                 for (int k = 0; k < 5; ++k) {
                     Ap += Kp[i][k] * lp[k] * Kip[k][j];
                     An += Kn[i][k] * ln[k] * Kin[k][j];
@@ -521,15 +536,15 @@ struct NS {
     void kennedy_gruber_xflux(const State& l, const State& r, State& f) const {
         T1 x; x[0] = 1;
         kennedy_gruber_vol_flux(l, r, x, f);
-        const double el = get_e(l), er = get_e(r);
-        const double lm = std::max(std::fabs(l[1] / l[0]) + std::sqrt(gma * (gma - 1) * el),
+        const Real el = get_e(l), er = get_e(r);
+        const Real lm = std::max(std::fabs(l[1] / l[0]) + std::sqrt(gma * (gma - 1) * el),
                                    std::fabs(r[1] / r[0]) + std::sqrt(gma * (gma - 1) * er));
         for (int v = 0; v < 5; ++v) f[v] -= 0.5 * lm * (r[v] - l[v]);
     }
     // Ismail-Roe with entropy-variable matrix dissipation, navier_stokes.cc:1381-1446
     void ismail_roe_xflux(const State& l, const State& r, State& f) const {
         T1 x; x[0] = 1;
-        double rh, vh[3], p2, hh;
+        Real rh, vh[3], p2, hh;
         ismail_roe_means(l, r, rh, vh, p2, hh);
         ismail_roe_vol_flux(l, r, x, f);
         State ch;
@@ -539,7 +554,7 @@ struct NS {
         State Wl, Wr;
         cons_to_entropy(l, Wl);
         cons_to_entropy(r, Wr);
-        double H[5][5];
+        Real H[5][5];
         for (int i = 0; i < 5; ++i) H[i][0] = ch[i];
         for (int d = 0; d < 3; ++d) {
             T1 e; e[d] = 1;
@@ -549,9 +564,9 @@ struct NS {
         }
         for (int i = 0; i < 4; ++i) H[i][4] = H[4][i];
         H[4][4] = rh * hh * hh - gma * p2 * p2 / (rh * (gma - 1));
-        const double lm = std::fabs(vh[0]) + std::sqrt(gma * p2 / rh);
+        const Real lm = std::fabs(vh[0]) + std::sqrt(gma * p2 / rh);
         for (int i = 0; i < 5; ++i) {
-            double s = 0;
+            Real s = 0;
             for (int j = 0; j < 5; ++j) s += H[i][j] * (Wr[j] - Wl[j]);
             f[i] -= 0.5 * lm * s;
         }
@@ -574,23 +589,23 @@ struct NS {
     void get_inv_surf_flux(const State& o, const State& nb, const T1& n, State& f) const {
         T1 x; x[0] = 1;
         T1 m = cross(x, n);
-        const double Mm = m.norm();
-        double R[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        const Real Mm = m.norm();
+        Real R[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         if (Mm > 1e-3) {
             for (int d = 0; d < 3; ++d) m[d] /= Mm;
-            const double c = dot(x, n), t = 1 - c, s = std::sqrt(1 - c * c);
+            const Real c = dot(x, n), t = 1 - c, s = std::sqrt(1 - c * c);
             R[0][0] = t * m[0] * m[0] + c;     R[0][1] = t * m[0] * m[1] - s * m[2]; R[0][2] = t * m[0] * m[2] + s * m[1];
             R[1][0] = t * m[0] * m[1] + s * m[2]; R[1][1] = t * m[1] * m[1] + c;     R[1][2] = t * m[1] * m[2] - s * m[0];
             R[2][0] = t * m[0] * m[2] - s * m[1]; R[2][1] = t * m[1] * m[2] + s * m[0]; R[2][2] = t * m[2] * m[2] + c;
         } else {
-            const double dg = n[0] < 0 ? -1 : 1;
+            const Real dg = n[0] < 0 ? -1 : 1;
             for (int d = 0; d < 3; ++d) R[d][d] = dg;
         }
         T1 om, nm, omr, nmr;
         for (int d = 0; d < 3; ++d) { om[d] = o[1 + d]; nm[d] = nb[1 + d]; }
         for (int row = 0; row < 3; ++row)
             for (int col = 0; col < 3; ++col) {
-                const double RT = R[col][row];
+                const Real RT = R[col][row];
                 omr[row] += RT * om[col];
                 nmr[row] += RT * nm[col];
             }
@@ -609,6 +624,7 @@ struct NS {
         f[4] = fr[4];
     }
     void vol_flux(const State& c1, const State& c2, const T1& dir, State& f) const {  // .cc:203-228
+        FLOP_SCOPE(10);
         if (vscheme == 1) kennedy_gruber_vol_flux(c1, c2, dir, f);
         else if (vscheme == 2) ismail_roe_vol_flux(c1, c2, dir, f);
         else chandrashekhar_vol_flux(c1, c2, dir, f);
@@ -621,23 +637,23 @@ struct NS {
             v2[d] = c2[1 + d] / c2[0];
             va[d] = 0.5 * (v1[d] + v2[d]);
         }
-        const double ra = 0.5 * (c1[0] + c2[0]);
-        const double e1 = get_e(c1), e2 = get_e(c2);
-        const double Ea = 0.5 * (e1 + e2 + 0.5 * (dot(v1, v1) + dot(v2, v2)));
-        const double pa = (gma - 1) * 0.5 * (c1[0] * e1 + c2[0] * e2);
-        const double vn = dot(va, dir);
+        const Real ra = 0.5 * (c1[0] + c2[0]);
+        const Real e1 = get_e(c1), e2 = get_e(c2);
+        const Real Ea = 0.5 * (e1 + e2 + 0.5 * (dot(v1, v1) + dot(v2, v2)));
+        const Real pa = (gma - 1) * 0.5 * (c1[0] * e1 + c2[0] * e2);
+        const Real vn = dot(va, dir);
         f[0] = vn * ra;
         for (int d = 0; d < 3; ++d) f[1 + d] = ra * vn * va[d] + pa * dir[d];
         f[4] = vn * (ra * Ea + pa);
     }
     // Ismail-Roe means through z = sqrt(rho/p) (1, v, p), navier_stokes.cc:1590-1610
-    void ismail_roe_means(const State& c1, const State& c2, double& rh, double vh[3], double& p2,
-                          double& hh) const {
-        const double a1 = get_a(c1), a2 = get_a(c2), g = std::sqrt(gma);
-        const double z11 = g / a1, z12 = g / a2, z51 = c1[0] * a1 / g, z52 = c2[0] * a2 / g;
-        const double z5ln = log_avg(z51, z52);
+    void ismail_roe_means(const State& c1, const State& c2, Real& rh, Real vh[3], Real& p2,
+                          Real& hh) const {
+        const Real a1 = get_a(c1), a2 = get_a(c2), g = std::sqrt(gma);
+        const Real z11 = g / a1, z12 = g / a2, z51 = c1[0] * a1 / g, z52 = c2[0] * a2 / g;
+        const Real z5ln = log_avg(z51, z52);
         rh = 0.5 * (z11 + z12) * z5ln;
-        const double p1 = (z51 + z52) / (z11 + z12);
+        const Real p1 = (z51 + z52) / (z11 + z12);
         p2 = 0.5 * ((gma + 1) * z5ln / log_avg(z11, z12) + (gma - 1) * p1) / gma;
         hh = gma * p2 / ((gma - 1) * rh);
         for (int d = 0; d < 3; ++d) {
@@ -647,38 +663,38 @@ struct NS {
     }
     // Ismail-Roe volume flux, navier_stokes.cc:1590-1622
     void ismail_roe_vol_flux(const State& c1, const State& c2, const T1& dir, State& f) const {
-        const double a1 = get_a(c1), a2 = get_a(c2), g = std::sqrt(gma);
-        const double p1 = (c1[0] * a1 / g + c2[0] * a2 / g) / (g / a1 + g / a2);
-        double rh, vh[3], p2, hh;
+        const Real a1 = get_a(c1), a2 = get_a(c2), g = std::sqrt(gma);
+        const Real p1 = (c1[0] * a1 / g + c2[0] * a2 / g) / (g / a1 + g / a2);
+        Real rh, vh[3], p2, hh;
         ismail_roe_means(c1, c2, rh, vh, p2, hh);
-        const double vn = vh[0] * dir[0] + vh[1] * dir[1] + vh[2] * dir[2];
+        const Real vn = vh[0] * dir[0] + vh[1] * dir[1] + vh[2] * dir[2];
         f[0] = rh * vn;
         for (int d = 0; d < 3; ++d) f[1 + d] = rh * vh[d] * vn + p1 * dir[d];
         f[4] = rh * vn * hh;
     }
     // Chandrashekhar volume flux, navier_stokes.cc:1510-1542
     void chandrashekhar_vol_flux(const State& c1, const State& c2, const T1& dir, State& f) const {
-        const double p1 = get_p(c1), p2 = get_p(c2);
-        const double b1 = 0.5 * c1[0] / p1, b2 = 0.5 * c2[0] / p2, bln = log_avg(b1, b2);
-        const double p_hat = 0.5 * (c1[0] + c2[0]) / (b1 + b2);
-        const double rho_ln = log_avg(c1[0], c2[0]);
+        const Real p1 = get_p(c1), p2 = get_p(c2);
+        const Real b1 = 0.5 * c1[0] / p1, b2 = 0.5 * c2[0] / p2, bln = log_avg(b1, b2);
+        const Real p_hat = 0.5 * (c1[0] + c2[0]) / (b1 + b2);
+        const Real rho_ln = log_avg(c1[0], c2[0]);
         T1 va, vsq;
-        const double r1 = 1 / c1[0], r2 = 1 / c2[0];
+        const Real r1 = 1 / c1[0], r2 = 1 / c2[0];
         for (int d = 0; d < 3; ++d) {
-            const double v1 = c1[1 + d] * r1, v2 = c2[1 + d] * r2;
+            const Real v1 = c1[1 + d] * r1, v2 = c2[1 + d] * r2;
             va[d] = 0.5 * (v1 + v2);
             vsq[d] = 0.5 * (v1 * v1 + v2 * v2);
         }
-        double H_hat = 0.5 / ((gma - 1) * bln) + p_hat / rho_ln;
+        Real H_hat = 0.5 / ((gma - 1) * bln) + p_hat / rho_ln;
         for (int d = 0; d < 3; ++d) H_hat += va[d] * va[d] - 0.5 * vsq[d];
-        const double vn = dot(va, dir);
+        const Real vn = dot(va, dir);
         f[0] = vn * rho_ln;
         for (int d = 0; d < 3; ++d) f[1 + d] = rho_ln * vn * va[d] + p_hat * dir[d];
         f[4] = rho_ln * vn * H_hat;
     }
     // diffusive flux, navier_stokes.cc:599-624 (stress tensor from avars, .cc:578-589)
     static void dif_flux(const State& c, const Avars& av, const T1& dir, State& f) {
-        double st[3][3];
+        Real st[3][3];
         int i = 0;
         for (int row = 0; row < 3; ++row)
             for (int col = row; col < 3; ++col) {
@@ -691,9 +707,9 @@ struct NS {
             for (int b = 0; b < 3; ++b) mf[a] += st[a][b] * dir[b];
         f[0] = 0;
         f[4] = 0;
-        const double ri = 1.0 / c[0];
+        const Real ri = 1.0 / c[0];
         for (int d = 0; d < 3; ++d) {
-            const double v = c[1 + d] * ri;
+            const Real v = c[1 + d] * ri;
             f[1 + d] = mf[d];
             f[4] += -av[6 + d] * dir[d] + v * mf[d];
         }
@@ -702,7 +718,7 @@ struct NS {
 
 // ---------------------------------------------------------------------------------------------
 // 1D bases (QGaussLobatto / FE_DGQ<1> shape_grad / Legendre / QGauss), computed here in long
-// double by Newton on P_N' using the Legendre ODE for P_N''.
+// Real by Newton on P_N' using the Legendre ODE for P_N''.
 // ---------------------------------------------------------------------------------------------
 void leg(int n, long double x, long double& P, long double& dP) {
     long double p0 = 1, p1 = x;
@@ -723,7 +739,7 @@ long double leg_val(int n, long double x) {
     }
     return p1;
 }
-void gll(int N, std::vector<double>& x01, std::vector<double>& w01) {
+void gll(int N, std::vector<Real>& x01, std::vector<Real>& w01) {
     std::vector<long double> x(N + 1);
     x[0] = -1; x[N] = 1;
     const long double pi = 3.14159265358979323846264338327950288L;
@@ -742,12 +758,12 @@ void gll(int N, std::vector<double>& x01, std::vector<double>& w01) {
     x01.resize(N + 1); w01.resize(N + 1);
     for (int j = 0; j <= N; ++j) {
         const long double P = leg_val(N, x[j]);
-        x01[j] = (double)((x[j] + 1) / 2);
-        w01[j] = (double)(1.0L / (N * (N + 1) * P * P));
+        x01[j] = (Real)((x[j] + 1) / 2);
+        w01[j] = (Real)(1.0L / (N * (N + 1) * P * P));
     }
 }
 // FE_DGQ<1>::shape_grad(j, x_i): derivative of the Lagrange polynomial l_j at x_i
-double lagrange_deriv(const std::vector<double>& xn, int j, long double x) {
+Real lagrange_deriv(const std::vector<Real>& xn, int j, long double x) {
     const int np = (int)xn.size();
     long double s = 0;
     for (int k = 0; k < np; ++k) {
@@ -757,15 +773,15 @@ double lagrange_deriv(const std::vector<double>& xn, int j, long double x) {
             if (m != j && m != k) p *= (x - xn[m]) / ((long double)xn[j] - xn[m]);
         s += p;
     }
-    return (double)s;
+    return (Real)s;
 }
-long double lagrange_val(const std::vector<double>& xn, int j, long double x) {
+long double lagrange_val(const std::vector<Real>& xn, int j, long double x) {
     long double p = 1;
     for (int m = 0; m < (int)xn.size(); ++m)
         if (m != j) p *= (x - xn[m]) / ((long double)xn[j] - xn[m]);
     return p;
 }
-void gauss(int m, std::vector<double>& xg, std::vector<double>& wg) {
+void gauss(int m, std::vector<Real>& xg, std::vector<Real>& wg) {
     xg.resize(m); wg.resize(m);
     const long double pi = 3.14159265358979323846264338327950288L;
     for (int j = 0; j < m; ++j) {
@@ -777,34 +793,34 @@ void gauss(int m, std::vector<double>& xg, std::vector<double>& wg) {
             if (fabsl(dt) < 1e-32L) break;
         }
         leg(m, t, P, dP);
-        xg[j] = (double)((t + 1) / 2);
-        wg[j] = (double)(1.0L / ((1 - t * t) * dP * dP));
+        xg[j] = (Real)((t + 1) / 2);
+        wg[j] = (Real)(1.0L / ((1 - t * t) * dP * dP));
     }
 }
 // ChangeOfBasisMatrix ctor (dgsem/change_of_basis_matrix.h:74-98): C_ij = sum_q w_q psi_i(p_q) phi_j(p_q)
 // over QGauss<3>(N+1) with FE_DGQLegendre (orthonormal Legendre on [0,1]) and FE_DGQ (GLL Lagrange).
-void cob_matrix(int N, std::vector<double>& C) {
-    std::vector<double> xn, wn, xg, wg;
+void cob_matrix(int N, std::vector<Real>& C) {
+    std::vector<Real> xn, wn, xg, wg;
     gll(N, xn, wn);
     gauss(N + 1, xg, wg);
     const int np = N + 1, n = np * np * np;
-    std::vector<double> psi(np * np), phi(np * np);  // [poly][quad pt]
+    std::vector<Real> psi(np * np), phi(np * np);  // [poly][quad pt]
     for (int i = 0; i < np; ++i)
         for (int q = 0; q < np; ++q) {
-            psi[i * np + q] = (double)(sqrtl(2.0L * i + 1) * leg_val(i, 2.0L * xg[q] - 1));
-            phi[i * np + q] = (double)lagrange_val(xn, i, xg[q]);
+            psi[i * np + q] = (Real)(sqrtl(2.0L * i + 1) * leg_val(i, 2.0L * xg[q] - 1));
+            phi[i * np + q] = (Real)lagrange_val(xn, i, xg[q]);
         }
     C.assign((size_t)n * n, 0.0);
     for (int i = 0; i < n; ++i) {
         const int i0 = i % np, i1 = (i / np) % np, i2 = i / (np * np);
         for (int j = 0; j < n; ++j) {
             const int j0 = j % np, j1 = (j / np) % np, j2 = j / (np * np);
-            double s = 0;
+            Real s = 0;
             for (int q = 0; q < n; ++q) {
                 const int q0 = q % np, q1 = (q / np) % np, q2 = q / (np * np);
-                const double w = wg[q0] * wg[q1] * wg[q2];
-                const double ps = psi[i0 * np + q0] * psi[i1 * np + q1] * psi[i2 * np + q2];
-                const double ph = phi[j0 * np + q0] * phi[j1 * np + q1] * phi[j2 * np + q2];
+                const Real w = wg[q0] * wg[q1] * wg[q2];
+                const Real ps = psi[i0 * np + q0] * psi[i1 * np + q1] * psi[i2 * np + q2];
+                const Real ph = phi[j0 * np + q0] * phi[j1 * np + q1] * phi[j2 * np + q2];
                 s += w * ps * ph;
             }
             C[(size_t)i * n + j] = s;
@@ -827,7 +843,7 @@ int face_map(int N, int f, int q) {
 
 // RK4Stage5Register3 ctor, dgsem/rk4_stage5_register3.cc:12-36
 struct RKC {
-    double b[5], ao[5], ai[4];
+    Real b[5], ao[5], ai[4];
     RKC() {
         b[0] = 314199625218.0 / 7198350928319; b[1] = 6410344372641.0 / 17000082738695;
         b[2] = 292278564125.0 / 5593752632744; b[3] = 5010207514426.0 / 21876007855139;
@@ -840,17 +856,20 @@ struct RKC {
     }
 };
 
-double blender_from_values(const NS& ns, const oracle_physics& ph, int N, const std::vector<double>& C,
-                           const double* vals) {
+Real blender_from_values(const NS& ns, const oracle_physics& ph, int N, const std::vector<Real>& C,
+                           const Real* vals) {
     // BlenderCalculator::get_blender / get_trouble, dgsem/blender_calculator.cc:123-187, 277-303
     const int np = N + 1, n = np * np * np;
-    std::vector<double> modes(n);
-    for (int row = 0; row < n; ++row) {
-        modes[row] = 0;
-        for (int col = 0; col < n; ++col) modes[row] += C[(size_t)row * n + col] * vals[col];
+    std::vector<Real> modes(n);
+    {
+        FLOP_SCOPE(11);
+        for (int row = 0; row < n; ++row) {
+            modes[row] = 0;
+            for (int col = 0; col < n; ++col) modes[row] += C[(size_t)row * n + col] * vals[col];
+        }
     }
-    double etot = 0, e1 = 0, e2 = 0;
-    for (double m : modes) etot += m * m;
+    Real etot = 0, e1 = 0, e2 = 0;
+    for (Real m : modes) etot += m * m;
     for (int k = 0; k < N; ++k)
         for (int j = 0; j < N; ++j)
             for (int i = 0; i < N; ++i) {
@@ -864,9 +883,9 @@ double blender_from_values(const NS& ns, const oracle_physics& ph, int N, const 
                     const int id = i + j * np + k * np * np;
                     e2 += modes[id] * modes[id];
                 }
-    const double trouble = (N > 2) ? std::max(1 - e1 / etot, 1 - e2 / e1) : 1 - e1 / etot;
-    const double thr = ph.threshold_factor * std::pow(10, -ph.threshold_exp_factor * (std::pow(N + 1, 0.25)));
-    const double at = 1 / (1 + std::exp(-ph.sharpness_factor * (trouble / thr - 1)));
+    const Real trouble = (N > 2) ? std::max(1 - e1 / etot, 1 - e2 / e1) : 1 - e1 / etot;
+    const Real thr = ph.threshold_factor * std::pow(10, -ph.threshold_exp_factor * (std::pow(N + 1, 0.25)));
+    const Real at = 1 / (1 + std::exp(-ph.sharpness_factor * (trouble / thr - 1)));
     (void)ns;
     if (at < ph.alpha_min) return 0;
     else if (at < ph.alpha_max) return at;
@@ -885,30 +904,30 @@ struct oracle {
     oracle_physics ph;
     std::vector<oracle_bc> bcs;
     std::vector<int32_t> nbr, bid;
-    std::vector<double> w, x1, D, Qm;                   // w_1d, ref_D_1d, ref_Q_1d
-    std::vector<double> C;                              // change-of-basis matrix
+    std::vector<Real> w, x1, D, Qm;                   // w_1d, ref_D_1d, ref_Q_1d
+    std::vector<Real> C;                              // change-of-basis matrix
     // metric terms: JxContra[c][node][dir] (T1), detJ[c][node], Jinv[c][node][3][3]
     std::vector<T1> Ja;
-    std::vector<double> detJ, Jinv, hmin, normals;     // normals [c][f][q][3]
+    std::vector<Real> detJ, Jinv, hmin, normals;     // normals [c][f][q][3]
     std::vector<T1> sub;                                // subcell normals [c][dir][(N+2)(N+1)^2]
     // solution vectors
-    std::vector<double> Q, avars, mu, kap, alpha, alpha_d, rhs, rhs1, rhs2, Qn, loc_dt;
-    std::vector<double> alpha_given, alpha_d_given;  // checker hook: blender values to use, when set
-    std::vector<double> raw_given;                    // checker hook: indicator values (pre-override)
+    std::vector<Real> Q, avars, mu, kap, alpha, alpha_d, rhs, rhs1, rhs2, Qn, loc_dt;
+    std::vector<Real> alpha_given, alpha_d_given;  // checker hook: blender values to use, when set
+    std::vector<Real> raw_given;                    // checker hook: indicator values (pre-override)
     std::string x_courant, x_wall, x_region;          // expression-valued Courant number, wall blender
                                                       // limit, viscous blending region (empty: unset)
     int64_t blended_cell_stages = 0;                  // (cell, RHS evaluation) pairs with alpha > 0
-    std::vector<double> s1, s2, s3;                     // surface terms [var][c][f][q]
-    std::vector<double> ws1;                            // entropy variables of s1 (default build)
-    std::vector<double> verts;                          // cell vertices [c][8][3]
-    double bc_time = 0;                                 // BC::time_ = cur_time of the step
+    std::vector<Real> s1, s2, s3;                     // surface terms [var][c][f][q]
+    std::vector<Real> ws1;                            // entropy variables of s1 (default build)
+    std::vector<Real> verts;                          // cell vertices [c][8][3]
+    Real bc_time = 0;                                 // BC::time_ = cur_time of the step
     std::vector<std::vector<std::string>> bexpr;       // varying inflow: p, T, u, v, w per bc
     int mq = 1;                                         // degree of the MappingQGeneric
-    std::vector<double> sup, tq;                        // support points [c][(mq+1)^3][3], GLL(mq)
-    std::vector<double> rhoE_old;                       // rho E at the start of the last step
+    std::vector<Real> sup, tq;                        // support points [c][(mq+1)^3][3], GLL(mq)
+    std::vector<Real> rhoE_old;                       // rho E at the start of the last step
     // local time stepping ("time integration" parameters, dgsem/plens.cc:540-575, 1481-1482)
     int lts_request = 0, lts_limit = 0;
-    double lts_start = 100, lts_nf = 1.05, lts_gf = 1.0;
+    Real lts_start = 100, lts_nf = 1.05, lts_gf = 1.0;
     int64_t fail_dof = -1;
     int fail_kind = 0;
     double timers[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -936,23 +955,23 @@ struct oracle {
 
     // location of node `node` of cell c (the DoF location): the map at the GLL point
     T1 node_location(int c, int node) const {
-        const double xi[3] = {x1[node % np], x1[(node / np) % np], x1[node / (np * np)]};
+        const Real xi[3] = {x1[node % np], x1[(node / np) % np], x1[node / (np * np)]};
         T1 X;
         if (mq == 1) {
-            const double* cv = &verts[(size_t)c * 24];
+            const Real* cv = &verts[(size_t)c * 24];
             for (int v = 0; v < 8; ++v) {
-                double w = 1;
+                Real w = 1;
                 for (int a = 0; a < 3; ++a) w *= ((v >> a) & 1) ? xi[a] : 1.0 - xi[a];
                 for (int d = 0; d < 3; ++d) X[d] += w * cv[3 * v + d];
             }
             return X;
         }
         const int qp = mq + 1;
-        const double* S = &sup[(size_t)c * qp * qp * qp * 3];
+        const Real* S = &sup[(size_t)c * qp * qp * qp * 3];
         for (int k = 0; k < qp; ++k)
             for (int j = 0; j < qp; ++j)
                 for (int i = 0; i < qp; ++i) {
-                    const double l = (double)(lagrange_val(tq, i, xi[0]) * lagrange_val(tq, j, xi[1]) *
+                    const Real l = (Real)(lagrange_val(tq, i, xi[0]) * lagrange_val(tq, j, xi[1]) *
                                               lagrange_val(tq, k, xi[2]));
                     for (int d = 0; d < 3; ++d) X[d] += l * S[3 * ((size_t)i + qp * ((size_t)j + (size_t)qp * k)) + d];
                 }
@@ -961,15 +980,15 @@ struct oracle {
 
     // Jacobian J[a][b] = dx_a/dxi_b of cell c's map at reference point xi: trilinear from the 8
     // vertices (MappingQGeneric(1)), else the degree-mq Lagrange interpolant of the support points
-    void jacobian(int c, const double xi[3], double J[3][3]) const {
+    void jacobian(int c, const Real xi[3], Real J[3][3]) const {
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 3; ++b) J[a][b] = 0;
         if (mq == 1) {
-            const double* cv = &verts[(size_t)c * 24];
+            const Real* cv = &verts[(size_t)c * 24];
             for (int v = 0; v < 8; ++v) {
                 const int bb[3] = {v & 1, (v >> 1) & 1, (v >> 2) & 1};
                 for (int b = 0; b < 3; ++b) {
-                    double g = bb[b] ? 1.0 : -1.0;
+                    Real g = bb[b] ? 1.0 : -1.0;
                     for (int o = 0; o < 3; ++o)
                         if (o != b) g *= bb[o] ? xi[o] : 1.0 - xi[o];
                     for (int a = 0; a < 3; ++a) J[a][b] += g * cv[3 * v + a];
@@ -978,18 +997,18 @@ struct oracle {
             return;
         }
         const int qp = mq + 1;
-        std::vector<double> L(3 * qp), dL(3 * qp);
+        std::vector<Real> L(3 * qp), dL(3 * qp);
         for (int d = 0; d < 3; ++d)
             for (int j = 0; j < qp; ++j) {
-                L[d * qp + j] = (double)lagrange_val(tq, j, xi[d]);
+                L[d * qp + j] = (Real)lagrange_val(tq, j, xi[d]);
                 dL[d * qp + j] = lagrange_deriv(tq, j, xi[d]);
             }
-        const double* S = &sup[(size_t)c * qp * qp * qp * 3];
+        const Real* S = &sup[(size_t)c * qp * qp * qp * 3];
         for (int k = 0; k < qp; ++k)
             for (int j = 0; j < qp; ++j)
                 for (int i = 0; i < qp; ++i) {
-                    const double* sp = S + 3 * ((size_t)i + qp * ((size_t)j + (size_t)qp * k));
-                    const double g[3] = {dL[i] * L[qp + j] * L[2 * qp + k], L[i] * dL[qp + j] * L[2 * qp + k],
+                    const Real* sp = S + 3 * ((size_t)i + qp * ((size_t)j + (size_t)qp * k));
+                    const Real g[3] = {dL[i] * L[qp + j] * L[2 * qp + k], L[i] * dL[qp + j] * L[2 * qp + k],
                                          L[i] * L[qp + j] * dL[2 * qp + k]};
                     for (int a = 0; a < 3; ++a)
                         for (int b = 0; b < 3; ++b) J[a][b] += g[b] * sp[a];
@@ -997,7 +1016,7 @@ struct oracle {
     }
 
     // ---- MetricTerms::reinit, dgsem/metric_terms.cc:67-142 ----------------------------------
-    void metrics(const double* V, int transpose_y) {
+    void metrics(const Real* V, int transpose_y) {
         Ja.assign((size_t)nc * n * 3, T1());
         detJ.assign((size_t)nc * n, 0);
         Jinv.assign((size_t)nc * n * 9, 0);
@@ -1005,11 +1024,11 @@ struct oracle {
         normals.assign((size_t)nc * 6 * nf * 3, 0);
         sub.assign((size_t)nc * 3 * (N + 2) * np * np, T1());
         for (int c = 0; c < nc; ++c) {
-            const double* cv = V + (size_t)c * 24;
-            double dmin = 1e300;
+            const Real* cv = V + (size_t)c * 24;
+            Real dmin = 1e300;
             for (int a = 0; a < 8; ++a)
                 for (int b = a + 1; b < 8; ++b) {
-                    double d2 = 0;
+                    Real d2 = 0;
                     for (int q = 0; q < 3; ++q) d2 += (cv[3 * a + q] - cv[3 * b + q]) * (cv[3 * a + q] - cv[3 * b + q]);
                     dmin = std::min(dmin, d2);
                 }
@@ -1018,15 +1037,15 @@ struct oracle {
                 for (int j = 0; j < np; ++j)
                     for (int i = 0; i < np; ++i) {
                         const int node = tl(i, j, k);
-                        const double xi[3] = {x1[i], x1[j], x1[k]};
-                        double J[3][3];  // J[a][b] = dx_a/dxi_b
+                        const Real xi[3] = {x1[i], x1[j], x1[k]};
+                        Real J[3][3];  // J[a][b] = dx_a/dxi_b
                         jacobian(c, xi, J);
-                        const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                        const Real det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
                                            J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
                                            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
                         detJ[gid(c, node)] = det;
                         // inverse by cofactors
-                        double* Ji = &Jinv[gid(c, node) * 9];
+                        Real* Ji = &Jinv[gid(c, node) * 9];
                         Ji[0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
                         Ji[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
                         Ji[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
@@ -1064,12 +1083,12 @@ struct oracle {
             // FEFaceValues normals at face dofs (H1: faces 2/3 read the transposed point)
             for (int f = 0; f < 6; ++f) {
                 const int dir = f / 2;
-                const double sg = (f % 2) ? 1.0 : -1.0;
+                const Real sg = (f % 2) ? 1.0 : -1.0;
                 for (int q = 0; q < nf; ++q) {
                     int qq = q;
                     if ((f == 2 || f == 3) && transpose_y) qq = (q % np) * np + q / np;
                     const T1& J = JaV(c, face_map(N, f, qq), dir);
-                    const double nr = J.norm();
+                    const Real nr = J.norm();
                     for (int d = 0; d < 3; ++d) normals[(((size_t)c * 6 + f) * nf + q) * 3 + d] = sg * J[d] / nr;
                 }
             }
@@ -1101,7 +1120,7 @@ struct oracle {
             case 1: {  // uniform inflow (uniform_inflow.cc:23-77)
                 State pr;
                 T1 v; v[0] = b.vel[0]; v[1] = b.vel[1]; v[2] = b.vel[2];
-                const double rho = b.p / (ns.R() * b.T);  // set_BC, plens.cc:1268-1270
+                const Real rho = b.p / (ns.R() * b.T);  // set_BC, plens.cc:1268-1270
                 ns.prim_to_cons(rho, v, b.p, pr);
                 if (stage == 2 || (stage == 1 && g_mutation == kMutInflowS1)) g = pr;
                 else for (int k = 0; k < 5; ++k) g[k] = 2 * pr[k] - in[k];
@@ -1109,11 +1128,11 @@ struct oracle {
             }
             case 2: {  // outflow (outflow.cc:18-75)
                 if (stage == 2) {
-                    const double M = ns.get_M(in);
+                    const Real M = ns.get_M(in);
                     if ((ns.inviscid() && g_mutation != kMutOutflowInviscid) || M >= 1) g = in;
                     else {
                         g = in;
-                        const double pin = ns.get_p(in);
+                        const Real pin = ns.get_p(in);
                         if (g_mutation == kMutOutflowSign) g[4] += 2 * (pin - b.p) / (ns.gma - 1);
                         else g[4] += 2 * (b.p - pin) / (ns.gma - 1);
                     }
@@ -1127,9 +1146,9 @@ struct oracle {
             case 6: {  // empty (empty.cc): as symmetry, except the stage-2 ghost
                 T1 vin, vt;
                 for (int d = 0; d < 3; ++d) vin[d] = in[1 + d] / in[0];
-                const double vn = dot(vin, nrm);
+                const Real vn = dot(vin, nrm);
                 for (int d = 0; d < 3; ++d) vt[d] = vin[d] - vn * nrm[d];
-                const double pin = ns.get_p(in);
+                const Real pin = ns.get_p(in);
                 State pr;
                 ns.prim_to_cons(in[0], vt, pin, pr);
                 if (stage != 2) {
@@ -1139,7 +1158,7 @@ struct oracle {
                 } else {
                     T1 sm;
                     for (int d = 0; d < 3; ++d) sm[d] = in[1 + d];
-                    const double smn = dot(sm, nrm);
+                    const Real smn = dot(sm, nrm);
                     g[0] = in[0];
                     g[4] = in[4];
                     for (int d = 0; d < 3; ++d) g[1 + d] = sm[d] - 2 * smn * nrm[d];
@@ -1150,7 +1169,7 @@ struct oracle {
                        // by the expressions at the DoF location and BC::time_
                 const std::vector<std::string>& ex = bexpr[bid[(size_t)c * 6 + f]];
                 const T1 X = node_location(c, face_map(N, f, q));
-                double val[5];
+                Real val[5];
                 for (int k = 0; k < 5; ++k) val[k] = eval_expression(ex[k], X[0], X[1], X[2], bc_time);
                 T1 v; v[0] = val[2]; v[1] = val[3]; v[2] = val[4];
                 State pr;
@@ -1177,7 +1196,7 @@ struct oracle {
                     if (ns.inviscid()) {
                         T1 sm, smr;
                         for (int d = 0; d < 3; ++d) { sm[d] = in[1 + d]; smr[d] = in[1 + d] - in[0] * vw[d]; }
-                        const double nsm = dot(smr, nrm);
+                        const Real nsm = dot(smr, nrm);
                         for (int d = 0; d < 3; ++d) g[1 + d] = sm[d] - 2 * nsm * nrm[d];
                     } else if (g_mutation == kMutWallAbsVel) {
                         for (int d = 0; d < 3; ++d) g[1 + d] = -in[1 + d];
@@ -1186,16 +1205,16 @@ struct oracle {
                     }
                 } else {
                     State pr;
-                    const double pin = ns.get_p(in);
+                    const Real pin = ns.get_p(in);
                     const bool inner_rho = b.type == 7 || g_mutation == kMutWallRho;
-                    const double rpr = inner_rho ? in[0] : pin / (ns.R() * b.T);
+                    const Real rpr = inner_rho ? in[0] : pin / (ns.R() * b.T);
                     ns.prim_to_cons(rpr, vw, pin, pr);
                     for (int k = 0; k < 5; ++k) g[k] = 2 * pr[k] - in[k];
                     if (b.type == 7 && stage == 3) {
                         // heat flux with its normal component reversed, shear stresses kept
                         // (insulated_wall.cc:95-101 as documented there; the reference's loop writes
                         // heat_flux_in[6..8], out of the bounds of the 3-vector, at line 99)
-                        double qn = 0;
+                        Real qn = 0;
                         for (int d = 0; d < 3; ++d) qn += ain[6 + d] * nrm[d];
                         for (int d = 0; d < 3; ++d) ag[6 + d] = ain[6 + d] - 2 * qn * nrm[d];
                     }
@@ -1211,6 +1230,7 @@ struct oracle {
         if (stage == 1) {
             for (int v = 0; v < 5; ++v) f[v] = 0.5 * (u[v] + ug[v]);
         } else if (stage == 2) {
+            FLOP_SCOPE(12);
             nsl.get_inv_surf_flux(u, ug, nrm, f);
         } else {
             State f1, f2;
@@ -1221,9 +1241,9 @@ struct oracle {
     }
 
     // ---- PLENS::calc_surf_flux, dgsem/plens.cc:1717-1913 --------------------------------------
-    void calc_surf_flux(int stage, std::vector<double>& sf) {
+    void calc_surf_flux(int stage, std::vector<Real>& sf) {
         sf.assign((size_t)5 * nc * 6 * nf, 0.0);
-        const double rsign = (stage == 1) ? 1.0f : -1.0f;
+        const Real rsign = (stage == 1) ? 1.0f : -1.0f;
 #pragma omp parallel for schedule(static)
         for (int c = 0; c < nc; ++c) {
             NS nsl = ns;  // per-thread copy: the reference mutates flux_blender_value
@@ -1263,10 +1283,10 @@ struct oracle {
     }
 
     // ---- PLENS::calc_cell_cons_grad, dgsem/plens.cc:2174-2285 ---------------------------------
-    void calc_cell_cons_grad(int c, const std::vector<double>& s1f, std::vector<std::array<State, 3>>& g) const {
+    void calc_cell_cons_grad(int c, const std::vector<Real>& s1f, std::vector<std::array<State, 3>>& g) const {
         std::vector<State> st(n);
         for (int i = 0; i < n; ++i) st[i] = cons(c, i);
-        const double ssign[2] = {1, -1};
+        const Real ssign[2] = {1, -1};
         for (int d = 0; d < 3; ++d)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < 5; ++v) g[i][d][v] = 0;
@@ -1282,8 +1302,8 @@ struct oracle {
                         const int io = tl(to[0], to[1], to[2]);
                         const State co = st[io];
                         for (int v = 0; v < 5; ++v) fl[v] = 0.5 * (ct[v] + co[v]);
-                        const double Jav = 0.5 * (JaV(c, it, md)[gd] + JaV(c, io, md)[gd]);
-                        const double Dm2 = 2 * D[ti[md] * np + m];
+                        const Real Jav = 0.5 * (JaV(c, it, md)[gd] + JaV(c, io, md)[gd]);
+                        const Real Dm2 = 2 * D[ti[md] * np + m];
                         for (int v = 0; v < 5; ++v) g[it][gd][v] += Dm2 * fl[v] * Jav;
                     }
             }
@@ -1294,14 +1314,14 @@ struct oracle {
                         const int node = face_map(N, f, q);
                         const State cs = st[node];
                         for (int v = 0; v < 5; ++v) {
-                            const double fin = cs[v] * JaV(c, node, sd)[gd];
-                            const double fsu = s1f[sidx(v, c, f, q)] * JaV(c, node, sd)[gd];
+                            const Real fin = cs[v] * JaV(c, node, sd)[gd];
+                            const Real fsu = s1f[sidx(v, c, f, q)] * JaV(c, node, sd)[gd];
                             g[node][gd][v] -= ssign[lr] * (fsu - fin) / w[lr * N];
                         }
                     }
                 }
             for (int i = 0; i < n; ++i) {
-                const double Ji = 1.0 / detJ[gid(c, i)];
+                const Real Ji = 1.0 / detJ[gid(c, i)];
                 for (int v = 0; v < 5; ++v) g[i][gd][v] *= Ji;
             }
         }
@@ -1309,10 +1329,10 @@ struct oracle {
 
     // ---- PLENS::calc_cell_evar_grad, dgsem/plens.cc:2294-2405 (default build) ------------------
     // ws1: entropy variables of the stage-1 face values (calc_aux_vars, 2610-2637)
-    void calc_cell_evar_grad(int c, const std::vector<double>& ws1, std::vector<std::array<State, 3>>& g) const {
+    void calc_cell_evar_grad(int c, const std::vector<Real>& ws1, std::vector<std::array<State, 3>>& g) const {
         std::vector<State> ev(n);
         for (int i = 0; i < n; ++i) ns.cons_to_entropy(cons(c, i), ev[i]);
-        const double ssign[2] = {1, -1};
+        const Real ssign[2] = {1, -1};
         for (int d = 0; d < 3; ++d)
             for (int i = 0; i < n; ++i)
                 for (int v = 0; v < 5; ++v) g[i][d][v] = 0;
@@ -1325,8 +1345,8 @@ struct oracle {
                         to[md] = m;
                         const int io = tl(to[0], to[1], to[2]);
                         const State eo = ev[io];
-                        const double Jc = JaV(c, io, md)[gd];
-                        const double Dv = D[ti[md] * np + m];
+                        const Real Jc = JaV(c, io, md)[gd];
+                        const Real Dv = D[ti[md] * np + m];
                         for (int v = 0; v < 5; ++v) g[it][gd][v] += Dv * eo[v] * Jc;
                     }
             }
@@ -1337,14 +1357,14 @@ struct oracle {
                         const int node = face_map(N, f, q);
                         const State e = ev[node];
                         for (int v = 0; v < 5; ++v) {
-                            const double fin = e[v] * JaV(c, node, sd)[gd];
-                            const double fsu = ws1[sidx(v, c, f, q)] * JaV(c, node, sd)[gd];
+                            const Real fin = e[v] * JaV(c, node, sd)[gd];
+                            const Real fsu = ws1[sidx(v, c, f, q)] * JaV(c, node, sd)[gd];
                             g[node][gd][v] -= ssign[lr] * (fsu - fin) / w[lr * N];
                         }
                     }
                 }
             for (int i = 0; i < n; ++i) {
-                const double Ji = 1.0 / detJ[gid(c, i)];
+                const Real Ji = 1.0 / detJ[gid(c, i)];
                 for (int v = 0; v < 5; ++v) g[i][gd][v] *= Ji;
             }
         }
@@ -1363,11 +1383,11 @@ struct oracle {
 
     // ---- PLENS::calc_aux_vars, dgsem/plens.cc:2593-2748 (both builds) --------------------------
     void calc_aux_vars() {
-        const double cvv = ns.cv();
+        const Real cvv = ns.cv();
         for (size_t i = 0; i < ndof; ++i) {
             State s;
             for (int v = 0; v < 5; ++v) s[v] = Q[v * ndof + i];
-            const double e = NS::get_e(s);
+            const Real e = NS::get_e(s);
             mu[i] = ns.mu(e / cvv);
             kap[i] = ns.kappa(mu[i]);
         }
@@ -1393,13 +1413,13 @@ struct oracle {
                 calc_cell_cons_grad(c, s1, g);
                 for (int i = 0; i < n; ++i) {
                     const State s = cons(c, i);
-                    const double vel[3] = {s[1] / s[0], s[2] / s[0], s[3] / s[0]};
-                    double vg[3][3];
+                    const Real vel[3] = {s[1] / s[0], s[2] / s[0], s[3] / s[0]};
+                    Real vg[3][3];
                     for (int vd = 0; vd < 3; ++vd)
                         for (int gd = 0; gd < 3; ++gd)
                             vg[vd][gd] = (g[i][gd][1 + vd] - g[i][gd][0] * vel[vd]) / s[0];
                     int sid = 0;
-                    double tr = 0;
+                    Real tr = 0;
                     for (int d = 0; d < 3; ++d) tr += vg[d][d];
                     const size_t gi = gid(c, i);
                     for (int row = 0; row < 3; ++row)
@@ -1408,8 +1428,8 @@ struct oracle {
                             else avars[sid * ndof + gi] = vg[row][col] + vg[col][row];
                             ++sid;
                         }
-                    const double e = NS::get_e(s);
-                    double eg[3];
+                    const Real e = NS::get_e(s);
+                    Real eg[3];
                     for (int gd = 0; gd < 3; ++gd) {
                         eg[gd] = (g[i][gd][4] - g[i][gd][0] * e) / s[0];
                         for (int vd = 0; vd < 3; ++vd)
@@ -1436,12 +1456,12 @@ struct oracle {
                 for (int i = 0; i < n; ++i) {
                     State et;
                     ns.cons_to_entropy(cons(c, i), et);
-                    double vg[3][3];
+                    Real vg[3][3];
                     for (int vd = 0; vd < 3; ++vd)
                         for (int gd = 0; gd < 3; ++gd)
                             vg[vd][gd] = (et[1 + vd] * g[i][gd][4] / et[4] - g[i][gd][1 + vd]) / et[4];
                     int sid = 0;
-                    double tr = 0;
+                    Real tr = 0;
                     for (int d = 0; d < 3; ++d) tr += vg[d][d];
                     const size_t gi = gid(c, i);
                     for (int row = 0; row < 3; ++row)
@@ -1463,10 +1483,10 @@ struct oracle {
 
     // ---- PLENS::calc_blender, dgsem/plens.cc:2858-2962 ------------------------------------------
     void calc_blender() {
-        std::vector<double> raw(nc);
+        std::vector<Real> raw(nc);
 #pragma omp parallel
         {
-            std::vector<double> vals(n);
+            std::vector<Real> vals(n);
 #pragma omp for schedule(static)
             for (int c = 0; c < nc; ++c) {
                 for (int i = 0; i < n; ++i) {
@@ -1478,7 +1498,7 @@ struct oracle {
         }
         if (!raw_given.empty()) raw = raw_given;  // checker hook: given indicator values
         // wall_blender_limit_function.value(Point<dim>()) at cur_time (dgsem/plens.cc:2884-2898)
-        const double wlim = x_wall.empty() ? ph.wall_blender_limit : eval_expression(x_wall, 0.0, 0.0, 0.0, bc_time);
+        const Real wlim = x_wall.empty() ? Real(ph.wall_blender_limit) : eval_expression(x_wall, 0.0, 0.0, 0.0, bc_time);
         auto wall_override = [&]() {
             for (int c = 0; c < nc; ++c) {
                 for (int f = 0; f < 6; ++f) {
@@ -1490,37 +1510,37 @@ struct oracle {
         };
         alpha = raw;
         if (g_mutation != kMutBlendOverrideLate) wall_override();
-        const std::vector<double> gh = alpha;  // gh_gcrk_alpha = gcrk_alpha
-        const std::vector<double>& nsrc = g_mutation == kMutBlendLive ? alpha : gh;
+        const std::vector<Real> gh = alpha;  // gh_gcrk_alpha = gcrk_alpha
+        const std::vector<Real>& nsrc = g_mutation == kMutBlendLive ? alpha : gh;
         for (int c = 0; c < nc; ++c)
             for (int f = 0; f < 6; ++f) {
                 // face->at_boundary() (periodic included)
                 const int b = bid[(size_t)c * 6 + f];
                 if (b >= 0 && !(g_mutation == kMutBlendPeriodic && bcs[b].type == 0)) continue;
-                const double cur = alpha[c], nei = nsrc[nbr[(size_t)c * 6 + f]];
+                const Real cur = alpha[c], nei = nsrc[nbr[(size_t)c * 6 + f]];
                 alpha[c] = std::max(cur, 0.5 * nei);
             }
         if (g_mutation == kMutBlendOverrideLate) wall_override();
         for (int c = 0; c < nc; ++c) {  // vis_blending_region.value(cell->center()) > 0 (2950-2960)
             bool in_region = true;
             if (!x_region.empty()) {
-                double ctr[3] = {0, 0, 0};  // TriaAccessor::center(): the mean of the vertices
+                Real ctr[3] = {0, 0, 0};  // TriaAccessor::center(): the mean of the vertices
                 for (int v = 0; v < 8; ++v)
                     for (int d = 0; d < 3; ++d) ctr[d] += verts[(size_t)c * 24 + 3 * v + d] / 8.0;
                 in_region = eval_expression(x_region, ctr[0], ctr[1], ctr[2], bc_time) > 0;
             }
-            alpha_d[c] = !in_region ? 0.0
+            alpha_d[c] = !in_region ? Real(0.0)
                          : g_mutation == kMutBlendNoCap ? alpha[c] / ph.alpha_max
-                                                        : std::min(1.0, alpha[c] / ph.alpha_max);
+                                                        : std::min(Real(1.0), alpha[c] / ph.alpha_max);
         }
     }
 
     // ---- PLENS::calc_cell_ho_inv_residual, dgsem/plens.cc:3153-3275 ----------------------------
     void ho_inv(int c, std::vector<State>& r) const {
-        const double ss = -1;
+        const Real ss = -1;
         std::vector<State> st(n);
         for (int i = 0; i < n; ++i) st[i] = cons(c, i);
-        const double sgn[2] = {1, -1};
+        const Real sgn[2] = {1, -1};
         for (int i = 0; i < n; ++i) r[i] = State{0, 0, 0, 0, 0};
         for (int it = 0; it < n; ++it) {
             const State ct = st[it];
@@ -1533,20 +1553,21 @@ struct oracle {
                     const int io = tl(to[0], to[1], to[2]);
                     T1 dir;
                     for (int a = 0; a < 3; ++a) dir[a] = 0.5 * (JaV(c, it, md)[a] + JaV(c, io, md)[a]);
-                    const double nr = dir.norm();
+                    const Real nr = dir.norm();
                     for (int a = 0; a < 3; ++a) dir[a] /= nr;
                     ns.vol_flux(ct, st[io], dir, fl);
-                    const double Dn = 2 * D[ti[md] * np + m] * nr;
+                    const Real Dn = 2 * D[ti[md] * np + m] * nr;
                     for (int v = 0; v < 5; ++v) r[it][v] += Dn * fl[v];
                 }
         }
+        FLOP_BUCKET(14);
         for (int sd = 0; sd < 3; ++sd)
             for (int lr = 0; lr <= 1; ++lr) {
                 const int f = 2 * sd + lr;
                 for (int q = 0; q < nf; ++q) {
                     const int node = face_map(N, f, q);
                     T1 dir = JaV(c, node, sd);
-                    const double nr = dir.norm();
+                    const Real nr = dir.norm();
                     for (int a = 0; a < 3; ++a) dir[a] /= nr;
                     State fs, fi;
                     for (int v = 0; v < 5; ++v) fs[v] = -sgn[lr] * s2[sidx(v, c, f, q)];
@@ -1555,18 +1576,18 @@ struct oracle {
                 }
             }
         for (int i = 0; i < n; ++i) {
-            const double jm = 1 / (ss * detJ[gid(c, i)]);
+            const Real jm = 1 / (ss * detJ[gid(c, i)]);
             for (int v = 0; v < 5; ++v) r[i][v] *= jm;
         }
     }
 
     // ---- PLENS::calc_cell_ho_dif_residual, dgsem/plens.cc:3283-3413 ----------------------------
     void ho_dif(int c, std::vector<State>& r) const {
-        const double ss = 1;
+        const Real ss = 1;
         std::vector<State> st(n);
         std::vector<Avars> sa(n);
         for (int i = 0; i < n; ++i) { st[i] = cons(c, i); sa[i] = av(c, i); }
-        const double sgn[2] = {1, -1};
+        const Real sgn[2] = {1, -1};
         for (int i = 0; i < n; ++i) r[i] = State{0, 0, 0, 0, 0};
         for (int it = 0; it < n; ++it) {
             const int ti[3] = {it % np, (it / np) % np, it / (np * np)};
@@ -1577,20 +1598,24 @@ struct oracle {
                     to[md] = m;
                     const int io = tl(to[0], to[1], to[2]);
                     T1 dir = JaV(c, io, md);
-                    const double nr = dir.norm();
+                    const Real nr = dir.norm();
                     for (int a = 0; a < 3; ++a) dir[a] /= nr;
-                    NS::dif_flux(st[io], sa[io], dir, fl);
-                    const double Dn = D[ti[md] * np + m] * nr;
+                    {
+                        FLOP_SCOPE(13);
+                        NS::dif_flux(st[io], sa[io], dir, fl);
+                    }
+                    const Real Dn = D[ti[md] * np + m] * nr;
                     for (int v = 0; v < 5; ++v) r[it][v] += Dn * fl[v];
                 }
         }
+        FLOP_BUCKET(15);
         for (int sd = 0; sd < 3; ++sd)
             for (int lr = 0; lr <= 1; ++lr) {
                 const int f = 2 * sd + lr;
                 for (int q = 0; q < nf; ++q) {
                     const int node = face_map(N, f, q);
                     T1 dir = JaV(c, node, sd);
-                    const double nr = dir.norm();
+                    const Real nr = dir.norm();
                     for (int a = 0; a < 3; ++a) dir[a] /= nr;
                     State fs, fi;
                     for (int v = 0; v < 5; ++v) fs[v] = -sgn[lr] * s3[sidx(v, c, f, q)];
@@ -1599,7 +1624,7 @@ struct oracle {
                 }
             }
         for (int i = 0; i < n; ++i) {
-            const double jm = 1 / (ss * detJ[gid(c, i)]);
+            const Real jm = 1 / (ss * detJ[gid(c, i)]);
             for (int v = 0; v < 5; ++v) r[i][v] *= jm;
         }
     }
@@ -1619,7 +1644,7 @@ struct oracle {
                         const int il = tl(tlf[0], tlf[1], tlf[2]), ir = tl(trt[0], trt[1], trt[2]);
                         const State cl = cons(c, il), cr = cons(c, ir);
                         T1 nd = subn(c, dir, trt);
-                        const double nn = nd.norm();
+                        const Real nn = nd.norm();
                         for (int a = 0; a < 3; ++a) nd[a] /= nn;
                         State fl;
                         nsl.get_inv_surf_flux(cl, cr, nd, fl);
@@ -1635,9 +1660,9 @@ struct oracle {
                 const float cs = lr ? 1.0f : -1.0f, fs = lr ? 1.0f : -1.0f;
                 for (int q = 0; q < nf; ++q) {
                     const int node = face_map(N, f, q);
-                    const double nn = JaV(c, node, dir).norm();
+                    const Real nn = JaV(c, node, dir).norm();
                     for (int v = 0; v < 5; ++v) {
-                        const double fl = fs * s2[sidx(v, c, f, q)];
+                        const Real fl = fs * s2[sidx(v, c, f, q)];
                         r[node][v] += cs * fl * nn / w[lr * N];
                     }
                 }
@@ -1654,10 +1679,13 @@ struct oracle {
             timers[k] += std::chrono::duration<double>(t1 - t0).count();
             t0 = t1;
         };
+        FLOP_BUCKET(0);
         if (!assert_positivity()) return -3;
         lap(0);
+        FLOP_BUCKET(1);
         if (!ns.inviscid()) calc_aux_vars();
         lap(1);
+        FLOP_BUCKET(2);
         if (!alpha_given.empty()) {
             alpha = alpha_given;
             alpha_d = alpha_d_given;
@@ -1666,6 +1694,7 @@ struct oracle {
         }
         for (int c = 0; c < nc; ++c) blended_cell_stages += alpha[c] > 0.0;
         lap(2);
+        FLOP_BUCKET(3);
         calc_surf_flux(2, s2);
         if (!ns.inviscid()) calc_surf_flux(3, s3);
         lap(3);
@@ -1677,17 +1706,21 @@ struct oracle {
 #pragma omp for schedule(static)
             for (int c = 0; c < nc; ++c) {
                 auto a0 = std::chrono::steady_clock::now();
+                FLOP_BUCKET(4);
                 ho_inv(c, hi);
+                FLOP_BUCKET(5);
                 if (!ns.inviscid()) ho_dif(c, hd);
                 auto a1 = std::chrono::steady_clock::now();
+                FLOP_BUCKET(6);
                 lo_inv(c, lo, nsl);
+                FLOP_BUCKET(7);
                 auto a2 = std::chrono::steady_clock::now();
                 tho += std::chrono::duration<double>(a1 - a0).count();
                 tlo += std::chrono::duration<double>(a2 - a1).count();
-                const double al = alpha[c], ad = alpha_d[c];
+                const Real al = alpha[c], ad = alpha_d[c];
                 for (int i = 0; i < n; ++i)
                     for (int v = 0; v < 5; ++v) {
-                        double& o = rhs[v * ndof + gid(c, i)];
+                        Real& o = rhs[v * ndof + gid(c, i)];
                         if (!ns.inviscid()) o = (1 - ad) * hd[i][v] + al * lo[i][v] + (1 - al) * hi[i][v];
                         else o = al * lo[i][v] + (1 - al) * hi[i][v];
                     }
@@ -1705,38 +1738,38 @@ struct oracle {
 
     // ---- PLENS::calc_time_step, dgsem/plens.cc:3810-3912: per-cell steps into loc_dt, then global
     //      stepping (every cell gets the minimum) or, past the start time, local stepping
-    double calc_time_step(double Co, double cur_time = 0.0) {
+    Real calc_time_step(Real Co, Real cur_time = 0.0) {
         // courant_function.value(Point<dim>()) at cur_time (dgsem/plens.cc:3816-3817)
         if (!x_courant.empty()) Co = eval_expression(x_courant, 0.0, 0.0, 0.0, cur_time);
         if (Co >= 1 && g_mutation != kMutDtNoCap) Co = 0.9;
-        double proc = 1e6;
+        Real proc = 1e6;
         loc_dt.assign(nc, 0.0);
         for (int c = 0; c < nc; ++c) {
-            double cs = 1e6;
-            const double h = hmin[c];
+            Real cs = 1e6;
+            const Real h = hmin[c];
             for (int i = 0; i < n; ++i) {
                 const State s = cons(c, i);
-                const double a = ns.get_a(s);
-                const double vel[3] = {s[1] / s[0], s[2] / s[0], s[3] / s[0]};
-                const double* Ji = &Jinv[gid(c, i) * 9];
-                double adv[3] = {0, 0, 0};
+                const Real a = ns.get_a(s);
+                const Real vel[3] = {s[1] / s[0], s[2] / s[0], s[3] / s[0]};
+                const Real* Ji = &Jinv[gid(c, i) * 9];
+                Real adv[3] = {0, 0, 0};
                 for (int row = 0; row < 3; ++row)
                     for (int col = 0; col < 3; ++col)  // JinvT
                         adv[row] += (g_mutation == kMutDtJinv ? Ji[row * 3 + col] : Ji[col * 3 + row]) * vel[col];
                 if (g_mutation == kMutDtAbs)
-                    for (double& x : adv) x = std::fabs(x);
-                const double madv = std::max({adv[0], adv[1], adv[2]});
-                const double pw = std::pow(N, g_mutation == kMutDtDegreeSq ? 2.0 : 1.5);
-                const double cur = Co / (pw * (madv + a / h + N * N * mu[gid(c, i)] / (h * h * s[0])));
+                    for (Real& x : adv) x = std::fabs(x);
+                const Real madv = std::max({adv[0], adv[1], adv[2]});
+                const Real pw = std::pow(N, g_mutation == kMutDtDegreeSq ? 2.0 : 1.5);
+                const Real cur = Co / (pw * (madv + a / h + N * N * mu[gid(c, i)] / (h * h * s[0])));
                 if (cur < cs) cs = cur;
             }
             loc_dt[c] = cs;
             if (cs < proc) proc = cs;
         }
         if (lts_request && cur_time > lts_start) {
-            const std::vector<double> gh = loc_dt;  // gh_loc_time_steps
+            const std::vector<Real> gh = loc_dt;  // gh_loc_time_steps
             for (int c = 0; c < nc; ++c) {
-                double m = gh[c];
+                Real m = gh[c];
                 if (lts_limit) {
                     for (int f = 0; f < 6; ++f)
                         if (bid[(size_t)c * 6 + f] < 0) m = std::min(m, lts_nf * gh[nbr[(size_t)c * 6 + f]]);
@@ -1752,31 +1785,31 @@ struct oracle {
 
     // ---- PLENS::calc_ss_error, dgsem/plens.cc:3985-4035: L2 norm of (rhoE - rhoE_old)/dt_cell over
     //      the L2 norm of rhoE, both by QGauss(N+1) through the mapping
-    double calc_ss_error() const {
-        std::vector<double> xg, wg;
+    Real calc_ss_error() const {
+        std::vector<Real> xg, wg;
         gauss(np, xg, wg);
-        std::vector<double> L((size_t)np * np);  // L[q][i] = l_i(xg_q), GLL Lagrange basis
+        std::vector<Real> L((size_t)np * np);  // L[q][i] = l_i(xg_q), GLL Lagrange basis
         for (int q = 0; q < np; ++q)
-            for (int i = 0; i < np; ++i) L[q * np + i] = (double)lagrange_val(x1, i, xg[q]);
-        double num = 0, den = 0;
+            for (int i = 0; i < np; ++i) L[q * np + i] = (Real)lagrange_val(x1, i, xg[q]);
+        Real num = 0, den = 0;
         for (int c = 0; c < nc; ++c) {
             for (int qk = 0; qk < np; ++qk)
                 for (int qj = 0; qj < np; ++qj)
                     for (int qi = 0; qi < np; ++qi) {
-                        const double xi[3] = {xg[qi], xg[qj], xg[qk]};
-                        double J[3][3];
+                        const Real xi[3] = {xg[qi], xg[qj], xg[qk]};
+                        Real J[3][3];
                         jacobian(c, xi, J);
-                        const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                        const Real det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
                                            J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
                                            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-                        const double jxw = std::fabs(det) * wg[qi] * wg[qj] * wg[qk];
-                        double e = 0, d = 0;
+                        const Real jxw = std::fabs(det) * wg[qi] * wg[qj] * wg[qk];
+                        Real e = 0, d = 0;
                         for (int k = 0; k < np; ++k)
                             for (int j = 0; j < np; ++j)
                                 for (int i = 0; i < np; ++i) {
-                                    const double lw = L[qi * np + i] * L[qj * np + j] * L[qk * np + k];
+                                    const Real lw = L[qi * np + i] * L[qj * np + j] * L[qk * np + k];
                                     const size_t gi = gid(c, tl(i, j, k));
-                                    const double E = Q[4 * ndof + gi];
+                                    const Real E = Q[4 * ndof + gi];
                                     e += lw * E;
                                     d += lw * (E - rhoE_old[gi]) / loc_dt[c];
                                 }
@@ -1788,15 +1821,16 @@ struct oracle {
     }
 
     // ---- PLENS::update + update_rk4, dgsem/plens.cc:4334-4403, 4484-4518 --------------------------
-    int rk4_step(double Co, double* t, double fixed_dt = 0) {
+    int rk4_step(Real Co, double* t, Real fixed_dt = 0) {
         if (t) bc_time = *t;  // BC::set_time(cur_time), dgsem/plens.cc:1798
         // gcrk_cvars = g_cvars (the state lives in Q throughout)
         int e = calc_rhs();
         if (e) return e;
         auto t0 = std::chrono::steady_clock::now();
-        const double dt = fixed_dt > 0 ? fixed_dt : calc_time_step(Co, *t);
+        FLOP_BUCKET(8);
+        const Real dt = fixed_dt > 0 ? fixed_dt : calc_time_step(Co, *t);
         if (fixed_dt > 0) loc_dt.assign(nc, fixed_dt);
-        std::vector<double> E0(Q.begin() + 4 * ndof, Q.begin() + 5 * ndof);  // g_cvars[4] at the start
+        std::vector<Real> E0(Q.begin() + 4 * ndof, Q.begin() + 5 * ndof);  // g_cvars[4] at the start
         auto t1 = std::chrono::steady_clock::now();
         timers[6] += std::chrono::duration<double>(t1 - t0).count();
         for (int stage = 1; stage <= 5; ++stage) {
@@ -1805,6 +1839,7 @@ struct oracle {
                 if (e) return e;
             }
             auto u0 = std::chrono::steady_clock::now();
+            FLOP_BUCKET(9);
             for (int v = 0; v < 5; ++v)  // multiply_time_step_to_rhs (per-cell steps)
                 for (size_t i = 0; i < ndof; ++i) rhs[v * ndof + i] *= loc_dt[i / n];
             if (stage == 1) {
@@ -1879,7 +1914,7 @@ int oracle_create(int32_t n_cells, int32_t degree, const double* vertices, const
         }
     cob_matrix(degree, o->C);
     o->verts.assign(vertices, vertices + (size_t)n_cells * 24);
-    o->metrics(vertices, transpose_y);
+    o->metrics(o->verts.data(), transpose_y);
     const size_t nd = o->ndof;
     o->Q.assign(5 * nd, 0); o->avars.assign(9 * nd, 0); o->mu.assign(nd, 0); o->kap.assign(nd, 0);
     o->alpha.assign(n_cells, 0); o->alpha_d.assign(n_cells, 0);
@@ -1908,9 +1943,9 @@ int oracle_create_mapped(int32_t n_cells, int32_t degree, int32_t map_degree, co
     oracle* o = *out;
     o->mq = map_degree;
     o->sup.assign(support, support + (size_t)n_cells * nsp * 3);
-    std::vector<double> wq;
+    std::vector<Real> wq;
     gll(map_degree, o->tq, wq);
-    o->metrics(corners.data(), transpose_y);
+    o->metrics(o->verts.data(), transpose_y);
     return 0;
 }
 
@@ -1951,7 +1986,7 @@ int oracle_calc_dt(oracle* o, double courant, double* dt) {
     for (size_t i = 0; i < o->ndof; ++i) {
         State s;
         for (int v = 0; v < 5; ++v) s[v] = o->Q[v * o->ndof + i];
-        o->mu[i] = o->ns.inviscid() ? 0.0 : o->ns.mu(NS::get_e(s) / cvv);
+        o->mu[i] = o->ns.inviscid() ? Real(0.0) : o->ns.mu(NS::get_e(s) / cvv);
     }
     *dt = o->calc_time_step(courant, o->bc_time);
     return 0;
@@ -2113,6 +2148,23 @@ int oracle_ss_error(const oracle* o, double* err) {
     return 0;
 }
 
+// flop counters of the counting build (liboracle_flops.so) per bucket and the number of entries
+// into each nested bucket; -1 in the oracle proper
+int oracle_flop_counts(uint64_t* flops, uint64_t* calls, int32_t n, int32_t reset) {
+#ifdef ORACLE_COUNT_FLOPS
+    for (int i = 0; i < n && i < flopcount::kBuckets; ++i) {
+        if (flops) flops[i] = flopcount::g_flops[i];
+        if (calls) calls[i] = flopcount::g_calls[i];
+    }
+    if (reset)
+        for (int i = 0; i < flopcount::kBuckets; ++i) flopcount::g_flops[i] = flopcount::g_calls[i] = 0;
+    return flopcount::kBuckets;
+#else
+    (void)flops; (void)calls; (void)n; (void)reset;
+    return -1;
+#endif
+}
+
 int oracle_section_times(const oracle* o, double* t, int32_t n) {
     for (int i = 0; i < n && i < 8; ++i) t[i] = o->timers[i];
     return 0;
@@ -2146,7 +2198,7 @@ int oracle_cell_subcell_normals(const oracle* o, int32_t c, double* sn) {
 }
 
 int oracle_basis(int32_t N, double* x, double* w, double* D) {
-    std::vector<double> xx, ww;
+    std::vector<Real> xx, ww;
     gll(N, xx, ww);
     for (int i = 0; i <= N; ++i) { x[i] = xx[i]; w[i] = ww[i]; }
     if (D)
@@ -2156,9 +2208,9 @@ int oracle_basis(int32_t N, double* x, double* w, double* D) {
 }
 
 int oracle_cob_matrix(int32_t N, double* C) {
-    std::vector<double> c;
+    std::vector<Real> c;
     cob_matrix(N, c);
-    std::memcpy(C, c.data(), sizeof(double) * c.size());
+    for (size_t i = 0; i < c.size(); ++i) C[i] = c[i];
     return 0;
 }
 
@@ -2210,10 +2262,11 @@ void oracle_dif_flux(const double* u, const double* av, const double* dir, doubl
 }
 
 double oracle_blender_value(const oracle_physics* p, int32_t N, const double* var) {
-    std::vector<double> C;
+    std::vector<Real> C;
     cob_matrix(N, C);
     NS ns = make_ns(p, 0);
-    return blender_from_values(ns, *p, N, C, var);
+    const std::vector<Real> vals(var, var + (size_t)(N + 1) * (N + 1) * (N + 1));
+    return blender_from_values(ns, *p, N, C, vals.data());
 }
 
 void oracle_rk4_coeffs(double* ao, double* ai, double* b) {

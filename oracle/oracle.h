@@ -111,6 +111,13 @@ int oracle_ss_error(const oracle* o, double* err);
  * [1] aux vars, [2] blender, [3] surface fluxes 2&3, [4] ho residual, [5] lo residual,
  * [6] time step, [7] updating cvars */
 int oracle_section_times(const oracle* o, double* t, int32_t n);
+/* flop counters of the counting build liboracle_flops.so (oracle/flopcount.h): per bucket
+ * [0] positivity, [1] aux vars, [2] blender, [3] surface fluxes, [4] ho inviscid residual, [5] ho
+ * viscous residual, [6] lo residual, [7] residual blend, [8] time step, [9] RK update, and the
+ * nested [10] two-point volume fluxes, [11] dense change of basis, [12] stage-2 numerical fluxes,
+ * [13] viscous fluxes of the ho viscous volume term, and [14], [15] the surface parts of the ho
+ * inviscid and viscous residuals; calls[i] counts the entries into the nested buckets. Returns the number of buckets, or -1 in liboracle.so. reset != 0 zeroes the counters. */
+int oracle_flop_counts(uint64_t* flops, uint64_t* calls, int32_t n, int32_t reset);
 
 /* metric terms of the oracle for one cell: Ja [9][n], detJ [n], face normals [6][3][n_f] */
 int oracle_cell_metrics(const oracle* o, int32_t cell, double* Ja, double* detJ, double* normals);

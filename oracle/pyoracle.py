@@ -10,6 +10,13 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+# the flop-counting build of the same source (single-threaded; oracle_flop_counts reads the counters)
+FLOPS_LIB_PATH = os.path.join(HERE, "liboracle_flops.so")
+FLOP_BUCKETS = ["positivity", "aux vars", "blender", "surface fluxes", "ho inv residual (volume, outside the pair fluxes)",
+                "ho dif residual (volume, outside the viscous fluxes)", "lo inv residual", "residual blend",
+                "time step", "RK update", "two-point volume fluxes", "dense change of basis",
+                "stage-2 numerical fluxes", "viscous fluxes of the ho viscous volume term",
+                "ho inv residual (surface part)", "ho dif residual (surface part)"]
 _dp = C.POINTER(C.c_double)
 
 
@@ -27,17 +34,16 @@ class OBC(C.Structure):
                 ("p_expr", C.c_char_p), ("T_expr", C.c_char_p), ("vel_expr", C.c_char_p)]
 
 
-_lib = None
+_libs = {}
 
 
-def load():
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
+def load(path=LIB_PATH):
+    if path in _libs:
+        return _libs[path]
+    if not os.path.exists(path):
         from . import build
         build.build()
-    lib = C.CDLL(LIB_PATH)
+    lib = C.CDLL(path)
     vp = C.c_void_p
     ip = C.POINTER(C.c_int32)
     sig = {
@@ -66,6 +72,7 @@ def load():
         "oracle_rk4_step_fixed_dt": (C.c_int, [vp, C.c_double, _dp]),
         "oracle_last_failure": (C.c_int, [vp, C.POINTER(C.c_int64), ip]),
         "oracle_section_times": (C.c_int, [vp, _dp, C.c_int32]),
+        "oracle_flop_counts": (C.c_int, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_int32, C.c_int32]),
         "oracle_cell_metrics": (C.c_int, [vp, C.c_int32, _dp, _dp, _dp]),
         "oracle_cell_subcell_normals": (C.c_int, [vp, C.c_int32, _dp]),
         "oracle_basis": (C.c_int, [C.c_int32, _dp, _dp, _dp]),
@@ -84,7 +91,7 @@ def load():
     for k, (r, a) in sig.items():
         f = getattr(lib, k)
         f.restype, f.argtypes = r, a
-    _lib = lib
+    _libs[path] = lib
     return lib
 
 
@@ -118,10 +125,10 @@ def ophys_from(ph):
 class OraclePlens:
     """CPU restatement of PLENS::calc_rhs / calc_time_step / update (RK4)."""
 
-    def __init__(self, case=None, mesh=None, physics=None, bcs=None, threads=0):
+    def __init__(self, case=None, mesh=None, physics=None, bcs=None, threads=0, lib_path=LIB_PATH):
         if case is not None:
             mesh, physics, bcs = case.mesh, case.physics, case.bcs
-        self.lib = load()
+        self.lib = load(lib_path)
         if threads:
             self.lib.oracle_set_threads(threads)
         self.n_cells, self.n_dof, self.n = mesh.n_cells, mesh.n_dof, mesh.n
@@ -280,6 +287,14 @@ class OraclePlens:
         if self.lib.oracle_ss_error(self._h, C.byref(e)):
             raise RuntimeError("no step taken yet")
         return e.value
+
+    def flop_counts(self, reset=False):
+        """(flops, calls) per bucket (FLOP_BUCKETS) of the counting build since the last reset"""
+        f = (C.c_uint64 * 16)()
+        k = (C.c_uint64 * 16)()
+        if self.lib.oracle_flop_counts(f, k, 16, int(reset)) < 0:
+            raise RuntimeError("not the flop-counting build of the oracle")
+        return np.array(f[:], dtype=np.int64), np.array(k[:], dtype=np.int64)
 
     def section_times(self):
         t = np.zeros(8)
