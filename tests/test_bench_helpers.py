@@ -49,3 +49,25 @@ def test_profile_parsers_return_none_for_unknown_kernels():
     assert bench.measured_traffic("fp64", "k_nothing<4>") == (None, None)
     flops, src, busy, bsrc = bench.measured_compute("fp64", "k_nothing<4>")
     assert flops == {} and src is None and busy is None and bsrc is None
+
+
+def test_algorithmic_flops_of_a_small_case():
+    # the reduction from the reference-as-written count to the minimal algorithm on a viscous
+    # periodic cube: 3 (N+1) ordered pair fluxes per node against 3 (N/2 + 2/(N+1)); each saving is
+    # non-negative, and the subcell residual is kept only for the blended fraction of the cells
+    from plens_amd import periodic_cube
+    case = periodic_cube(n=4, degree=3)
+    af = bench.algorithmic_flops(case, case.Q0)
+    N = 3
+    assert af["pair_fluxes_per_node_ref"] == pytest.approx(3 * (N + 1))
+    assert af["pair_fluxes_per_node_min"] == pytest.approx(3 * (N / 2 + 2 / (N + 1)))
+    sv = af["minimal_savings"]
+    assert all(v >= 0 for v in sv.values())
+    fb = af["blended_cell_fraction"]
+    assert 0.0 <= fb <= 1.0
+    assert sv["subcell_residual_blended_only"] == pytest.approx(af["sections_ref"]["lo inv residual"] * (1 - fb))
+    assert af["min_flop_per_dof_stage"] == pytest.approx(af["ref_flop_per_dof_stage"] - sum(sv.values()))
+    assert 0 < af["min_flop_per_dof_stage"] < af["ref_flop_per_dof_stage"]
+    # the dense change of basis is 2 n flops per DoF and RHS; its sum-factorised form 3 (N+1) x 2
+    assert af["sections_ref"]["dense change of basis"] == pytest.approx(2 * (N + 1) ** 3)
+    assert sv["sum_factorised_change_of_basis"] == pytest.approx(2 * (N + 1) ** 3 - 6 * (N + 1))
