@@ -81,8 +81,8 @@ FLOPCOUNT_CMP(==)
 FLOPCOUNT_CMP(!=)
 #undef FLOPCOUNT_CMP
 
-// elementary functions: overloads beside the double ones, found by the oracle's std:: calls
-namespace std {
+// elementary functions of Real, in Real's namespace so that the oracle's unqualified calls find them
+// by argument-dependent lookup (the double ones come from its using-declarations of std's)
 #define FLOPCOUNT_FN1(f)                                                     \
     inline Real f(Real a) { flopcount::tick(); return Real(::std::f(a.v)); }
 FLOPCOUNT_FN1(sqrt)
@@ -97,9 +97,3 @@ inline Real pow(Real a, Real b) { flopcount::tick(); return Real(::std::pow(a.v,
 inline Real pow(Real a, double b) { flopcount::tick(); return Real(::std::pow(a.v, b)); }
 inline Real pow(double a, Real b) { flopcount::tick(); return Real(::std::pow(a, b.v)); }
 inline Real fabs(Real a) { return Real(::std::fabs(a.v)); }
-inline Real abs(Real a) { return Real(::std::fabs(a.v)); }
-inline Real fmin(Real a, Real b) { return Real(::std::fmin(a.v, b.v)); }
-inline Real fmax(Real a, Real b) { return Real(::std::fmax(a.v, b.v)); }
-inline bool isfinite(Real a) { return ::std::isfinite(a.v); }
-inline bool isnan(Real a) { return ::std::isnan(a.v); }
-}  // namespace std

@@ -33,6 +33,18 @@ using Real = double;    // the oracle proper
 
 namespace {
 
+// elementary functions called unqualified: the std ones for double, and (found by argument-dependent
+// lookup) the counting ones of flopcount.h for the counting Real
+using std::cos;
+using std::exp;
+using std::fabs;
+using std::log;
+using std::pow;
+using std::sin;
+using std::sqrt;
+using std::tan;
+using std::tanh;
+
 using State = std::array<Real, 5>;
 using Avars = std::array<Real, 9>;
 
@@ -60,7 +72,7 @@ struct T1 {  // rank-1 tensor in 3D (deal.II Tensor<1,3>)
     Real v[3] = {0, 0, 0};
     Real& operator[](int i) { return v[i]; }
     Real operator[](int i) const { return v[i]; }
-    Real norm() const { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+    Real norm() const { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
 };
 inline Real dot(const T1& a, const T1& b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 inline T1 cross(const T1& a, const T1& b) {
@@ -137,7 +149,7 @@ struct ExprEval {
         if (eat("-")) return -sign();
         if (eat("+")) return sign();
         const Real b = atom();
-        if (eat("^")) return std::pow(b, sign());
+        if (eat("^")) return pow(b, sign());
         return b;
     }
     Real atom() {
@@ -175,34 +187,34 @@ struct ExprEval {
         const size_t n = a.size();
         auto one = [&](Real v) { if (n != 1) ok = false; return v; };
         const Real a0 = n ? a[0] : Real(0);
-        if (f == "sin") return one(std::sin(a0));
-        if (f == "cos") return one(std::cos(a0));
-        if (f == "tan") return one(std::tan(a0));
+        if (f == "sin") return one(sin(a0));
+        if (f == "cos") return one(cos(a0));
+        if (f == "tan") return one(tan(a0));
         if (f == "asin") return one(std::asin(a0));
         if (f == "acos") return one(std::acos(a0));
         if (f == "atan") return one(std::atan(a0));
         if (f == "sinh") return one(std::sinh(a0));
         if (f == "cosh") return one(std::cosh(a0));
-        if (f == "tanh") return one(std::tanh(a0));
+        if (f == "tanh") return one(tanh(a0));
         if (f == "asinh") return one(std::asinh(a0));
         if (f == "acosh") return one(std::acosh(a0));
         if (f == "atanh") return one(std::atanh(a0));
         if (f == "log2") return one(std::log2(a0));
         if (f == "log10") return one(std::log10(a0));
-        if (f == "log" || f == "ln") return one(std::log(a0));
-        if (f == "exp") return one(std::exp(a0));
-        if (f == "sqrt") return one(std::sqrt(a0));
+        if (f == "log" || f == "ln") return one(log(a0));
+        if (f == "exp") return one(exp(a0));
+        if (f == "sqrt") return one(sqrt(a0));
         if (f == "sign") return one(a0 > 0 ? 1.0 : (a0 < 0 ? -1.0 : 0.0));
         if (f == "rint") return one(std::rint(a0));
-        if (f == "abs") return one(std::fabs(a0));
+        if (f == "abs") return one(fabs(a0));
         if (f == "floor") return one(std::floor(a0));
         if (f == "ceil") return one(std::ceil(a0));
         if (f == "int") return one((Real)(long long)a0);
-        if (f == "cot") return one(1 / std::tan(a0));
-        if (f == "csc") return one(1 / std::sin(a0));
-        if (f == "sec") return one(1 / std::cos(a0));
+        if (f == "cot") return one(1 / tan(a0));
+        if (f == "csc") return one(1 / sin(a0));
+        if (f == "sec") return one(1 / cos(a0));
         if (f == "erfc") return one(std::erfc(a0));
-        if (f == "pow" && n == 2) return std::pow(a[0], a[1]);
+        if (f == "pow" && n == 2) return pow(a[0], a[1]);
         if (f == "atan2" && n == 2) return std::atan2(a[0], a[1]);
         if (f == "if" && n == 3) return a[0] != 0 ? a[1] : a[2];
         if (n >= 1 && (f == "min" || f == "max" || f == "sum" || f == "avg")) {
@@ -249,7 +261,7 @@ struct NS {
     Real R() const { return R0 / M; }                       // navier_stokes.h:337
     Real cv() const { return R() / (gma - 1); }             // navier_stokes.h:342
     Real mu(Real T) const {                               // navier_stokes.h:348-355
-        return mu0 * std::pow(T / T0, 1.5) * (T0 + S) / (T + S);
+        return mu0 * pow(T / T0, 1.5) * (T0 + S) / (T + S);
     }
     Real kappa(Real m) const { return m * R() * gma / ((gma - 1) * Pr); }  // .h:360
     bool inviscid() const { return mu0 < 1e-16; }             // .h:438
@@ -271,10 +283,10 @@ struct NS {
         ske *= 0.5 / c[0];
         return (gma - 1) * (c[4] - ske);
     }
-    Real get_a(const State& c) const { return std::sqrt(gma * get_p(c) / c[0]); }  // .cc:396-399
+    Real get_a(const State& c) const { return sqrt(gma * get_p(c) / c[0]); }  // .cc:396-399
     void cons_to_entropy(const State& c, State& w) const {    // navier_stokes.h:404-430
         const Real p = get_p(c);
-        const Real s = std::log(p) - gma * std::log(c[0]);
+        const Real s = log(p) - gma * log(c[0]);
         const Real beta = 0.5 * c[0] / p;
         const Real rhoinv = 1 / c[0];
         Real v[3], vv = 0;
@@ -290,7 +302,7 @@ struct NS {
         Real s = 0;
         const Real ri = 1 / c[0];
         for (int d = 0; d < 3; ++d) s += (c[1 + d] * c[1 + d]) * (ri * ri);
-        return std::sqrt(s) / get_a(c);
+        return sqrt(s) / get_a(c);
     }
     void prim_to_cons(Real rho, const T1& vel, Real p, State& c) const {  // .h:365-378
         c[0] = rho;
@@ -312,7 +324,7 @@ struct NS {
     }
     static Real log_avg(Real xl, Real xr) {             // navier_stokes.h:494-501
         const Real xi = xl / xr, f = (xi - 1) / (xi + 1), u = f * f,
-                     F = (u < 1e-2 ? 1 + u / 3 + u * u / 5 + u * u * u / 7 : 0.5 * std::log(xi) / f);
+                     F = (u < 1e-2 ? 1 + u / 3 + u * u / 5 + u * u * u / 7 : 0.5 * log(xi) / f);
         return 0.5 * (xl + xr) / F;
     }
 
@@ -322,24 +334,24 @@ struct NS {
         const Real pl = get_p(l), pr = get_p(r);
         const Real pmin = std::min(pl, pr), pmax = std::max(pl, pr);
         const Real ul = l[1] / l[0], ur = r[1] / r[0];
-        const Real al = std::sqrt(gma * pl / l[0]), ar = std::sqrt(gma * pr / r[0]);
+        const Real al = sqrt(gma * pl / l[0]), ar = sqrt(gma * pr / r[0]);
         const Real p_pvrs = 0.5 * (pl + pr) - 0.5 * (ur - ul) * 0.5 * (l[0] + r[0]) * 0.5 * (al + ar);
         Real ps;
         if (pmax / pmin < 2 && pmin <= p_pvrs && p_pvrs <= pmax) {
             ps = p_pvrs;
         } else if (p_pvrs < pmin) {
             const Real z = (gma - 1) / (2 * gma);
-            ps = std::pow(((al + ar) - (gma - 1) / 2 * (ur - ul)) /
-                              (al / std::pow(pl, z) + ar / std::pow(pr, z)),
+            ps = pow(((al + ar) - (gma - 1) / 2 * (ur - ul)) /
+                              (al / pow(pl, z) + ar / pow(pr, z)),
                           1 / z);
         } else {
             const Real p0 = std::max(Real(0.0), p_pvrs);
-            const Real gl = std::sqrt(2 / ((gma + 1) * l[0]) / (p0 + (gma - 1) / (gma + 1) * pl)),
-                         gr = std::sqrt(2 / ((gma + 1) * r[0]) / (p0 + (gma - 1) / (gma + 1) * pr));
+            const Real gl = sqrt(2 / ((gma + 1) * l[0]) / (p0 + (gma - 1) / (gma + 1) * pl)),
+                         gr = sqrt(2 / ((gma + 1) * r[0]) / (p0 + (gma - 1) / (gma + 1) * pr));
             ps = (gl * pl + gr * pr - (ur - ul)) / (gl + gr);
         }
-        const Real ql = (ps <= pl ? Real(1) : std::sqrt(1 + (gma + 1) / (2 * gma) * (ps / pl - 1))),
-                     qr = (ps <= pr ? Real(1) : std::sqrt(1 + (gma + 1) / (2 * gma) * (ps / pr - 1)));
+        const Real ql = (ps <= pl ? Real(1) : sqrt(1 + (gma + 1) / (2 * gma) * (ps / pl - 1))),
+                     qr = (ps <= pr ? Real(1) : sqrt(1 + (gma + 1) / (2 * gma) * (ps / pr - 1)));
         const Real sl = ul - al * ql, sr = ur + ar * qr;
         const Real s = (pr - pl + l[1] * (sl - ul) - r[1] * (sr - ur)) / (l[0] * (sl - ul) - r[0] * (sr - ur));
         if (sl > 0) {
@@ -370,22 +382,22 @@ struct NS {
         const Real ul = l[1] / l[0], ur = r[1] / r[0];
         get_inv_flux(l, x, lf);
         get_inv_flux(r, x, rf);
-        const Real S = std::max(std::fabs(ul) + al, std::fabs(ur) + ar);
+        const Real S = std::max(fabs(ul) + al, fabs(ur) + ar);
         for (int v = 0; v < 5; ++v) f[v] = 0.5 * (lf[v] + rf[v]) - 0.5 * S * (r[v] - l[v]);
     }
     // AUSM+-up, navier_stokes.cc:1070-1121 with modelling/ausm_functions.h; note the
     // integer division 3/16 == 0 of the reference (navier_stokes.cc:1090) is kept
-    static Real m1p(Real M) { return 0.5 * (M + std::fabs(M)); }
-    static Real m1n(Real M) { return 0.5 * (M - std::fabs(M)); }
+    static Real m1p(Real M) { return 0.5 * (M + fabs(M)); }
+    static Real m1n(Real M) { return 0.5 * (M - fabs(M)); }
     static Real m2p(Real M) { return 0.25 * (M + 1) * (M + 1); }
     static Real m2n(Real M) { return -0.25 * (M - 1) * (M - 1); }
-    static Real m4p(Real M) { return std::fabs(M) >= 1 ? m1p(M) : m2p(M) * (1 - 16 * 0.125 * m2n(M)); }
-    static Real m4n(Real M) { return std::fabs(M) >= 1 ? m1n(M) : m2n(M) * (1 + 16 * 0.125 * m2p(M)); }
+    static Real m4p(Real M) { return fabs(M) >= 1 ? m1p(M) : m2p(M) * (1 - 16 * 0.125 * m2n(M)); }
+    static Real m4n(Real M) { return fabs(M) >= 1 ? m1n(M) : m2n(M) * (1 + 16 * 0.125 * m2p(M)); }
     static Real p5p(Real M, Real a) {
-        return std::fabs(M) >= 1 ? m1p(M) / M : m2p(M) * ((2 - M) - 16 * a * M * m2n(M));
+        return fabs(M) >= 1 ? m1p(M) / M : m2p(M) * ((2 - M) - 16 * a * M * m2n(M));
     }
     static Real p5n(Real M, Real a) {
-        return std::fabs(M) >= 1 ? m1n(M) / M : m2n(M) * ((-2 - M) + 16 * a * M * m2p(M));
+        return fabs(M) >= 1 ? m1n(M) / M : m2n(M) * ((-2 - M) + 16 * a * M * m2p(M));
     }
     void ausm_xflux(const State& l, const State& r, State& f) const {
         const Real Minf = 0.3, Kp = 0.25, Ku = 0.75, sigma = 1.0;
@@ -396,7 +408,7 @@ struct NS {
         const Real a12 = 0.5 * (get_a(l) + get_a(r));
         const Real Ml = vl[0] / a12, Mr = vr[0] / a12;
         const Real Mavgsq = 0.5 * (Ml * Ml + Mr * Mr);
-        const Real Mo = std::sqrt(std::min(Real(1.0), std::max(Mavgsq, Minf * Minf)));
+        const Real Mo = sqrt(std::min(Real(1.0), std::max(Mavgsq, Minf * Minf)));
         const Real fa = Mo * (2 - Mo), alpha = 3 / 16 * (-4 + 5 * fa * fa);
         const Real M12 = m4p(Ml) + m4n(Mr) -
                            Kp / fa * std::max(1 - sigma * Mavgsq, Real(0.0)) * 2 * (pr - pl) / (a12 * a12 * (l[0] + r[0]));
@@ -466,16 +478,16 @@ struct NS {
         f[2] = rho_ln * va[0] * va[1];
         f[3] = rho_ln * va[0] * va[2];
         f[4] = rho_ln * va[0] * H_hat;
-        const Real sql = std::sqrt(l[0]), sqr = std::sqrt(r[0]);
+        const Real sql = sqrt(l[0]), sqr = sqrt(r[0]);
         T1 vi;
         const Real inv = 1 / (sql + sqr);
         for (int d = 0; d < 3; ++d) vi[d] = inv * (sql * vl[d] + sqr * vr[d]);
         const Real Hi = ((l[4] + pl) / sql + (r[4] + pr) / sqr) / (sql + sqr);
-        const Real ai = std::sqrt((gma - 1) * (Hi - 0.5 * dot(vi, vi)));
+        const Real ai = sqrt((gma - 1) * (Hi - 0.5 * dot(vi, vi)));
         Real K[5][5], Ki[5][5], A[5][5];
         get_xK(vi, ai, Hi, K);
         get_xKinv(vi, ai, Hi, Ki);
-        const Real uabs = std::fabs(vi[0]), lmax = uabs + ai,
+        const Real uabs = fabs(vi[0]), lmax = uabs + ai,
                      l2 = blender * lmax + (1 - blender) * uabs;
         const Real eig[5] = {lmax, l2, l2, l2, lmax};
         for (int i = 0; i < 5; ++i)
@@ -507,7 +519,7 @@ struct NS {
         }
         const Real rp = (1 - om) * l[0] + om * r[0], rn = (1 - om) * r[0] + om * l[0];
         const Real pp = (1 - om) * pl + om * pr, pn = (1 - om) * pr + om * pl;
-        const Real ap = std::sqrt(gma * pp / rp), an = std::sqrt(gma * pn / rn);
+        const Real ap = sqrt(gma * pp / rp), an = sqrt(gma * pn / rn);
         const Real Hp = gma * pp / ((gma - 1) * rp) + 0.5 * dot(vp, vp),
                      Hn = gma * pn / ((gma - 1) * rn) + 0.5 * dot(vn, vn);
         Real Kp[5][5], Kip[5][5], Kn[5][5], Kin[5][5];
@@ -515,8 +527,8 @@ struct NS {
         get_xK(vn, an, Hn, Kn);
         get_xKinv(vp, ap, Hp, Kip);
         get_xKinv(vn, an, Hn, Kin);
-        auto ps = [](Real x, Real e) { return 0.5 * (x + std::sqrt(x * x + e * e)); };
-        auto ng = [](Real x, Real e) { return 0.5 * (x - std::sqrt(x * x + e * e)); };
+        auto ps = [](Real x, Real e) { return 0.5 * (x + sqrt(x * x + e * e)); };
+        auto ng = [](Real x, Real e) { return 0.5 * (x - sqrt(x * x + e * e)); };
         const Real ep = 0.3 * ap, en = 0.3 * an;
         const Real lp[5] = {ps(vp[0] - ap, ep), ps(vp[0], ep), ps(vp[0], ep), ps(vp[0], ep), ps(vp[0] + ap, ep)};
         const Real ln[5] = {ng(vn[0] - an, en), ng(vn[0], en), ng(vn[0], en), ng(vn[0], en), ng(vn[0] + an, en)};
@@ -537,8 +549,8 @@ struct NS {
         T1 x; x[0] = 1;
         kennedy_gruber_vol_flux(l, r, x, f);
         const Real el = get_e(l), er = get_e(r);
-        const Real lm = std::max(std::fabs(l[1] / l[0]) + std::sqrt(gma * (gma - 1) * el),
-                                   std::fabs(r[1] / r[0]) + std::sqrt(gma * (gma - 1) * er));
+        const Real lm = std::max(fabs(l[1] / l[0]) + sqrt(gma * (gma - 1) * el),
+                                   fabs(r[1] / r[0]) + sqrt(gma * (gma - 1) * er));
         for (int v = 0; v < 5; ++v) f[v] -= 0.5 * lm * (r[v] - l[v]);
     }
     // Ismail-Roe with entropy-variable matrix dissipation, navier_stokes.cc:1381-1446
@@ -564,7 +576,7 @@ struct NS {
         }
         for (int i = 0; i < 4; ++i) H[i][4] = H[4][i];
         H[4][4] = rh * hh * hh - gma * p2 * p2 / (rh * (gma - 1));
-        const Real lm = std::fabs(vh[0]) + std::sqrt(gma * p2 / rh);
+        const Real lm = fabs(vh[0]) + sqrt(gma * p2 / rh);
         for (int i = 0; i < 5; ++i) {
             Real s = 0;
             for (int j = 0; j < 5; ++j) s += H[i][j] * (Wr[j] - Wl[j]);
@@ -593,7 +605,7 @@ struct NS {
         Real R[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         if (Mm > 1e-3) {
             for (int d = 0; d < 3; ++d) m[d] /= Mm;
-            const Real c = dot(x, n), t = 1 - c, s = std::sqrt(1 - c * c);
+            const Real c = dot(x, n), t = 1 - c, s = sqrt(1 - c * c);
             R[0][0] = t * m[0] * m[0] + c;     R[0][1] = t * m[0] * m[1] - s * m[2]; R[0][2] = t * m[0] * m[2] + s * m[1];
             R[1][0] = t * m[0] * m[1] + s * m[2]; R[1][1] = t * m[1] * m[1] + c;     R[1][2] = t * m[1] * m[2] - s * m[0];
             R[2][0] = t * m[0] * m[2] - s * m[1]; R[2][1] = t * m[1] * m[2] + s * m[0]; R[2][2] = t * m[2] * m[2] + c;
@@ -649,7 +661,7 @@ struct NS {
     // Ismail-Roe means through z = sqrt(rho/p) (1, v, p), navier_stokes.cc:1590-1610
     void ismail_roe_means(const State& c1, const State& c2, Real& rh, Real vh[3], Real& p2,
                           Real& hh) const {
-        const Real a1 = get_a(c1), a2 = get_a(c2), g = std::sqrt(gma);
+        const Real a1 = get_a(c1), a2 = get_a(c2), g = sqrt(gma);
         const Real z11 = g / a1, z12 = g / a2, z51 = c1[0] * a1 / g, z52 = c2[0] * a2 / g;
         const Real z5ln = log_avg(z51, z52);
         rh = 0.5 * (z11 + z12) * z5ln;
@@ -663,7 +675,7 @@ struct NS {
     }
     // Ismail-Roe volume flux, navier_stokes.cc:1590-1622
     void ismail_roe_vol_flux(const State& c1, const State& c2, const T1& dir, State& f) const {
-        const Real a1 = get_a(c1), a2 = get_a(c2), g = std::sqrt(gma);
+        const Real a1 = get_a(c1), a2 = get_a(c2), g = sqrt(gma);
         const Real p1 = (c1[0] * a1 / g + c2[0] * a2 / g) / (g / a1 + g / a2);
         Real rh, vh[3], p2, hh;
         ismail_roe_means(c1, c2, rh, vh, p2, hh);
@@ -884,8 +896,8 @@ Real blender_from_values(const NS& ns, const oracle_physics& ph, int N, const st
                     e2 += modes[id] * modes[id];
                 }
     const Real trouble = (N > 2) ? std::max(1 - e1 / etot, 1 - e2 / e1) : 1 - e1 / etot;
-    const Real thr = ph.threshold_factor * std::pow(10, -ph.threshold_exp_factor * (std::pow(N + 1, 0.25)));
-    const Real at = 1 / (1 + std::exp(-ph.sharpness_factor * (trouble / thr - 1)));
+    const Real thr = ph.threshold_factor * pow(10, -ph.threshold_exp_factor * (pow(N + 1, 0.25)));
+    const Real at = 1 / (1 + exp(-ph.sharpness_factor * (trouble / thr - 1)));
     (void)ns;
     if (at < ph.alpha_min) return 0;
     else if (at < ph.alpha_max) return at;
@@ -1032,7 +1044,7 @@ struct oracle {
                     for (int q = 0; q < 3; ++q) d2 += (cv[3 * a + q] - cv[3 * b + q]) * (cv[3 * a + q] - cv[3 * b + q]);
                     dmin = std::min(dmin, d2);
                 }
-            hmin[c] = std::sqrt(dmin);
+            hmin[c] = sqrt(dmin);
             for (int k = 0; k < np; ++k)
                 for (int j = 0; j < np; ++j)
                     for (int i = 0; i < np; ++i) {
@@ -1757,9 +1769,9 @@ struct oracle {
                     for (int col = 0; col < 3; ++col)  // JinvT
                         adv[row] += (g_mutation == kMutDtJinv ? Ji[row * 3 + col] : Ji[col * 3 + row]) * vel[col];
                 if (g_mutation == kMutDtAbs)
-                    for (Real& x : adv) x = std::fabs(x);
+                    for (Real& x : adv) x = fabs(x);
                 const Real madv = std::max({adv[0], adv[1], adv[2]});
-                const Real pw = std::pow(N, g_mutation == kMutDtDegreeSq ? 2.0 : 1.5);
+                const Real pw = pow(N, g_mutation == kMutDtDegreeSq ? 2.0 : 1.5);
                 const Real cur = Co / (pw * (madv + a / h + N * N * mu[gid(c, i)] / (h * h * s[0])));
                 if (cur < cs) cs = cur;
             }
@@ -1802,7 +1814,7 @@ struct oracle {
                         const Real det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
                                            J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
                                            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-                        const Real jxw = std::fabs(det) * wg[qi] * wg[qj] * wg[qk];
+                        const Real jxw = fabs(det) * wg[qi] * wg[qj] * wg[qk];
                         Real e = 0, d = 0;
                         for (int k = 0; k < np; ++k)
                             for (int j = 0; j < np; ++j)
@@ -1817,7 +1829,7 @@ struct oracle {
                         num += d * d * jxw;
                     }
         }
-        return std::sqrt(num) / std::sqrt(den);
+        return sqrt(num) / sqrt(den);
     }
 
     // ---- PLENS::update + update_rk4, dgsem/plens.cc:4334-4403, 4484-4518 --------------------------
