@@ -179,7 +179,7 @@ def algorithmic_flops(case, state):
     ref = float(f.sum()) / ds
     pairs_ref, pairs_min = k[10] / ds, 3 * (N / 2 + 2 / (N + 1))
     save_pairs = (pairs_ref - pairs_min) * (f[10] + f[4]) / k[10]
-    save_visc = (k[13] / ds - 3) * f[13] / k[13] + (f[5] / ds - 3 * (N + 1) * 4 * 2)
+    save_visc = ((k[13] / ds - 3) * f[13] / k[13] + (f[5] / ds - 3 * (N + 1) * 4 * 2)) if k[13] else 0.0
     save_cob = f[11] / ds - 3 * (N + 1) * 2
     save_lo = f[6] / ds * (1.0 - blended)
     fmin = ref - save_pairs - save_visc - save_cob - save_lo
