@@ -182,22 +182,19 @@ __device__ void hllc_nflux(const T l[5], const T r[5], const T nin[3], T gma, T 
     const T qr = (ps <= pr) ? T(1) : sq(T(1) + c1 * (dv(ps, pr) - T(1)));
     const T sl = ul - al * ql, sr = ur + ar * qr;
     const T s = dv(pr - pl + mnl * (sl - ul) - mnr * (sr - ur), l[0] * (sl - ul) - r[0] * (sr - ur));
-    if (sl > T(0)) {
-        inv_flux(l, n, gma, f);
-    } else if (s > T(0)) {
-        const T t = dv(sl - ul, sl - s);
-        inv_flux(l, n, gma, f);
-        f[0] += sl * (t * l[0] - l[0]);
-        for (int a = 0; a < 3; ++a) f[1 + a] += sl * (t * (l[0] * s * n[a] + l[1 + a] - mnl * n[a]) - l[1 + a]);
-        f[4] += sl * (t * (l[4] + (s - ul) * (l[0] * s + dv(pl, sl - ul))) - l[4]);
-    } else if (sr > T(0)) {
-        const T t = dv(sr - ur, sr - s);
-        inv_flux(r, n, gma, f);
-        f[0] += sr * (t * r[0] - r[0]);
-        for (int a = 0; a < 3; ++a) f[1 + a] += sr * (t * (r[0] * s * n[a] + r[1 + a] - mnr * n[a]) - r[1 + a]);
-        f[4] += sr * (t * (r[4] + (s - ur) * (r[0] * s + dv(pr, sr - ur))) - r[4]);
-    } else {
-        inv_flux(r, n, gma, f);
+    // the reference's four branches (sl > 0: F_L; s > 0: F_L*; sr > 0: F_R*; else F_R) as one
+    // evaluation on the chosen side K, so that lanes of a wavefront on different sides share it
+    const bool left = sl > T(0) || s > T(0);
+    const bool star = !(sl > T(0)) && (s > T(0) || sr > T(0));
+    T k[5];
+    for (int v = 0; v < 5; ++v) k[v] = left ? l[v] : r[v];
+    const T sk = left ? sl : sr, uk = left ? ul : ur, mnk = left ? mnl : mnr, pk = left ? pl : pr;
+    inv_flux(k, n, gma, f);
+    if (star) {
+        const T t = dv(sk - uk, sk - s);
+        f[0] += sk * (t * k[0] - k[0]);
+        for (int a = 0; a < 3; ++a) f[1 + a] += sk * (t * (k[0] * s * n[a] + k[1 + a] - mnk * n[a]) - k[1 + a]);
+        f[4] += sk * (t * (k[4] + (s - uk) * (k[0] * s + dv(pk, sk - uk))) - k[4]);
     }
 }
 
