@@ -1473,7 +1473,7 @@ __device__ __forceinline__ void pair_ismail_roe(const PairNode<TF>& A, const Pai
 // residual in registers and leaves F in LDS, where the partner picks it up with its own D entry
 // (F#(u_a, u_m) = F#(u_m, u_a)). Own pair data stay in registers; no pair index is decoded.
 // =============================================================================================
-// ONE: one cell per workgroup (cells_per_block 1, the default for N >= 4): the cell index is the
+// ONE: one cell per workgroup (cells_per_block 1, the default for N >= 3): the cell index is the
 // workgroup index, uniform over the workgroup, so per-cell data (map coefficients, blender values,
 // the step) are loaded once into scalar registers
 template <int N, typename TF, bool EXT, bool ONE>
@@ -1846,7 +1846,7 @@ int default_cpb(int N) {
     switch (N) {
         case 1: return 16;
         case 2: return 4;
-        case 3: return 2;
+        case 3: return 1;
         default: return 1;
     }
 }
@@ -1930,8 +1930,8 @@ int launch_one_t(plens_gpu* h, const KP& kp, int kernel) {
     // pair fluxes (k_vol)
     switch (kernel) {
         case 0:
-            if (N >= 4 && kp.cpb == 1)
-                hipLaunchKernelGGL((k_grad<N, TF, (N >= 4)>), dim3(blocks), dim3(thr), lds_grad_bytes<TF>(kp.cpb, NN),
+            if (N >= 3 && kp.cpb == 1)
+                hipLaunchKernelGGL((k_grad<N, TF, (N >= 3)>), dim3(blocks), dim3(thr), lds_grad_bytes<TF>(kp.cpb, NN),
                                    h->stream, kp);
             else
                 hipLaunchKernelGGL((k_grad<N, TF, false>), dim3(blocks), dim3(thr), lds_grad_bytes<TF>(kp.cpb, NN),
@@ -1976,8 +1976,8 @@ int launch_one_t(plens_gpu* h, const KP& kp, int kernel) {
             // FP64 pair-flux data (2B) + flux-type pair data (5B), direction metrics (3B) and the
             // pair fluxes of NP/2 slots (5 NP/2 B, which also holds the viscous scratch)
             const size_t lds_vol = sizeof(double) * 2 * B + sizeof(TF) * (8 * B + 5 * (size_t)(NP / 2) * B);
-            const bool one = N >= 4 && kp.cpb == 1;
-            if constexpr (N >= 4) {
+            const bool one = N >= 3 && kp.cpb == 1;
+            if constexpr (N >= 3) {
                 if (one && ext)
                     hipLaunchKernelGGL((k_vol<N, TF, true, true>), dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
                 else if (one)
