@@ -1845,7 +1845,7 @@ namespace {
 int default_cpb(int N) {
     switch (N) {
         case 1: return 16;
-        case 2: return 4;
+        case 2: return 7;  // 189 of 192 lanes busy
         case 3: return 1;
         default: return 1;
     }
@@ -3494,7 +3494,11 @@ int tune_group(plens_gpu* h, std::map<std::string, std::pair<int, int>>& db, boo
     }
     int best_cpb = h->cpb;
     double best = 1e30;
-    for (int cpb = 1; threads_for(cpb, h->NN) <= 256; cpb *= 2) {
+    // one candidate per workgroup size of 1-4 wavefronts: the most cells whose nodes fit in it (a
+    // smaller count in the same size only leaves more lanes idle)
+    for (int waves = 1; waves <= 4; ++waves) {
+        const int cpb = 64 * waves / h->NN;
+        if (cpb < 1 || (waves > 1 && cpb == 64 * (waves - 1) / h->NN)) continue;
         apply_shape(h, cpb, 256);
         const double t = time_rhs(h, 3);
         if (t < best) {
