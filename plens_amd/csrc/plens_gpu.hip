@@ -1472,7 +1472,14 @@ __device__ __forceinline__ void pair_ismail_roe(const PairNode<TF>& A, const Pai
 // same pair), which covers every unordered pair exactly once. The node adds 2 D_am F to its own
 // residual in registers and leaves F in LDS, where the partner picks it up with its own D entry
 // (F#(u_a, u_m) = F#(u_m, u_a)). Own pair data stay in registers; no pair index is decoded.
+// The slots k < NP/2 hold one flux per node. For even NP the slot k = NP/2 is written by half the
+// nodes of every line; in FP64 it is packed per line (NP/2 entries each), so that a one-cell
+// workgroup fits four to a CU's LDS at N=5 (43.2 -> 38.9 KB) and sixteen at N=3 (10.2 -> 9.0 KB).
+// FP32 pair data never bind the LDS, and the packed addressing costs them a wave per SIMD.
 // =============================================================================================
+template <int NP, typename TF>
+constexpr bool pack_last_slot() { return NP % 2 == 0 && sizeof(TF) == sizeof(double); }
+
 // ONE: one cell per workgroup (cells_per_block 1, the default for N >= 3): the cell index is the
 // workgroup index, uniform over the workgroup, so per-cell data (map coefficients, blender values,
 // the step) are loaded once into scalar registers
@@ -1487,7 +1494,11 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
     double* sR = smem;                               // 2B: per-node FP64 data of the pair flux
     TF* sA = reinterpret_cast<TF*>(sR + 2 * B);      // 5B: per-node flux-type data of the pair flux
     TF* sJa = sA + 5 * B;                            // 3B: Ja^md of the direction being processed
-    TF* sF = sJa + 3 * B;                            // 5 KP B pair fluxes / 5B viscous scratch
+    TF* sF = sJa + 3 * B;                            // 5 KF B pair fluxes / 5B viscous scratch
+    constexpr bool PK = pack_last_slot<NP, TF>();
+    constexpr int KF = PK ? KP - 1 : KP;             // full slots (one flux per node)
+    constexpr int HN = NN / 2;                       // packed slot k = NP/2 per cell
+    TF* sH = sF + 5 * KF * B;                        // 5 B/2 packed fluxes
     load_basis<N>(p.basis, sD, sW, sQm, sC);
 
     const int tid = threadIdx.x, lc = ONE ? 0 : tid / NN, node = ONE ? tid : tid - lc * NN;
@@ -1552,6 +1563,8 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
     for (int md = 0; md < 3; ++md) {
         const int a = tt[md];
         const int st = md == 0 ? 1 : (md == 1 ? NP : NF);  // node stride along the line
+        // the line's entries in the packed slot: NP/2 per line, lines numbered by the other two indices
+        const int hl = lc * HN + (NP / 2) * (md == 0 ? tj + NP * tk : (md == 1 ? ti + NP * tk : ti + NP * tj));
         if (inblk)
             for (int q = 0; q < 3; ++q) sJa[q * B + li] = ja_own[3 * md + q];
         // all reads of the previous direction's pair fluxes (and of sJa) are done: the barrier
@@ -1576,9 +1589,11 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
                 else if (vs == PLENS_VOL_FLUX_IR) pair_ismail_roe(own, oth, dir, gma, fp);
                 else pair_chandrashekhar(own, oth, dir, half_gm1_inv, fp);
                 const double d2m = 2.0 * sD[a * NP + m];
+                TF* dst = (PK && k == KP) ? sH + hl + a : sF + (k - 1) * B + li;
+                const int vst = (PK && k == KP) ? B / 2 : KF * B;
                 for (int v = 0; v < 5; ++v) {
                     ri[v] += d2m * (double)fp[v];
-                    sF[(v * KP + (k - 1)) * B + li] = fp[v];
+                    dst[v * vst] = fp[v];
                 }
             }
         }
@@ -1590,7 +1605,11 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
                 const int m = a - k >= 0 ? a - k : a - k + NP;
                 const int j = li + (m - a) * st;
                 const double d2m = 2.0 * sD[a * NP + m];
-                for (int v = 0; v < 5; ++v) ri[v] += d2m * (double)sF[(v * KP + (k - 1)) * B + j];
+                if (PK && k == KP) {  // m = a - NP/2 on the same line
+                    for (int v = 0; v < 5; ++v) ri[v] += d2m * (double)sH[v * (B / 2) + hl + m];
+                } else {
+                    for (int v = 0; v < 5; ++v) ri[v] += d2m * (double)sF[(v * KF + (k - 1)) * B + j];
+                }
             }
         }
     }
@@ -1974,8 +1993,11 @@ int launch_one_t(plens_gpu* h, const KP& kp, int kernel) {
             break;
         case 4: {
             // FP64 pair-flux data (2B) + flux-type pair data (5B), direction metrics (3B) and the
-            // pair fluxes of NP/2 slots (5 NP/2 B, which also holds the viscous scratch)
-            const size_t lds_vol = sizeof(double) * 2 * B + sizeof(TF) * (8 * B + 5 * (size_t)(NP / 2) * B);
+            // pair fluxes: NP/2 full slots (5B each), or (NP-1)/2 and the packed slot (5B/2); at
+            // least the viscous scratch (5B)
+            constexpr bool pk = pack_last_slot<NP, TF>();
+            const size_t pair_b = pk ? 5 * (size_t)((NP - 1) / 2) * B + 5 * B / 2 : 5 * (size_t)(NP / 2) * B;
+            const size_t lds_vol = sizeof(double) * 2 * B + sizeof(TF) * (8 * B + std::max(pair_b, 5 * B));
             const bool one = N >= 3 && kp.cpb == 1;
             if constexpr (N >= 3) {
                 if (one && ext)
