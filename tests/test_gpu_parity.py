@@ -462,3 +462,29 @@ def test_hllc_normals_near_the_x_axis():
     err = rel_l2(rg, ro)
     print(f"near-x normals, 15 steps in ({int((ag > 0).sum())} blended cells): operator rel L2 {err:.3e}", flush=True)
     assert err <= RHS_TOL
+
+
+# every cells-per-workgroup count of at most four wavefronts (the autotuner picks among the largest
+# count for each wavefront count; the others check that partial cells in a wavefront are handled)
+_SHAPES = [(2, c) for c in range(1, 10)] + [(3, c) for c in range(1, 5)] + [(4, 1), (4, 2), (5, 1)]
+
+
+@pytest.mark.parametrize("degree,cpb", _SHAPES)
+def test_cells_per_workgroup_shapes(degree, cpb):
+    """RHS and 5 steps of the blended shock tube with cpb cells per workgroup against the oracle"""
+    from oracle.pyoracle import OraclePlens
+    from plens_amd import GpuPlens, shock_tube
+    case = shock_tube(degree=degree, nx=16, ny=2, nz=2)
+    nn = (degree + 1) ** 3
+    assert ((cpb * nn + 63) // 64) * 64 <= 256
+    g, o = GpuPlens(case, cells_per_block=cpb), OraclePlens(case)
+    rg, ag, _ = g.calc_rhs()
+    ro, ao, _ = o.calc_rhs()
+    err_rhs = rel_l2(rg, ro)
+    assert (ao > 0).sum() > 0 and np.allclose(ag, ao, rtol=1e-10, atol=1e-13)
+    g.run(5, case.courant)
+    o.run(5, case.courant)
+    err = rel_l2(g.get_state(), o.get_state())
+    print(f"N={degree} cpb={cpb}: rhs rel L2 {err_rhs:.3e}, 5 steps {err:.3e}", flush=True)
+    assert err_rhs <= RHS_TOL
+    assert err <= RUN_TOL
