@@ -1839,7 +1839,7 @@ struct plens_gpu {
     double *dt_cell = nullptr, *scal = nullptr;   // scal[0..ninst) = dt, scal[ninst..2 ninst) = time accumulators
     double* cfl = nullptr;                        // Courant numbers of the instances (batch groups)
     unsigned long long* fail = nullptr;
-    int face_threads = 256;
+    int face_threads = 128;  // face-kernel workgroup (the batch autotuner may change it)
     bool own_stream = true;                       // false for the groups of a batch (batch stream)
     bool state_in_b = false;
     KP base{};
