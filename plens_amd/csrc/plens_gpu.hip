@@ -3386,7 +3386,8 @@ int merge_group(const std::vector<plens_gpu*>& mem, hipStream_t stream, plens_gp
     if (rc) return fail(rc);
     const int NN = h->NN, NF = h->NF;
     std::vector<int> nbr((size_t)h->nc * 6), bid((size_t)h->nc * 6), tasks((size_t)std::max(h->ntasks, 1)),
-        cinst(h->nc), ioff(h->ninst + 1);
+        cinst(h->nc), ioff(h->ninst + 1), corder(h->nc);
+    bool any_order = false;
     std::vector<double> visc(3 * (size_t)h->ninst);
     int co = 0, to = 0, bo = 0;
     for (int i = 0; i < h->ninst; ++i) {
@@ -3424,6 +3425,14 @@ int merge_group(const std::vector<plens_gpu*>& mem, hipStream_t stream, plens_gp
             if (bid[k] >= 0) bid[k] += bo;
         }
         for (int k = to; k < to + m->ntasks; ++k) tasks[k] += co * 6;
+        // k_grad's cell order: each member's Morton order (index order without one), offset
+        if (m->corder) {
+            HIPCHK(hipMemcpy(corder.data() + co, m->corder, sizeof(int) * m->nc, hipMemcpyDeviceToHost));
+            any_order = true;
+        } else {
+            for (int c = 0; c < m->nc; ++c) corder[co + c] = c;
+        }
+        for (int c = co; c < co + m->nc; ++c) corder[c] += co;
         for (int c = co; c < co + m->nc; ++c) cinst[c] = i;
         ioff[i] = co;
         visc[3 * i] = m->base.gas.mu0;
@@ -3442,6 +3451,11 @@ int merge_group(const std::vector<plens_gpu*>& mem, hipStream_t stream, plens_gp
         hipMemcpy(h->visc, visc.data(), sizeof(double) * visc.size(), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->basis, m0->basis, sizeof(Basis), hipMemcpyDeviceToDevice) != hipSuccess)
         return fail(PLENS_E_HIP);
+    if (any_order) {
+        if ((rc = dmalloc(&h->corder, (size_t)h->nc))) return fail(rc);
+        if (hipMemcpy(h->corder, corder.data(), sizeof(int) * corder.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(PLENS_E_HIP);
+    }
     if ((rc = build_task_desc(h))) return fail(rc);
     if ((rc = dmalloc(&h->cfl, (size_t)h->ninst))) return fail(rc);
     h->base = m0->base;
