@@ -49,6 +49,8 @@ constexpr int kMlpB1 = kMlpH * kMlpIn, kMlpW2 = kMlpB1 + kMlpH, kMlpB2 = kMlpW2 
               kMlpW3 = kMlpB2 + kMlpH, kMlpB3 = kMlpW3 + kMlpH;
 constexpr int kThreadsFace = 256;
 constexpr int kThreadsAlpha = 256;
+// MI355X's Infinity Cache (the memory-side cache behind the L2s)
+constexpr double kInfinityCacheBytes = 256.0 * 1024 * 1024;
 
 struct BCDev {
     int type;
@@ -170,6 +172,19 @@ template <>
 __device__ __forceinline__ const float* tsel<float>(const double*, const float* f) { return f; }
 __device__ __forceinline__ double rcpt(double x) { return rcp64(x); }
 __device__ __forceinline__ float rcpt(float x) { return __builtin_amdgcn_rcpf(x); }
+// accesses to data that no later kernel of the stage reads again (RK registers, surface terms),
+// with the nontemporal hint in the streaming instantiations (stream_handle), so that they do not
+// displace the states and auxiliary variables the next kernels gather
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_nt(const T* a) {
+    if constexpr (NT) return __builtin_nontemporal_load(a);
+    else return *a;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void st_nt(T v, T* a) {
+    if constexpr (NT) __builtin_nontemporal_store(v, a);
+    else *a = v;
+}
 
 // Workgroups are dispatched round-robin over the 8 XCDs of MI355X (workgroup b on XCD b % 8), each
 // XCD with its own L2. This bijection of [0, nblk) gives XCD x one contiguous range of work, so face
@@ -1110,8 +1125,9 @@ __global__ void __launch_bounds__(256) k_ss_reduce(const double* part, int nc, d
 // k_face
 // =============================================================================================
 // HN: HLLC (the scheme of the handle) in the normal frame (hllc_nflux), an instantiation that
-// leaves out the rotation of the other schemes and so needs fewer registers
-template <int N, typename TF, bool EXT, bool HN>
+// leaves out the rotation of the other schemes and so needs fewer registers; NT: streaming stores of
+// the surface terms (large handles)
+template <int N, typename TF, bool EXT, bool HN, bool NT = false>
 __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
     // the tasks are walked from the last Morton position to the first: the auxiliary variables that
@@ -1208,9 +1224,9 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     const TF wic = (TF)((1.0 - ac) * -rcp64(Jc)), win = (TF)((1.0 - an) * -rcp64(Jn));
     TF* sface = tsel<TF>(p.sface, p.sface_f);
     if (gs.inviscid) {
-        for (int v = 0; v < 5; ++v) sface[o + v * NF] = wic * tc[v];
+        for (int v = 0; v < 5; ++v) st_nt<NT>(wic * tc[v], sface + o + v * NF);
         if (interior)
-            for (int v = 0; v < 5; ++v) sface[on + v * NF] = win * tn[v];
+            for (int v = 0; v < 5; ++v) st_nt<NT>(win * tn[v], sface + on + v * NF);
         return;
     }
     const TF wvc = (TF)((1.0 - p.alpha_d[c]) * rcp64(Jc));
@@ -1239,10 +1255,11 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     for (int v = 0; v < 5; ++v) F[v] = TF(0.5) * (f1[v] + f2[v]);
     TF fi[5];
     dif_flux(ut, av, dirc, fi);
-    for (int v = 0; v < 5; ++v) sface[o + v * NF] = wic * tc[v] + wvc * (-sgc * (-sgc * F[v] - fi[v]) * nwc);
+    for (int v = 0; v < 5; ++v) st_nt<NT>(wic * tc[v] + wvc * (-sgc * (-sgc * F[v] - fi[v]) * nwc), sface + o + v * NF);
     if (interior) {
         dif_flux(ugt, avg, dirn, fi);
-        for (int v = 0; v < 5; ++v) sface[on + v * NF] = win * tn[v] + wvn * (-sgn * (-sgn * (-F[v]) - fi[v]) * nwn);
+        for (int v = 0; v < 5; ++v)
+            st_nt<NT>(win * tn[v] + wvn * (-sgn * (-sgn * (-F[v]) - fi[v]) * nwn), sface + on + v * NF);
     }
 }
 
@@ -1482,8 +1499,9 @@ constexpr bool pack_last_slot() { return NP % 2 == 0 && sizeof(TF) == sizeof(dou
 
 // ONE: one cell per workgroup (cells_per_block 1, the default for N >= 3): the cell index is the
 // workgroup index, uniform over the workgroup, so per-cell data (map coefficients, blender values,
-// the step) are loaded once into scalar registers
-template <int N, typename TF, bool EXT, bool ONE>
+// the step) are loaded once into scalar registers. NT: streaming loads of the surface terms and the
+// RK registers and streaming stores of the new register (large handles)
+template <int N, typename TF, bool EXT, bool ONE, bool NT = false>
 __global__ void __launch_bounds__(256) k_vol(KP p) {
     constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
     constexpr int KP = NP / 2;                     // pair slots per node and direction
@@ -1650,7 +1668,7 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
         if (tt[sd] != 0 && tt[sd] != N) continue;
         const int f = 2 * sd + (tt[sd] == N ? 1 : 0);
         const long long o = ((long long)c * 6 + f) * 5 * NF + face_dof<N>(f, ti, tj, tk);
-        for (int v = 0; v < 5; ++v) sf[v] += (double)sface[o + v * NF];
+        for (int v = 0; v < 5; ++v) sf[v] += (double)ld_nt<NT>(sface + o + v * NF);
     }
     {
         const double jp = rcp64(detJ), jm = -jp;
@@ -1680,12 +1698,12 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
         if (p.stage == 1) {
             qn = u[v] + p.c3 * R;
         } else if (p.stage == 2) {
-            qn = u[v] + (p.c2 * p.R1[k] + p.c3 * R);
+            qn = u[v] + (p.c2 * ld_nt<NT>(p.R1 + k) + p.c3 * R);
         } else {
-            qn = u[v] + (p.c1 * p.R2[k] + p.c2 * p.R1[k] + p.c3 * R);
+            qn = u[v] + (p.c1 * ld_nt<NT>(p.R2 + k) + p.c2 * ld_nt<NT>(p.R1 + k) + p.c3 * R);
         }
         p.Qout[k] = qn;
-        p.Rnew[k] = R;
+        st_nt<NT>(R, p.Rnew + k);
     }
 }
 
@@ -1933,6 +1951,17 @@ int launch_grad(plens_gpu* h, const KP& kp) {
 // kernel's register-limited occupancy (idle groups exit at once when few cells are blended)
 constexpr int kLoGrid = 2048;
 
+// The face and volume kernels' streaming instantiations (RK registers and surface terms with the
+// nontemporal hint) run when the data the next kernels gather again — the states and auxiliary
+// variables, 40 + 9 sizeof(TF) bytes per DoF — fit in the Infinity Cache but would not next to the
+// two RK registers (80 bytes per DoF): then streaming the registers keeps the gathered data cached.
+// Smaller handles keep everything cached; larger ones cannot cache the gathered data either way.
+bool stream_handle(const plens_gpu* h) {
+    const double n = (double)h->ndof;
+    const double reuse = (40.0 + 9.0 * (h->precision == PLENS_PREC_FP32_BULK ? 4.0 : 8.0)) * n;
+    return reuse <= kInfinityCacheBytes && reuse + 80.0 * n > kInfinityCacheBytes;
+}
+
 // Kernel classes of a stage, in launch order: 0 k_grad (+ k_mlp with a learned indicator),
 // 1 k_alpha, 2 k_face, 3 k_lo, 4 k_vol.
 template <int N, typename TF>
@@ -1972,10 +2001,15 @@ int launch_one_t(plens_gpu* h, const KP& kp, int kernel) {
             const long long fthreads = (long long)kp.ntasks * NF;
             const int ft = kp.face_threads;
             const dim3 fgrid((unsigned)((fthreads + ft - 1) / ft));
+            const bool nt = stream_handle(h);
             if (ext)
                 hipLaunchKernelGGL((k_face<N, TF, true, false>), fgrid, dim3(ft), 0, h->stream, kp);
+            else if (kp.scheme == PLENS_FLUX_HLLC && nt)
+                hipLaunchKernelGGL((k_face<N, TF, false, true, true>), fgrid, dim3(ft), 0, h->stream, kp);
             else if (kp.scheme == PLENS_FLUX_HLLC)
                 hipLaunchKernelGGL((k_face<N, TF, false, true>), fgrid, dim3(ft), 0, h->stream, kp);
+            else if (nt)
+                hipLaunchKernelGGL((k_face<N, TF, false, false, true>), fgrid, dim3(ft), 0, h->stream, kp);
             else
                 hipLaunchKernelGGL((k_face<N, TF, false, false>), fgrid, dim3(ft), 0, h->stream, kp);
             break;
@@ -1998,15 +2032,19 @@ int launch_one_t(plens_gpu* h, const KP& kp, int kernel) {
             constexpr bool pk = pack_last_slot<NP, TF>();
             const size_t pair_b = pk ? 5 * (size_t)((NP - 1) / 2) * B + 5 * B / 2 : 5 * (size_t)(NP / 2) * B;
             const size_t lds_vol = sizeof(double) * 2 * B + sizeof(TF) * (8 * B + std::max(pair_b, 5 * B));
-            const bool one = N >= 3 && kp.cpb == 1;
+            const bool one = N >= 3 && kp.cpb == 1, nt = stream_handle(h);
             if constexpr (N >= 3) {
                 if (one && ext)
                     hipLaunchKernelGGL((k_vol<N, TF, true, true>), dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
+                else if (one && nt)
+                    hipLaunchKernelGGL((k_vol<N, TF, false, true, true>), dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
                 else if (one)
                     hipLaunchKernelGGL((k_vol<N, TF, false, true>), dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
             }
             if (!one && ext)
                 hipLaunchKernelGGL((k_vol<N, TF, true, false>), dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
+            else if (!one && nt)
+                hipLaunchKernelGGL((k_vol<N, TF, false, false, true>), dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
             else if (!one)
                 hipLaunchKernelGGL((k_vol<N, TF, false, false>), dim3(blocks), dim3(thr), lds_vol, h->stream, kp);
             break;
