@@ -117,3 +117,20 @@ def test_fp32_run_100_steps_periodic_cube():
     err = rel_l2(g.get_state(), o.get_state())
     print(f"fp32-bulk cube 100 steps rel L2 {err:.3e}", flush=True)
     assert err <= RUN_TOL
+
+
+def test_fp32_full_size_wall_flow():
+    """the measured wall flow at full size (128x64x2 cells, N=4, 2,048,000 DoF), whose handle runs the
+    streaming face and volume kernels: RHS against the oracle, and 5 steps against the FP64 path
+    (before the blender starts switching cells, so rounding is not amplified yet)"""
+    from plens_amd import GpuPlens, wall_flow
+    case = wall_flow()
+    assert case.mesh.n_dof == 2_048_000
+    _rhs(case, "wall flow full size")
+    g32, _ = _pair(case)
+    g64 = GpuPlens(case)
+    g32.run(5, case.courant)
+    g64.run(5, case.courant)
+    err = rel_l2(g32.get_state(), g64.get_state())
+    print(f"wall flow full size, 5 steps: fp32-bulk vs fp64 rel L2 {err:.3e}", flush=True)
+    assert err <= RUN_TOL
