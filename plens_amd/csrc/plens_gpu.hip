@@ -172,9 +172,10 @@ template <>
 __device__ __forceinline__ const float* tsel<float>(const double*, const float* f) { return f; }
 __device__ __forceinline__ double rcpt(double x) { return rcp64(x); }
 __device__ __forceinline__ float rcpt(float x) { return __builtin_amdgcn_rcpf(x); }
-// accesses to data that no later kernel of the stage reads again (RK registers, surface terms),
-// with the nontemporal hint in the streaming instantiations (stream_handle), so that they do not
-// displace the states and auxiliary variables the next kernels gather
+// accesses to data that no later kernel of the stage reads again (RK registers, surface terms, the
+// stage's input state in k_vol), with the nontemporal hint in the streaming instantiations
+// (stream_handle), so that they do not displace the new state and the auxiliary variables the next
+// kernels gather
 template <bool NT, typename T>
 __device__ __forceinline__ T ld_nt(const T* a) {
     if constexpr (NT) return __builtin_nontemporal_load(a);
@@ -1499,8 +1500,8 @@ constexpr bool pack_last_slot() { return NP % 2 == 0 && sizeof(TF) == sizeof(dou
 
 // ONE: one cell per workgroup (cells_per_block 1, the default for N >= 3): the cell index is the
 // workgroup index, uniform over the workgroup, so per-cell data (map coefficients, blender values,
-// the step) are loaded once into scalar registers. NT: streaming loads of the surface terms and the
-// RK registers and streaming stores of the new register (large handles)
+// the step) are loaded once into scalar registers. NT: streaming loads of the state, the surface
+// terms and the RK registers and streaming stores of the new register (stream_handle)
 template <int N, typename TF, bool EXT, bool ONE, bool NT = false>
 __global__ void __launch_bounds__(256) k_vol(KP p) {
     constexpr int NP = N + 1, NN = NP * NP * NP, NF = NP * NP;
@@ -1531,7 +1532,8 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
     double u[5] = {1, 0, 0, 0, 2.5}, detJ = 1;
     double jm[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     if (act) {
-        load_state(p.Q, p.ndof, g, u);
+        // the last read of the stage's input state (the next stage writes its new state here)
+        for (int v = 0; v < 5; ++v) u[v] = ld_nt<NT>(p.Q + v * p.ndof + g);
         node_metrics<N>(p, c, node, jm, detJ);
     }
     const int vs = EXT ? p.vscheme : PLENS_VOL_FLUX_CHANDRASHEKHAR;
@@ -1951,8 +1953,8 @@ int launch_grad(plens_gpu* h, const KP& kp) {
 // kernel's register-limited occupancy (idle groups exit at once when few cells are blended)
 constexpr int kLoGrid = 2048;
 
-// The face and volume kernels' streaming instantiations (RK registers and surface terms with the
-// nontemporal hint) run when the data the next kernels gather again — the states and auxiliary
+// The face and volume kernels' streaming instantiations (RK registers, surface terms and k_vol's
+// state reads with the nontemporal hint) run when the data the next kernels gather again — the states and auxiliary
 // variables, 40 + 9 sizeof(TF) bytes per DoF — fit in the Infinity Cache but would not next to the
 // two RK registers (80 bytes per DoF): then streaming the registers keeps the gathered data cached.
 // Smaller handles keep everything cached; larger ones cannot cache the gathered data either way.
