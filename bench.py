@@ -71,6 +71,17 @@ def stage_bytes_per_dof(N, fb=8):
     return 8 * (5 + 5) + 8 * (5 + 4 + 3) + fb * 18 + 24 * 8 / (N + 1) ** 3
 
 
+INFINITY_CACHE_BYTES = 256 * 2 ** 20
+
+
+def streaming_handle(n_dof, fb):
+    """whether the library runs the streaming face and volume kernels for a handle of n_dof DoF with
+    fb-byte flux-type values: the gathered states and auxiliary variables ((40 + 9 fb) B per DoF) fit
+    in the Infinity Cache, but not next to the two RK registers (80 B per DoF)"""
+    reuse = (40 + 9 * fb) * n_dof
+    return reuse <= INFINITY_CACHE_BYTES < reuse + 80 * n_dof
+
+
 def measured_traffic(precision, kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary of this workload
     (calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE passes), or None"""
@@ -357,11 +368,13 @@ def main():
     fb = 8 if args.precision == "fp64" else 4
     stage_bytes = stage_bytes_per_dof(N, fb)
     # kernel name as rocprofv3 prints it: k_grad<N, type, one cell per workgroup>, k_vol<N, type,
-    # extended scheme, one cell per workgroup>, k_face and k_lo <N, type, extended scheme, HLLC in the
-    # normal frame> (here: HLLC surface flux, Chandrashekhar volume flux, N = 4 with one cell per
-    # workgroup)
-    one = "true" if (N >= 4 and args.cells_per_block in (0, 1)) else "false"
-    tail = {"grad": f",{one}>", "vol": f",false,{one}>", "face": ",false,true>", "lo": ",false,true>"}
+    # extended scheme, one cell per workgroup[, streaming]>, k_face<N, type, extended scheme, HLLC in
+    # the normal frame[, streaming]>, k_lo<N, type, extended scheme, HLLC> (here: HLLC surface flux,
+    # Chandrashekhar volume flux, one cell per workgroup); the streaming instantiations run when the
+    # gathered states and auxiliary variables fit in the Infinity Cache only without the RK registers
+    one = "true" if (N >= 3 and args.cells_per_block in (0, 1)) else "false"
+    st = ",true" if streaming_handle(n_dof, fb) else ""
+    tail = {"grad": f",{one}>", "vol": f",false,{one}{st}>", "face": f",false,true{st}>", "lo": ",false,true>"}
     tname = ("k_alpha" if dom == "alpha" else
              f"k_{dom}<{N},{'double' if args.precision == 'fp64' else 'float'}" + tail[dom])
     traffic, tsrc = measured_traffic(args.precision, tname)

@@ -71,3 +71,12 @@ def test_algorithmic_flops_of_a_small_case():
     # the dense change of basis is 2 n flops per DoF and RHS; its sum-factorised form 3 (N+1) x 2
     assert af["sections_ref"]["dense change of basis"] == pytest.approx(2 * (N + 1) ** 3)
     assert sv["sum_factorised_change_of_basis"] == pytest.approx(2 * (N + 1) ** 3 - 6 * (N + 1))
+
+
+def test_streaming_rule_of_the_kernel_names():
+    # the measured wall flow (2,048,000 DoF) runs the streaming face/volume kernels in both
+    # precisions; the shock tube (524,288 DoF) fits in the Infinity Cache with its RK registers and a
+    # 34^3 N=4 cube (4,913,000 DoF) cannot keep its gathered data there either way
+    assert bench.streaming_handle(2_048_000, 8) and bench.streaming_handle(2_048_000, 4)
+    assert not bench.streaming_handle(524_288, 8)
+    assert not bench.streaming_handle(4_913_000, 8)
