@@ -90,7 +90,7 @@ struct KP {
     unsigned int* dtcount;      // blocks of k_alpha done with their minimum
     const BCDev* bcs;
     const Basis* basis;
-    double* avars;           // stress and heat flux per node, [n_dof][9] (a face node's 9 values are adjacent)
+    double* avars;           // stress and heat flux per node, laid out by av_idx
     double* araw;
     double* alpha;
     double* alpha_d;
@@ -176,6 +176,15 @@ __device__ __forceinline__ float rcpt(float x) { return __builtin_amdgcn_rcpf(x)
 // stage's input state in k_vol), with the nontemporal hint in the streaming instantiations
 // (stream_handle), so that they do not displace the new state and the auxiliary variables the next
 // kernels gather
+// Layout of the auxiliary variables (9 per node) by degree: N >= 4 per node ([n_dof][9]: the face
+// kernel's gathers of a face node's 9 values share cache lines), N <= 3 per variable ([9][n_dof]:
+// the gradient kernel's stores and the volume kernel's loads coalesce; at these degrees the face
+// nodes are a larger share of all nodes and the coalesced stores win, 1-3% per step)
+template <int N>
+__device__ __forceinline__ long long av_idx(long long g, int a, long long ndof) {
+    if constexpr (N <= 3) return (long long)a * ndof + g;
+    else return g * 9 + a;
+}
 template <bool NT, typename T>
 __device__ __forceinline__ T ld_nt(const T* a) {
     if constexpr (NT) return __builtin_nontemporal_load(a);
@@ -769,8 +778,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     av[4] = vg[1][2] + vg[2][1];
     av[5] = vg[2][2] + vg[2][2] - c23 * tr;
     TF* avo = tsel<TF>(p.avars, p.avars_f);
-    for (int a = 0; a < 6; ++a) avo[g * 9 + a] = av[a] * (TF)mu;
-    for (int a = 6; a < 9; ++a) avo[g * 9 + a] = av[a] * (TF)kap;
+    for (int a = 0; a < 6; ++a) avo[av_idx<N>(g, a, p.ndof)] = av[a] * (TF)mu;
+    for (int a = 6; a < 9; ++a) avo[av_idx<N>(g, a, p.ndof)] = av[a] * (TF)kap;
 }
 
 // =============================================================================================
@@ -1235,9 +1244,9 @@ __global__ void __launch_bounds__(kThreadsFace) k_face(KP p) {
     // stage 3: BR1 average of the viscous fluxes (navier_stokes.cc:1644-1652)
     const TF* avars = tsel<TF>(p.avars, p.avars_f);
     TF av[9], avg[9];
-    for (int a = 0; a < 9; ++a) av[a] = avars[g * 9 + a];
+    for (int a = 0; a < 9; ++a) av[a] = avars[av_idx<N>(g, a, p.ndof)];
     if (interior || periodic) {
-        for (int a = 0; a < 9; ++a) avg[a] = avars[gn * 9 + a];
+        for (int a = 0; a < 9; ++a) avg[a] = avars[av_idx<N>(gn, a, p.ndof)];
     } else {
         for (int a = 0; a < 9; ++a) avg[a] = av[a];
         if (td.w == PLENS_BC_INSULATED_WALL) {
@@ -1639,7 +1648,7 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
     TF av[9];
     if (!gs.inviscid) {
         const TF* avars = tsel<TF>(p.avars, p.avars_f);
-        for (int a = 0; a < 9; ++a) av[a] = act ? avars[g * 9 + a] : TF(0);
+        for (int a = 0; a < 9; ++a) av[a] = act ? avars[av_idx<N>(g, a, p.ndof)] : TF(0);
 #pragma unroll
         for (int md = 0; md < 3; ++md) {
             __syncthreads();
@@ -1759,7 +1768,7 @@ __global__ void __launch_bounds__(256) k_post(KP p, double* out) {
         double u[5];
         load_state(p.Q, p.ndof, g, u);
         for (int a = 0; a < 9; ++a)
-            s0[a * NN + node] = p.avars ? p.avars[g * 9 + a] : (double)p.avars_f[g * 9 + a];
+            s0[a * NN + node] = p.avars ? p.avars[av_idx<N>(g, a, p.ndof)] : (double)p.avars_f[av_idx<N>(g, a, p.ndof)];
         s0[9 * NN + node] = u[0];
         s0[10 * NN + node] = get_p(u, p.gas.gma);
         s0[11 * NN + node] = get_e(u) / p.gas.cv;
