@@ -85,3 +85,24 @@ def test_batch_positivity_names_instance():
     with pytest.raises(PositivityError) as ei:
         batch.run(1, 0.4)
     assert ei.value.instance == 3 and ei.value.dof == 5 and ei.value.kind == "density"
+
+
+def test_streaming_kernels_match_the_cached_ones_bitwise():
+    """The measured wall flow alone (2,048,000 DoF) runs the streaming face and volume kernels (its
+    states and auxiliary variables, 112 B per DoF, fit the 256 MB Infinity Cache only without the RK
+    registers); merged in a batch with a second wall flow (524,288 DoF) its group no longer fits and
+    the cached kernels run. The cache hints change no arithmetic: the same state bit for bit."""
+    from plens_amd import GpuBatch, GpuPlens, wall_flow
+    cache = 256 * 2 ** 20
+
+    def streams(n):
+        return 112 * n <= cache < 192 * n
+
+    wf, wf2 = wall_flow(), wall_flow(nx=64, ny=32, nz=2)
+    assert streams(wf.mesh.n_dof) and not streams(wf.mesh.n_dof + wf2.mesh.n_dof)
+    solo = GpuPlens(wf)
+    batch = GpuBatch([GpuPlens(wf), GpuPlens(wf2)])
+    ts = solo.run(5, wf.courant)
+    tb = batch.run(5, [wf.courant, wf2.courant])
+    assert tb[0] == ts
+    assert np.array_equal(solo.get_state(), batch.get_state(0))
