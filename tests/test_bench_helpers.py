@@ -32,9 +32,12 @@ def test_bytes_per_dof_at_degree_4():
     assert bench.kernel_bytes_per_dof(4, fb=4)["vol"] == pytest.approx(8 * 22 + 4 * 15 + m)
 
 
-@pytest.mark.parametrize("precision,kernel", [("fp64", "k_vol<4,double,false,true>"),
-                                              ("fp32_bulk", "k_vol<4,float,false,true>")])
+# the measured wall flow (2,048,000 DoF) runs the streaming volume kernel in both precisions: its
+# name carries the streaming template argument
+@pytest.mark.parametrize("precision,kernel", [("fp64", "k_vol<4,double,false,true,true>"),
+                                              ("fp32_bulk", "k_vol<4,float,false,true,true>")])
 def test_profile_parsers_find_the_dominant_kernel(precision, kernel):
+    assert bench.streaming_handle(2_048_000, 8 if precision == "fp64" else 4)
     traffic, src = bench.measured_traffic(precision, kernel)
     assert src is not None and traffic > 1e8
     flops, fsrc, busy, _ = bench.measured_compute(precision, kernel)
