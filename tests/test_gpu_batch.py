@@ -106,3 +106,24 @@ def test_streaming_kernels_match_the_cached_ones_bitwise():
     tb = batch.run(5, [wf.courant, wf2.courant])
     assert tb[0] == ts
     assert np.array_equal(solo.get_state(), batch.get_state(0))
+
+
+@pytest.mark.parametrize("degree,precision,n_main,n_extra", [(3, "fp64", 32, 20), (5, "fp32_bulk", 22, 20)])
+def test_streaming_kernels_match_the_cached_ones_per_degree(degree, precision, n_main, n_extra):
+    """As above for periodic cubes at other degrees and in FP32-bulk (auxiliary variables of
+    9 sizeof(TF) bytes): the cube alone streams, merged with a second cube its group is cached."""
+    from plens_amd import GpuBatch, GpuPlens, periodic_cube, with_precision
+    cache, fb = 256 * 2 ** 20, 8 if precision == "fp64" else 4
+
+    def streams(n):
+        return (40 + 9 * fb) * n <= cache < (120 + 9 * fb) * n
+
+    main = with_precision(periodic_cube(degree=degree, n=n_main, flux="HLLC"), precision)
+    extra = with_precision(periodic_cube(degree=degree, n=n_extra, flux="HLLC", seed=3), precision)
+    assert streams(main.mesh.n_dof) and not streams(main.mesh.n_dof + extra.mesh.n_dof)
+    solo = GpuPlens(main)
+    batch = GpuBatch([GpuPlens(main), GpuPlens(extra)])
+    ts = solo.run(5, main.courant)
+    tb = batch.run(5, [main.courant, extra.courant])
+    assert tb[0] == ts
+    assert np.array_equal(solo.get_state(), batch.get_state(0))
