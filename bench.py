@@ -72,14 +72,15 @@ def stage_bytes_per_dof(N, fb=8):
 
 
 INFINITY_CACHE_BYTES = 256 * 2 ** 20
+STREAM_FROM_BYTES = 176 * 2 ** 20
 
 
 def streaming_handle(n_dof, fb):
     """whether the library runs the streaming face and volume kernels for a handle of n_dof DoF with
     fb-byte flux-type values: the gathered states and auxiliary variables ((40 + 9 fb) B per DoF) fit
-    in the Infinity Cache, but not next to the two RK registers (80 B per DoF)"""
+    in the Infinity Cache, and with the two RK registers (80 B per DoF) exceed 176 MiB of it"""
     reuse = (40 + 9 * fb) * n_dof
-    return reuse <= INFINITY_CACHE_BYTES < reuse + 80 * n_dof
+    return reuse <= INFINITY_CACHE_BYTES and reuse + 80 * n_dof > STREAM_FROM_BYTES
 
 
 def measured_traffic(precision, kernel):

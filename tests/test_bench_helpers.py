@@ -83,3 +83,6 @@ def test_streaming_rule_of_the_kernel_names():
     assert bench.streaming_handle(2_048_000, 8) and bench.streaming_handle(2_048_000, 4)
     assert not bench.streaming_handle(524_288, 8)
     assert not bench.streaming_handle(4_913_000, 8)
+    # at the measured crossovers: N=4 cubes of 22^3 cells (1,331,000 DoF, FP64) stream, 21^3
+    # (1,157,625 DoF, FP32-bulk) do not
+    assert bench.streaming_handle(1_331_000, 8) and not bench.streaming_handle(1_157_625, 4)

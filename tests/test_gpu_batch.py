@@ -89,14 +89,15 @@ def test_batch_positivity_names_instance():
 
 def test_streaming_kernels_match_the_cached_ones_bitwise():
     """The measured wall flow alone (2,048,000 DoF) runs the streaming face and volume kernels (its
-    states and auxiliary variables, 112 B per DoF, fit the 256 MB Infinity Cache only without the RK
-    registers); merged in a batch with a second wall flow (524,288 DoF) its group no longer fits and
-    the cached kernels run. The cache hints change no arithmetic: the same state bit for bit."""
+    states and auxiliary variables, 112 B per DoF, fit the 256 MB Infinity Cache, with the RK
+    registers they exceed the 176 MiB of it that holds them in practice); merged in a batch with a
+    second wall flow (524,288 DoF) its group no longer fits and the cached kernels run. The cache
+    hints change no arithmetic: the same state bit for bit."""
     from plens_amd import GpuBatch, GpuPlens, wall_flow
-    cache = 256 * 2 ** 20
+    cache, stream_from = 256 * 2 ** 20, 176 * 2 ** 20
 
     def streams(n):
-        return 112 * n <= cache < 192 * n
+        return 112 * n <= cache and 192 * n > stream_from
 
     wf, wf2 = wall_flow(), wall_flow(nx=64, ny=32, nz=2)
     assert streams(wf.mesh.n_dof) and not streams(wf.mesh.n_dof + wf2.mesh.n_dof)
@@ -113,10 +114,10 @@ def test_streaming_kernels_match_the_cached_ones_per_degree(degree, precision, n
     """As above for periodic cubes at other degrees and in FP32-bulk (auxiliary variables of
     9 sizeof(TF) bytes): the cube alone streams, merged with a second cube its group is cached."""
     from plens_amd import GpuBatch, GpuPlens, periodic_cube, with_precision
-    cache, fb = 256 * 2 ** 20, 8 if precision == "fp64" else 4
+    cache, stream_from, fb = 256 * 2 ** 20, 176 * 2 ** 20, 8 if precision == "fp64" else 4
 
     def streams(n):
-        return (40 + 9 * fb) * n <= cache < (120 + 9 * fb) * n
+        return (40 + 9 * fb) * n <= cache and (120 + 9 * fb) * n > stream_from
 
     main = with_precision(periodic_cube(degree=degree, n=n_main, flux="HLLC"), precision)
     extra = with_precision(periodic_cube(degree=degree, n=n_extra, flux="HLLC", seed=3), precision)
