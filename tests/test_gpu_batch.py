@@ -109,7 +109,8 @@ def test_streaming_kernels_match_the_cached_ones_bitwise():
     assert np.array_equal(solo.get_state(), batch.get_state(0))
 
 
-@pytest.mark.parametrize("degree,precision,n_main,n_extra", [(3, "fp64", 32, 20), (5, "fp32_bulk", 22, 20)])
+@pytest.mark.parametrize("degree,precision,n_main,n_extra", [(2, "fp64", 40, 30), (3, "fp64", 32, 20),
+                                                            (5, "fp32_bulk", 22, 20)])
 def test_streaming_kernels_match_the_cached_ones_per_degree(degree, precision, n_main, n_extra):
     """As above for periodic cubes at other degrees and in FP32-bulk (auxiliary variables of
     9 sizeof(TF) bytes): the cube alone streams, merged with a second cube its group is cached."""
