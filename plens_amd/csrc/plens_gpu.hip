@@ -1608,7 +1608,7 @@ __global__ void __launch_bounds__(256) k_vol(KP p) {
             inv_flux(ut, dj, gma, fi);
             const double d2 = 2.0 * sD[a * NP + a];
             for (int v = 0; v < 5; ++v) ri[v] += d2 * (double)fi[v];
-#pragma unroll 1
+#pragma unroll  // the pair fluxes of a node are independent: unrolled, their chains interleave
             for (int k = 1; k <= KP; ++k) {
                 if (EVEN && k == NP / 2 && a >= NP / 2) continue;
                 const int m = a + k < NP ? a + k : a + k - NP;
