@@ -1742,10 +1742,13 @@ __global__ void __launch_bounds__(256) k_stream_copy(const vd2* __restrict__ src
 // Face-task descriptors (cell*6+face, neighbour, boundary id, boundary type) from the task list,
 // the face tables and the boundary conditions, so that k_face reads one 16-byte record per task.
 __global__ void __launch_bounds__(256) k_task_desc(const int* tasks, const int* nbr, const int* bid,
-                                                   const BCDev* bcs, int ntasks, int4* out) {
+                                                   const BCDev* bcs, int ntasks, int pair_periodic, int4* out) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntasks) return;
-    const int cf = tasks[t], b = bid[cf];
+    const int cf = tasks[t];
+    int b = bid[cf];
+    // a periodic pair computed once is an interior face of the face kernel (both sides written)
+    if (pair_periodic && b >= 0 && bcs[b].type == PLENS_BC_PERIODIC) b = -1;
     out[t] = make_int4(cf, nbr[cf], b, b < 0 ? -1 : bcs[b].type);
 }
 
@@ -1849,6 +1852,9 @@ struct plens_gpu {
     uint64_t mesh_fp = 0;        // fingerprint of the mesh (checkpoints)
     long long steps_taken = 0;
     bool use_cmap = false;
+    // periodic face pairs computed once by the lower-index side (HLLC: the numerical flux does not
+    // depend on the blender value that the reference evaluates each periodic side with)
+    bool pair_periodic = false;
     int *nbr = nullptr, *bid = nullptr, *wall = nullptr, *tasks = nullptr;
     int* corder = nullptr;  // cells in the Morton order of their centroids (k_grad)
     int4* tdesc = nullptr;
@@ -2328,7 +2334,7 @@ void bind_buffers(plens_gpu* h) {
 int build_task_desc(plens_gpu* h) {
     if (h->ntasks == 0) return PLENS_OK;
     hipLaunchKernelGGL(k_task_desc, dim3((unsigned)((h->ntasks + 255) / 256)), dim3(256), 0, h->stream, h->tasks,
-                       h->nbr, h->bid, h->bcs, h->ntasks, h->tdesc);
+                       h->nbr, h->bid, h->bcs, h->ntasks, (int)h->pair_periodic, h->tdesc);
     return (hipGetLastError() == hipSuccess && hipStreamSynchronize(h->stream) == hipSuccess) ? PLENS_OK : PLENS_E_HIP;
 }
 
@@ -2499,16 +2505,19 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
     };
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(PLENS_E_HIP);
     const size_t nd = (size_t)h->ndof;
-    // face tasks: boundary faces of every cell, interior faces from the lower-index cell
+    // face tasks: boundary faces of every cell, interior faces from the lower-index cell; with HLLC
+    // the periodic pairs too (a cell periodic with itself: from its lower face)
+    h->pair_periodic = ph->inv_surf_flux == PLENS_FLUX_HLLC;
     std::vector<int> tasks;
     std::vector<int> wall(nc, 0);
     for (int c = 0; c < nc; ++c)
         for (int f = 0; f < 6; ++f) {
-            const int b = m->face_bid[c * 6 + f];
+            const int b = m->face_bid[c * 6 + f], nb = m->face_nbr[c * 6 + f];
             if (b >= 0) {
-                tasks.push_back(c * 6 + f);
+                if (!(h->pair_periodic && bcs[b].type == PLENS_BC_PERIODIC) || c < nb || (c == nb && (f & 1) == 0))
+                    tasks.push_back(c * 6 + f);
                 if (bcs[b].type == PLENS_BC_UNIFORM_TEMP_WALL || bcs[b].type == PLENS_BC_INSULATED_WALL) wall[c] = 1;
-            } else if (c < m->face_nbr[c * 6 + f]) {
+            } else if (c < nb) {
                 tasks.push_back(c * 6 + f);
             }
         }
@@ -2690,7 +2699,8 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
         std::vector<double> fm((size_t)h->ntasks * 10 * NF, 0.0);
         for (int t = 0; t < h->ntasks; ++t) {
             const int cf = tasks[t], c = cf / 6, f = cf - 6 * c, sd = f >> 1;
-            const bool interior = m->face_bid[cf] < 0;
+            const int bf = m->face_bid[cf];
+            const bool interior = bf < 0 || (h->pair_periodic && bcs[bf].type == PLENS_BC_PERIODIC);
             const int nb = m->face_nbr[cf];
             for (int q = 0; q < NF; ++q) {
                 double* o = fm.data() + (size_t)t * 10 * NF + q;
@@ -3424,6 +3434,7 @@ int merge_group(const std::vector<plens_gpu*>& mem, hipStream_t stream, plens_gp
         return code;
     };
     h->N = m0->N; h->NN = m0->NN; h->NF = m0->NF; h->precision = m0->precision; h->use_cmap = m0->use_cmap;
+    h->pair_periodic = m0->pair_periodic;  // same surface scheme across the group
     h->device = m0->device;
     h->stream = stream;
     h->own_stream = false;
