@@ -1852,8 +1852,8 @@ struct plens_gpu {
     uint64_t mesh_fp = 0;        // fingerprint of the mesh (checkpoints)
     long long steps_taken = 0;
     bool use_cmap = false;
-    // periodic face pairs computed once by the lower-index side (HLLC: the numerical flux does not
-    // depend on the blender value that the reference evaluates each periodic side with)
+    // periodic face pairs computed once by the lower-index side (alpha_free_flux: the numerical flux
+    // does not depend on the blender value that the reference evaluates each periodic side with)
     bool pair_periodic = false;
     int *nbr = nullptr, *bid = nullptr, *wall = nullptr, *tasks = nullptr;
     int* corder = nullptr;  // cells in the Morton order of their centroids (k_grad)
@@ -2331,6 +2331,14 @@ void bind_buffers(plens_gpu* h) {
 }
 
 // fills h->tdesc from the uploaded task list, face tables and boundary conditions
+// Surface fluxes that do not read the flux blender value (the Chandrashekhar flux and the Rusanov
+// blends do; AUSM+-up takes it as 0 in the reference). They are antisymmetric in (left, right,
+// normal), so a periodic face pair gives the same flux, up to rounding, from either side.
+bool alpha_free_flux(int scheme) {
+    return scheme == PLENS_FLUX_HLLC || scheme == PLENS_FLUX_RUSANOV || scheme == PLENS_FLUX_AUSM_PLUS_UP ||
+           scheme == PLENS_FLUX_MODIFIED_SW || scheme == PLENS_FLUX_KG || scheme == PLENS_FLUX_IR;
+}
+
 int build_task_desc(plens_gpu* h) {
     if (h->ntasks == 0) return PLENS_OK;
     hipLaunchKernelGGL(k_task_desc, dim3((unsigned)((h->ntasks + 255) / 256)), dim3(256), 0, h->stream, h->tasks,
@@ -2505,9 +2513,10 @@ int plens_gpu_create(const plens_mesh_desc* m, const plens_physics* ph, const pl
     };
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(PLENS_E_HIP);
     const size_t nd = (size_t)h->ndof;
-    // face tasks: boundary faces of every cell, interior faces from the lower-index cell; with HLLC
-    // the periodic pairs too (a cell periodic with itself: from its lower face)
-    h->pair_periodic = ph->inv_surf_flux == PLENS_FLUX_HLLC;
+    // face tasks: boundary faces of every cell, interior faces from the lower-index cell; with a
+    // surface flux that ignores the blender value the periodic pairs too (a cell periodic with
+    // itself: from its lower face)
+    h->pair_periodic = alpha_free_flux(ph->inv_surf_flux);
     std::vector<int> tasks;
     std::vector<int> wall(nc, 0);
     for (int c = 0; c < nc; ++c)
